@@ -1,0 +1,189 @@
+/* C ABI of the MI355X (gfx950) implementation of the per-frequency HEM solve.
+ *
+ * Part 1 re-exports the reference's own C API for the solve path with unchanged
+ * names, argument order, argument meaning, return codes and ownership (every
+ * array is allocated by the caller; pointers are host pointers). A driver
+ * written against the reference headers links against libtags_hip.so instead
+ * of libhphem.so. Citations are reference file:line of the prototype each
+ * declaration replaces.
+ *
+ * Part 2 adds entry points with no reference counterpart: a device-resident
+ * sweep over frequency samples that never moves the N x N matrix over PCIe.
+ */
+#ifndef TAGS_HIP_H_
+#define TAGS_HIP_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+#define TAGS_COMPLEX double _Complex
+extern "C" {
+#else
+#include <complex.h>
+#define TAGS_COMPLEX _Complex double
+#endif
+
+/* ---------------------------------------------------------------------------
+ * Part 1: the reference API
+ * ------------------------------------------------------------------------- */
+
+/* src/electrode.h:18-31 */
+enum Integration_type { INTG_NONE = 1, INTG_DOUBLE = 2, INTG_SINGLE = 3, INTG_MHEM = 4 };
+
+/* src/electrode.h:34-45 (88 bytes, same field order) */
+typedef struct {
+    double start_point[3];
+    double end_point[3];
+    double middle_point[3];
+    double length;
+    double radius;
+} Electrode;
+
+/* src/grid.h:35-52 */
+typedef struct {
+    unsigned int vertices_x;
+    unsigned int vertices_y;
+    double length_x;
+    double length_y;
+    unsigned int edge_segments_x;
+    unsigned int edge_segments_y;
+    double radius;
+    double depth;
+} Grid;
+
+/* src/electrode.h:104-106. Host code. 0 ok, 1 equal points, 2 radius <= 0. */
+int populate_electrode(Electrode *electrode, const double start_point[3],
+                       const double end_point[3], double radius);
+
+/* src/electrode.h:153-155. Host code. */
+int segment_electrode(Electrode *electrodes, double *nodes, size_t num_segments,
+                      const double *start_point, const double *end_point, double radius);
+
+/* src/grid.h:55-71. Host code. */
+int number_segments(const Grid grid);
+int number_nodes(const Grid grid);
+int electrode_grid(const Grid grid, Electrode *electrodes, double *nodes);
+
+/* src/electrode.h:226-227. Closed form, independent of gamma. */
+TAGS_COMPLEX self_integral(const Electrode *sender);
+
+/* src/electrode.h:247-250. INTG_DOUBLE runs the GPU pair quadrature (fixed-
+ * order Gauss-Legendre with analytic treatment of touching segments) and
+ * returns L_r L_s times the unit-square integral of exp(-gamma r)/r, the same
+ * quantity as the reference; `error` receives the difference to a rule of
+ * higher order. INTG_NONE returns the distance between middle points. max_eval,
+ * req_abs_error and req_rel_error are accepted; the rule meets a relative
+ * error far below any req_rel_error >= 1e-10. Returns 0, -10 for an unknown
+ * type, -11 for INTG_SINGLE / INTG_MHEM (not part of this build). */
+int integral(const Electrode *sender, const Electrode *receiver, TAGS_COMPLEX gamma,
+             size_t max_eval, double req_abs_error, double req_rel_error,
+             int integration_type, double result[2], double error[2]);
+
+/* src/electrode.h:276-281. Writes the lower half (flat index i*m + k, k >= i)
+ * of Z_L and Z_T; entries k < i are left untouched; aliasing zl == zt behaves as
+ * in the reference (zt wins on the diagonal, zl off it). */
+int calculate_impedances(TAGS_COMPLEX *zl, TAGS_COMPLEX *zt, const Electrode *electrodes,
+                         size_t num_electrodes, TAGS_COMPLEX gamma, TAGS_COMPLEX s,
+                         double mur, TAGS_COMPLEX kappa, size_t max_eval,
+                         double req_abs_error, double req_rel_error, int integration_type);
+
+/* src/electrode.h:310-317. Adds the image contributions (k >= i, k == i
+ * included) to the lower half of Z_L and Z_T. */
+int impedances_images(TAGS_COMPLEX *zl, TAGS_COMPLEX *zt, const Electrode *electrodes,
+                      const Electrode *images, size_t num_electrodes, TAGS_COMPLEX gamma,
+                      TAGS_COMPLEX s, double mur, TAGS_COMPLEX kappa, TAGS_COMPLEX ref_l,
+                      TAGS_COMPLEX ref_t, size_t max_eval, double req_abs_error,
+                      double req_rel_error, int integration_type);
+
+/* src/linalg.h:38-41. W column-major, ld = 2m + n. Returns 0, 1 if a node has
+ * no segment, -sum(A) or sum(B) - m if the incidence sums are wrong. */
+int fill_incidence_imm(TAGS_COMPLEX *wg, const Electrode *electrodes, size_t num_electrodes,
+                       const double *nodes, size_t num_nodes);
+
+/* src/linalg.h:56-59. Copies -Z_L, -Z_T into W (all i, k), zeroes the cross
+ * blocks and the Y_E block; the incidence blocks are not touched. */
+int fill_impedance_imm(TAGS_COMPLEX *wg, const TAGS_COMPLEX *zl, const TAGS_COMPLEX *zt,
+                       size_t num_electrodes, size_t num_nodes);
+
+/* src/linalg.h:79-81. Solves W x = ie for the symmetric W whose lower triangle
+ * is stored; x replaces ie ([U; I_L; I_T]). Factorisation: complex LU with
+ * partial pivoting in FP64 on the GPU (the reference uses LAPACK zsysv).
+ * Returns 0, or i > 0 when U(i,i) is exactly zero. W itself is not modified. */
+int solve_immittance(TAGS_COMPLEX *wg, TAGS_COMPLEX *ie, size_t num_electrodes,
+                     size_t num_nodes);
+
+/* ---------------------------------------------------------------------------
+ * Part 2: device-resident sweep (no reference counterpart)
+ * ------------------------------------------------------------------------- */
+
+enum tags_precision {
+    TAGS_PREC_FP64 = 0,   /* complex128 LU throughout (the plain form) */
+    TAGS_PREC_MIXED = 1,  /* complex64 LU on MFMA + FP64 iterative refinement,
+                             automatic FP64 fallback when refinement stalls */
+};
+
+typedef struct {
+    int precision_used;   /* tags_precision actually used for this sample */
+    int refine_iters;     /* refinement steps (mixed) */
+    int info;             /* 0, or first zero pivot (1-based) */
+    double backward_err;  /* ||b - W x||_inf / (||W||_inf ||x||_inf) in FP64 */
+} tags_item_stats;
+
+typedef struct {
+    double ms_fill;       /* pair quadrature + assembly */
+    double ms_panel;      /* cooperative panel factorisation */
+    double ms_gemm;       /* MFMA trailing / block updates */
+    double ms_other;      /* diagonal-block inverses, gathers, conversions */
+    double ms_solve;      /* triangular solves */
+    double ms_residual;   /* FP64 residuals */
+    double gemm_flops;    /* real flops issued by the GEMM kernel (4 real products
+                             per complex multiply-add, 8 flops) */
+    double trailing_ms;   /* time of trailing-update GEMM launches only */
+    double trailing_flops;
+    long trailing_launches;
+} tags_timers;
+
+typedef struct tags_sweep tags_sweep;
+
+/* Uploads a geometry. `images` may be NULL (z -> -z images of `electrodes`).
+ * Node/segment incidence is matched on the GPU with the reference tolerance
+ * (1e-5 per coordinate, src/linalg.c:26). Returns the fill_incidence_imm code. */
+int tags_sweep_create(tags_sweep **out, const Electrode *electrodes,
+                      const Electrode *images, size_t num_electrodes, const double *nodes,
+                      size_t num_nodes);
+
+/* Solves `nfreq` frequency samples for a unit current injected at `inj_node`.
+ * s, kappa: per-sample complex frequency and soil complex conductivity.
+ * gamma, ref_l, ref_t may be NULL: then gamma = csqrt(s mu0 mur kappa),
+ * ref_l = 1, ref_t = (kappa - s eps0)/(kappa + s eps0) (the reference caller's
+ * formulas). u_inj[f] receives U[inj_node] (= Z_h for unit current).
+ * x_out (nullable): nfreq * N complex, the full [U; I_L; I_T] per sample.
+ * stats (nullable): nfreq entries. */
+int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
+                     const TAGS_COMPLEX *kappa, double mur, const TAGS_COMPLEX *gamma,
+                     const TAGS_COMPLEX *ref_l, const TAGS_COMPLEX *ref_t, size_t inj_node,
+                     int precision, TAGS_COMPLEX *u_inj, TAGS_COMPLEX *x_out,
+                     tags_item_stats *stats);
+
+/* Accumulates HIP-event timings of every kernel family on the sweep's stream. */
+int tags_sweep_set_profiling(tags_sweep *sw, int on);
+int tags_sweep_get_timers(tags_sweep *sw, tags_timers *t);
+void tags_sweep_destroy(tags_sweep *sw);
+
+/* Dense solve of a device-resident or host complex matrix, for tests and
+ * library use. `a` is row-major n x n (for the symmetric W the same as column-
+ * major), `b` is n; on return `b` holds x. precision: tags_precision.
+ * Host pointers. Returns 0, info > 0, or a negative error code. */
+int tags_dense_solve(const TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t n, int precision,
+                     tags_item_stats *stats);
+
+/* Library version string. */
+const char *tags_version(void);
+
+/* Waits for all work on the current device (hipDeviceSynchronize). */
+int tags_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TAGS_HIP_H_ */

@@ -1,0 +1,286 @@
+"""CPU oracle for the per-frequency HEM solve -- TEST INFRASTRUCTURE ONLY.
+
+ctypes front end of ``tags_oracle.c`` (the C restatement of the reference's
+integrals, assembly and geometry) plus the LAPACK solve the reference calls.
+Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may import this module; the product path never does.
+
+The solve step follows ``solve_immittance`` (src/linalg.c:113-139): LAPACK
+``zsysv`` with ``uplo='L'`` on the lower triangle of W. Here it is scipy's
+bundled OpenBLAS/LAPACK (the same routine the reference links from the system
+LAPACK, src/lapack.h:24-25).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+
+ELECTRODE_DTYPE = np.dtype(
+    [("start_point", "<f8", 3), ("end_point", "<f8", 3), ("middle_point", "<f8", 3),
+     ("length", "<f8"), ("radius", "<f8")], align=False)
+assert ELECTRODE_DTYPE.itemsize == 88
+
+INTG_NONE, INTG_DOUBLE, INTG_SINGLE, INTG_MHEM = 1, 2, 3, 4
+
+# src/auxiliary.c:17-21
+PI = 3.1415926535897932384626433832795029
+TWO_PI = 6.283185307179586476925286766559
+FOUR_PI = 12.5663706143591729538505735331
+MU0 = 1.256637061435917e-6
+EPS0 = 8.854187817620e-12
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("vertices_x", ctypes.c_uint), ("vertices_y", ctypes.c_uint),
+                ("length_x", ctypes.c_double), ("length_y", ctypes.c_double),
+                ("edge_segments_x", ctypes.c_uint), ("edge_segments_y", ctypes.c_uint),
+                ("radius", ctypes.c_double), ("depth", ctypes.c_double)]
+
+
+_lib = None
+
+
+def build() -> str:
+    """Compile liboracle.so with the committed Makefile (gcc, no GPU)."""
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        D = ctypes.c_double
+        S = ctypes.c_size_t
+        I = ctypes.c_int
+        L.or_calculate_impedances.argtypes = [P, P, P, S, P, P, D, P, S, D, D, I, I]
+        L.or_calculate_impedances.restype = I
+        L.or_impedances_images.argtypes = [P, P, P, P, S, P, P, D, P, P, P, S, D, D, I, I]
+        L.or_impedances_images.restype = I
+        L.or_fill_incidence_imm.argtypes = [P, P, S, P, S]
+        L.or_fill_incidence_imm.restype = ctypes.c_long
+        L.or_fill_impedance_imm.argtypes = [P, P, P, S, S]
+        L.or_fill_impedance_imm.restype = I
+        L.or_self_integral.argtypes = [P]
+        L.or_self_integral.restype = D
+        L.or_integral.argtypes = [P, P, P, S, D, D, I, P, P]
+        L.or_integral.restype = I
+        L.or_populate_electrode.argtypes = [P, P, P, D]
+        L.or_populate_electrode.restype = I
+        L.or_segment_electrode.argtypes = [P, P, S, P, P, D]
+        L.or_segment_electrode.restype = I
+        L.or_number_segments.argtypes = [_Grid]
+        L.or_number_segments.restype = I
+        L.or_number_nodes.argtypes = [_Grid]
+        L.or_number_nodes.restype = I
+        L.or_electrode_grid.argtypes = [_Grid, P, P]
+        L.or_electrode_grid.restype = I
+        L.or_pair_sample.argtypes = [P, P, P, S, P, D, P, I]
+        L.or_pair_sample.restype = I
+        L.or_logspace.argtypes = [D, D, S, P]
+        L.or_logspace.restype = I
+        L.or_linspace.argtypes = [D, D, S, P]
+        L.or_linspace.restype = I
+        _lib = L
+    return _lib
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _c2(z) -> np.ndarray:
+    z = complex(z)
+    return np.array([z.real, z.imag], dtype=np.float64)
+
+
+# ----------------------------------------------------------------------------
+# geometry
+# ----------------------------------------------------------------------------
+def populate_electrode(start, end, radius):
+    e = np.zeros(1, dtype=ELECTRODE_DTYPE)
+    s = np.ascontiguousarray(start, dtype=np.float64)
+    t = np.ascontiguousarray(end, dtype=np.float64)
+    err = lib().or_populate_electrode(_p(e), _p(s), _p(t), float(radius))
+    return err, e
+
+
+def segment_electrode(start, end, num_segments, radius):
+    e = np.zeros(num_segments, dtype=ELECTRODE_DTYPE)
+    nodes = np.zeros((num_segments + 1, 3))
+    s = np.ascontiguousarray(start, dtype=np.float64)
+    t = np.ascontiguousarray(end, dtype=np.float64)
+    err = lib().or_segment_electrode(_p(e), _p(nodes), num_segments, _p(s), _p(t),
+                                      float(radius))
+    if err:
+        raise ValueError("segment_electrode failed: %d" % err)
+    return e, nodes
+
+
+def grid(vertices_x, vertices_y, length_x, length_y, edge_segments_x, edge_segments_y,
+         radius, depth):
+    g = _Grid(vertices_x, vertices_y, length_x, length_y, edge_segments_x,
+              edge_segments_y, radius, depth)
+    m = lib().or_number_segments(g)
+    n = lib().or_number_nodes(g)
+    e = np.zeros(m, dtype=ELECTRODE_DTYPE)
+    nodes = np.zeros((n, 3))
+    lib().or_electrode_grid(g, _p(e), _p(nodes))
+    return e, nodes
+
+
+def images_of(e: np.ndarray) -> np.ndarray:
+    """Image electrodes through the soil-air interface z = 0 (caller pattern of
+    examples/harmonic_impedance.c:58-65)."""
+    im = e.copy()
+    im["start_point"][:, 2] *= -1
+    im["end_point"][:, 2] *= -1
+    im["middle_point"][:, 2] *= -1
+    return im
+
+
+def logspace(a, b, n):
+    u = np.zeros(n)
+    lib().or_logspace(float(a), float(b), n, _p(u))
+    return u
+
+
+# ----------------------------------------------------------------------------
+# per-frequency scalars (examples/harmonic_impedance.c:116-123)
+# ----------------------------------------------------------------------------
+def soil_scalars(freq, sigma, epsr, mur=1.0):
+    s = complex(0.0, TWO_PI * freq)
+    kappa = sigma + s * epsr * EPS0
+    gamma = np.sqrt(s * mur * MU0 * kappa + 0j)
+    ref_t = (kappa - s * EPS0) / (kappa + s * EPS0)
+    return dict(s=s, kappa=kappa, gamma=complex(gamma), ref_l=1.0 + 0j, ref_t=ref_t,
+                mur=mur)
+
+
+# ----------------------------------------------------------------------------
+# integrals and matrices
+# ----------------------------------------------------------------------------
+def self_integral(e) -> float:
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE).reshape(1)
+    return lib().or_self_integral(_p(e))
+
+
+def integral(sender, receiver, gamma, rel_tol=1e-6, abs_tol=0.0, max_eval=0,
+             integration_type=INTG_DOUBLE):
+    s = np.ascontiguousarray(sender, dtype=ELECTRODE_DTYPE).reshape(1)
+    r = np.ascontiguousarray(receiver, dtype=ELECTRODE_DTYPE).reshape(1)
+    g = _c2(gamma)
+    res = np.zeros(2)
+    err = np.zeros(2)
+    rc = lib().or_integral(_p(s), _p(r), _p(g), max_eval, abs_tol, rel_tol,
+                           integration_type, _p(res), _p(err))
+    return rc, complex(res[0], res[1]), complex(err[0], err[1])
+
+
+def calculate_impedances(e, gamma, s, mur, kappa, rel_tol=1e-6, abs_tol=0.0, max_eval=0,
+                         integration_type=INTG_DOUBLE, threads=1, zl=None, zt=None):
+    """Returns (rc, zl, zt) as m*m complex flat arrays in the reference's
+    lower-half convention (entry (i, k >= i) at i*m + k)."""
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    m = len(e)
+    zl = np.zeros(m * m, np.complex128) if zl is None else zl
+    zt = np.zeros(m * m, np.complex128) if zt is None else zt
+    rc = lib().or_calculate_impedances(_p(zl), _p(zt), _p(e), m, _p(_c2(gamma)),
+                                       _p(_c2(s)), float(mur), _p(_c2(kappa)), max_eval,
+                                       abs_tol, rel_tol, integration_type, threads)
+    return rc, zl, zt
+
+
+def impedances_images(zl, zt, e, images, gamma, s, mur, kappa, ref_l, ref_t, rel_tol=1e-6,
+                      abs_tol=0.0, max_eval=0, integration_type=INTG_DOUBLE, threads=1):
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    images = np.ascontiguousarray(images, dtype=ELECTRODE_DTYPE)
+    m = len(e)
+    return lib().or_impedances_images(_p(zl), _p(zt), _p(e), _p(images), m, _p(_c2(gamma)),
+                                      _p(_c2(s)), float(mur), _p(_c2(kappa)),
+                                      _p(_c2(ref_l)), _p(_c2(ref_t)), max_eval, abs_tol,
+                                      rel_tol, integration_type, threads)
+
+
+def fill_incidence_imm(e, nodes, wg=None):
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    m, n = len(e), len(nodes)
+    N = n + 2 * m
+    wg = np.zeros(N * N, np.complex128) if wg is None else wg
+    rc = lib().or_fill_incidence_imm(_p(wg), _p(e), m, _p(nodes), n)
+    return rc, wg
+
+
+def fill_impedance_imm(wg, zl, zt, m, n):
+    return lib().or_fill_impedance_imm(_p(wg), _p(zl), _p(zt), m, n)
+
+
+def solve_immittance(wg, ie, m, n):
+    """zsysv('L') on the column-major W (src/linalg.c:113-139). Returns (info, x)."""
+    from scipy.linalg import lapack
+    N = n + 2 * m
+    W = wg.reshape(N, N).T  # column-major view -> row/col indexable
+    _, _, x, info = lapack.zsysv(np.ascontiguousarray(W), ie.reshape(N, 1), lower=1)
+    return info, x.reshape(N)
+
+
+def pair_sample(e, images, pairs, gamma, rel_tol=1e-6, threads=1):
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    images = np.ascontiguousarray(images, dtype=ELECTRODE_DTYPE)
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64)
+    out = np.zeros((len(pairs), 4))
+    lib().or_pair_sample(_p(e), _p(images), _p(pairs), len(pairs), _p(_c2(gamma)), rel_tol,
+                         _p(out), threads)
+    return out[:, 0] + 1j * out[:, 1], out[:, 2] + 1j * out[:, 3]
+
+
+def assemble(e, nodes, images, sc, rel_tol=1e-6, threads=1):
+    """Full W (column-major flat) for one frequency, reference pipeline:
+    calculate_impedances -> impedances_images -> fill_incidence_imm ->
+    fill_impedance_imm (caller pattern of examples/harmonic_impedance.c)."""
+    m, n = len(e), len(nodes)
+    rc, zl, zt = calculate_impedances(e, sc["gamma"], sc["s"], sc["mur"], sc["kappa"],
+                                      rel_tol=rel_tol, threads=threads)
+    if rc:
+        raise RuntimeError("calculate_impedances failed: %d" % rc)
+    rc = impedances_images(zl, zt, e, images, sc["gamma"], sc["s"], sc["mur"], sc["kappa"],
+                           sc["ref_l"], sc["ref_t"], rel_tol=rel_tol, threads=threads)
+    if rc:
+        raise RuntimeError("impedances_images failed: %d" % rc)
+    rc, wg = fill_incidence_imm(e, nodes)
+    if rc:
+        raise RuntimeError("fill_incidence_imm failed: %d" % rc)
+    fill_impedance_imm(wg, zl, zt, m, n)
+    return wg, zl, zt
+
+
+def harmonic_impedance(e, nodes, freqs, sigma, epsr, inj_node=0, mur=1.0, rel_tol=1e-6,
+                       threads=1, return_x=False):
+    """Z_h(f) = U[inj] for a unit current injected at `inj_node`."""
+    images = images_of(e)
+    m, n = len(e), len(nodes)
+    N = n + 2 * m
+    zh = np.zeros(len(freqs), np.complex128)
+    xs = []
+    for i, f in enumerate(freqs):
+        sc = soil_scalars(f, sigma, epsr, mur)
+        wg, _, _ = assemble(e, nodes, images, sc, rel_tol=rel_tol, threads=threads)
+        ie = np.zeros(N, np.complex128)
+        ie[inj_node] = 1.0
+        info, x = solve_immittance(wg, ie, m, n)
+        if info != 0:
+            raise RuntimeError("zsysv info=%d at f=%g" % (info, f))
+        zh[i] = x[inj_node]
+        if return_x:
+            xs.append(x)
+    return (zh, xs) if return_x else zh

@@ -1,0 +1,229 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle, the
+mpmath fixtures, LAPACK (scipy) and size-independent properties.
+
+Tolerances:
+* pair integrals vs mpmath: 1e-10 relative (fixed-order Gauss-Legendre with
+  analytic treatment of touching segments);
+* Z_L / Z_T entries vs the oracle's adaptive cubature at rel tol 1e-10:
+  1e-8 relative to the largest entry of the matrix;
+* harmonic impedance vs the oracle pipeline (Cubature-class integrals + LAPACK
+  zsysv): 1e-8 with the oracle at rel tol 1e-10, and 1e-6 with the oracle at
+  the reference's own tolerance (1e-6);
+* mixed precision (complex64 LU + FP64 refinement) vs the FP64 path: 1e-8.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import tags_pkg
+
+T = tags_pkg.load()
+O = tags_pkg.oracle()
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+pytestmark = pytest.mark.gpu
+
+
+def _c1():
+    return O.segment_electrode([0, 0, -0.5], [15, 0, -0.5], 30, 7e-3)
+
+
+def _small_grid():
+    return O.grid(4, 3, 6.0, 4.0, 3, 2, 6e-3, -0.5)
+
+
+def test_pair_integrals_vs_mpmath():
+    with open(os.path.join(GOLD, "pair_integrals.json")) as fh:
+        pairs = json.load(fh)["pairs"]
+    worst = 0.0
+    for p in pairs:
+        _, s = T.populate_electrode(p["sender"][0], p["sender"][1], 0.005)
+        _, r = T.populate_electrode(p["receiver"][0], p["receiver"][1], 0.005)
+        rc, v, err = T.integral(s, r, complex(*p["gamma"]))
+        assert rc == 0
+        ref = complex(*p["integral"])
+        e = abs(v - ref) / abs(ref)
+        worst = max(worst, e)
+        assert e <= 1e-10, (p["geometry"], p["gamma_name"], e)
+        assert abs(err) <= 1e-8 * abs(ref)
+    print("worst pair error vs mpmath: %.2e" % worst)
+
+
+@pytest.mark.parametrize("case", ["c1", "grid"])
+@pytest.mark.parametrize("freq", [100.0, 2e5, 1e7])
+def test_impedances_vs_oracle(case, freq):
+    e, nodes = _c1() if case == "c1" else _small_grid()
+    m = len(e)
+    sc = O.soil_scalars(freq, 0.01, 10.0)
+    img = O.images_of(e)
+    rc, zlo, zto = O.calculate_impedances(e, sc["gamma"], sc["s"], 1.0, sc["kappa"],
+                                          rel_tol=1e-10, threads=8)
+    assert rc == 0
+    O.impedances_images(zlo, zto, e, img, sc["gamma"], sc["s"], 1.0, sc["kappa"],
+                        sc["ref_l"], sc["ref_t"], rel_tol=1e-10, threads=8)
+    # sentinel in the untouched half must survive
+    zl = np.full(m * m, 3.0 + 4.0j)
+    zt = np.full(m * m, 5.0 - 6.0j)
+    rc = T.calculate_impedances(zl, zt, e, sc["gamma"], sc["s"], 1.0, sc["kappa"])
+    assert rc == 0
+    rc = T.impedances_images(zl, zt, e, img, sc["gamma"], sc["s"], 1.0, sc["kappa"],
+                             sc["ref_l"], sc["ref_t"])
+    assert rc == 0
+    up = np.triu_indices(m)
+    lo = np.tril_indices(m, -1)
+    ZL, ZT = zl.reshape(m, m), zt.reshape(m, m)
+    ZLo, ZTo = zlo.reshape(m, m), zto.reshape(m, m)
+    assert np.all(ZL[lo] == 3.0 + 4.0j) and np.all(ZT[lo] == 5.0 - 6.0j)
+    for Z, Zo in ((ZL, ZLo), (ZT, ZTo)):
+        scale = np.abs(Zo[up]).max()
+        err = np.abs(Z[up] - Zo[up]).max() / scale
+        assert err <= 1e-8, err
+
+
+def test_impedances_aliasing_and_none():
+    """zl == zt: zl wins off the diagonal, zt on it (src/electrode.c:346-372)."""
+    e, _ = _small_grid()
+    m = len(e)
+    sc = O.soil_scalars(1e5, 1e-3, 15.0)
+    z = np.zeros(m * m, complex)
+    assert T.calculate_impedances(z, z, e, sc["gamma"], sc["s"], 1.0, sc["kappa"]) == 0
+    zl = np.zeros(m * m, complex)
+    zt = np.zeros(m * m, complex)
+    T.calculate_impedances(zl, zt, e, sc["gamma"], sc["s"], 1.0, sc["kappa"])
+    Z, ZL, ZT = z.reshape(m, m), zl.reshape(m, m), zt.reshape(m, m)
+    assert np.array_equal(np.diag(Z), np.diag(ZT))
+    iu = np.triu_indices(m, 1)
+    assert np.array_equal(Z[iu], ZL[iu])
+    # INTG_NONE: scalars are 1 and the "integral" is the middle-point distance
+    zl = np.zeros(m * m, complex)
+    zt = np.zeros(m * m, complex)
+    assert T.calculate_impedances(zl, zt, e, 0, 0, 0, 0, integration_type=T.INTG_NONE) == 0
+    rc, zlo, zto = O.calculate_impedances(e, 0, 0, 0, 0, integration_type=O.INTG_NONE)
+    assert np.allclose(zl, zlo, rtol=1e-14, atol=0) and np.allclose(zt, zto, rtol=1e-14, atol=0)
+
+
+def test_assembly_vs_oracle():
+    e, nodes = _small_grid()
+    m, n = len(e), len(nodes)
+    N = n + 2 * m
+    wg = np.zeros(N * N, complex)
+    assert T.fill_incidence_imm(wg, e, nodes) == 0
+    rc, wgo = O.fill_incidence_imm(e, nodes)
+    assert rc == 0 and np.array_equal(wg, wgo)
+    rng = np.random.default_rng(3)
+    zl = rng.standard_normal(m * m) + 1j * rng.standard_normal(m * m)
+    zt = rng.standard_normal(m * m) + 1j * rng.standard_normal(m * m)
+    w1 = np.full(N * N, 2.0 - 1.0j)
+    w2 = w1.copy()
+    assert T.fill_impedance_imm(w1, zl, zt, m, n) == 0
+    O.fill_impedance_imm(w2, zl, zt, m, n)
+    assert np.array_equal(w1, w2)
+    # error codes: a node without a segment (1), a duplicated node (sum(B) != m)
+    bad = np.vstack([nodes, [[50.0, 50.0, -0.5]]])
+    Nb = len(bad) + 2 * m
+    assert T.fill_incidence_imm(np.zeros(Nb * Nb, complex), e, bad) == \
+        O.fill_incidence_imm(e, bad)[0] == 1
+    dup = np.vstack([nodes, nodes[2:3] + 1e-7])
+    Nd = len(dup) + 2 * m
+    wd = np.zeros(Nd * Nd, complex)
+    rc = T.fill_incidence_imm(wd, e, dup)
+    rco, wdo = O.fill_incidence_imm(e, dup)
+    assert rc == rco != 0 and np.array_equal(wd, wdo)
+
+
+def _assembled(e, nodes, freq, sigma, epsr, rel_tol=1e-10):
+    sc = O.soil_scalars(freq, sigma, epsr)
+    wg, _, _ = O.assemble(e, nodes, O.images_of(e), sc, rel_tol=rel_tol, threads=8)
+    return wg
+
+
+@pytest.mark.parametrize("case", ["c1", "grid"])
+def test_solve_immittance_vs_lapack(case):
+    e, nodes = _c1() if case == "c1" else _small_grid()
+    m, n = len(e), len(nodes)
+    N = n + 2 * m
+    wg = _assembled(e, nodes, 3e4, 0.01, 10.0)
+    keep = wg.copy()
+    ie = np.zeros(N, complex)
+    ie[0] = 1.0
+    info_o, xo = O.solve_immittance(wg, ie.copy(), m, n)
+    x = ie.copy()
+    assert T.solve_immittance(wg, x, m, n) == 0
+    assert info_o == 0
+    assert np.array_equal(wg, keep)
+    assert np.abs(x - xo).max() <= 1e-10 * np.abs(xo).max()
+    assert abs(x[n + m:].sum() - 1.0) < 1e-9  # KCL
+
+
+def test_solve_singular_reports_info():
+    a = np.zeros((70, 70), complex)
+    a[np.arange(69), np.arange(69)] = 1.0
+    rc, _, st = T.dense_solve(a, np.ones(70), T.PREC_FP64)
+    assert rc == 70 and st.info == 70
+
+
+@pytest.mark.parametrize("n", [1, 7, 64, 65, 127, 128, 129, 300, 1000, 2049])
+@pytest.mark.parametrize("prec", [0, 1])
+def test_dense_solve_random(n, prec):
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    rc, x, st = T.dense_solve(a, b, prec)
+    assert rc == 0
+    x0 = np.linalg.solve(a, b)
+    assert np.abs(x - x0).max() <= 1e-9 * np.abs(x0).max()
+    assert st.precision_used == prec
+
+
+def test_c1_sweep_vs_oracle():
+    """Config 1 (15 m conductor, 200 frequencies) end to end."""
+    e, nodes = _c1()
+    f = O.logspace(2, 7, 200)
+    sel = f[::10]
+    zo_tight = O.harmonic_impedance(e, nodes, sel, 0.01, 10.0, 0, rel_tol=1e-10, threads=8)
+    zo_ref = O.harmonic_impedance(e, nodes, sel, 0.01, 10.0, 0, rel_tol=1e-6, threads=8)
+    sw = T.Sweep(e, nodes)
+    s = 2j * np.pi * f
+    kap = 0.01 + s * 10.0 * T.EPS0
+    for prec in (T.PREC_FP64, T.PREC_MIXED):
+        u, st = sw.solve(s, kap, 0, prec)
+        us = u[::10]
+        assert np.max(np.abs(us - zo_tight) / np.abs(zo_tight)) <= 1e-8
+        assert np.max(np.abs(us - zo_ref) / np.abs(zo_ref)) <= 1e-6
+        assert np.all(u.real > 0)
+    sw.close()
+
+
+def test_c2_sweep_vs_oracle_and_mixed():
+    """Config 2 (20 x 20 m grid, N = 1,185): FP64 vs the oracle at a few
+    frequencies, mixed vs FP64 on 40 frequencies, KCL on every sample."""
+    rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
+    f = O.logspace(2, 7, 400)[::10]
+    s = 2j * np.pi * f
+    kap = 1e-3 + s * 15.0 * T.EPS0
+    sw = T.Sweep(e, nodes)
+    u64, st64, x64 = sw.solve(s, kap, 0, T.PREC_FP64, want_x=True)
+    umx, stmx = sw.solve(s, kap, 0, T.PREC_MIXED)
+    assert np.max(np.abs(umx - u64) / np.abs(u64)) <= 1e-8
+    m, n = len(e), len(nodes)
+    assert np.max(np.abs(x64[:, n + m:].sum(1) - 1.0)) < 1e-8
+    sel = [0, 20, 39]
+    zo = O.harmonic_impedance(e, nodes, f[sel], 1e-3, 15.0, 0, rel_tol=1e-8, threads=8)
+    assert np.max(np.abs(u64[sel] - zo) / np.abs(zo)) <= 1e-6
+    sw.close()
+
+
+def test_c3_mixed_vs_fp64_property():
+    """Full 60 x 60 m grid (N = 21,421): mixed vs FP64 at the band edges."""
+    rc, e, nodes = T.electrode_grid(31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)
+    f = np.array([100.0, 5e6])
+    s = 2j * np.pi * f
+    kap = 2e-3 + s * 20.0 * T.EPS0
+    sw = T.Sweep(e, nodes)
+    u64, _ = sw.solve(s, kap, 0, T.PREC_FP64)
+    umx, st = sw.solve(s, kap, 0, T.PREC_MIXED)
+    assert np.max(np.abs(umx - u64) / np.abs(u64)) <= 1e-8
+    assert all(q["backward_err"] <= 1e-13 for q in st)
+    sw.close()

@@ -1,0 +1,327 @@
+"""Host-side mirror of the reference's C API for the per-frequency HEM solve,
+backed by ``libtags_hip.so`` (HIP kernels for gfx950).
+
+The functions keep the reference names, argument meaning and return codes
+(src/electrode.h:247-317, src/linalg.h:38-81, src/grid.h:55-71) and operate on
+numpy arrays laid out exactly as the reference expects:
+
+* ``Electrode`` arrays: numpy structured arrays of ``ELECTRODE_DTYPE`` (88 bytes);
+* complex matrices: flat ``complex128`` arrays (``zl``/``zt`` m*m in the
+  reference's lower-half convention, ``W`` (2m+n)^2 column-major).
+
+There is no CPU fallback: loading fails loudly when the shared library is
+missing, and every call runs the HIP path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libtags_hip.so")
+CSRC = os.path.join(HERE, "csrc")
+SOURCES = ["fill.hip", "lu.hip", "api.hip"]
+
+ELECTRODE_DTYPE = np.dtype(
+    [("start_point", "<f8", 3), ("end_point", "<f8", 3), ("middle_point", "<f8", 3),
+     ("length", "<f8"), ("radius", "<f8")], align=False)
+assert ELECTRODE_DTYPE.itemsize == 88
+
+INTG_NONE, INTG_DOUBLE, INTG_SINGLE, INTG_MHEM = 1, 2, 3, 4
+PREC_FP64, PREC_MIXED = 0, 1
+
+# src/auxiliary.c:17-21
+TWO_PI = 6.283185307179586476925286766559
+FOUR_PI = 12.5663706143591729538505735331
+MU0 = 1.256637061435917e-6
+EPS0 = 8.854187817620e-12
+
+
+def hipcc_command(out=LIB_PATH):
+    return (["hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-result", "-Wno-unused-value",
+             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-o", out] +
+            [os.path.join(CSRC, s) for s in SOURCES])
+
+
+def build(verbose=False) -> str:
+    """Compile libtags_hip.so for gfx950 in-tree (cross-compiles without a GPU)."""
+    tmp = LIB_PATH + ".tmp"
+    r = subprocess.run(hipcc_command(tmp), capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+    if verbose:
+        print(r.stderr)
+    os.replace(tmp, LIB_PATH)
+    return LIB_PATH
+
+
+class CDouble(ctypes.Structure):
+    """x86-64 SysV passes `_Complex double` exactly like this struct."""
+    _fields_ = [("re", ctypes.c_double), ("im", ctypes.c_double)]
+
+    @staticmethod
+    def of(z):
+        z = complex(z)
+        return CDouble(z.real, z.imag)
+
+    def value(self):
+        return complex(self.re, self.im)
+
+
+class Grid(ctypes.Structure):
+    _fields_ = [("vertices_x", ctypes.c_uint), ("vertices_y", ctypes.c_uint),
+                ("length_x", ctypes.c_double), ("length_y", ctypes.c_double),
+                ("edge_segments_x", ctypes.c_uint), ("edge_segments_y", ctypes.c_uint),
+                ("radius", ctypes.c_double), ("depth", ctypes.c_double)]
+
+
+class ItemStats(ctypes.Structure):
+    _fields_ = [("precision_used", ctypes.c_int), ("refine_iters", ctypes.c_int),
+                ("info", ctypes.c_int), ("backward_err", ctypes.c_double)]
+
+
+class Timers(ctypes.Structure):
+    _fields_ = [("ms_fill", ctypes.c_double), ("ms_panel", ctypes.c_double),
+                ("ms_gemm", ctypes.c_double), ("ms_other", ctypes.c_double),
+                ("ms_solve", ctypes.c_double), ("ms_residual", ctypes.c_double),
+                ("gemm_flops", ctypes.c_double), ("trailing_ms", ctypes.c_double),
+                ("trailing_flops", ctypes.c_double), ("trailing_launches", ctypes.c_long)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "number_nodes",
+            "electrode_grid", "self_integral", "integral", "calculate_impedances",
+            "impedances_images", "fill_incidence_imm", "fill_impedance_imm",
+            "solve_immittance", "tags_sweep_create", "tags_sweep_solve",
+            "tags_sweep_set_profiling", "tags_sweep_get_timers", "tags_sweep_destroy",
+            "tags_dense_solve", "tags_version", "tags_device_synchronize"]
+
+_lib = None
+
+
+def lib():
+    """Loads libtags_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libtags_hip.so is missing; run __graft_entry__.build() "
+                           "(hipcc for gfx950) first")
+    L = ctypes.CDLL(LIB_PATH)
+    P, D, S, I = ctypes.c_void_p, ctypes.c_double, ctypes.c_size_t, ctypes.c_int
+    C = CDouble
+    sig = {
+        "populate_electrode": ([P, P, P, D], I),
+        "segment_electrode": ([P, P, S, P, P, D], I),
+        "number_segments": ([Grid], I),
+        "number_nodes": ([Grid], I),
+        "electrode_grid": ([Grid, P, P], I),
+        "self_integral": ([P], C),
+        "integral": ([P, P, C, S, D, D, I, P, P], I),
+        "calculate_impedances": ([P, P, P, S, C, C, D, C, S, D, D, I], I),
+        "impedances_images": ([P, P, P, P, S, C, C, D, C, C, C, S, D, D, I], I),
+        "fill_incidence_imm": ([P, P, S, P, S], I),
+        "fill_impedance_imm": ([P, P, P, S, S], I),
+        "solve_immittance": ([P, P, S, S], I),
+        "tags_sweep_create": ([ctypes.POINTER(P), P, P, S, P, S], I),
+        "tags_sweep_solve": ([P, S, P, P, D, P, P, P, S, I, P, P, P], I),
+        "tags_sweep_set_profiling": ([P, I], I),
+        "tags_sweep_get_timers": ([P, ctypes.POINTER(Timers)], I),
+        "tags_sweep_destroy": ([P], None),
+        "tags_dense_solve": ([P, P, S, I, P], I),
+        "tags_version": ([], ctypes.c_char_p),
+        "tags_device_synchronize": ([], I),
+    }
+    for name, (args, res) in sig.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _electrodes(e):
+    e = np.ascontiguousarray(e)
+    if e.dtype != ELECTRODE_DTYPE:
+        raise TypeError("electrodes must use ELECTRODE_DTYPE")
+    return e
+
+
+# ---------------------------------------------------------------------------
+# reference API (src/electrode.h, src/grid.h, src/linalg.h)
+# ---------------------------------------------------------------------------
+def populate_electrode(start, end, radius):
+    """Returns (code, electrode[1]); code 1: equal points, 2: radius <= 0."""
+    e = np.zeros(1, ELECTRODE_DTYPE)
+    s = np.ascontiguousarray(start, np.float64)
+    t = np.ascontiguousarray(end, np.float64)
+    return lib().populate_electrode(_p(e), _p(s), _p(t), float(radius)), e
+
+
+def segment_electrode(start, end, num_segments, radius):
+    e = np.zeros(max(num_segments, 1), ELECTRODE_DTYPE)
+    nodes = np.zeros((max(num_segments, 1) + 1, 3))
+    s = np.ascontiguousarray(start, np.float64)
+    t = np.ascontiguousarray(end, np.float64)
+    rc = lib().segment_electrode(_p(e), _p(nodes), num_segments, _p(s), _p(t), float(radius))
+    return rc, e[:num_segments], nodes[:num_segments + 1]
+
+
+def electrode_grid(vertices_x, vertices_y, length_x, length_y, edge_segments_x,
+                   edge_segments_y, radius, depth):
+    g = Grid(vertices_x, vertices_y, length_x, length_y, edge_segments_x, edge_segments_y,
+             radius, depth)
+    m, n = lib().number_segments(g), lib().number_nodes(g)
+    e = np.zeros(m, ELECTRODE_DTYPE)
+    nodes = np.zeros((n, 3))
+    rc = lib().electrode_grid(g, _p(e), _p(nodes))
+    return rc, e, nodes
+
+
+def images_of(e):
+    """Image electrodes through the soil-air interface z = 0 (caller pattern of
+    examples/harmonic_impedance.c:58-65)."""
+    im = np.array(e, copy=True)
+    for f in ("start_point", "end_point", "middle_point"):
+        im[f][:, 2] *= -1
+    return im
+
+
+def self_integral(e):
+    e = _electrodes(e).reshape(-1)[:1]
+    return lib().self_integral(_p(e)).value()
+
+
+def integral(sender, receiver, gamma, max_eval=0, req_abs_error=0.0, req_rel_error=1e-6,
+             integration_type=INTG_DOUBLE):
+    s = _electrodes(sender).reshape(-1)[:1]
+    r = _electrodes(receiver).reshape(-1)[:1]
+    res = np.zeros(2)
+    err = np.zeros(2)
+    rc = lib().integral(_p(s), _p(r), CDouble.of(gamma), max_eval, req_abs_error,
+                        req_rel_error, integration_type, _p(res), _p(err))
+    return rc, complex(res[0], res[1]), complex(err[0], err[1])
+
+
+def calculate_impedances(zl, zt, electrodes, gamma, s, mur, kappa, max_eval=0,
+                         req_abs_error=0.0, req_rel_error=1e-6, integration_type=INTG_DOUBLE):
+    """In place on zl, zt (m*m complex128); returns the reference's code."""
+    e = _electrodes(electrodes)
+    return lib().calculate_impedances(_p(zl), _p(zt), _p(e), len(e), CDouble.of(gamma),
+                                      CDouble.of(s), float(mur), CDouble.of(kappa), max_eval,
+                                      req_abs_error, req_rel_error, integration_type)
+
+
+def impedances_images(zl, zt, electrodes, images, gamma, s, mur, kappa, ref_l, ref_t,
+                      max_eval=0, req_abs_error=0.0, req_rel_error=1e-6,
+                      integration_type=INTG_DOUBLE):
+    e = _electrodes(electrodes)
+    im = _electrodes(images)
+    return lib().impedances_images(_p(zl), _p(zt), _p(e), _p(im), len(e), CDouble.of(gamma),
+                                   CDouble.of(s), float(mur), CDouble.of(kappa),
+                                   CDouble.of(ref_l), CDouble.of(ref_t), max_eval,
+                                   req_abs_error, req_rel_error, integration_type)
+
+
+def fill_incidence_imm(wg, electrodes, nodes):
+    e = _electrodes(electrodes)
+    nodes = np.ascontiguousarray(nodes, np.float64)
+    return lib().fill_incidence_imm(_p(wg), _p(e), len(e), _p(nodes), len(nodes))
+
+
+def fill_impedance_imm(wg, zl, zt, num_electrodes, num_nodes):
+    return lib().fill_impedance_imm(_p(wg), _p(zl), _p(zt), num_electrodes, num_nodes)
+
+
+def solve_immittance(wg, ie, num_electrodes, num_nodes):
+    """ie (complex128, N) is overwritten by [U; I_L; I_T]; returns info."""
+    return lib().solve_immittance(_p(wg), _p(ie), num_electrodes, num_nodes)
+
+
+def dense_solve(a, b, precision=PREC_FP64):
+    """Solves a x = b (a: n x n complex128 row-major). Returns (code, x, stats)."""
+    a = np.ascontiguousarray(a, np.complex128)
+    x = np.array(b, dtype=np.complex128, copy=True)
+    st = ItemStats()
+    rc = lib().tags_dense_solve(_p(a), _p(x), len(x), precision, ctypes.byref(st))
+    return rc, x, st
+
+
+# ---------------------------------------------------------------------------
+# per-frequency scalars of the reference caller (examples/harmonic_impedance.c:116-123)
+# ---------------------------------------------------------------------------
+def soil_scalars(freq, sigma, epsr, mur=1.0):
+    s = complex(0.0, TWO_PI * freq)
+    kappa = sigma + s * epsr * EPS0
+    gamma = complex(np.sqrt(s * mur * MU0 * kappa + 0j))
+    ref_t = (kappa - s * EPS0) / (kappa + s * EPS0)
+    return dict(s=s, kappa=kappa, gamma=gamma, ref_l=1.0 + 0j, ref_t=ref_t, mur=mur)
+
+
+# ---------------------------------------------------------------------------
+# device-resident sweep
+# ---------------------------------------------------------------------------
+class Sweep:
+    """Device-resident frequency sweep of one grounding geometry."""
+
+    def __init__(self, electrodes, nodes, images=None):
+        e = _electrodes(electrodes)
+        nodes = np.ascontiguousarray(nodes, np.float64)
+        im = None if images is None else _electrodes(images)
+        self._h = ctypes.c_void_p()
+        rc = lib().tags_sweep_create(ctypes.byref(self._h), _p(e), _p(im), len(e), _p(nodes),
+                                     len(nodes))
+        if rc != 0:
+            raise RuntimeError("tags_sweep_create failed: %d" % rc)
+        self.m, self.n = len(e), len(nodes)
+        self.N = self.n + 2 * self.m
+
+    def solve(self, s, kappa, inj_node=0, precision=PREC_FP64, mur=1.0, gamma=None,
+              ref_l=None, ref_t=None, want_x=False):
+        s = np.ascontiguousarray(s, np.complex128)
+        kappa = np.ascontiguousarray(kappa, np.complex128)
+        nf = len(s)
+        u = np.zeros(nf, np.complex128)
+        x = np.zeros((nf, self.N), np.complex128) if want_x else None
+        stats = (ItemStats * nf)()
+        conv = [None if a is None else np.ascontiguousarray(a, np.complex128)
+                for a in (gamma, ref_l, ref_t)]
+        rc = lib().tags_sweep_solve(self._h, nf, _p(s), _p(kappa), float(mur), _p(conv[0]),
+                                    _p(conv[1]), _p(conv[2]), inj_node, precision, _p(u),
+                                    _p(x), ctypes.cast(stats, ctypes.c_void_p))
+        if rc != 0:
+            raise RuntimeError("tags_sweep_solve failed: %d" % rc)
+        st = [dict(precision_used=q.precision_used, refine_iters=q.refine_iters, info=q.info,
+                   backward_err=q.backward_err) for q in stats]
+        return (u, st, x) if want_x else (u, st)
+
+    def set_profiling(self, on=True):
+        lib().tags_sweep_set_profiling(self._h, 1 if on else 0)
+
+    def timers(self):
+        t = Timers()
+        lib().tags_sweep_get_timers(self._h, ctypes.byref(t))
+        return t.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().tags_sweep_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,931 @@
+// C ABI of the library: the reference's solve-path API (host pointers, same
+// semantics) and the device-resident sweep.
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <complex>
+#include <cstring>
+#include <memory>
+#include <vector>
+
+#include "lu.h"
+
+namespace tags {
+
+struct DevInfo {
+    int ok = 0;
+    int num_cu = 0;
+    size_t lds_limit = 65536;
+};
+
+static DevInfo& dev_info()
+{
+    static DevInfo d;
+    if (!d.ok) {
+        int dev = 0;
+        hipGetDevice(&dev);
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, dev) == hipSuccess) {
+            d.num_cu = p.multiProcessorCount;
+            size_t lim = p.sharedMemPerBlock;
+            if (p.sharedMemPerBlockOptin > lim) lim = p.sharedMemPerBlockOptin;
+            if (lim > 160 * 1024) lim = 160 * 1024;
+            d.lds_limit = lim;
+            d.ok = 1;
+        }
+    }
+    return d;
+}
+
+// Device copy of an electrode array in structure-of-arrays form.
+struct DevSegs {
+    double* buf = nullptr;  // 8 arrays of m doubles
+    long m = 0;
+    int upload(const Electrode* e, long m_, hipStream_t st)
+    {
+        m = m_;
+        std::vector<double> h(8 * (size_t)m);
+        for (long i = 0; i < m; i++) {
+            h[0 * m + i] = e[i].start_point[0];
+            h[1 * m + i] = e[i].start_point[1];
+            h[2 * m + i] = e[i].start_point[2];
+            h[3 * m + i] = e[i].end_point[0];
+            h[4 * m + i] = e[i].end_point[1];
+            h[5 * m + i] = e[i].end_point[2];
+            h[6 * m + i] = e[i].length;
+            h[7 * m + i] = e[i].radius;
+        }
+        TAGS_HIP(hipMalloc(&buf, sizeof(double) * 8 * (size_t)(m > 0 ? m : 1)));
+        TAGS_HIP(hipMemcpyAsync(buf, h.data(), sizeof(double) * 8 * m, hipMemcpyHostToDevice, st));
+        TAGS_HIP(hipStreamSynchronize(st));
+        return 0;
+    }
+    SegSoA soa() const
+    {
+        return SegSoA{buf, buf + m, buf + 2 * m, buf + 3 * m, buf + 4 * m, buf + 5 * m,
+                      buf + 6 * m, buf + 7 * m};
+    }
+    ~DevSegs() { hipFree(buf); }
+};
+
+struct Stream {
+    hipStream_t s = nullptr;
+    Stream() { hipStreamCreate(&s); }
+    ~Stream() { hipStreamDestroy(s); }
+};
+
+template <typename X>
+struct DevBuf {
+    X* p = nullptr;
+    int alloc(size_t n)
+    {
+        TAGS_HIP(hipMalloc(&p, sizeof(X) * (n ? n : 1)));
+        return 0;
+    }
+    ~DevBuf() { hipFree(p); }
+};
+
+static PairScalars make_scalars(c64 gamma, c64 s, double mur, c64 kappa, c64 ref_l, c64 ref_t,
+                                int type)
+{
+    PairScalars sc;
+    sc.gamma = gamma;
+    sc.intg_type = type;
+    if (type == INTG_NONE || type == INTG_MHEM) {
+        sc.zl_dir = sc.zt_dir = sc.zl_img = sc.zt_img = c64{1.0, 0.0};
+        return sc;
+    }
+    // src/electrode.c:317-318 and :397-398
+    const c64 iwu = c64{s.x * mur * kMu0 / kFourPi, s.y * mur * kMu0 / kFourPi};
+    const c64 fk = c64{kFourPi * kappa.x, kFourPi * kappa.y};
+    sc.zl_dir = iwu;
+    sc.zt_dir = crecip(fk);
+    sc.zl_img = cmul(ref_l, iwu);
+    sc.zt_img = cdiv(ref_t, fk);
+    return sc;
+}
+
+// ---------------------------------------------------------------------------
+// Dense solver shared by solve_immittance, tags_dense_solve and the sweep.
+// ---------------------------------------------------------------------------
+struct Solver {
+    int n = 0;
+    hipStream_t st = nullptr;
+    Profiler* pr = nullptr;
+    LUWork<double> w64;
+    LUWork<float> w32;
+    c32* W32 = nullptr;
+    double* dscale = nullptr;
+    c64* x = nullptr;
+    c64* r = nullptr;
+    c64* b = nullptr;
+    c32* t32 = nullptr;
+    unsigned long long* norms = nullptr;
+    unsigned long long* hnorms = nullptr;  // pinned
+
+    int ensure(int n_, bool mixed)
+    {
+        DevInfo& di = dev_info();
+        if (!di.ok) return TAGS_ERR_HIP;
+        if (n_ != n) {
+            free_vecs();
+            n = n_;
+            TAGS_HIP(hipMalloc(&dscale, sizeof(double) * n));
+            TAGS_HIP(hipMalloc(&x, sizeof(c64) * n));
+            TAGS_HIP(hipMalloc(&r, sizeof(c64) * n));
+            TAGS_HIP(hipMalloc(&b, sizeof(c64) * n));
+            TAGS_HIP(hipMalloc(&t32, sizeof(c32) * n));
+            TAGS_HIP(hipMalloc(&norms, sizeof(unsigned long long) * 4));
+            TAGS_HIP(hipHostMalloc(&hnorms, sizeof(unsigned long long) * 4, 0));
+        }
+        int rc = w64.alloc(n, di.num_cu, di.lds_limit);
+        if (rc) return rc;
+        if (mixed) {
+            if ((rc = w32.alloc(n, di.num_cu, di.lds_limit))) return rc;
+            if (!W32) TAGS_HIP(hipMalloc(&W32, sizeof(c32) * (size_t)n * n));
+        }
+        return 0;
+    }
+    void free_vecs()
+    {
+        hipFree(dscale);
+        hipFree(x);
+        hipFree(r);
+        hipFree(b);
+        hipFree(t32);
+        hipFree(norms);
+        if (hnorms) hipHostFree(hnorms);
+        hipFree(W32);
+        W32 = nullptr;
+        dscale = nullptr;
+        x = r = b = nullptr;
+        t32 = nullptr;
+        norms = hnorms = nullptr;
+    }
+    ~Solver() { free_vecs(); }
+
+    int read_info(LUWork<double>& w, int* info)
+    {
+        int f[2];
+        TAGS_HIP(hipMemcpyAsync(f, w.flags, sizeof(f), hipMemcpyDeviceToHost, st));
+        TAGS_HIP(hipStreamSynchronize(st));
+        if (f[1]) return TAGS_ERR_BARRIER;
+        *info = f[0];
+        return 0;
+    }
+
+    // Plain form: factor W (destroyed) in FP64 and solve; x returned in `b`.
+    int solve64(c64* W, c64* bvec, tags_item_stats* stt)
+    {
+        int rc = lu_factor<double>(w64, W, n, st, pr);
+        if (rc) return rc;
+        int info = 0;
+        if ((rc = read_info(w64, &info))) return rc;
+        if (stt) {
+            stt->precision_used = TAGS_PREC_FP64;
+            stt->refine_iters = 0;
+            stt->info = info;
+            stt->backward_err = -1.0;
+        }
+        if (info) return info;
+        if ((rc = lu_gather_rhs<double>(w64, bvec, w64.vec, st))) return rc;
+        if ((rc = lu_solve_inplace<double>(w64, W, n, w64.vec, st, pr))) return rc;
+        TAGS_HIP(hipMemcpyAsync(bvec, w64.vec, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
+        return 0;
+    }
+
+    // Mixed form: W (FP64, kept intact unless the FP64 fallback runs), b in
+    // `bvec`; solution returned in `bvec`.
+    int solve_mixed(c64* W, c64* bvec, tags_item_stats* stt)
+    {
+        int rc;
+        {
+            Timed tm(pr, FAM_OTHER, st);
+            if ((rc = row_scale(W, n, n, dscale, st))) return rc;
+            if ((rc = to_c32_scaled(W, n, n, dscale, W32, n, st))) return rc;
+            tm.end();
+        }
+        if ((rc = lu_factor<float>(w32, W32, n, st, pr))) return rc;
+        int f[2];
+        TAGS_HIP(hipMemcpyAsync(f, w32.flags, sizeof(f), hipMemcpyDeviceToHost, st));
+        TAGS_HIP(hipStreamSynchronize(st));
+        if (f[1]) return TAGS_ERR_BARRIER;
+        const int info32 = f[0];
+        TAGS_HIP(hipMemcpyAsync(b, bvec, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
+        TAGS_HIP(hipMemsetAsync(x, 0, sizeof(c64) * n, st));
+        double wnorm = 0, rnorm = 0, xnorm = 0, berr = 1.0, prev = 2.0;
+        int it = 0;
+        bool converged = false;
+        if (info32 == 0) {
+            // r = b initially (x = 0); |r| from a residual pass that also gives |W|
+            if ((rc = residual64(W, n, n, x, b, r, norms, true, st, pr))) return rc;
+            TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * 3,
+                                    hipMemcpyDeviceToHost, st));
+            TAGS_HIP(hipStreamSynchronize(st));
+            std::memcpy(&rnorm, &hnorms[0], 8);
+            std::memcpy(&wnorm, &hnorms[2], 8);
+            for (it = 0; it < 12 && rnorm > 0; it++) {
+                if ((rc = prep_rhs32(r, dscale, w32.perm, n, 1.0 / rnorm, t32, st))) return rc;
+                if ((rc = lu_solve_inplace<float>(w32, W32, n, t32, st, pr))) return rc;
+                if ((rc = update_x(x, t32, dscale, n, rnorm, st))) return rc;
+                if ((rc = residual64(W, n, n, x, b, r, norms, false, st, pr))) return rc;
+                TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * 2,
+                                        hipMemcpyDeviceToHost, st));
+                TAGS_HIP(hipStreamSynchronize(st));
+                std::memcpy(&rnorm, &hnorms[0], 8);
+                std::memcpy(&xnorm, &hnorms[1], 8);
+                berr = (wnorm > 0 && xnorm > 0) ? rnorm / (wnorm * xnorm) : 0.0;
+                if (!(berr == berr)) break;  // NaN
+                if (berr <= 1e-15) {
+                    converged = true;
+                    it++;
+                    break;
+                }
+                if (berr > 0.5 * prev) {
+                    it++;
+                    break;
+                }
+                prev = berr;
+            }
+            if (!converged && berr <= 1e-13) converged = true;
+        }
+        if (converged) {
+            if (stt) {
+                stt->precision_used = TAGS_PREC_MIXED;
+                stt->refine_iters = it;
+                stt->info = 0;
+                stt->backward_err = berr;
+            }
+            TAGS_HIP(hipMemcpyAsync(bvec, x, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
+            return 0;
+        }
+        // FP64 fallback on the intact W
+        rc = solve64(W, bvec, stt);
+        if (stt) stt->refine_iters = it;
+        return rc;
+    }
+};
+
+}  // namespace tags
+
+using namespace tags;
+
+extern "C" {
+
+const char* tags_version(void) { return "tags_hip 0.1 (gfx950)"; }
+
+int tags_device_synchronize(void)
+{
+    TAGS_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// geometry (host code), src/electrode.c:12-32, 103-127; src/grid.c:9-91
+// ---------------------------------------------------------------------------
+static double vlen(const double a[3], const double b[3])
+{
+    double r = 0.0;
+    for (int i = 0; i < 3; i++) r += std::pow(a[i] - b[i], 2.0);
+    return std::sqrt(r);
+}
+
+int populate_electrode(Electrode* e, const double start[3], const double end[3], double radius)
+{
+    bool eq = true;
+    for (int i = 0; i < 3; i++)
+        if (std::fabs(start[i] - end[i]) > DBL_EPSILON) eq = false;
+    if (eq) {
+        std::printf("Error: start_point and end_point are equal.\n");
+        return 1;
+    }
+    if (radius <= 0) {
+        std::printf("Error: radius < 0.\n");
+        return 2;
+    }
+    e->radius = radius;
+    e->length = vlen(start, end);
+    for (int i = 0; i < 3; i++) {
+        e->start_point[i] = start[i];
+        e->end_point[i] = end[i];
+        e->middle_point[i] = (start[i] + end[i]) / 2.0;
+    }
+    return 0;
+}
+
+static void lin(double a, double b, size_t n, double* u)
+{
+    const double c = (b - a) / (n - 1);
+    for (size_t i = 0; i < n - 1; ++i) u[i] = a + i * c;
+    u[n - 1] = b;
+}
+
+int segment_electrode(Electrode* e, double* nodes, size_t num_segments, const double* start,
+                      const double* end, double radius)
+{
+    if (num_segments < 1) {
+        std::printf("Error: number of segments should be greater than 0.\n");
+        return 1;
+    }
+    const size_t nn = num_segments + 1;
+    std::vector<double> x(nn), y(nn), z(nn);
+    lin(start[0], end[0], nn, x.data());
+    lin(start[1], end[1], nn, y.data());
+    lin(start[2], end[2], nn, z.data());
+    for (size_t i = 0; i < nn; i++) {
+        nodes[3 * i] = x[i];
+        nodes[3 * i + 1] = y[i];
+        nodes[3 * i + 2] = z[i];
+    }
+    for (size_t i = 0; i < num_segments; i++)
+        populate_electrode(&e[i], nodes + 3 * i, nodes + 3 * i + 3, radius);
+    return 0;
+}
+
+int number_segments(const Grid g)
+{
+    const unsigned N = g.edge_segments_x, vx = g.vertices_x;
+    const unsigned M = g.edge_segments_y, vy = g.vertices_y;
+    return (int)(N * vy * (vx - 1) + M * vx * (vy - 1));
+}
+
+int number_nodes(const Grid g)
+{
+    const unsigned N = g.edge_segments_x, vx = g.vertices_x;
+    const unsigned M = g.edge_segments_y, vy = g.vertices_y;
+    return (int)(vx * vy + vx * (vy - 1) * (M - 1) + vy * (vx - 1) * (N - 1));
+}
+
+int electrode_grid(const Grid g, Electrode* e, double* nodes)
+{
+    const size_t N = g.edge_segments_x, vx = g.vertices_x;
+    const size_t M = g.edge_segments_y, vy = g.vertices_y;
+    const double lx = g.length_x / (N * (vx - 1));
+    const double ly = g.length_y / (M * (vy - 1));
+    double sp[3] = {0.0, 0.0, g.depth}, ep[3] = {0.0, 0.0, g.depth};
+    size_t ed = 0, nd = 0;
+    for (size_t h = 0; h < vy; h++)
+        for (size_t n = 0; n < vx - 1; n++)
+            for (size_t k = 0; k < N; k++) {
+                sp[0] = lx * (n * N + k);
+                sp[1] = ly * M * h;
+                ep[0] = sp[0] + lx;
+                ep[1] = sp[1];
+                populate_electrode(&e[ed++], sp, ep, g.radius);
+                if (n == 0 && k == 0) {
+                    std::memcpy(nodes + 3 * nd, sp, sizeof(sp));
+                    nd++;
+                }
+                std::memcpy(nodes + 3 * nd, ep, sizeof(ep));
+                nd++;
+            }
+    for (size_t gg = 0; gg < vx; gg++)
+        for (size_t mm = 0; mm < vy - 1; mm++)
+            for (size_t k = 0; k < M; k++) {
+                sp[0] = lx * N * gg;
+                sp[1] = ly * (mm * M + k);
+                ep[0] = sp[0];
+                ep[1] = sp[1] + ly;
+                populate_electrode(&e[ed++], sp, ep, g.radius);
+                if (k < M - 1) {
+                    std::memcpy(nodes + 3 * nd, ep, sizeof(ep));
+                    nd++;
+                }
+            }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// integrals
+// ---------------------------------------------------------------------------
+TAGS_COMPLEX self_integral(const Electrode* sender)
+{
+    const double ls = sender->length;
+    const double k1 = sender->radius / ls;
+    const double k2 = std::sqrt(1.0 + k1 * k1);
+    return to_c(c64{2.0 * ls * (std::log((k2 + 1.0) / k1) - k2 + k1), 0.0});
+}
+
+int integral(const Electrode* sender, const Electrode* receiver, TAGS_COMPLEX gamma,
+             size_t max_eval, double req_abs_error, double req_rel_error, int integration_type,
+             double result[2], double error[2])
+{
+    (void)max_eval;
+    (void)req_abs_error;
+    (void)req_rel_error;
+    result[0] = result[1] = error[0] = error[1] = 0.0;
+    if (integration_type == INTG_NONE) {
+        result[0] = vlen(sender->middle_point, receiver->middle_point);
+        return 0;
+    }
+    if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
+        std::printf("integration type %d is not part of this build\n", integration_type);
+        return TAGS_ERR_UNSUPPORTED;
+    }
+    if (integration_type != INTG_DOUBLE) {
+        std::printf("integration failed with code: %i\n", -10);
+        return -10;
+    }
+    Stream S;
+    DevSegs a, b;
+    int rc;
+    if ((rc = a.upload(sender, 1, S.s))) return rc;
+    if ((rc = b.upload(receiver, 1, S.s))) return rc;
+    DevBuf<double> out;
+    if ((rc = out.alloc(4))) return rc;
+    if ((rc = launch_single_pair(a.soa(), b.soa(), from_c(gamma), out.p, S.s))) return rc;
+    double h[4];
+    TAGS_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    result[0] = h[0];
+    result[1] = h[1];
+    error[0] = h[2];
+    error[1] = h[3];
+    return 0;
+}
+
+static int pairs_dropin(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* e,
+                        const Electrode* img, size_t m, const PairScalars& sc, bool images)
+{
+    if (m == 0) return 0;
+    if (!zl || !zt || !e) return TAGS_ERR_ARG;
+    Stream S;
+    DevSegs de, di;
+    int rc;
+    if ((rc = de.upload(e, (long)m, S.s))) return rc;
+    if ((rc = di.upload(img ? img : e, (long)m, S.s))) return rc;
+    const bool alias = (void*)zl == (void*)zt;
+    const size_t bytes = sizeof(c64) * m * m;
+    DevBuf<c64> dl, dt;
+    if ((rc = dl.alloc(m * m))) return rc;
+    TAGS_HIP(hipMemcpyAsync(dl.p, zl, bytes, hipMemcpyHostToDevice, S.s));
+    c64* dtp = dl.p;
+    if (!alias) {
+        if ((rc = dt.alloc(m * m))) return rc;
+        TAGS_HIP(hipMemcpyAsync(dt.p, zt, bytes, hipMemcpyHostToDevice, S.s));
+        dtp = dt.p;
+    }
+    if ((rc = launch_pairs_dropin((double*)dl.p, (double*)dtp, alias, de.soa(), di.soa(),
+                                  (long)m, sc, images, S.s)))
+        return rc;
+    TAGS_HIP(hipMemcpyAsync(zl, dl.p, bytes, hipMemcpyDeviceToHost, S.s));
+    if (!alias) TAGS_HIP(hipMemcpyAsync(zt, dtp, bytes, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    return 0;
+}
+
+int calculate_impedances(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* electrodes,
+                         size_t num_electrodes, TAGS_COMPLEX gamma, TAGS_COMPLEX s, double mur,
+                         TAGS_COMPLEX kappa, size_t max_eval, double req_abs_error,
+                         double req_rel_error, int integration_type)
+{
+    (void)max_eval;
+    (void)req_abs_error;
+    (void)req_rel_error;
+    if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
+        std::printf("integration type %d is not part of this build\n", integration_type);
+        return TAGS_ERR_UNSUPPORTED;
+    }
+    if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE) {
+        std::printf("integration failed with code: %i\n", -10);
+        return -10;
+    }
+    PairScalars sc = make_scalars(from_c(gamma), from_c(s), mur, from_c(kappa), c64{1, 0},
+                                  c64{1, 0}, integration_type);
+    return pairs_dropin(zl, zt, electrodes, nullptr, num_electrodes, sc, false);
+}
+
+int impedances_images(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* electrodes,
+                      const Electrode* images, size_t num_electrodes, TAGS_COMPLEX gamma,
+                      TAGS_COMPLEX s, double mur, TAGS_COMPLEX kappa, TAGS_COMPLEX ref_l,
+                      TAGS_COMPLEX ref_t, size_t max_eval, double req_abs_error,
+                      double req_rel_error, int integration_type)
+{
+    (void)max_eval;
+    (void)req_abs_error;
+    (void)req_rel_error;
+    if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
+        std::printf("integration type %d is not part of this build\n", integration_type);
+        return TAGS_ERR_UNSUPPORTED;
+    }
+    if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE) {
+        std::printf("integration failed with code: %i\n", -10);
+        return -10;
+    }
+    if (!images) return TAGS_ERR_ARG;
+    PairScalars sc = make_scalars(from_c(gamma), from_c(s), mur, from_c(kappa), from_c(ref_l),
+                                  from_c(ref_t), integration_type);
+    return pairs_dropin(zl, zt, electrodes, images, num_electrodes, sc, true);
+}
+
+// ---------------------------------------------------------------------------
+// fill_incidence_imm (src/linalg.c:12-86)
+// ---------------------------------------------------------------------------
+}  // extern "C"
+
+namespace tags {
+// blk1: rows n..N of columns 0..nw (column-major, ld 2m): A (rows s) then B
+// (rows m+s); blk2: rows 0..nw of columns n..N (column-major, ld nw).
+// counts[0] = #start matches, counts[1] = #end matches (over nodes < nw).
+__global__ __launch_bounds__(256) void k_incidence_blocks(const double* __restrict__ sp, long m,
+                                                          const double* __restrict__ nodes,
+                                                          long nw, c64* blk1, c64* blk2,
+                                                          unsigned long long* counts)
+{
+    const double tol = 1e-5;
+    unsigned long long cs = 0, ce = 0;
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < nw * m;
+         idx += (long)gridDim.x * 256) {
+        const long nd = idx / m, s = idx % m;
+        const double x = nodes[3 * nd], y = nodes[3 * nd + 1], z = nodes[3 * nd + 2];
+        const bool st = fabs(sp[6 * s] - x) <= tol && fabs(sp[6 * s + 1] - y) <= tol &&
+                        fabs(sp[6 * s + 2] - z) <= tol;
+        const bool ed = !st && fabs(sp[6 * s + 3] - x) <= tol && fabs(sp[6 * s + 4] - y) <= tol &&
+                        fabs(sp[6 * s + 5] - z) <= tol;
+        const double a = st ? 1.0 : (ed ? -1.0 : 0.0);
+        const double b = (st || ed) ? 0.5 : 0.0;
+        blk1[s + 2 * m * nd] = c64{a, 0.0};
+        blk1[m + s + 2 * m * nd] = c64{b, 0.0};
+        blk2[nd + nw * s] = c64{a, 0.0};
+        blk2[nd + nw * (m + s)] = c64{b, 0.0};
+        cs += st;
+        ce += ed;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        cs += __shfl_xor(cs, off);
+        ce += __shfl_xor(ce, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&counts[0], cs);
+        atomicAdd(&counts[1], ce);
+    }
+}
+}  // namespace tags
+
+extern "C" {
+
+static std::vector<double> seg_points(const Electrode* e, size_t m)
+{
+    std::vector<double> sp(6 * m);
+    for (size_t s = 0; s < m; s++)
+        for (int c = 0; c < 3; c++) {
+            sp[6 * s + c] = e[s].start_point[c];
+            sp[6 * s + 3 + c] = e[s].end_point[c];
+        }
+    return sp;
+}
+
+int fill_incidence_imm(TAGS_COMPLEX* wg, const Electrode* electrodes, size_t num_electrodes,
+                       const double* nodes, size_t num_nodes)
+{
+    const long m = (long)num_electrodes, n = (long)num_nodes, N = n + 2 * m;
+    if (m == 0 || n == 0) return 0;
+    Stream S;
+    int rc;
+    std::vector<double> sp = seg_points(electrodes, m);
+    DevBuf<double> dsp, dnodes;
+    DevBuf<int> dsn, den, dhits;
+    if ((rc = dsp.alloc(6 * m)) || (rc = dnodes.alloc(3 * n)) || (rc = dsn.alloc(m)) ||
+        (rc = den.alloc(m)) || (rc = dhits.alloc(n + 2 * m)))
+        return rc;
+    TAGS_HIP(hipMemcpyAsync(dsp.p, sp.data(), sizeof(double) * 6 * m, hipMemcpyHostToDevice, S.s));
+    TAGS_HIP(hipMemcpyAsync(dnodes.p, nodes, sizeof(double) * 3 * n, hipMemcpyHostToDevice, S.s));
+    TAGS_HIP(hipMemsetAsync(dhits.p, 0, sizeof(int) * (n + 2 * m), S.s));
+    if ((rc = launch_incidence(dsp.p, m, dnodes.p, n, dsn.p, den.p, dhits.p, S.s))) return rc;
+    std::vector<int> hits(n + 2 * m);
+    TAGS_HIP(hipMemcpyAsync(hits.data(), dhits.p, sizeof(int) * n, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    // the reference stops after the first node without a segment (:72-75)
+    long nw = n, bad = -1;
+    for (long i = 0; i < n; i++)
+        if (hits[i] == 0) {
+            bad = i;
+            nw = i + 1;
+            break;
+        }
+    DevBuf<c64> b1, b2;
+    DevBuf<unsigned long long> cnt;
+    if ((rc = b1.alloc((size_t)2 * m * nw)) || (rc = b2.alloc((size_t)2 * m * nw)) ||
+        (rc = cnt.alloc(2)))
+        return rc;
+    TAGS_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, S.s));
+    {
+        long tot = nw * m;
+        unsigned g = (unsigned)std::min<long>((tot + 255) / 256, 4096);
+        hipLaunchKernelGGL(k_incidence_blocks, dim3(g), dim3(256), 0, S.s, dsp.p, m, dnodes.p, nw,
+                           b1.p, b2.p, cnt.p);
+        TAGS_HIP(hipGetLastError());
+    }
+    // blk1 -> W rows n..N, columns 0..nw ; blk2 -> W rows 0..nw, columns n..N
+    c64* W = (c64*)wg;
+    TAGS_HIP(hipMemcpy2DAsync(W + n, sizeof(c64) * N, b1.p, sizeof(c64) * 2 * m,
+                              sizeof(c64) * 2 * m, nw, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipMemcpy2DAsync(W + N * n, sizeof(c64) * N, b2.p, sizeof(c64) * nw,
+                              sizeof(c64) * nw, 2 * m, hipMemcpyDeviceToHost, S.s));
+    unsigned long long c[2];
+    TAGS_HIP(hipMemcpyAsync(c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    if (bad >= 0) {
+        std::printf("No electrode is connected to node[%i]\n", (int)bad);
+        return 1;
+    }
+    const long suma = (long)c[0] - (long)c[1];
+    const double sumb = 0.5 * (double)(c[0] + c[1]);
+    if (suma != 0) {
+        std::printf("incidence matrix A is wrong. sum(A) = %li != 0\n", suma);
+        return (int)-suma;
+    }
+    if (sumb != (double)m) {
+        std::printf("incidence matrix B is wrong. sum(B) = %.1f != %.1f\n", sumb, (double)m);
+        return (int)(sumb - m);
+    }
+    return 0;
+}
+
+}  // extern "C"
+
+namespace tags {
+// Z-region of W (rows/cols n..N), column-major with ld 2m:
+// [[-zl, 0], [0, -zt]] with zl[i + m k] at (i, k) (src/linalg.c:96-103).
+__global__ void k_zregion(const c64* zl, const c64* zt, long m, c64* blk)
+{
+    const long M2 = 2 * m;
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < M2 * M2;
+         idx += (long)gridDim.x * 256) {
+        const long col = idx / M2, row = idx % M2;
+        c64 v = c64{0.0, 0.0};
+        if (row < m && col < m) {
+            const c64 z = zl[row + m * col];
+            v = c64{-z.x, -z.y};
+        } else if (row >= m && col >= m) {
+            const c64 z = zt[(row - m) + m * (col - m)];
+            v = c64{-z.x, -z.y};
+        }
+        blk[idx] = v;
+    }
+}
+}  // namespace tags
+
+extern "C" {
+
+int fill_impedance_imm(TAGS_COMPLEX* wg, const TAGS_COMPLEX* zl, const TAGS_COMPLEX* zt,
+                       size_t num_electrodes, size_t num_nodes)
+{
+    const long m = (long)num_electrodes, n = (long)num_nodes, N = n + 2 * m;
+    Stream S;
+    int rc;
+    c64* W = (c64*)wg;
+    if (m > 0) {
+        DevBuf<c64> dl, dt, blk;
+        if ((rc = dl.alloc(m * m)) || (rc = dt.alloc(m * m)) || (rc = blk.alloc(4 * m * m)))
+            return rc;
+        TAGS_HIP(hipMemcpyAsync(dl.p, zl, sizeof(c64) * m * m, hipMemcpyHostToDevice, S.s));
+        TAGS_HIP(hipMemcpyAsync(dt.p, zt, sizeof(c64) * m * m, hipMemcpyHostToDevice, S.s));
+        unsigned g = (unsigned)std::min<long>((4 * m * m + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_zregion, dim3(g), dim3(256), 0, S.s, dl.p, dt.p, m, blk.p);
+        TAGS_HIP(hipGetLastError());
+        TAGS_HIP(hipMemcpy2DAsync(W + n + N * n, sizeof(c64) * N, blk.p, sizeof(c64) * 2 * m,
+                                  sizeof(c64) * 2 * m, 2 * m, hipMemcpyDeviceToHost, S.s));
+        TAGS_HIP(hipStreamSynchronize(S.s));
+    }
+    for (long k = 0; k < n; k++) std::memset(W + N * k, 0, sizeof(c64) * n);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// solve_immittance (src/linalg.c:113-139)
+// ---------------------------------------------------------------------------
+static int dense_solve_host(const c64* a, c64* bx, long n, int precision, bool sym_lower_cm,
+                            tags_item_stats* stt)
+{
+    if (n <= 0) return 0;
+    if (n >= 65536) return TAGS_ERR_TOO_LARGE;
+    Stream S;
+    Solver sv;
+    sv.st = S.s;
+    int rc;
+    const bool mixed = precision == TAGS_PREC_MIXED;
+    if ((rc = sv.ensure((int)n, mixed))) return rc;
+    DevBuf<c64> W, bv;
+    if ((rc = W.alloc((size_t)n * n)) || (rc = bv.alloc(n))) return rc;
+    TAGS_HIP(hipMemcpyAsync(W.p, a, sizeof(c64) * n * n, hipMemcpyHostToDevice, S.s));
+    TAGS_HIP(hipMemcpyAsync(bv.p, bx, sizeof(c64) * n, hipMemcpyHostToDevice, S.s));
+    if (sym_lower_cm && (rc = sym_from_upper(W.p, n, (int)n, S.s))) return rc;
+    rc = mixed ? sv.solve_mixed(W.p, bv.p, stt) : sv.solve64(W.p, bv.p, stt);
+    if (rc) return rc;
+    TAGS_HIP(hipMemcpyAsync(bx, bv.p, sizeof(c64) * n, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    return 0;
+}
+
+int solve_immittance(TAGS_COMPLEX* wg, TAGS_COMPLEX* ie, size_t num_electrodes, size_t num_nodes)
+{
+    const long N = (long)(2 * num_electrodes + num_nodes);
+    int rc = dense_solve_host((const c64*)wg, (c64*)ie, N, TAGS_PREC_FP64, true, nullptr);
+    if (rc > 0) {
+        std::printf("The diagonal element of the triangular factor of YN,\n");
+        std::printf("U(%i,%i) is zero, so that YN is singular;\n", rc, rc);
+        std::printf("the solution could not be computed.\n");
+    }
+    return rc;
+}
+
+int tags_dense_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t n, int precision,
+                     tags_item_stats* stats)
+{
+    return dense_solve_host((const c64*)a, (c64*)b, (long)n, precision, false, stats);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// device-resident sweep
+// ---------------------------------------------------------------------------
+struct tags_sweep {
+    long m = 0, n = 0, N = 0;
+    hipStream_t st = nullptr;
+    tags::DevSegs e, img;
+    int* sn = nullptr;
+    int* en = nullptr;
+    tags::c64* W = nullptr;
+    tags::c64* rhs = nullptr;
+    tags::Solver sv;
+    tags::Profiler prof;
+};
+
+extern "C" {
+
+int tags_sweep_create(tags_sweep** out, const Electrode* electrodes, const Electrode* images,
+                      size_t num_electrodes, const double* nodes, size_t num_nodes)
+{
+    if (!out || !electrodes || !nodes || num_electrodes == 0 || num_nodes == 0)
+        return TAGS_ERR_ARG;
+    std::unique_ptr<tags_sweep> sw(new tags_sweep());
+    sw->m = (long)num_electrodes;
+    sw->n = (long)num_nodes;
+    sw->N = sw->n + 2 * sw->m;
+    if (sw->N >= 65536) return TAGS_ERR_TOO_LARGE;
+    TAGS_HIP(hipStreamCreate(&sw->st));
+    int rc;
+    std::vector<Electrode> im;
+    if (!images) {
+        im.assign(electrodes, electrodes + num_electrodes);
+        for (auto& x : im) {
+            x.start_point[2] = -x.start_point[2];
+            x.end_point[2] = -x.end_point[2];
+            x.middle_point[2] = -x.middle_point[2];
+        }
+        images = im.data();
+    }
+    if ((rc = sw->e.upload(electrodes, sw->m, sw->st))) return rc;
+    if ((rc = sw->img.upload(images, sw->m, sw->st))) return rc;
+    const long m = sw->m, n = sw->n;
+    std::vector<double> sp = seg_points(electrodes, m);
+    DevBuf<double> dsp, dnodes;
+    DevBuf<int> dhits;
+    if ((rc = dsp.alloc(6 * m)) || (rc = dnodes.alloc(3 * n)) || (rc = dhits.alloc(n + 2 * m)))
+        return rc;
+    TAGS_HIP(hipMalloc(&sw->sn, sizeof(int) * m));
+    TAGS_HIP(hipMalloc(&sw->en, sizeof(int) * m));
+    TAGS_HIP(hipMemsetAsync(sw->sn, 0xff, sizeof(int) * m, sw->st));
+    TAGS_HIP(hipMemsetAsync(sw->en, 0xff, sizeof(int) * m, sw->st));
+    TAGS_HIP(hipMemcpyAsync(dsp.p, sp.data(), sizeof(double) * 6 * m, hipMemcpyHostToDevice,
+                            sw->st));
+    TAGS_HIP(hipMemcpyAsync(dnodes.p, nodes, sizeof(double) * 3 * n, hipMemcpyHostToDevice,
+                            sw->st));
+    TAGS_HIP(hipMemsetAsync(dhits.p, 0, sizeof(int) * (n + 2 * m), sw->st));
+    if ((rc = launch_incidence(dsp.p, m, dnodes.p, n, sw->sn, sw->en, dhits.p, sw->st))) return rc;
+    std::vector<int> hits(n + 2 * m);
+    TAGS_HIP(hipMemcpyAsync(hits.data(), dhits.p, sizeof(int) * (n + 2 * m),
+                            hipMemcpyDeviceToHost, sw->st));
+    TAGS_HIP(hipStreamSynchronize(sw->st));
+    for (long i = 0; i < n; i++)
+        if (hits[i] == 0) {
+            std::printf("No electrode is connected to node[%i]\n", (int)i);
+            return 1;
+        }
+    long cs = 0, ce = 0;
+    for (long s = 0; s < m; s++) {
+        cs += hits[n + 2 * s];
+        ce += hits[n + 2 * s + 1];
+        if (hits[n + 2 * s] != 1 || hits[n + 2 * s + 1] != 1) {
+            // every segment end must match exactly one node for the sweep
+            if (cs != ce) return (int)-(cs - ce);
+            return TAGS_ERR_ARG;
+        }
+    }
+    sw->sv.st = sw->st;
+    sw->sv.pr = &sw->prof;
+    *out = sw.release();
+    return 0;
+}
+
+int tags_sweep_set_profiling(tags_sweep* sw, int on)
+{
+    if (!sw) return TAGS_ERR_ARG;
+    sw->prof.on = on != 0;
+    sw->prof.reset();
+    return 0;
+}
+
+int tags_sweep_get_timers(tags_sweep* sw, tags_timers* t)
+{
+    if (!sw || !t) return TAGS_ERR_ARG;
+    TAGS_HIP(hipStreamSynchronize(sw->st));
+    sw->prof.resolve();
+    t->ms_fill = sw->prof.ms[FAM_FILL];
+    t->ms_panel = sw->prof.ms[FAM_PANEL];
+    t->ms_gemm = sw->prof.ms[FAM_GEMM];
+    t->ms_other = sw->prof.ms[FAM_OTHER];
+    t->ms_solve = sw->prof.ms[FAM_SOLVE];
+    t->ms_residual = sw->prof.ms[FAM_RESID];
+    t->gemm_flops = sw->prof.gemm_flops;
+    t->trailing_ms = sw->prof.trailing_ms;
+    t->trailing_flops = sw->prof.trailing_flops;
+    t->trailing_launches = sw->prof.trailing_launches;
+    return 0;
+}
+
+void tags_sweep_destroy(tags_sweep* sw)
+{
+    if (!sw) return;
+    hipStreamSynchronize(sw->st);
+    hipFree(sw->sn);
+    hipFree(sw->en);
+    hipFree(sw->W);
+    hipFree(sw->rhs);
+    hipStream_t st = sw->st;
+    delete sw;
+    hipStreamDestroy(st);
+}
+
+int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
+                     const TAGS_COMPLEX* kappa, double mur, const TAGS_COMPLEX* gamma,
+                     const TAGS_COMPLEX* ref_l, const TAGS_COMPLEX* ref_t, size_t inj_node,
+                     int precision, TAGS_COMPLEX* u_inj, TAGS_COMPLEX* x_out,
+                     tags_item_stats* stats)
+{
+    if (!sw || !s || !kappa) return TAGS_ERR_ARG;
+    if ((long)inj_node >= sw->n) return TAGS_ERR_ARG;
+    const long N = sw->N, n = sw->n, m = sw->m;
+    const bool mixed = precision == TAGS_PREC_MIXED;
+    int rc;
+    if ((rc = sw->sv.ensure((int)N, mixed))) return rc;
+    if (!sw->W) TAGS_HIP(hipMalloc(&sw->W, sizeof(c64) * N * N));
+    if (!sw->rhs) TAGS_HIP(hipMalloc(&sw->rhs, sizeof(c64) * N));
+    for (size_t f = 0; f < nfreq; f++) {
+        const c64 sf = from_c(s[f]), kf = from_c(kappa[f]);
+        c64 gf, rl, rt;
+        if (gamma) {
+            gf = from_c(gamma[f]);
+        } else {
+            // gamma = csqrt(s mu0 mur kappa), principal branch
+            // (examples/harmonic_impedance.c:118)
+            const c64 z = cmul(c64{sf.x * kMu0 * mur, sf.y * kMu0 * mur}, kf);
+            const std::complex<double> g = std::sqrt(std::complex<double>(z.x, z.y));
+            gf = c64{g.real(), g.imag()};
+        }
+        rl = ref_l ? from_c(ref_l[f]) : c64{1.0, 0.0};
+        if (ref_t) {
+            rt = from_c(ref_t[f]);
+        } else {
+            // (kappa - s eps0)/(kappa + s eps0), examples/harmonic_impedance.c:122
+            const c64 se = c64{sf.x * kEps0, sf.y * kEps0};
+            rt = cdiv(csub(kf, se), cadd(kf, se));
+        }
+        PairScalars sc = make_scalars(gf, sf, mur, kf, rl, rt, INTG_DOUBLE);
+        {
+            Timed tm(&sw->prof, FAM_FILL, sw->st);
+            if ((rc = launch_fill_static(sw->W, N, n, m, sw->sn, sw->en, sw->st))) return rc;
+            if ((rc = launch_fill_zblocks(sw->W, N, n, m, sw->e.soa(), sw->img.soa(), sc, sw->st)))
+                return rc;
+            tm.end();
+        }
+        // b = unit current at the injection node
+        if ((rc = fill_c64(sw->rhs, N, c64{0.0, 0.0}, sw->st))) return rc;
+        if ((rc = fill_c64(sw->rhs + inj_node, 1, c64{1.0, 0.0}, sw->st))) return rc;
+        tags_item_stats stt{};
+        rc = mixed ? sw->sv.solve_mixed(sw->W, sw->rhs, &stt) : sw->sv.solve64(sw->W, sw->rhs, &stt);
+        if (rc < 0) return rc;
+        if (stats) stats[f] = stt;
+        if (rc > 0) {
+            if (u_inj) u_inj[f] = to_c(c64{NAN, NAN});
+            continue;
+        }
+        c64 u;
+        TAGS_HIP(hipMemcpyAsync(&u, sw->rhs + inj_node, sizeof(c64), hipMemcpyDeviceToHost,
+                                sw->st));
+        if (x_out)
+            TAGS_HIP(hipMemcpyAsync(x_out + f * N, sw->rhs, sizeof(c64) * N,
+                                    hipMemcpyDeviceToHost, sw->st));
+        TAGS_HIP(hipStreamSynchronize(sw->st));
+        if (u_inj) u_inj[f] = to_c(u);
+    }
+    if (sw->prof.on) {
+        TAGS_HIP(hipStreamSynchronize(sw->st));
+        sw->prof.resolve();
+    }
+    return 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,122 @@
+// Shared definitions of the HIP implementation of the per-frequency HEM solve.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+
+#include "tags_hip.h"
+
+namespace tags {
+
+// Interleaved complex value as the reference stores _Complex double.
+template <typename T>
+struct alignas(2 * sizeof(T)) cx {
+    T x, y;
+};
+using c64 = cx<double>;
+using c32 = cx<float>;
+
+template <typename T>
+__host__ __device__ inline cx<T> cmk(T a, T b) { return cx<T>{a, b}; }
+template <typename T>
+__host__ __device__ inline cx<T> cadd(cx<T> a, cx<T> b) { return cx<T>{a.x + b.x, a.y + b.y}; }
+template <typename T>
+__host__ __device__ inline cx<T> csub(cx<T> a, cx<T> b) { return cx<T>{a.x - b.x, a.y - b.y}; }
+template <typename T>
+__host__ __device__ inline cx<T> cmul(cx<T> a, cx<T> b) {
+    return cx<T>{a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__host__ __device__ inline cx<T> cscale(cx<T> a, T s) { return cx<T>{a.x * s, a.y * s}; }
+// Reciprocal with the larger component as pivot (Smith's scaling) to avoid
+// overflow for tiny or huge values.
+template <typename T>
+__host__ __device__ inline cx<T> crecip(cx<T> a) {
+    T ax = a.x < 0 ? -a.x : a.x, ay = a.y < 0 ? -a.y : a.y;
+    if (ax >= ay) {
+        T r = a.y / a.x, d = a.x + a.y * r;
+        return cx<T>{T(1) / d, -r / d};
+    }
+    T r = a.x / a.y, d = a.y + a.x * r;
+    return cx<T>{r / d, T(-1) / d};
+}
+template <typename T>
+__host__ __device__ inline cx<T> cdiv(cx<T> a, cx<T> b) { return cmul(a, crecip(b)); }
+template <typename T>
+__host__ __device__ inline T cabs1(cx<T> a) {
+    return (a.x < 0 ? -a.x : a.x) + (a.y < 0 ? -a.y : a.y);
+}
+
+inline c64 from_c(double _Complex z) { return c64{__real__ z, __imag__ z}; }
+inline double _Complex to_c(c64 z) {
+    double _Complex r;
+    __real__ r = z.x;
+    __imag__ r = z.y;
+    return r;
+}
+
+// Library error codes beyond the reference's own (all negative and below -10,
+// which the reference already uses for an unknown integration type).
+enum : int {
+    TAGS_ERR_HIP = -100,          // a HIP runtime call failed
+    TAGS_ERR_UNSUPPORTED = -11,   // integration type not implemented on the GPU
+    TAGS_ERR_ARG = -12,           // bad size or pointer
+    TAGS_ERR_TOO_LARGE = -13,     // problem exceeds a kernel's design limit
+    TAGS_ERR_BARRIER = -14,       // a grid barrier timed out (kernel exited early)
+};
+
+// Reports and converts a HIP error; used by every host entry point.
+#define TAGS_HIP(call)                                                                  \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            std::fprintf(stderr, "tags_hip: %s failed: %s (%s:%d)\n", #call,           \
+                         hipGetErrorString(e_), __FILE__, __LINE__);                    \
+            return TAGS_ERR_HIP;                                                        \
+        }                                                                               \
+    } while (0)
+
+// Physical constants with the reference's literal values (src/auxiliary.c:17-21).
+constexpr double kPi = 3.1415926535897932384626433832795029;
+constexpr double kTwoPi = 6.283185307179586476925286766559;
+constexpr double kFourPi = 12.5663706143591729538505735331;
+constexpr double kMu0 = 1.256637061435917e-6;
+constexpr double kEps0 = 8.854187817620e-12;
+
+// Segment geometry in structure-of-arrays form on the device.
+struct SegSoA {
+    const double* ax;  // start point
+    const double* ay;
+    const double* az;
+    const double* bx;  // end point
+    const double* by;
+    const double* bz;
+    const double* len;
+    const double* rad;
+};
+
+// Scalars of one frequency sample that the pair kernels need.
+struct PairScalars {
+    c64 gamma;
+    c64 zl_dir, zt_dir;   // s mu / 4 pi and 1 / (4 pi kappa)
+    c64 zl_img, zt_img;   // ref_l s mu / 4 pi and ref_t / (4 pi kappa)
+    int intg_type;        // INTG_DOUBLE or INTG_NONE
+};
+
+// ---- fill.hip ----
+int gl_tables_init();
+int launch_pairs_dropin(double* zl, double* zt, bool alias, SegSoA e, SegSoA img,
+                        const long m, const PairScalars& sc, bool images, hipStream_t st);
+int launch_fill_zblocks(c64* W, long ldw, long n, long m, SegSoA e, SegSoA img,
+                        const PairScalars& sc, hipStream_t st);
+int launch_fill_static(c64* W, long ldw, long n, long m, const int* seg_start_node,
+                       const int* seg_end_node, hipStream_t st);
+int launch_single_pair(SegSoA e, SegSoA r, c64 gamma, double* out4, hipStream_t st);
+int launch_incidence(const double* seg_pts, long m, const double* nodes, long n,
+                     int* seg_start_node, int* seg_end_node, int* node_hits,
+                     hipStream_t st);
+
+}  // namespace tags
