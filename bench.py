@@ -42,7 +42,7 @@ def load_pkg():
     return tags_pkg.load()
 
 
-def cpu_baseline(T, e, nodes, N, threads, pair_sample=20000, lu_n=3000, seed=7):
+def cpu_baseline(T, e, nodes, N, threads, pair_sample=2000000, lu_n=4000, seed=7):
     """The CPU restatement (oracle: adaptive Genz-Malik cubature at rel tol 1e-6
     plus LAPACK zsysv from scipy's OpenBLAS) on a bounded sample: `pair_sample`
     random segment pairs (direct + image integrals) scaled to all m(m+1)/2
@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--plan", type=int, default=1,
+                    help="0 dense, 1 structured (default), 2 structured with Z_L-block pivoting")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; every frequency sample is "
@@ -102,7 +105,11 @@ def main():
     s = 2j * np.pi * freqs[idx]
     kap = SIGMA + s * EPSR * T.EPS0
     sw = T.Sweep(e, nodes)
+    sw.set_plan(args.plan)
     L = T.lib()
+    tuned = {}
+    if not args.no_autotune:
+        tuned = {"fp32": T.autotune(T.PREC_MIXED), "fp64": T.autotune(T.PREC_FP64)}
 
     def run(prec, profile):
         if W:
@@ -134,7 +141,9 @@ def main():
         "config": {"workload": "60x60 m grid, 2 m mesh, 0.5 m segments (m=7440, n=6541, "
                                "N=21421), rho=500, eps_r=20, 256 freqs 100 Hz-5 MHz, unit "
                                "current at corner node",
-                   "samples_timed": [float(f) for f in freqs[idx]]},
+                   "samples_timed": [float(f) for f in freqs[idx]],
+                   "plan": ["dense", "structured", "block_pivot"][args.plan],
+                   "gemm_variants": tuned},
         "mixed": {"refine_iters": [q["refine_iters"] for q in st_mx],
                   "precision_used": [q["precision_used"] for q in st_mx],
                   "backward_err_max": max(q["backward_err"] for q in st_mx),

@@ -127,6 +127,7 @@ typedef struct {
     int refine_iters;     /* refinement steps (mixed) */
     int info;             /* 0, or first zero pivot (1-based) */
     double backward_err;  /* ||b - W x||_inf / (||W||_inf ||x||_inf) in FP64 */
+    double refine_hist[8];/* backward error after each refinement step (mixed) */
 } tags_item_stats;
 
 typedef struct {
@@ -165,6 +166,36 @@ int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
                      int precision, TAGS_COMPLEX *u_inj, TAGS_COMPLEX *x_out,
                      tags_item_stats *stats);
 
+/* Factorisation plan of the sweep (default TAGS_PLAN_STRUCTURED). All plans
+ * are partial pivoting; the structured ones skip rows and columns that are
+ * exactly zero for the columns being factored.
+ * TAGS_PLAN_DENSE       unknowns in the reference order [U; I_L; I_T], all N
+ *                       rows and columns;
+ * TAGS_PLAN_STRUCTURED  reference order; the node rows are zero in the node
+ *                       columns (Y_E = 0), so the first n columns pivot and
+ *                       update over the 2m impedance rows only. Same pivots as
+ *                       DENSE;
+ * TAGS_PLAN_ZL_FIRST    unknowns stored as [I_L; I_T; U]; the Z_T rows are
+ *                       zero in the Z_L columns and are skipped there (fewer
+ *                       flops; its complex64 factor is less accurate at the
+ *                       lowest frequencies);
+ * TAGS_PLAN_BLOCK_PIVOT as ZL_FIRST but the Z_L columns pivot among Z_L rows
+ *                       only, which also skips the Z_T columns (fewest flops,
+ *                       weakest pivoting). */
+enum tags_plan {
+    TAGS_PLAN_DENSE = 0,
+    TAGS_PLAN_STRUCTURED = 1,
+    TAGS_PLAN_ZL_FIRST = 2,
+    TAGS_PLAN_BLOCK_PIVOT = 3
+};
+int tags_sweep_set_plan(tags_sweep *sw, int plan);
+
+/* Assembles W for one frequency sample on the GPU exactly as tags_sweep_solve
+ * does (pair integrals, images, incidence) and copies it to `wg` (N x N,
+ * column-major, reference unknown order [U; I_L; I_T], both triangles). */
+int tags_sweep_assemble(tags_sweep *sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, double mur,
+                        TAGS_COMPLEX *wg);
+
 /* Accumulates HIP-event timings of every kernel family on the sweep's stream. */
 int tags_sweep_set_profiling(tags_sweep *sw, int on);
 int tags_sweep_get_timers(tags_sweep *sw, tags_timers *t);
@@ -177,11 +208,28 @@ void tags_sweep_destroy(tags_sweep *sw);
 int tags_dense_solve(const TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t n, int precision,
                      tags_item_stats *stats);
 
+/* As tags_dense_solve for an immittance matrix stored row-major in the
+ * storage order of the given tags_plan: [U; I_L; I_T] for DENSE and
+ * STRUCTURED, [I_L; I_T; U] for ZL_FIRST and BLOCK_PIVOT. */
+int tags_immittance_solve(const TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t num_electrodes,
+                          size_t num_nodes, int precision, int plan, tags_item_stats *stats);
+
 /* Library version string. */
 const char *tags_version(void);
 
 /* Waits for all work on the current device (hipDeviceSynchronize). */
 int tags_device_synchronize(void);
+
+/* GEMM tuning. The trailing update C -= A B of the LU has several MFMA tile
+ * variants per precision (precision: tags_precision). tags_gemm_bench times
+ * one variant on an M x N x K problem (mean ms per launch); tags_autotune
+ * times every variant on a large (8192 x 8192 x 128) and a small
+ * (1024 x 1024 x 128) problem and installs the fastest of each for this
+ * process; tags_gemm_select installs variants explicitly. */
+int tags_gemm_bench(int M, int N, int K, int precision, int variant, int iters, double *ms);
+int tags_gemm_variants(int precision);
+int tags_gemm_select(int precision, int big_variant, int small_variant);
+int tags_autotune(int precision, int *big_variant, int *small_variant);
 
 #ifdef __cplusplus
 }

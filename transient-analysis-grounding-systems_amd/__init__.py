@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libtags_hip.so")
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["fill.hip", "lu.hip", "api.hip"]
+SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "api.hip"]
 
 ELECTRODE_DTYPE = np.dtype(
     [("start_point", "<f8", 3), ("end_point", "<f8", 3), ("middle_point", "<f8", 3),
@@ -33,6 +33,7 @@ assert ELECTRODE_DTYPE.itemsize == 88
 
 INTG_NONE, INTG_DOUBLE, INTG_SINGLE, INTG_MHEM = 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
+PLAN_DENSE, PLAN_STRUCTURED, PLAN_ZL_FIRST, PLAN_BLOCK_PIVOT = 0, 1, 2, 3
 
 # src/auxiliary.c:17-21
 TWO_PI = 6.283185307179586476925286766559
@@ -83,7 +84,8 @@ class Grid(ctypes.Structure):
 
 class ItemStats(ctypes.Structure):
     _fields_ = [("precision_used", ctypes.c_int), ("refine_iters", ctypes.c_int),
-                ("info", ctypes.c_int), ("backward_err", ctypes.c_double)]
+                ("info", ctypes.c_int), ("backward_err", ctypes.c_double),
+                ("refine_hist", ctypes.c_double * 8)]
 
 
 class Timers(ctypes.Structure):
@@ -102,7 +104,9 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "impedances_images", "fill_incidence_imm", "fill_impedance_imm",
             "solve_immittance", "tags_sweep_create", "tags_sweep_solve",
             "tags_sweep_set_profiling", "tags_sweep_get_timers", "tags_sweep_destroy",
-            "tags_dense_solve", "tags_version", "tags_device_synchronize"]
+            "tags_dense_solve", "tags_version", "tags_device_synchronize", "tags_gemm_bench",
+            "tags_gemm_variants", "tags_gemm_select", "tags_autotune", "tags_sweep_set_plan",
+            "tags_sweep_assemble", "tags_immittance_solve"]
 
 _lib = None
 
@@ -139,6 +143,13 @@ def lib():
         "tags_dense_solve": ([P, P, S, I, P], I),
         "tags_version": ([], ctypes.c_char_p),
         "tags_device_synchronize": ([], I),
+        "tags_gemm_bench": ([I, I, I, I, I, I, ctypes.POINTER(D)], I),
+        "tags_gemm_variants": ([I], I),
+        "tags_gemm_select": ([I, I, I], I),
+        "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
+        "tags_sweep_set_plan": ([P, I], I),
+        "tags_sweep_assemble": ([P, C, C, D, P], I),
+        "tags_immittance_solve": ([P, P, S, S, I, I, P], I),
     }
     for name, (args, res) in sig.items():
         f = getattr(L, name)
@@ -259,6 +270,34 @@ def dense_solve(a, b, precision=PREC_FP64):
     return rc, x, st
 
 
+def gemm_bench(M, N, K, precision, variant, iters=5):
+    """Mean time (ms) of one C -= A B launch of the given MFMA tile variant."""
+    ms = ctypes.c_double()
+    rc = lib().tags_gemm_bench(M, N, K, precision, variant, iters, ctypes.byref(ms))
+    if rc:
+        raise RuntimeError("tags_gemm_bench failed: %d" % rc)
+    return ms.value
+
+
+def autotune(precision):
+    """Times every GEMM variant and installs the fastest (big, small) pair."""
+    b, s = ctypes.c_int(), ctypes.c_int()
+    rc = lib().tags_autotune(precision, ctypes.byref(b), ctypes.byref(s))
+    if rc:
+        raise RuntimeError("tags_autotune failed: %d" % rc)
+    return b.value, s.value
+
+
+def immittance_solve(a, b, m, n, precision=PREC_FP64, plan=PLAN_STRUCTURED):
+    """Solves the immittance system stored row-major in the plan's order:
+    [U; I_L; I_T] for PLAN_DENSE / PLAN_STRUCTURED, [I_L; I_T; U] otherwise."""
+    a = np.ascontiguousarray(a, np.complex128)
+    x = np.array(b, dtype=np.complex128, copy=True)
+    st = ItemStats()
+    rc = lib().tags_immittance_solve(_p(a), _p(x), m, n, precision, plan, ctypes.byref(st))
+    return rc, x, st
+
+
 # ---------------------------------------------------------------------------
 # per-frequency scalars of the reference caller (examples/harmonic_impedance.c:116-123)
 # ---------------------------------------------------------------------------
@@ -304,8 +343,22 @@ class Sweep:
         if rc != 0:
             raise RuntimeError("tags_sweep_solve failed: %d" % rc)
         st = [dict(precision_used=q.precision_used, refine_iters=q.refine_iters, info=q.info,
-                   backward_err=q.backward_err) for q in stats]
+                   backward_err=q.backward_err,
+                   refine_hist=[v for v in q.refine_hist if v >= 0]) for q in stats]
         return (u, st, x) if want_x else (u, st)
+
+    def assemble(self, s, kappa, mur=1.0):
+        """W of one sample as assembled on the GPU (column-major, reference order)."""
+        w = np.zeros(self.N * self.N, np.complex128)
+        rc = lib().tags_sweep_assemble(self._h, CDouble.of(s), CDouble.of(kappa), float(mur), _p(w))
+        if rc:
+            raise RuntimeError("tags_sweep_assemble failed: %d" % rc)
+        return w
+
+    def set_plan(self, plan):
+        rc = lib().tags_sweep_set_plan(self._h, plan)
+        if rc:
+            raise RuntimeError("tags_sweep_set_plan failed: %d" % rc)
 
     def set_profiling(self, on=True):
         lib().tags_sweep_set_profiling(self._h, 1 if on else 0)

@@ -174,10 +174,13 @@ struct Solver {
         return 0;
     }
 
+    const LUPlan* plan = nullptr;  // nullptr: dense partial pivoting
+    bool equilibrate = false;      // symmetric row-max scaling before the FP32 factor
+
     // Plain form: factor W (destroyed) in FP64 and solve; x returned in `b`.
     int solve64(c64* W, c64* bvec, tags_item_stats* stt)
     {
-        int rc = lu_factor<double>(w64, W, n, st, pr);
+        int rc = lu_factor<double>(w64, W, n, st, pr, plan);
         if (rc) return rc;
         int info = 0;
         if ((rc = read_info(w64, &info))) return rc;
@@ -199,13 +202,15 @@ struct Solver {
     int solve_mixed(c64* W, c64* bvec, tags_item_stats* stt)
     {
         int rc;
+        if (stt)
+            for (int i = 0; i < 8; i++) stt->refine_hist[i] = -1.0;
         {
             Timed tm(pr, FAM_OTHER, st);
-            if ((rc = row_scale(W, n, n, dscale, st))) return rc;
+            if ((rc = row_scale(W, n, n, dscale, equilibrate, st))) return rc;
             if ((rc = to_c32_scaled(W, n, n, dscale, W32, n, st))) return rc;
             tm.end();
         }
-        if ((rc = lu_factor<float>(w32, W32, n, st, pr))) return rc;
+        if ((rc = lu_factor<float>(w32, W32, n, st, pr, plan))) return rc;
         int f[2];
         TAGS_HIP(hipMemcpyAsync(f, w32.flags, sizeof(f), hipMemcpyDeviceToHost, st));
         TAGS_HIP(hipStreamSynchronize(st));
@@ -235,6 +240,7 @@ struct Solver {
                 std::memcpy(&rnorm, &hnorms[0], 8);
                 std::memcpy(&xnorm, &hnorms[1], 8);
                 berr = (wnorm > 0 && xnorm > 0) ? rnorm / (wnorm * xnorm) : 0.0;
+                if (stt && it < 8) stt->refine_hist[it] = berr;
                 if (!(berr == berr)) break;  // NaN
                 if (berr <= 1e-15) {
                     converged = true;
@@ -278,6 +284,119 @@ int tags_device_synchronize(void)
 {
     TAGS_HIP(hipDeviceSynchronize());
     return 0;
+}
+
+}  // extern "C"
+
+namespace tags {
+__global__ void k_fill_pattern(double* p, long n, unsigned seed)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) {
+        unsigned h = (unsigned)i * 2654435761u ^ seed;
+        h ^= h >> 13;
+        h *= 0x5bd1e995u;
+        h ^= h >> 15;
+        p[i] = (double)(h & 0xffff) / 65536.0 - 0.5;
+    }
+}
+__global__ void k_to_f32(const double* a, float* b, long n)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+// Times C -= A B (M x N x K, row-major, a random row map on A and C like the
+// LU trailing update) for one variant; returns the mean kernel time in ms.
+template <typename T>
+static int gemm_bench_t(int M, int N, int K, int variant, int iters, double* ms)
+{
+    Stream S;
+    const long na = (long)M * K, nb = (long)K * N, nc = (long)M * N;
+    DevBuf<double> tmp;
+    DevBuf<cx<T>> A, B, C;
+    DevBuf<int> rows;
+    int rc;
+    if ((rc = tmp.alloc(2 * std::max(na, std::max(nb, nc)))) || (rc = A.alloc(na)) ||
+        (rc = B.alloc(nb)) || (rc = C.alloc(nc)) || (rc = rows.alloc(M)))
+        return rc;
+    cx<T>* bufs[3] = {A.p, B.p, C.p};
+    const long sz[3] = {na, nb, nc};
+    for (int b = 0; b < 3; b++) {
+        hipLaunchKernelGGL(k_fill_pattern, dim3((unsigned)((2 * sz[b] + 255) / 256)), dim3(256), 0,
+                           S.s, tmp.p, 2 * sz[b], 17u + b);
+        if (sizeof(T) == sizeof(double)) {
+            TAGS_HIP(hipMemcpyAsync(bufs[b], tmp.p, sizeof(double) * 2 * sz[b],
+                                    hipMemcpyDeviceToDevice, S.s));
+        } else {
+            hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((2 * sz[b] + 255) / 256)), dim3(256), 0,
+                               S.s, tmp.p, (float*)bufs[b], 2 * sz[b]);
+        }
+    }
+    std::vector<int> h(M);
+    for (int i = 0; i < M; i++) h[i] = (int)(((long)i * 7919) % M);
+    bool perm_ok = (M % 7919) != 0;
+    if (!perm_ok)
+        for (int i = 0; i < M; i++) h[i] = i;
+    TAGS_HIP(hipMemcpyAsync(rows.p, h.data(), sizeof(int) * M, hipMemcpyHostToDevice, S.s));
+    GemmArgs<T> g{M, N, K, A.p, K, rows.p, B.p, N, nullptr, C.p, N, rows.p, 1};
+    if ((rc = launch_gemm_variant<T>(g, variant, S.s))) return rc;  // warm-up
+    hipEvent_t e0, e1;
+    TAGS_HIP(hipEventCreate(&e0));
+    TAGS_HIP(hipEventCreate(&e1));
+    TAGS_HIP(hipEventRecord(e0, S.s));
+    for (int i = 0; i < iters; i++)
+        if ((rc = launch_gemm_variant<T>(g, variant, S.s))) return rc;
+    TAGS_HIP(hipEventRecord(e1, S.s));
+    TAGS_HIP(hipEventSynchronize(e1));
+    float t = 0;
+    TAGS_HIP(hipEventElapsedTime(&t, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    *ms = t / iters;
+    return 0;
+}
+}  // namespace tags
+
+extern "C" {
+
+int tags_gemm_bench(int M, int N, int K, int precision, int variant, int iters, double* ms)
+{
+    if (M <= 0 || N <= 0 || K <= 0 || iters <= 0 || !ms) return TAGS_ERR_ARG;
+    return precision == TAGS_PREC_MIXED ? gemm_bench_t<float>(M, N, K, variant, iters, ms)
+                                        : gemm_bench_t<double>(M, N, K, variant, iters, ms);
+}
+
+int tags_gemm_variants(int precision) { return gemm_variant_count(precision); }
+
+int tags_gemm_select(int precision, int big_variant, int small_variant)
+{
+    return gemm_set_variants(precision, big_variant, small_variant);
+}
+
+int tags_autotune(int precision, int* big_out, int* small_out)
+{
+    // large: the first trailing updates of the 21,421 system; small: the tail
+    const int nv = gemm_variant_count(precision);
+    int best[2] = {0, 0};
+    const int sizes[2][3] = {{8192, 8192, 128}, {1024, 1024, 128}};
+    for (int s = 0; s < 2; s++) {
+        double bt = 1e30;
+        for (int v = 0; v < nv; v++) {
+            double ms = 0;
+            int rc = tags_gemm_bench(sizes[s][0], sizes[s][1], sizes[s][2], precision, v,
+                                     s == 0 ? 3 : 20, &ms);
+            if (rc) return rc;
+            if (ms < bt) {
+                bt = ms;
+                best[s] = v;
+            }
+        }
+    }
+    int rc = gemm_set_variants(precision, best[0], best[1]);
+    if (big_out) *big_out = best[0];
+    if (small_out) *small_out = best[1];
+    return rc;
 }
 
 // ---------------------------------------------------------------------------
@@ -696,7 +815,7 @@ int fill_impedance_imm(TAGS_COMPLEX* wg, const TAGS_COMPLEX* zl, const TAGS_COMP
 // solve_immittance (src/linalg.c:113-139)
 // ---------------------------------------------------------------------------
 static int dense_solve_host(const c64* a, c64* bx, long n, int precision, bool sym_lower_cm,
-                            tags_item_stats* stt)
+                            tags_item_stats* stt, const LUPlan* plan = nullptr)
 {
     if (n <= 0) return 0;
     if (n >= 65536) return TAGS_ERR_TOO_LARGE;
@@ -711,6 +830,7 @@ static int dense_solve_host(const c64* a, c64* bx, long n, int precision, bool s
     TAGS_HIP(hipMemcpyAsync(W.p, a, sizeof(c64) * n * n, hipMemcpyHostToDevice, S.s));
     TAGS_HIP(hipMemcpyAsync(bv.p, bx, sizeof(c64) * n, hipMemcpyHostToDevice, S.s));
     if (sym_lower_cm && (rc = sym_from_upper(W.p, n, (int)n, S.s))) return rc;
+    sv.plan = plan;
     rc = mixed ? sv.solve_mixed(W.p, bv.p, stt) : sv.solve64(W.p, bv.p, stt);
     if (rc) return rc;
     TAGS_HIP(hipMemcpyAsync(bx, bv.p, sizeof(c64) * n, hipMemcpyDeviceToHost, S.s));
@@ -736,6 +856,21 @@ int tags_dense_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t n, int preci
     return dense_solve_host((const c64*)a, (c64*)b, (long)n, precision, false, stats);
 }
 
+int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_electrodes,
+                          size_t num_nodes, int precision, int plan, tags_item_stats* stats)
+{
+    const long m = (long)num_electrodes, nn = (long)num_nodes;
+    LUPlan p;
+    switch (plan) {
+    case TAGS_PLAN_DENSE: p = LUPlan::dense((int)(nn + 2 * m)); break;
+    case TAGS_PLAN_STRUCTURED: p = LUPlan::nodes_first((int)m, (int)nn); break;
+    case TAGS_PLAN_ZL_FIRST: p = LUPlan::zl_first((int)m, (int)nn, false); break;
+    case TAGS_PLAN_BLOCK_PIVOT: p = LUPlan::zl_first((int)m, (int)nn, true); break;
+    default: return TAGS_ERR_ARG;
+    }
+    return dense_solve_host((const c64*)a, (c64*)b, nn + 2 * m, precision, false, stats, &p);
+}
+
 }  // extern "C"
 
 // ---------------------------------------------------------------------------
@@ -743,6 +878,9 @@ int tags_dense_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t n, int preci
 // ---------------------------------------------------------------------------
 struct tags_sweep {
     long m = 0, n = 0, N = 0;
+    int plan_kind = TAGS_PLAN_STRUCTURED;
+    tags::LUPlan plan;
+    tags::BlockLayout layout{};
     hipStream_t st = nullptr;
     tags::DevSegs e, img;
     int* sn = nullptr;
@@ -752,6 +890,64 @@ struct tags_sweep {
     tags::Solver sv;
     tags::Profiler prof;
 };
+
+
+static int sweep_layout(tags_sweep* sw)
+{
+    const long n = sw->n, m = sw->m;
+    switch (sw->plan_kind) {
+    case TAGS_PLAN_DENSE:
+        sw->layout = BlockLayout{n, m, 0, n, n + m};
+        sw->sv.plan = nullptr;
+        break;
+    case TAGS_PLAN_STRUCTURED:
+        sw->layout = BlockLayout{n, m, 0, n, n + m};
+        sw->plan = LUPlan::nodes_first((int)m, (int)n);
+        sw->sv.plan = &sw->plan;
+        break;
+    default:
+        sw->layout = BlockLayout{n, m, 2 * m, 0, m};
+        sw->plan = LUPlan::zl_first((int)m, (int)n, sw->plan_kind == TAGS_PLAN_BLOCK_PIVOT);
+        sw->sv.plan = &sw->plan;
+        break;
+    }
+    return 0;
+}
+
+// Per-frequency scalars (reference caller formulas) and the assembly of W in
+// the sweep's block layout.
+static int sweep_fill(tags_sweep* sw, TAGS_COMPLEX s_, TAGS_COMPLEX kappa_, double mur,
+                      const TAGS_COMPLEX* gamma_, const TAGS_COMPLEX* ref_l_,
+                      const TAGS_COMPLEX* ref_t_)
+{
+    const long N = sw->N;
+    const c64 sf = from_c(s_), kf = from_c(kappa_);
+    c64 gf, rl, rt;
+    if (gamma_) {
+        gf = from_c(*gamma_);
+    } else {
+        // gamma = csqrt(s mu0 mur kappa), principal branch (examples/harmonic_impedance.c:118)
+        const c64 z = cmul(c64{sf.x * kMu0 * mur, sf.y * kMu0 * mur}, kf);
+        const std::complex<double> g = std::sqrt(std::complex<double>(z.x, z.y));
+        gf = c64{g.real(), g.imag()};
+    }
+    rl = ref_l_ ? from_c(*ref_l_) : c64{1.0, 0.0};
+    if (ref_t_) {
+        rt = from_c(*ref_t_);
+    } else {
+        // (kappa - s eps0)/(kappa + s eps0), examples/harmonic_impedance.c:122
+        const c64 se = c64{sf.x * kEps0, sf.y * kEps0};
+        rt = cdiv(csub(kf, se), cadd(kf, se));
+    }
+    PairScalars sc = make_scalars(gf, sf, mur, kf, rl, rt, INTG_DOUBLE);
+    Timed tm(&sw->prof, FAM_FILL, sw->st);
+    int rc;
+    if ((rc = launch_fill_static(sw->W, N, sw->layout, sw->sn, sw->en, sw->st))) return rc;
+    if ((rc = launch_fill_zblocks(sw->W, N, sw->layout, sw->e.soa(), sw->img.soa(), sc, sw->st)))
+        return rc;
+    tm.end();
+    return 0;
+}
 
 extern "C" {
 
@@ -820,6 +1016,37 @@ int tags_sweep_create(tags_sweep** out, const Electrode* electrodes, const Elect
     return 0;
 }
 
+int tags_sweep_assemble(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, double mur,
+                        TAGS_COMPLEX* wg)
+{
+    if (!sw || !wg) return TAGS_ERR_ARG;
+    const long N = sw->N;
+    if (!sw->W) TAGS_HIP(hipMalloc(&sw->W, sizeof(c64) * N * N));
+    sweep_layout(sw);
+    int rc = sweep_fill(sw, s, kappa, mur, nullptr, nullptr, nullptr);
+    if (rc) return rc;
+    std::vector<c64> h((size_t)N * N);
+    TAGS_HIP(hipMemcpyAsync(h.data(), sw->W, sizeof(c64) * N * N, hipMemcpyDeviceToHost, sw->st));
+    TAGS_HIP(hipStreamSynchronize(sw->st));
+    // unknown index in the reference order [U; I_L; I_T] -> storage index
+    const BlockLayout& L = sw->layout;
+    std::vector<long> pos(N);
+    for (long i = 0; i < sw->n; i++) pos[i] = L.node_off + i;
+    for (long i = 0; i < sw->m; i++) pos[sw->n + i] = L.zl_off + i;
+    for (long i = 0; i < sw->m; i++) pos[sw->n + sw->m + i] = L.zt_off + i;
+    c64* out = (c64*)wg;
+    for (long c = 0; c < N; c++)
+        for (long r = 0; r < N; r++) out[r + N * c] = h[(size_t)pos[r] * N + pos[c]];
+    return 0;
+}
+
+int tags_sweep_set_plan(tags_sweep* sw, int plan)
+{
+    if (!sw || plan < 0 || plan > 3) return TAGS_ERR_ARG;
+    sw->plan_kind = plan;
+    return 0;
+}
+
 int tags_sweep_set_profiling(tags_sweep* sw, int on)
 {
     if (!sw) return TAGS_ERR_ARG;
@@ -873,37 +1100,15 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     if ((rc = sw->sv.ensure((int)N, mixed))) return rc;
     if (!sw->W) TAGS_HIP(hipMalloc(&sw->W, sizeof(c64) * N * N));
     if (!sw->rhs) TAGS_HIP(hipMalloc(&sw->rhs, sizeof(c64) * N));
+    sweep_layout(sw);
+    const BlockLayout& L = sw->layout;
     for (size_t f = 0; f < nfreq; f++) {
-        const c64 sf = from_c(s[f]), kf = from_c(kappa[f]);
-        c64 gf, rl, rt;
-        if (gamma) {
-            gf = from_c(gamma[f]);
-        } else {
-            // gamma = csqrt(s mu0 mur kappa), principal branch
-            // (examples/harmonic_impedance.c:118)
-            const c64 z = cmul(c64{sf.x * kMu0 * mur, sf.y * kMu0 * mur}, kf);
-            const std::complex<double> g = std::sqrt(std::complex<double>(z.x, z.y));
-            gf = c64{g.real(), g.imag()};
-        }
-        rl = ref_l ? from_c(ref_l[f]) : c64{1.0, 0.0};
-        if (ref_t) {
-            rt = from_c(ref_t[f]);
-        } else {
-            // (kappa - s eps0)/(kappa + s eps0), examples/harmonic_impedance.c:122
-            const c64 se = c64{sf.x * kEps0, sf.y * kEps0};
-            rt = cdiv(csub(kf, se), cadd(kf, se));
-        }
-        PairScalars sc = make_scalars(gf, sf, mur, kf, rl, rt, INTG_DOUBLE);
-        {
-            Timed tm(&sw->prof, FAM_FILL, sw->st);
-            if ((rc = launch_fill_static(sw->W, N, n, m, sw->sn, sw->en, sw->st))) return rc;
-            if ((rc = launch_fill_zblocks(sw->W, N, n, m, sw->e.soa(), sw->img.soa(), sc, sw->st)))
-                return rc;
-            tm.end();
-        }
+        if ((rc = sweep_fill(sw, s[f], kappa[f], mur, gamma ? &gamma[f] : nullptr,
+                             ref_l ? &ref_l[f] : nullptr, ref_t ? &ref_t[f] : nullptr)))
+            return rc;
         // b = unit current at the injection node
         if ((rc = fill_c64(sw->rhs, N, c64{0.0, 0.0}, sw->st))) return rc;
-        if ((rc = fill_c64(sw->rhs + inj_node, 1, c64{1.0, 0.0}, sw->st))) return rc;
+        if ((rc = fill_c64(sw->rhs + L.node_off + inj_node, 1, c64{1.0, 0.0}, sw->st))) return rc;
         tags_item_stats stt{};
         rc = mixed ? sw->sv.solve_mixed(sw->W, sw->rhs, &stt) : sw->sv.solve64(sw->W, sw->rhs, &stt);
         if (rc < 0) return rc;
@@ -913,11 +1118,18 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
             continue;
         }
         c64 u;
-        TAGS_HIP(hipMemcpyAsync(&u, sw->rhs + inj_node, sizeof(c64), hipMemcpyDeviceToHost,
-                                sw->st));
-        if (x_out)
-            TAGS_HIP(hipMemcpyAsync(x_out + f * N, sw->rhs, sizeof(c64) * N,
+        TAGS_HIP(hipMemcpyAsync(&u, sw->rhs + L.node_off + inj_node, sizeof(c64),
+                                hipMemcpyDeviceToHost, sw->st));
+        if (x_out) {
+            // reference order [U; I_L; I_T]
+            c64* xo = (c64*)(x_out + f * N);
+            TAGS_HIP(hipMemcpyAsync(xo, sw->rhs + L.node_off, sizeof(c64) * n,
                                     hipMemcpyDeviceToHost, sw->st));
+            TAGS_HIP(hipMemcpyAsync(xo + n, sw->rhs + L.zl_off, sizeof(c64) * m,
+                                    hipMemcpyDeviceToHost, sw->st));
+            TAGS_HIP(hipMemcpyAsync(xo + n + m, sw->rhs + L.zt_off, sizeof(c64) * m,
+                                    hipMemcpyDeviceToHost, sw->st));
+        }
         TAGS_HIP(hipStreamSynchronize(sw->st));
         if (u_inj) u_inj[f] = to_c(u);
     }

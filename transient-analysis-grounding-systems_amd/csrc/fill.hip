@@ -417,8 +417,8 @@ int launch_pairs_dropin(double* zl, double* zt, bool alias, SegSoA e, SegSoA img
 // ---------------------------------------------------------------------------
 constexpr int kFT = 32;
 
-__global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long n, long m,
-                                                       SegSoA e, SegSoA img, PairScalars sc)
+__global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long zl_off, long zt_off,
+                                                       long m, SegSoA e, SegSoA img, PairScalars sc)
 {
     // triangular tile index -> (ti >= tk)
     const long b = blockIdx.x;
@@ -490,13 +490,13 @@ __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long n, 
         tt[a][c] = c64{-vt.x, -vt.y};
     }
     __syncthreads();
-    // pass 1: entries (i, k), k <= i; rows n+i (Z_L) and n+m+i (Z_T)
+    // pass 1: entries (i, k), k <= i of the Z_L and Z_T blocks
     for (int p = t; p < kFT * kFT; p += 256) {
         const int a = p / kFT, c = p % kFT;
         const long i = i0 + a, k = k0 + c;
         if (i < m && k < m && k <= i) {
-            W[(n + i) * ldw + (n + k)] = tl[a][c];
-            W[(n + m + i) * ldw + (n + m + k)] = tt[a][c];
+            W[(zl_off + i) * ldw + (zl_off + k)] = tl[a][c];
+            W[(zt_off + i) * ldw + (zt_off + k)] = tt[a][c];
         }
     }
     // pass 2: transposed entries (k, i), k < i
@@ -504,47 +504,56 @@ __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long n, 
         const int c = p / kFT, a = p % kFT;  // consecutive threads walk i
         const long i = i0 + a, k = k0 + c;
         if (i < m && k < m && k < i) {
-            W[(n + k) * ldw + (n + i)] = tl[a][c];
-            W[(n + m + k) * ldw + (n + m + i)] = tt[a][c];
+            W[(zl_off + k) * ldw + (zl_off + i)] = tl[a][c];
+            W[(zt_off + k) * ldw + (zt_off + i)] = tt[a][c];
         }
     }
 }
 
-int launch_fill_zblocks(c64* W, long ldw, long n, long m, SegSoA e, SegSoA img,
+int launch_fill_zblocks(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
                         const PairScalars& sc, hipStream_t st)
 {
+    const long m = L.m;
     if (m <= 0) return 0;
     int rc = gl_tables_init();
     if (rc) return rc;
     const long T = (m + kFT - 1) / kFT;
     const long nt = T * (T + 1) / 2;
-    hipLaunchKernelGGL(k_fill_zblocks, dim3((unsigned)nt), dim3(256), 0, st, W, ldw, n, m, e,
-                       img, sc);
+    hipLaunchKernelGGL(k_fill_zblocks, dim3((unsigned)nt), dim3(256), 0, st, W, ldw, L.zl_off,
+                       L.zt_off, m, e, img, sc);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
 // Every entry of W outside the two impedance blocks: zero Y_E block, zero
 // cross blocks, incidence A (+1 start node, -1 end node), B (0.5) and their
-// transposes (src/linalg.c:41-66, 96-108).
-__global__ __launch_bounds__(256) void k_fill_static(c64* W, long ldw, long n, long m,
+// transposes (src/linalg.c:41-66, 96-108), for any placement of the three
+// unknown blocks (offsets of I_L, I_T and U).
+__device__ inline int block_of(long i, const BlockLayout& L)
+{
+    if (i >= L.node_off && i < L.node_off + L.n) return 0;
+    if (i >= L.zl_off && i < L.zl_off + L.m) return 1;
+    return 2;
+}
+
+__global__ __launch_bounds__(256) void k_fill_static(c64* W, long ldw, BlockLayout L,
                                                       const int* __restrict__ sn,
                                                       const int* __restrict__ en)
 {
-    const long N = n + 2 * m;
+    const long N = L.n + 2 * L.m;
     for (long r = blockIdx.y; r < N; r += gridDim.y) {
-    const int rb = r < n ? 0 : (r < n + m ? 1 : 2);
+    const int rb = block_of(r, L);
+    const long rl = rb == 0 ? r - L.node_off : (rb == 1 ? r - L.zl_off : r - L.zt_off);
     for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < N; c += (long)gridDim.x * 256) {
-        const int cb = c < n ? 0 : (c < n + m ? 1 : 2);
+        const int cb = block_of(c, L);
         if (rb == cb && rb != 0) continue;  // impedance blocks
+        const long cl = cb == 0 ? c - L.node_off : (cb == 1 ? c - L.zl_off : c - L.zt_off);
         double v = 0.0;
         if (rb == 0 && cb != 0) {
-            const long s = cb == 1 ? c - n : c - n - m;
-            const bool st = sn[s] == r, ed = en[s] == r;
+            const bool st = sn[cl] == rl, ed = en[cl] == rl;
             v = cb == 1 ? (st ? 1.0 : (ed ? -1.0 : 0.0)) : ((st || ed) ? 0.5 : 0.0);
         } else if (cb == 0 && rb != 0) {
-            const long s = rb == 1 ? r - n : r - n - m;
-            const bool st = sn[s] == c, ed = en[s] == c;
+            const bool st = sn[rl] == cl, ed = en[rl] == cl;
             v = rb == 1 ? (st ? 1.0 : (ed ? -1.0 : 0.0)) : ((st || ed) ? 0.5 : 0.0);
         }
         W[r * ldw + c] = c64{v, 0.0};
@@ -552,16 +561,16 @@ __global__ __launch_bounds__(256) void k_fill_static(c64* W, long ldw, long n, l
     }
 }
 
-int launch_fill_static(c64* W, long ldw, long n, long m, const int* seg_start_node,
+int launch_fill_static(c64* W, long ldw, const BlockLayout& L, const int* seg_start_node,
                        const int* seg_end_node, hipStream_t st)
 {
-    const long N = n + 2 * m;
+    const long N = L.n + 2 * L.m;
     if (N <= 0) return 0;
     unsigned gx = (unsigned)((N + 255) / 256);
     if (gx > 16) gx = 16;
     const unsigned gy = (unsigned)(N < 65535 ? N : 65535);
-    hipLaunchKernelGGL(k_fill_static, dim3(gx, gy), dim3(256), 0, st, W, ldw, n, m,
-                       seg_start_node, seg_end_node);
+    hipLaunchKernelGGL(k_fill_static, dim3(gx, gy), dim3(256), 0, st, W, ldw, L, seg_start_node,
+                       seg_end_node);
     TAGS_HIP(hipGetLastError());
     return 0;
 }

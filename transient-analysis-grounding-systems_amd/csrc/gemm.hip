@@ -1,0 +1,307 @@
+// Complex GEMM on the MFMA cores of gfx950: C = A B or C -= A B, row-major
+// operands read and written through optional row maps (the LU's logical-to-
+// physical row permutation). This is the trailing update of the blocked LU
+// (A22 -= L21 U12) and the TRSM-as-GEMM of the panel rows.
+//
+// Kernel shape: 256 threads = 4 waves in a 2 x 2 arrangement; work-group tile
+// BM x BN; K staged through double-buffered LDS in chunks of BK, stored
+// k-major (one row of the tile per k) with a 128-byte row skew so that the
+// four 16-lane groups of an MFMA operand read hit disjoint banks; the next
+// chunk is fetched from HBM into registers while the MFMAs of the current one
+// run. Complex products use four real MFMAs (re.re, im.im, re.im, im.re) or,
+// with M3, three (Gauss: re.re, im.im, (re+im).(re+im)).
+#include <algorithm>
+
+#include "lu.h"
+
+namespace tags {
+
+template <typename T>
+struct Mfma;
+// 16x16x4 blocks; lane l supplies A[l%16][l/16] and B[l/16][l%16]. Result
+// element v of lane l is D[row(l, v)][l%16] with row = l/16 + 4v for the f64
+// instruction and 4(l/16) + v for the f32 one (measured on gfx950).
+template <>
+struct Mfma<double> {
+    typedef double v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 run(double a, double b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
+};
+template <>
+struct Mfma<float> {
+    typedef float v4 __attribute__((ext_vector_type(4)));
+    static __device__ inline v4 run(float a, float b, v4 c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
+};
+
+template <typename T, int BM, int BN, int BK, bool M3>
+__global__ __launch_bounds__(256) void k_cgemm(GemmArgs<T> g)
+{
+    using V4 = typename Mfma<T>::v4;
+    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    constexpr int SKEW = 128 / (int)sizeof(cx<T>);
+    constexpr int SA = BM + SKEW, SB = BN + SKEW;
+    constexpr int LA = BM * BK / 256, LB = BK * BN / 256;
+    static_assert(LA * 256 == BM * BK && LB * 256 == BK * BN, "tile/thread mismatch");
+    __shared__ cx<T> As[2][BK * SA];
+    __shared__ cx<T> Bs[2][BK * SB];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int tiles_n = (g.N + BN - 1) / BN;
+    const int tiles_m = (g.M + BM - 1) / BM;
+    const int ntiles = tiles_n * tiles_m;
+    // XCD-aware order: dispatch id b runs on XCD b % 8; give each XCD a
+    // contiguous range of tiles so neighbouring tiles share its L2.
+    int bid;
+    {
+        const int per = (ntiles + 7) / 8;
+        bid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    }
+    if (bid >= ntiles) return;
+    const int tm = bid / tiles_n, tn = bid % tiles_n;
+    const int m0 = tm * BM, n0 = tn * BN;
+
+    // global-load slots of this thread (fixed rows of A, fixed columns of B)
+    long aoff[LA];
+    int acol[LA], arw[LA];
+#pragma unroll
+    for (int q = 0; q < LA; q++) {
+        const int e = tid + 256 * q;
+        const int r = e / BK, c = e % BK, gr = m0 + r;
+        arw[q] = r;
+        acol[q] = c;
+        aoff[q] = gr < g.M ? (long)(g.arow ? g.arow[gr] : gr) * g.lda : -1;
+    }
+    // B slots: row r of the chunk, column c (fixed for the whole K loop)
+    int bcol[LB], brw[LB];
+#pragma unroll
+    for (int q = 0; q < LB; q++) {
+        const int e = tid + 256 * q;
+        brw[q] = e / BN;
+        bcol[q] = (n0 + e % BN) < g.N ? e % BN : -1;
+    }
+    cx<T> ra[LA], rb[LB];
+#define TAGS_GLOAD(k0_)                                                                   \
+    {                                                                                     \
+        _Pragma("unroll") for (int q = 0; q < LA; q++)                                    \
+        {                                                                                 \
+            const int gc = (k0_) + acol[q];                                               \
+            const bool ok = aoff[q] >= 0 && gc < g.K;                                     \
+            const cx<T> v = g.A[ok ? aoff[q] + gc : 0];                                   \
+            ra[q] = ok ? v : cx<T>{0, 0};                                                 \
+        }                                                                                 \
+        _Pragma("unroll") for (int q = 0; q < LB; q++)                                    \
+        {                                                                                 \
+            const int gk = (k0_) + brw[q];                                                \
+            const bool ok = gk < g.K && bcol[q] >= 0;                                     \
+            const long pr = ok ? (long)(g.brow ? g.brow[gk] : gk) * g.ldb + n0 + bcol[q] : 0; \
+            const cx<T> v = g.B[pr];                                                      \
+            rb[q] = ok ? v : cx<T>{0, 0};                                                 \
+        }                                                                                 \
+    }
+#define TAGS_SSTORE(buf_)                                                                 \
+    {                                                                                     \
+        _Pragma("unroll") for (int q = 0; q < LA; q++) As[buf_][acol[q] * SA + arw[q]] = ra[q]; \
+        _Pragma("unroll") for (int q = 0; q < LB; q++)                                    \
+        {                                                                                 \
+            const int e = tid + 256 * q;                                                  \
+            Bs[buf_][brw[q] * SB + (e % BN)] = rb[q];                                     \
+        }                                                                                 \
+    }
+
+    V4 acc0[TM][TN], acc1[TM][TN], acc2[M3 ? TM : 1][M3 ? TN : 1];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            acc0[i][j] = V4{0, 0, 0, 0};
+            acc1[i][j] = V4{0, 0, 0, 0};
+            if constexpr (M3) acc2[i][j] = V4{0, 0, 0, 0};
+        }
+
+    TAGS_GLOAD(0);
+    TAGS_SSTORE(0);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        const bool more = k0 + BK < g.K;
+        if (more) TAGS_GLOAD(k0 + BK);
+        const cx<T>* as = As[buf];
+        const cx<T>* bs = Bs[buf];
+#pragma unroll
+        for (int kk = 0; kk < BK; kk += 4) {
+            cx<T> a[TM], b[TN];
+            const int kr = kk + (lane >> 4);
+#pragma unroll
+            for (int i = 0; i < TM; i++) a[i] = as[kr * SA + wm * WM + i * 16 + (lane & 15)];
+#pragma unroll
+            for (int j = 0; j < TN; j++) b[j] = bs[kr * SB + wn * WN + j * 16 + (lane & 15)];
+            if constexpr (M3) {
+                T asum[TM], bsum[TN];
+#pragma unroll
+                for (int i = 0; i < TM; i++) asum[i] = a[i].x + a[i].y;
+#pragma unroll
+                for (int j = 0; j < TN; j++) bsum[j] = b[j].x + b[j].y;
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++) {
+                        acc0[i][j] = Mfma<T>::run(a[i].x, b[j].x, acc0[i][j]);
+                        acc1[i][j] = Mfma<T>::run(a[i].y, b[j].y, acc1[i][j]);
+                        acc2[i][j] = Mfma<T>::run(asum[i], bsum[j], acc2[i][j]);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++) {
+                        acc0[i][j] = Mfma<T>::run(a[i].x, b[j].x, acc0[i][j]);
+                        acc0[i][j] = Mfma<T>::run(-a[i].y, b[j].y, acc0[i][j]);
+                        acc1[i][j] = Mfma<T>::run(a[i].x, b[j].y, acc1[i][j]);
+                        acc1[i][j] = Mfma<T>::run(a[i].y, b[j].x, acc1[i][j]);
+                    }
+            }
+        }
+        if (more) TAGS_SSTORE(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+#undef TAGS_GLOAD
+#undef TAGS_SSTORE
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int row = m0 + wm * WM + i * 16 + Mfma<T>::row(lane, v);
+            if (row >= g.M) continue;
+            cx<T>* crow = g.C + (long)(g.crow ? g.crow[row] : row) * g.ldc;
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                const int col = n0 + wn * WN + j * 16 + (lane & 15);
+                if (col >= g.N) continue;
+                T re, im;
+                if constexpr (M3) {
+                    re = acc0[i][j][v] - acc1[i][j][v];
+                    im = acc2[i][j][v] - acc0[i][j][v] - acc1[i][j][v];
+                } else {
+                    re = acc0[i][j][v];
+                    im = acc1[i][j][v];
+                }
+                if (g.mode == 0) {
+                    crow[col] = cx<T>{re, im};
+                } else {
+                    const cx<T> c = crow[col];
+                    crow[col] = cx<T>{c.x - re, c.y - im};
+                }
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------
+// variants and selection
+// ---------------------------------------------------------------------------
+template <typename T, int BM, int BN, int BK, bool M3>
+static int launch_v(const GemmArgs<T>& g, hipStream_t st)
+{
+    const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+    const long grid = (tiles + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3>), dim3((unsigned)grid), dim3(256), 0, st, g);
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+// Variant table: 0..3 four-multiply, 4..5 three-multiply (Gauss).
+template <typename T>
+struct GemmTable;
+template <>
+struct GemmTable<float> {
+    static constexpr int count = 6;
+    static int launch(int v, const GemmArgs<float>& g, hipStream_t st)
+    {
+        switch (v) {
+        case 0: return launch_v<float, 64, 64, 16, false>(g, st);
+        case 1: return launch_v<float, 128, 128, 16, false>(g, st);
+        case 2: return launch_v<float, 128, 64, 16, false>(g, st);
+        case 3: return launch_v<float, 64, 128, 16, false>(g, st);
+        case 4: return launch_v<float, 128, 128, 16, true>(g, st);
+        default: return launch_v<float, 64, 64, 16, true>(g, st);
+        }
+    }
+    static int tile_m(int v) { return (v == 1 || v == 2 || v == 4) ? 128 : 64; }
+    static int tile_n(int v) { return (v == 1 || v == 3 || v == 4) ? 128 : 64; }
+};
+template <>
+struct GemmTable<double> {
+    static constexpr int count = 4;
+    static int launch(int v, const GemmArgs<double>& g, hipStream_t st)
+    {
+        switch (v) {
+        case 0: return launch_v<double, 64, 64, 8, false>(g, st);
+        case 1: return launch_v<double, 128, 64, 8, false>(g, st);
+        case 2: return launch_v<double, 64, 128, 8, false>(g, st);
+        default: return launch_v<double, 128, 64, 16, false>(g, st);
+        }
+    }
+    static int tile_m(int v) { return (v == 1 || v == 3) ? 128 : 64; }
+    static int tile_n(int v) { return v == 2 ? 128 : 64; }
+};
+
+// Selected variant for large and small problems (set by the autotuner).
+static int g_big_variant[2] = {1, 4};    // [double, float]
+static int g_small_variant[2] = {0, 5};
+
+template <typename T>
+static constexpr int prec_index()
+{
+    return sizeof(T) == sizeof(float) ? 1 : 0;
+}
+
+template <typename T>
+int launch_gemm(const GemmArgs<T>& g, hipStream_t st)
+{
+    if (g.M <= 0 || g.N <= 0) return 0;
+    const int p = prec_index<T>();
+    const int big = g_big_variant[p];
+    const long tiles_big = (long)((g.M + GemmTable<T>::tile_m(big) - 1) / GemmTable<T>::tile_m(big)) *
+                           ((g.N + GemmTable<T>::tile_n(big) - 1) / GemmTable<T>::tile_n(big));
+    const int v = tiles_big >= 512 ? big : g_small_variant[p];
+    return GemmTable<T>::launch(v, g, st);
+}
+
+template <typename T>
+int launch_gemm_variant(const GemmArgs<T>& g, int variant, hipStream_t st)
+{
+    if (g.M <= 0 || g.N <= 0) return 0;
+    if (variant < 0 || variant >= GemmTable<T>::count) return TAGS_ERR_ARG;
+    return GemmTable<T>::launch(variant, g, st);
+}
+
+int gemm_set_variants(int precision, int big, int small)
+{
+    const int p = precision == 1 ? 1 : 0;
+    const int cnt = p ? GemmTable<float>::count : GemmTable<double>::count;
+    if (big < 0 || big >= cnt || small < 0 || small >= cnt) return TAGS_ERR_ARG;
+    g_big_variant[p] = big;
+    g_small_variant[p] = small;
+    return 0;
+}
+
+int gemm_variant_count(int precision)
+{
+    return precision == 1 ? GemmTable<float>::count : GemmTable<double>::count;
+}
+
+template int launch_gemm<double>(const GemmArgs<double>&, hipStream_t);
+template int launch_gemm<float>(const GemmArgs<float>&, hipStream_t);
+template int launch_gemm_variant<double>(const GemmArgs<double>&, int, hipStream_t);
+template int launch_gemm_variant<float>(const GemmArgs<float>&, int, hipStream_t);
+
+}  // namespace tags

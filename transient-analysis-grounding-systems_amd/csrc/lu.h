@@ -113,35 +113,105 @@ struct GemmArgs {
     int mode;  // 0: C = A B; 1: C -= A B
 };
 
+// One work-group's pivot candidate for one column of the panel.
+template <typename T>
+struct PanelSlot {
+    unsigned long long key;
+    unsigned long long pad;
+    cx<T> row[64];
+};
+
 template <typename T>
 struct PanelArgs {
     cx<T>* A;
     long ld;
     int* perm;
     int r0, M, c0, w, R;
-    unsigned long long* keys;
-    cx<T>* cand;
+    int cand_end;         // logical rows >= cand_end are updated but never pivots
+    PanelSlot<T>* slots;  // [2][P]
     unsigned* bar;
     int* info;
     int* err;
 };
 
+// Column phases of a factorisation (see lu.hip).
+struct LUPhase {
+    int c_begin, c_end;   // columns factored in this phase
+    int row_end;          // logical rows [column, row_end) may be non-zero
+    int cand_end;         // pivot candidates are logical rows [column, cand_end)
+    int x_begin, x_end;   // extra trailing columns beyond c_end (empty if x_begin >= x_end)
+};
+
+struct LUPlan {
+    std::vector<LUPhase> phases;
+    std::vector<int> init_perm;  // logical -> physical row order at start (empty: identity)
+    static LUPlan dense(int n)
+    {
+        LUPlan p;
+        p.phases.push_back(LUPhase{0, n, n, n, 0, 0});
+        return p;
+    }
+    // Immittance matrix in the reference unknown order [U (n), I_L (m), I_T (m)]:
+    //   [[0, A^T, B^T], [A, -Z_L, 0], [B, 0, -Z_T]].
+    // The node rows are exactly zero in the node columns (Y_E = 0) and stay
+    // so while those columns are factored, so the first n columns pivot and
+    // update over the 2m impedance rows only (logical rows start as
+    // [Z_L rows, Z_T rows, node rows]). Same pivots as dense partial pivoting.
+    static LUPlan nodes_first(int m, int n)
+    {
+        LUPlan p;
+        const int N = n + 2 * m;
+        p.phases.push_back(LUPhase{0, n, 2 * m, 2 * m, n, N});
+        p.phases.push_back(LUPhase{n, N, N, N, 0, 0});
+        p.init_perm.resize(N);
+        for (int i = 0; i < 2 * m; i++) p.init_perm[i] = n + i;
+        for (int i = 0; i < n; i++) p.init_perm[2 * m + i] = i;
+        return p;
+    }
+    // Immittance matrix stored with unknowns [I_L (m), I_T (m), U (n)]:
+    //   [[-Z_L, 0, A], [0, -Z_T, B], [A^T, B^T, 0]].
+    // Logical rows start as [Z_L rows, node rows, Z_T rows]. While the Z_L
+    // columns are factored the Z_T rows are exactly zero in them (and receive
+    // no update), so they are skipped; the rest is ordinary partial pivoting
+    // (exact). With block_pivot the Z_L columns pivot among Z_L rows only, the
+    // Z_T columns then stay untouched too (fewest flops, weaker pivoting).
+    static LUPlan zl_first(int m, int n, bool block_pivot = false)
+    {
+        LUPlan p;
+        const int N = n + 2 * m;
+        if (block_pivot)
+            p.phases.push_back(LUPhase{0, m, m + n, m, 2 * m, N});
+        else
+            p.phases.push_back(LUPhase{0, m, m + n, m + n, m, N});
+        p.phases.push_back(LUPhase{m, N, N, N, 0, 0});
+        p.init_perm.resize(N);
+        for (int i = 0; i < m; i++) p.init_perm[i] = i;
+        for (int i = 0; i < n; i++) p.init_perm[m + i] = 2 * m + i;
+        for (int i = 0; i < m; i++) p.init_perm[m + n + i] = m + i;
+        return p;
+    }
+};
+
 // Device work space of one LU of order n in precision T.
 template <typename T>
 struct LUWork {
-    int n = 0, nblk = 0;
+    int n = 0, nblk = 0, nblk_cap = 0;
     int max_wgs = 256;
     int num_cu = 256;
     size_t lds_limit = 65536;
     int* perm = nullptr;
-    cx<T>* invL = nullptr;
+    cx<T>* invL = nullptr;  // dense 64 x 64 inverses of the diagonal blocks
     cx<T>* invU = nullptr;
     cx<T>* tmp = nullptr;
-    cx<T>* cand = nullptr;
     cx<T>* vec = nullptr;
-    unsigned long long* keys = nullptr;
+    PanelSlot<T>* slots = nullptr;
     unsigned* bar = nullptr;
-    int* flags = nullptr;  // [0] info, [1] barrier error
+    int* flags = nullptr;   // [0] info, [1] barrier error
+    int* d_blk = nullptr;   // device copy of blk
+    int* d_init = nullptr;  // device copy of the plan's initial row order
+    std::vector<int> blk;   // start column of every diagonal block, in order
+    std::vector<int> h_blk, h_init;  // what d_blk / d_init currently hold
+    double flops = 0;       // real flops of the last factorisation (GEMM + panel)
 
     int alloc(int n_, int max_panel_wgs, size_t panel_lds_limit);
     void release();
@@ -150,18 +220,23 @@ struct LUWork {
 };
 
 template <typename T>
-int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr);
+int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr,
+              const LUPlan* plan = nullptr);
 template <typename T>
 int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
                      Profiler* pr);
 template <typename T>
 int lu_gather_rhs(LUWork<T>& w, const cx<T>* b, cx<T>* y, hipStream_t st);
 template <typename T>
-int gemm_public(const GemmArgs<T>& g, hipStream_t st);
+int launch_gemm(const GemmArgs<T>& g, hipStream_t st);
+template <typename T>
+int launch_gemm_variant(const GemmArgs<T>& g, int variant, hipStream_t st);
+int gemm_set_variants(int precision, int big, int small);
+int gemm_variant_count(int precision);
 
 int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
                unsigned long long* norms, bool want_wnorm, hipStream_t st, Profiler* pr);
-int row_scale(const c64* W, long ld, int n, double* d, hipStream_t st);
+int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t st);
 int to_c32_scaled(const c64* W, long ld, int n, const double* d, c32* W32, long ld32,
                   hipStream_t st);
 int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv_scale, c32* t,

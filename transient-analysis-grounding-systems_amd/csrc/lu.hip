@@ -9,13 +9,20 @@
 // the factorisation row perm[i] holds row i of L (unit, strictly lower) and U,
 // i.e. P A = L U with (P A)[i] = A[perm[i]].
 //
-// Per 128-column step j (two 64-column sub-panels a, b):
+// A factorisation follows an LUPlan: a list of column phases, each with the
+// logical row range that can hold non-zeros in its columns and an optional
+// extra range of trailing columns. Rows and columns that are structurally zero
+// for a phase are skipped, which is exact for partial pivoting (a zero row is
+// never the largest candidate unless the whole column is zero).
+//
+// Per 128-column step j (two <= 64-column sub-panels a, b):
 //   panel(a)            cooperative kernel, one grid barrier per column
-//   inv(diag a)         64 x 64 unit-lower and upper inverses in LDS
+//   inv(diag a)         unit-lower inverse of the diagonal block in LDS
 //   update b            U_ab = inv(L_aa) A_ab; A_b -= L_a U_ab   (MFMA GEMM)
 //   panel(b), inv(diag b)
-//   U12                 inv(L11) applied to the 128 pivot rows    (MFMA GEMM)
-//   trailing update     A22 -= L21 U12, K = 128                   (MFMA GEMM)
+//   U12                 inv(L11) applied to the pivot rows          (MFMA GEMM)
+//   trailing update     A22 -= L21 U12, K = 128                     (MFMA GEMM)
+// and after the last step the upper inverses of all diagonal blocks.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -25,154 +32,14 @@
 namespace tags {
 
 // ---------------------------------------------------------------------------
-// MFMA wrappers: 16x16x4 blocks; lane l supplies A[l%16][l/16] and
-// B[l/16][l%16]. Result element v of lane l is D[row(l, v)][l%16] with
-// row = l/16 + 4v for the f64 instruction and 4(l/16) + v for the f32 one
-// (measured on gfx950).
-// ---------------------------------------------------------------------------
-template <typename T>
-struct Mfma;
-template <>
-struct Mfma<double> {
-    typedef double v4 __attribute__((ext_vector_type(4)));
-    static __device__ inline v4 run(double a, double b, v4 c)
-    {
-        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int lane, int v) { return (lane >> 4) + 4 * v; }
-};
-template <>
-struct Mfma<float> {
-    typedef float v4 __attribute__((ext_vector_type(4)));
-    static __device__ inline v4 run(float a, float b, v4 c)
-    {
-        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-    }
-    static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
-};
-
-// ---------------------------------------------------------------------------
-// Complex GEMM  C (+)= s * A B  with row maps, row-major operands.
-// Work-group tile BM x BN, 4 waves in a 2 x 2 arrangement, K staged through
-// LDS in chunks of BK; each wave owns (BM/2) x (BN/2) as 16 x 16 MFMA blocks,
-// four real MFMAs per complex block product (re*re - im*im, re*im + im*re).
-// ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void k_cgemm(GemmArgs<T> g)
-{
-    using V4 = typename Mfma<T>::v4;
-    constexpr int TM = BM / 32, TN = BN / 32;
-    __shared__ cx<T> As[BM][BK + 1];
-    __shared__ cx<T> Bs[BK][BN + 1];
-    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wm = wave >> 1, wn = wave & 1;
-    const int tiles_n = (g.N + BN - 1) / BN;
-    const int tiles_m = (g.M + BM - 1) / BM;
-    const int ntiles = tiles_n * tiles_m;
-    // XCD-aware order: consecutive dispatch ids land on different XCDs; give
-    // each XCD a contiguous range of tiles (row-band) so A rows stay in its L2.
-    int bid = blockIdx.x;
-    {
-        const int per = (ntiles + 7) / 8;
-        const int xcd = bid & 7, idx = bid >> 3;
-        bid = xcd * per + idx;  // a bijection of [0, 8 * per)
-    }
-    if (bid >= ntiles) return;
-    const int tm = bid / tiles_n, tn = bid % tiles_n;
-    const int m0 = tm * BM, n0 = tn * BN;
-
-    V4 are[TM][TN], aim[TM][TN];
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-            are[i][j] = V4{0, 0, 0, 0};
-            aim[i][j] = V4{0, 0, 0, 0};
-        }
-
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        __syncthreads();
-        for (int e = tid; e < BM * BK; e += 256) {
-            const int r = e / BK, c = e % BK;
-            const int gr = m0 + r, gc = k0 + c;
-            cx<T> v = cx<T>{0, 0};
-            if (gr < g.M && gc < g.K) {
-                const long pr = g.arow ? g.arow[gr] : gr;
-                v = g.A[pr * g.lda + gc];
-            }
-            As[r][c] = v;
-        }
-        for (int e = tid; e < BK * BN; e += 256) {
-            const int r = e / BN, c = e % BN;
-            const int gr = k0 + r, gc = n0 + c;
-            cx<T> v = cx<T>{0, 0};
-            if (gr < g.K && gc < g.N) {
-                const long pr = g.brow ? g.brow[gr] : gr;
-                v = g.B[pr * g.ldb + gc];
-            }
-            Bs[r][c] = v;
-        }
-        __syncthreads();
-#pragma unroll
-        for (int kk = 0; kk < BK; kk += 4) {
-            cx<T> a[TM], b[TN];
-#pragma unroll
-            for (int i = 0; i < TM; i++)
-                a[i] = As[wm * (BM / 2) + i * 16 + (lane & 15)][kk + (lane >> 4)];
-#pragma unroll
-            for (int j = 0; j < TN; j++)
-                b[j] = Bs[kk + (lane >> 4)][wn * (BN / 2) + j * 16 + (lane & 15)];
-#pragma unroll
-            for (int i = 0; i < TM; i++)
-#pragma unroll
-                for (int j = 0; j < TN; j++) {
-                    are[i][j] = Mfma<T>::run(a[i].x, b[j].x, are[i][j]);
-                    are[i][j] = Mfma<T>::run(-a[i].y, b[j].y, are[i][j]);
-                    aim[i][j] = Mfma<T>::run(a[i].x, b[j].y, aim[i][j]);
-                    aim[i][j] = Mfma<T>::run(a[i].y, b[j].x, aim[i][j]);
-                }
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < TM; i++)
-#pragma unroll
-        for (int j = 0; j < TN; j++)
-#pragma unroll
-            for (int v = 0; v < 4; v++) {
-                const int row = m0 + wm * (BM / 2) + i * 16 + Mfma<T>::row(lane, v);
-                const int col = n0 + wn * (BN / 2) + j * 16 + (lane & 15);
-                if (row < g.M && col < g.N) {
-                    const long pr = g.crow ? g.crow[row] : row;
-                    cx<T>* p = g.C + pr * g.ldc + col;
-                    if (g.mode == 0) {
-                        *p = cx<T>{are[i][j][v], aim[i][j][v]};
-                    } else {
-                        cx<T> c = *p;
-                        *p = cx<T>{c.x - are[i][j][v], c.y - aim[i][j][v]};
-                    }
-                }
-            }
-}
-
-template <typename T>
-int launch_gemm(const GemmArgs<T>& g, hipStream_t st)
-{
-    if (g.M <= 0 || g.N <= 0) return 0;
-    constexpr int BM = 64, BN = 64, BK = 16;
-    const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
-    const long grid = (tiles + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK>), dim3((unsigned)grid), dim3(256), 0, st, g);
-    TAGS_HIP(hipGetLastError());
-    return 0;
-}
-
-// ---------------------------------------------------------------------------
-// Cooperative panel factorisation of `w` <= 64 columns starting at column c0
-// over logical rows r0 .. r0+M-1. Every work-group holds R rows of the panel in
-// LDS. Per column: local pivot search, one 64-bit atomicMax on a packed key
-// (39 bits of |re|+|im| | 16 bits inverted row offset | 9 bits work-group),
-// grid barrier, broadcast of the winning row through a global candidate slot,
-// relabelling of the two logical positions, scaling and rank-1 update.
+// Cooperative panel factorisation of w <= 64 columns starting at column c0 over
+// logical rows r0 .. r0+M-1. Each work-group holds R rows of the panel in LDS.
+// Per column: local pivot search; the work-group writes its best candidate
+// (packed key: 39 bits of |re|+|im|, 16 bits inverted row offset, 9 bits
+// work-group id) and that row into its own slot; one grid barrier (release
+// arrival, acquire wait); every work-group reads all keys, agrees on the
+// winner, reads its row, relabels the two logical positions, scales and does
+// the rank-1 update. No contended atomics besides the barrier counter.
 // ---------------------------------------------------------------------------
 __device__ inline bool grid_barrier(unsigned* bar, unsigned target, int* err)
 {
@@ -180,10 +47,9 @@ __device__ inline bool grid_barrier(unsigned* bar, unsigned target, int* err)
     __shared__ int s_fail;
     if (threadIdx.x == 0) {
         s_fail = 0;
-        __threadfence();
-        atomicAdd(bar, 1u);
+        __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         long spins = 0;
-        while (__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        while (__hip_atomic_load(bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
             __builtin_amdgcn_s_sleep(1);
             if ((++spins & 1023) == 0) {
                 if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ||
@@ -194,7 +60,6 @@ __device__ inline bool grid_barrier(unsigned* bar, unsigned target, int* err)
                 }
             }
         }
-        __threadfence();
     }
     __syncthreads();
     return s_fail == 0;
@@ -204,7 +69,7 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    cx<T>* rows = reinterpret_cast<cx<T>*>(smem);   // [R][64]
+    cx<T>* rows = reinterpret_cast<cx<T>*>(smem);  // [R][64]
     int* phys = reinterpret_cast<int*>(rows + (size_t)p.R * 64);
     int* lpos = phys + p.R;
     __shared__ cx<T> prow[64];
@@ -239,9 +104,10 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
         const int d = p.r0 + c;
         unsigned long long best = 0;
         for (int r = t; r < p.R; r += 256) {
-            if (lpos[r] >= d) {
+            if (lpos[r] >= d && lpos[r] < p.cand_end) {
                 const double v = (double)cabs1(rows[r * 64 + c]);
-                const unsigned long long bits = (unsigned long long)__double_as_longlong(v);
+                const unsigned long long bits =
+                    (unsigned long long)__double_as_longlong(v) & 0x7FFFFFFFFFFFFFFFull;
                 const unsigned long long key =
                     ((bits >> 25) << 25) |
                     ((unsigned long long)(0xFFFF - (lpos[r] - p.r0)) << 9) | (unsigned)wg;
@@ -261,28 +127,44 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
             s_best = -1;
         }
         __syncthreads();
+        PanelSlot<T>* mine = p.slots + (size_t)(c & 1) * P + wg;
         const unsigned long long wkey = s_key;
         if (wkey) {
             const int lpw = p.r0 + 0xFFFF - (int)((wkey >> 9) & 0xFFFF);
             for (int r = t; r < p.R; r += 256)
                 if (lpos[r] == lpw) s_best = r;
             __syncthreads();
-            cx<T>* dst = p.cand + ((size_t)(c & 1) * P + wg) * 64;
             const int rb = s_best;
             for (int q = t; q < 64; q += 256)
-                if (q >= c && q < p.w) dst[q] = rows[rb * 64 + q];
-            if (t == 0) atomicMax(&p.keys[c % 3], wkey);
+                if (q >= c && q < p.w) mine->row[q] = rows[rb * 64 + q];
         }
+        if (t == 0) mine->key = wkey;
         ok = grid_barrier(p.bar, (unsigned)(c + 1) * P, p.err);
         if (!ok) break;
-        if (wg == 0 && t == 0) p.keys[(c + 2) % 3] = 0ull;
-        const unsigned long long key =
-            __hip_atomic_load(&p.keys[c % 3], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // agree on the winner: max over all slots' keys
+        unsigned long long kb = 0;
+        for (int i = t; i < P; i += 256) {
+            const unsigned long long k = p.slots[(size_t)(c & 1) * P + i].key;
+            kb = k > kb ? k : kb;
+        }
+        for (int off = 32; off > 0; off >>= 1) {
+            const unsigned long long o = __shfl_xor(kb, off);
+            kb = o > kb ? o : kb;
+        }
+        if ((t & 63) == 0) red[t >> 6] = kb;
+        __syncthreads();
+        if (t == 0) {
+            unsigned long long b = red[0];
+            for (int i = 1; i < 4; i++) b = red[i] > b ? red[i] : b;
+            s_key = b;
+        }
+        __syncthreads();
+        const unsigned long long key = s_key;
         const int pwg = (int)(key & 0x1FF);
         const int plp = p.r0 + 0xFFFF - (int)((key >> 9) & 0xFFFF);
-        const cx<T>* src = p.cand + ((size_t)(c & 1) * P + pwg) * 64;
+        const PanelSlot<T>* src = p.slots + (size_t)(c & 1) * P + pwg;
         for (int q = t; q < 64; q += 256)
-            if (q >= c && q < p.w) prow[q] = src[q];
+            if (q >= c && q < p.w) prow[q] = src->row[q];
         __syncthreads();
         for (int r = t; r < p.R; r += 256) {
             const int l = lpos[r];
@@ -319,31 +201,44 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
         for (int r = t; r < p.R; r += 256)
             if (phys[r] >= 0) p.perm[lpos[r]] = phys[r];
     }
+    // the last work-group out resets the barrier counters for the next launch
     __syncthreads();
     if (t == 0) {
-        __threadfence();
-        const unsigned prev = atomicAdd(&p.bar[1], 1u);
+        const unsigned prev = __hip_atomic_fetch_add(&p.bar[1], 1u, __ATOMIC_ACQ_REL,
+                                                     __HIP_MEMORY_SCOPE_AGENT);
         if (prev == (unsigned)P - 1) {
-            p.bar[0] = 0;
-            p.bar[1] = 0;
-            p.keys[0] = p.keys[1] = p.keys[2] = 0ull;
-            __threadfence();
+            __hip_atomic_store(&p.bar[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&p.bar[1], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
     }
 }
 
 // ---------------------------------------------------------------------------
-// Inverses of the 64 x 64 diagonal block at (c0, c0): unit lower L and upper U,
-// inverted in place in LDS (columns right-to-left for L, left-to-right for U),
-// written as dense 64 x 64 row-major blocks (identity padding when w < 64).
+// Inverses of the 64 x 64 diagonal blocks, written as dense 64 x 64 row-major
+// blocks (identity padding past column w). The block at column c0 has logical
+// rows c0 .. c0+w (physical perm[c0 + r]).
+//   unit lower: X[i][j] = -(L[i][j] + sum_{k=j+1}^{i-1} X[i][k] L[k][j]), i > j,
+//               columns right to left;
+//   upper:      X[j][j] = 1/u_jj, X[i][j] = -(sum_{k=i}^{j-1} X[i][k] U[k][j]) / u_jj,
+//               columns left to right.
+// Each row's dot product is split over 4 adjacent lanes; one barrier per
+// column. blockIdx.x selects the block (one work-group per block).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ __launch_bounds__(256) void k_diag_inv(const cx<T>* A, long ld, const int* perm,
-                                                   int c0, int w, cx<T>* invL, cx<T>* invU)
+__device__ inline cx<T> quad_sum(cx<T> s)
 {
-    __shared__ cx<T> B[64][64];
-    const int t = threadIdx.x;
-    for (int e = t; e < 64 * 64; e += 256) {
+    s.x += __shfl_xor(s.x, 1);
+    s.y += __shfl_xor(s.y, 1);
+    s.x += __shfl_xor(s.x, 2);
+    s.y += __shfl_xor(s.y, 2);
+    return s;
+}
+
+template <typename T>
+__device__ inline void load_diag_block(const cx<T>* A, long ld, const int* perm, int c0, int w,
+                                       cx<T> (*B)[64])
+{
+    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
         const int r = e >> 6, c = e & 63;
         cx<T> v = cx<T>{0, 0};
         if (r < w && c < w)
@@ -352,40 +247,70 @@ __global__ __launch_bounds__(256) void k_diag_inv(const cx<T>* A, long ld, const
             v = cx<T>{1, 0};
         B[r][c] = v;
     }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv_lower(const cx<T>* A, long ld, const int* perm,
+                                                    int c0, int w, cx<T>* out)
+{
+    __shared__ cx<T> B[64][64];
+    __shared__ cx<T> X[64][64];
+    load_diag_block(A, ld, perm, c0, w, B);
+    const int t = threadIdx.x, row_off = t >> 2, part = t & 3;
+    for (int e = t; e < 64 * 64; e += 256) (&X[0][0])[e] = cx<T>{0, 0};
     __syncthreads();
-    // L^{-1}: X[i][j] = -(L[i][j] + sum_{k=j+1}^{i-1} X[i][k] L[k][j]),  i > j
-    for (int j = 62; j >= 0; j--) {
+    for (int j = 63; j >= 0; j--) {
+        const int i = j + 1 + row_off;
         cx<T> s = cx<T>{0, 0};
-        const int i = t;
-        if (i < 64 && i > j) {
-            s = B[i][j];
-            for (int k = j + 1; k < i; k++) s = cadd(s, cmul(B[i][k], B[k][j]));
+        if (i < 64) {
+            for (int k = j + 1 + part; k < i; k += 4) s = cadd(s, cmul(X[i][k], B[k][j]));
         }
-        __syncthreads();
-        if (i < 64 && i > j) B[i][j] = cx<T>{-s.x, -s.y};
-        __syncthreads();
-    }
-    // U^{-1}: X[j][j] = 1/u_jj; X[i][j] = -(sum_{k=i}^{j-1} X[i][k] U[k][j]) / u_jj
-    for (int j = 0; j < 64; j++) {
-        const cx<T> ujj = B[j][j];
-        const cx<T> inv = crecip(ujj);
-        cx<T> s = cx<T>{0, 0};
-        const int i = t;
-        if (i < j) {
-            for (int k = i; k < j; k++) s = cadd(s, cmul(B[i][k], B[k][j]));
+        s = quad_sum(s);
+        if (i < 64 && part == 0) {
+            const cx<T> v = cadd(s, B[i][j]);
+            X[i][j] = cx<T>{-v.x, -v.y};
         }
-        __syncthreads();
-        if (i < j) {
-            const cx<T> v = cmul(s, inv);
-            B[i][j] = cx<T>{-v.x, -v.y};
-        }
-        if (i == j) B[j][j] = inv;
         __syncthreads();
     }
     for (int e = t; e < 64 * 64; e += 256) {
         const int r = e >> 6, c = e & 63;
-        invL[e] = r > c ? B[r][c] : (r == c ? cx<T>{1, 0} : cx<T>{0, 0});
-        invU[e] = r <= c ? B[r][c] : cx<T>{0, 0};
+        out[e] = r > c ? X[r][c] : (r == c ? cx<T>{1, 0} : cx<T>{0, 0});
+    }
+}
+
+// one work-group per diagonal block b: columns blk[b] .. blk[b+1] (or n)
+template <typename T>
+__global__ __launch_bounds__(256) void k_inv_upper(const cx<T>* A, long ld, const int* perm,
+                                                    const int* blk, int nblk, int n,
+                                                    cx<T>* inv_base)
+{
+    __shared__ cx<T> B[64][64];
+    __shared__ cx<T> X[64][64];
+    const int b = blockIdx.x;
+    const int c0 = blk[b];
+    const int w = (b + 1 < nblk ? blk[b + 1] : n) - c0;
+    cx<T>* out = inv_base + (size_t)b * 4096;
+    load_diag_block(A, ld, perm, c0, w, B);
+    const int t = threadIdx.x, row = t >> 2, part = t & 3;
+    for (int e = t; e < 64 * 64; e += 256) (&X[0][0])[e] = cx<T>{0, 0};
+    __syncthreads();
+    for (int j = 0; j < 64; j++) {
+        const cx<T> inv = crecip(B[j][j]);
+        cx<T> s = cx<T>{0, 0};
+        if (row < j) {
+            for (int k = row + part; k < j; k += 4) s = cadd(s, cmul(X[row][k], B[k][j]));
+        }
+        s = quad_sum(s);
+        if (row < j && part == 0) {
+            const cx<T> v = cmul(s, inv);
+            X[row][j] = cx<T>{-v.x, -v.y};
+        }
+        if (row == j && part == 0) X[j][j] = inv;
+        __syncthreads();
+    }
+    for (int e = t; e < 64 * 64; e += 256) {
+        const int r = e >> 6, c = e & 63;
+        out[e] = r <= c ? X[r][c] : cx<T>{0, 0};
     }
 }
 
@@ -401,10 +326,12 @@ __global__ void k_gather_rows(const cx<T>* A, long ld, const int* rowmap, int c0
         dst[(long)r * cols + c] = A[pr * ld + c0 + c];
 }
 
-__global__ void k_iota(int* p, int n)
+// perm = init (or identity); flags (info, barrier error) cleared
+__global__ void k_lu_init(int* perm, const int* init, int n, int* flags)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < n) p[i] = i;
+    if (i < n) perm[i] = init ? init[i] : i;
+    if (i < 2) flags[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -489,10 +416,16 @@ __global__ __launch_bounds__(256) void k_residual(const c64* W, long ld, int n, 
     }
 }
 
-// row maxima of |W_ij| (for symmetric equilibration): out[i] = 1/sqrt(max_j |W_ij|)
-__global__ __launch_bounds__(256) void k_row_scale(const c64* W, long ld, int n, double* d)
+// row maxima of |W_ij| (for symmetric equilibration): out[i] = 1/sqrt(max_j |W_ij|),
+// or 1 when equilibration is off
+__global__ __launch_bounds__(256) void k_row_scale(const c64* W, long ld, int n, double* d,
+                                                    int enabled)
 {
     const int row = blockIdx.x;
+    if (!enabled) {
+        if (threadIdx.x == 0) d[row] = 1.0;
+        return;
+    }
     double mx = 0;
     for (int c = threadIdx.x; c < n; c += 256) {
         const c64 a = W[(long)row * ld + c];
@@ -581,84 +514,97 @@ template <typename T>
 int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit)
 {
     num_cu = max_panel_wgs;
+    max_wgs = max_panel_wgs;
+    lds_limit = panel_lds_limit;
     if (n_ <= 0) return TAGS_ERR_ARG;
     if (n_ >= 65536) return TAGS_ERR_TOO_LARGE;
     if (n == n_) return 0;
     release();
     n = n_;
-    nblk = (n + 63) / 64;
+    nblk_cap = (n + 63) / 64 + 8;
     TAGS_HIP(hipMalloc(&perm, sizeof(int) * n));
-    TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * 64 * 64 * nblk));
-    TAGS_HIP(hipMalloc(&invU, sizeof(cx<T>) * 64 * 64 * nblk));
+    TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * 4096 * (size_t)nblk_cap));
+    TAGS_HIP(hipMalloc(&invU, sizeof(cx<T>) * 4096 * (size_t)nblk_cap));
     TAGS_HIP(hipMalloc(&tmp, sizeof(cx<T>) * 128 * (size_t)n));
-    TAGS_HIP(hipMalloc(&cand, sizeof(cx<T>) * 2 * 512 * 64));
-    TAGS_HIP(hipMalloc(&keys, sizeof(unsigned long long) * 4));
-    TAGS_HIP(hipMalloc(&bar, sizeof(unsigned) * 4));
+    TAGS_HIP(hipMalloc(&slots, sizeof(PanelSlot<T>) * 2 * 512));
+    TAGS_HIP(hipMalloc(&bar, sizeof(unsigned) * 64));
     TAGS_HIP(hipMalloc(&flags, sizeof(int) * 4));
     TAGS_HIP(hipMalloc(&vec, sizeof(cx<T>) * n));
-    TAGS_HIP(hipMemset(keys, 0, sizeof(unsigned long long) * 4));
-    TAGS_HIP(hipMemset(bar, 0, sizeof(unsigned) * 4));
+    TAGS_HIP(hipMalloc(&d_blk, sizeof(int) * nblk_cap));
+    TAGS_HIP(hipMalloc(&d_init, sizeof(int) * n));
+    TAGS_HIP(hipMemset(bar, 0, sizeof(unsigned) * 64));
     TAGS_HIP(hipMemset(flags, 0, sizeof(int) * 4));
-    max_wgs = max_panel_wgs;
-    lds_limit = panel_lds_limit;
     return 0;
 }
 
 template <typename T>
 void LUWork<T>::release()
 {
-    hipFree(perm);
-    hipFree(invL);
-    hipFree(invU);
-    hipFree(tmp);
-    hipFree(cand);
-    hipFree(keys);
-    hipFree(bar);
-    hipFree(flags);
-    hipFree(vec);
+    (void)hipFree(perm);
+    (void)hipFree(invL);
+    (void)hipFree(invU);
+    (void)hipFree(tmp);
+    (void)hipFree(slots);
+    (void)hipFree(bar);
+    (void)hipFree(flags);
+    (void)hipFree(vec);
+    (void)hipFree(d_blk);
+    (void)hipFree(d_init);
+    d_init = nullptr;
+    h_init.clear();
     perm = nullptr;
-    invL = invU = tmp = cand = vec = nullptr;
-    keys = nullptr;
+    invL = invU = tmp = vec = nullptr;
+    slots = nullptr;
     bar = nullptr;
     flags = nullptr;
-    n = 0;
+    d_blk = nullptr;
+    blk.clear();
+    n = nblk = 0;
 }
 
 template <typename T>
 static int gemm(Profiler* pr, hipStream_t st, int M, int N, int K, const cx<T>* A, long lda,
                 const int* arow, const cx<T>* B, long ldb, const int* brow, cx<T>* C, long ldc,
-                const int* crow, int mode, bool trailing)
+                const int* crow, int mode, bool trailing, double* flops)
 {
+    if (M <= 0 || N <= 0 || K <= 0) return 0;
     GemmArgs<T> g{M, N, K, A, lda, arow, B, ldb, brow, C, ldc, crow, mode};
     Timed tm(pr, FAM_GEMM, st);
     int rc = launch_gemm<T>(g, st);
-    tm.end(8.0 * M * N * (double)K, trailing);
+    const double f = 8.0 * M * N * (double)K;
+    tm.end(f, trailing);
+    if (flops) *flops += f;
     return rc;
+}
+
+// rows of the panel held by one work-group (bounded by the LDS it may use)
+template <typename T>
+static int panel_rows_max(size_t lds_limit)
+{
+    const size_t per_row = 64 * sizeof(cx<T>) + 2 * sizeof(int);
+    return (int)((lds_limit - 4096) / per_row);
 }
 
 template <typename T>
 static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int c0,
-                 int cols)
+                 int cols, int row_end, int cand_end)
 {
-    const int M = w.n - c0;
-    // rows per work-group: at least 64, bounded by the LDS a work-group may use
+    const int M = row_end - c0;
     const size_t per_row = 64 * sizeof(cx<T>) + 2 * sizeof(int);
-    const int rmax = (int)((w.lds_limit - 2048) / per_row);
-    int P = std::min(w.max_wgs, (M + 63) / 64);
+    const int rmax = panel_rows_max<T>(w.lds_limit);
+    // few work-groups keep the grid barrier cheap; ~160 rows each
+    int R = std::min(rmax, 160);
+    int P = (M + R - 1) / R;
     if (P < 1) P = 1;
-    int R = (M + P - 1) / P;
-    if (R > rmax) {
-        R = rmax;
-        P = (M + R - 1) / R;
-    }
+    R = (M + P - 1) / P;
     {
         // every work-group of a cooperative launch must be resident at once
         int occ = 0;
-        TAGS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-            &occ, (const void*)k_panel<T>, 256, (size_t)R * per_row));
+        TAGS_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, (const void*)k_panel<T>, 256,
+                                                              (size_t)R * per_row));
         if (P > occ * w.num_cu || P > 511) return TAGS_ERR_TOO_LARGE;
     }
-    PanelArgs<T> p{A, ld, w.perm, c0, M, c0, cols, R, w.keys, w.cand, w.bar, w.flags,
+    PanelArgs<T> p{A, ld, w.perm, c0, M, c0, cols, R, cand_end, w.slots, w.bar, w.flags,
                    w.flags + 1};
     const size_t lds = (size_t)R * per_row;
     void* args[] = {&p};
@@ -666,6 +612,7 @@ static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, 
     TAGS_HIP(hipLaunchCooperativeKernel((const void*)k_panel<T>, dim3(P), dim3(256), args,
                                         (unsigned)lds, st));
     tm.end();
+    w.flops += 8.0 * (double)M * cols * cols / 2;
     return 0;
 }
 
@@ -686,82 +633,137 @@ int LUWork<T>::setup_kernel_attrs()
 }
 
 template <typename T>
-int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr)
+static int gather(Profiler* pr, hipStream_t st, const cx<T>* A, long ld, const int* rowmap,
+                  int c0, int rows, int cols, cx<T>* dst)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    Timed tm(pr, FAM_OTHER, st);
+    const unsigned gx = (unsigned)std::min(64, (cols + 255) / 256);
+    hipLaunchKernelGGL(k_gather_rows<T>, dim3(gx, rows), dim3(256), 0, st, A, ld, rowmap, c0, rows,
+                       cols, dst);
+    tm.end();
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+// U12 for logical rows [r0, r0+h) of the column range [cb, ce): rows are
+// overwritten with inv(L_block) * rows (block inverse `inv`, h <= 64).
+template <typename T>
+static int apply_linv_rows(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
+                           const cx<T>* inv, int r0, int h, int cb, int ce)
+{
+    const int cols = ce - cb;
+    if (cols <= 0) return 0;
+    int rc = gather<T>(pr, st, A, ld, w.perm + r0, cb, h, cols, w.tmp);
+    if (rc) return rc;
+    return gemm<T>(pr, st, h, cols, h, inv, 64, nullptr, w.tmp, cols, nullptr, A + cb, ld,
+                   w.perm + r0, 0, false, &w.flops);
+}
+
+template <typename T>
+int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, const LUPlan* plan)
 {
     const int n = w.n;
     int rc = w.setup_kernel_attrs();
     if (rc) return rc;
-    hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, st, w.perm, n);
-    TAGS_HIP(hipMemsetAsync(w.flags, 0, sizeof(int) * 2, st));
+    LUPlan dense;
+    if (!plan) {
+        dense = LUPlan::dense(n);
+        plan = &dense;
+    }
+    // Initial row order and cleared flags are set by an ordinary kernel on the
+    // stream: the cooperative panel launches are ordered after kernels, and the
+    // initial order is uploaded once with a blocking copy when it changes.
+    const int* init = nullptr;
+    if (!plan->init_perm.empty()) {
+        if ((int)plan->init_perm.size() != n) return TAGS_ERR_ARG;
+        if (w.h_init != plan->init_perm) {
+            TAGS_HIP(hipStreamSynchronize(st));
+            TAGS_HIP(hipMemcpy(w.d_init, plan->init_perm.data(), sizeof(int) * n,
+                               hipMemcpyHostToDevice));
+            w.h_init = plan->init_perm;
+        }
+        init = w.d_init;
+    }
+    hipLaunchKernelGGL(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, st, w.perm, init, n,
+                       w.flags);
+    TAGS_HIP(hipGetLastError());
     int* perm = w.perm;
-    for (int j = 0; j < n; j += 128) {
-        const int jb = std::min(128, n - j);
-        const int wa = std::min(64, jb), wb = jb - wa;
-        // sub-panel a
-        if ((rc = panel<T>(w, pr, st, A, ld, j, wa))) return rc;
-        {
-            Timed tm(pr, FAM_OTHER, st);
-            hipLaunchKernelGGL(k_diag_inv<T>, dim3(1), dim3(256), 0, st, A, ld, perm, j, wa,
-                               w.invL + (size_t)(j / 64) * 4096, w.invU + (size_t)(j / 64) * 4096);
-            tm.end();
-        }
-        if (wb > 0) {
-            const int c0 = j + 64;
-            // U_ab = inv(L_aa) A_ab
+    w.blk.clear();
+    w.flops = 0;
+    for (const LUPhase& ph : plan->phases) {
+        const int row_end = ph.row_end;
+        for (int j = ph.c_begin; j < ph.c_end; j += 128) {
+            const int jb = std::min(128, ph.c_end - j);
+            const int wa = std::min(64, jb), wb = jb - wa;
+            const int c1 = j + jb;
+            if ((int)w.blk.size() + 2 > w.nblk_cap) return TAGS_ERR_TOO_LARGE;
+            // sub-panel a
+            if ((rc = panel<T>(w, pr, st, A, ld, j, wa, row_end, ph.cand_end))) return rc;
+            cx<T>* inva = w.invL + (size_t)w.blk.size() * 4096;
+            w.blk.push_back(j);
             {
                 Timed tm(pr, FAM_OTHER, st);
-                hipLaunchKernelGGL(k_gather_rows<T>, dim3(1, 64), dim3(256), 0, st, A, ld,
-                                   perm + j, c0, 64, wb, w.tmp);
+                hipLaunchKernelGGL(k_inv_lower<T>, dim3(1), dim3(256), 0, st, A, ld, perm, j, wa,
+                                   inva);
                 tm.end();
             }
-            if ((rc = gemm<T>(pr, st, 64, wb, 64, w.invL + (size_t)(j / 64) * 4096, 64, nullptr,
-                              w.tmp, wb, nullptr, A + c0, ld, perm + j, 0, false)))
-                return rc;
-            // A_b -= L_a U_ab  (logical rows j+64 .. n)
-            if ((rc = gemm<T>(pr, st, n - c0, wb, 64, A + j, ld, perm + c0, A + c0, ld,
-                              perm + j, A + c0, ld, perm + c0, 1, false)))
-                return rc;
-            if ((rc = panel<T>(w, pr, st, A, ld, c0, wb))) return rc;
-            Timed tm(pr, FAM_OTHER, st);
-            hipLaunchKernelGGL(k_diag_inv<T>, dim3(1), dim3(256), 0, st, A, ld, perm, c0, wb,
-                               w.invL + (size_t)(c0 / 64) * 4096,
-                               w.invU + (size_t)(c0 / 64) * 4096);
-            tm.end();
-        }
-        const int c1 = j + jb;
-        const int M2 = n - c1;
-        if (M2 <= 0) break;
-        // U12 rows j .. j+64: inv(L_aa) A
-        {
-            Timed tm(pr, FAM_OTHER, st);
-            hipLaunchKernelGGL(k_gather_rows<T>, dim3((M2 + 255) / 256 < 64 ? (M2 + 255) / 256 : 64, wa),
-                               dim3(256), 0, st, A, ld, perm + j, c1, wa, M2, w.tmp);
-            tm.end();
-        }
-        if ((rc = gemm<T>(pr, st, wa, M2, wa, w.invL + (size_t)(j / 64) * 4096, 64, nullptr,
-                          w.tmp, M2, nullptr, A + c1, ld, perm + j, 0, false)))
-            return rc;
-        if (wb > 0) {
-            const int c0 = j + 64;
-            // rows c0 .. c0+wb: A -= L_ba U_a; then inv(L_bb)
-            if ((rc = gemm<T>(pr, st, wb, M2, 64, A + j, ld, perm + c0, A + c1, ld, perm + j,
-                              A + c1, ld, perm + c0, 1, false)))
-                return rc;
-            {
+            cx<T>* invb = nullptr;
+            if (wb > 0) {
+                const int c0 = j + 64;
+                // U_ab = inv(L_aa) A_ab; A_b -= L_a U_ab (logical rows c0 .. row_end)
+                if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, inva, j, 64, c0, c0 + wb))) return rc;
+                if ((rc = gemm<T>(pr, st, row_end - c0, wb, 64, A + j, ld, perm + c0, A + c0, ld,
+                                  perm + j, A + c0, ld, perm + c0, 1, false, &w.flops)))
+                    return rc;
+                if ((rc = panel<T>(w, pr, st, A, ld, c0, wb, row_end, ph.cand_end))) return rc;
+                invb = w.invL + (size_t)w.blk.size() * 4096;
+                w.blk.push_back(c0);
                 Timed tm(pr, FAM_OTHER, st);
-                hipLaunchKernelGGL(k_gather_rows<T>,
-                                   dim3((M2 + 255) / 256 < 64 ? (M2 + 255) / 256 : 64, wb),
-                                   dim3(256), 0, st, A, ld, perm + c0, c1, wb, M2, w.tmp);
+                hipLaunchKernelGGL(k_inv_lower<T>, dim3(1), dim3(256), 0, st, A, ld, perm, c0, wb,
+                                   invb);
                 tm.end();
             }
-            if ((rc = gemm<T>(pr, st, wb, M2, wb, w.invL + (size_t)(c0 / 64) * 4096, 64,
-                              nullptr, w.tmp, M2, nullptr, A + c1, ld, perm + c0, 0, false)))
-                return rc;
+            // trailing column ranges: [c1, c_end) and the phase's extra range
+            int rb[2] = {c1, ph.x_begin}, re[2] = {ph.c_end, ph.x_end};
+            if (ph.x_begin == ph.c_end && ph.x_end > ph.x_begin) {
+                re[0] = ph.x_end;
+                re[1] = rb[1];
+            }
+            for (int q = 0; q < 2; q++) {
+                if (re[q] <= rb[q]) continue;
+                // U12 rows j .. j+wa
+                if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, inva, j, wa, rb[q], re[q]))) return rc;
+                if (wb > 0) {
+                    const int c0 = j + 64;
+                    if ((rc = gemm<T>(pr, st, wb, re[q] - rb[q], 64, A + j, ld, perm + c0,
+                                      A + rb[q], ld, perm + j, A + rb[q], ld, perm + c0, 1, false,
+                                      &w.flops)))
+                        return rc;
+                    if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, invb, c0, wb, rb[q], re[q])))
+                        return rc;
+                }
+                // trailing update A22 -= L21 U12 (logical rows c1 .. row_end)
+                if ((rc = gemm<T>(pr, st, row_end - c1, re[q] - rb[q], jb, A + j, ld, perm + c1,
+                                  A + rb[q], ld, perm + j, A + rb[q], ld, perm + c1, 1, true,
+                                  &w.flops)))
+                    return rc;
+            }
         }
-        // trailing update A22 -= L21 U12
-        if ((rc = gemm<T>(pr, st, M2, M2, jb, A + j, ld, perm + c1, A + c1, ld, perm + j,
-                          A + c1, ld, perm + c1, 1, true)))
-            return rc;
+    }
+    w.nblk = (int)w.blk.size();
+    if (w.h_blk != w.blk) {
+        TAGS_HIP(hipStreamSynchronize(st));
+        TAGS_HIP(hipMemcpy(w.d_blk, w.blk.data(), sizeof(int) * w.nblk, hipMemcpyHostToDevice));
+        w.h_blk = w.blk;
+    }
+    // inverses of the upper diagonal blocks, needed only by the solves
+    {
+        Timed tm(pr, FAM_OTHER, st);
+        hipLaunchKernelGGL(k_inv_upper<T>, dim3(w.nblk), dim3(256), 0, st, A, ld, perm, w.d_blk,
+                           w.nblk, n, w.invU);
+        TAGS_HIP(hipGetLastError());
+        tm.end();
     }
     return 0;
 }
@@ -775,7 +777,8 @@ int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_
     const int n = w.n;
     Timed tm(pr, FAM_SOLVE, st);
     for (int b = 0; b < w.nblk; b++) {
-        const int r0 = b * 64, bw = std::min(64, n - r0);
+        const int r0 = w.blk[b];
+        const int bw = (b + 1 < w.nblk ? w.blk[b + 1] : n) - r0;
         hipLaunchKernelGGL(k_block_apply<T>, dim3(1), dim3(64), 0, st,
                            w.invL + (size_t)b * 4096, y, r0, bw);
         const int lo = r0 + bw;
@@ -784,7 +787,8 @@ int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_
                                w.perm, y, lo, n, r0, bw);
     }
     for (int b = w.nblk - 1; b >= 0; b--) {
-        const int r0 = b * 64, bw = std::min(64, n - r0);
+        const int r0 = w.blk[b];
+        const int bw = (b + 1 < w.nblk ? w.blk[b + 1] : n) - r0;
         hipLaunchKernelGGL(k_block_apply<T>, dim3(1), dim3(64), 0, st,
                            w.invU + (size_t)b * 4096, y, r0, bw);
         if (r0 > 0)
@@ -817,9 +821,9 @@ int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
     return 0;
 }
 
-int row_scale(const c64* W, long ld, int n, double* d, hipStream_t st)
+int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_row_scale, dim3(n), dim3(256), 0, st, W, ld, n, d);
+    hipLaunchKernelGGL(k_row_scale, dim3(n), dim3(256), 0, st, W, ld, n, d, enabled ? 1 : 0);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
@@ -865,23 +869,17 @@ int sym_from_upper(c64* W, long ld, int n, hipStream_t st)
     return 0;
 }
 
-template <typename T>
-int gemm_public(const GemmArgs<T>& g, hipStream_t st)
-{
-    return launch_gemm<T>(g, st);
-}
 
 template struct LUWork<double>;
 template struct LUWork<float>;
-template int lu_factor<double>(LUWork<double>&, c64*, long, hipStream_t, Profiler*);
-template int lu_factor<float>(LUWork<float>&, c32*, long, hipStream_t, Profiler*);
+template int lu_factor<double>(LUWork<double>&, c64*, long, hipStream_t, Profiler*,
+                               const LUPlan*);
+template int lu_factor<float>(LUWork<float>&, c32*, long, hipStream_t, Profiler*, const LUPlan*);
 template int lu_solve_inplace<double>(LUWork<double>&, const c64*, long, c64*, hipStream_t,
                                       Profiler*);
 template int lu_solve_inplace<float>(LUWork<float>&, const c32*, long, c32*, hipStream_t,
                                      Profiler*);
 template int lu_gather_rhs<double>(LUWork<double>&, const c64*, c64*, hipStream_t);
 template int lu_gather_rhs<float>(LUWork<float>&, const c32*, c32*, hipStream_t);
-template int gemm_public<double>(const GemmArgs<double>&, hipStream_t);
-template int gemm_public<float>(const GemmArgs<float>&, hipStream_t);
 
 }  // namespace tags

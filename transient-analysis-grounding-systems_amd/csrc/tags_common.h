@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cmath>
 #include <cstdio>
 
 #include "tags_hip.h"
@@ -45,10 +46,10 @@ __host__ __device__ inline cx<T> crecip(cx<T> a) {
 }
 template <typename T>
 __host__ __device__ inline cx<T> cdiv(cx<T> a, cx<T> b) { return cmul(a, crecip(b)); }
-template <typename T>
-__host__ __device__ inline T cabs1(cx<T> a) {
-    return (a.x < 0 ? -a.x : a.x) + (a.y < 0 ? -a.y : a.y);
-}
+// |re| + |im| with the sign bit cleared (also for -0.0, which matters where the
+// result is compared as an unsigned bit pattern)
+__host__ __device__ inline double cabs1(cx<double> a) { return fabs(a.x) + fabs(a.y); }
+__host__ __device__ inline float cabs1(cx<float> a) { return fabsf(a.x) + fabsf(a.y); }
 
 inline c64 from_c(double _Complex z) { return c64{__real__ z, __imag__ z}; }
 inline double _Complex to_c(c64 z) {
@@ -106,13 +107,21 @@ struct PairScalars {
     int intg_type;        // INTG_DOUBLE or INTG_NONE
 };
 
+// Placement of the three unknown blocks of the immittance system (row and
+// column offsets): U (n nodes), I_L and I_T (m segments each). The reference
+// order is node_off = 0, zl_off = n, zt_off = n + m.
+struct BlockLayout {
+    long n, m;
+    long node_off, zl_off, zt_off;
+};
+
 // ---- fill.hip ----
 int gl_tables_init();
 int launch_pairs_dropin(double* zl, double* zt, bool alias, SegSoA e, SegSoA img,
                         const long m, const PairScalars& sc, bool images, hipStream_t st);
-int launch_fill_zblocks(c64* W, long ldw, long n, long m, SegSoA e, SegSoA img,
+int launch_fill_zblocks(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
                         const PairScalars& sc, hipStream_t st);
-int launch_fill_static(c64* W, long ldw, long n, long m, const int* seg_start_node,
+int launch_fill_static(c64* W, long ldw, const BlockLayout& L, const int* seg_start_node,
                        const int* seg_end_node, hipStream_t st);
 int launch_single_pair(SegSoA e, SegSoA r, c64 gamma, double* out4, hipStream_t st);
 int launch_incidence(const double* seg_pts, long m, const double* nodes, long n,
