@@ -30,6 +30,7 @@ PEAK = {"fp64": 78.6, "fp32": 157.3}  # dense MFMA TFLOP/s, MI355X
 GRID = (31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)
 SIGMA, EPSR = 1.0 / 500.0, 20.0
 NFREQ = 256
+PLAN_NAMES = ["dense", "structured", "zl_first", "block_pivot"]
 
 
 def lu_flops(N):
@@ -42,41 +43,72 @@ def load_pkg():
     return tags_pkg.load()
 
 
-def cpu_baseline(T, e, nodes, N, threads, pair_sample=2000000, lu_n=4000, seed=7):
-    """The CPU restatement (oracle: adaptive Genz-Malik cubature at rel tol 1e-6
-    plus LAPACK zsysv from scipy's OpenBLAS) on a bounded sample: `pair_sample`
-    random segment pairs (direct + image integrals) scaled to all m(m+1)/2
-    pairs, and zsysv of order `lu_n` scaled by (N / lu_n)^3."""
+def cpu_baseline(T, e, nodes, N, threads, lu_n=4000, seed=7, freq=1e5):
+    """The CPU restatement of the reference (oracle) on a bounded sample of one
+    frequency sample: the full pair fill (calculate_impedances then
+    impedances_images, adaptive Genz-Malik cubature at rel tol 1e-6, OpenMP
+    over `threads`) timed whole, and LAPACK zsysv (scipy's OpenBLAS) of order
+    `lu_n` scaled by (N / lu_n)^3 to the immittance order N."""
     import tags_pkg
     O = tags_pkg.oracle()
     from scipy.linalg import lapack
     m = len(e)
-    rng = np.random.default_rng(seed)
-    i = rng.integers(0, m, pair_sample)
-    k = rng.integers(0, m, pair_sample)
-    pairs = np.stack([np.minimum(i, k), np.maximum(i, k)], 1)
-    sc = T.soil_scalars(1e5, SIGMA, EPSR)
+    sc = O.soil_scalars(freq, SIGMA, EPSR)
     img = O.images_of(e)
     t0 = time.perf_counter()
-    O.pair_sample(e, img, pairs, sc["gamma"], rel_tol=1e-6, threads=threads)
-    t_pairs = time.perf_counter() - t0
-    t_fill = t_pairs * (m * (m + 1) / 2) / pair_sample
+    rc, zl, zt = O.calculate_impedances(e, sc["gamma"], sc["s"], 1.0, sc["kappa"],
+                                        threads=threads)
+    O.impedances_images(zl, zt, e, img, sc["gamma"], sc["s"], 1.0, sc["kappa"],
+                        sc["ref_l"], sc["ref_t"], threads=threads)
+    t_fill = time.perf_counter() - t0
+    del zl, zt
+    rng = np.random.default_rng(seed)
     a = rng.standard_normal((lu_n, lu_n)) + 1j * rng.standard_normal((lu_n, lu_n))
     a = a + a.T
     b = np.ones((lu_n, 1), complex)
-    lapack.zsysv(a, b, lower=1)  # warm
+    lapack.zsysv(a[:500, :500], b[:500], lower=1)  # warm the library
     t0 = time.perf_counter()
     lapack.zsysv(a, b, lower=1)
-    t_lu = (time.perf_counter() - t0) * (N / lu_n) ** 3
+    t_lu_n = time.perf_counter() - t0
+    t_lu = t_lu_n * (N / lu_n) ** 3
     per_sample = t_fill + t_lu
     return {"value": 1.0 / per_sample, "unit": "freq samples/s", "cores": threads,
             "kind": "port",
-            "sample": ("oracle on the box host: %d random segment pairs (direct + image, "
-                       "cubature rel tol 1e-6) in %.1f s scaled to %d pairs, and LAPACK zsysv "
-                       "of order %d in %.2f s scaled by (%d/%d)^3; extrapolated %.0f s per "
-                       "sample (fill %.0f s, solve %.0f s)"
-                       % (pair_sample, t_pairs, m * (m + 1) // 2, lu_n, t_lu / (N / lu_n) ** 3,
-                          N, lu_n, per_sample, t_fill, t_lu))}
+            "sample": ("oracle on the host, %d threads, f = %g Hz: full pair fill of m = %d "
+                       "segments (%d direct + image pairs, cubature rel tol 1e-6) in %.1f s; "
+                       "LAPACK zsysv of order %d in %.2f s scaled by (%d/%d)^3 to %.0f s; "
+                       "%.0f s per sample"
+                       % (threads, freq, m, m * (m + 1) // 2, t_fill, lu_n, t_lu_n, N, lu_n,
+                          t_lu, per_sample))}
+
+
+def copy_calibration(mib):
+    """Device-to-device copy of known size: reads and writes mib MiB per launch."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_per_launch": n * 4, "launches": 5}))
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_update_f32_grid60_traffic.txt")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the trailing complex64 GEMM, from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a copy kernel), or
+    None when that summary is absent."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch"):
+                    return float(line.split("=")[1])
+    except OSError:
+        pass
+    return None
 
 
 def main():
@@ -86,14 +118,24 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--plan", type=int, default=1,
-                    help="0 dense, 1 structured (default), 2 structured with Z_L-block pivoting")
-    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--plan-mixed", type=int, default=1,
+                    help="LU plan of the mixed form: 0 dense, 1 structured (default), "
+                         "2 Z_L first, 3 Z_L-block pivoting")
+    ap.add_argument("--plan-fp64", type=int, default=2,
+                    help="LU plan of the FP64 form (default 2, Z_L first)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time every GEMM variant first and install the fastest")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB five times (the "
+                         "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; every frequency sample is "
               "independent, see the sharding paragraph of DESIGN.md", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
     T = load_pkg()
     rc, e, nodes = T.electrode_grid(*GRID)
     m, n = len(e), len(nodes)
@@ -105,13 +147,13 @@ def main():
     s = 2j * np.pi * freqs[idx]
     kap = SIGMA + s * EPSR * T.EPS0
     sw = T.Sweep(e, nodes)
-    sw.set_plan(args.plan)
     L = T.lib()
-    tuned = {}
-    if not args.no_autotune:
+    tuned = "defaults"
+    if args.autotune:
         tuned = {"fp32": T.autotune(T.PREC_MIXED), "fp64": T.autotune(T.PREC_FP64)}
 
     def run(prec, profile):
+        sw.set_plan(args.plan_mixed if prec == T.PREC_MIXED else args.plan_fp64)
         if W:
             sw.solve(s[:W], kap[:W], 0, prec)
         sw.set_profiling(profile)
@@ -142,7 +184,7 @@ def main():
                                "N=21421), rho=500, eps_r=20, 256 freqs 100 Hz-5 MHz, unit "
                                "current at corner node",
                    "samples_timed": [float(f) for f in freqs[idx]],
-                   "plan": ["dense", "structured", "block_pivot"][args.plan],
+                   "plan": PLAN_NAMES[args.plan_mixed],
                    "gemm_variants": tuned},
         "mixed": {"refine_iters": [q["refine_iters"] for q in st_mx],
                   "precision_used": [q["precision_used"] for q in st_mx],
@@ -153,6 +195,7 @@ def main():
         u64, st64, dt64, tm64 = run(T.PREC_FP64, True)
         rel = float(np.max(np.abs(u_mx - u64) / np.abs(u64)))
         out["fp64"] = {"value": K / dt64, "ms_per_step": 1e3 * dt64 / K,
+                       "plan": PLAN_NAMES[args.plan_fp64],
                        "timers_ms": {k: v for k, v in tm64.items() if k.startswith("ms_")},
                        "trailing_gemm_tflops": tm64["trailing_flops"] / tm64["trailing_ms"] / 1e9,
                        "lu_tflops_per_sample": lu_flops(N) * K / dt64 / 1e12}
@@ -160,8 +203,9 @@ def main():
         out["max_rel_err_zh_vs_fp64"] = rel
     ach = tm_mx["trailing_flops"] / tm_mx["trailing_ms"] / 1e9
     out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"],
-                       "unit": "TFLOP/s", "frac": ach / PEAK["fp32"], "traffic": None,
-                       "kernel": "k_cgemm<float> trailing update (A22 -= L21 U12, K=128)",
+                       "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
+                       "traffic": measured_traffic(),
+                       "kernel": "k_trailing_update<float, 64x64 tile, 3-multiply> (A22 -= L21 U12, K=128)",
                        "launches": tm_mx["trailing_launches"],
                        "avg_launch_ms": tm_mx["trailing_ms"] / max(1, tm_mx["trailing_launches"])}
     out["lu_algorithmic_tflops"] = lu_flops(N) * K / dt_mx / 1e12

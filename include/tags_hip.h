@@ -214,6 +214,17 @@ int tags_dense_solve(const TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t n, int preci
 int tags_immittance_solve(const TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t num_electrodes,
                           size_t num_nodes, int precision, int plan, tags_item_stats *stats);
 
+/* Panel algorithm of every LU factorisation in the process:
+ * TAGS_PANEL_TOURNAMENT (default): tournament pivoting (communication-avoiding
+ *   LU): candidate rows are reduced in a tree of local partial-pivoting
+ *   factorisations; an ordinary launch, any number of concurrent callers;
+ * TAGS_PANEL_PARTIAL: partial pivoting over the whole panel (LAPACK's pivot
+ *   order) with a cooperative launch and one grid barrier per column; needs the
+ *   device's cooperative-launch support, which one process holds at a time.
+ * Returns 0, or TAGS_ERR_ARG (-12) for another value. */
+enum tags_panel { TAGS_PANEL_TOURNAMENT = 0, TAGS_PANEL_PARTIAL = 1 };
+int tags_set_panel(int kind);
+
 /* Library version string. */
 const char *tags_version(void);
 

@@ -98,3 +98,12 @@ def test_images_of():
     im = T.images_of(e)
     assert np.all(im["start_point"][:, 2] == 0.5) and np.all(im["length"] == e["length"])
     assert np.array_equal(im.tobytes(), O.images_of(e).tobytes())
+
+
+def test_c_driver_compiles_and_links(tmp_path):
+    """A plain-C11 driver written against include/tags_hip.h, making the
+    reference caller's calls, compiles without warnings and links against
+    libtags_hip.so (every symbol it uses resolves)."""
+    out = str(tmp_path / "harmonic_c1")
+    tags_pkg.build_c_driver(out)
+    assert os.path.getsize(out) > 0

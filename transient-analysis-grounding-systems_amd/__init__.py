@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libtags_hip.so")
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "api.hip"]
+SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "api.hip"]
 
 ELECTRODE_DTYPE = np.dtype(
     [("start_point", "<f8", 3), ("end_point", "<f8", 3), ("middle_point", "<f8", 3),
@@ -146,6 +146,7 @@ def lib():
         "tags_gemm_bench": ([I, I, I, I, I, I, ctypes.POINTER(D)], I),
         "tags_gemm_variants": ([I], I),
         "tags_gemm_select": ([I, I, I], I),
+        "tags_set_panel": ([I], I),
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
         "tags_sweep_assemble": ([P, C, C, D, P], I),
@@ -277,6 +278,16 @@ def gemm_bench(M, N, K, precision, variant, iters=5):
     if rc:
         raise RuntimeError("tags_gemm_bench failed: %d" % rc)
     return ms.value
+
+
+PANEL_TOURNAMENT, PANEL_PARTIAL = 0, 1
+
+
+def set_panel(kind):
+    """Panel algorithm of every LU in the process (PANEL_TOURNAMENT default)."""
+    rc = lib().tags_set_panel(kind)
+    if rc:
+        raise RuntimeError("tags_set_panel failed: %d" % rc)
 
 
 def autotune(precision):

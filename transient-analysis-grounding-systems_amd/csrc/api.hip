@@ -6,6 +6,7 @@
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <vector>
 
 #include "lu.h"
@@ -21,6 +22,8 @@ struct DevInfo {
 static DevInfo& dev_info()
 {
     static DevInfo d;
+    static std::mutex mu;
+    std::lock_guard<std::mutex> lock(mu);
     if (!d.ok) {
         int dev = 0;
         hipGetDevice(&dev);
@@ -368,6 +371,13 @@ int tags_gemm_bench(int M, int N, int K, int precision, int variant, int iters, 
 }
 
 int tags_gemm_variants(int precision) { return gemm_variant_count(precision); }
+
+int tags_set_panel(int kind)
+{
+    if (kind != TAGS_PANEL_TOURNAMENT && kind != TAGS_PANEL_PARTIAL) return TAGS_ERR_ARG;
+    lu_set_panel_kind(kind);
+    return 0;
+}
 
 int tags_gemm_select(int precision, int big_variant, int small_variant)
 {

@@ -18,6 +18,7 @@
 // The diagonal of the direct part is the reference's gamma-independent closed
 // form (src/electrode.c:238-245, used at :346-348).
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "tags_common.h"
@@ -31,7 +32,9 @@ __constant__ double g_glx[128];
 __constant__ double g_glw[128];
 __constant__ int g_gloff[kNumOrders];
 
+// set once per process under g_gl_mutex (callers may be concurrent threads)
 static bool g_gl_ready = false;
+static std::mutex g_gl_mutex;
 
 // Legendre nodes by Newton iteration on P_n from the Chebyshev-like guess.
 static void gauss_legendre(int n, double* x, double* w)
@@ -59,6 +62,7 @@ static void gauss_legendre(int n, double* x, double* w)
 
 int gl_tables_init()
 {
+    std::lock_guard<std::mutex> lock(g_gl_mutex);
     if (g_gl_ready) return 0;
     std::vector<double> x(128, 0.0), w(128, 0.0);
     int off[kNumOrders];

@@ -11,6 +11,7 @@
 // run. Complex products use four real MFMAs (re.re, im.im, re.im, im.re) or,
 // with M3, three (Gauss: re.re, im.im, (re+im).(re+im)).
 #include <algorithm>
+#include <atomic>
 
 #include "lu.h"
 
@@ -41,7 +42,7 @@ struct Mfma<float> {
 };
 
 template <typename T, int BM, int BN, int BK, bool M3>
-__global__ __launch_bounds__(256) void k_cgemm(GemmArgs<T> g)
+__device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
 {
     using V4 = typename Mfma<T>::v4;
     constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
@@ -205,15 +206,35 @@ __global__ __launch_bounds__(256) void k_cgemm(GemmArgs<T> g)
         }
 }
 
+template <typename T, int BM, int BN, int BK, bool M3>
+__global__ __launch_bounds__(256) void k_cgemm(GemmArgs<T> g)
+{
+    cgemm_block<T, BM, BN, BK, M3>(g);
+}
+
+// The same product under its own name for the LU's trailing update
+// A22 -= L21 U12, so that kernel traces and counters attribute it apart from
+// the other products (panel rows, triangular solves).
+template <typename T, int BM, int BN, int BK, bool M3>
+__global__ __launch_bounds__(256) void k_trailing_update(GemmArgs<T> g)
+{
+    cgemm_block<T, BM, BN, BK, M3>(g);
+}
+
 // ---------------------------------------------------------------------------
 // variants and selection
 // ---------------------------------------------------------------------------
 template <typename T, int BM, int BN, int BK, bool M3>
-static int launch_v(const GemmArgs<T>& g, hipStream_t st)
+static int launch_v(const GemmArgs<T>& g, hipStream_t st, bool trailing)
 {
     const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
     const long grid = (tiles + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3>), dim3((unsigned)grid), dim3(256), 0, st, g);
+    if (trailing)
+        hipLaunchKernelGGL((k_trailing_update<T, BM, BN, BK, M3>), dim3((unsigned)grid),
+                           dim3(256), 0, st, g);
+    else
+        hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3>), dim3((unsigned)grid), dim3(256), 0,
+                           st, g);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
@@ -224,15 +245,15 @@ struct GemmTable;
 template <>
 struct GemmTable<float> {
     static constexpr int count = 6;
-    static int launch(int v, const GemmArgs<float>& g, hipStream_t st)
+    static int launch(int v, const GemmArgs<float>& g, hipStream_t st, bool tr)
     {
         switch (v) {
-        case 0: return launch_v<float, 64, 64, 16, false>(g, st);
-        case 1: return launch_v<float, 128, 128, 16, false>(g, st);
-        case 2: return launch_v<float, 128, 64, 16, false>(g, st);
-        case 3: return launch_v<float, 64, 128, 16, false>(g, st);
-        case 4: return launch_v<float, 128, 128, 16, true>(g, st);
-        default: return launch_v<float, 64, 64, 16, true>(g, st);
+        case 0: return launch_v<float, 64, 64, 16, false>(g, st, tr);
+        case 1: return launch_v<float, 128, 128, 16, false>(g, st, tr);
+        case 2: return launch_v<float, 128, 64, 16, false>(g, st, tr);
+        case 3: return launch_v<float, 64, 128, 16, false>(g, st, tr);
+        case 4: return launch_v<float, 128, 128, 16, true>(g, st, tr);
+        default: return launch_v<float, 64, 64, 16, true>(g, st, tr);
         }
     }
     static int tile_m(int v) { return (v == 1 || v == 2 || v == 4) ? 128 : 64; }
@@ -241,22 +262,24 @@ struct GemmTable<float> {
 template <>
 struct GemmTable<double> {
     static constexpr int count = 4;
-    static int launch(int v, const GemmArgs<double>& g, hipStream_t st)
+    static int launch(int v, const GemmArgs<double>& g, hipStream_t st, bool tr)
     {
         switch (v) {
-        case 0: return launch_v<double, 64, 64, 8, false>(g, st);
-        case 1: return launch_v<double, 128, 64, 8, false>(g, st);
-        case 2: return launch_v<double, 64, 128, 8, false>(g, st);
-        default: return launch_v<double, 128, 64, 16, false>(g, st);
+        case 0: return launch_v<double, 64, 64, 8, false>(g, st, tr);
+        case 1: return launch_v<double, 128, 64, 8, false>(g, st, tr);
+        case 2: return launch_v<double, 64, 128, 8, false>(g, st, tr);
+        default: return launch_v<double, 128, 64, 16, false>(g, st, tr);
         }
     }
     static int tile_m(int v) { return (v == 1 || v == 3) ? 128 : 64; }
     static int tile_n(int v) { return v == 2 ? 128 : 64; }
 };
 
-// Selected variant for large and small problems (set by the autotuner).
-static int g_big_variant[2] = {1, 4};    // [double, float]
-static int g_small_variant[2] = {0, 5};
+// Selected variant for large and small problems, [double, float]. The
+// defaults are the autotuner's choice on MI355X (64 x 64 tiles; three-multiply
+// for complex64); tags_autotune may replace them.
+static std::atomic<int> g_big_variant[2] = {0, 5};
+static std::atomic<int> g_small_variant[2] = {0, 5};
 
 template <typename T>
 static constexpr int prec_index()
@@ -265,15 +288,15 @@ static constexpr int prec_index()
 }
 
 template <typename T>
-int launch_gemm(const GemmArgs<T>& g, hipStream_t st)
+int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing)
 {
     if (g.M <= 0 || g.N <= 0) return 0;
     const int p = prec_index<T>();
-    const int big = g_big_variant[p];
+    const int big = g_big_variant[p].load(std::memory_order_relaxed);
     const long tiles_big = (long)((g.M + GemmTable<T>::tile_m(big) - 1) / GemmTable<T>::tile_m(big)) *
                            ((g.N + GemmTable<T>::tile_n(big) - 1) / GemmTable<T>::tile_n(big));
-    const int v = tiles_big >= 512 ? big : g_small_variant[p];
-    return GemmTable<T>::launch(v, g, st);
+    const int v = tiles_big >= 512 ? big : g_small_variant[p].load(std::memory_order_relaxed);
+    return GemmTable<T>::launch(v, g, st, trailing);
 }
 
 template <typename T>
@@ -281,7 +304,7 @@ int launch_gemm_variant(const GemmArgs<T>& g, int variant, hipStream_t st)
 {
     if (g.M <= 0 || g.N <= 0) return 0;
     if (variant < 0 || variant >= GemmTable<T>::count) return TAGS_ERR_ARG;
-    return GemmTable<T>::launch(variant, g, st);
+    return GemmTable<T>::launch(variant, g, st, false);
 }
 
 int gemm_set_variants(int precision, int big, int small)
@@ -299,8 +322,8 @@ int gemm_variant_count(int precision)
     return precision == 1 ? GemmTable<float>::count : GemmTable<double>::count;
 }
 
-template int launch_gemm<double>(const GemmArgs<double>&, hipStream_t);
-template int launch_gemm<float>(const GemmArgs<float>&, hipStream_t);
+template int launch_gemm<double>(const GemmArgs<double>&, hipStream_t, bool);
+template int launch_gemm<float>(const GemmArgs<float>&, hipStream_t, bool);
 template int launch_gemm_variant<double>(const GemmArgs<double>&, int, hipStream_t);
 template int launch_gemm_variant<float>(const GemmArgs<float>&, int, hipStream_t);
 

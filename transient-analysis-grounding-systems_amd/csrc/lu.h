@@ -134,6 +134,29 @@ struct PanelArgs {
     int* err;
 };
 
+// Reduction tree of the tournament-pivoting panel (tslu.hip): level 0 holds
+// the leaves (one per leaf_rows candidate rows, leaf_rows / 64 children per
+// node), the last level the root.
+constexpr int TSLU_MAX_LEVELS = 12;
+struct TsluTree {
+    int nlev;
+    int nodes[TSLU_MAX_LEVELS];
+    int cand_off[TSLU_MAX_LEVELS];  // offset of each level's nominee lists (ints)
+    int cnt_off[TSLU_MAX_LEVELS];   // offset of each level's arrival counters
+    int cand_total, cnt_total;
+};
+int tslu_tree(int cand_rows, int leaf_rows, TsluTree* tr);
+int tslu_min_leaf_rows();  // smallest leaf of any precision (largest tree)
+template <typename T>
+int launch_tslu(cx<T>* A, long ld, int* perm, int c0, int w, int row_end, int cand_end,
+                int* cand, unsigned* cnt, int* info, hipStream_t st);
+
+// Panel algorithm: tournament pivoting (default) or partial pivoting over the
+// whole panel with a cooperative launch.
+enum PanelKind { PANEL_TOURNAMENT = 0, PANEL_PARTIAL = 1 };
+void lu_set_panel_kind(int kind);
+int lu_panel_kind();
+
 // Column phases of a factorisation (see lu.hip).
 struct LUPhase {
     int c_begin, c_end;   // columns factored in this phase
@@ -209,6 +232,8 @@ struct LUWork {
     int* flags = nullptr;   // [0] info, [1] barrier error
     int* d_blk = nullptr;   // device copy of blk
     int* d_init = nullptr;  // device copy of the plan's initial row order
+    int* tcand = nullptr;       // tournament nominee lists
+    unsigned* tcnt = nullptr;   // tournament arrival counters (kept at zero)
     std::vector<int> blk;   // start column of every diagonal block, in order
     std::vector<int> h_blk, h_init;  // what d_blk / d_init currently hold
     double flops = 0;       // real flops of the last factorisation (GEMM + panel)
@@ -227,8 +252,9 @@ int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_
                      Profiler* pr);
 template <typename T>
 int lu_gather_rhs(LUWork<T>& w, const cx<T>* b, cx<T>* y, hipStream_t st);
+// trailing selects the kernel name k_trailing_update (same code as k_cgemm)
 template <typename T>
-int launch_gemm(const GemmArgs<T>& g, hipStream_t st);
+int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing = false);
 template <typename T>
 int launch_gemm_variant(const GemmArgs<T>& g, int variant, hipStream_t st);
 int gemm_set_variants(int precision, int big, int small);

@@ -24,7 +24,9 @@
 //   trailing update     A22 -= L21 U12, K = 128                     (MFMA GEMM)
 // and after the last step the upper inverses of all diagonal blocks.
 #include <algorithm>
+#include <atomic>
 #include <cmath>
+#include <mutex>
 #include <vector>
 
 #include "lu.h"
@@ -234,15 +236,23 @@ __device__ inline cx<T> quad_sum(cx<T> s)
     return s;
 }
 
+// Row map of the block staged in LDS first, then all 16 loads of a thread
+// issued together (a loop of dependent map -> row loads would pay the memory
+// latency once per iteration).
 template <typename T>
 __device__ inline void load_diag_block(const cx<T>* A, long ld, const int* perm, int c0, int w,
                                        cx<T> (*B)[64])
 {
-    for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    __shared__ int s_pm[64];
+    if (threadIdx.x < 64) s_pm[threadIdx.x] = threadIdx.x < w ? perm[c0 + threadIdx.x] : 0;
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+        const int e = threadIdx.x + it * 256;
         const int r = e >> 6, c = e & 63;
         cx<T> v = cx<T>{0, 0};
         if (r < w && c < w)
-            v = A[(long)perm[c0 + r] * ld + c0 + c];
+            v = A[(long)s_pm[r] * ld + c0 + c];
         else if (r == c)
             v = cx<T>{1, 0};
         B[r][c] = v;
@@ -534,6 +544,12 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit)
     TAGS_HIP(hipMalloc(&d_init, sizeof(int) * n));
     TAGS_HIP(hipMemset(bar, 0, sizeof(unsigned) * 64));
     TAGS_HIP(hipMemset(flags, 0, sizeof(int) * 4));
+    TsluTree tr;
+    int rc = tslu_tree(n, tslu_min_leaf_rows(), &tr);
+    if (rc) return rc;
+    TAGS_HIP(hipMalloc(&tcand, sizeof(int) * (size_t)tr.cand_total));
+    TAGS_HIP(hipMalloc(&tcnt, sizeof(unsigned) * (size_t)std::max(1, tr.cnt_total)));
+    TAGS_HIP(hipMemset(tcnt, 0, sizeof(unsigned) * (size_t)std::max(1, tr.cnt_total)));
     return 0;
 }
 
@@ -550,6 +566,10 @@ void LUWork<T>::release()
     (void)hipFree(vec);
     (void)hipFree(d_blk);
     (void)hipFree(d_init);
+    (void)hipFree(tcand);
+    (void)hipFree(tcnt);
+    tcand = nullptr;
+    tcnt = nullptr;
     d_init = nullptr;
     h_init.clear();
     perm = nullptr;
@@ -570,7 +590,7 @@ static int gemm(Profiler* pr, hipStream_t st, int M, int N, int K, const cx<T>* 
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     GemmArgs<T> g{M, N, K, A, lda, arow, B, ldb, brow, C, ldc, crow, mode};
     Timed tm(pr, FAM_GEMM, st);
-    int rc = launch_gemm<T>(g, st);
+    int rc = launch_gemm<T>(g, st, trailing);
     const double f = 8.0 * M * N * (double)K;
     tm.end(f, trailing);
     if (flops) *flops += f;
@@ -585,11 +605,24 @@ static int panel_rows_max(size_t lds_limit)
     return (int)((lds_limit - 4096) / per_row);
 }
 
+static std::atomic<int> g_panel_kind{PANEL_TOURNAMENT};
+
+void lu_set_panel_kind(int kind) { g_panel_kind.store(kind == PANEL_PARTIAL ? 1 : 0); }
+int lu_panel_kind() { return g_panel_kind.load(); }
+
 template <typename T>
 static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int c0,
                  int cols, int row_end, int cand_end)
 {
     const int M = row_end - c0;
+    if (g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT) {
+        Timed tm(pr, FAM_PANEL, st);
+        const int rc = launch_tslu<T>(A, ld, w.perm, c0, cols, row_end, cand_end, w.tcand,
+                                      w.tcnt, w.flags, st);
+        tm.end();
+        w.flops += 8.0 * (double)M * cols * cols / 2;
+        return rc;
+    }
     const size_t per_row = 64 * sizeof(cx<T>) + 2 * sizeof(int);
     const int rmax = panel_rows_max<T>(w.lds_limit);
     // few work-groups keep the grid barrier cheap; ~160 rows each
@@ -619,16 +652,22 @@ static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, 
 template <typename T>
 int LUWork<T>::setup_kernel_attrs()
 {
-    static bool done = false;
-    if (done) return 0;
-    // raise the dynamic-LDS cap of the panel kernel; keep the default cap
-    // (and a smaller panel chunk) when the runtime refuses
-    if (hipFuncSetAttribute((const void*)k_panel<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                            (int)lds_limit) != hipSuccess) {
-        (void)hipGetLastError();
-        lds_limit = 65536;
+    // raise the dynamic-LDS cap of the panel kernel once per process; keep
+    // the default cap (and a smaller panel chunk) when the runtime refuses
+    static std::mutex mu;
+    static int state = 0;  // 0 not tried, 1 raised, 2 refused
+    std::lock_guard<std::mutex> lock(mu);
+    if (state == 0) {
+        if (hipFuncSetAttribute((const void*)k_panel<T>,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                (int)lds_limit) != hipSuccess) {
+            (void)hipGetLastError();
+            state = 2;
+        } else {
+            state = 1;
+        }
     }
-    done = true;
+    if (state == 2) lds_limit = 65536;
     return 0;
 }
 

@@ -46,6 +46,13 @@ __host__ __device__ inline cx<T> crecip(cx<T> a) {
 }
 template <typename T>
 __host__ __device__ inline cx<T> cdiv(cx<T> a, cx<T> b) { return cmul(a, crecip(b)); }
+// c ? a : b, component-wise (a select of the struct itself would be lowered to
+// a copy through a selected pointer, which keeps register arrays in memory)
+template <typename T>
+__host__ __device__ inline cx<T> csel(bool c, cx<T> a, cx<T> b)
+{
+    return cx<T>{c ? a.x : b.x, c ? a.y : b.y};
+}
 // |re| + |im| with the sign bit cleared (also for -0.0, which matters where the
 // result is compared as an unsigned bit pattern)
 __host__ __device__ inline double cabs1(cx<double> a) { return fabs(a.x) + fabs(a.y); }

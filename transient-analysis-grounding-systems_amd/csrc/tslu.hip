@@ -1,0 +1,552 @@
+// Panel factorisation with tournament pivoting (communication-avoiding LU,
+// "CALU"), used in place of column-by-column partial pivoting across the
+// whole panel.
+//
+// A panel is w <= 64 columns starting at column c0. Its candidate pivot rows
+// are the logical rows [c0, cand_end); every row in [c0, row_end) receives its
+// L entries.
+//
+//  k_tslu       one ordinary launch, one work-group per leaf of S candidate
+//               rows (256 for complex64, 128 for complex128). Each leaf runs
+//               partial pivoting on its rows (in registers) and nominates up
+//               to 64 rows, in pivot order. Tree nodes merge F = S / 64
+//               children: the LAST child work-group to
+//               finish (atomic arrival counter, acquire/release at agent
+//               scope) stacks the children's nominees (their current, i.e.
+//               unfactored, rows), runs partial pivoting on that stack and
+//               nominates again; the others exit. No work-group ever waits for
+//               another, so the launch needs no co-residency. The root's
+//               factored stack holds L11 \ U11 of the winners in pivot order;
+//               it writes them to the matrix, moves the winners to logical
+//               rows c0 .. c0+w-1 (swaps of the row map only) and resets the
+//               counters.
+//  k_l21        L21 = A21 inv(U11) for logical rows [c0+w, row_end), by
+//               substitution against U11 held in LDS.
+//
+// Register layout of a work-group's stack (S rows x 64 columns): thread t
+// holds rows {i * 64 + t/8 : i < S/64} x columns {8 (t%8) + q : q < 8}. The
+// eight threads of one row sit in adjacent lanes of one wave, so a multiplier
+// is broadcast with one lane shuffle; a pivot row is broadcast through LDS.
+// Pivot choice per column: largest |re| + |im| (as LAPACK's izamax), ties to
+// the lower stack index; the sign bit is cleared so -0.0 never wins.
+#include "lu.h"
+
+namespace tags {
+
+constexpr int TS_NT = 512;         // threads per work-group
+constexpr int TS_NW = TS_NT / 64;  // waves; wave w owns panel columns 8w .. 8w+7
+
+// Rows per lane: 4 for complex64 (S = 256 rows per work-group, 4 children per
+// tree node), 2 for complex128 (S = 128, 2 children).
+template <typename T>
+struct TsShape {
+    static constexpr int RPT = sizeof(T) == 8 ? 2 : 4;
+    static constexpr int S = 64 * RPT;
+    static constexpr int F = S / 64;
+};
+
+template <typename T>
+struct TsluArgs {
+    cx<T>* A;
+    long ld;
+    int* perm;
+    int c0, w, cand_end;
+    int* cand;      // nominee lists, 64 logical rows per node (-1: none)
+    unsigned* cnt;  // arrival counters of the tree nodes (zero between launches)
+    int* info;
+    TsluTree tree;
+};
+
+// Largest 64-bit key of the wave (DPP row operations, then lane 63); uniform.
+__device__ inline unsigned long long wave_max_u64(unsigned long long v)
+{
+#define TS_DPP_MAX(CTRL, RM)                                                                \
+    {                                                                                        \
+        const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);                     \
+        const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, RM, 0xF, false); \
+        const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, RM, 0xF, false); \
+        const unsigned long long o = ((unsigned long long)ohi << 32) | olo;                  \
+        v = o > v ? o : v;                                                                   \
+    }
+    TS_DPP_MAX(0xB1, 0xF)   // quad_perm [1, 0, 3, 2]
+    TS_DPP_MAX(0x4E, 0xF)   // quad_perm [2, 3, 0, 1]
+    TS_DPP_MAX(0x124, 0xF)  // row_ror 4
+    TS_DPP_MAX(0x128, 0xF)  // row_ror 8: every lane holds its 16-lane row's maximum
+    TS_DPP_MAX(0x142, 0xA)  // row_bcast 15 into rows 1 and 3
+    TS_DPP_MAX(0x143, 0xC)  // row_bcast 31 into rows 2 and 3: lane 63 holds the maximum
+#undef TS_DPP_MAX
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
+    return ((unsigned long long)hi << 32) | lo;
+}
+
+__device__ inline cx<float> readlane_cx(cx<float> v, int l)
+{
+    return cx<float>{__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), l)),
+                     __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.y), l))};
+}
+__device__ inline double readlane_d(double v, int l)
+{
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ inline cx<double> readlane_cx(cx<double> v, int l)
+{
+    return cx<double>{readlane_d(v.x, l), readlane_d(v.y, l)};
+}
+
+// Row i * 64 + lane of the stack, register row `ri` selected at run time.
+template <typename T, int RPT>
+__device__ __forceinline__ cx<T> pick_row(const cx<T> (&a)[RPT][8], int ri, int q)
+{
+    cx<T> v = a[0][q];
+#pragma unroll
+    for (int i = 1; i < RPT; i++) v = csel(i == ri, a[i][q], v);
+    return v;
+}
+
+// Column K of the block held by this wave (partial pivoting within the wave:
+// DPP maximum, pivot row by lane reads, rank-1 update of the block's columns
+// right of K). Returns false when no live row is left.
+template <typename T, int K>
+__device__ __forceinline__ bool cw_step(cx<T> (&a)[TsShape<T>::RPT][8], unsigned& live, int kn,
+                                        int j, int (&pvk)[8], int& zc)
+{
+    constexpr int RPT = TsShape<T>::RPT;
+    if (K >= kn) return false;
+    const int lane = threadIdx.x & 63;
+    unsigned long long key = 0;
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        const double v = (double)cabs1(a[i][K]);
+        unsigned long long kk =
+            ((unsigned long long)__double_as_longlong(v) & 0x7FFFFFFFFFFFFE00ull) |
+            (unsigned long long)(0x1FF - (i * 64 + lane));
+        kk = ((live >> i) & 1) ? kk : 0ull;
+        key = kk > key ? kk : key;
+    }
+    key = wave_max_u64(key);
+    if (key == 0) return false;
+    const int li = 0x1FF - (int)(key & 0x1FF);
+    const int wr = li & 63, wi = li >> 6;
+    pvk[K] = li;
+    cx<T> u[8];
+#pragma unroll
+    for (int q = K; q < 8; q++) u[q] = readlane_cx(pick_row<T, RPT>(a, wi, q), wr);
+    if (lane == wr) live &= ~(1u << wi);
+    if ((key >> 9) != 0) {
+        const cx<T> rinv = crecip(u[K]);
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const bool lv = (live >> i) & 1;
+            const cx<T> l = csel(lv, cmul(a[i][K], rinv), cx<T>{0, 0});
+            a[i][K] = csel(lv, l, a[i][K]);
+#pragma unroll
+            for (int q = K + 1; q < 8; q++) a[i][q] = csub(a[i][q], cmul(l, u[q]));
+        }
+    } else if (zc < 0) {
+        zc = j;
+    }
+    return true;
+}
+
+template <typename T>
+struct TsLds {
+    cx<T> L[2][8][TsShape<T>::S];  // multipliers of the last block, by parity
+                                   // (column-major: lanes read consecutive rows)
+    cx<T> P[TS_NW][8][8];          // per wave: its columns of the block's pivot rows
+    cx<T> U[TS_NW][8][8];          // per wave: its columns of U12 for the block
+    int pv[2][8];                  // pivot stack rows of the block
+    int nb[2];                     // pivots found in the block
+};
+
+__device__ inline void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Partial pivoting on the register stack (S rows x 64 columns; lane l holds
+// rows {i * 64 + l}, wave w columns 8w .. 8w+7); s_lrow[li] is the logical row
+// of stack row li (-1 empty). Blocked by the 8 columns of a wave: the owning
+// wave factors its block alone, publishes the multipliers and pivots, and
+// after one barrier every wave to the right forms its part of the pivot rows
+// (U12 = inv(L11) A12, 8 x 8 per wave) and applies the rank-8 update; the next
+// block's wave does this first and goes on while the others update. Pivots
+// (stack rows) go to s_piv in order; returns their number. Columns whose pivot
+// is exactly zero are left unscaled (as LAPACK), the first is reported in
+// *zero_col.
+template <typename T>
+__device__ __forceinline__ int tslu_gepp(cx<T> (&a)[TsShape<T>::RPT][8], const int* s_lrow,
+                                         int w, int* s_piv, TsLds<T>& sh, int* zero_col)
+{
+    constexpr int RPT = TsShape<T>::RPT, S = TsShape<T>::S;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    unsigned live = 0;
+#pragma unroll
+    for (int i = 0; i < RPT; i++)
+        if (s_lrow[i * 64 + lane] >= 0) live |= 1u << i;
+    int npiv = 0;
+    const int nblk = (w + 7) / 8;
+    for (int jb = 0; jb < nblk; jb++) {
+        const int par = jb & 1;
+        if (wave == jb) {
+            const int kn = min(8, w - jb * 8);
+            const unsigned live0 = live;
+            int pvk[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+            int zc = -1, nb = 0;
+            bool go = true;
+#define TS_STEP(K)                                                      \
+    if (go) {                                                           \
+        go = cw_step<T, K>(a, live, kn, jb * 8 + K, pvk, zc);           \
+        nb += go ? 1 : 0;                                               \
+    }
+            TS_STEP(0) TS_STEP(1) TS_STEP(2) TS_STEP(3)
+            TS_STEP(4) TS_STEP(5) TS_STEP(6) TS_STEP(7)
+#undef TS_STEP
+            // publish multipliers: rows live at the block start, a pivot row
+            // only its multipliers from before it was chosen
+#pragma unroll
+            for (int i = 0; i < RPT; i++) {
+                const int li = i * 64 + lane;
+                int rank = 8;
+#pragma unroll
+                for (int t = 0; t < 8; t++) rank = (t < nb && pvk[t] == li) ? t : rank;
+                const bool lv0 = (live0 >> i) & 1;
+#pragma unroll
+                for (int q = 0; q < 8; q++)
+                    sh.L[par][q][li] = csel(lv0 && q < rank && q < kn, a[i][q], cx<T>{0, 0});
+            }
+            if (lane == 0) {
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    sh.pv[par][t] = pvk[t];
+                    if (t < nb) s_piv[jb * 8 + t] = pvk[t];
+                }
+                sh.nb[par] = nb;
+                if (zero_col && zc >= 0 && *zero_col < 0) *zero_col = zc;
+            }
+        }
+        __syncthreads();
+        const int nb = sh.nb[par];
+        npiv += nb;
+        if (wave != jb) {
+            for (int t = 0; t < nb; t++) {
+                const int pr = sh.pv[par][t];
+                if (lane == (pr & 63)) live &= ~(1u << (pr >> 6));
+            }
+        }
+        if (wave > jb && nb > 0) {
+            // this wave's columns of the block's pivot rows
+            for (int t = 0; t < nb; t++) {
+                const int pr = sh.pv[par][t];
+                if (lane == (pr & 63)) {
+#pragma unroll
+                    for (int q = 0; q < 8; q++) sh.P[wave][t][q] = pick_row<T, RPT>(a, pr >> 6, q);
+                }
+            }
+            wave_lds_fence();
+            if (lane < 8) {  // U12 column `lane`: forward substitution with L11
+                cx<T> u[8];
+#pragma unroll
+                for (int t = 0; t < 8; t++) {
+                    if (t < nb) {
+                        const int pr = sh.pv[par][t];
+                        cx<T> v = sh.P[wave][t][lane];
+#pragma unroll
+                        for (int s2 = 0; s2 < t; s2++) v = csub(v, cmul(sh.L[par][s2][pr], u[s2]));
+                        u[t] = v;
+                        sh.U[wave][t][lane] = v;
+                    }
+                }
+            }
+            wave_lds_fence();
+            // rank-nb update; pivot rows then take their U12 entries
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                if (t < nb) {
+                    cx<T> l[RPT];
+#pragma unroll
+                    for (int i = 0; i < RPT; i++) l[i] = sh.L[par][t][i * 64 + lane];
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const cx<T> u = sh.U[wave][t][q];
+#pragma unroll
+                        for (int i = 0; i < RPT; i++) a[i][q] = csub(a[i][q], cmul(l[i], u));
+                    }
+                }
+            }
+            for (int t = 0; t < nb; t++) {
+                const int pr = sh.pv[par][t];
+                if (lane == (pr & 63)) {
+                    const int ri = pr >> 6;
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const cx<T> u = sh.U[wave][t][q];
+#pragma unroll
+                        for (int i = 0; i < RPT; i++) a[i][q] = csel(i == ri, u, a[i][q]);
+                    }
+                }
+            }
+        }
+        if (nb < min(8, w - jb * 8)) break;  // no live row left
+    }
+    __syncthreads();
+    (void)S;
+    return npiv;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TS_NT) void k_tslu(TsluArgs<T> p)
+{
+    constexpr int TS_RPT = TsShape<T>::RPT, TS_S = TsShape<T>::S, TS_F = TsShape<T>::F;
+    __shared__ int s_lrow[TS_S];
+    __shared__ int s_piv[64];
+    __shared__ int s_rank[TS_S];
+    __shared__ TsLds<T> sh;
+    __shared__ int s_tflag[64], s_vac[64];
+    __shared__ int s_last, s_zero;
+
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const TsluTree& tr = p.tree;
+    int level = 0, node = blockIdx.x;
+    // leaf: consecutive candidate rows
+    {
+        const int base = p.c0 + node * TS_S;
+        for (int li = t; li < TS_S; li += TS_NT)
+            s_lrow[li] = base + li < p.cand_end ? base + li : -1;
+    }
+    if (t == 0) s_zero = -1;
+    __syncthreads();
+    cx<T> a[TS_RPT][8];
+    for (;;) {
+        // load the stack: current values of the rows in s_lrow
+#pragma unroll
+        for (int i = 0; i < TS_RPT; i++) {
+            const int lr = s_lrow[i * 64 + lane];
+            const cx<T>* src =
+                lr >= 0 ? p.A + (long)p.perm[lr] * p.ld + p.c0 + wave * 8 : nullptr;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                a[i][q] = (src && wave * 8 + q < p.w) ? src[q] : cx<T>{0, 0};
+        }
+        const bool root = level == tr.nlev - 1;
+        const int npiv = tslu_gepp<T>(a, s_lrow, p.w, s_piv, sh, root ? &s_zero : nullptr);
+        if (root) {
+            // winners' factored rows L11 \ U11 into the matrix (physical rows as
+            // of the start of the panel), then the row map
+            for (int li = t; li < TS_S; li += TS_NT) s_rank[li] = -1;
+            __syncthreads();
+            if (t < npiv) s_rank[s_piv[t]] = t;
+            __syncthreads();
+#pragma unroll
+            for (int i = 0; i < TS_RPT; i++) {
+                const int li = i * 64 + lane;
+                if (s_rank[li] >= 0) {
+                    cx<T>* dst = p.A + (long)p.perm[s_lrow[li]] * p.ld + p.c0 + wave * 8;
+#pragma unroll
+                    for (int q = 0; q < 8; q++)
+                        if (wave * 8 + q < p.w) dst[q] = a[i][q];
+                }
+            }
+            // Row map: logical row c0 + j <- winner j. Target rows that are not
+            // winners take, in order, the positions vacated by winners from
+            // outside the target range (all reads before any write: one wave).
+            if (t < 64) s_tflag[t] = 0;
+            __syncthreads();
+            const int wpos = t < npiv ? s_lrow[s_piv[t]] : -1;
+            if (t < npiv && wpos < p.c0 + npiv) s_tflag[wpos - p.c0] = 1;
+            __syncthreads();
+            bool disp = false;
+            int drank = 0;
+            if (t < 64) {  // wave 0
+                const unsigned long long below = (1ull << t) - 1ull;
+                const bool vac = t < npiv && wpos >= p.c0 + npiv;
+                const unsigned long long vmask = __ballot(vac);
+                if (vac) s_vac[__popcll(vmask & below)] = wpos;
+                disp = t < npiv && s_tflag[t] == 0;
+                const unsigned long long dmask = __ballot(disp);
+                drank = __popcll(dmask & below);
+            }
+            __syncthreads();
+            if (t < 64) {
+                const int old_t = t < npiv ? p.perm[p.c0 + t] : 0;
+                const int old_w = t < npiv ? p.perm[wpos] : 0;
+                const int dst = disp ? s_vac[drank] : 0;
+                if (t < npiv) p.perm[p.c0 + t] = old_w;
+                if (disp) p.perm[dst] = old_t;
+                if (t == 0) {
+                    int zc = s_zero;
+                    if (zc < 0 && npiv < p.w) zc = npiv;
+                    if (zc >= 0 && *p.info == 0) *p.info = p.c0 + zc + 1;
+                }
+            }
+            // counters back to zero for the next launch (every arrival is done)
+            for (int l = 1; l < tr.nlev; l++)
+                for (int q = t; q < tr.nodes[l]; q += TS_NT) p.cnt[tr.cnt_off[l] + q] = 0u;
+            return;
+        }
+        // nominate: wave 0 writes the list, then its thread 0 arrives at the
+        // parent with one acquire-release atomic (the release orders wave 0's
+        // stores before the arrival; the acquire of the last arrival makes the
+        // siblings' lists visible; the barrier passes it to the other waves).
+        if (t < 64) {
+            int* out = p.cand + tr.cand_off[level] + (long)node * 64;
+            __hip_atomic_store(out + t, t < npiv ? s_lrow[s_piv[t]] : -1, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            if (t == 0) {
+                const int parent = node / TS_F;
+                const int nch = min(TS_F, tr.nodes[level] - parent * TS_F);
+                const unsigned old =
+                    __hip_atomic_fetch_add(&p.cnt[tr.cnt_off[level + 1] + parent], 1u,
+                                           __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+                s_last = (int)old == nch - 1;
+            }
+        }
+        __syncthreads();
+        if (!s_last) return;
+        // last child in: stack the nominees of all children of the parent
+        level++;
+        node /= TS_F;
+        {
+            const int first = node * TS_F;
+            const int nch = min(TS_F, tr.nodes[level - 1] - first);
+            const int* in = p.cand + tr.cand_off[level - 1] + (long)first * 64;
+            for (int li = t; li < TS_S; li += TS_NT) {
+                const int ch = li >> 6;
+                s_lrow[li] = ch < nch ? __hip_atomic_load(in + li, __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : -1;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// L21 = A21 inv(U11) for logical rows [r_begin, r_end): 64 rows per
+// work-group of 256 threads (thread t: rows {i * 32 + t/8 : i < 2}, columns
+// 8 (t%8) .. +8), forward substitution over the columns with U11 in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_l21(cx<T>* A, long ld, const int* perm, int c0, int w,
+                                             int r_begin, int r_end)
+{
+    __shared__ cx<T> U[64][64];
+    __shared__ int s_pm[64];
+    const int t = threadIdx.x, rg = t >> 3, cg = t & 7, lane = t & 63;
+    if (t < 64) s_pm[t] = t < w ? perm[c0 + t] : 0;
+    cx<T> a[2][8];
+    cx<T>* rowp[2];
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const int lr = r_begin + blockIdx.x * 64 + i * 32 + rg;
+        rowp[i] = lr < r_end ? A + (long)perm[lr] * ld + c0 : nullptr;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int c = cg * 8 + q;
+            a[i][q] = (rowp[i] && c < w) ? rowp[i][c] : cx<T>{0, 0};
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < 16; it++) {
+        const int e = t + it * 256;
+        const int k = e >> 6, c = e & 63;
+        U[k][c] = (k < w && c < w && c >= k) ? A[(long)s_pm[k] * ld + c0 + c] : cx<T>{0, 0};
+    }
+    __syncthreads();
+    for (int jb = 0; jb < 8; jb++) {
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const int j = jb * 8 + k;
+            if (j >= w) break;
+            const cx<T> d = U[j][j];
+            if (d.x == T(0) && d.y == T(0)) continue;  // singular: left unscaled
+            const cx<T> rinv = crecip(d);
+            const int src = (lane & ~7) | jb;
+            cx<T> l[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                l[i] = cmul(a[i][k], rinv);
+                l[i].x = __shfl(l[i].x, src);
+                l[i].y = __shfl(l[i].y, src);
+                a[i][k] = csel(cg == jb, l[i], a[i][k]);
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                cx<T> u = U[j][cg * 8 + q];
+                u = csel(cg * 8 + q > j, u, cx<T>{0, 0});
+#pragma unroll
+                for (int i = 0; i < 2; i++) a[i][q] = csub(a[i][q], cmul(l[i], u));
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        if (!rowp[i]) continue;
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (cg * 8 + q < w) rowp[i][cg * 8 + q] = a[i][q];
+    }
+}
+
+int tslu_tree(int cand_rows, int leaf_rows, TsluTree* tr)
+{
+    if (cand_rows <= 0 || leaf_rows < 128) return TAGS_ERR_ARG;
+    const int fan = leaf_rows / 64;
+    int nodes = (cand_rows + leaf_rows - 1) / leaf_rows;
+    int coff = 0, noff = 0, lev = 0;
+    for (;;) {
+        if (lev >= TSLU_MAX_LEVELS) return TAGS_ERR_TOO_LARGE;
+        tr->nodes[lev] = nodes;
+        tr->cand_off[lev] = coff;
+        tr->cnt_off[lev] = noff;
+        coff += nodes * 64;
+        if (lev > 0) noff += nodes;
+        lev++;
+        if (nodes == 1) break;
+        nodes = (nodes + fan - 1) / fan;
+    }
+    tr->nlev = lev;
+    tr->cand_total = coff;
+    tr->cnt_total = noff;
+    return 0;
+}
+
+int tslu_min_leaf_rows() { return TsShape<double>::S; }
+
+template <typename T>
+int launch_tslu(cx<T>* A, long ld, int* perm, int c0, int w, int row_end, int cand_end,
+                int* cand, unsigned* cnt, int* info, hipStream_t st)
+{
+    if (w <= 0 || w > 64 || cand_end - c0 < w || row_end < cand_end) return TAGS_ERR_ARG;
+    TsluArgs<T> p;
+    p.A = A;
+    p.ld = ld;
+    p.perm = perm;
+    p.c0 = c0;
+    p.w = w;
+    p.cand_end = cand_end;
+    p.cand = cand;
+    p.cnt = cnt;
+    p.info = info;
+    int rc = tslu_tree(cand_end - c0, TsShape<T>::S, &p.tree);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_tslu<T>, dim3(p.tree.nodes[0]), dim3(TS_NT), 0, st, p);
+    TAGS_HIP(hipGetLastError());
+    const int rb = c0 + w;
+    if (row_end > rb) {
+        hipLaunchKernelGGL(k_l21<T>, dim3((row_end - rb + 63) / 64), dim3(256), 0, st, A, ld,
+                           perm, c0, w, rb, row_end);
+        TAGS_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+template int launch_tslu<double>(c64*, long, int*, int, int, int, int, int*, unsigned*, int*,
+                                 hipStream_t);
+template int launch_tslu<float>(c32*, long, int*, int, int, int, int, int*, unsigned*, int*,
+                                hipStream_t);
+
+}  // namespace tags
