@@ -241,7 +241,7 @@ __device__ inline cx<T> quad_sum(cx<T> s)
 // latency once per iteration).
 template <typename T>
 __device__ inline void load_diag_block(const cx<T>* A, long ld, const int* perm, int c0, int w,
-                                       cx<T> (*B)[64])
+                                       cx<T> (*B)[65])
 {
     __shared__ int s_pm[64];
     if (threadIdx.x < 64) s_pm[threadIdx.x] = threadIdx.x < w ? perm[c0 + threadIdx.x] : 0;
@@ -263,11 +263,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_inv_lower(const cx<T>* A, long ld, const int* perm,
                                                     int c0, int w, cx<T>* out)
 {
-    __shared__ cx<T> B[64][64];
-    __shared__ cx<T> X[64][64];
+    __shared__ cx<T> B[64][65];  // row stride 65: rows of a column fall in different banks
+    __shared__ cx<T> X[64][65];
     load_diag_block(A, ld, perm, c0, w, B);
     const int t = threadIdx.x, row_off = t >> 2, part = t & 3;
-    for (int e = t; e < 64 * 64; e += 256) (&X[0][0])[e] = cx<T>{0, 0};
+    for (int e = t; e < 64 * 65; e += 256) (&X[0][0])[e] = cx<T>{0, 0};
     __syncthreads();
     for (int j = 63; j >= 0; j--) {
         const int i = j + 1 + row_off;
@@ -294,15 +294,15 @@ __global__ __launch_bounds__(256) void k_inv_upper(const cx<T>* A, long ld, cons
                                                     const int* blk, int nblk, int n,
                                                     cx<T>* inv_base)
 {
-    __shared__ cx<T> B[64][64];
-    __shared__ cx<T> X[64][64];
+    __shared__ cx<T> B[64][65];  // row stride 65: rows of a column fall in different banks
+    __shared__ cx<T> X[64][65];
     const int b = blockIdx.x;
     const int c0 = blk[b];
     const int w = (b + 1 < nblk ? blk[b + 1] : n) - c0;
     cx<T>* out = inv_base + (size_t)b * 4096;
     load_diag_block(A, ld, perm, c0, w, B);
     const int t = threadIdx.x, row = t >> 2, part = t & 3;
-    for (int e = t; e < 64 * 64; e += 256) (&X[0][0])[e] = cx<T>{0, 0};
+    for (int e = t; e < 64 * 65; e += 256) (&X[0][0])[e] = cx<T>{0, 0};
     __syncthreads();
     for (int j = 0; j < 64; j++) {
         const cx<T> inv = crecip(B[j][j]);
@@ -533,6 +533,11 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit)
     n = n_;
     nblk_cap = (n + 63) / 64 + 8;
     TAGS_HIP(hipMalloc(&perm, sizeof(int) * n));
+    TAGS_HIP(hipMalloc(&perm_alt, sizeof(int) * n));
+    TAGS_HIP(hipMalloc(&tmp2, sizeof(cx<T>) * 64 * 64));
+    TAGS_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+    TAGS_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
+    TAGS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
     TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * 4096 * (size_t)nblk_cap));
     TAGS_HIP(hipMalloc(&invU, sizeof(cx<T>) * 4096 * (size_t)nblk_cap));
     TAGS_HIP(hipMalloc(&tmp, sizeof(cx<T>) * 128 * (size_t)n));
@@ -556,7 +561,17 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit)
 template <typename T>
 void LUWork<T>::release()
 {
+    if (s2) (void)hipStreamSynchronize(s2);
     (void)hipFree(perm);
+    (void)hipFree(perm_alt);
+    (void)hipFree(tmp2);
+    if (s2) (void)hipStreamDestroy(s2);
+    if (ev_left) (void)hipEventDestroy(ev_left);
+    if (ev_panel) (void)hipEventDestroy(ev_panel);
+    perm_alt = nullptr;
+    tmp2 = nullptr;
+    s2 = nullptr;
+    ev_left = ev_panel = nullptr;
     (void)hipFree(invL);
     (void)hipFree(invU);
     (void)hipFree(tmp);
@@ -611,13 +626,13 @@ void lu_set_panel_kind(int kind) { g_panel_kind.store(kind == PANEL_PARTIAL ? 1 
 int lu_panel_kind() { return g_panel_kind.load(); }
 
 template <typename T>
-static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int c0,
-                 int cols, int row_end, int cand_end)
+static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
+                 int c0, int cols, int row_end, int cand_end)
 {
     const int M = row_end - c0;
     if (g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT) {
         Timed tm(pr, FAM_PANEL, st);
-        const int rc = launch_tslu<T>(A, ld, w.perm, c0, cols, row_end, cand_end, w.tcand,
+        const int rc = launch_tslu<T>(A, ld, pm, c0, cols, row_end, cand_end, w.tcand,
                                       w.tcnt, w.flags, st);
         tm.end();
         w.flops += 8.0 * (double)M * cols * cols / 2;
@@ -637,7 +652,7 @@ static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, 
                                                               (size_t)R * per_row));
         if (P > occ * w.num_cu || P > 511) return TAGS_ERR_TOO_LARGE;
     }
-    PanelArgs<T> p{A, ld, w.perm, c0, M, c0, cols, R, cand_end, w.slots, w.bar, w.flags,
+    PanelArgs<T> p{A, ld, pm, c0, M, c0, cols, R, cand_end, w.slots, w.bar, w.flags,
                    w.flags + 1};
     const size_t lds = (size_t)R * per_row;
     void* args[] = {&p};
@@ -689,14 +704,81 @@ static int gather(Profiler* pr, hipStream_t st, const cx<T>* A, long ld, const i
 // overwritten with inv(L_block) * rows (block inverse `inv`, h <= 64).
 template <typename T>
 static int apply_linv_rows(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
-                           const cx<T>* inv, int r0, int h, int cb, int ce)
+                           const int* pm, cx<T>* tmp, const cx<T>* inv, int r0, int h, int cb,
+                           int ce)
 {
     const int cols = ce - cb;
     if (cols <= 0) return 0;
-    int rc = gather<T>(pr, st, A, ld, w.perm + r0, cb, h, cols, w.tmp);
+    int rc = gather<T>(pr, st, A, ld, pm + r0, cb, h, cols, tmp);
     if (rc) return rc;
-    return gemm<T>(pr, st, h, cols, h, inv, 64, nullptr, w.tmp, cols, nullptr, A + cb, ld,
-                   w.perm + r0, 0, false, &w.flops);
+    return gemm<T>(pr, st, h, cols, h, inv, 64, nullptr, tmp, cols, nullptr, A + cb, ld, pm + r0,
+                   0, false, &w.flops);
+}
+
+// One 128-column step of the factorisation.
+struct LUStep {
+    int j, jb;    // first column, width (<= 128)
+    int phase;    // index into the plan's phases
+    int slot;     // index of its first diagonal block in the block list
+};
+
+// Panel work of a step on stream st with row map pm: sub-panel a, its
+// unit-lower inverse, then (if the step has a second sub-panel) U_ab =
+// inv(L_aa) A_ab, A_b -= L_a U_ab, sub-panel b and its inverse.
+template <typename T>
+static int step_panels(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
+                       cx<T>* tmp, const LUPhase& ph, const LUStep& s)
+{
+    const int j = s.j, wa = std::min(64, s.jb), wb = s.jb - wa;
+    int rc;
+    if ((rc = panel<T>(w, pr, st, A, ld, pm, j, wa, ph.row_end, ph.cand_end))) return rc;
+    cx<T>* inva = w.invL + (size_t)s.slot * 4096;
+    {
+        Timed tm(pr, FAM_OTHER, st);
+        hipLaunchKernelGGL(k_inv_lower<T>, dim3(1), dim3(256), 0, st, A, ld, pm, j, wa, inva);
+        TAGS_HIP(hipGetLastError());
+        tm.end();
+    }
+    if (wb > 0) {
+        const int c0 = j + 64;
+        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, tmp, inva, j, 64, c0, c0 + wb)))
+            return rc;
+        if ((rc = gemm<T>(pr, st, ph.row_end - c0, wb, 64, A + j, ld, pm + c0, A + c0, ld, pm + j,
+                          A + c0, ld, pm + c0, 1, false, &w.flops)))
+            return rc;
+        if ((rc = panel<T>(w, pr, st, A, ld, pm, c0, wb, ph.row_end, ph.cand_end))) return rc;
+        Timed tm(pr, FAM_OTHER, st);
+        hipLaunchKernelGGL(k_inv_lower<T>, dim3(1), dim3(256), 0, st, A, ld, pm, c0, wb,
+                           w.invL + (size_t)(s.slot + 1) * 4096);
+        TAGS_HIP(hipGetLastError());
+        tm.end();
+    }
+    return 0;
+}
+
+// Update of a step for the trailing columns [cb, ce): U12 of both sub-panels'
+// pivot rows, then A22 -= L21 U12 over logical rows [j + jb, row_end).
+template <typename T>
+static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
+                       const int* pm, cx<T>* tmp, const LUPhase& ph, const LUStep& s, int cb,
+                       int ce)
+{
+    if (ce <= cb) return 0;
+    const int j = s.j, wa = std::min(64, s.jb), wb = s.jb - wa, c1 = j + s.jb;
+    const cx<T>* inva = w.invL + (size_t)s.slot * 4096;
+    int rc;
+    if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, tmp, inva, j, wa, cb, ce))) return rc;
+    if (wb > 0) {
+        const int c0 = j + 64;
+        if ((rc = gemm<T>(pr, st, wb, ce - cb, 64, A + j, ld, pm + c0, A + cb, ld, pm + j, A + cb,
+                          ld, pm + c0, 1, false, &w.flops)))
+            return rc;
+        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, tmp,
+                                     w.invL + (size_t)(s.slot + 1) * 4096, c0, wb, cb, ce)))
+            return rc;
+    }
+    return gemm<T>(pr, st, ph.row_end - c1, ce - cb, s.jb, A + j, ld, pm + c1, A + cb, ld, pm + j,
+                   A + cb, ld, pm + c1, 1, true, &w.flops);
 }
 
 template <typename T>
@@ -727,69 +809,88 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     hipLaunchKernelGGL(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, st, w.perm, init, n,
                        w.flags);
     TAGS_HIP(hipGetLastError());
-    int* perm = w.perm;
+    // flat list of steps over the phases
+    std::vector<LUStep> steps;
     w.blk.clear();
     w.flops = 0;
-    for (const LUPhase& ph : plan->phases) {
-        const int row_end = ph.row_end;
+    for (int p = 0; p < (int)plan->phases.size(); p++) {
+        const LUPhase& ph = plan->phases[p];
         for (int j = ph.c_begin; j < ph.c_end; j += 128) {
             const int jb = std::min(128, ph.c_end - j);
-            const int wa = std::min(64, jb), wb = jb - wa;
-            const int c1 = j + jb;
-            if ((int)w.blk.size() + 2 > w.nblk_cap) return TAGS_ERR_TOO_LARGE;
-            // sub-panel a
-            if ((rc = panel<T>(w, pr, st, A, ld, j, wa, row_end, ph.cand_end))) return rc;
-            cx<T>* inva = w.invL + (size_t)w.blk.size() * 4096;
+            steps.push_back(LUStep{j, jb, p, (int)w.blk.size()});
             w.blk.push_back(j);
-            {
-                Timed tm(pr, FAM_OTHER, st);
-                hipLaunchKernelGGL(k_inv_lower<T>, dim3(1), dim3(256), 0, st, A, ld, perm, j, wa,
-                                   inva);
-                tm.end();
-            }
-            cx<T>* invb = nullptr;
-            if (wb > 0) {
-                const int c0 = j + 64;
-                // U_ab = inv(L_aa) A_ab; A_b -= L_a U_ab (logical rows c0 .. row_end)
-                if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, inva, j, 64, c0, c0 + wb))) return rc;
-                if ((rc = gemm<T>(pr, st, row_end - c0, wb, 64, A + j, ld, perm + c0, A + c0, ld,
-                                  perm + j, A + c0, ld, perm + c0, 1, false, &w.flops)))
-                    return rc;
-                if ((rc = panel<T>(w, pr, st, A, ld, c0, wb, row_end, ph.cand_end))) return rc;
-                invb = w.invL + (size_t)w.blk.size() * 4096;
-                w.blk.push_back(c0);
-                Timed tm(pr, FAM_OTHER, st);
-                hipLaunchKernelGGL(k_inv_lower<T>, dim3(1), dim3(256), 0, st, A, ld, perm, c0, wb,
-                                   invb);
-                tm.end();
-            }
-            // trailing column ranges: [c1, c_end) and the phase's extra range
-            int rb[2] = {c1, ph.x_begin}, re[2] = {ph.c_end, ph.x_end};
-            if (ph.x_begin == ph.c_end && ph.x_end > ph.x_begin) {
-                re[0] = ph.x_end;
-                re[1] = rb[1];
-            }
-            for (int q = 0; q < 2; q++) {
-                if (re[q] <= rb[q]) continue;
-                // U12 rows j .. j+wa
-                if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, inva, j, wa, rb[q], re[q]))) return rc;
-                if (wb > 0) {
-                    const int c0 = j + 64;
-                    if ((rc = gemm<T>(pr, st, wb, re[q] - rb[q], 64, A + j, ld, perm + c0,
-                                      A + rb[q], ld, perm + j, A + rb[q], ld, perm + c0, 1, false,
-                                      &w.flops)))
-                        return rc;
-                    if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, invb, c0, wb, rb[q], re[q])))
-                        return rc;
-                }
-                // trailing update A22 -= L21 U12 (logical rows c1 .. row_end)
-                if ((rc = gemm<T>(pr, st, row_end - c1, re[q] - rb[q], jb, A + j, ld, perm + c1,
-                                  A + rb[q], ld, perm + j, A + rb[q], ld, perm + c1, 1, true,
-                                  &w.flops)))
-                    return rc;
-            }
+            if (jb > 64) w.blk.push_back(j + 64);
         }
     }
+    if ((int)w.blk.size() > w.nblk_cap) return TAGS_ERR_TOO_LARGE;
+    // Look-ahead: the panel work of step s+1 runs on the second stream with its
+    // own copy of the row map while this stream finishes step s's update of
+    // the columns the panel does not touch. The update of the next panel's
+    // columns goes first; the panel only permutes logical rows at or after its
+    // first column, so the rest of step s (pivot rows before it, the same set
+    // of physical rows after it, other columns) is unaffected by its changes.
+    // The cooperative panel needs the whole device and is not overlapped.
+    const bool overlap = g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT;
+    int* pcur = w.perm;
+    int* palt = w.perm_alt;
+    if (!steps.empty()) {
+        const LUStep& s0 = steps[0];
+        if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, w.tmp, plan->phases[s0.phase], s0)))
+            return rc;
+    }
+    for (size_t k = 0; k < steps.size(); k++) {
+        const LUStep& s = steps[k];
+        const LUPhase& ph = plan->phases[s.phase];
+        const int c1 = s.j + s.jb;
+        // trailing column ranges of the step: [c1, c_end) and the phase's extra range
+        int rb[2] = {c1, ph.x_begin}, re[2] = {ph.c_end, ph.x_end};
+        if (ph.x_begin == ph.c_end && ph.x_end > ph.x_begin) {
+            re[0] = ph.x_end;
+            re[1] = rb[1];
+        }
+        if (k + 1 < steps.size()) {
+            const LUStep& nx = steps[k + 1];
+            const int nb = nx.j, ne = nx.j + nx.jb;
+            // columns of the next panel first
+            for (int q = 0; q < 2; q++) {
+                const int cb = std::max(rb[q], nb), ce = std::min(re[q], ne);
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, cb, ce))) return rc;
+            }
+            hipStream_t ps = st;
+            if (overlap) {
+                TAGS_HIP(hipEventRecord(w.ev_left, st));
+                TAGS_HIP(hipStreamWaitEvent(w.s2, w.ev_left, 0));
+                TAGS_HIP(hipMemcpyAsync(palt, pcur, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                                        w.s2));
+                ps = w.s2;
+            }
+            int* pnext = overlap ? palt : pcur;
+            if ((rc = step_panels<T>(w, pr, ps, A, ld, pnext, overlap ? w.tmp2 : w.tmp,
+                                     plan->phases[nx.phase], nx)))
+                return rc;
+            if (overlap) TAGS_HIP(hipEventRecord(w.ev_panel, w.s2));
+            // the remaining columns of this step
+            for (int q = 0; q < 2; q++) {
+                if (re[q] <= rb[q]) continue;
+                const int lo1 = rb[q], hi1 = std::min(re[q], nb);
+                const int lo2 = std::max(rb[q], ne), hi2 = re[q];
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, lo1, hi1)))
+                    return rc;
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, lo2, hi2)))
+                    return rc;
+            }
+            if (overlap) {
+                TAGS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
+                std::swap(pcur, palt);
+            }
+        } else {
+            for (int q = 0; q < 2; q++)
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, rb[q], re[q])))
+                    return rc;
+        }
+    }
+    if (pcur != w.perm) std::swap(w.perm, w.perm_alt);
+    int* perm = w.perm;
     w.nblk = (int)w.blk.size();
     if (w.h_blk != w.blk) {
         TAGS_HIP(hipStreamSynchronize(st));

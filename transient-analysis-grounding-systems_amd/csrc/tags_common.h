@@ -46,6 +46,17 @@ __host__ __device__ inline cx<T> crecip(cx<T> a) {
 }
 template <typename T>
 __host__ __device__ inline cx<T> cdiv(cx<T> a, cx<T> b) { return cmul(a, crecip(b)); }
+// a - l * u with four fused multiply-adds
+template <typename T>
+__host__ __device__ inline cx<T> cfnma(cx<T> a, cx<T> l, cx<T> u)
+{
+    T x = fma(-l.x, u.x, a.x);
+    x = fma(l.y, u.y, x);
+    T y = fma(-l.x, u.y, a.y);
+    y = fma(-l.y, u.x, y);
+    return cx<T>{x, y};
+}
+
 // c ? a : b, component-wise (a select of the struct itself would be lowered to
 // a copy through a selected pointer, which keeps register arrays in memory)
 template <typename T>

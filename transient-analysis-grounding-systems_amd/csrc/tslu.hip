@@ -57,29 +57,6 @@ struct TsluArgs {
     TsluTree tree;
 };
 
-// Largest 64-bit key of the wave (DPP row operations, then lane 63); uniform.
-__device__ inline unsigned long long wave_max_u64(unsigned long long v)
-{
-#define TS_DPP_MAX(CTRL, RM)                                                                \
-    {                                                                                        \
-        const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);                     \
-        const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, RM, 0xF, false); \
-        const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, RM, 0xF, false); \
-        const unsigned long long o = ((unsigned long long)ohi << 32) | olo;                  \
-        v = o > v ? o : v;                                                                   \
-    }
-    TS_DPP_MAX(0xB1, 0xF)   // quad_perm [1, 0, 3, 2]
-    TS_DPP_MAX(0x4E, 0xF)   // quad_perm [2, 3, 0, 1]
-    TS_DPP_MAX(0x124, 0xF)  // row_ror 4
-    TS_DPP_MAX(0x128, 0xF)  // row_ror 8: every lane holds its 16-lane row's maximum
-    TS_DPP_MAX(0x142, 0xA)  // row_bcast 15 into rows 1 and 3
-    TS_DPP_MAX(0x143, 0xC)  // row_bcast 31 into rows 2 and 3: lane 63 holds the maximum
-#undef TS_DPP_MAX
-    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)v, 63);
-    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), 63);
-    return ((unsigned long long)hi << 32) | lo;
-}
-
 __device__ inline cx<float> readlane_cx(cx<float> v, int l)
 {
     return cx<float>{__int_as_float(__builtin_amdgcn_readlane(__float_as_int(v.x), l)),
@@ -97,6 +74,58 @@ __device__ inline cx<double> readlane_cx(cx<double> v, int l)
     return cx<double>{readlane_d(v.x, l), readlane_d(v.y, l)};
 }
 
+// 1 / p for p != 0: p is scaled by a power of two so that |p|^2 lies in
+// [1/4, 2), then the hardware reciprocal is refined by Newton steps (one for
+// float, two for double) and the scale undone.
+__device__ inline float rcp_nr(float d)
+{
+    float r = __builtin_amdgcn_rcpf(d);
+    return r * fmaf(-d, r, 2.0f);
+}
+__device__ inline double rcp_nr(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = r * fma(-d, r, 2.0);
+    return r * fma(-d, r, 2.0);
+}
+template <typename T>
+__device__ inline cx<T> crecip_fast(cx<T> p)
+{
+    int e;
+    (void)frexp(fmax(fabs(p.x), fabs(p.y)), &e);
+    const T sx = ldexp(p.x, -e), sy = ldexp(p.y, -e);
+    const T r = rcp_nr(fma(sx, sx, sy * sy));
+    return cx<T>{ldexp(sx * r, -e), ldexp(-sy * r, -e)};
+}
+
+// Wave maximum of a per-lane value (DPP row operations, then lane 63); uniform.
+__device__ inline float wave_max(float v)
+{
+#define TS_DPP_MAXF(CTRL, RM)                                                               \
+    v = fmaxf(v, __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(v),             \
+                                                            __float_as_int(v), CTRL, RM,   \
+                                                            0xF, false)));
+    TS_DPP_MAXF(0xB1, 0xF) TS_DPP_MAXF(0x4E, 0xF) TS_DPP_MAXF(0x124, 0xF)
+    TS_DPP_MAXF(0x128, 0xF) TS_DPP_MAXF(0x142, 0xA) TS_DPP_MAXF(0x143, 0xC)
+#undef TS_DPP_MAXF
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+__device__ inline double wave_max(double v)
+{
+#define TS_DPP_MAXD(CTRL, RM)                                                               \
+    {                                                                                        \
+        const long long b = __double_as_longlong(v);                                        \
+        const int lo = (int)(unsigned)b, hi = (int)(unsigned)(b >> 32);                     \
+        const unsigned olo = (unsigned)__builtin_amdgcn_update_dpp(lo, lo, CTRL, RM, 0xF, false); \
+        const unsigned ohi = (unsigned)__builtin_amdgcn_update_dpp(hi, hi, CTRL, RM, 0xF, false); \
+        v = fmax(v, __longlong_as_double((long long)(((unsigned long long)ohi << 32) | olo))); \
+    }
+    TS_DPP_MAXD(0xB1, 0xF) TS_DPP_MAXD(0x4E, 0xF) TS_DPP_MAXD(0x124, 0xF)
+    TS_DPP_MAXD(0x128, 0xF) TS_DPP_MAXD(0x142, 0xA) TS_DPP_MAXD(0x143, 0xC)
+#undef TS_DPP_MAXD
+    return readlane_d(v, 63);
+}
+
 // Row i * 64 + lane of the stack, register row `ri` selected at run time.
 template <typename T, int RPT>
 __device__ __forceinline__ cx<T> pick_row(const cx<T> (&a)[RPT][8], int ri, int q)
@@ -107,9 +136,10 @@ __device__ __forceinline__ cx<T> pick_row(const cx<T> (&a)[RPT][8], int ri, int 
     return v;
 }
 
-// Column K of the block held by this wave (partial pivoting within the wave:
-// DPP maximum, pivot row by lane reads, rank-1 update of the block's columns
-// right of K). Returns false when no live row is left.
+// Column K of the block held by this wave: partial pivoting within the wave
+// (largest |re| + |im| by a wave maximum; ties to the lowest lane, then the
+// lowest register row), pivot row by lane reads, rank-1 update of the block's
+// columns right of K. Returns false when no live row is left.
 template <typename T, int K>
 __device__ __forceinline__ bool cw_step(cx<T> (&a)[TsShape<T>::RPT][8], unsigned& live, int kn,
                                         int j, int (&pvk)[8], int& zc)
@@ -117,34 +147,34 @@ __device__ __forceinline__ bool cw_step(cx<T> (&a)[TsShape<T>::RPT][8], unsigned
     constexpr int RPT = TsShape<T>::RPT;
     if (K >= kn) return false;
     const int lane = threadIdx.x & 63;
-    unsigned long long key = 0;
+    T best = T(-1);
+    int bi = 0;
 #pragma unroll
     for (int i = 0; i < RPT; i++) {
-        const double v = (double)cabs1(a[i][K]);
-        unsigned long long kk =
-            ((unsigned long long)__double_as_longlong(v) & 0x7FFFFFFFFFFFFE00ull) |
-            (unsigned long long)(0x1FF - (i * 64 + lane));
-        kk = ((live >> i) & 1) ? kk : 0ull;
-        key = kk > key ? kk : key;
+        const T v = ((live >> i) & 1) ? cabs1(a[i][K]) : T(-1);
+        bi = v > best ? i : bi;
+        best = v > best ? v : best;
     }
-    key = wave_max_u64(key);
-    if (key == 0) return false;
-    const int li = 0x1FF - (int)(key & 0x1FF);
-    const int wr = li & 63, wi = li >> 6;
+    const T vmax = wave_max(best);
+    if (vmax < T(0)) return false;
+    const unsigned long long hit = __ballot(best == vmax);
+    const int wr = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
+    const int wi = __builtin_amdgcn_readlane(bi, wr);
+    const int li = wi * 64 + wr;
     pvk[K] = li;
     cx<T> u[8];
 #pragma unroll
     for (int q = K; q < 8; q++) u[q] = readlane_cx(pick_row<T, RPT>(a, wi, q), wr);
     if (lane == wr) live &= ~(1u << wi);
-    if ((key >> 9) != 0) {
-        const cx<T> rinv = crecip(u[K]);
+    if (vmax > T(0)) {
+        const cx<T> rinv = crecip_fast(u[K]);
 #pragma unroll
         for (int i = 0; i < RPT; i++) {
             const bool lv = (live >> i) & 1;
             const cx<T> l = csel(lv, cmul(a[i][K], rinv), cx<T>{0, 0});
             a[i][K] = csel(lv, l, a[i][K]);
 #pragma unroll
-            for (int q = K + 1; q < 8; q++) a[i][q] = csub(a[i][q], cmul(l, u[q]));
+            for (int q = K + 1; q < 8; q++) a[i][q] = cfnma(a[i][q], l, u[q]);
         }
     } else if (zc < 0) {
         zc = j;
@@ -256,7 +286,7 @@ __device__ __forceinline__ int tslu_gepp(cx<T> (&a)[TsShape<T>::RPT][8], const i
                         const int pr = sh.pv[par][t];
                         cx<T> v = sh.P[wave][t][lane];
 #pragma unroll
-                        for (int s2 = 0; s2 < t; s2++) v = csub(v, cmul(sh.L[par][s2][pr], u[s2]));
+                        for (int s2 = 0; s2 < t; s2++) v = cfnma(v, sh.L[par][s2][pr], u[s2]);
                         u[t] = v;
                         sh.U[wave][t][lane] = v;
                     }
@@ -274,7 +304,7 @@ __device__ __forceinline__ int tslu_gepp(cx<T> (&a)[TsShape<T>::RPT][8], const i
                     for (int q = 0; q < 8; q++) {
                         const cx<T> u = sh.U[wave][t][q];
 #pragma unroll
-                        for (int i = 0; i < RPT; i++) a[i][q] = csub(a[i][q], cmul(l[i], u));
+                        for (int i = 0; i < RPT; i++) a[i][q] = cfnma(a[i][q], l[i], u);
                     }
                 }
             }
@@ -433,6 +463,7 @@ __global__ __launch_bounds__(256) void k_l21(cx<T>* A, long ld, const int* perm,
                                              int r_begin, int r_end)
 {
     __shared__ cx<T> U[64][64];
+    __shared__ cx<T> s_rd[64];  // reciprocals of U's diagonal (zero for a zero pivot)
     __shared__ int s_pm[64];
     const int t = threadIdx.x, rg = t >> 3, cg = t & 7, lane = t & 63;
     if (t < 64) s_pm[t] = t < w ? perm[c0 + t] : 0;
@@ -456,14 +487,18 @@ __global__ __launch_bounds__(256) void k_l21(cx<T>* A, long ld, const int* perm,
         U[k][c] = (k < w && c < w && c >= k) ? A[(long)s_pm[k] * ld + c0 + c] : cx<T>{0, 0};
     }
     __syncthreads();
+    if (t < 64) {
+        const cx<T> d = U[t][t];
+        s_rd[t] = (d.x == T(0) && d.y == T(0)) ? cx<T>{0, 0} : crecip_fast(d);
+    }
+    __syncthreads();
     for (int jb = 0; jb < 8; jb++) {
 #pragma unroll
         for (int k = 0; k < 8; k++) {
             const int j = jb * 8 + k;
             if (j >= w) break;
-            const cx<T> d = U[j][j];
-            if (d.x == T(0) && d.y == T(0)) continue;  // singular: left unscaled
-            const cx<T> rinv = crecip(d);
+            const cx<T> rinv = s_rd[j];
+            if (rinv.x == T(0) && rinv.y == T(0)) continue;  // singular: left unscaled
             const int src = (lane & ~7) | jb;
             cx<T> l[2];
 #pragma unroll
@@ -478,7 +513,7 @@ __global__ __launch_bounds__(256) void k_l21(cx<T>* A, long ld, const int* perm,
                 cx<T> u = U[j][cg * 8 + q];
                 u = csel(cg * 8 + q > j, u, cx<T>{0, 0});
 #pragma unroll
-                for (int i = 0; i < 2; i++) a[i][q] = csub(a[i][q], cmul(l[i], u));
+                for (int i = 0; i < 2; i++) a[i][q] = cfnma(a[i][q], l[i], u);
             }
         }
     }
