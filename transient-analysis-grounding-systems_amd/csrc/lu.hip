@@ -259,33 +259,50 @@ __device__ inline void load_diag_block(const cx<T>* A, long ld, const int* perm,
     }
 }
 
+// Unit-lower inverse X = L^-1 of the diagonal block at column c0 (logical rows
+// c0 .. c0+w), written as a dense 64 x 64 row-major block with identity
+// padding past w. Row sweep in registers: thread (r, p) holds columns
+// 16p .. 16p+15 of row r of L and of X (X starts as the identity); at step k
+// row k of X is final and is broadcast through LDS, and every row r > k does
+// X[r] -= L[r][k] X[k] (L[r][k] comes from the thread of the quad holding
+// column k). 2 or 4 KB of LDS, so the kernel fits beside other work-groups.
 template <typename T>
 __global__ __launch_bounds__(256) void k_inv_lower(const cx<T>* A, long ld, const int* perm,
                                                     int c0, int w, cx<T>* out)
 {
-    __shared__ cx<T> B[64][65];  // row stride 65: rows of a column fall in different banks
-    __shared__ cx<T> X[64][65];
-    load_diag_block(A, ld, perm, c0, w, B);
-    const int t = threadIdx.x, row_off = t >> 2, part = t & 3;
-    for (int e = t; e < 64 * 65; e += 256) (&X[0][0])[e] = cx<T>{0, 0};
-    __syncthreads();
-    for (int j = 63; j >= 0; j--) {
-        const int i = j + 1 + row_off;
-        cx<T> s = cx<T>{0, 0};
-        if (i < 64) {
-            for (int k = j + 1 + part; k < i; k += 4) s = cadd(s, cmul(X[i][k], B[k][j]));
-        }
-        s = quad_sum(s);
-        if (i < 64 && part == 0) {
-            const cx<T> v = cadd(s, B[i][j]);
-            X[i][j] = cx<T>{-v.x, -v.y};
-        }
-        __syncthreads();
+    __shared__ cx<T> xk[2][64];
+    __builtin_amdgcn_s_setprio(1);  // on the panel's critical path
+    const int t = threadIdx.x, r = t >> 2, part = t & 3, lane = t & 63;
+    cx<T> L[16], X[16];
+    const cx<T>* rowp = r < w ? A + (long)perm[c0 + r] * ld + c0 : nullptr;
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        const int c = part * 16 + q;
+        L[q] = (rowp && c < r && c < w) ? rowp[c] : cx<T>{0, 0};
+        X[q] = cx<T>{c == r ? T(1) : T(0), T(0)};
     }
-    for (int e = t; e < 64 * 64; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        out[e] = r > c ? X[r][c] : (r == c ? cx<T>{1, 0} : cx<T>{0, 0});
+    for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+        for (int qq = 0; qq < 16; qq++) {
+            const int k = kb * 16 + qq;
+            if (k >= 63) break;
+            if (r == k) {
+#pragma unroll
+                for (int q = 0; q < 16; q++) xk[k & 1][part * 16 + q] = X[q];
+            }
+            __syncthreads();
+            const int src = (lane & ~3) | kb;
+            cx<T> l;
+            l.x = __shfl(L[qq].x, src);
+            l.y = __shfl(L[qq].y, src);
+            if (r > k) {
+#pragma unroll
+                for (int q = 0; q < 16; q++) X[q] = cfnma(X[q], l, xk[k & 1][part * 16 + q]);
+            }
+        }
     }
+#pragma unroll
+    for (int q = 0; q < 16; q++) out[r * 64 + part * 16 + q] = X[q];
 }
 
 // one work-group per diagonal block b: columns blk[b] .. blk[b+1] (or n)

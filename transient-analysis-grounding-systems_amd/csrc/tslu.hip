@@ -7,40 +7,40 @@
 // L entries.
 //
 //  k_tslu       one ordinary launch, one work-group per leaf of S candidate
-//               rows (256 for complex64, 128 for complex128). Each leaf runs
+//               rows (512 for complex64, 256 for complex128). Each leaf runs
 //               partial pivoting on its rows (in registers) and nominates up
 //               to 64 rows, in pivot order. Tree nodes merge F = S / 64
-//               children: the LAST child work-group to
-//               finish (atomic arrival counter, acquire/release at agent
-//               scope) stacks the children's nominees (their current, i.e.
-//               unfactored, rows), runs partial pivoting on that stack and
-//               nominates again; the others exit. No work-group ever waits for
-//               another, so the launch needs no co-residency. The root's
-//               factored stack holds L11 \ U11 of the winners in pivot order;
-//               it writes them to the matrix, moves the winners to logical
-//               rows c0 .. c0+w-1 (swaps of the row map only) and resets the
-//               counters.
+//               children: the LAST child work-group to finish (atomic arrival
+//               counter, acquire/release at agent scope) stacks the children's
+//               nominees (their current, i.e. unfactored, rows), runs partial
+//               pivoting on that stack and nominates again; the others exit.
+//               No work-group ever waits for another, so the launch needs no
+//               co-residency. The root's factored stack holds L11 \ U11 of the
+//               winners in pivot order; it writes them to the matrix, moves the
+//               winners to logical rows c0 .. c0+w-1 (row-map entries only) and
+//               resets the counters.
 //  k_l21        L21 = A21 inv(U11) for logical rows [c0+w, row_end), by
 //               substitution against U11 held in LDS.
 //
-// Register layout of a work-group's stack (S rows x 64 columns): thread t
-// holds rows {i * 64 + t/8 : i < S/64} x columns {8 (t%8) + q : q < 8}. The
-// eight threads of one row sit in adjacent lanes of one wave, so a multiplier
-// is broadcast with one lane shuffle; a pivot row is broadcast through LDS.
-// Pivot choice per column: largest |re| + |im| (as LAPACK's izamax), ties to
-// the lower stack index; the sign bit is cleared so -0.0 never wins.
+// Register layout of a work-group's stack (S rows x 64 columns, 16 waves):
+// lane l holds rows {i * 64 + l : i < S/64}, wave w the columns 4w .. 4w+3.
+// A column's candidates therefore all sit in one wave: its pivot search is a
+// wave maximum (DPP) with no barrier. Pivot choice per column: largest
+// |re| + |im| (as LAPACK's izamax; the sign bit is cleared, so -0.0 never
+// wins), ties to the lowest lane, then the lowest register row.
 #include "lu.h"
 
 namespace tags {
 
-constexpr int TS_NT = 512;         // threads per work-group
-constexpr int TS_NW = TS_NT / 64;  // waves; wave w owns panel columns 8w .. 8w+7
+constexpr int TS_NT = 1024;        // threads per work-group
+constexpr int TS_NW = TS_NT / 64;  // waves
+constexpr int TS_CW = 64 / TS_NW;  // panel columns per wave: wave w owns 4w .. 4w+3
 
-// Rows per lane: 4 for complex64 (S = 256 rows per work-group, 4 children per
-// tree node), 2 for complex128 (S = 128, 2 children).
+// Rows per lane: 8 for complex64 (S = 512 rows per work-group, 8 children per
+// tree node), 4 for complex128 (S = 256, 4 children).
 template <typename T>
 struct TsShape {
-    static constexpr int RPT = sizeof(T) == 8 ? 2 : 4;
+    static constexpr int RPT = sizeof(T) == 8 ? 4 : 8;
     static constexpr int S = 64 * RPT;
     static constexpr int F = S / 64;
 };
@@ -126,70 +126,45 @@ __device__ inline double wave_max(double v)
     return readlane_d(v, 63);
 }
 
-// Row i * 64 + lane of the stack, register row `ri` selected at run time.
-template <typename T, int RPT>
-__device__ __forceinline__ cx<T> pick_row(const cx<T> (&a)[RPT][8], int ri, int q)
+// A lane's part of the stack: RPT rows x TS_CW columns (row i * 64 + lane).
+template <typename T>
+using StackArr = cx<T>[TsShape<T>::RPT][TS_CW];
+
+template <typename T>
+__device__ __forceinline__ cx<T> sget(const StackArr<T>& a, int i, int q)
+{
+    return a[i][q];
+}
+template <typename T>
+__device__ __forceinline__ void sset(StackArr<T>& a, int i, int q, cx<T> c)
+{
+    a[i][q] = c;
+}
+// Row `ri` (wave-uniform, chosen at run time) of this lane's registers: a
+// chain of selects (a run-time register index would put the stack in memory).
+template <typename T>
+__device__ __forceinline__ cx<T> spick(const StackArr<T>& a, int ri, int q)
 {
     cx<T> v = a[0][q];
 #pragma unroll
-    for (int i = 1; i < RPT; i++) v = csel(i == ri, a[i][q], v);
+    for (int i = 1; i < TsShape<T>::RPT; i++) v = csel(i == ri, a[i][q], v);
     return v;
 }
-
-// Column K of the block held by this wave: partial pivoting within the wave
-// (largest |re| + |im| by a wave maximum; ties to the lowest lane, then the
-// lowest register row), pivot row by lane reads, rank-1 update of the block's
-// columns right of K. Returns false when no live row is left.
-template <typename T, int K>
-__device__ __forceinline__ bool cw_step(cx<T> (&a)[TsShape<T>::RPT][8], unsigned& live, int kn,
-                                        int j, int (&pvk)[8], int& zc)
+template <typename T>
+__device__ __forceinline__ void sput(StackArr<T>& a, int ri, int q, cx<T> c)
 {
-    constexpr int RPT = TsShape<T>::RPT;
-    if (K >= kn) return false;
-    const int lane = threadIdx.x & 63;
-    T best = T(-1);
-    int bi = 0;
 #pragma unroll
-    for (int i = 0; i < RPT; i++) {
-        const T v = ((live >> i) & 1) ? cabs1(a[i][K]) : T(-1);
-        bi = v > best ? i : bi;
-        best = v > best ? v : best;
-    }
-    const T vmax = wave_max(best);
-    if (vmax < T(0)) return false;
-    const unsigned long long hit = __ballot(best == vmax);
-    const int wr = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
-    const int wi = __builtin_amdgcn_readlane(bi, wr);
-    const int li = wi * 64 + wr;
-    pvk[K] = li;
-    cx<T> u[8];
-#pragma unroll
-    for (int q = K; q < 8; q++) u[q] = readlane_cx(pick_row<T, RPT>(a, wi, q), wr);
-    if (lane == wr) live &= ~(1u << wi);
-    if (vmax > T(0)) {
-        const cx<T> rinv = crecip_fast(u[K]);
-#pragma unroll
-        for (int i = 0; i < RPT; i++) {
-            const bool lv = (live >> i) & 1;
-            const cx<T> l = csel(lv, cmul(a[i][K], rinv), cx<T>{0, 0});
-            a[i][K] = csel(lv, l, a[i][K]);
-#pragma unroll
-            for (int q = K + 1; q < 8; q++) a[i][q] = cfnma(a[i][q], l, u[q]);
-        }
-    } else if (zc < 0) {
-        zc = j;
-    }
-    return true;
+    for (int i = 0; i < TsShape<T>::RPT; i++) a[i][q] = csel(i == ri, c, a[i][q]);
 }
 
 template <typename T>
 struct TsLds {
-    cx<T> L[2][8][TsShape<T>::S];  // multipliers of the last block, by parity
-                                   // (column-major: lanes read consecutive rows)
-    cx<T> P[TS_NW][8][8];          // per wave: its columns of the block's pivot rows
-    cx<T> U[TS_NW][8][8];          // per wave: its columns of U12 for the block
-    int pv[2][8];                  // pivot stack rows of the block
-    int nb[2];                     // pivots found in the block
+    cx<T> L[2][TS_CW][TsShape<T>::S];  // multipliers of the last block, by parity
+                                       // (column-major: lanes write consecutive rows)
+    cx<T> P[TS_NW][TS_CW][TS_CW];      // per wave: its columns of the block's pivot rows
+    cx<T> U[TS_NW][TS_CW][TS_CW];      // per wave: its columns of U12 for the block
+    int pv[2][TS_CW];                  // pivot stack rows of the block
+    int nb[2];                         // pivots found in the block
 };
 
 __device__ inline void wave_lds_fence()
@@ -198,133 +173,183 @@ __device__ inline void wave_lds_fence()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Partial pivoting on the register stack (S rows x 64 columns; lane l holds
-// rows {i * 64 + l}, wave w columns 8w .. 8w+7); s_lrow[li] is the logical row
-// of stack row li (-1 empty). Blocked by the 8 columns of a wave: the owning
-// wave factors its block alone, publishes the multipliers and pivots, and
-// after one barrier every wave to the right forms its part of the pivot rows
-// (U12 = inv(L11) A12, 8 x 8 per wave) and applies the rank-8 update; the next
-// block's wave does this first and goes on while the others update. Pivots
-// (stack rows) go to s_piv in order; returns their number. Columns whose pivot
-// is exactly zero are left unscaled (as LAPACK), the first is reported in
-// *zero_col.
-template <typename T>
-__device__ __forceinline__ int tslu_gepp(cx<T> (&a)[TsShape<T>::RPT][8], const int* s_lrow,
-                                         int w, int* s_piv, TsLds<T>& sh, int* zero_col)
+// Column K of the block held by this wave: partial pivoting within the wave
+// (largest |re| + |im| by a wave maximum; ties to the lowest lane, then the
+// lowest register row), pivot row by lane reads, rank-1 update of the block's
+// columns right of K. The multipliers (zero for rows that are not live) go
+// straight to the block's LDS slot. Returns false when no live row is left.
+template <typename T, int K>
+__device__ __forceinline__ bool cw_step(StackArr<T>& a, unsigned& live, int kn, int j,
+                                        int (&pvk)[TS_CW], int& zc, cx<T> (*Lk)[TsShape<T>::S])
 {
-    constexpr int RPT = TsShape<T>::RPT, S = TsShape<T>::S;
+    constexpr int RPT = TsShape<T>::RPT;
+    if (K >= kn) return false;
+    const int lane = threadIdx.x & 63;
+    T best = T(-1);
+    int bi = 0;
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+        const T v = ((live >> i) & 1) ? cabs1(sget<T>(a, i, K)) : T(-1);
+        bi = v > best ? i : bi;
+        best = v > best ? v : best;
+    }
+    const T vmax = wave_max(best);
+    if (vmax < T(0)) return false;
+    const unsigned long long hit = __ballot(best == vmax);
+    const int wr = __builtin_amdgcn_readfirstlane(__ffsll((long long)hit) - 1);
+    const int wi = __builtin_amdgcn_readlane(bi, wr);
+    pvk[K] = wi * 64 + wr;
+    cx<T> u[TS_CW];
+#pragma unroll
+    for (int q = K; q < TS_CW; q++) u[q] = readlane_cx(spick<T>(a, wi, q), wr);
+    if (lane == wr) live &= ~(1u << wi);
+    if (vmax > T(0)) {
+        const cx<T> rinv = crecip_fast(u[K]);
+#pragma unroll
+        for (int i = 0; i < RPT; i++) {
+            const bool lv = (live >> i) & 1;
+            const cx<T> l = csel(lv, cmul(sget<T>(a, i, K), rinv), cx<T>{0, 0});
+            sset<T>(a, i, K, csel(lv, l, sget<T>(a, i, K)));
+            Lk[K][i * 64 + lane] = l;
+#pragma unroll
+            for (int q = K + 1; q < TS_CW; q++) sset<T>(a, i, q, cfnma(sget<T>(a, i, q), l, u[q]));
+        }
+    } else {
+        if (zc < 0) zc = j;
+#pragma unroll
+        for (int i = 0; i < RPT; i++) Lk[K][i * 64 + lane] = cx<T>{0, 0};
+    }
+    return true;
+}
+
+// Partial pivoting on the register stack (S rows x 64 columns; lane l holds
+// rows {i * 64 + l}, wave w columns 4w .. 4w+3); s_lrow[li] is the logical row
+// of stack row li (-1 empty). Blocked by the 4 columns of a wave: the owning
+// wave factors its block alone (its multipliers going to LDS as they are
+// formed), and after one barrier every wave to the right forms its part of the
+// pivot rows (U12 = inv(L11) A12, 4 x 4 per wave) and applies the rank-4
+// update; the next block's wave does this first and goes on while the others
+// update. With `keep_pivot_rows` the pivot rows also take their U12 entries
+// (the root needs the factored winners; other levels only their indices).
+// Pivots (stack rows) go to s_piv in order; returns their number. Columns
+// whose pivot is exactly zero are left unscaled (as LAPACK), the first is
+// reported in *zero_col.
+template <typename T>
+__device__ __forceinline__ int tslu_gepp(StackArr<T>& a, const int* s_lrow, int w, int* s_piv,
+                                         TsLds<T>& sh, int* zero_col, bool keep_pivot_rows)
+{
+    constexpr int RPT = TsShape<T>::RPT;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     unsigned live = 0;
 #pragma unroll
     for (int i = 0; i < RPT; i++)
         if (s_lrow[i * 64 + lane] >= 0) live |= 1u << i;
     int npiv = 0;
-    const int nblk = (w + 7) / 8;
+    const int nblk = (w + TS_CW - 1) / TS_CW;
     for (int jb = 0; jb < nblk; jb++) {
         const int par = jb & 1;
         if (wave == jb) {
-            const int kn = min(8, w - jb * 8);
-            const unsigned live0 = live;
-            int pvk[8] = {-1, -1, -1, -1, -1, -1, -1, -1};
+            // the owning wave's chain is the critical path: issue it first on
+            // its SIMD (instruction priority, this wave only)
+            __builtin_amdgcn_s_setprio(3);
+            const int kn = min(TS_CW, w - jb * TS_CW);
+            int pvk[TS_CW] = {-1, -1, -1, -1};
             int zc = -1, nb = 0;
             bool go = true;
-#define TS_STEP(K)                                                      \
-    if (go) {                                                           \
-        go = cw_step<T, K>(a, live, kn, jb * 8 + K, pvk, zc);           \
-        nb += go ? 1 : 0;                                               \
+#define TS_STEP(K)                                                           \
+    if (go) {                                                                \
+        go = cw_step<T, K>(a, live, kn, jb * TS_CW + K, pvk, zc, sh.L[par]); \
+        nb += go ? 1 : 0;                                                    \
     }
+            static_assert(TS_CW == 4, "column steps below");
             TS_STEP(0) TS_STEP(1) TS_STEP(2) TS_STEP(3)
-            TS_STEP(4) TS_STEP(5) TS_STEP(6) TS_STEP(7)
 #undef TS_STEP
-            // publish multipliers: rows live at the block start, a pivot row
-            // only its multipliers from before it was chosen
+            // columns without a pivot (past the panel, or no live row left)
+            for (int q = nb; q < TS_CW; q++)
 #pragma unroll
-            for (int i = 0; i < RPT; i++) {
-                const int li = i * 64 + lane;
-                int rank = 8;
-#pragma unroll
-                for (int t = 0; t < 8; t++) rank = (t < nb && pvk[t] == li) ? t : rank;
-                const bool lv0 = (live0 >> i) & 1;
-#pragma unroll
-                for (int q = 0; q < 8; q++)
-                    sh.L[par][q][li] = csel(lv0 && q < rank && q < kn, a[i][q], cx<T>{0, 0});
-            }
+                for (int i = 0; i < RPT; i++) sh.L[par][q][i * 64 + lane] = cx<T>{0, 0};
             if (lane == 0) {
 #pragma unroll
-                for (int t = 0; t < 8; t++) {
+                for (int t = 0; t < TS_CW; t++) {
                     sh.pv[par][t] = pvk[t];
-                    if (t < nb) s_piv[jb * 8 + t] = pvk[t];
+                    if (t < nb) s_piv[jb * TS_CW + t] = pvk[t];
                 }
                 sh.nb[par] = nb;
                 if (zero_col && zc >= 0 && *zero_col < 0) *zero_col = zc;
             }
+            __builtin_amdgcn_s_setprio(1);
         }
         __syncthreads();
         const int nb = sh.nb[par];
         npiv += nb;
+        int prs[TS_CW];
+#pragma unroll
+        for (int t = 0; t < TS_CW; t++) prs[t] = sh.pv[par][t];
         if (wave != jb) {
-            for (int t = 0; t < nb; t++) {
-                const int pr = sh.pv[par][t];
-                if (lane == (pr & 63)) live &= ~(1u << (pr >> 6));
-            }
+#pragma unroll
+            for (int t = 0; t < TS_CW; t++)
+                if (t < nb && lane == (prs[t] & 63)) live &= ~(1u << (prs[t] >> 6));
         }
         if (wave > jb && nb > 0) {
+            if (wave == jb + 1) __builtin_amdgcn_s_setprio(2);  // next owner updates first
             // this wave's columns of the block's pivot rows
-            for (int t = 0; t < nb; t++) {
-                const int pr = sh.pv[par][t];
-                if (lane == (pr & 63)) {
 #pragma unroll
-                    for (int q = 0; q < 8; q++) sh.P[wave][t][q] = pick_row<T, RPT>(a, pr >> 6, q);
+            for (int t = 0; t < TS_CW; t++) {
+                if (t < nb) {
+                    const int ri = __builtin_amdgcn_readfirstlane(prs[t] >> 6);
+                    if (lane == (prs[t] & 63)) {
+#pragma unroll
+                        for (int q = 0; q < TS_CW; q++) sh.P[wave][t][q] = spick<T>(a, ri, q);
+                    }
                 }
             }
             wave_lds_fence();
-            if (lane < 8) {  // U12 column `lane`: forward substitution with L11
-                cx<T> u[8];
+            if (lane < TS_CW) {  // U12 column `lane`: forward substitution with L11
+                cx<T> u[TS_CW];
 #pragma unroll
-                for (int t = 0; t < 8; t++) {
+                for (int t = 0; t < TS_CW; t++) {
                     if (t < nb) {
-                        const int pr = sh.pv[par][t];
                         cx<T> v = sh.P[wave][t][lane];
 #pragma unroll
-                        for (int s2 = 0; s2 < t; s2++) v = cfnma(v, sh.L[par][s2][pr], u[s2]);
+                        for (int s2 = 0; s2 < t; s2++) v = cfnma(v, sh.L[par][s2][prs[t]], u[s2]);
                         u[t] = v;
                         sh.U[wave][t][lane] = v;
                     }
                 }
             }
             wave_lds_fence();
-            // rank-nb update; pivot rows then take their U12 entries
+            // rank-nb update (multipliers of non-live rows are zero)
 #pragma unroll
-            for (int t = 0; t < 8; t++) {
+            for (int t = 0; t < TS_CW; t++) {
                 if (t < nb) {
                     cx<T> l[RPT];
 #pragma unroll
                     for (int i = 0; i < RPT; i++) l[i] = sh.L[par][t][i * 64 + lane];
 #pragma unroll
-                    for (int q = 0; q < 8; q++) {
+                    for (int q = 0; q < TS_CW; q++) {
                         const cx<T> u = sh.U[wave][t][q];
 #pragma unroll
-                        for (int i = 0; i < RPT; i++) a[i][q] = cfnma(a[i][q], l[i], u);
+                        for (int i = 0; i < RPT; i++) sset<T>(a, i, q, cfnma(sget<T>(a, i, q), l[i], u));
                     }
                 }
             }
-            for (int t = 0; t < nb; t++) {
-                const int pr = sh.pv[par][t];
-                if (lane == (pr & 63)) {
-                    const int ri = pr >> 6;
+            if (keep_pivot_rows) {
 #pragma unroll
-                    for (int q = 0; q < 8; q++) {
-                        const cx<T> u = sh.U[wave][t][q];
+                for (int t = 0; t < TS_CW; t++) {
+                    if (t < nb) {
+                        const int ri = __builtin_amdgcn_readfirstlane(prs[t] >> 6);
+                        if (lane == (prs[t] & 63)) {
 #pragma unroll
-                        for (int i = 0; i < RPT; i++) a[i][q] = csel(i == ri, u, a[i][q]);
+                            for (int q = 0; q < TS_CW; q++) sput<T>(a, ri, q, sh.U[wave][t][q]);
+                        }
                     }
                 }
             }
+            if (wave == jb + 1) __builtin_amdgcn_s_setprio(1);
         }
-        if (nb < min(8, w - jb * 8)) break;  // no live row left
+        if (nb < min(TS_CW, w - jb * TS_CW)) break;  // no live row left
     }
     __syncthreads();
-    (void)S;
     return npiv;
 }
 
@@ -340,6 +365,9 @@ __global__ __launch_bounds__(TS_NT) void k_tslu(TsluArgs<T> p)
     __shared__ int s_last, s_zero;
 
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    // the panel chain is the critical path of the factorisation: its waves issue
+    // ahead of trailing-update waves that share their SIMDs
+    __builtin_amdgcn_s_setprio(1);
     const TsluTree& tr = p.tree;
     int level = 0, node = blockIdx.x;
     // leaf: consecutive candidate rows
@@ -350,20 +378,20 @@ __global__ __launch_bounds__(TS_NT) void k_tslu(TsluArgs<T> p)
     }
     if (t == 0) s_zero = -1;
     __syncthreads();
-    cx<T> a[TS_RPT][8];
+    StackArr<T> a;
     for (;;) {
         // load the stack: current values of the rows in s_lrow
 #pragma unroll
         for (int i = 0; i < TS_RPT; i++) {
             const int lr = s_lrow[i * 64 + lane];
             const cx<T>* src =
-                lr >= 0 ? p.A + (long)p.perm[lr] * p.ld + p.c0 + wave * 8 : nullptr;
+                lr >= 0 ? p.A + (long)p.perm[lr] * p.ld + p.c0 + wave * TS_CW : nullptr;
 #pragma unroll
-            for (int q = 0; q < 8; q++)
-                a[i][q] = (src && wave * 8 + q < p.w) ? src[q] : cx<T>{0, 0};
+            for (int q = 0; q < TS_CW; q++)
+                sset<T>(a, i, q, (src && wave * TS_CW + q < p.w) ? src[q] : cx<T>{0, 0});
         }
         const bool root = level == tr.nlev - 1;
-        const int npiv = tslu_gepp<T>(a, s_lrow, p.w, s_piv, sh, root ? &s_zero : nullptr);
+        const int npiv = tslu_gepp<T>(a, s_lrow, p.w, s_piv, sh, root ? &s_zero : nullptr, root);
         if (root) {
             // winners' factored rows L11 \ U11 into the matrix (physical rows as
             // of the start of the panel), then the row map
@@ -375,10 +403,10 @@ __global__ __launch_bounds__(TS_NT) void k_tslu(TsluArgs<T> p)
             for (int i = 0; i < TS_RPT; i++) {
                 const int li = i * 64 + lane;
                 if (s_rank[li] >= 0) {
-                    cx<T>* dst = p.A + (long)p.perm[s_lrow[li]] * p.ld + p.c0 + wave * 8;
+                    cx<T>* dst = p.A + (long)p.perm[s_lrow[li]] * p.ld + p.c0 + wave * TS_CW;
 #pragma unroll
-                    for (int q = 0; q < 8; q++)
-                        if (wave * 8 + q < p.w) dst[q] = a[i][q];
+                    for (int q = 0; q < TS_CW; q++)
+                        if (wave * TS_CW + q < p.w) dst[q] = sget<T>(a, i, q);
                 }
             }
             // Row map: logical row c0 + j <- winner j. Target rows that are not
@@ -466,6 +494,7 @@ __global__ __launch_bounds__(256) void k_l21(cx<T>* A, long ld, const int* perm,
     __shared__ cx<T> s_rd[64];  // reciprocals of U's diagonal (zero for a zero pivot)
     __shared__ int s_pm[64];
     const int t = threadIdx.x, rg = t >> 3, cg = t & 7, lane = t & 63;
+    __builtin_amdgcn_s_setprio(1);  // on the panel's critical path
     if (t < 64) s_pm[t] = t < w ? perm[c0 + t] : 0;
     cx<T> a[2][8];
     cx<T>* rowp[2];
