@@ -123,6 +123,8 @@ def main():
                          "2 Z_L first, 3 Z_L-block pivoting")
     ap.add_argument("--plan-fp64", type=int, default=2,
                     help="LU plan of the FP64 form (default 2, Z_L first)")
+    ap.add_argument("--no-lookahead", action="store_true",
+                    help="factor the steps strictly in order on one stream")
     ap.add_argument("--autotune", action="store_true",
                     help="time every GEMM variant first and install the fastest")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -148,6 +150,8 @@ def main():
     kap = SIGMA + s * EPSR * T.EPS0
     sw = T.Sweep(e, nodes)
     L = T.lib()
+    if args.no_lookahead:
+        T.set_lookahead(False)
     tuned = "defaults"
     if args.autotune:
         tuned = {"fp32": T.autotune(T.PREC_MIXED), "fp64": T.autotune(T.PREC_FP64)}

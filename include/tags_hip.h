@@ -225,6 +225,11 @@ int tags_immittance_solve(const TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t num_ele
 enum tags_panel { TAGS_PANEL_TOURNAMENT = 0, TAGS_PANEL_PARTIAL = 1 };
 int tags_set_panel(int kind);
 
+/* Look-ahead of every later LU factorisation in the process: 1 (default)
+ * factors the next panel on a second stream while the trailing update of the
+ * current step runs; 0 runs the steps strictly in order on one stream. */
+int tags_set_lookahead(int on);
+
 /* Library version string. */
 const char *tags_version(void);
 

@@ -281,6 +281,41 @@ def test_c2_sweep_vs_oracle_and_mixed(plan):
     sw.close()
 
 
+@pytest.mark.parametrize("plan", [0, 1, 2, 3])
+def test_lu_modes_agree(plan):
+    """Config 2 sweep under the three factorisation schedules: look-ahead
+    (default; the next panel on a second stream with its own row map),
+    strictly in order, and partial pivoting over the whole panel (cooperative
+    launch, in order). Every plan has a phase boundary where the next panel
+    permutes rows beyond the current step's row range; all three must give the
+    same Z_h, and the mixed form must converge without the FP64 fallback."""
+    rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
+    f = O.logspace(2, 7, 400)[::40]
+    s = 2j * np.pi * f
+    kap = 1e-3 + s * 15.0 * T.EPS0
+    res = {}
+    try:
+        for mode in ("lookahead", "inorder", "partial"):
+            T.set_lookahead(mode == "lookahead")
+            T.set_panel(T.PANEL_PARTIAL if mode == "partial" else T.PANEL_TOURNAMENT)
+            sw = T.Sweep(e, nodes)
+            sw.set_plan(plan)
+            u64, _ = sw.solve(s, kap, 0, T.PREC_FP64)
+            umx, stmx = sw.solve(s, kap, 0, T.PREC_MIXED)
+            sw.close()
+            if plan != 3:  # plan 3 pivots within the Z_L block only
+                assert all(q["precision_used"] == 1 for q in stmx), (mode, stmx)
+            res[mode] = (u64, umx)
+    finally:
+        T.set_lookahead(True)
+        T.set_panel(T.PANEL_TOURNAMENT)
+    ref = res["lookahead"][0]
+    tol = 1e-6 if plan == 3 else 1e-9
+    for mode, (u64, umx) in res.items():
+        assert np.max(np.abs(u64 - ref) / np.abs(ref)) <= tol, mode
+        assert np.max(np.abs(umx - ref) / np.abs(ref)) <= max(tol, 1e-8), mode
+
+
 def test_c3_mixed_vs_fp64_property():
     """Full 60 x 60 m grid (N = 21,421): mixed vs FP64 at the band edges, and
     the structured factorisation plan vs the dense one."""

@@ -147,6 +147,7 @@ def lib():
         "tags_gemm_variants": ([I], I),
         "tags_gemm_select": ([I, I, I], I),
         "tags_set_panel": ([I], I),
+        "tags_set_lookahead": ([I], I),
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
         "tags_sweep_assemble": ([P, C, C, D, P], I),
@@ -288,6 +289,11 @@ def set_panel(kind):
     rc = lib().tags_set_panel(kind)
     if rc:
         raise RuntimeError("tags_set_panel failed: %d" % rc)
+
+
+def set_lookahead(on=True):
+    """Look-ahead of every later LU in the process (on by default)."""
+    lib().tags_set_lookahead(1 if on else 0)
 
 
 def autotune(precision):

@@ -379,6 +379,12 @@ int tags_set_panel(int kind)
     return 0;
 }
 
+int tags_set_lookahead(int on)
+{
+    lu_set_lookahead(on != 0);
+    return 0;
+}
+
 int tags_gemm_select(int precision, int big_variant, int small_variant)
 {
     return gemm_set_variants(precision, big_variant, small_variant);

@@ -156,6 +156,8 @@ int launch_tslu(cx<T>* A, long ld, int* perm, int c0, int w, int row_end, int ca
 enum PanelKind { PANEL_TOURNAMENT = 0, PANEL_PARTIAL = 1 };
 void lu_set_panel_kind(int kind);
 int lu_panel_kind();
+// Look-ahead (next panel on a second stream during the trailing update): on by default
+void lu_set_lookahead(bool on);
 
 // Column phases of a factorisation (see lu.hip).
 struct LUPhase {

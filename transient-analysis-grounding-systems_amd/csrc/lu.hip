@@ -642,6 +642,9 @@ static std::atomic<int> g_panel_kind{PANEL_TOURNAMENT};
 void lu_set_panel_kind(int kind) { g_panel_kind.store(kind == PANEL_PARTIAL ? 1 : 0); }
 int lu_panel_kind() { return g_panel_kind.load(); }
 
+static std::atomic<bool> g_lookahead{true};
+void lu_set_lookahead(bool on) { g_lookahead.store(on); }
+
 template <typename T>
 static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
                  int c0, int cols, int row_end, int cand_end)
@@ -847,7 +850,8 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     // first column, so the rest of step s (pivot rows before it, the same set
     // of physical rows after it, other columns) is unaffected by its changes.
     // The cooperative panel needs the whole device and is not overlapped.
-    const bool overlap = g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT;
+    const bool overlap = g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT &&
+                         g_lookahead.load(std::memory_order_relaxed);
     int* pcur = w.perm;
     int* palt = w.perm_alt;
     if (!steps.empty()) {
@@ -865,7 +869,17 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             re[0] = ph.x_end;
             re[1] = rb[1];
         }
-        if (k + 1 < steps.size()) {
+        if (k + 1 < steps.size() && !overlap) {
+            // in order: the whole update of this step, then the next panel (the
+            // next panel may permute rows beyond this step's row range, which
+            // the rest of this update still addresses through the row map)
+            for (int q = 0; q < 2; q++)
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, rb[q], re[q])))
+                    return rc;
+            const LUStep& nx = steps[k + 1];
+            if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, w.tmp, plan->phases[nx.phase], nx)))
+                return rc;
+        } else if (k + 1 < steps.size()) {
             const LUStep& nx = steps[k + 1];
             const int nb = nx.j, ne = nx.j + nx.jb;
             // columns of the next panel first
@@ -873,19 +887,14 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                 const int cb = std::max(rb[q], nb), ce = std::min(re[q], ne);
                 if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, cb, ce))) return rc;
             }
-            hipStream_t ps = st;
-            if (overlap) {
-                TAGS_HIP(hipEventRecord(w.ev_left, st));
-                TAGS_HIP(hipStreamWaitEvent(w.s2, w.ev_left, 0));
-                TAGS_HIP(hipMemcpyAsync(palt, pcur, sizeof(int) * n, hipMemcpyDeviceToDevice,
-                                        w.s2));
-                ps = w.s2;
-            }
-            int* pnext = overlap ? palt : pcur;
-            if ((rc = step_panels<T>(w, pr, ps, A, ld, pnext, overlap ? w.tmp2 : w.tmp,
-                                     plan->phases[nx.phase], nx)))
+            // the next panel on the second stream, with its own copy of the row map
+            TAGS_HIP(hipEventRecord(w.ev_left, st));
+            TAGS_HIP(hipStreamWaitEvent(w.s2, w.ev_left, 0));
+            TAGS_HIP(hipMemcpyAsync(palt, pcur, sizeof(int) * n, hipMemcpyDeviceToDevice, w.s2));
+            if ((rc = step_panels<T>(w, pr, w.s2, A, ld, palt, w.tmp2, plan->phases[nx.phase],
+                                     nx)))
                 return rc;
-            if (overlap) TAGS_HIP(hipEventRecord(w.ev_panel, w.s2));
+            TAGS_HIP(hipEventRecord(w.ev_panel, w.s2));
             // the remaining columns of this step
             for (int q = 0; q < 2; q++) {
                 if (re[q] <= rb[q]) continue;
@@ -896,10 +905,8 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                 if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, lo2, hi2)))
                     return rc;
             }
-            if (overlap) {
-                TAGS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
-                std::swap(pcur, palt);
-            }
+            TAGS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
+            std::swap(pcur, palt);
         } else {
             for (int q = 0; q < 2; q++)
                 if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, rb[q], re[q])))
