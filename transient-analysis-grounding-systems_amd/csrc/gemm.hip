@@ -41,20 +41,22 @@ struct Mfma<float> {
     static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <typename T, int BM, int BN, int BK, bool M3>
+template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN>
 __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
 {
     using V4 = typename Mfma<T>::v4;
-    constexpr int WM = BM / 2, WN = BN / 2, TM = WM / 16, TN = WN / 16;
+    constexpr int NT = 64 * WGM * WGN;
+    constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
     constexpr int SKEW = 128 / (int)sizeof(cx<T>);
     constexpr int SA = BM + SKEW, SB = BN + SKEW;
-    constexpr int LA = BM * BK / 256, LB = BK * BN / 256;
-    static_assert(LA * 256 == BM * BK && LB * 256 == BK * BN, "tile/thread mismatch");
+    constexpr int LA = BM * BK / NT, LB = BK * BN / NT;
+    static_assert(LA * NT == BM * BK && LB * NT == BK * BN, "tile/thread mismatch");
+    static_assert(TM * 16 == WM && TN * 16 == WN, "wave tile");
     __shared__ cx<T> As[2][BK * SA];
     __shared__ cx<T> Bs[2][BK * SB];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WGN, wn = wave % WGN;
     const int tiles_n = (g.N + BN - 1) / BN;
     const int tiles_m = (g.M + BM - 1) / BM;
     const int ntiles = tiles_n * tiles_m;
@@ -74,7 +76,7 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
     int acol[LA], arw[LA];
 #pragma unroll
     for (int q = 0; q < LA; q++) {
-        const int e = tid + 256 * q;
+        const int e = tid + NT * q;
         const int r = e / BK, c = e % BK, gr = m0 + r;
         arw[q] = r;
         acol[q] = c;
@@ -84,7 +86,7 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
     int bcol[LB], brw[LB];
 #pragma unroll
     for (int q = 0; q < LB; q++) {
-        const int e = tid + 256 * q;
+        const int e = tid + NT * q;
         brw[q] = e / BN;
         bcol[q] = (n0 + e % BN) < g.N ? e % BN : -1;
     }
@@ -112,7 +114,7 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
         _Pragma("unroll") for (int q = 0; q < LA; q++) As[buf_][acol[q] * SA + arw[q]] = ra[q]; \
         _Pragma("unroll") for (int q = 0; q < LB; q++)                                    \
         {                                                                                 \
-            const int e = tid + 256 * q;                                                  \
+            const int e = tid + NT * q;                                                   \
             Bs[buf_][brw[q] * SB + (e % BN)] = rb[q];                                     \
         }                                                                                 \
     }
@@ -206,45 +208,50 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
         }
 }
 
-template <typename T, int BM, int BN, int BK, bool M3>
-__global__ __launch_bounds__(256) void k_cgemm(GemmArgs<T> g)
+template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN>
+__global__ __launch_bounds__(64 * WGM * WGN) void k_cgemm(GemmArgs<T> g)
 {
-    cgemm_block<T, BM, BN, BK, M3>(g);
+    cgemm_block<T, BM, BN, BK, M3, WGM, WGN>(g);
 }
 
 // The same product under its own name for the LU's trailing update
 // A22 -= L21 U12, so that kernel traces and counters attribute it apart from
 // the other products (panel rows, triangular solves).
-template <typename T, int BM, int BN, int BK, bool M3>
-__global__ __launch_bounds__(256) void k_trailing_update(GemmArgs<T> g)
+template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN>
+__global__ __launch_bounds__(64 * WGM * WGN) void k_trailing_update(GemmArgs<T> g)
 {
-    cgemm_block<T, BM, BN, BK, M3>(g);
+    cgemm_block<T, BM, BN, BK, M3, WGM, WGN>(g);
 }
 
 // ---------------------------------------------------------------------------
 // variants and selection
 // ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, bool M3>
+template <typename T, int BM, int BN, int BK, bool M3, int WGM = 2, int WGN = 2>
 static int launch_v(const GemmArgs<T>& g, hipStream_t st, bool trailing)
 {
     const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
     const long grid = (tiles + 7) / 8 * 8;
+    const dim3 block(64 * WGM * WGN);
     if (trailing)
-        hipLaunchKernelGGL((k_trailing_update<T, BM, BN, BK, M3>), dim3((unsigned)grid),
-                           dim3(256), 0, st, g);
+        hipLaunchKernelGGL((k_trailing_update<T, BM, BN, BK, M3, WGM, WGN>), dim3((unsigned)grid),
+                           block, 0, st, g);
     else
-        hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3>), dim3((unsigned)grid), dim3(256), 0,
-                           st, g);
+        hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3, WGM, WGN>), dim3((unsigned)grid), block,
+                           0, st, g);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
-// Variant table: 0..3 four-multiply, 4..5 three-multiply (Gauss).
+// Variant table (tile BM x BN, K chunk BK, four- or three-multiply complex
+// product, waves WGM x WGN).
 template <typename T>
 struct GemmTable;
+struct GemmShape {
+    int bm, bn;
+};
 template <>
 struct GemmTable<float> {
-    static constexpr int count = 6;
+    static constexpr int count = 18;
     static int launch(int v, const GemmArgs<float>& g, hipStream_t st, bool tr)
     {
         switch (v) {
@@ -253,33 +260,72 @@ struct GemmTable<float> {
         case 2: return launch_v<float, 128, 64, 16, false>(g, st, tr);
         case 3: return launch_v<float, 64, 128, 16, false>(g, st, tr);
         case 4: return launch_v<float, 128, 128, 16, true>(g, st, tr);
-        default: return launch_v<float, 64, 64, 16, true>(g, st, tr);
+        case 5: return launch_v<float, 64, 64, 16, true>(g, st, tr);
+        case 6: return launch_v<float, 128, 128, 16, true, 2, 4>(g, st, tr);
+        case 7: return launch_v<float, 128, 128, 16, true, 4, 2>(g, st, tr);
+        case 8: return launch_v<float, 128, 128, 8, true, 2, 4>(g, st, tr);
+        case 9: return launch_v<float, 256, 128, 8, true, 4, 4>(g, st, tr);
+        case 10: return launch_v<float, 128, 64, 16, true, 4, 2>(g, st, tr);
+        case 11: return launch_v<float, 128, 128, 16, true, 4, 4>(g, st, tr);
+        case 12: return launch_v<float, 128, 128, 8, true, 4, 4>(g, st, tr);
+        case 13: return launch_v<float, 64, 128, 16, true, 2, 4>(g, st, tr);
+        case 14: return launch_v<float, 128, 64, 8, true, 4, 2>(g, st, tr);
+        case 15: return launch_v<float, 128, 128, 8, true, 4, 2>(g, st, tr);
+        case 16: return launch_v<float, 256, 64, 16, true, 8, 2>(g, st, tr);
+        default: return launch_v<float, 64, 64, 8, true, 2, 2>(g, st, tr);
         }
     }
-    static int tile_m(int v) { return (v == 1 || v == 2 || v == 4) ? 128 : 64; }
-    static int tile_n(int v) { return (v == 1 || v == 3 || v == 4) ? 128 : 64; }
+    static GemmShape shape(int v)
+    {
+        static const GemmShape t[count] = {{64, 64},   {128, 128}, {128, 64},  {64, 128},
+                                           {128, 128}, {64, 64},   {128, 128}, {128, 128},
+                                           {128, 128}, {256, 128}, {128, 64},  {128, 128},
+                                           {128, 128}, {64, 128},  {128, 64},  {128, 128},
+                                           {256, 64},  {64, 64}};
+        return t[v];
+    }
+    static int tile_m(int v) { return shape(v).bm; }
+    static int tile_n(int v) { return shape(v).bn; }
 };
 template <>
 struct GemmTable<double> {
-    static constexpr int count = 4;
+    static constexpr int count = 13;
     static int launch(int v, const GemmArgs<double>& g, hipStream_t st, bool tr)
     {
         switch (v) {
         case 0: return launch_v<double, 64, 64, 8, false>(g, st, tr);
         case 1: return launch_v<double, 128, 64, 8, false>(g, st, tr);
         case 2: return launch_v<double, 64, 128, 8, false>(g, st, tr);
-        default: return launch_v<double, 128, 64, 16, false>(g, st, tr);
+        case 3: return launch_v<double, 128, 64, 16, false>(g, st, tr);
+        case 4: return launch_v<double, 128, 128, 8, false, 2, 4>(g, st, tr);
+        case 5: return launch_v<double, 128, 128, 8, false, 4, 2>(g, st, tr);
+        case 6: return launch_v<double, 128, 64, 8, false, 4, 2>(g, st, tr);
+        case 7: return launch_v<double, 128, 128, 8, false, 4, 4>(g, st, tr);
+        case 8: return launch_v<double, 128, 64, 16, false, 4, 2>(g, st, tr);
+        case 9: return launch_v<double, 64, 128, 8, false, 2, 4>(g, st, tr);
+        case 10: return launch_v<double, 256, 64, 8, false, 4, 2>(g, st, tr);
+        case 11: return launch_v<double, 64, 64, 16, false, 2, 2>(g, st, tr);
+        default: return launch_v<double, 64, 64, 4, false, 2, 2>(g, st, tr);
         }
     }
-    static int tile_m(int v) { return (v == 1 || v == 3) ? 128 : 64; }
-    static int tile_n(int v) { return v == 2 ? 128 : 64; }
+    static GemmShape shape(int v)
+    {
+        static const GemmShape t[count] = {{64, 64},  {128, 64},  {64, 128}, {128, 64},
+                                           {128, 128}, {128, 128}, {128, 64}, {128, 128},
+                                           {128, 64}, {64, 128},  {256, 64},  {64, 64},
+                                           {64, 64}};
+        return t[v];
+    }
+    static int tile_m(int v) { return shape(v).bm; }
+    static int tile_n(int v) { return shape(v).bn; }
 };
 
 // Selected variant for large and small problems, [double, float]. The
-// defaults are the autotuner's choice on MI355X (64 x 64 tiles; three-multiply
-// for complex64); tags_autotune may replace them.
-static std::atomic<int> g_big_variant[2] = {0, 5};
-static std::atomic<int> g_small_variant[2] = {0, 5};
+// defaults are the fastest on MI355X at the LU's shapes (K = 128): 128 x 64
+// tiles of 8 waves (32 x 32 per wave; three-multiply for complex64) for large
+// products, 64 x 64 tiles for small ones; tags_autotune may replace them.
+static std::atomic<int> g_big_variant[2] = {6, 14};
+static std::atomic<int> g_small_variant[2] = {12, 17};
 
 template <typename T>
 static constexpr int prec_index()
