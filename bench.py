@@ -123,6 +123,8 @@ def main():
                          "2 Z_L first, 3 Z_L-block pivoting")
     ap.add_argument("--plan-fp64", type=int, default=2,
                     help="LU plan of the FP64 form (default 2, Z_L first)")
+    ap.add_argument("--update-chunks", type=int, default=None,
+                    help="launches per step for the trailing update beside the panel")
     ap.add_argument("--no-lookahead", action="store_true",
                     help="factor the steps strictly in order on one stream")
     ap.add_argument("--autotune", action="store_true",
@@ -152,6 +154,8 @@ def main():
     L = T.lib()
     if args.no_lookahead:
         T.set_lookahead(False)
+    if args.update_chunks is not None:
+        T.set_update_chunks(args.update_chunks)
     tuned = "defaults"
     if args.autotune:
         tuned = {"fp32": T.autotune(T.PREC_MIXED), "fp64": T.autotune(T.PREC_FP64)}
@@ -209,7 +213,8 @@ def main():
     out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"],
                        "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
                        "traffic": measured_traffic(),
-                       "kernel": "k_trailing_update<float, 64x64 tile, 3-multiply> (A22 -= L21 U12, K=128)",
+                       "kernel": "k_trailing_update<float, 128x64 tile, 8 waves, 3-multiply> "
+                                 "(A22 -= L21 U12, K=128)",
                        "launches": tm_mx["trailing_launches"],
                        "avg_launch_ms": tm_mx["trailing_ms"] / max(1, tm_mx["trailing_launches"])}
     out["lu_algorithmic_tflops"] = lu_flops(N) * K / dt_mx / 1e12

@@ -230,6 +230,12 @@ int tags_set_panel(int kind);
  * current step runs; 0 runs the steps strictly in order on one stream. */
 int tags_set_lookahead(int on);
 
+/* Number of launches (split by columns) of the part of each step's trailing
+ * update that runs beside the look-ahead panel (default 2). Each launch
+ * boundary lets panel work-groups waiting for a CU start. Returns 0, or
+ * TAGS_ERR_ARG for n < 1. */
+int tags_set_update_chunks(int n);
+
 /* Library version string. */
 const char *tags_version(void);
 

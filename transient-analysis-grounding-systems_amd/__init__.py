@@ -148,6 +148,7 @@ def lib():
         "tags_gemm_select": ([I, I, I], I),
         "tags_set_panel": ([I], I),
         "tags_set_lookahead": ([I], I),
+        "tags_set_update_chunks": ([I], I),
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
         "tags_sweep_assemble": ([P, C, C, D, P], I),
@@ -294,6 +295,13 @@ def set_panel(kind):
 def set_lookahead(on=True):
     """Look-ahead of every later LU in the process (on by default)."""
     lib().tags_set_lookahead(1 if on else 0)
+
+
+def set_update_chunks(n):
+    """Launches each step's look-ahead trailing update is split into."""
+    rc = lib().tags_set_update_chunks(int(n))
+    if rc:
+        raise RuntimeError("tags_set_update_chunks failed: %d" % rc)
 
 
 def autotune(precision):

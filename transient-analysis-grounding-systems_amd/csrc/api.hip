@@ -385,6 +385,13 @@ int tags_set_lookahead(int on)
     return 0;
 }
 
+int tags_set_update_chunks(int n)
+{
+    if (n < 1) return TAGS_ERR_ARG;
+    lu_set_update_chunks(n);
+    return 0;
+}
+
 int tags_gemm_select(int precision, int big_variant, int small_variant)
 {
     return gemm_set_variants(precision, big_variant, small_variant);

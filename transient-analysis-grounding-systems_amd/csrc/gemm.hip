@@ -323,9 +323,10 @@ struct GemmTable<double> {
 // Selected variant for large and small problems, [double, float]. The
 // defaults are the fastest on MI355X at the LU's shapes (K = 128): 128 x 64
 // tiles of 8 waves (32 x 32 per wave; three-multiply for complex64) for large
-// products, 64 x 64 tiles for small ones; tags_autotune may replace them.
+// products, 64 x 64 tiles of 4 waves for small ones; tags_autotune may replace
+// them.
 static std::atomic<int> g_big_variant[2] = {6, 14};
-static std::atomic<int> g_small_variant[2] = {12, 17};
+static std::atomic<int> g_small_variant[2] = {0, 5};
 
 template <typename T>
 static constexpr int prec_index()

@@ -158,6 +158,8 @@ void lu_set_panel_kind(int kind);
 int lu_panel_kind();
 // Look-ahead (next panel on a second stream during the trailing update): on by default
 void lu_set_lookahead(bool on);
+// Launches the look-ahead trailing update of a step is split into (by columns)
+void lu_set_update_chunks(int n);
 
 // Column phases of a factorisation (see lu.hip).
 struct LUPhase {
