@@ -134,8 +134,8 @@ def main():
                          "reference kernel for scaling FETCH_SIZE / WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py: this build runs on one GPU; every frequency sample is "
-              "independent, see the sharding paragraph of DESIGN.md", file=sys.stderr)
+        print("bench.py: measures one GPU only; frequency samples are independent, so "
+              "more GPUs would each take a share of the samples", file=sys.stderr)
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)

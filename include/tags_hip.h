@@ -74,7 +74,7 @@ TAGS_COMPLEX self_integral(const Electrode *sender);
  * higher order. INTG_NONE returns the distance between middle points. max_eval,
  * req_abs_error and req_rel_error are accepted; the rule meets a relative
  * error far below any req_rel_error >= 1e-10. Returns 0, -10 for an unknown
- * type, -11 for INTG_SINGLE / INTG_MHEM (not part of this build). */
+ * type, -11 for INTG_SINGLE / INTG_MHEM (not supported by this library). */
 int integral(const Electrode *sender, const Electrode *receiver, TAGS_COMPLEX gamma,
              size_t max_eval, double req_abs_error, double req_rel_error,
              int integration_type, double result[2], double error[2]);
@@ -106,9 +106,10 @@ int fill_impedance_imm(TAGS_COMPLEX *wg, const TAGS_COMPLEX *zl, const TAGS_COMP
                        size_t num_electrodes, size_t num_nodes);
 
 /* src/linalg.h:79-81. Solves W x = ie for the symmetric W whose lower triangle
- * is stored; x replaces ie ([U; I_L; I_T]). Factorisation: complex LU with
- * partial pivoting in FP64 on the GPU (the reference uses LAPACK zsysv).
- * Returns 0, or i > 0 when U(i,i) is exactly zero. W itself is not modified. */
+ * is stored; x replaces ie ([U; I_L; I_T]). Factorisation: complex LU in FP64
+ * on the GPU with tournament pivoting (or partial pivoting, tags_set_panel);
+ * the reference uses LAPACK zsysv. Returns 0, or i > 0 when U(i,i) is exactly
+ * zero. W itself is not modified. */
 int solve_immittance(TAGS_COMPLEX *wg, TAGS_COMPLEX *ie, size_t num_electrodes,
                      size_t num_nodes);
 
@@ -166,15 +167,15 @@ int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
                      int precision, TAGS_COMPLEX *u_inj, TAGS_COMPLEX *x_out,
                      tags_item_stats *stats);
 
-/* Factorisation plan of the sweep (default TAGS_PLAN_STRUCTURED). All plans
- * are partial pivoting; the structured ones skip rows and columns that are
- * exactly zero for the columns being factored.
+/* Factorisation plan of the sweep (default TAGS_PLAN_STRUCTURED). Every plan
+ * pivots with the process's panel algorithm (tags_set_panel); the structured
+ * ones skip rows and columns that are exactly zero for the columns being
+ * factored, which changes no arithmetic on the non-zero entries.
  * TAGS_PLAN_DENSE       unknowns in the reference order [U; I_L; I_T], all N
  *                       rows and columns;
  * TAGS_PLAN_STRUCTURED  reference order; the node rows are zero in the node
  *                       columns (Y_E = 0), so the first n columns pivot and
- *                       update over the 2m impedance rows only. Same pivots as
- *                       DENSE;
+ *                       update over the 2m impedance rows only;
  * TAGS_PLAN_ZL_FIRST    unknowns stored as [I_L; I_T; U]; the Z_T rows are
  *                       zero in the Z_L columns and are skipped there (fewer
  *                       flops; its complex64 factor is less accurate at the

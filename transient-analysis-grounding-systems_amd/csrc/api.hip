@@ -561,7 +561,7 @@ int integral(const Electrode* sender, const Electrode* receiver, TAGS_COMPLEX ga
         return 0;
     }
     if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
-        std::printf("integration type %d is not part of this build\n", integration_type);
+        std::printf("integration type %d is not supported by this library\n", integration_type);
         return TAGS_ERR_UNSUPPORTED;
     }
     if (integration_type != INTG_DOUBLE) {
@@ -625,7 +625,7 @@ int calculate_impedances(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* el
     (void)req_abs_error;
     (void)req_rel_error;
     if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
-        std::printf("integration type %d is not part of this build\n", integration_type);
+        std::printf("integration type %d is not supported by this library\n", integration_type);
         return TAGS_ERR_UNSUPPORTED;
     }
     if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE) {
@@ -647,7 +647,7 @@ int impedances_images(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* elect
     (void)req_abs_error;
     (void)req_rel_error;
     if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
-        std::printf("integration type %d is not part of this build\n", integration_type);
+        std::printf("integration type %d is not supported by this library\n", integration_type);
         return TAGS_ERR_UNSUPPORTED;
     }
     if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE) {

@@ -16,13 +16,23 @@
 // never the largest candidate unless the whole column is zero).
 //
 // Per 128-column step j (two <= 64-column sub-panels a, b):
-//   panel(a)            cooperative kernel, one grid barrier per column
-//   inv(diag a)         unit-lower inverse of the diagonal block in LDS
+//   panel(a)            tournament pivoting (tslu.hip), or partial pivoting
+//                       over the whole panel (cooperative kernel, one grid
+//                       barrier per column)
+//   inv(diag a)         unit-lower inverse of the diagonal block
 //   update b            U_ab = inv(L_aa) A_ab; A_b -= L_a U_ab   (MFMA GEMM)
 //   panel(b), inv(diag b)
 //   U12                 inv(L11) applied to the pivot rows          (MFMA GEMM)
 //   trailing update     A22 -= L21 U12, K = 128                     (MFMA GEMM)
 // and after the last step the upper inverses of all diagonal blocks.
+//
+// Look-ahead: step j first updates the columns of step j+1's panel; step j+1's
+// panel work then runs on a second stream with its own copy of the row map
+// while the rest of step j's update runs (in two launches, whose boundary lets
+// waiting panel work-groups start). The panel works on its own copy of the row
+// map, so the rest of step j addresses exactly the physical rows it would
+// without the panel running; the two touch disjoint columns, and the next
+// step starts from the panel's row map once both are done.
 #include <algorithm>
 #include <atomic>
 #include <cmath>
