@@ -30,7 +30,7 @@ PEAK = {"fp64": 78.6, "fp32": 157.3}  # dense MFMA TFLOP/s, MI355X
 GRID = (31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)
 SIGMA, EPSR = 1.0 / 500.0, 20.0
 NFREQ = 256
-PLAN_NAMES = ["dense", "structured", "zl_first", "block_pivot"]
+PLAN_NAMES = ["dense", "structured", "zl_first", "block_pivot", "grounded", "grounded_pivoted"]
 
 
 def lu_flops(N):
@@ -120,7 +120,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--plan-mixed", type=int, default=1,
                     help="LU plan of the mixed form: 0 dense, 1 structured (default), "
-                         "2 Z_L first, 3 Z_L-block pivoting")
+                         "2 Z_L first, 3 Z_L-block pivoting, 4 grounded without row "
+                         "exchanges, 5 grounded with pivoting")
     ap.add_argument("--plan-fp64", type=int, default=2,
                     help="LU plan of the FP64 form (default 2, Z_L first)")
     ap.add_argument("--update-chunks", type=int, default=None,

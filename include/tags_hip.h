@@ -168,7 +168,8 @@ int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
                      tags_item_stats *stats);
 
 /* Factorisation plan of the sweep (default TAGS_PLAN_STRUCTURED). Every plan
- * pivots with the process's panel algorithm (tags_set_panel); the structured
+ * but the mixed factor of GROUNDED pivots with the process's panel algorithm
+ * (tags_set_panel); the structured
  * ones skip rows and columns that are exactly zero for the columns being
  * factored, which changes no arithmetic on the non-zero entries.
  * TAGS_PLAN_DENSE       unknowns in the reference order [U; I_L; I_T], all N
@@ -182,12 +183,25 @@ int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
  *                       lowest frequencies);
  * TAGS_PLAN_BLOCK_PIVOT as ZL_FIRST but the Z_L columns pivot among Z_L rows
  *                       only, which also skips the Z_T columns (fewest flops,
- *                       weakest pivoting). */
+ *                       weakest pivoting);
+ * TAGS_PLAN_GROUNDED    stored as ZL_FIRST and factored as ZL_FIRST in FP64.
+ *                       The mixed form's complex64 factor is of the grounded
+ *                       matrix T^T W T (node voltages replaced by the first
+ *                       node's voltage and the differences to it), factored
+ *                       without row exchanges: Z_L columns, Z_T columns, then
+ *                       the node block, each pivoting on its own diagonal.
+ *                       FP64 refinement runs on W itself, and a complex64
+ *                       breakdown or stalled refinement falls back to the
+ *                       pivoted FP64 factor;
+ * TAGS_PLAN_GROUNDED_PIVOTED as GROUNDED, but the complex64 factor of T^T W T
+ *                       pivots like ZL_FIRST. */
 enum tags_plan {
     TAGS_PLAN_DENSE = 0,
     TAGS_PLAN_STRUCTURED = 1,
     TAGS_PLAN_ZL_FIRST = 2,
-    TAGS_PLAN_BLOCK_PIVOT = 3
+    TAGS_PLAN_BLOCK_PIVOT = 3,
+    TAGS_PLAN_GROUNDED = 4,
+    TAGS_PLAN_GROUNDED_PIVOTED = 5
 };
 int tags_sweep_set_plan(tags_sweep *sw, int plan);
 
@@ -211,7 +225,7 @@ int tags_dense_solve(const TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t n, int preci
 
 /* As tags_dense_solve for an immittance matrix stored row-major in the
  * storage order of the given tags_plan: [U; I_L; I_T] for DENSE and
- * STRUCTURED, [I_L; I_T; U] for ZL_FIRST and BLOCK_PIVOT. */
+ * STRUCTURED, [I_L; I_T; U] for ZL_FIRST, BLOCK_PIVOT and GROUNDED. */
 int tags_immittance_solve(const TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t num_electrodes,
                           size_t num_nodes, int precision, int plan, tags_item_stats *stats);
 

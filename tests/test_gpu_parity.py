@@ -255,7 +255,7 @@ def test_c1_sweep_vs_oracle():
     sw.close()
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
 def test_c2_sweep_vs_oracle_and_mixed(plan):
     """Config 2 (20 x 20 m grid, N = 1,185), every factorisation plan: FP64 vs
     the oracle at a few frequencies, mixed vs FP64 on 40 frequencies, KCL and
@@ -272,6 +272,8 @@ def test_c2_sweep_vs_oracle_and_mixed(plan):
     # (backward error up to ~1e-13 at low frequency) than the refined solution
     tol = 1e-6 if plan == 3 else 1e-8
     assert np.max(np.abs(umx - u64) / np.abs(u64)) <= tol
+    if plan in (4, 5):  # the grounded complex64 factors converge on this grid
+        assert all(q["precision_used"] == 1 for q in stmx), stmx
     m, n = len(e), len(nodes)
     assert np.array_equal(x64[:, 0], u64)
     assert np.max(np.abs(x64[:, n + m:].sum(1) - 1.0)) < tol  # KCL: sum I_T = I_E
@@ -281,7 +283,7 @@ def test_c2_sweep_vs_oracle_and_mixed(plan):
     sw.close()
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
 def test_lu_modes_agree(plan):
     """Config 2 sweep under the three factorisation schedules: look-ahead
     (default; the next panel on a second stream with its own row map),
@@ -318,7 +320,8 @@ def test_lu_modes_agree(plan):
 
 def test_c3_mixed_vs_fp64_property():
     """Full 60 x 60 m grid (N = 21,421): mixed vs FP64 at the band edges, and
-    the structured factorisation plan vs the dense one."""
+    the structured factorisation plan vs the dense one, and the grounded
+    complex64 factor without row exchanges (no FP64 fallback)."""
     rc, e, nodes = T.electrode_grid(31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)
     f = np.array([100.0, 5e6])
     s = 2j * np.pi * f
@@ -328,13 +331,24 @@ def test_c3_mixed_vs_fp64_property():
     umx, st = sw.solve(s, kap, 0, T.PREC_MIXED)
     assert np.max(np.abs(umx - u64) / np.abs(u64)) <= 1e-8
     assert all(q["backward_err"] <= 1e-13 for q in st)
+    # grounded matrix, tournament pivoting: no FP64 fallback at either edge
+    sw.set_plan(T.PLAN_GROUNDED_PIVOTED)
+    ug, stg = sw.solve(s, kap, 0, T.PREC_MIXED)
+    assert all(q["precision_used"] == 1 and q["backward_err"] <= 1e-13 for q in stg), stg
+    assert np.max(np.abs(ug - u64) / np.abs(u64)) <= 1e-8
+    # grounded matrix without row exchanges: converges at 5 MHz; at 100 Hz the
+    # refinement stalls on this grid and the FP64 fallback gives the answer
+    sw.set_plan(T.PLAN_GROUNDED)
+    ug, stg = sw.solve(s, kap, 0, T.PREC_MIXED)
+    assert stg[1]["precision_used"] == 1 and stg[1]["backward_err"] <= 1e-13, stg
+    assert np.max(np.abs(ug - u64) / np.abs(u64)) <= 1e-8
     sw.set_plan(T.PLAN_DENSE)
     ud, _ = sw.solve(s, kap, 0, T.PREC_FP64)
     assert np.max(np.abs(ud - u64) / np.abs(u64)) <= 1e-8
     sw.close()
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
 def test_sweep_assembly_vs_oracle(plan):
     """W assembled in HBM by the sweep (pair integrals, images, incidence) vs
     the oracle pipeline, in the reference unknown order, for every storage
@@ -354,7 +368,7 @@ def test_sweep_assembly_vs_oracle(plan):
     sw.close()
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_immittance_solve_plans(plan, prec):
     """Structured factorisation plans on matrices with the immittance zero
@@ -379,3 +393,26 @@ def test_immittance_solve_plans(plan, prec):
     x0 = np.linalg.solve(A, b)
     tol = 1e-7 if plan == 3 else 1e-9  # plan 3 pivots within the Z_L block only
     assert np.abs(x - x0).max() <= tol * np.abs(x0).max()
+
+
+@pytest.mark.parametrize("freq", [100.0, 3e5, 1e7])
+def test_grounded_unpivoted_vs_lapack(freq):
+    """The oracle's W of a small grid in the [I_L; I_T; U] order, mixed form of
+    the grounded plan (complex64 factor of T^T W T without row exchanges, FP64
+    refinement on W): no fallback, and x equals LAPACK's solution of W."""
+    e, nodes = _small_grid()
+    m, n = len(e), len(nodes)
+    N = n + 2 * m
+    wo = _assembled(e, nodes, freq, 1e-3, 15.0).reshape(N, N).T
+    Wr = np.tril(wo) + np.tril(wo, -1).T  # reference order [U; I_L; I_T]
+    order = np.concatenate([n + np.arange(2 * m), np.arange(n)])
+    A = np.ascontiguousarray(Wr[np.ix_(order, order)])
+    b = np.zeros(N, complex)
+    b[2 * m + 3] = 1.0  # unit current at node 3
+    rc, x, st = T.immittance_solve(A, b, m, n, T.PREC_MIXED, T.PLAN_GROUNDED)
+    assert rc == 0 and st.info == 0
+    assert st.precision_used == 1 and st.backward_err <= 1e-13, (
+        st.precision_used, st.refine_iters, st.backward_err)
+    x0 = np.linalg.solve(A, b)
+    assert np.abs(x - x0).max() <= 1e-9 * np.abs(x0).max()
+

@@ -33,7 +33,8 @@ assert ELECTRODE_DTYPE.itemsize == 88
 
 INTG_NONE, INTG_DOUBLE, INTG_SINGLE, INTG_MHEM = 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
-PLAN_DENSE, PLAN_STRUCTURED, PLAN_ZL_FIRST, PLAN_BLOCK_PIVOT = 0, 1, 2, 3
+PLAN_DENSE, PLAN_STRUCTURED, PLAN_ZL_FIRST, PLAN_BLOCK_PIVOT, PLAN_GROUNDED = 0, 1, 2, 3, 4
+PLAN_GROUNDED_PIVOTED = 5
 
 # src/auxiliary.c:17-21
 TWO_PI = 6.283185307179586476925286766559

@@ -125,6 +125,13 @@ struct Solver {
     c32* t32 = nullptr;
     unsigned long long* norms = nullptr;
     unsigned long long* hnorms = nullptr;  // pinned
+    // grounded node unknowns of the mixed factor (ground.g < 0: off)
+    Grounding ground;
+    bool ground_valid = false;  // gcol / grow / gsum hold the sums of the current W's node part
+    c64* gcol = nullptr;
+    c64* grow = nullptr;
+    c64* gsum = nullptr;  // [0] node block sum, [1] node sum of the right-hand side
+    c64* gpart = nullptr;
 
     int ensure(int n_, bool mixed)
     {
@@ -159,6 +166,12 @@ struct Solver {
         hipFree(norms);
         if (hnorms) hipHostFree(hnorms);
         hipFree(W32);
+        hipFree(gcol);
+        hipFree(grow);
+        hipFree(gsum);
+        hipFree(gpart);
+        gcol = grow = gsum = gpart = nullptr;
+        ground_valid = false;
         W32 = nullptr;
         dscale = nullptr;
         x = r = b = nullptr;
@@ -178,7 +191,9 @@ struct Solver {
     }
 
     const LUPlan* plan = nullptr;  // nullptr: dense partial pivoting
+    const LUPlan* plan32 = nullptr;  // plan of the mixed form's factor (nullptr: `plan`)
     bool equilibrate = false;      // symmetric row-max scaling before the FP32 factor
+                                   // (not applied with grounding)
 
     // Plain form: factor W (destroyed) in FP64 and solve; x returned in `b`.
     int solve64(c64* W, c64* bvec, tags_item_stats* stt)
@@ -207,13 +222,33 @@ struct Solver {
         int rc;
         if (stt)
             for (int i = 0; i < 8; i++) stt->refine_hist[i] = -1.0;
+        const bool grounded = ground.g >= 0;
         {
             Timed tm(pr, FAM_OTHER, st);
-            if ((rc = row_scale(W, n, n, dscale, equilibrate, st))) return rc;
-            if ((rc = to_c32_scaled(W, n, n, dscale, W32, n, st))) return rc;
+            if (grounded) {
+                if (!gcol) {
+                    TAGS_HIP(hipMalloc(&gcol, sizeof(c64) * n));
+                    TAGS_HIP(hipMalloc(&grow, sizeof(c64) * n));
+                    TAGS_HIP(hipMalloc(&gsum, sizeof(c64) * 2));
+                    TAGS_HIP(hipMalloc(&gpart, sizeof(c64) * kGroundChunks * (size_t)n));
+                    ground_valid = false;
+                }
+                if (!ground_valid) {
+                    if ((rc = ground_sums(W, n, n, ground, gcol, grow, gsum, gpart, st)))
+                        return rc;
+                    ground_valid = true;
+                }
+                if ((rc = row_scale(W, n, n, dscale, false, st))) return rc;
+                if ((rc = to_c32_grounded(W, n, n, ground, gcol, grow, gsum, W32, n, st)))
+                    return rc;
+            } else {
+                if ((rc = row_scale(W, n, n, dscale, equilibrate, st))) return rc;
+                if ((rc = to_c32_scaled(W, n, n, dscale, W32, n, st))) return rc;
+            }
             tm.end();
         }
-        if ((rc = lu_factor<float>(w32, W32, n, st, pr, plan))) return rc;
+        const Grounding* gr = grounded ? &ground : nullptr;
+        if ((rc = lu_factor<float>(w32, W32, n, st, pr, plan32 ? plan32 : plan))) return rc;
         int f[2];
         TAGS_HIP(hipMemcpyAsync(f, w32.flags, sizeof(f), hipMemcpyDeviceToHost, st));
         TAGS_HIP(hipStreamSynchronize(st));
@@ -233,9 +268,11 @@ struct Solver {
             std::memcpy(&rnorm, &hnorms[0], 8);
             std::memcpy(&wnorm, &hnorms[2], 8);
             for (it = 0; it < 12 && rnorm > 0; it++) {
-                if ((rc = prep_rhs32(r, dscale, w32.perm, n, 1.0 / rnorm, t32, st))) return rc;
+                if ((rc = prep_rhs32(r, dscale, w32.perm, n, 1.0 / rnorm, t32, st, gr,
+                                     grounded ? gsum + 1 : nullptr)))
+                    return rc;
                 if ((rc = lu_solve_inplace<float>(w32, W32, n, t32, st, pr))) return rc;
-                if ((rc = update_x(x, t32, dscale, n, rnorm, st))) return rc;
+                if ((rc = update_x(x, t32, dscale, n, rnorm, st, gr))) return rc;
                 if ((rc = residual64(W, n, n, x, b, r, norms, false, st, pr))) return rc;
                 TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * 2,
                                         hipMemcpyDeviceToHost, st));
@@ -838,7 +875,8 @@ int fill_impedance_imm(TAGS_COMPLEX* wg, const TAGS_COMPLEX* zl, const TAGS_COMP
 // solve_immittance (src/linalg.c:113-139)
 // ---------------------------------------------------------------------------
 static int dense_solve_host(const c64* a, c64* bx, long n, int precision, bool sym_lower_cm,
-                            tags_item_stats* stt, const LUPlan* plan = nullptr)
+                            tags_item_stats* stt, const LUPlan* plan = nullptr,
+                            const LUPlan* plan32 = nullptr, Grounding ground = Grounding{})
 {
     if (n <= 0) return 0;
     if (n >= 65536) return TAGS_ERR_TOO_LARGE;
@@ -854,6 +892,8 @@ static int dense_solve_host(const c64* a, c64* bx, long n, int precision, bool s
     TAGS_HIP(hipMemcpyAsync(bv.p, bx, sizeof(c64) * n, hipMemcpyHostToDevice, S.s));
     if (sym_lower_cm && (rc = sym_from_upper(W.p, n, (int)n, S.s))) return rc;
     sv.plan = plan;
+    sv.plan32 = plan32;
+    sv.ground = ground;
     rc = mixed ? sv.solve_mixed(W.p, bv.p, stt) : sv.solve64(W.p, bv.p, stt);
     if (rc) return rc;
     TAGS_HIP(hipMemcpyAsync(bx, bv.p, sizeof(c64) * n, hipMemcpyDeviceToHost, S.s));
@@ -883,15 +923,25 @@ int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_ele
                           size_t num_nodes, int precision, int plan, tags_item_stats* stats)
 {
     const long m = (long)num_electrodes, nn = (long)num_nodes;
-    LUPlan p;
+    if (m <= 0 || nn <= 0) return TAGS_ERR_ARG;
+    LUPlan p, p32;
+    Grounding gr;
     switch (plan) {
     case TAGS_PLAN_DENSE: p = LUPlan::dense((int)(nn + 2 * m)); break;
     case TAGS_PLAN_STRUCTURED: p = LUPlan::nodes_first((int)m, (int)nn); break;
     case TAGS_PLAN_ZL_FIRST: p = LUPlan::zl_first((int)m, (int)nn, false); break;
     case TAGS_PLAN_BLOCK_PIVOT: p = LUPlan::zl_first((int)m, (int)nn, true); break;
+    case TAGS_PLAN_GROUNDED:
+    case TAGS_PLAN_GROUNDED_PIVOTED:
+        p = LUPlan::zl_first((int)m, (int)nn, false);
+        p32 = plan == TAGS_PLAN_GROUNDED ? LUPlan::zl_first_unpivoted((int)m, (int)nn) : p;
+        gr = Grounding{(int)(2 * m), (int)(2 * m), (int)(2 * m + nn)};
+        break;
     default: return TAGS_ERR_ARG;
     }
-    return dense_solve_host((const c64*)a, (c64*)b, nn + 2 * m, precision, false, stats, &p);
+    const bool g = plan == TAGS_PLAN_GROUNDED || plan == TAGS_PLAN_GROUNDED_PIVOTED;
+    return dense_solve_host((const c64*)a, (c64*)b, nn + 2 * m, precision, false, stats, &p,
+                            g ? &p32 : nullptr, gr);
 }
 
 }  // extern "C"
@@ -902,7 +952,9 @@ int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_ele
 struct tags_sweep {
     long m = 0, n = 0, N = 0;
     int plan_kind = TAGS_PLAN_STRUCTURED;
+    int layout_kind = -1;  // plan kind the current layout and grounding sums belong to
     tags::LUPlan plan;
+    tags::LUPlan plan32;
     tags::BlockLayout layout{};
     hipStream_t st = nullptr;
     tags::DevSegs e, img;
@@ -914,10 +966,14 @@ struct tags_sweep {
     tags::Profiler prof;
 };
 
-
 static int sweep_layout(tags_sweep* sw)
 {
     const long n = sw->n, m = sw->m;
+    if (sw->layout_kind != sw->plan_kind) sw->sv.ground_valid = false;
+    sw->layout_kind = sw->plan_kind;
+    sw->sv.plan = &sw->plan;
+    sw->sv.plan32 = nullptr;
+    sw->sv.ground = Grounding{};
     switch (sw->plan_kind) {
     case TAGS_PLAN_DENSE:
         sw->layout = BlockLayout{n, m, 0, n, n + m};
@@ -926,12 +982,20 @@ static int sweep_layout(tags_sweep* sw)
     case TAGS_PLAN_STRUCTURED:
         sw->layout = BlockLayout{n, m, 0, n, n + m};
         sw->plan = LUPlan::nodes_first((int)m, (int)n);
-        sw->sv.plan = &sw->plan;
+        break;
+    case TAGS_PLAN_GROUNDED:
+    case TAGS_PLAN_GROUNDED_PIVOTED:
+        sw->layout = BlockLayout{n, m, 2 * m, 0, m};
+        sw->plan = LUPlan::zl_first((int)m, (int)n, false);
+        sw->plan32 = sw->plan_kind == TAGS_PLAN_GROUNDED
+                         ? LUPlan::zl_first_unpivoted((int)m, (int)n)
+                         : LUPlan::zl_first((int)m, (int)n, false);
+        sw->sv.plan32 = &sw->plan32;
+        sw->sv.ground = Grounding{(int)(2 * m), (int)(2 * m), (int)(2 * m + n)};
         break;
     default:
         sw->layout = BlockLayout{n, m, 2 * m, 0, m};
         sw->plan = LUPlan::zl_first((int)m, (int)n, sw->plan_kind == TAGS_PLAN_BLOCK_PIVOT);
-        sw->sv.plan = &sw->plan;
         break;
     }
     return 0;
@@ -1065,7 +1129,7 @@ int tags_sweep_assemble(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, doub
 
 int tags_sweep_set_plan(tags_sweep* sw, int plan)
 {
-    if (!sw || plan < 0 || plan > 3) return TAGS_ERR_ARG;
+    if (!sw || plan < 0 || plan > TAGS_PLAN_GROUNDED_PIVOTED) return TAGS_ERR_ARG;
     sw->plan_kind = plan;
     return 0;
 }

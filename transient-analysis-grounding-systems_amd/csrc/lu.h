@@ -167,6 +167,9 @@ struct LUPhase {
     int row_end;          // logical rows [column, row_end) may be non-zero
     int cand_end;         // pivot candidates are logical rows [column, cand_end)
     int x_begin, x_end;   // extra trailing columns beyond c_end (empty if x_begin >= x_end)
+    bool pivot = true;    // false: no row exchanges (the diagonal is the pivot)
+    int order_off = -1;   // >= 0: logical rows [c_begin, n) are set at the start of the
+                          // phase to orders[order_off ...] (phases without pivoting only)
 };
 
 struct LUPlan {
@@ -217,6 +220,32 @@ struct LUPlan {
         for (int i = 0; i < m; i++) p.init_perm[m + n + i] = m + i;
         return p;
     }
+    // The same matrix factored without row exchanges (the mixed form's factor of
+    // the grounded matrix): the Z_L columns over rows [Z_L, node] (the Z_T rows
+    // are zero there and receive no update; the pivot rows are Z_L rows, zero in
+    // the Z_T columns, which therefore receive no update either), then the
+    // remaining columns over rows reordered to [Z_T, node], so that each
+    // column's pivot is its own diagonal entry.
+    static LUPlan zl_first_unpivoted(int m, int n)
+    {
+        LUPlan p = zl_first(m, n, false);
+        const int N = n + 2 * m;
+        p.phases[0].pivot = false;
+        p.phases[0].cand_end = m + n;
+        p.phases[0].x_begin = 2 * m;
+        p.phases[1].pivot = false;
+        p.phases[1].order_off = 0;
+        p.orders.resize(N - m);
+        for (int i = 0; i < m + n; i++) p.orders[i] = m + i;
+        return p;
+    }
+    std::vector<int> orders;  // row orders referenced by phases (order_off)
+    bool pivots() const
+    {
+        for (const LUPhase& ph : phases)
+            if (ph.pivot) return true;
+        return phases.empty();
+    }
 };
 
 // Device work space of one LU of order n in precision T.
@@ -238,12 +267,12 @@ struct LUWork {
     PanelSlot<T>* slots = nullptr;
     unsigned* bar = nullptr;
     int* flags = nullptr;   // [0] info, [1] barrier error
-    int* d_blk = nullptr;   // device copy of blk
     int* d_init = nullptr;  // device copy of the plan's initial row order
+    int* d_orders = nullptr;  // device copy of the plan's phase row orders (capacity n)
     int* tcand = nullptr;       // tournament nominee lists
     unsigned* tcnt = nullptr;   // tournament arrival counters (kept at zero)
     std::vector<int> blk;   // start column of every diagonal block, in order
-    std::vector<int> h_blk, h_init;  // what d_blk / d_init currently hold
+    std::vector<int> h_init, h_orders;  // what d_init / d_orders hold
     double flops = 0;       // real flops of the last factorisation (GEMM + panel)
 
     int alloc(int n_, int max_panel_wgs, size_t panel_lds_limit);
@@ -273,9 +302,23 @@ int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
 int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t st);
 int to_c32_scaled(const c64* W, long ld, int n, const double* d, c32* W32, long ld32,
                   hipStream_t st);
+// Grounded node unknowns of the mixed form (lu.hip): node rows and columns
+// [nb, ne) of the physical order, reference node g (g < 0: off).
+struct Grounding {
+    int g = -1;
+    int nb = 0, ne = 0;
+};
+constexpr int kGroundChunks = 32;  // row chunks of the node column sums
+// gcol, grow: n each; gg: 1; part: kGroundChunks * n
+int ground_sums(const c64* W, long ld, int n, const Grounding& gr, c64* gcol, c64* grow, c64* gg,
+                c64* part, hipStream_t st);
+int to_c32_grounded(const c64* W, long ld, int n, const Grounding& gr, const c64* gcol,
+                    const c64* grow, const c64* gg, c32* W32, long ld32, hipStream_t st);
+// rg: one c64 of device scratch (used with grounding)
 int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv_scale, c32* t,
-               hipStream_t st);
-int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st);
+               hipStream_t st, const Grounding* gr = nullptr, c64* rg = nullptr);
+int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st,
+             const Grounding* gr = nullptr);
 int fill_c64(c64* p, long n, c64 v, hipStream_t st);
 int sym_from_upper(c64* W, long ld, int n, hipStream_t st);
 

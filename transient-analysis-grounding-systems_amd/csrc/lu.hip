@@ -16,15 +16,17 @@
 // never the largest candidate unless the whole column is zero).
 //
 // Per 128-column step j (two <= 64-column sub-panels a, b):
-//   panel(a)            tournament pivoting (tslu.hip), or partial pivoting
-//                       over the whole panel (cooperative kernel, one grid
-//                       barrier per column)
-//   inv(diag a)         unit-lower inverse of the diagonal block
-//   update b            U_ab = inv(L_aa) A_ab; A_b -= L_a U_ab   (MFMA GEMM)
-//   panel(b), inv(diag b)
+//   panel(a)            pivots and L11 \ U11: tournament pivoting (tslu.hip),
+//                       partial pivoting over the whole panel (cooperative
+//                       kernel, one grid barrier per column), or, for phases
+//                       without row exchanges, the diagonal block's own LU
+//   inv(diag a)         inverses of both triangles of the diagonal block
+//   L21(a)              A21 inv(U11)                                (MFMA GEMM)
+//   update b            U_ab = inv(L_aa) A_ab; A_b -= L_a U_ab      (MFMA GEMM)
+//   panel(b), inv(diag b), L21(b)
 //   U12                 inv(L11) applied to the pivot rows          (MFMA GEMM)
 //   trailing update     A22 -= L21 U12, K = 128                     (MFMA GEMM)
-// and after the last step the upper inverses of all diagonal blocks.
+// The block inverses are kept for the triangular solves.
 //
 // Look-ahead: step j first updates the columns of step j+1's panel; step j+1's
 // panel work then runs on a second stream with its own copy of the row map
@@ -226,129 +228,138 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
 }
 
 // ---------------------------------------------------------------------------
-// Inverses of the 64 x 64 diagonal blocks, written as dense 64 x 64 row-major
-// blocks (identity padding past column w). The block at column c0 has logical
-// rows c0 .. c0+w (physical perm[c0 + r]).
-//   unit lower: X[i][j] = -(L[i][j] + sum_{k=j+1}^{i-1} X[i][k] L[k][j]), i > j,
-//               columns right to left;
-//   upper:      X[j][j] = 1/u_jj, X[i][j] = -(sum_{k=i}^{j-1} X[i][k] U[k][j]) / u_jj,
-//               columns left to right.
-// Each row's dot product is split over 4 adjacent lanes; one barrier per
-// column. blockIdx.x selects the block (one work-group per block).
+// Diagonal block of a panel (w x w at column c0, logical rows c0 .. c0+w,
+// physical pm[c0 + r]): with FACTOR, its LU without row exchanges in place;
+// then (in every case) the dense 64 x 64 inverses of its unit-lower and upper
+// triangles (identity padding past w), which the panel's L21 = A21 inv(U11),
+// the rows' U12 = inv(L11) A12 and the triangular solves use. One work-group
+// of 256 threads; thread (r, p) holds columns 16p .. 16p+15 of row r in
+// registers. Rows and columns past w are padded with the identity, which the
+// sweeps leave unchanged.
+//   LU:      at step k row k is final; its entries right of k and the
+//            reciprocal of its pivot are broadcast through LDS and every row
+//            r > k forms l = a[r][k] / u_kk and does a[r] -= l u[k].
+//   inverses, one sweep for both: inv(L) top-down (row k final at step k,
+//            X[r] -= L[r][k] X[k] for r > k), inv(U) bottom-up (row k divided
+//            by u_kk at its step, X[r] -= U[r][k] X[k] for r < k).
+// One barrier per step (row buffers alternate by parity). A zero pivot gives
+// zero multipliers in its column and is reported in *info (first such column,
+// 1-based, as LAPACK); a caller then falls back to a pivoted factorisation.
 // ---------------------------------------------------------------------------
-template <typename T>
-__device__ inline cx<T> quad_sum(cx<T> s)
+template <typename T, bool FACTOR>
+__global__ __launch_bounds__(256) void k_diag_block(cx<T>* A, long ld, const int* pm, int c0,
+                                                    int w, cx<T>* invL, cx<T>* invU, int* info)
 {
-    s.x += __shfl_xor(s.x, 1);
-    s.y += __shfl_xor(s.y, 1);
-    s.x += __shfl_xor(s.x, 2);
-    s.y += __shfl_xor(s.y, 2);
-    return s;
-}
-
-// Row map of the block staged in LDS first, then all 16 loads of a thread
-// issued together (a loop of dependent map -> row loads would pay the memory
-// latency once per iteration).
-template <typename T>
-__device__ inline void load_diag_block(const cx<T>* A, long ld, const int* perm, int c0, int w,
-                                       cx<T> (*B)[65])
-{
+    __shared__ cx<T> s_u[2][64];
+    __shared__ cx<T> s_v[2][64];
+    __shared__ cx<T> s_rp[2];
     __shared__ int s_pm[64];
-    if (threadIdx.x < 64) s_pm[threadIdx.x] = threadIdx.x < w ? perm[c0 + threadIdx.x] : 0;
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < 16; it++) {
-        const int e = threadIdx.x + it * 256;
-        const int r = e >> 6, c = e & 63;
-        cx<T> v = cx<T>{0, 0};
-        if (r < w && c < w)
-            v = A[(long)s_pm[r] * ld + c0 + c];
-        else if (r == c)
-            v = cx<T>{1, 0};
-        B[r][c] = v;
-    }
-}
-
-// Unit-lower inverse X = L^-1 of the diagonal block at column c0 (logical rows
-// c0 .. c0+w), written as a dense 64 x 64 row-major block with identity
-// padding past w. Row sweep in registers: thread (r, p) holds columns
-// 16p .. 16p+15 of row r of L and of X (X starts as the identity); at step k
-// row k of X is final and is broadcast through LDS, and every row r > k does
-// X[r] -= L[r][k] X[k] (L[r][k] comes from the thread of the quad holding
-// column k). 2 or 4 KB of LDS, so the kernel fits beside other work-groups.
-template <typename T>
-__global__ __launch_bounds__(256) void k_inv_lower(const cx<T>* A, long ld, const int* perm,
-                                                    int c0, int w, cx<T>* out)
-{
-    __shared__ cx<T> xk[2][64];
+    __shared__ int s_zero;
     __builtin_amdgcn_s_setprio(1);  // on the panel's critical path
-    const int t = threadIdx.x, r = t >> 2, part = t & 3, lane = t & 63;
-    cx<T> L[16], X[16];
-    const cx<T>* rowp = r < w ? A + (long)perm[c0 + r] * ld + c0 : nullptr;
+    const int t = threadIdx.x, r = t >> 2, p = t & 3, lane = t & 63;
+    const int quad = lane & ~3;
+    if (t < 64) s_pm[t] = t < w ? pm[c0 + t] : 0;
+    if (t == 0) s_zero = -1;
+    __syncthreads();
+    cx<T> a[16];
+    {
+        const cx<T>* src = r < w ? A + (long)s_pm[r] * ld + c0 + 16 * p : nullptr;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int c = 16 * p + q;
+            a[q] = src ? (c < w ? src[q] : cx<T>{0, 0}) : cx<T>{c == r ? T(1) : T(0), T(0)};
+        }
+    }
+    for (int kb = 0; FACTOR && kb < 4; kb++) {
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            const int k = kb * 16 + kk, par = k & 1;
+            if (r == k) {
+#pragma unroll
+                for (int q = 0; q < 16; q++)
+                    s_u[par][16 * p + q] = 16 * p + q > k ? a[q] : cx<T>{0, 0};
+                if (p == kb) {
+                    const bool z = a[kk].x == T(0) && a[kk].y == T(0);
+                    s_rp[par] = z ? cx<T>{0, 0} : crecip_fast(a[kk]);
+                    if (z && s_zero < 0) s_zero = k;
+                }
+            }
+            __syncthreads();
+            const cx<T> lo = cmul(a[kk], s_rp[par]);
+            cx<T> l;
+            l.x = __shfl(lo.x, quad | kb);
+            l.y = __shfl(lo.y, quad | kb);
+            if (r > k) {
+                if (p == kb) a[kk] = l;
+#pragma unroll
+                for (int q = 0; q < 16; q++) a[q] = cfnma(a[q], l, s_u[par][16 * p + q]);
+            }
+        }
+    }
+    // a: L \ U of row r. Inverses of both triangles in one sweep.
+    cx<T> xl[16], xu[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
-        const int c = part * 16 + q;
-        L[q] = (rowp && c < r && c < w) ? rowp[c] : cx<T>{0, 0};
-        X[q] = cx<T>{c == r ? T(1) : T(0), T(0)};
+        xl[q] = cx<T>{16 * p + q == r ? T(1) : T(0), T(0)};
+        xu[q] = xl[q];
     }
     for (int kb = 0; kb < 4; kb++) {
 #pragma unroll
-        for (int qq = 0; qq < 16; qq++) {
-            const int k = kb * 16 + qq;
-            if (k >= 63) break;
+        for (int kk = 0; kk < 16; kk++) {
+            const int k = kb * 16 + kk, par = k & 1;
+            const int ku = 63 - k, kbu = 3 - kb, kku = 15 - kk;
+            // u_{ku,ku} (held by the quad's thread kbu of row ku) for row ku of inv(U)
+            cx<T> d;
+            d.x = __shfl(a[kku].x, quad | kbu);
+            d.y = __shfl(a[kku].y, quad | kbu);
             if (r == k) {
 #pragma unroll
-                for (int q = 0; q < 16; q++) xk[k & 1][part * 16 + q] = X[q];
+                for (int q = 0; q < 16; q++) s_v[par][16 * p + q] = xl[q];
+            }
+            if (r == ku) {
+                const bool z = d.x == T(0) && d.y == T(0);
+                const cx<T> rd = z ? cx<T>{0, 0} : crecip_fast(d);
+#pragma unroll
+                for (int q = 0; q < 16; q++) {
+                    xu[q] = cmul(xu[q], rd);
+                    s_u[par][16 * p + q] = xu[q];
+                }
             }
             __syncthreads();
-            const int src = (lane & ~3) | kb;
-            cx<T> l;
-            l.x = __shfl(L[qq].x, src);
-            l.y = __shfl(L[qq].y, src);
+            cx<T> lk, uk;
+            lk.x = __shfl(a[kk].x, quad | kb);
+            lk.y = __shfl(a[kk].y, quad | kb);
+            uk.x = __shfl(a[kku].x, quad | kbu);
+            uk.y = __shfl(a[kku].y, quad | kbu);
             if (r > k) {
 #pragma unroll
-                for (int q = 0; q < 16; q++) X[q] = cfnma(X[q], l, xk[k & 1][part * 16 + q]);
+                for (int q = 0; q < 16; q++) xl[q] = cfnma(xl[q], lk, s_v[par][16 * p + q]);
+            }
+            if (r < ku) {
+#pragma unroll
+                for (int q = 0; q < 16; q++) xu[q] = cfnma(xu[q], uk, s_u[par][16 * p + q]);
             }
         }
     }
+    if (FACTOR && r < w) {
+        cx<T>* dst = A + (long)s_pm[r] * ld + c0 + 16 * p;
 #pragma unroll
-    for (int q = 0; q < 16; q++) out[r * 64 + part * 16 + q] = X[q];
+        for (int q = 0; q < 16; q++)
+            if (16 * p + q < w) dst[q] = a[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+        invL[r * 64 + 16 * p + q] = xl[q];
+        invU[r * 64 + 16 * p + q] = xu[q];
+    }
+    if (FACTOR && t == 0 && s_zero >= 0 && *info == 0) *info = c0 + s_zero + 1;
 }
 
-// one work-group per diagonal block b: columns blk[b] .. blk[b+1] (or n)
-template <typename T>
-__global__ __launch_bounds__(256) void k_inv_upper(const cx<T>* A, long ld, const int* perm,
-                                                    const int* blk, int nblk, int n,
-                                                    cx<T>* inv_base)
+// perm[c0 + i] = order[i] for i < len (a phase's row order)
+__global__ void k_set_order(int* perm, int c0, int len, const int* order)
 {
-    __shared__ cx<T> B[64][65];  // row stride 65: rows of a column fall in different banks
-    __shared__ cx<T> X[64][65];
-    const int b = blockIdx.x;
-    const int c0 = blk[b];
-    const int w = (b + 1 < nblk ? blk[b + 1] : n) - c0;
-    cx<T>* out = inv_base + (size_t)b * 4096;
-    load_diag_block(A, ld, perm, c0, w, B);
-    const int t = threadIdx.x, row = t >> 2, part = t & 3;
-    for (int e = t; e < 64 * 65; e += 256) (&X[0][0])[e] = cx<T>{0, 0};
-    __syncthreads();
-    for (int j = 0; j < 64; j++) {
-        const cx<T> inv = crecip(B[j][j]);
-        cx<T> s = cx<T>{0, 0};
-        if (row < j) {
-            for (int k = row + part; k < j; k += 4) s = cadd(s, cmul(X[row][k], B[k][j]));
-        }
-        s = quad_sum(s);
-        if (row < j && part == 0) {
-            const cx<T> v = cmul(s, inv);
-            X[row][j] = cx<T>{-v.x, -v.y};
-        }
-        if (row == j && part == 0) X[j][j] = inv;
-        __syncthreads();
-    }
-    for (int e = t; e < 64 * 64; e += 256) {
-        const int r = e >> 6, c = e & 63;
-        out[e] = r <= c ? X[r][c] : cx<T>{0, 0};
-    }
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < len) perm[c0 + i] = order[i];
 }
 
 // dst[r][c] = A[rowmap[r]][c0 + c] for r < rows, c < cols (dst ld = cols)
@@ -491,24 +502,118 @@ __global__ __launch_bounds__(256) void k_to_c32_scaled(const c64* W, long ld, in
     }
 }
 
-// t[i] = (float) (d[perm[i]] * r[perm[i]] * inv_scale)
+// ---------------------------------------------------------------------------
+// Grounded node unknowns of the mixed form. The node voltages U (physical rows
+// and columns [nb, ne)) are replaced by U_g and V_i = U_i - U_g (i != g):
+// W' = T^T W T, T = I plus ones in column g on the node rows, x = T y. Only row
+// and column g change: column g of W' is the sum of W's node columns, row g the
+// sum of its node rows, and W'[g][g] the sum of its node block. For the
+// immittance matrix the incidence rows sum to 0 (A) and 1 (B), so the
+// Laplacian-like part of the node block no longer holds the constant vector
+// that a low-precision factor cannot resolve at low frequencies.
+// ---------------------------------------------------------------------------
+__device__ inline c64 block_sum256(c64 s, c64* red)
+{
+    for (int off = 32; off > 0; off >>= 1) {
+        s.x += __shfl_xor(s.x, off);
+        s.y += __shfl_xor(s.y, off);
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    return cadd(cadd(red[0], red[1]), cadd(red[2], red[3]));
+}
+
+// gcol[i] = sum_{c in [nb, ne)} W[i][c], one work-group per row
+__global__ __launch_bounds__(256) void k_ground_rowsums(const c64* W, long ld, int nb, int ne,
+                                                         c64* gcol)
+{
+    __shared__ c64 red[4];
+    const long row = blockIdx.x;
+    c64 s{0.0, 0.0};
+    for (int c = nb + (int)threadIdx.x; c < ne; c += 256) s = cadd(s, W[row * ld + c]);
+    s = block_sum256(s, red);
+    if (threadIdx.x == 0) gcol[row] = s;
+}
+
+// part[k][j] = sum of W[r][j] over the k-th of `nch` chunks of the node rows
+__global__ __launch_bounds__(256) void k_ground_colsums(const c64* W, long ld, int n, int nb,
+                                                         int ne, int nch, c64* part)
+{
+    const int j = blockIdx.x * 256 + threadIdx.x;
+    if (j >= n) return;
+    const int len = (ne - nb + nch - 1) / nch;
+    const int r0 = nb + (int)blockIdx.y * len, r1 = min(ne, r0 + len);
+    c64 s{0.0, 0.0};
+    for (int r = r0; r < r1; r++) s = cadd(s, W[(long)r * ld + j]);
+    part[(long)blockIdx.y * n + j] = s;
+}
+
+// grow[j] = sum over the chunks (fixed order); gg = sum_{c in [nb, ne)} grow[c]
+__global__ __launch_bounds__(256) void k_ground_finish(const c64* part, int n, int nb, int ne,
+                                                        int nch, c64* grow, c64* gg)
+{
+    __shared__ c64 red[4];
+    c64 acc{0.0, 0.0};
+    for (int j = threadIdx.x; j < n; j += 256) {
+        c64 s{0.0, 0.0};
+        for (int k = 0; k < nch; k++) s = cadd(s, part[(long)k * n + j]);
+        grow[j] = s;
+        if (j >= nb && j < ne) acc = cadd(acc, s);
+    }
+    acc = block_sum256(acc, red);
+    if (threadIdx.x == 0) *gg = acc;
+}
+
+// W32 = T^T W T (unscaled)
+__global__ __launch_bounds__(256) void k_to_c32_grounded(const c64* W, long ld, int n, int g,
+                                                          const c64* gcol, const c64* grow,
+                                                          const c64* gg, c32* W32, long ld32)
+{
+    const int row = blockIdx.y;
+    const c64* src = row == g ? grow : W + (long)row * ld;
+    for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) {
+        const c64 a = c == g ? (row == g ? *gg : gcol[row]) : src[c];
+        W32[(long)row * ld32 + c] = c32{(float)a.x, (float)a.y};
+    }
+}
+
+// *out = sum_{i in [nb, ne)} r[i] (one work-group)
+__global__ __launch_bounds__(256) void k_ground_rhs(const c64* r, int nb, int ne, c64* out)
+{
+    __shared__ c64 red[4];
+    c64 s{0.0, 0.0};
+    for (int i = nb + (int)threadIdx.x; i < ne; i += 256) s = cadd(s, r[i]);
+    s = block_sum256(s, red);
+    if (threadIdx.x == 0) *out = s;
+}
+
+// t[i] = (float) (d[p] * r'[p] * inv_scale), p = perm[i]; with grounding
+// (g >= 0) r'[g] = *rg (the sum of the node entries) and r'[p] = r[p] otherwise
 __global__ void k_prep_rhs32(const c64* r, const double* d, const int* perm, int n,
-                             double inv_scale, c32* t)
+                             double inv_scale, int g, const c64* rg, c32* t)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     const int p = perm[i];
     const double s = d[p] * inv_scale;
-    t[i] = c32{(float)(r[p].x * s), (float)(r[p].y * s)};
+    const c64 v = p == g ? *rg : r[p];
+    t[i] = c32{(float)(v.x * s), (float)(v.y * s)};
 }
 
-// x[j] += scale * d[j] * z[j]
-__global__ void k_update_x(c64* x, const c32* z, const double* d, int n, double scale)
+// x[j] += scale * d[j] * z'[j]; with grounding (g >= 0) z'[j] = z[j] + z[g] on
+// the node range except g itself (x = T y)
+__global__ void k_update_x(c64* x, const c32* z, const double* d, int n, double scale, int g,
+                           int nb, int ne)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
     const double s = scale * d[j];
-    x[j] = c64{x[j].x + s * (double)z[j].x, x[j].y + s * (double)z[j].y};
+    double zx = (double)z[j].x, zy = (double)z[j].y;
+    if (g >= 0 && j != g && j >= nb && j < ne) {
+        zx += (double)z[g].x;
+        zy += (double)z[g].y;
+    }
+    x[j] = c64{x[j].x + s * zx, x[j].y + s * zy};
 }
 
 // t[i] = b[perm[i]]
@@ -572,8 +677,8 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit)
     TAGS_HIP(hipMalloc(&bar, sizeof(unsigned) * 64));
     TAGS_HIP(hipMalloc(&flags, sizeof(int) * 4));
     TAGS_HIP(hipMalloc(&vec, sizeof(cx<T>) * n));
-    TAGS_HIP(hipMalloc(&d_blk, sizeof(int) * nblk_cap));
     TAGS_HIP(hipMalloc(&d_init, sizeof(int) * n));
+    TAGS_HIP(hipMalloc(&d_orders, sizeof(int) * n));
     TAGS_HIP(hipMemset(bar, 0, sizeof(unsigned) * 64));
     TAGS_HIP(hipMemset(flags, 0, sizeof(int) * 4));
     TsluTree tr;
@@ -606,20 +711,21 @@ void LUWork<T>::release()
     (void)hipFree(bar);
     (void)hipFree(flags);
     (void)hipFree(vec);
-    (void)hipFree(d_blk);
     (void)hipFree(d_init);
+    (void)hipFree(d_orders);
     (void)hipFree(tcand);
     (void)hipFree(tcnt);
     tcand = nullptr;
     tcnt = nullptr;
     d_init = nullptr;
+    d_orders = nullptr;
     h_init.clear();
+    h_orders.clear();
     perm = nullptr;
     invL = invU = tmp = vec = nullptr;
     slots = nullptr;
     bar = nullptr;
     flags = nullptr;
-    d_blk = nullptr;
     blk.clear();
     n = nblk = 0;
 }
@@ -756,36 +862,72 @@ struct LUStep {
     int slot;     // index of its first diagonal block in the block list
 };
 
-// Panel work of a step on stream st with row map pm: sub-panel a, its
-// unit-lower inverse, then (if the step has a second sub-panel) U_ab =
-// inv(L_aa) A_ab, A_b -= L_a U_ab, sub-panel b and its inverse.
+// L21 = A21 inv(U11) for logical rows [c0 + cols, row_end), one product written
+// in place: N = cols <= 64 is a single column tile of every GEMM variant, so
+// each work-group reads its rows in full before it writes them, and no other
+// work-group touches them.
+template <typename T>
+static int l21_gemm(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, const int* pm,
+                    int c0, int cols, int row_end, const cx<T>* invu)
+{
+    const int rb = c0 + cols;
+    if (row_end <= rb) return 0;
+    return gemm<T>(pr, st, row_end - rb, cols, cols, A + c0, ld, pm + rb, invu, 64, nullptr,
+                   A + c0, ld, pm + rb, 0, false, &w.flops);
+}
+
+// One sub-panel of `cols` columns at c0 whose diagonal block is block `slot`:
+//   without row exchanges  k_diag_block<FACTOR> (LU of the diagonal block and
+//                          the inverses of its triangles), L21 product;
+//   tournament             k_tslu (pivots, L11 \ U11), the inverses, L21 product;
+//   partial                cooperative k_panel (pivots, L11 \ U11 and L21), the
+//                          inverses.
+template <typename T>
+static int sub_panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
+                     const LUPhase& ph, int c0, int cols, int slot)
+{
+    cx<T>* invl = w.invL + (size_t)slot * 4096;
+    cx<T>* invu = w.invU + (size_t)slot * 4096;
+    const bool tournament =
+        g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT;
+    int rc;
+    if (ph.pivot && (rc = panel<T>(w, pr, st, A, ld, pm, c0, cols, ph.row_end, ph.cand_end)))
+        return rc;
+    {
+        Timed tm(pr, ph.pivot ? FAM_OTHER : FAM_PANEL, st);
+        if (ph.pivot)
+            hipLaunchKernelGGL((k_diag_block<T, false>), dim3(1), dim3(256), 0, st, A, ld, pm, c0,
+                               cols, invl, invu, w.flags);
+        else
+            hipLaunchKernelGGL((k_diag_block<T, true>), dim3(1), dim3(256), 0, st, A, ld, pm, c0,
+                               cols, invl, invu, w.flags);
+        TAGS_HIP(hipGetLastError());
+        tm.end();
+    }
+    if (!ph.pivot) w.flops += 8.0 * (double)cols * cols * cols / 3;
+    if (!ph.pivot || tournament) return l21_gemm<T>(w, pr, st, A, ld, pm, c0, cols, ph.row_end, invu);
+    return 0;
+}
+
+// Panel work of a step on stream st with row map pm: sub-panel a, then (if the
+// step has a second sub-panel) U_ab = inv(L_aa) A_ab, A_b -= L_a U_ab and
+// sub-panel b.
 template <typename T>
 static int step_panels(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
                        cx<T>* tmp, const LUPhase& ph, const LUStep& s)
 {
     const int j = s.j, wa = std::min(64, s.jb), wb = s.jb - wa;
     int rc;
-    if ((rc = panel<T>(w, pr, st, A, ld, pm, j, wa, ph.row_end, ph.cand_end))) return rc;
-    cx<T>* inva = w.invL + (size_t)s.slot * 4096;
-    {
-        Timed tm(pr, FAM_OTHER, st);
-        hipLaunchKernelGGL(k_inv_lower<T>, dim3(1), dim3(256), 0, st, A, ld, pm, j, wa, inva);
-        TAGS_HIP(hipGetLastError());
-        tm.end();
-    }
+    if ((rc = sub_panel<T>(w, pr, st, A, ld, pm, ph, j, wa, s.slot))) return rc;
     if (wb > 0) {
         const int c0 = j + 64;
-        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, tmp, inva, j, 64, c0, c0 + wb)))
+        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, tmp, w.invL + (size_t)s.slot * 4096, j,
+                                     64, c0, c0 + wb)))
             return rc;
         if ((rc = gemm<T>(pr, st, ph.row_end - c0, wb, 64, A + j, ld, pm + c0, A + c0, ld, pm + j,
                           A + c0, ld, pm + c0, 1, false, &w.flops)))
             return rc;
-        if ((rc = panel<T>(w, pr, st, A, ld, pm, c0, wb, ph.row_end, ph.cand_end))) return rc;
-        Timed tm(pr, FAM_OTHER, st);
-        hipLaunchKernelGGL(k_inv_lower<T>, dim3(1), dim3(256), 0, st, A, ld, pm, c0, wb,
-                           w.invL + (size_t)(s.slot + 1) * 4096);
-        TAGS_HIP(hipGetLastError());
-        tm.end();
+        if ((rc = sub_panel<T>(w, pr, st, A, ld, pm, ph, c0, wb, s.slot + 1))) return rc;
     }
     return 0;
 }
@@ -840,9 +982,30 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
         }
         init = w.d_init;
     }
+    if (!plan->orders.empty() && w.h_orders != plan->orders) {
+        if ((int)plan->orders.size() > n) return TAGS_ERR_ARG;
+        TAGS_HIP(hipStreamSynchronize(st));
+        TAGS_HIP(hipMemcpy(w.d_orders, plan->orders.data(), sizeof(int) * plan->orders.size(),
+                           hipMemcpyHostToDevice));
+        w.h_orders = plan->orders;
+    }
+    for (const LUPhase& ph : plan->phases)
+        if (ph.order_off >= 0 &&
+            (ph.pivot || ph.order_off + (n - ph.c_begin) > (int)plan->orders.size()))
+            return TAGS_ERR_ARG;
     hipLaunchKernelGGL(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, st, w.perm, init, n,
                        w.flags);
     TAGS_HIP(hipGetLastError());
+    // a phase's row order, set at its first step on the row map that step uses
+    auto set_order = [&](const LUStep& sp, int* pm, hipStream_t sq) -> int {
+        const LUPhase& ph = plan->phases[sp.phase];
+        if (ph.order_off < 0 || sp.j != ph.c_begin) return 0;
+        const int len = n - ph.c_begin;
+        hipLaunchKernelGGL(k_set_order, dim3((len + 255) / 256), dim3(256), 0, sq, pm,
+                           ph.c_begin, len, w.d_orders + ph.order_off);
+        TAGS_HIP(hipGetLastError());
+        return 0;
+    };
     // flat list of steps over the phases
     std::vector<LUStep> steps;
     w.blk.clear();
@@ -864,12 +1027,16 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     // first column, so the rest of step s (pivot rows before it, the same set
     // of physical rows after it, other columns) is unaffected by its changes.
     // The cooperative panel needs the whole device and is not overlapped.
-    const bool overlap = g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT &&
-                         g_lookahead.load(std::memory_order_relaxed);
+    // Panels without row exchanges leave the row map alone: they share it with
+    // the update (unless their step sets a new phase order, which they do on a
+    // copy like a pivoting panel).
+    const bool lookahead = g_lookahead.load(std::memory_order_relaxed);
+    const bool tournament = g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT;
     int* pcur = w.perm;
     int* palt = w.perm_alt;
     if (!steps.empty()) {
         const LUStep& s0 = steps[0];
+        if ((rc = set_order(s0, pcur, st))) return rc;
         if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, w.tmp, plan->phases[s0.phase], s0)))
             return rc;
     }
@@ -883,7 +1050,9 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             re[0] = ph.x_end;
             re[1] = rb[1];
         }
-        if (k + 1 < steps.size() && !overlap) {
+        const LUPhase* nph = k + 1 < steps.size() ? &plan->phases[steps[k + 1].phase] : nullptr;
+        const bool overlap = nph && lookahead && (tournament || !nph->pivot);
+        if (nph && !overlap) {
             // in order: the whole update of this step, then the next panel (the
             // next panel may permute rows beyond this step's row range, which
             // the rest of this update still addresses through the row map)
@@ -891,9 +1060,10 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                 if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, rb[q], re[q])))
                     return rc;
             const LUStep& nx = steps[k + 1];
+            if ((rc = set_order(nx, pcur, st))) return rc;
             if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, w.tmp, plan->phases[nx.phase], nx)))
                 return rc;
-        } else if (k + 1 < steps.size()) {
+        } else if (nph) {
             const LUStep& nx = steps[k + 1];
             const int nb = nx.j, ne = nx.j + nx.jb;
             // columns of the next panel first
@@ -901,13 +1071,20 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                 const int cb = std::max(rb[q], nb), ce = std::min(re[q], ne);
                 if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, cb, ce))) return rc;
             }
-            // the next panel on the second stream, with its own copy of the row map
+            // the next panel on the second stream, with its own copy of the row
+            // map when it changes the map
+            const bool own_map =
+                nph->pivot || (nph->order_off >= 0 && nx.j == nph->c_begin);
             TAGS_HIP(hipEventRecord(w.ev_left, st));
             TAGS_HIP(hipStreamWaitEvent(w.s2, w.ev_left, 0));
-            TAGS_HIP(hipMemcpyAsync(palt, pcur, sizeof(int) * n, hipMemcpyDeviceToDevice, w.s2));
-            if ((rc = step_panels<T>(w, pr, w.s2, A, ld, palt, w.tmp2, plan->phases[nx.phase],
-                                     nx)))
-                return rc;
+            int* pnx = pcur;
+            if (own_map) {
+                TAGS_HIP(hipMemcpyAsync(palt, pcur, sizeof(int) * n, hipMemcpyDeviceToDevice,
+                                        w.s2));
+                if ((rc = set_order(nx, palt, w.s2))) return rc;
+                pnx = palt;
+            }
+            if ((rc = step_panels<T>(w, pr, w.s2, A, ld, pnx, w.tmp2, *nph, nx))) return rc;
             TAGS_HIP(hipEventRecord(w.ev_panel, w.s2));
             // the remaining columns of this step, in g_update_chunks launches
             // (each launch boundary lets waiting panel work-groups start)
@@ -926,7 +1103,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                         return rc;
             }
             TAGS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
-            std::swap(pcur, palt);
+            if (own_map) std::swap(pcur, palt);
         } else {
             for (int q = 0; q < 2; q++)
                 if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, rb[q], re[q])))
@@ -934,21 +1111,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
         }
     }
     if (pcur != w.perm) std::swap(w.perm, w.perm_alt);
-    int* perm = w.perm;
     w.nblk = (int)w.blk.size();
-    if (w.h_blk != w.blk) {
-        TAGS_HIP(hipStreamSynchronize(st));
-        TAGS_HIP(hipMemcpy(w.d_blk, w.blk.data(), sizeof(int) * w.nblk, hipMemcpyHostToDevice));
-        w.h_blk = w.blk;
-    }
-    // inverses of the upper diagonal blocks, needed only by the solves
-    {
-        Timed tm(pr, FAM_OTHER, st);
-        hipLaunchKernelGGL(k_inv_upper<T>, dim3(w.nblk), dim3(256), 0, st, A, ld, perm, w.d_blk,
-                           w.nblk, n, w.invU);
-        TAGS_HIP(hipGetLastError());
-        tm.end();
-    }
     return 0;
 }
 
@@ -1022,17 +1185,48 @@ int to_c32_scaled(const c64* W, long ld, int n, const double* d, c32* W32, long 
 }
 
 int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv_scale, c32* t,
-               hipStream_t st)
+               hipStream_t st, const Grounding* gr, c64* rg)
 {
+    const bool on = gr && gr->g >= 0;
+    if (on) {
+        hipLaunchKernelGGL(k_ground_rhs, dim3(1), dim3(256), 0, st, r, gr->nb, gr->ne, rg);
+        TAGS_HIP(hipGetLastError());
+    }
     hipLaunchKernelGGL(k_prep_rhs32, dim3((n + 255) / 256), dim3(256), 0, st, r, d, perm, n,
-                       inv_scale, t);
+                       inv_scale, on ? gr->g : -1, rg, t);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
-int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st)
+int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st,
+             const Grounding* gr)
 {
-    hipLaunchKernelGGL(k_update_x, dim3((n + 255) / 256), dim3(256), 0, st, x, z, d, n, scale);
+    const bool on = gr && gr->g >= 0;
+    hipLaunchKernelGGL(k_update_x, dim3((n + 255) / 256), dim3(256), 0, st, x, z, d, n, scale,
+                       on ? gr->g : -1, on ? gr->nb : 0, on ? gr->ne : 0);
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+int ground_sums(const c64* W, long ld, int n, const Grounding& gr, c64* gcol, c64* grow, c64* gg,
+                c64* part, hipStream_t st)
+{
+    if (gr.g < gr.nb || gr.g >= gr.ne || gr.nb < 0 || gr.ne > n) return TAGS_ERR_ARG;
+    hipLaunchKernelGGL(k_ground_rowsums, dim3(n), dim3(256), 0, st, W, ld, gr.nb, gr.ne, gcol);
+    hipLaunchKernelGGL(k_ground_colsums, dim3((n + 255) / 256, kGroundChunks), dim3(256), 0, st,
+                       W, ld, n, gr.nb, gr.ne, kGroundChunks, part);
+    hipLaunchKernelGGL(k_ground_finish, dim3(1), dim3(256), 0, st, part, n, gr.nb, gr.ne,
+                       kGroundChunks, grow, gg);
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+int to_c32_grounded(const c64* W, long ld, int n, const Grounding& gr, const c64* gcol,
+                    const c64* grow, const c64* gg, c32* W32, long ld32, hipStream_t st)
+{
+    unsigned gx = (unsigned)std::min(16, (n + 255) / 256);
+    hipLaunchKernelGGL(k_to_c32_grounded, dim3(gx, n), dim3(256), 0, st, W, ld, n, gr.g, gcol,
+                       grow, gg, W32, ld32);
     TAGS_HIP(hipGetLastError());
     return 0;
 }

@@ -69,6 +69,30 @@ __host__ __device__ inline cx<T> csel(bool c, cx<T> a, cx<T> b)
 __host__ __device__ inline double cabs1(cx<double> a) { return fabs(a.x) + fabs(a.y); }
 __host__ __device__ inline float cabs1(cx<float> a) { return fabsf(a.x) + fabsf(a.y); }
 
+// 1 / p for p != 0: p is scaled by a power of two so that |p|^2 lies in
+// [1/4, 2), then the hardware reciprocal is refined by Newton steps (one for
+// float, two for double) and the scale undone.
+__device__ inline float rcp_nr(float d)
+{
+    float r = __builtin_amdgcn_rcpf(d);
+    return r * fmaf(-d, r, 2.0f);
+}
+__device__ inline double rcp_nr(double d)
+{
+    double r = __builtin_amdgcn_rcp(d);
+    r = r * fma(-d, r, 2.0);
+    return r * fma(-d, r, 2.0);
+}
+template <typename T>
+__device__ inline cx<T> crecip_fast(cx<T> p)
+{
+    int e;
+    (void)frexp(fmax(fabs(p.x), fabs(p.y)), &e);
+    const T sx = ldexp(p.x, -e), sy = ldexp(p.y, -e);
+    const T r = rcp_nr(fma(sx, sx, sy * sy));
+    return cx<T>{ldexp(sx * r, -e), ldexp(-sy * r, -e)};
+}
+
 inline c64 from_c(double _Complex z) { return c64{__real__ z, __imag__ z}; }
 inline double _Complex to_c(c64 z) {
     double _Complex r;

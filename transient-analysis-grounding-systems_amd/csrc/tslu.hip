@@ -3,8 +3,8 @@
 // whole panel.
 //
 // A panel is w <= 64 columns starting at column c0. Its candidate pivot rows
-// are the logical rows [c0, cand_end); every row in [c0, row_end) receives its
-// L entries.
+// are the logical rows [c0, cand_end); the rows [c0 + w, row_end) receive
+// their L entries afterwards.
 //
 //  k_tslu       one ordinary launch, one work-group per leaf of S candidate
 //               rows (512 for complex64, 256 for complex128). Each leaf runs
@@ -19,8 +19,9 @@
 //               winners in pivot order; it writes them to the matrix, moves the
 //               winners to logical rows c0 .. c0+w-1 (row-map entries only) and
 //               resets the counters.
-//  k_l21        L21 = A21 inv(U11) for logical rows [c0+w, row_end), by
-//               substitution against U11 held in LDS.
+//
+// L21 = A21 inv(U11) for the rows below is formed afterwards by the caller
+// (one MFMA product with the inverse of U11).
 //
 // Register layout of a work-group's stack (S rows x 64 columns, 16 waves):
 // lane l holds rows {i * 64 + l : i < S/64}, wave w the columns 4w .. 4w+3.
@@ -72,30 +73,6 @@ __device__ inline double readlane_d(double v, int l)
 __device__ inline cx<double> readlane_cx(cx<double> v, int l)
 {
     return cx<double>{readlane_d(v.x, l), readlane_d(v.y, l)};
-}
-
-// 1 / p for p != 0: p is scaled by a power of two so that |p|^2 lies in
-// [1/4, 2), then the hardware reciprocal is refined by Newton steps (one for
-// float, two for double) and the scale undone.
-__device__ inline float rcp_nr(float d)
-{
-    float r = __builtin_amdgcn_rcpf(d);
-    return r * fmaf(-d, r, 2.0f);
-}
-__device__ inline double rcp_nr(double d)
-{
-    double r = __builtin_amdgcn_rcp(d);
-    r = r * fma(-d, r, 2.0);
-    return r * fma(-d, r, 2.0);
-}
-template <typename T>
-__device__ inline cx<T> crecip_fast(cx<T> p)
-{
-    int e;
-    (void)frexp(fmax(fabs(p.x), fabs(p.y)), &e);
-    const T sx = ldexp(p.x, -e), sy = ldexp(p.y, -e);
-    const T r = rcp_nr(fma(sx, sx, sy * sy));
-    return cx<T>{ldexp(sx * r, -e), ldexp(-sy * r, -e)};
 }
 
 // Wave maximum of a per-lane value (DPP row operations, then lane 63); uniform.
@@ -483,78 +460,6 @@ __global__ __launch_bounds__(TS_NT) void k_tslu(TsluArgs<T> p)
     }
 }
 
-// L21 = A21 inv(U11) for logical rows [r_begin, r_end): 64 rows per
-// work-group of 256 threads (thread t: rows {i * 32 + t/8 : i < 2}, columns
-// 8 (t%8) .. +8), forward substitution over the columns with U11 in LDS.
-template <typename T>
-__global__ __launch_bounds__(256) void k_l21(cx<T>* A, long ld, const int* perm, int c0, int w,
-                                             int r_begin, int r_end)
-{
-    __shared__ cx<T> U[64][64];
-    __shared__ cx<T> s_rd[64];  // reciprocals of U's diagonal (zero for a zero pivot)
-    __shared__ int s_pm[64];
-    const int t = threadIdx.x, rg = t >> 3, cg = t & 7, lane = t & 63;
-    __builtin_amdgcn_s_setprio(1);  // on the panel's critical path
-    if (t < 64) s_pm[t] = t < w ? perm[c0 + t] : 0;
-    cx<T> a[2][8];
-    cx<T>* rowp[2];
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-        const int lr = r_begin + blockIdx.x * 64 + i * 32 + rg;
-        rowp[i] = lr < r_end ? A + (long)perm[lr] * ld + c0 : nullptr;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int c = cg * 8 + q;
-            a[i][q] = (rowp[i] && c < w) ? rowp[i][c] : cx<T>{0, 0};
-        }
-    }
-    __syncthreads();
-#pragma unroll
-    for (int it = 0; it < 16; it++) {
-        const int e = t + it * 256;
-        const int k = e >> 6, c = e & 63;
-        U[k][c] = (k < w && c < w && c >= k) ? A[(long)s_pm[k] * ld + c0 + c] : cx<T>{0, 0};
-    }
-    __syncthreads();
-    if (t < 64) {
-        const cx<T> d = U[t][t];
-        s_rd[t] = (d.x == T(0) && d.y == T(0)) ? cx<T>{0, 0} : crecip_fast(d);
-    }
-    __syncthreads();
-    for (int jb = 0; jb < 8; jb++) {
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const int j = jb * 8 + k;
-            if (j >= w) break;
-            const cx<T> rinv = s_rd[j];
-            if (rinv.x == T(0) && rinv.y == T(0)) continue;  // singular: left unscaled
-            const int src = (lane & ~7) | jb;
-            cx<T> l[2];
-#pragma unroll
-            for (int i = 0; i < 2; i++) {
-                l[i] = cmul(a[i][k], rinv);
-                l[i].x = __shfl(l[i].x, src);
-                l[i].y = __shfl(l[i].y, src);
-                a[i][k] = csel(cg == jb, l[i], a[i][k]);
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                cx<T> u = U[j][cg * 8 + q];
-                u = csel(cg * 8 + q > j, u, cx<T>{0, 0});
-#pragma unroll
-                for (int i = 0; i < 2; i++) a[i][q] = cfnma(a[i][q], l[i], u);
-            }
-        }
-    }
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-        if (!rowp[i]) continue;
-#pragma unroll
-        for (int q = 0; q < 8; q++)
-            if (cg * 8 + q < w) rowp[i][cg * 8 + q] = a[i][q];
-    }
-}
-
 int tslu_tree(int cand_rows, int leaf_rows, TsluTree* tr)
 {
     if (cand_rows <= 0 || leaf_rows < 128) return TAGS_ERR_ARG;
@@ -599,12 +504,6 @@ int launch_tslu(cx<T>* A, long ld, int* perm, int c0, int w, int row_end, int ca
     if (rc) return rc;
     hipLaunchKernelGGL(k_tslu<T>, dim3(p.tree.nodes[0]), dim3(TS_NT), 0, st, p);
     TAGS_HIP(hipGetLastError());
-    const int rb = c0 + w;
-    if (row_end > rb) {
-        hipLaunchKernelGGL(k_l21<T>, dim3((row_end - rb + 63) / 64), dim3(256), 0, st, A, ld,
-                           perm, c0, w, rb, row_end);
-        TAGS_HIP(hipGetLastError());
-    }
     return 0;
 }
 
