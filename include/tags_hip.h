@@ -205,6 +205,14 @@ enum tags_plan {
 };
 int tags_sweep_set_plan(tags_sweep *sw, int plan);
 
+/* Frequency samples a sweep keeps in flight (1 .. 4, default 2), each with its
+ * own HIP stream, W and factor work space (about 11 GB for the 60 x 60 m grid
+ * in the mixed form). The calling thread queues a sample on every slot, then
+ * finishes them in issue order, queuing the next sample on each slot as it
+ * frees up, so that one sample's trailing updates keep the GPU busy while
+ * another waits on its panel chain. Results do not depend on the count. */
+int tags_sweep_set_inflight(tags_sweep *sw, int samples);
+
 /* Assembles W for one frequency sample on the GPU exactly as tags_sweep_solve
  * does (pair integrals, images, incidence) and copies it to `wg` (N x N,
  * column-major, reference unknown order [U; I_L; I_T], both triangles). */

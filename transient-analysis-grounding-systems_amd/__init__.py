@@ -107,7 +107,7 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_sweep_set_profiling", "tags_sweep_get_timers", "tags_sweep_destroy",
             "tags_dense_solve", "tags_version", "tags_device_synchronize", "tags_gemm_bench",
             "tags_gemm_variants", "tags_gemm_select", "tags_autotune", "tags_sweep_set_plan",
-            "tags_sweep_assemble", "tags_immittance_solve"]
+            "tags_sweep_assemble", "tags_immittance_solve", "tags_sweep_set_inflight"]
 
 _lib = None
 
@@ -152,6 +152,7 @@ def lib():
         "tags_set_update_chunks": ([I], I),
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
+        "tags_sweep_set_inflight": ([P, I], I),
         "tags_sweep_assemble": ([P, C, C, D, P], I),
         "tags_immittance_solve": ([P, P, S, S, I, I, P], I),
     }
@@ -385,6 +386,11 @@ class Sweep:
         rc = lib().tags_sweep_set_plan(self._h, plan)
         if rc:
             raise RuntimeError("tags_sweep_set_plan failed: %d" % rc)
+
+    def set_inflight(self, samples):
+        rc = lib().tags_sweep_set_inflight(self._h, samples)
+        if rc:
+            raise RuntimeError("tags_sweep_set_inflight failed: %d" % rc)
 
     def set_profiling(self, on=True):
         lib().tags_sweep_set_profiling(self._h, 1 if on else 0)

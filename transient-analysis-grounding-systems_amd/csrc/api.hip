@@ -112,6 +112,8 @@ static PairScalars make_scalars(c64 gamma, c64 s, double mur, c64 kappa, c64 ref
 // Dense solver shared by solve_immittance, tags_dense_solve and the sweep.
 // ---------------------------------------------------------------------------
 struct Solver {
+    static constexpr int kSpecIters = 3;   // refinement steps queued before the first check
+    static constexpr int kMaxIters = 12;   // refinement steps in all
     int n = 0;
     hipStream_t st = nullptr;
     Profiler* pr = nullptr;
@@ -123,8 +125,11 @@ struct Solver {
     c64* r = nullptr;
     c64* b = nullptr;
     c32* t32 = nullptr;
+    // residual norms of refinement step k at norms[3k ..]: |r|, |x| and (k = 0) |W|,
+    // as the bit patterns of non-negative doubles; hnorms is its pinned copy
     unsigned long long* norms = nullptr;
-    unsigned long long* hnorms = nullptr;  // pinned
+    unsigned long long* hnorms = nullptr;
+    int* hflags = nullptr;  // pinned copy of a factorisation's (info, barrier error)
     // grounded node unknowns of the mixed factor (ground.g < 0: off)
     Grounding ground;
     bool ground_valid = false;  // gcol / grow / gsum hold the sums of the current W's node part
@@ -132,6 +137,9 @@ struct Solver {
     c64* grow = nullptr;
     c64* gsum = nullptr;  // [0] node block sum, [1] node sum of the right-hand side
     c64* gpart = nullptr;
+    // the solve in flight (issue_* then finish)
+    bool pending_mixed = false;
+    int queued = 0;  // refinement steps queued by issue_mixed
 
     int ensure(int n_, bool mixed)
     {
@@ -145,8 +153,9 @@ struct Solver {
             TAGS_HIP(hipMalloc(&r, sizeof(c64) * n));
             TAGS_HIP(hipMalloc(&b, sizeof(c64) * n));
             TAGS_HIP(hipMalloc(&t32, sizeof(c32) * n));
-            TAGS_HIP(hipMalloc(&norms, sizeof(unsigned long long) * 4));
-            TAGS_HIP(hipHostMalloc(&hnorms, sizeof(unsigned long long) * 4, 0));
+            TAGS_HIP(hipMalloc(&norms, sizeof(unsigned long long) * 3 * (kMaxIters + 1)));
+            TAGS_HIP(hipHostMalloc(&hnorms, sizeof(unsigned long long) * 3 * (kMaxIters + 1), 0));
+            TAGS_HIP(hipHostMalloc(&hflags, sizeof(int) * 2, 0));
         }
         int rc = w64.alloc(n, di.num_cu, di.lds_limit);
         if (rc) return rc;
@@ -165,6 +174,7 @@ struct Solver {
         hipFree(t32);
         hipFree(norms);
         if (hnorms) hipHostFree(hnorms);
+        if (hflags) hipHostFree(hflags);
         hipFree(W32);
         hipFree(gcol);
         hipFree(grow);
@@ -177,51 +187,78 @@ struct Solver {
         x = r = b = nullptr;
         t32 = nullptr;
         norms = hnorms = nullptr;
+        hflags = nullptr;
     }
     ~Solver() { free_vecs(); }
-
-    int read_info(LUWork<double>& w, int* info)
-    {
-        int f[2];
-        TAGS_HIP(hipMemcpyAsync(f, w.flags, sizeof(f), hipMemcpyDeviceToHost, st));
-        TAGS_HIP(hipStreamSynchronize(st));
-        if (f[1]) return TAGS_ERR_BARRIER;
-        *info = f[0];
-        return 0;
-    }
 
     const LUPlan* plan = nullptr;  // nullptr: dense partial pivoting
     const LUPlan* plan32 = nullptr;  // plan of the mixed form's factor (nullptr: `plan`)
     bool equilibrate = false;      // symmetric row-max scaling before the FP32 factor
                                    // (not applied with grounding)
 
-    // Plain form: factor W (destroyed) in FP64 and solve; x returned in `b`.
-    int solve64(c64* W, c64* bvec, tags_item_stats* stt)
+    double norm_at(int k, int i) const
     {
+        double v;
+        std::memcpy(&v, &hnorms[3 * k + i], 8);
+        return v;
+    }
+
+    // Plain form, first half: factor W (destroyed) in FP64, solve into w64.vec
+    // and copy x to `bvec`, all queued on the stream.
+    int issue64(c64* W, c64* bvec)
+    {
+        pending_mixed = false;
         int rc = lu_factor<double>(w64, W, n, st, pr, plan);
         if (rc) return rc;
-        int info = 0;
-        if ((rc = read_info(w64, &info))) return rc;
+        TAGS_HIP(hipMemcpyAsync(hflags, w64.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+        if ((rc = lu_gather_rhs<double>(w64, bvec, w64.vec, st))) return rc;
+        if ((rc = lu_solve_inplace<double>(w64, W, n, w64.vec, st, pr))) return rc;
+        TAGS_HIP(hipMemcpyAsync(bvec, w64.vec, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
+        return 0;
+    }
+    // Second half: waits, returns 0, info > 0 (x not valid) or an error.
+    int finish64(tags_item_stats* stt)
+    {
+        TAGS_HIP(hipStreamSynchronize(st));
+        if (hflags[1]) return TAGS_ERR_BARRIER;
+        const int info = hflags[0];
         if (stt) {
             stt->precision_used = TAGS_PREC_FP64;
             stt->refine_iters = 0;
             stt->info = info;
             stt->backward_err = -1.0;
         }
-        if (info) return info;
-        if ((rc = lu_gather_rhs<double>(w64, bvec, w64.vec, st))) return rc;
-        if ((rc = lu_solve_inplace<double>(w64, W, n, w64.vec, st, pr))) return rc;
-        TAGS_HIP(hipMemcpyAsync(bvec, w64.vec, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
-        return 0;
+        return info;
+    }
+    int solve64(c64* W, c64* bvec, tags_item_stats* stt)
+    {
+        int rc = issue64(W, bvec);
+        return rc ? rc : finish64(stt);
     }
 
-    // Mixed form: W (FP64, kept intact unless the FP64 fallback runs), b in
-    // `bvec`; solution returned in `bvec`.
-    int solve_mixed(c64* W, c64* bvec, tags_item_stats* stt)
+    // One refinement step k >= 1 queued: correction from the complex64 factor
+    // with the right-hand side scaled by |r_{k-1}| (read on the device),
+    // x += correction, r_k = b - W x and its norms into slot k.
+    int queue_step(c64* W, int k, const Grounding* gr)
     {
         int rc;
-        if (stt)
-            for (int i = 0; i < 8; i++) stt->refine_hist[i] = -1.0;
+        const unsigned long long* rn = norms + 3 * (k - 1);
+        if ((rc = prep_rhs32(r, dscale, w32.perm, n, 1.0, t32, st, gr,
+                             gr ? gsum + 1 : nullptr, rn)))
+            return rc;
+        if ((rc = lu_solve_inplace<float>(w32, W32, n, t32, st, pr))) return rc;
+        if ((rc = update_x(x, t32, dscale, n, 1.0, st, gr, rn))) return rc;
+        return residual64(W, n, n, x, b, r, norms + 3 * k, false, st, pr);
+    }
+
+    // Mixed form, first half: W (FP64, kept intact unless the FP64 fallback
+    // runs) to complex64 (grounded or scaled), its factor, r_0 = b and the
+    // first kSpecIters refinement steps, all queued on the stream without a
+    // host wait; `bvec` holds b.
+    int issue_mixed(c64* W, c64* bvec)
+    {
+        int rc;
+        pending_mixed = true;
         const bool grounded = ground.g >= 0;
         {
             Timed tm(pr, FAM_OTHER, st);
@@ -247,58 +284,63 @@ struct Solver {
             }
             tm.end();
         }
-        const Grounding* gr = grounded ? &ground : nullptr;
         if ((rc = lu_factor<float>(w32, W32, n, st, pr, plan32 ? plan32 : plan))) return rc;
-        int f[2];
-        TAGS_HIP(hipMemcpyAsync(f, w32.flags, sizeof(f), hipMemcpyDeviceToHost, st));
-        TAGS_HIP(hipStreamSynchronize(st));
-        if (f[1]) return TAGS_ERR_BARRIER;
-        const int info32 = f[0];
+        TAGS_HIP(hipMemcpyAsync(hflags, w32.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         TAGS_HIP(hipMemcpyAsync(b, bvec, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
         TAGS_HIP(hipMemsetAsync(x, 0, sizeof(c64) * n, st));
-        double wnorm = 0, rnorm = 0, xnorm = 0, berr = 1.0, prev = 2.0;
-        int it = 0;
-        bool converged = false;
-        if (info32 == 0) {
-            // r = b initially (x = 0); |r| from a residual pass that also gives |W|
-            if ((rc = residual64(W, n, n, x, b, r, norms, true, st, pr))) return rc;
-            TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * 3,
-                                    hipMemcpyDeviceToHost, st));
+        // r_0 = b (x = 0), with |W|
+        if ((rc = residual64(W, n, n, x, b, r, norms, true, st, pr))) return rc;
+        const Grounding* gr = grounded ? &ground : nullptr;
+        for (int k = 1; k <= kSpecIters; k++)
+            if ((rc = queue_step(W, k, gr))) return rc;
+        queued = kSpecIters;
+        TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * 3 * (queued + 1),
+                                hipMemcpyDeviceToHost, st));
+        return 0;
+    }
+
+    // Second half: waits and judges the queued steps by the backward error
+    // |b - W x| / (|W| |x|) of each iterate; while it is above 1e-15 and still
+    // falling by at least half per step, refines further one step at a time (up
+    // to kMaxIters). The iterate in x (the last step run) is accepted at
+    // <= 1e-13; otherwise, or after a complex64 breakdown, the FP64 factor of
+    // the intact W gives the answer. Solution returned in `bvec`.
+    int finish_mixed(c64* W, c64* bvec, tags_item_stats* stt)
+    {
+        TAGS_HIP(hipStreamSynchronize(st));
+        pending_mixed = false;
+        if (hflags[1]) return TAGS_ERR_BARRIER;
+        if (stt)
+            for (int i = 0; i < 8; i++) stt->refine_hist[i] = -1.0;
+        const int info32 = hflags[0];
+        const double wnorm = norm_at(0, 2);
+        const Grounding* gr = ground.g >= 0 ? &ground : nullptr;
+        auto berr_at = [&](int k) {
+            const double rn = norm_at(k, 0), xn = norm_at(k, 1);
+            if (rn == 0.0) return 0.0;
+            return (wnorm > 0 && xn > 0) ? rn / (wnorm * xn) : 1.0;
+        };
+        for (int k = 1; k <= queued && stt && k <= 8; k++) stt->refine_hist[k - 1] = berr_at(k);
+        int done = queued;
+        double berr = berr_at(done);
+        auto keep_going = [&]() {
+            if (info32 != 0 || !(berr == berr) || berr <= 1e-15 || done >= kMaxIters) return false;
+            return done < 2 || berr <= 0.5 * berr_at(done - 1);
+        };
+        while (keep_going()) {
+            int rc = queue_step(W, done + 1, gr);
+            if (rc) return rc;
+            TAGS_HIP(hipMemcpyAsync(hnorms + 3 * (done + 1), norms + 3 * (done + 1),
+                                    sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, st));
             TAGS_HIP(hipStreamSynchronize(st));
-            std::memcpy(&rnorm, &hnorms[0], 8);
-            std::memcpy(&wnorm, &hnorms[2], 8);
-            for (it = 0; it < 12 && rnorm > 0; it++) {
-                if ((rc = prep_rhs32(r, dscale, w32.perm, n, 1.0 / rnorm, t32, st, gr,
-                                     grounded ? gsum + 1 : nullptr)))
-                    return rc;
-                if ((rc = lu_solve_inplace<float>(w32, W32, n, t32, st, pr))) return rc;
-                if ((rc = update_x(x, t32, dscale, n, rnorm, st, gr))) return rc;
-                if ((rc = residual64(W, n, n, x, b, r, norms, false, st, pr))) return rc;
-                TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * 2,
-                                        hipMemcpyDeviceToHost, st));
-                TAGS_HIP(hipStreamSynchronize(st));
-                std::memcpy(&rnorm, &hnorms[0], 8);
-                std::memcpy(&xnorm, &hnorms[1], 8);
-                berr = (wnorm > 0 && xnorm > 0) ? rnorm / (wnorm * xnorm) : 0.0;
-                if (stt && it < 8) stt->refine_hist[it] = berr;
-                if (!(berr == berr)) break;  // NaN
-                if (berr <= 1e-15) {
-                    converged = true;
-                    it++;
-                    break;
-                }
-                if (berr > 0.5 * prev) {
-                    it++;
-                    break;
-                }
-                prev = berr;
-            }
-            if (!converged && berr <= 1e-13) converged = true;
+            done++;
+            berr = berr_at(done);
+            if (stt && done <= 8) stt->refine_hist[done - 1] = berr;
         }
-        if (converged) {
+        if (info32 == 0 && berr == berr && berr <= 1e-13) {
             if (stt) {
                 stt->precision_used = TAGS_PREC_MIXED;
-                stt->refine_iters = it;
+                stt->refine_iters = done;
                 stt->info = 0;
                 stt->backward_err = berr;
             }
@@ -306,9 +348,14 @@ struct Solver {
             return 0;
         }
         // FP64 fallback on the intact W
-        rc = solve64(W, bvec, stt);
-        if (stt) stt->refine_iters = it;
+        int rc = solve64(W, bvec, stt);
+        if (stt) stt->refine_iters = done;
         return rc;
+    }
+    int solve_mixed(c64* W, c64* bvec, tags_item_stats* stt)
+    {
+        int rc = issue_mixed(W, bvec);
+        return rc ? rc : finish_mixed(W, bvec, stt);
     }
 };
 
@@ -949,35 +996,84 @@ int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_ele
 // ---------------------------------------------------------------------------
 // device-resident sweep
 // ---------------------------------------------------------------------------
+// A stream owned for the lifetime of a slot: declared first in a slot, so that
+// it is destroyed after everything that records work or events on it.
+struct OwnedStream {
+    hipStream_t s = nullptr;
+    ~OwnedStream()
+    {
+        if (s) {
+            hipStreamSynchronize(s);
+            hipStreamDestroy(s);
+        }
+    }
+};
+
+// One frequency sample in flight: its own stream, W, right-hand side, solver
+// (factor work space, complex64 copy) and kernel timers. The sweep queues a
+// sample on every slot, then finishes them in turn from the calling thread,
+// queuing the next sample on each slot as it frees up: one sample's trailing
+// updates keep the GPU busy while another waits on its panel chain.
+struct SweepSlot {
+    OwnedStream stream;
+    DevBuf<c64> W, rhs;
+    Profiler prof;
+    Solver sv;
+    long f = -1;  // sample in flight
+    int init(long N)
+    {
+        if (!stream.s) TAGS_HIP(hipStreamCreate(&stream.s));
+        int rc;
+        if (!W.p && (rc = W.alloc((size_t)N * N))) return rc;
+        if (!rhs.p && (rc = rhs.alloc((size_t)N))) return rc;
+        sv.st = stream.s;
+        sv.pr = &prof;
+        return 0;
+    }
+};
+
 struct tags_sweep {
     long m = 0, n = 0, N = 0;
     int plan_kind = TAGS_PLAN_STRUCTURED;
     int layout_kind = -1;  // plan kind the current layout and grounding sums belong to
+    int inflight = 2;      // frequency samples in flight
     tags::LUPlan plan;
     tags::LUPlan plan32;
     tags::BlockLayout layout{};
-    hipStream_t st = nullptr;
+    hipStream_t st = nullptr;  // set-up and assembly stream
     tags::DevSegs e, img;
     int* sn = nullptr;
     int* en = nullptr;
-    tags::c64* W = nullptr;
-    tags::c64* rhs = nullptr;
-    tags::Solver sv;
-    tags::Profiler prof;
+    std::vector<std::unique_ptr<SweepSlot>> slots;
+    bool profiling = false;
 };
+
+// The first k slots exist and are allocated for order N (precision `mixed`).
+static int sweep_slots(tags_sweep* sw, int k, bool mixed)
+{
+    while ((int)sw->slots.size() < k) sw->slots.emplace_back(new SweepSlot());
+    for (int i = 0; i < k; i++) {
+        SweepSlot& sl = *sw->slots[i];
+        int rc = sl.init(sw->N);
+        if (rc) return rc;
+        if ((rc = sl.sv.ensure((int)sw->N, mixed))) return rc;
+        sl.prof.on = sw->profiling;
+    }
+    return 0;
+}
 
 static int sweep_layout(tags_sweep* sw)
 {
     const long n = sw->n, m = sw->m;
-    if (sw->layout_kind != sw->plan_kind) sw->sv.ground_valid = false;
+    const bool changed = sw->layout_kind != sw->plan_kind;
     sw->layout_kind = sw->plan_kind;
-    sw->sv.plan = &sw->plan;
-    sw->sv.plan32 = nullptr;
-    sw->sv.ground = Grounding{};
+    const LUPlan* p64 = &sw->plan;
+    const LUPlan* p32 = nullptr;
+    Grounding gr{};
     switch (sw->plan_kind) {
     case TAGS_PLAN_DENSE:
         sw->layout = BlockLayout{n, m, 0, n, n + m};
-        sw->sv.plan = nullptr;
+        p64 = nullptr;
         break;
     case TAGS_PLAN_STRUCTURED:
         sw->layout = BlockLayout{n, m, 0, n, n + m};
@@ -990,22 +1086,28 @@ static int sweep_layout(tags_sweep* sw)
         sw->plan32 = sw->plan_kind == TAGS_PLAN_GROUNDED
                          ? LUPlan::zl_first_unpivoted((int)m, (int)n)
                          : LUPlan::zl_first((int)m, (int)n, false);
-        sw->sv.plan32 = &sw->plan32;
-        sw->sv.ground = Grounding{(int)(2 * m), (int)(2 * m), (int)(2 * m + n)};
+        p32 = &sw->plan32;
+        gr = Grounding{(int)(2 * m), (int)(2 * m), (int)(2 * m + n)};
         break;
     default:
         sw->layout = BlockLayout{n, m, 2 * m, 0, m};
         sw->plan = LUPlan::zl_first((int)m, (int)n, sw->plan_kind == TAGS_PLAN_BLOCK_PIVOT);
         break;
     }
+    for (auto& sl : sw->slots) {
+        if (changed) sl->sv.ground_valid = false;
+        sl->sv.plan = p64;
+        sl->sv.plan32 = p32;
+        sl->sv.ground = gr;
+    }
     return 0;
 }
 
 // Per-frequency scalars (reference caller formulas) and the assembly of W in
 // the sweep's block layout.
-static int sweep_fill(tags_sweep* sw, TAGS_COMPLEX s_, TAGS_COMPLEX kappa_, double mur,
-                      const TAGS_COMPLEX* gamma_, const TAGS_COMPLEX* ref_l_,
-                      const TAGS_COMPLEX* ref_t_)
+static int sweep_fill(tags_sweep* sw, c64* W, hipStream_t st, Profiler* prof, TAGS_COMPLEX s_,
+                      TAGS_COMPLEX kappa_, double mur, const TAGS_COMPLEX* gamma_,
+                      const TAGS_COMPLEX* ref_l_, const TAGS_COMPLEX* ref_t_)
 {
     const long N = sw->N;
     const c64 sf = from_c(s_), kf = from_c(kappa_);
@@ -1027,10 +1129,10 @@ static int sweep_fill(tags_sweep* sw, TAGS_COMPLEX s_, TAGS_COMPLEX kappa_, doub
         rt = cdiv(csub(kf, se), cadd(kf, se));
     }
     PairScalars sc = make_scalars(gf, sf, mur, kf, rl, rt, INTG_DOUBLE);
-    Timed tm(&sw->prof, FAM_FILL, sw->st);
+    Timed tm(prof, FAM_FILL, st);
     int rc;
-    if ((rc = launch_fill_static(sw->W, N, sw->layout, sw->sn, sw->en, sw->st))) return rc;
-    if ((rc = launch_fill_zblocks(sw->W, N, sw->layout, sw->e.soa(), sw->img.soa(), sc, sw->st)))
+    if ((rc = launch_fill_static(W, N, sw->layout, sw->sn, sw->en, st))) return rc;
+    if ((rc = launch_fill_zblocks(W, N, sw->layout, sw->e.soa(), sw->img.soa(), sc, st)))
         return rc;
     tm.end();
     return 0;
@@ -1097,8 +1199,6 @@ int tags_sweep_create(tags_sweep** out, const Electrode* electrodes, const Elect
             return TAGS_ERR_ARG;
         }
     }
-    sw->sv.st = sw->st;
-    sw->sv.pr = &sw->prof;
     *out = sw.release();
     return 0;
 }
@@ -1108,12 +1208,14 @@ int tags_sweep_assemble(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, doub
 {
     if (!sw || !wg) return TAGS_ERR_ARG;
     const long N = sw->N;
-    if (!sw->W) TAGS_HIP(hipMalloc(&sw->W, sizeof(c64) * N * N));
     sweep_layout(sw);
-    int rc = sweep_fill(sw, s, kappa, mur, nullptr, nullptr, nullptr);
+    DevBuf<c64> W;
+    int rc = W.alloc((size_t)N * N);
     if (rc) return rc;
+    if ((rc = sweep_fill(sw, W.p, sw->st, nullptr, s, kappa, mur, nullptr, nullptr, nullptr)))
+        return rc;
     std::vector<c64> h((size_t)N * N);
-    TAGS_HIP(hipMemcpyAsync(h.data(), sw->W, sizeof(c64) * N * N, hipMemcpyDeviceToHost, sw->st));
+    TAGS_HIP(hipMemcpyAsync(h.data(), W.p, sizeof(c64) * N * N, hipMemcpyDeviceToHost, sw->st));
     TAGS_HIP(hipStreamSynchronize(sw->st));
     // unknown index in the reference order [U; I_L; I_T] -> storage index
     const BlockLayout& L = sw->layout;
@@ -1134,29 +1236,44 @@ int tags_sweep_set_plan(tags_sweep* sw, int plan)
     return 0;
 }
 
+int tags_sweep_set_inflight(tags_sweep* sw, int samples)
+{
+    if (!sw || samples < 1 || samples > 4) return TAGS_ERR_ARG;
+    sw->inflight = samples;
+    return 0;
+}
+
 int tags_sweep_set_profiling(tags_sweep* sw, int on)
 {
     if (!sw) return TAGS_ERR_ARG;
-    sw->prof.on = on != 0;
-    sw->prof.reset();
+    sw->profiling = on != 0;
+    for (auto& sl : sw->slots) {
+        TAGS_HIP(hipStreamSynchronize(sl->stream.s));
+        sl->prof.on = sw->profiling;
+        sl->prof.reset();
+    }
     return 0;
 }
 
 int tags_sweep_get_timers(tags_sweep* sw, tags_timers* t)
 {
     if (!sw || !t) return TAGS_ERR_ARG;
-    TAGS_HIP(hipStreamSynchronize(sw->st));
-    sw->prof.resolve();
-    t->ms_fill = sw->prof.ms[FAM_FILL];
-    t->ms_panel = sw->prof.ms[FAM_PANEL];
-    t->ms_gemm = sw->prof.ms[FAM_GEMM];
-    t->ms_other = sw->prof.ms[FAM_OTHER];
-    t->ms_solve = sw->prof.ms[FAM_SOLVE];
-    t->ms_residual = sw->prof.ms[FAM_RESID];
-    t->gemm_flops = sw->prof.gemm_flops;
-    t->trailing_ms = sw->prof.trailing_ms;
-    t->trailing_flops = sw->prof.trailing_flops;
-    t->trailing_launches = sw->prof.trailing_launches;
+    std::memset(t, 0, sizeof(*t));
+    for (auto& sl : sw->slots) {
+        TAGS_HIP(hipStreamSynchronize(sl->stream.s));
+        Profiler& p = sl->prof;
+        p.resolve();
+        t->ms_fill += p.ms[FAM_FILL];
+        t->ms_panel += p.ms[FAM_PANEL];
+        t->ms_gemm += p.ms[FAM_GEMM];
+        t->ms_other += p.ms[FAM_OTHER];
+        t->ms_solve += p.ms[FAM_SOLVE];
+        t->ms_residual += p.ms[FAM_RESID];
+        t->gemm_flops += p.gemm_flops;
+        t->trailing_ms += p.trailing_ms;
+        t->trailing_flops += p.trailing_flops;
+        t->trailing_launches += p.trailing_launches;
+    }
     return 0;
 }
 
@@ -1164,13 +1281,69 @@ void tags_sweep_destroy(tags_sweep* sw)
 {
     if (!sw) return;
     hipStreamSynchronize(sw->st);
+    sw->slots.clear();
     hipFree(sw->sn);
     hipFree(sw->en);
-    hipFree(sw->W);
-    hipFree(sw->rhs);
     hipStream_t st = sw->st;
     delete sw;
     hipStreamDestroy(st);
+}
+
+// Queues sample f on slot `sl`: assembly, right-hand side and the solve's
+// first half.
+static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const TAGS_COMPLEX* s,
+                       const TAGS_COMPLEX* kappa, double mur, const TAGS_COMPLEX* gamma,
+                       const TAGS_COMPLEX* ref_l, const TAGS_COMPLEX* ref_t, size_t inj_node,
+                       bool mixed)
+{
+    const long N = sw->N;
+    const BlockLayout& L = sw->layout;
+    int rc;
+    sl.f = (long)f;
+    if ((rc = sweep_fill(sw, sl.W.p, sl.stream.s, &sl.prof, s[f], kappa[f], mur,
+                         gamma ? &gamma[f] : nullptr, ref_l ? &ref_l[f] : nullptr,
+                         ref_t ? &ref_t[f] : nullptr)))
+        return rc;
+    // b = unit current at the injection node
+    if ((rc = fill_c64(sl.rhs.p, N, c64{0.0, 0.0}, sl.stream.s))) return rc;
+    if ((rc = fill_c64(sl.rhs.p + L.node_off + inj_node, 1, c64{1.0, 0.0}, sl.stream.s)))
+        return rc;
+    return mixed ? sl.sv.issue_mixed(sl.W.p, sl.rhs.p) : sl.sv.issue64(sl.W.p, sl.rhs.p);
+}
+
+// Finishes the sample in flight on slot `sl` and writes its results.
+static int sweep_finish(tags_sweep* sw, SweepSlot& sl, size_t inj_node, bool mixed,
+                        TAGS_COMPLEX* u_inj, TAGS_COMPLEX* x_out, tags_item_stats* stats)
+{
+    const long N = sw->N, n = sw->n, m = sw->m;
+    const BlockLayout& L = sw->layout;
+    const size_t f = (size_t)sl.f;
+    sl.f = -1;
+    tags_item_stats stt{};
+    int rc = mixed ? sl.sv.finish_mixed(sl.W.p, sl.rhs.p, &stt) : sl.sv.finish64(&stt);
+    if (rc < 0) return rc;
+    if (stats) stats[f] = stt;
+    if (rc > 0) {
+        if (u_inj) u_inj[f] = to_c(c64{NAN, NAN});
+        return 0;
+    }
+    c64 u;
+    hipStream_t st = sl.stream.s;
+    TAGS_HIP(hipMemcpyAsync(&u, sl.rhs.p + L.node_off + inj_node, sizeof(c64),
+                            hipMemcpyDeviceToHost, st));
+    if (x_out) {
+        // reference order [U; I_L; I_T]
+        c64* xo = (c64*)(x_out + f * N);
+        TAGS_HIP(hipMemcpyAsync(xo, sl.rhs.p + L.node_off, sizeof(c64) * n,
+                                hipMemcpyDeviceToHost, st));
+        TAGS_HIP(hipMemcpyAsync(xo + n, sl.rhs.p + L.zl_off, sizeof(c64) * m,
+                                hipMemcpyDeviceToHost, st));
+        TAGS_HIP(hipMemcpyAsync(xo + n + m, sl.rhs.p + L.zt_off, sizeof(c64) * m,
+                                hipMemcpyDeviceToHost, st));
+    }
+    TAGS_HIP(hipStreamSynchronize(st));
+    if (u_inj) u_inj[f] = to_c(u);
+    return 0;
 }
 
 int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
@@ -1181,50 +1354,34 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
 {
     if (!sw || !s || !kappa) return TAGS_ERR_ARG;
     if ((long)inj_node >= sw->n) return TAGS_ERR_ARG;
-    const long N = sw->N, n = sw->n, m = sw->m;
     const bool mixed = precision == TAGS_PREC_MIXED;
-    int rc;
-    if ((rc = sw->sv.ensure((int)N, mixed))) return rc;
-    if (!sw->W) TAGS_HIP(hipMalloc(&sw->W, sizeof(c64) * N * N));
-    if (!sw->rhs) TAGS_HIP(hipMalloc(&sw->rhs, sizeof(c64) * N));
+    // every configured slot is allocated up front (a warm-up call sets all of
+    // them up)
+    int rc = sweep_slots(sw, sw->inflight, mixed);
+    if (rc) return rc;
     sweep_layout(sw);
-    const BlockLayout& L = sw->layout;
-    for (size_t f = 0; f < nfreq; f++) {
-        if ((rc = sweep_fill(sw, s[f], kappa[f], mur, gamma ? &gamma[f] : nullptr,
-                             ref_l ? &ref_l[f] : nullptr, ref_t ? &ref_t[f] : nullptr)))
-            return rc;
-        // b = unit current at the injection node
-        if ((rc = fill_c64(sw->rhs, N, c64{0.0, 0.0}, sw->st))) return rc;
-        if ((rc = fill_c64(sw->rhs + L.node_off + inj_node, 1, c64{1.0, 0.0}, sw->st))) return rc;
-        tags_item_stats stt{};
-        rc = mixed ? sw->sv.solve_mixed(sw->W, sw->rhs, &stt) : sw->sv.solve64(sw->W, sw->rhs, &stt);
-        if (rc < 0) return rc;
-        if (stats) stats[f] = stt;
-        if (rc > 0) {
-            if (u_inj) u_inj[f] = to_c(c64{NAN, NAN});
-            continue;
-        }
-        c64 u;
-        TAGS_HIP(hipMemcpyAsync(&u, sw->rhs + L.node_off + inj_node, sizeof(c64),
-                                hipMemcpyDeviceToHost, sw->st));
-        if (x_out) {
-            // reference order [U; I_L; I_T]
-            c64* xo = (c64*)(x_out + f * N);
-            TAGS_HIP(hipMemcpyAsync(xo, sw->rhs + L.node_off, sizeof(c64) * n,
-                                    hipMemcpyDeviceToHost, sw->st));
-            TAGS_HIP(hipMemcpyAsync(xo + n, sw->rhs + L.zl_off, sizeof(c64) * m,
-                                    hipMemcpyDeviceToHost, sw->st));
-            TAGS_HIP(hipMemcpyAsync(xo + n + m, sw->rhs + L.zt_off, sizeof(c64) * m,
-                                    hipMemcpyDeviceToHost, sw->st));
-        }
-        TAGS_HIP(hipStreamSynchronize(sw->st));
-        if (u_inj) u_inj[f] = to_c(u);
+    const int k = (int)std::min<size_t>((size_t)sw->inflight, std::max<size_t>(nfreq, 1));
+    size_t next = 0;
+    for (int i = 0; i < k && next < nfreq; i++)
+        if ((rc = sweep_issue(sw, *sw->slots[i], next++, s, kappa, mur, gamma, ref_l, ref_t,
+                              inj_node, mixed)))
+            break;
+    // finish the slots in turn (samples in issue order), refilling each one
+    for (int i = 0; !rc; i = (i + 1) % k) {
+        SweepSlot& sl = *sw->slots[i];
+        if (sl.f < 0) break;  // the oldest slot is idle: every sample is done
+        if ((rc = sweep_finish(sw, sl, inj_node, mixed, u_inj, x_out, stats))) break;
+        if (next < nfreq &&
+            (rc = sweep_issue(sw, sl, next++, s, kappa, mur, gamma, ref_l, ref_t, inj_node,
+                              mixed)))
+            break;
     }
-    if (sw->prof.on) {
-        TAGS_HIP(hipStreamSynchronize(sw->st));
-        sw->prof.resolve();
+    // after an error: let every queued sample end before returning
+    for (int i = 0; i < k; i++) {
+        TAGS_HIP(hipStreamSynchronize(sw->slots[i]->stream.s));
+        sw->slots[i]->f = -1;
     }
-    return 0;
+    return rc;
 }
 
 }  // extern "C"

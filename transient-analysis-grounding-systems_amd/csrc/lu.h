@@ -314,11 +314,13 @@ int ground_sums(const c64* W, long ld, int n, const Grounding& gr, c64* gcol, c6
                 c64* part, hipStream_t st);
 int to_c32_grounded(const c64* W, long ld, int n, const Grounding& gr, const c64* gcol,
                     const c64* grow, const c64* gg, c32* W32, long ld32, hipStream_t st);
-// rg: one c64 of device scratch (used with grounding)
+// rg: one c64 of device scratch (used with grounding); rn: when given, |r|
+// (max norm as left by residual64 in device memory) replaces the host scale
 int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv_scale, c32* t,
-               hipStream_t st, const Grounding* gr = nullptr, c64* rg = nullptr);
+               hipStream_t st, const Grounding* gr = nullptr, c64* rg = nullptr,
+               const unsigned long long* rn = nullptr);
 int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st,
-             const Grounding* gr = nullptr);
+             const Grounding* gr = nullptr, const unsigned long long* rn = nullptr);
 int fill_c64(c64* p, long n, c64 v, hipStream_t st);
 int sym_from_upper(c64* W, long ld, int n, hipStream_t st);
 

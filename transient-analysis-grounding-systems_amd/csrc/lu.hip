@@ -587,26 +587,40 @@ __global__ __launch_bounds__(256) void k_ground_rhs(const c64* r, int nb, int ne
     if (threadIdx.x == 0) *out = s;
 }
 
-// t[i] = (float) (d[p] * r'[p] * inv_scale), p = perm[i]; with grounding
+// A scale factor given on the host, or as max |r| in device memory (the bit
+// pattern the residual kernel leaves), read by every thread.
+__device__ inline double rnorm_of(const unsigned long long* rn)
+{
+    return __longlong_as_double((long long)*rn);
+}
+
+// t[i] = (float) (d[p] * r'[p] / |r|), p = perm[i] (|r| = 1 / inv_scale, or
+// taken from rn when rn is given; a zero |r| gives t = 0); with grounding
 // (g >= 0) r'[g] = *rg (the sum of the node entries) and r'[p] = r[p] otherwise
 __global__ void k_prep_rhs32(const c64* r, const double* d, const int* perm, int n,
-                             double inv_scale, int g, const c64* rg, c32* t)
+                             double inv_scale, const unsigned long long* rn, int g,
+                             const c64* rg, c32* t)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
+    if (rn) {
+        const double v = rnorm_of(rn);
+        inv_scale = v > 0.0 ? 1.0 / v : 0.0;
+    }
     const int p = perm[i];
     const double s = d[p] * inv_scale;
     const c64 v = p == g ? *rg : r[p];
     t[i] = c32{(float)(v.x * s), (float)(v.y * s)};
 }
 
-// x[j] += scale * d[j] * z'[j]; with grounding (g >= 0) z'[j] = z[j] + z[g] on
-// the node range except g itself (x = T y)
-__global__ void k_update_x(c64* x, const c32* z, const double* d, int n, double scale, int g,
-                           int nb, int ne)
+// x[j] += scale * d[j] * z'[j] (scale = |r| from rn when given); with grounding
+// (g >= 0) z'[j] = z[j] + z[g] on the node range except g itself (x = T y)
+__global__ void k_update_x(c64* x, const c32* z, const double* d, int n, double scale,
+                           const unsigned long long* rn, int g, int nb, int ne)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
     if (j >= n) return;
+    if (rn) scale = rnorm_of(rn);
     const double s = scale * d[j];
     double zx = (double)z[j].x, zy = (double)z[j].y;
     if (g >= 0 && j != g && j >= nb && j < ne) {
@@ -1185,7 +1199,7 @@ int to_c32_scaled(const c64* W, long ld, int n, const double* d, c32* W32, long 
 }
 
 int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv_scale, c32* t,
-               hipStream_t st, const Grounding* gr, c64* rg)
+               hipStream_t st, const Grounding* gr, c64* rg, const unsigned long long* rn)
 {
     const bool on = gr && gr->g >= 0;
     if (on) {
@@ -1193,17 +1207,17 @@ int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv
         TAGS_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_prep_rhs32, dim3((n + 255) / 256), dim3(256), 0, st, r, d, perm, n,
-                       inv_scale, on ? gr->g : -1, rg, t);
+                       inv_scale, rn, on ? gr->g : -1, rg, t);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
 int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st,
-             const Grounding* gr)
+             const Grounding* gr, const unsigned long long* rn)
 {
     const bool on = gr && gr->g >= 0;
     hipLaunchKernelGGL(k_update_x, dim3((n + 255) / 256), dim3(256), 0, st, x, z, d, n, scale,
-                       on ? gr->g : -1, on ? gr->nb : 0, on ? gr->ne : 0);
+                       rn, on ? gr->g : -1, on ? gr->nb : 0, on ? gr->ne : 0);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
