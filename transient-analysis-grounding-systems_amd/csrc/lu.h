@@ -257,12 +257,10 @@ struct LUWork {
     size_t lds_limit = 65536;
     int* perm = nullptr;
     int* perm_alt = nullptr;      // row map of the look-ahead panel (swapped with perm)
-    cx<T>* tmp2 = nullptr;        // gather buffer of the look-ahead stream (64 x 64)
     hipStream_t s2 = nullptr;     // look-ahead stream (default priority)
     hipEvent_t ev_left = nullptr, ev_panel = nullptr;
     cx<T>* invL = nullptr;  // dense 64 x 64 inverses of the diagonal blocks
     cx<T>* invU = nullptr;
-    cx<T>* tmp = nullptr;
     cx<T>* vec = nullptr;
     PanelSlot<T>* slots = nullptr;
     unsigned* bar = nullptr;

@@ -232,16 +232,17 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
 // physical pm[c0 + r]): with FACTOR, its LU without row exchanges in place;
 // then (in every case) the dense 64 x 64 inverses of its unit-lower and upper
 // triangles (identity padding past w), which the panel's L21 = A21 inv(U11),
-// the rows' U12 = inv(L11) A12 and the triangular solves use. One work-group
-// of 256 threads; thread (r, p) holds columns 16p .. 16p+15 of row r in
+// the rows' U12 = inv(L11) A12 and the triangular solves use. Work-groups of
+// 256 threads; thread (r, p) holds columns 16p .. 16p+15 of row r in
 // registers. Rows and columns past w are padded with the identity, which the
-// sweeps leave unchanged.
+// sweeps leave unchanged. Launched with two work-groups (one per inverse)
+// without FACTOR, with one (which also writes the block back) with FACTOR.
 //   LU:      at step k row k is final; its entries right of k and the
 //            reciprocal of its pivot are broadcast through LDS and every row
 //            r > k forms l = a[r][k] / u_kk and does a[r] -= l u[k].
-//   inverses, one sweep for both: inv(L) top-down (row k final at step k,
-//            X[r] -= L[r][k] X[k] for r > k), inv(U) bottom-up (row k divided
-//            by u_kk at its step, X[r] -= U[r][k] X[k] for r < k).
+//   inverses: inv(L) top-down (row k final at step k, X[r] -= L[r][k] X[k]
+//            for r > k), inv(U) bottom-up (row k divided by u_kk at its step,
+//            X[r] -= U[r][k] X[k] for r < k).
 // One barrier per step (row buffers alternate by parity). A zero pivot gives
 // zero multipliers in its column and is reported in *info (first such column,
 // 1-based, as LAPACK); a caller then falls back to a pivoted factorisation.
@@ -296,63 +297,55 @@ __global__ __launch_bounds__(256) void k_diag_block(cx<T>* A, long ld, const int
             }
         }
     }
-    // a: L \ U of row r. Inverses of both triangles in one sweep.
-    cx<T> xl[16], xu[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        xl[q] = cx<T>{16 * p + q == r ? T(1) : T(0), T(0)};
-        xu[q] = xl[q];
-    }
-    for (int kb = 0; kb < 4; kb++) {
-#pragma unroll
-        for (int kk = 0; kk < 16; kk++) {
-            const int k = kb * 16 + kk, par = k & 1;
-            const int ku = 63 - k, kbu = 3 - kb, kku = 15 - kk;
-            // u_{ku,ku} (held by the quad's thread kbu of row ku) for row ku of inv(U)
-            cx<T> d;
-            d.x = __shfl(a[kku].x, quad | kbu);
-            d.y = __shfl(a[kku].y, quad | kbu);
-            if (r == k) {
-#pragma unroll
-                for (int q = 0; q < 16; q++) s_v[par][16 * p + q] = xl[q];
-            }
-            if (r == ku) {
-                const bool z = d.x == T(0) && d.y == T(0);
-                const cx<T> rd = z ? cx<T>{0, 0} : crecip_fast(d);
-#pragma unroll
-                for (int q = 0; q < 16; q++) {
-                    xu[q] = cmul(xu[q], rd);
-                    s_u[par][16 * p + q] = xu[q];
-                }
-            }
-            __syncthreads();
-            cx<T> lk, uk;
-            lk.x = __shfl(a[kk].x, quad | kb);
-            lk.y = __shfl(a[kk].y, quad | kb);
-            uk.x = __shfl(a[kku].x, quad | kbu);
-            uk.y = __shfl(a[kku].y, quad | kbu);
-            if (r > k) {
-#pragma unroll
-                for (int q = 0; q < 16; q++) xl[q] = cfnma(xl[q], lk, s_v[par][16 * p + q]);
-            }
-            if (r < ku) {
-#pragma unroll
-                for (int q = 0; q < 16; q++) xu[q] = cfnma(xu[q], uk, s_u[par][16 * p + q]);
-            }
-        }
-    }
+    // a: L \ U of row r. Without FACTOR the block is only read: work-group 0
+    // sweeps inv(L) and work-group 1 inv(U). With FACTOR the one work-group
+    // writes the factored block back, so it does both sweeps itself.
     if (FACTOR && r < w) {
         cx<T>* dst = A + (long)s_pm[r] * ld + c0 + 16 * p;
 #pragma unroll
         for (int q = 0; q < 16; q++)
             if (16 * p + q < w) dst[q] = a[q];
     }
-#pragma unroll
-    for (int q = 0; q < 16; q++) {
-        invL[r * 64 + 16 * p + q] = xl[q];
-        invU[r * 64 + 16 * p + q] = xu[q];
-    }
     if (FACTOR && t == 0 && s_zero >= 0 && *info == 0) *info = c0 + s_zero + 1;
+    const int pass0 = FACTOR ? 0 : (int)blockIdx.x, pass1 = FACTOR ? 2 : pass0 + 1;
+    for (int pass = pass0; pass < pass1; pass++) {
+        const bool lower = pass == 0;
+        cx<T> xv[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) xv[q] = cx<T>{16 * p + q == r ? T(1) : T(0), T(0)};
+        for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+            for (int kk = 0; kk < 16; kk++) {
+                // inv(L): row k = 16 kb + kk (top-down); inv(U): row 63 - k (bottom-up)
+                const int kbx = lower ? kb : 3 - kb;
+                const int k = lower ? kb * 16 + kk : 63 - (kb * 16 + kk), par = kk & 1;
+                // column k of this row (the multiplier L[r][k], or U[r][k] and for
+                // row k itself its pivot) from the quad's thread kbx
+                const cx<T> ak = lower ? a[kk] : a[15 - kk];
+                cx<T> c;
+                c.x = __shfl(ak.x, quad | kbx);
+                c.y = __shfl(ak.y, quad | kbx);
+                if (r == k) {
+                    if (!lower) {
+                        const bool z = c.x == T(0) && c.y == T(0);
+                        const cx<T> rd = z ? cx<T>{0, 0} : crecip_fast(c);
+#pragma unroll
+                        for (int q = 0; q < 16; q++) xv[q] = cmul(xv[q], rd);
+                    }
+#pragma unroll
+                    for (int q = 0; q < 16; q++) s_v[par][16 * p + q] = xv[q];
+                }
+                __syncthreads();
+                if (lower ? r > k : r < k) {
+#pragma unroll
+                    for (int q = 0; q < 16; q++) xv[q] = cfnma(xv[q], c, s_v[par][16 * p + q]);
+                }
+            }
+        }
+        cx<T>* out = lower ? invL : invU;
+#pragma unroll
+        for (int q = 0; q < 16; q++) out[r * 64 + 16 * p + q] = xv[q];
+    }
 }
 
 // perm[c0 + i] = order[i] for i < len (a phase's row order)
@@ -360,18 +353,6 @@ __global__ void k_set_order(int* perm, int c0, int len, const int* order)
 {
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < len) perm[c0 + i] = order[i];
-}
-
-// dst[r][c] = A[rowmap[r]][c0 + c] for r < rows, c < cols (dst ld = cols)
-template <typename T>
-__global__ void k_gather_rows(const cx<T>* A, long ld, const int* rowmap, int c0, int rows,
-                              int cols, cx<T>* dst)
-{
-    const int r = blockIdx.y;
-    if (r >= rows) return;
-    const long pr = rowmap[r];
-    for (int c = blockIdx.x * 256 + threadIdx.x; c < cols; c += gridDim.x * 256)
-        dst[(long)r * cols + c] = A[pr * ld + c0 + c];
 }
 
 // perm = init (or identity); flags (info, barrier error) cleared
@@ -680,13 +661,11 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit)
     nblk_cap = (n + 63) / 64 + 8;
     TAGS_HIP(hipMalloc(&perm, sizeof(int) * n));
     TAGS_HIP(hipMalloc(&perm_alt, sizeof(int) * n));
-    TAGS_HIP(hipMalloc(&tmp2, sizeof(cx<T>) * 64 * 64));
     TAGS_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
     TAGS_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
     TAGS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
     TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * 4096 * (size_t)nblk_cap));
     TAGS_HIP(hipMalloc(&invU, sizeof(cx<T>) * 4096 * (size_t)nblk_cap));
-    TAGS_HIP(hipMalloc(&tmp, sizeof(cx<T>) * 128 * (size_t)n));
     TAGS_HIP(hipMalloc(&slots, sizeof(PanelSlot<T>) * 2 * 512));
     TAGS_HIP(hipMalloc(&bar, sizeof(unsigned) * 64));
     TAGS_HIP(hipMalloc(&flags, sizeof(int) * 4));
@@ -710,17 +689,14 @@ void LUWork<T>::release()
     if (s2) (void)hipStreamSynchronize(s2);
     (void)hipFree(perm);
     (void)hipFree(perm_alt);
-    (void)hipFree(tmp2);
     if (s2) (void)hipStreamDestroy(s2);
     if (ev_left) (void)hipEventDestroy(ev_left);
     if (ev_panel) (void)hipEventDestroy(ev_panel);
     perm_alt = nullptr;
-    tmp2 = nullptr;
     s2 = nullptr;
     ev_left = ev_panel = nullptr;
     (void)hipFree(invL);
     (void)hipFree(invU);
-    (void)hipFree(tmp);
     (void)hipFree(slots);
     (void)hipFree(bar);
     (void)hipFree(flags);
@@ -736,7 +712,7 @@ void LUWork<T>::release()
     h_init.clear();
     h_orders.clear();
     perm = nullptr;
-    invL = invU = tmp = vec = nullptr;
+    invL = invU = vec = nullptr;
     slots = nullptr;
     bar = nullptr;
     flags = nullptr;
@@ -840,32 +816,19 @@ int LUWork<T>::setup_kernel_attrs()
     return 0;
 }
 
-template <typename T>
-static int gather(Profiler* pr, hipStream_t st, const cx<T>* A, long ld, const int* rowmap,
-                  int c0, int rows, int cols, cx<T>* dst)
-{
-    if (rows <= 0 || cols <= 0) return 0;
-    Timed tm(pr, FAM_OTHER, st);
-    const unsigned gx = (unsigned)std::min(64, (cols + 255) / 256);
-    hipLaunchKernelGGL(k_gather_rows<T>, dim3(gx, rows), dim3(256), 0, st, A, ld, rowmap, c0, rows,
-                       cols, dst);
-    tm.end();
-    TAGS_HIP(hipGetLastError());
-    return 0;
-}
-
 // U12 for logical rows [r0, r0+h) of the column range [cb, ce): rows are
-// overwritten with inv(L_block) * rows (block inverse `inv`, h <= 64).
+// overwritten with inv(L_block) * rows (block inverse `inv`, h <= 64), as one
+// product written in place: M = K = h <= 64 is a single row tile of every GEMM
+// variant, so the work-group of each column tile reads those rows of its
+// columns in full before it writes them, and no other work-group touches them.
 template <typename T>
 static int apply_linv_rows(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
-                           const int* pm, cx<T>* tmp, const cx<T>* inv, int r0, int h, int cb,
-                           int ce)
+                           const int* pm, const cx<T>* inv, int r0, int h, int cb, int ce)
 {
     const int cols = ce - cb;
     if (cols <= 0) return 0;
-    int rc = gather<T>(pr, st, A, ld, pm + r0, cb, h, cols, tmp);
-    if (rc) return rc;
-    return gemm<T>(pr, st, h, cols, h, inv, 64, nullptr, tmp, cols, nullptr, A + cb, ld, pm + r0,
+    if (h > 64) return TAGS_ERR_ARG;
+    return gemm<T>(pr, st, h, cols, h, inv, 64, nullptr, A + cb, ld, pm + r0, A + cb, ld, pm + r0,
                    0, false, &w.flops);
 }
 
@@ -910,7 +873,7 @@ static int sub_panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long 
     {
         Timed tm(pr, ph.pivot ? FAM_OTHER : FAM_PANEL, st);
         if (ph.pivot)
-            hipLaunchKernelGGL((k_diag_block<T, false>), dim3(1), dim3(256), 0, st, A, ld, pm, c0,
+            hipLaunchKernelGGL((k_diag_block<T, false>), dim3(2), dim3(256), 0, st, A, ld, pm, c0,
                                cols, invl, invu, w.flags);
         else
             hipLaunchKernelGGL((k_diag_block<T, true>), dim3(1), dim3(256), 0, st, A, ld, pm, c0,
@@ -928,14 +891,14 @@ static int sub_panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long 
 // sub-panel b.
 template <typename T>
 static int step_panels(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
-                       cx<T>* tmp, const LUPhase& ph, const LUStep& s)
+                       const LUPhase& ph, const LUStep& s)
 {
     const int j = s.j, wa = std::min(64, s.jb), wb = s.jb - wa;
     int rc;
     if ((rc = sub_panel<T>(w, pr, st, A, ld, pm, ph, j, wa, s.slot))) return rc;
     if (wb > 0) {
         const int c0 = j + 64;
-        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, tmp, w.invL + (size_t)s.slot * 4096, j,
+        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, w.invL + (size_t)s.slot * 4096, j,
                                      64, c0, c0 + wb)))
             return rc;
         if ((rc = gemm<T>(pr, st, ph.row_end - c0, wb, 64, A + j, ld, pm + c0, A + c0, ld, pm + j,
@@ -950,21 +913,20 @@ static int step_panels(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, lon
 // pivot rows, then A22 -= L21 U12 over logical rows [j + jb, row_end).
 template <typename T>
 static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
-                       const int* pm, cx<T>* tmp, const LUPhase& ph, const LUStep& s, int cb,
-                       int ce)
+                       const int* pm, const LUPhase& ph, const LUStep& s, int cb, int ce)
 {
     if (ce <= cb) return 0;
     const int j = s.j, wa = std::min(64, s.jb), wb = s.jb - wa, c1 = j + s.jb;
     const cx<T>* inva = w.invL + (size_t)s.slot * 4096;
     int rc;
-    if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, tmp, inva, j, wa, cb, ce))) return rc;
+    if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, inva, j, wa, cb, ce))) return rc;
     if (wb > 0) {
         const int c0 = j + 64;
         if ((rc = gemm<T>(pr, st, wb, ce - cb, 64, A + j, ld, pm + c0, A + cb, ld, pm + j, A + cb,
                           ld, pm + c0, 1, false, &w.flops)))
             return rc;
-        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, tmp,
-                                     w.invL + (size_t)(s.slot + 1) * 4096, c0, wb, cb, ce)))
+        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, w.invL + (size_t)(s.slot + 1) * 4096,
+                                     c0, wb, cb, ce)))
             return rc;
     }
     return gemm<T>(pr, st, ph.row_end - c1, ce - cb, s.jb, A + j, ld, pm + c1, A + cb, ld, pm + j,
@@ -1051,7 +1013,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     if (!steps.empty()) {
         const LUStep& s0 = steps[0];
         if ((rc = set_order(s0, pcur, st))) return rc;
-        if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, w.tmp, plan->phases[s0.phase], s0)))
+        if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, plan->phases[s0.phase], s0)))
             return rc;
     }
     for (size_t k = 0; k < steps.size(); k++) {
@@ -1071,11 +1033,11 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             // next panel may permute rows beyond this step's row range, which
             // the rest of this update still addresses through the row map)
             for (int q = 0; q < 2; q++)
-                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, rb[q], re[q])))
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, ph, s, rb[q], re[q])))
                     return rc;
             const LUStep& nx = steps[k + 1];
             if ((rc = set_order(nx, pcur, st))) return rc;
-            if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, w.tmp, plan->phases[nx.phase], nx)))
+            if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, plan->phases[nx.phase], nx)))
                 return rc;
         } else if (nph) {
             const LUStep& nx = steps[k + 1];
@@ -1083,7 +1045,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             // columns of the next panel first
             for (int q = 0; q < 2; q++) {
                 const int cb = std::max(rb[q], nb), ce = std::min(re[q], ne);
-                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, cb, ce))) return rc;
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, ph, s, cb, ce))) return rc;
             }
             // the next panel on the second stream, with its own copy of the row
             // map when it changes the map
@@ -1098,7 +1060,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                 if ((rc = set_order(nx, palt, w.s2))) return rc;
                 pnx = palt;
             }
-            if ((rc = step_panels<T>(w, pr, w.s2, A, ld, pnx, w.tmp2, *nph, nx))) return rc;
+            if ((rc = step_panels<T>(w, pr, w.s2, A, ld, pnx, *nph, nx))) return rc;
             TAGS_HIP(hipEventRecord(w.ev_panel, w.s2));
             // the remaining columns of this step, in g_update_chunks launches
             // (each launch boundary lets waiting panel work-groups start)
@@ -1107,12 +1069,12 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                 if (re[q] <= rb[q]) continue;
                 const int lo1 = rb[q], hi1 = std::min(re[q], nb);
                 const int lo2 = std::max(rb[q], ne), hi2 = re[q];
-                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, lo1, hi1)))
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, ph, s, lo1, hi1)))
                     return rc;
                 const int width = hi2 - lo2;
                 const int cw = std::max(128, (width / nch + 63) / 64 * 64);
                 for (int c = lo2; c < hi2; c += cw)
-                    if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, c,
+                    if ((rc = step_update<T>(w, pr, st, A, ld, pcur, ph, s, c,
                                              std::min(hi2, c + cw))))
                         return rc;
             }
@@ -1120,7 +1082,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             if (own_map) std::swap(pcur, palt);
         } else {
             for (int q = 0; q < 2; q++)
-                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, w.tmp, ph, s, rb[q], re[q])))
+                if ((rc = step_update<T>(w, pr, st, A, ld, pcur, ph, s, rb[q], re[q])))
                     return rc;
         }
     }
