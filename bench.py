@@ -130,6 +130,11 @@ def main():
                     help="LU plan of the FP64 form (default 2, Z_L first)")
     ap.add_argument("--update-chunks", type=int, default=None,
                     help="launches per step for the trailing update beside the panel")
+    ap.add_argument("--step-width-mixed", type=int, default=256, choices=(128, 256),
+                    help="columns per factorisation step of the mixed form (the trailing "
+                         "update's K; default 256)")
+    ap.add_argument("--step-width-fp64", type=int, default=128, choices=(128, 256),
+                    help="columns per factorisation step of the FP64 form (default 128)")
     ap.add_argument("--inflight", type=int, default=2,
                     help="frequency samples in flight in the timed sweep (default 2)")
     ap.add_argument("--no-lookahead", action="store_true",
@@ -171,7 +176,9 @@ def main():
         """W warm-up samples, then the K timed samples with `inflight` in
         flight; with `profile` every kernel launch is bracketed by HIP events on
         the stream it runs on."""
-        sw.set_plan(args.plan_mixed if prec == T.PREC_MIXED else args.plan_fp64)
+        mixed = prec == T.PREC_MIXED
+        sw.set_plan(args.plan_mixed if mixed else args.plan_fp64)
+        T.set_step_width(args.step_width_mixed if mixed else args.step_width_fp64)
         sw.set_inflight(inflight)
         if W:
             sw.solve(s[:max(W, inflight)], kap[:max(W, inflight)], 0, prec)
@@ -212,6 +219,7 @@ def main():
                                "current at corner node",
                    "samples_timed": [float(f) for f in freqs[idx]],
                    "plan": PLAN_NAMES[args.plan_mixed],
+                   "step_width": args.step_width_mixed,
                    "gemm_variants": tuned},
         "mixed": {"refine_iters": [q["refine_iters"] for q in st_mx],
                   "precision_used": [q["precision_used"] for q in st_mx],
@@ -231,6 +239,7 @@ def main():
         rel = float(np.max(np.abs(u_mx - u64) / np.abs(u64)))
         out["fp64"] = {"value": K / dt64, "ms_per_step": 1e3 * dt64 / K,
                        "plan": PLAN_NAMES[args.plan_fp64],
+                       "step_width": args.step_width_fp64,
                        "inflight": args.inflight,
                        "ms_per_step_one_in_flight": 1e3 * dt641 / K,
                        "timers_ms_one_in_flight": {k: v for k, v in tm64.items()
@@ -244,7 +253,7 @@ def main():
                        "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
                        "traffic": measured_traffic(),
                        "kernel": "k_trailing_update<float, 128x64 tile, 8 waves, 3-multiply> "
-                                 "(A22 -= L21 U12, K=128)",
+                                 "(A22 -= L21 U12, K=%d)" % args.step_width_mixed,
                        "launches": tm_mx["trailing_launches"],
                        "avg_launch_ms": tm_mx["trailing_ms"] / max(1, tm_mx["trailing_launches"]),
                        "measured_in": ("the timed samples" if one else

@@ -259,6 +259,10 @@ int tags_set_lookahead(int on);
  * TAGS_ERR_ARG for n < 1. */
 int tags_set_update_chunks(int n);
 
+/* Columns per factorisation step of every LU in the process: 256 (default,
+ * four sub-panels of 64; the trailing update is a K = 256 product) or 128. */
+int tags_set_step_width(int columns);
+
 /* Library version string. */
 const char *tags_version(void);
 
@@ -273,6 +277,14 @@ int tags_device_synchronize(void);
  * process; tags_gemm_select installs variants explicitly. */
 int tags_gemm_bench(int M, int N, int K, int precision, int variant, int iters, double *ms);
 int tags_gemm_variants(int precision);
+/* One product through one variant, for validation: C (M x N) = A B (mode 0) or
+ * C -= A B (mode 1), host operands row-major (A M x K, B K x N, C M x N) in
+ * complex128, rounded to the precision's type on the way in; optional row
+ * maps (permutations of [0, M), [0, K), [0, M)) address A, B and C rows as the
+ * LU's row map does. No counterpart in the reference. */
+int tags_gemm_run(int M, int N, int K, int precision, int variant, const TAGS_COMPLEX *A,
+                  const TAGS_COMPLEX *B, TAGS_COMPLEX *C, const int *arow, const int *brow,
+                  const int *crow, int mode);
 int tags_gemm_select(int precision, int big_variant, int small_variant);
 int tags_autotune(int precision, int *big_variant, int *small_variant);
 

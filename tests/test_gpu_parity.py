@@ -268,9 +268,11 @@ def test_c2_sweep_vs_oracle_and_mixed(plan):
     sw.set_plan(plan)
     u64, st64, x64 = sw.solve(s, kap, 0, T.PREC_FP64, want_x=True)
     umx, stmx = sw.solve(s, kap, 0, T.PREC_MIXED)
-    # plan 3 pivots only inside the Z_L block: its FP64 factor is less accurate
-    # (backward error up to ~1e-13 at low frequency) than the refined solution
-    tol = 1e-6 if plan == 3 else 1e-8
+    # plan 3 pivots only inside the Z_L block: its unrefined FP64 factor is less
+    # accurate (backward error up to ~1e-13 at low frequency, forward error up to
+    # ~1e-6 here, varying with the step width's rounding) than the refined
+    # solution
+    tol = 5e-6 if plan == 3 else 1e-8
     assert np.max(np.abs(umx - u64) / np.abs(u64)) <= tol
     if plan in (4, 5):  # the grounded complex64 factors converge on this grid
         assert all(q["precision_used"] == 1 for q in stmx), stmx
@@ -285,21 +287,23 @@ def test_c2_sweep_vs_oracle_and_mixed(plan):
 
 @pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
 def test_lu_modes_agree(plan):
-    """Config 2 sweep under the three factorisation schedules: look-ahead
-    (default; the next panel on a second stream with its own row map),
-    strictly in order, and partial pivoting over the whole panel (cooperative
-    launch, in order). Every plan has a phase boundary where the next panel
-    permutes rows beyond the current step's row range; all three must give the
-    same Z_h, and the mixed form must converge without the FP64 fallback."""
+    """Config 2 sweep under the factorisation schedules: look-ahead (default;
+    the next panel on a second stream with its own row map), strictly in
+    order, partial pivoting over the whole panel (cooperative launch, in
+    order), and look-ahead with 128-column steps instead of 256. Every plan has
+    a phase boundary where the next panel permutes rows beyond the current
+    step's row range; all must give the same Z_h, and the mixed form must
+    converge without the FP64 fallback."""
     rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
     f = O.logspace(2, 7, 400)[::40]
     s = 2j * np.pi * f
     kap = 1e-3 + s * 15.0 * T.EPS0
     res = {}
     try:
-        for mode in ("lookahead", "inorder", "partial"):
-            T.set_lookahead(mode == "lookahead")
+        for mode in ("lookahead", "inorder", "partial", "width128"):
+            T.set_lookahead(mode in ("lookahead", "width128"))
             T.set_panel(T.PANEL_PARTIAL if mode == "partial" else T.PANEL_TOURNAMENT)
+            T.set_step_width(128 if mode == "width128" else 256)
             sw = T.Sweep(e, nodes)
             sw.set_plan(plan)
             u64, _ = sw.solve(s, kap, 0, T.PREC_FP64)
@@ -311,6 +315,7 @@ def test_lu_modes_agree(plan):
     finally:
         T.set_lookahead(True)
         T.set_panel(T.PANEL_TOURNAMENT)
+        T.set_step_width(256)
     ref = res["lookahead"][0]
     tol = 1e-6 if plan == 3 else 1e-9
     for mode, (u64, umx) in res.items():
@@ -441,3 +446,38 @@ def test_sweep_inflight_agree():
             assert [q["refine_iters"] for q in a[1]] == [q["refine_iters"] for q in b[1]]
     assert np.all(np.isfinite(res[1][0][0])) and np.all(np.isfinite(res[1][1][0]))
     assert lone[0] == res[2][1][0][4]
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_gemm_variants_vs_numpy(prec):
+    """Every MFMA GEMM variant of a precision, through tags_gemm_run, against
+    numpy on the same (rounded) operands: C -= A B with row maps on A, B and C
+    (as the LU's trailing update addresses them), and C = A B with ragged
+    M, N, K; elementwise error <= tol (|A| |B| + |C|)."""
+    rng = np.random.default_rng(5)
+    rt = np.complex64 if prec == 1 else np.complex128
+    tol = 1e-5 if prec == 1 else 1e-14
+    cases = [(300, 200, 128, True, 1), (129, 65, 29, False, 0), (64, 64, 64, True, 1)]
+    nv = T.lib().tags_gemm_variants(prec)
+    assert nv >= 1
+    for (M, N, K, maps, mode) in cases:
+        a = (rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))).astype(rt)
+        b = (rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))).astype(rt)
+        c = (rng.standard_normal((M, N)) + 1j * rng.standard_normal((M, N))).astype(rt)
+        ar = rng.permutation(M) if maps else None
+        br = rng.permutation(K) if maps else None
+        cr = rng.permutation(M) if maps else None
+        A = a[ar] if maps else a
+        B = b[br] if maps else b
+        prod = A.astype(np.complex128) @ B.astype(np.complex128)
+        ref = c.astype(np.complex128).copy()
+        rows = cr if maps else np.arange(M)
+        ref[rows] = (ref[rows] - prod) if mode == 1 else prod
+        bound = np.abs(A) @ np.abs(B)
+        bound_full = np.zeros((M, N))
+        bound_full[rows] = bound
+        bound_full += np.abs(c)
+        for v in range(nv):
+            out = T.gemm_run(a, b, c, prec, v, ar, br, cr, mode)
+            err = np.abs(out - ref)
+            assert np.all(err <= tol * bound_full), (v, M, N, K, float((err / bound_full).max()))

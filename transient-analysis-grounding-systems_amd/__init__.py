@@ -107,7 +107,8 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_sweep_set_profiling", "tags_sweep_get_timers", "tags_sweep_destroy",
             "tags_dense_solve", "tags_version", "tags_device_synchronize", "tags_gemm_bench",
             "tags_gemm_variants", "tags_gemm_select", "tags_autotune", "tags_sweep_set_plan",
-            "tags_sweep_assemble", "tags_immittance_solve", "tags_sweep_set_inflight"]
+            "tags_sweep_assemble", "tags_immittance_solve", "tags_sweep_set_inflight",
+            "tags_gemm_run", "tags_set_step_width"]
 
 _lib = None
 
@@ -147,9 +148,11 @@ def lib():
         "tags_gemm_bench": ([I, I, I, I, I, I, ctypes.POINTER(D)], I),
         "tags_gemm_variants": ([I], I),
         "tags_gemm_select": ([I, I, I], I),
+        "tags_gemm_run": ([I, I, I, I, I, P, P, P, P, P, P, I], I),
         "tags_set_panel": ([I], I),
         "tags_set_lookahead": ([I], I),
         "tags_set_update_chunks": ([I], I),
+        "tags_set_step_width": ([I], I),
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
         "tags_sweep_set_inflight": ([P, I], I),
@@ -287,6 +290,21 @@ def gemm_bench(M, N, K, precision, variant, iters=5):
 PANEL_TOURNAMENT, PANEL_PARTIAL = 0, 1
 
 
+def gemm_run(a, b, c, precision, variant, arow=None, brow=None, crow=None, mode=1):
+    """C = A B (mode 0) or C -= A B (mode 1) through one GEMM variant; returns C."""
+    a = np.ascontiguousarray(a, np.complex128)
+    b = np.ascontiguousarray(b, np.complex128)
+    c = np.array(c, dtype=np.complex128, copy=True)
+    M, K = a.shape
+    N = b.shape[1]
+    maps = [None if r is None else np.ascontiguousarray(r, np.int32) for r in (arow, brow, crow)]
+    rc = lib().tags_gemm_run(M, N, K, precision, variant, _p(a), _p(b), _p(c), _p(maps[0]),
+                             _p(maps[1]), _p(maps[2]), mode)
+    if rc:
+        raise RuntimeError("tags_gemm_run failed: %d" % rc)
+    return c
+
+
 def set_panel(kind):
     """Panel algorithm of every LU in the process (PANEL_TOURNAMENT default)."""
     rc = lib().tags_set_panel(kind)
@@ -297,6 +315,12 @@ def set_panel(kind):
 def set_lookahead(on=True):
     """Look-ahead of every later LU in the process (on by default)."""
     lib().tags_set_lookahead(1 if on else 0)
+
+
+def set_step_width(columns):
+    rc = lib().tags_set_step_width(columns)
+    if rc:
+        raise RuntimeError("tags_set_step_width failed: %d" % rc)
 
 
 def set_update_chunks(n):

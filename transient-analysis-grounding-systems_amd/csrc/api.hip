@@ -114,6 +114,9 @@ static PairScalars make_scalars(c64 gamma, c64 s, double mur, c64 kappa, c64 ref
 struct Solver {
     static constexpr int kSpecIters = 3;   // refinement steps queued before the first check
     static constexpr int kMaxIters = 12;   // refinement steps in all
+    // backward error at which refinement stops (about 2 FP64 ulps: the mixed
+    // solution is then at least as accurate as a pivoted FP64 factor's)
+    static constexpr double kConverged = 2e-16;
     int n = 0;
     hipStream_t st = nullptr;
     Profiler* pr = nullptr;
@@ -300,9 +303,9 @@ struct Solver {
     }
 
     // Second half: waits and judges the queued steps by the backward error
-    // |b - W x| / (|W| |x|) of each iterate; while it is above 1e-15 and still
-    // falling by at least half per step, refines further one step at a time (up
-    // to kMaxIters). The iterate in x (the last step run) is accepted at
+    // |b - W x| / (|W| |x|) of each iterate; while it is above kConverged and
+    // still falling by at least half per step, refines further one step at a
+    // time (up to kMaxIters). The iterate in x (the last step run) is accepted at
     // <= 1e-13; otherwise, or after a complex64 breakdown, the FP64 factor of
     // the intact W gives the answer. Solution returned in `bvec`.
     int finish_mixed(c64* W, c64* bvec, tags_item_stats* stt)
@@ -324,7 +327,8 @@ struct Solver {
         int done = queued;
         double berr = berr_at(done);
         auto keep_going = [&]() {
-            if (info32 != 0 || !(berr == berr) || berr <= 1e-15 || done >= kMaxIters) return false;
+            if (info32 != 0 || !(berr == berr) || berr <= kConverged || done >= kMaxIters)
+                return false;
             return done < 2 || berr <= 0.5 * berr_at(done - 1);
         };
         while (keep_going()) {
@@ -443,6 +447,45 @@ static int gemm_bench_t(int M, int N, int K, int variant, int iters, double* ms)
     *ms = t / iters;
     return 0;
 }
+// C (M x N) = A B or C -= A B with host operands (row-major, lda = K, ldb = ldc
+// = N) and optional row maps, through one GEMM variant.
+template <typename T>
+static int gemm_run_t(int M, int N, int K, int variant, const c64* A, const c64* B, c64* C,
+                      const int* arow, const int* brow, const int* crow, int mode)
+{
+    auto bad_map = [](const int* r, int n) {
+        if (!r) return false;
+        for (int i = 0; i < n; i++)
+            if (r[i] < 0 || r[i] >= n) return true;
+        return false;
+    };
+    if (bad_map(arow, M) || bad_map(brow, K) || bad_map(crow, M)) return TAGS_ERR_ARG;
+    const size_t na = (size_t)M * K, nb = (size_t)K * N, nc = (size_t)M * N;
+    std::vector<cx<T>> ha(na), hb(nb), hc(nc);
+    for (size_t i = 0; i < na; i++) ha[i] = cx<T>{(T)A[i].x, (T)A[i].y};
+    for (size_t i = 0; i < nb; i++) hb[i] = cx<T>{(T)B[i].x, (T)B[i].y};
+    for (size_t i = 0; i < nc; i++) hc[i] = cx<T>{(T)C[i].x, (T)C[i].y};
+    Stream S;
+    DevBuf<cx<T>> dA, dB, dC;
+    DevBuf<int> m1, m2, m3;
+    int rc;
+    if ((rc = dA.alloc(na)) || (rc = dB.alloc(nb)) || (rc = dC.alloc(nc)) ||
+        (rc = m1.alloc(M)) || (rc = m2.alloc(K)) || (rc = m3.alloc(M)))
+        return rc;
+    TAGS_HIP(hipMemcpyAsync(dA.p, ha.data(), sizeof(cx<T>) * na, hipMemcpyHostToDevice, S.s));
+    TAGS_HIP(hipMemcpyAsync(dB.p, hb.data(), sizeof(cx<T>) * nb, hipMemcpyHostToDevice, S.s));
+    TAGS_HIP(hipMemcpyAsync(dC.p, hc.data(), sizeof(cx<T>) * nc, hipMemcpyHostToDevice, S.s));
+    if (arow) TAGS_HIP(hipMemcpyAsync(m1.p, arow, sizeof(int) * M, hipMemcpyHostToDevice, S.s));
+    if (brow) TAGS_HIP(hipMemcpyAsync(m2.p, brow, sizeof(int) * K, hipMemcpyHostToDevice, S.s));
+    if (crow) TAGS_HIP(hipMemcpyAsync(m3.p, crow, sizeof(int) * M, hipMemcpyHostToDevice, S.s));
+    GemmArgs<T> g{M, N, K, dA.p, K, arow ? m1.p : nullptr, dB.p, N, brow ? m2.p : nullptr,
+                  dC.p, N, crow ? m3.p : nullptr, mode ? 1 : 0};
+    if ((rc = launch_gemm_variant<T>(g, variant, S.s))) return rc;
+    TAGS_HIP(hipMemcpyAsync(hc.data(), dC.p, sizeof(cx<T>) * nc, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    for (size_t i = 0; i < nc; i++) C[i] = c64{(double)hc[i].x, (double)hc[i].y};
+    return 0;
+}
 }  // namespace tags
 
 extern "C" {
@@ -455,6 +498,18 @@ int tags_gemm_bench(int M, int N, int K, int precision, int variant, int iters, 
 }
 
 int tags_gemm_variants(int precision) { return gemm_variant_count(precision); }
+
+int tags_gemm_run(int M, int N, int K, int precision, int variant, const TAGS_COMPLEX* A,
+                  const TAGS_COMPLEX* B, TAGS_COMPLEX* C, const int* arow, const int* brow,
+                  const int* crow, int mode)
+{
+    if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) return TAGS_ERR_ARG;
+    return precision == TAGS_PREC_MIXED
+               ? gemm_run_t<float>(M, N, K, variant, (const c64*)A, (const c64*)B, (c64*)C, arow,
+                                   brow, crow, mode)
+               : gemm_run_t<double>(M, N, K, variant, (const c64*)A, (const c64*)B, (c64*)C, arow,
+                                    brow, crow, mode);
+}
 
 int tags_set_panel(int kind)
 {
@@ -473,6 +528,13 @@ int tags_set_update_chunks(int n)
 {
     if (n < 1) return TAGS_ERR_ARG;
     lu_set_update_chunks(n);
+    return 0;
+}
+
+int tags_set_step_width(int columns)
+{
+    if (columns != 128 && columns != 256) return TAGS_ERR_ARG;
+    lu_set_step_width(columns);
     return 0;
 }
 

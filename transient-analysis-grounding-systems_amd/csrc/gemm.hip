@@ -224,6 +224,164 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_trailing_update(GemmArgs<T> 
 }
 
 // ---------------------------------------------------------------------------
+// complex64 with 32 x 32 x 2 MFMA blocks: 128 x 128 tiles of 4 waves (2 x 2),
+// each wave 64 x 64 (2 x 2 blocks of v_mfma_f32_32x32x2f32), Gauss
+// three-multiply product, K staged through double-buffered LDS in chunks of
+// 16. Against the 16 x 16 x 4 tiles above, a wave reads half the LDS bytes per
+// MFMA cycle and the work-group meets half the barriers per K. Lane l supplies
+// A[l % 32][l / 32] and B[l / 32][l % 32]; result value v of lane l is
+// D[(v % 4) + 8 (v / 4) + 4 (l / 32)][l % 32] (measured on gfx950).
+// At most 256 registers per lane, so two work-groups share a CU.
+// ---------------------------------------------------------------------------
+constexpr int G32_BM = 128, G32_BN = 128, G32_BK = 16, G32_NT = 256;
+constexpr int G32_SA = G32_BM + 4, G32_SB = G32_BN + 4;  // row skew of 4 complex
+
+__device__ __forceinline__ void cgemm32_block(const GemmArgs<float>& g)
+{
+    typedef float v16 __attribute__((ext_vector_type(16)));
+    __shared__ cx<float> As[2][G32_BK * G32_SA];
+    __shared__ cx<float> Bs[2][G32_BK * G32_SB];
+
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int tiles_n = (g.N + G32_BN - 1) / G32_BN;
+    const int tiles_m = (g.M + G32_BM - 1) / G32_BM;
+    const int ntiles = tiles_n * tiles_m;
+    int bid;
+    {
+        // XCD-aware order: dispatch id b runs on XCD b % 8; each XCD takes a
+        // contiguous range of tiles so that neighbouring tiles share its L2
+        const int per = (ntiles + 7) / 8;
+        bid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    }
+    if (bid >= ntiles) return;
+    const int m0 = (bid / tiles_n) * G32_BM, n0 = (bid % tiles_n) * G32_BN;
+
+    // global-load slots: A row m0 + (tid >> 1), 8 consecutive k from
+    // 8 (tid & 1); B chunk row tid >> 4, 8 consecutive columns from 8 (tid & 15)
+    const int ar = tid >> 1, ak = (tid & 1) * 8;
+    const long aoff = m0 + ar < g.M ? (long)(g.arow ? g.arow[m0 + ar] : m0 + ar) * g.lda : -1;
+    const int bk = tid >> 4, bc = (tid & 15) * 8;
+    cx<float> ra[8], rb[8];
+    auto gload = [&](int k0) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int gk = k0 + ak + q;
+            const bool ok = aoff >= 0 && gk < g.K;
+            const cx<float> v = g.A[ok ? aoff + gk : 0];
+            ra[q] = ok ? v : cx<float>{0, 0};
+        }
+        const int gk = k0 + bk;
+        const long brow = gk < g.K ? (long)(g.brow ? g.brow[gk] : gk) * g.ldb : 0;
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int gc = n0 + bc + q;
+            const bool ok = gk < g.K && gc < g.N;
+            const cx<float> v = g.B[ok ? brow + gc : 0];
+            rb[q] = ok ? v : cx<float>{0, 0};
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int q = 0; q < 8; q++) As[buf][(ak + q) * G32_SA + ar] = ra[q];
+#pragma unroll
+        for (int q = 0; q < 8; q++) Bs[buf][bk * G32_SB + bc + q] = rb[q];
+    };
+
+    v16 acc0[2][2], acc1[2][2], acc2[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                acc0[i][j][v] = 0.f;
+                acc1[i][j][v] = 0.f;
+                acc2[i][j][v] = 0.f;
+            }
+
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    int buf = 0;
+    const int kl = lane >> 5, ml = lane & 31;
+    for (int k0 = 0; k0 < g.K; k0 += G32_BK) {
+        const bool more = k0 + G32_BK < g.K;
+        if (more) gload(k0 + G32_BK);
+        const cx<float>* as = As[buf] + wm * 64 + ml;
+        const cx<float>* bs = Bs[buf] + wn * 64 + ml;
+#pragma unroll
+        for (int kk = 0; kk < G32_BK; kk += 2) {
+            cx<float> a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) a[i] = as[(kk + kl) * G32_SA + i * 32];
+#pragma unroll
+            for (int j = 0; j < 2; j++) b[j] = bs[(kk + kl) * G32_SB + j * 32];
+            float asum[2], bsum[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) asum[i] = a[i].x + a[i].y;
+#pragma unroll
+            for (int j = 0; j < 2; j++) bsum[j] = b[j].x + b[j].y;
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++) {
+                    acc0[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].x, b[j].x, acc0[i][j],
+                                                                      0, 0, 0);
+                    acc1[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(a[i].y, b[j].y, acc1[i][j],
+                                                                      0, 0, 0);
+                    acc2[i][j] = __builtin_amdgcn_mfma_f32_32x32x2f32(asum[i], bsum[j],
+                                                                      acc2[i][j], 0, 0, 0);
+                }
+        }
+        if (more) sstore(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int row = m0 + wm * 64 + i * 32 + (v & 3) + 8 * (v >> 2) + 4 * kl;
+            if (row >= g.M) continue;
+            cx<float>* crow = g.C + (long)(g.crow ? g.crow[row] : row) * g.ldc;
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const int col = n0 + wn * 64 + j * 32 + ml;
+                if (col >= g.N) continue;
+                const float re = acc0[i][j][v] - acc1[i][j][v];
+                const float im = acc2[i][j][v] - acc0[i][j][v] - acc1[i][j][v];
+                if (g.mode == 0) {
+                    crow[col] = cx<float>{re, im};
+                } else {
+                    const cx<float> c = crow[col];
+                    crow[col] = cx<float>{c.x - re, c.y - im};
+                }
+            }
+        }
+}
+
+__global__ __launch_bounds__(G32_NT, 2) void k_cgemm32(GemmArgs<float> g) { cgemm32_block(g); }
+// the same product under the trailing update's name (see k_trailing_update)
+__global__ __launch_bounds__(G32_NT, 2) void k_trailing_update32(GemmArgs<float> g)
+{
+    cgemm32_block(g);
+}
+
+static int launch_v32(const GemmArgs<float>& g, hipStream_t st, bool trailing)
+{
+    const long tiles = (long)((g.M + G32_BM - 1) / G32_BM) * ((g.N + G32_BN - 1) / G32_BN);
+    const long grid = (tiles + 7) / 8 * 8;
+    if (trailing)
+        hipLaunchKernelGGL(k_trailing_update32, dim3((unsigned)grid), dim3(G32_NT), 0, st, g);
+    else
+        hipLaunchKernelGGL(k_cgemm32, dim3((unsigned)grid), dim3(G32_NT), 0, st, g);
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // variants and selection
 // ---------------------------------------------------------------------------
 template <typename T, int BM, int BN, int BK, bool M3, int WGM = 2, int WGN = 2>
@@ -251,7 +409,7 @@ struct GemmShape {
 };
 template <>
 struct GemmTable<float> {
-    static constexpr int count = 18;
+    static constexpr int count = 19;
     static int launch(int v, const GemmArgs<float>& g, hipStream_t st, bool tr)
     {
         switch (v) {
@@ -272,6 +430,7 @@ struct GemmTable<float> {
         case 14: return launch_v<float, 128, 64, 8, true, 4, 2>(g, st, tr);
         case 15: return launch_v<float, 128, 128, 8, true, 4, 2>(g, st, tr);
         case 16: return launch_v<float, 256, 64, 16, true, 8, 2>(g, st, tr);
+        case 18: return launch_v32(g, st, tr);
         default: return launch_v<float, 64, 64, 8, true, 2, 2>(g, st, tr);
         }
     }
@@ -281,7 +440,7 @@ struct GemmTable<float> {
                                            {128, 128}, {64, 64},   {128, 128}, {128, 128},
                                            {128, 128}, {256, 128}, {128, 64},  {128, 128},
                                            {128, 128}, {64, 128},  {128, 64},  {128, 128},
-                                           {256, 64},  {64, 64}};
+                                           {256, 64},  {64, 64},   {G32_BM, G32_BN}};
         return t[v];
     }
     static int tile_m(int v) { return shape(v).bm; }

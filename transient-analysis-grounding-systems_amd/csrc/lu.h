@@ -160,6 +160,9 @@ int lu_panel_kind();
 void lu_set_lookahead(bool on);
 // Launches the look-ahead trailing update of a step is split into (by columns)
 void lu_set_update_chunks(int n);
+// Columns per factorisation step: 128 or 256 (default)
+void lu_set_step_width(int w);
+int lu_step_width();
 
 // Column phases of a factorisation (see lu.hip).
 struct LUPhase {

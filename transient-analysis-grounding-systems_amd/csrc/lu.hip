@@ -15,17 +15,17 @@
 // for a phase are skipped, which is exact for partial pivoting (a zero row is
 // never the largest candidate unless the whole column is zero).
 //
-// Per 128-column step j (two <= 64-column sub-panels a, b):
-//   panel(a)            pivots and L11 \ U11: tournament pivoting (tslu.hip),
+// Per step j of 256 columns (lu_set_step_width; sub-panels t of <= 64 columns):
+//   per sub-panel t     pivots and L_tt \ U_tt: tournament pivoting (tslu.hip),
 //                       partial pivoting over the whole panel (cooperative
 //                       kernel, one grid barrier per column), or, for phases
-//                       without row exchanges, the diagonal block's own LU
-//   inv(diag a)         inverses of both triangles of the diagonal block
-//   L21(a)              A21 inv(U11)                                (MFMA GEMM)
-//   update b            U_ab = inv(L_aa) A_ab; A_b -= L_a U_ab      (MFMA GEMM)
-//   panel(b), inv(diag b), L21(b)
-//   U12                 inv(L11) applied to the pivot rows          (MFMA GEMM)
-//   trailing update     A22 -= L21 U12, K = 128                     (MFMA GEMM)
+//                       without row exchanges, the diagonal block's own LU;
+//                       inverses of both triangles of the diagonal block;
+//                       L21 = A21 inv(U_tt)                          (MFMA GEMM)
+//                       and the step's later columns: U = inv(L_tt) A,
+//                       A_below -= L_t U                              (MFMA GEMM)
+//   U12                 blocked forward solve of the step's pivot rows (MFMA GEMM)
+//   trailing update     A22 -= L21 U12, K = 256                        (MFMA GEMM)
 // The block inverses are kept for the triangular solves.
 //
 // Look-ahead: step j first updates the columns of step j+1's panel; step j+1's
@@ -755,6 +755,13 @@ void lu_set_lookahead(bool on) { g_lookahead.store(on); }
 static std::atomic<int> g_update_chunks{2};
 void lu_set_update_chunks(int n) { g_update_chunks.store(n < 1 ? 1 : n); }
 
+// Columns per step (four sub-panels of 64): the trailing update's K. 256 runs
+// the complex64 product ~26 % faster than 128 (the A22 read-modify-write is
+// shared by twice the flops).
+static std::atomic<int> g_step_width{256};
+void lu_set_step_width(int w) { g_step_width.store(w <= 128 ? 128 : 256); }
+int lu_step_width() { return g_step_width.load(); }
+
 template <typename T>
 static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
                  int c0, int cols, int row_end, int cand_end)
@@ -832,9 +839,9 @@ static int apply_linv_rows(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A,
                    0, false, &w.flops);
 }
 
-// One 128-column step of the factorisation.
+// One step of the factorisation (up to the step width in columns).
 struct LUStep {
-    int j, jb;    // first column, width (<= 128)
+    int j, jb;    // first column, width (<= the step width)
     int phase;    // index into the plan's phases
     int slot;     // index of its first diagonal block in the block list
 };
@@ -886,47 +893,48 @@ static int sub_panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long 
     return 0;
 }
 
-// Panel work of a step on stream st with row map pm: sub-panel a, then (if the
-// step has a second sub-panel) U_ab = inv(L_aa) A_ab, A_b -= L_a U_ab and
-// sub-panel b.
+// Panel work of a step on stream st with row map pm: its sub-panels of <= 64
+// columns in order, each followed (right-looking within the step) by the
+// update of the step's later columns: U_t,rest = inv(L_tt) A_t,rest and
+// A_below,rest -= L_t U_t,rest.
 template <typename T>
 static int step_panels(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
                        const LUPhase& ph, const LUStep& s)
 {
-    const int j = s.j, wa = std::min(64, s.jb), wb = s.jb - wa;
+    const int end = s.j + s.jb;
     int rc;
-    if ((rc = sub_panel<T>(w, pr, st, A, ld, pm, ph, j, wa, s.slot))) return rc;
-    if (wb > 0) {
-        const int c0 = j + 64;
-        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, w.invL + (size_t)s.slot * 4096, j,
-                                     64, c0, c0 + wb)))
+    for (int t = 0, c0 = s.j; c0 < end; t++, c0 += 64) {
+        const int wt = std::min(64, end - c0), c1 = c0 + wt;
+        if ((rc = sub_panel<T>(w, pr, st, A, ld, pm, ph, c0, wt, s.slot + t))) return rc;
+        if (c1 >= end) break;
+        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, w.invL + (size_t)(s.slot + t) * 4096,
+                                     c0, wt, c1, end)))
             return rc;
-        if ((rc = gemm<T>(pr, st, ph.row_end - c0, wb, 64, A + j, ld, pm + c0, A + c0, ld, pm + j,
-                          A + c0, ld, pm + c0, 1, false, &w.flops)))
+        if ((rc = gemm<T>(pr, st, ph.row_end - c1, end - c1, wt, A + c0, ld, pm + c1, A + c1, ld,
+                          pm + c0, A + c1, ld, pm + c1, 1, false, &w.flops)))
             return rc;
-        if ((rc = sub_panel<T>(w, pr, st, A, ld, pm, ph, c0, wb, s.slot + 1))) return rc;
     }
     return 0;
 }
 
-// Update of a step for the trailing columns [cb, ce): U12 of both sub-panels'
-// pivot rows, then A22 -= L21 U12 over logical rows [j + jb, row_end).
+// Update of a step for the trailing columns [cb, ce): U12 of the step's pivot
+// rows, a blocked forward solve over its sub-panels (rows of sub-panel t take
+// the earlier sub-panels' products, then inv(L_tt)), then A22 -= L21 U12 over
+// logical rows [j + jb, row_end) with K = jb.
 template <typename T>
 static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
                        const int* pm, const LUPhase& ph, const LUStep& s, int cb, int ce)
 {
     if (ce <= cb) return 0;
-    const int j = s.j, wa = std::min(64, s.jb), wb = s.jb - wa, c1 = j + s.jb;
-    const cx<T>* inva = w.invL + (size_t)s.slot * 4096;
+    const int j = s.j, c1 = j + s.jb;
     int rc;
-    if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, inva, j, wa, cb, ce))) return rc;
-    if (wb > 0) {
-        const int c0 = j + 64;
-        if ((rc = gemm<T>(pr, st, wb, ce - cb, 64, A + j, ld, pm + c0, A + cb, ld, pm + j, A + cb,
-                          ld, pm + c0, 1, false, &w.flops)))
+    for (int t = 0, r0 = j; r0 < c1; t++, r0 += 64) {
+        const int wt = std::min(64, c1 - r0);
+        if (r0 > j && (rc = gemm<T>(pr, st, wt, ce - cb, r0 - j, A + j, ld, pm + r0, A + cb, ld,
+                                    pm + j, A + cb, ld, pm + r0, 1, false, &w.flops)))
             return rc;
-        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, w.invL + (size_t)(s.slot + 1) * 4096,
-                                     c0, wb, cb, ce)))
+        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, w.invL + (size_t)(s.slot + t) * 4096,
+                                     r0, wt, cb, ce)))
             return rc;
     }
     return gemm<T>(pr, st, ph.row_end - c1, ce - cb, s.jb, A + j, ld, pm + c1, A + cb, ld, pm + j,
@@ -986,13 +994,13 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     std::vector<LUStep> steps;
     w.blk.clear();
     w.flops = 0;
+    const int width = g_step_width.load(std::memory_order_relaxed);
     for (int p = 0; p < (int)plan->phases.size(); p++) {
         const LUPhase& ph = plan->phases[p];
-        for (int j = ph.c_begin; j < ph.c_end; j += 128) {
-            const int jb = std::min(128, ph.c_end - j);
+        for (int j = ph.c_begin; j < ph.c_end; j += width) {
+            const int jb = std::min(width, ph.c_end - j);
             steps.push_back(LUStep{j, jb, p, (int)w.blk.size()});
-            w.blk.push_back(j);
-            if (jb > 64) w.blk.push_back(j + 64);
+            for (int c = j; c < j + jb; c += 64) w.blk.push_back(c);
         }
     }
     if ((int)w.blk.size() > w.nblk_cap) return TAGS_ERR_TOO_LARGE;
