@@ -348,6 +348,111 @@ __global__ __launch_bounds__(256) void k_diag_block(cx<T>* A, long ld, const int
     }
 }
 
+// U12 of a step for the columns [cb, cb + cols), in place: the blocked forward
+// solve over the step's sub-panels t (rows j + 64 t .. of the logical order,
+// physical pm[..]; jb <= 256 rows in all)
+//     U_t = inv(L_tt) (A_t - sum_{u<t} L_tu U_u),
+// with block t's unit-lower inverse (identity padding past its width) at
+// invl + 4096 t. Work-group b owns the CW columns cb + CW b ..: its U_u stay in
+// LDS for the later rows, L_tu and inv(L_tt) pass through LDS in 64 x KC
+// chunks, and thread (tr, tc) holds rows 4 tr .. 4 tr + 3 at columns
+// tc + 16 q (q < CW / 16) of each 64 x CW product.
+template <typename T, int CW, int KC>
+__global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* pm, int j, int jb,
+                                                  const cx<T>* invl, int cb, int cols)
+{
+    constexpr int CT = CW / 16;
+    __shared__ cx<T> Us[4][64][CW];
+    __shared__ cx<T> Ls[64][KC + 1];
+    __shared__ int s_pm[256];
+    const int tid = threadIdx.x, tr = tid >> 4, tc = tid & 15;
+    const int c0 = cb + blockIdx.x * CW, cend = cb + cols;
+    const int nsub = (jb + 63) / 64;
+    s_pm[tid] = tid < jb ? pm[j + tid] : 0;
+    __syncthreads();
+    for (int t = 0; t < nsub; t++) {
+        const int r0 = 64 * t, wt = min(64, jb - r0);
+        cx<T> acc[4][CT];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = 4 * tr + i;
+#pragma unroll
+            for (int q = 0; q < CT; q++) {
+                const int c = c0 + tc + 16 * q;
+                acc[i][q] = r < wt && c < cend ? A[(long)s_pm[r0 + r] * ld + c] : cx<T>{0, 0};
+            }
+        }
+        // acc -= L_tu U_u for the earlier sub-panels
+        for (int u = 0; u < t; u++) {
+            for (int kc = 0; kc < 64; kc += KC) {
+                for (int e = tid; e < 64 * KC; e += 256) {
+                    const int r = e / KC, kk = e % KC;
+                    Ls[r][kk] = r < wt ? A[(long)s_pm[r0 + r] * ld + j + 64 * u + kc + kk]
+                                       : cx<T>{0, 0};
+                }
+                __syncthreads();
+#pragma unroll
+                for (int kk = 0; kk < KC; kk++) {
+                    cx<T> x[CT];
+#pragma unroll
+                    for (int q = 0; q < CT; q++) x[q] = Us[u][kc + kk][tc + 16 * q];
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const cx<T> l = Ls[4 * tr + i][kk];
+#pragma unroll
+                        for (int q = 0; q < CT; q++) acc[i][q] = cfnma(acc[i][q], l, x[q]);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // U_t = inv(L_tt) acc (acc staged in U_t's slot; the inverse negated
+        // on staging so that the fused step adds)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int q = 0; q < CT; q++) Us[t][4 * tr + i][tc + 16 * q] = acc[i][q];
+        cx<T> out[4][CT];
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int q = 0; q < CT; q++) out[i][q] = cx<T>{0, 0};
+        const cx<T>* li = invl + (size_t)t * 4096;
+        for (int kc = 0; kc < 64; kc += KC) {
+            for (int e = tid; e < 64 * KC; e += 256) {
+                const int r = e / KC, kk = e % KC;
+                const cx<T> v = li[r * 64 + kc + kk];
+                Ls[r][kk] = cx<T>{-v.x, -v.y};
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < KC; kk++) {
+                cx<T> x[CT];
+#pragma unroll
+                for (int q = 0; q < CT; q++) x[q] = Us[t][kc + kk][tc + 16 * q];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const cx<T> l = Ls[4 * tr + i][kk];
+#pragma unroll
+                    for (int q = 0; q < CT; q++) out[i][q] = cfnma(out[i][q], l, x[q]);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int r = 4 * tr + i;
+#pragma unroll
+            for (int q = 0; q < CT; q++) {
+                const int c = c0 + tc + 16 * q;
+                Us[t][r][tc + 16 * q] = out[i][q];
+                if (r < wt && c < cend) A[(long)s_pm[r0 + r] * ld + c] = out[i][q];
+            }
+        }
+        __syncthreads();
+    }
+}
+
 // perm[c0 + i] = order[i] for i < len (a phase's row order)
 __global__ void k_set_order(int* perm, int c0, int len, const int* order)
 {
@@ -917,10 +1022,35 @@ static int step_panels(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, lon
     return 0;
 }
 
+// U12 of a step's pivot rows for the columns [cb, ce), in one launch of
+// k_step_u12 (the blocked forward solve over the step's sub-panels).
+template <typename T>
+static int step_u12(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
+                    const int* pm, const LUStep& s, int cb, int ce)
+{
+    constexpr int CW = sizeof(T) == 4 ? 32 : 16, KC = sizeof(T) == 4 ? 16 : 8;
+    const int cols = ce - cb;
+    if (cols <= 0) return 0;
+    if (s.jb > 256) return TAGS_ERR_ARG;
+    double flops = 0;
+    for (int r0 = 0; r0 < s.jb; r0 += 64) {
+        const int wt = std::min(64, s.jb - r0);
+        flops += 8.0 * wt * (double)cols * (r0 + wt);
+    }
+    Timed tm(pr, FAM_GEMM, st);
+    hipLaunchKernelGGL((k_step_u12<T, CW, KC>), dim3((cols + CW - 1) / CW), dim3(256), 0, st, A,
+                       ld, pm, s.j, s.jb, (const cx<T>*)(w.invL + (size_t)s.slot * 4096), cb,
+                       cols);
+    TAGS_HIP(hipGetLastError());
+    tm.end(flops);
+    w.flops += flops;
+    return 0;
+}
+
 // Update of a step for the trailing columns [cb, ce): U12 of the step's pivot
-// rows, a blocked forward solve over its sub-panels (rows of sub-panel t take
-// the earlier sub-panels' products, then inv(L_tt)), then A22 -= L21 U12 over
-// logical rows [j + jb, row_end) with K = jb.
+// rows (the blocked forward solve over its sub-panels: rows of sub-panel t
+// take the earlier sub-panels' products, then inv(L_tt)), then
+// A22 -= L21 U12 over logical rows [j + jb, row_end) with K = jb.
 template <typename T>
 static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
                        const int* pm, const LUPhase& ph, const LUStep& s, int cb, int ce)
@@ -928,15 +1058,7 @@ static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, lon
     if (ce <= cb) return 0;
     const int j = s.j, c1 = j + s.jb;
     int rc;
-    for (int t = 0, r0 = j; r0 < c1; t++, r0 += 64) {
-        const int wt = std::min(64, c1 - r0);
-        if (r0 > j && (rc = gemm<T>(pr, st, wt, ce - cb, r0 - j, A + j, ld, pm + r0, A + cb, ld,
-                                    pm + j, A + cb, ld, pm + r0, 1, false, &w.flops)))
-            return rc;
-        if ((rc = apply_linv_rows<T>(w, pr, st, A, ld, pm, w.invL + (size_t)(s.slot + t) * 4096,
-                                     r0, wt, cb, ce)))
-            return rc;
-    }
+    if ((rc = step_u12<T>(w, pr, st, A, ld, pm, s, cb, ce))) return rc;
     return gemm<T>(pr, st, ph.row_end - c1, ce - cb, s.jb, A + j, ld, pm + c1, A + cb, ld, pm + j,
                    A + cb, ld, pm + c1, 1, true, &w.flops);
 }
