@@ -41,7 +41,7 @@ struct Mfma<float> {
     static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
-template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN>
+template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN, int AL>
 __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
 {
     using V4 = typename Mfma<T>::v4;
@@ -49,10 +49,14 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
     constexpr int WM = BM / WGM, WN = BN / WGN, TM = WM / 16, TN = WN / 16;
     constexpr int SKEW = 128 / (int)sizeof(cx<T>);
     constexpr int SA = BM + SKEW, SB = BN + SKEW;
+    // AL = 1: the A tile row-major (k contiguous, row pitch BK + 1), so that
+    // the 16 rows of an MFMA operand read and the 8-lane row segments of the
+    // staging stores each fall on distinct banks
+    constexpr int SAK = BK + 1;
     constexpr int LA = BM * BK / NT, LB = BK * BN / NT;
     static_assert(LA * NT == BM * BK && LB * NT == BK * BN, "tile/thread mismatch");
     static_assert(TM * 16 == WM && TN * 16 == WN, "wave tile");
-    __shared__ cx<T> As[2][BK * SA];
+    __shared__ cx<T> As[2][AL ? BM * SAK : BK * SA];
     __shared__ cx<T> Bs[2][BK * SB];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
@@ -90,15 +94,17 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
         brw[q] = e / BN;
         bcol[q] = (n0 + e % BN) < g.N ? e % BN : -1;
     }
+    // registers of the next chunk, fetched from HBM while the MFMAs of the
+    // current one run
     cx<T> ra[LA], rb[LB];
-#define TAGS_GLOAD(k0_)                                                                   \
+#define TAGS_GLOAD(k0_, RA, RB)                                                           \
     {                                                                                     \
         _Pragma("unroll") for (int q = 0; q < LA; q++)                                    \
         {                                                                                 \
             const int gc = (k0_) + acol[q];                                               \
             const bool ok = aoff[q] >= 0 && gc < g.K;                                     \
             const cx<T> v = g.A[ok ? aoff[q] + gc : 0];                                   \
-            ra[q] = ok ? v : cx<T>{0, 0};                                                 \
+            RA[q] = ok ? v : cx<T>{0, 0};                                                 \
         }                                                                                 \
         _Pragma("unroll") for (int q = 0; q < LB; q++)                                    \
         {                                                                                 \
@@ -106,16 +112,17 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
             const bool ok = gk < g.K && bcol[q] >= 0;                                     \
             const long pr = ok ? (long)(g.brow ? g.brow[gk] : gk) * g.ldb + n0 + bcol[q] : 0; \
             const cx<T> v = g.B[pr];                                                      \
-            rb[q] = ok ? v : cx<T>{0, 0};                                                 \
+            RB[q] = ok ? v : cx<T>{0, 0};                                                 \
         }                                                                                 \
     }
-#define TAGS_SSTORE(buf_)                                                                 \
+#define TAGS_SSTORE(buf_, RA, RB)                                                         \
     {                                                                                     \
-        _Pragma("unroll") for (int q = 0; q < LA; q++) As[buf_][acol[q] * SA + arw[q]] = ra[q]; \
+        _Pragma("unroll") for (int q = 0; q < LA; q++)                                    \
+            As[buf_][AL ? arw[q] * SAK + acol[q] : acol[q] * SA + arw[q]] = RA[q];       \
         _Pragma("unroll") for (int q = 0; q < LB; q++)                                    \
         {                                                                                 \
             const int e = tid + NT * q;                                                   \
-            Bs[buf_][brw[q] * SB + (e % BN)] = rb[q];                                     \
+            Bs[buf_][brw[q] * SB + (e % BN)] = RB[q];                                     \
         }                                                                                 \
     }
 
@@ -129,21 +136,18 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
             if constexpr (M3) acc2[i][j] = V4{0, 0, 0, 0};
         }
 
-    TAGS_GLOAD(0);
-    TAGS_SSTORE(0);
-    __syncthreads();
-    int buf = 0;
-    for (int k0 = 0; k0 < g.K; k0 += BK) {
-        const bool more = k0 + BK < g.K;
-        if (more) TAGS_GLOAD(k0 + BK);
-        const cx<T>* as = As[buf];
-        const cx<T>* bs = Bs[buf];
+    // the MFMAs of the chunk in LDS buffer buf_
+    auto mma = [&](int buf_) {
+        const cx<T>* as = As[buf_];
+        const cx<T>* bs = Bs[buf_];
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             cx<T> a[TM], b[TN];
             const int kr = kk + (lane >> 4);
 #pragma unroll
-            for (int i = 0; i < TM; i++) a[i] = as[kr * SA + wm * WM + i * 16 + (lane & 15)];
+            for (int i = 0; i < TM; i++)
+                a[i] = AL ? as[(wm * WM + i * 16 + (lane & 15)) * SAK + kr]
+                          : as[kr * SA + wm * WM + i * 16 + (lane & 15)];
 #pragma unroll
             for (int j = 0; j < TN; j++) b[j] = bs[kr * SB + wn * WN + j * 16 + (lane & 15)];
             if constexpr (M3) {
@@ -172,7 +176,17 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
                     }
             }
         }
-        if (more) TAGS_SSTORE(buf ^ 1);
+    };
+
+    TAGS_GLOAD(0, ra, rb);
+    TAGS_SSTORE(0, ra, rb);
+    __syncthreads();
+    int buf = 0;
+    for (int k0 = 0; k0 < g.K; k0 += BK) {
+        const bool more = k0 + BK < g.K;
+        if (more) TAGS_GLOAD(k0 + BK, ra, rb);
+        mma(buf);
+        if (more) TAGS_SSTORE(buf ^ 1, ra, rb);
         __syncthreads();
         buf ^= 1;
     }
@@ -208,19 +222,19 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
         }
 }
 
-template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN>
+template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN, int AL>
 __global__ __launch_bounds__(64 * WGM * WGN) void k_cgemm(GemmArgs<T> g)
 {
-    cgemm_block<T, BM, BN, BK, M3, WGM, WGN>(g);
+    cgemm_block<T, BM, BN, BK, M3, WGM, WGN, AL>(g);
 }
 
 // The same product under its own name for the LU's trailing update
 // A22 -= L21 U12, so that kernel traces and counters attribute it apart from
 // the other products (panel rows, triangular solves).
-template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN>
+template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN, int AL>
 __global__ __launch_bounds__(64 * WGM * WGN) void k_trailing_update(GemmArgs<T> g)
 {
-    cgemm_block<T, BM, BN, BK, M3, WGM, WGN>(g);
+    cgemm_block<T, BM, BN, BK, M3, WGM, WGN, AL>(g);
 }
 
 // ---------------------------------------------------------------------------
@@ -384,18 +398,18 @@ static int launch_v32(const GemmArgs<float>& g, hipStream_t st, bool trailing)
 // ---------------------------------------------------------------------------
 // variants and selection
 // ---------------------------------------------------------------------------
-template <typename T, int BM, int BN, int BK, bool M3, int WGM = 2, int WGN = 2>
+template <typename T, int BM, int BN, int BK, bool M3, int WGM = 2, int WGN = 2, int AL = 0>
 static int launch_v(const GemmArgs<T>& g, hipStream_t st, bool trailing)
 {
     const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
     const long grid = (tiles + 7) / 8 * 8;
     const dim3 block(64 * WGM * WGN);
     if (trailing)
-        hipLaunchKernelGGL((k_trailing_update<T, BM, BN, BK, M3, WGM, WGN>), dim3((unsigned)grid),
-                           block, 0, st, g);
+        hipLaunchKernelGGL((k_trailing_update<T, BM, BN, BK, M3, WGM, WGN, AL>),
+                           dim3((unsigned)grid), block, 0, st, g);
     else
-        hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3, WGM, WGN>), dim3((unsigned)grid), block,
-                           0, st, g);
+        hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3, WGM, WGN, AL>), dim3((unsigned)grid),
+                           block, 0, st, g);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
@@ -409,7 +423,7 @@ struct GemmShape {
 };
 template <>
 struct GemmTable<float> {
-    static constexpr int count = 19;
+    static constexpr int count = 22;
     static int launch(int v, const GemmArgs<float>& g, hipStream_t st, bool tr)
     {
         switch (v) {
@@ -431,6 +445,9 @@ struct GemmTable<float> {
         case 15: return launch_v<float, 128, 128, 8, true, 4, 2>(g, st, tr);
         case 16: return launch_v<float, 256, 64, 16, true, 8, 2>(g, st, tr);
         case 18: return launch_v32(g, st, tr);
+        case 19: return launch_v<float, 128, 64, 8, true, 4, 2, 1>(g, st, tr);
+        case 20: return launch_v<float, 128, 64, 16, true, 4, 2, 1>(g, st, tr);
+        case 21: return launch_v<float, 128, 128, 8, true, 4, 4, 1>(g, st, tr);
         default: return launch_v<float, 64, 64, 8, true, 2, 2>(g, st, tr);
         }
     }
@@ -440,7 +457,8 @@ struct GemmTable<float> {
                                            {128, 128}, {64, 64},   {128, 128}, {128, 128},
                                            {128, 128}, {256, 128}, {128, 64},  {128, 128},
                                            {128, 128}, {64, 128},  {128, 64},  {128, 128},
-                                           {256, 64},  {64, 64},   {G32_BM, G32_BN}};
+                                           {256, 64},  {64, 64},   {G32_BM, G32_BN},
+                                           {128, 64},  {128, 64},  {128, 128}};
         return t[v];
     }
     static int tile_m(int v) { return shape(v).bm; }
@@ -448,7 +466,7 @@ struct GemmTable<float> {
 };
 template <>
 struct GemmTable<double> {
-    static constexpr int count = 13;
+    static constexpr int count = 15;
     static int launch(int v, const GemmArgs<double>& g, hipStream_t st, bool tr)
     {
         switch (v) {
@@ -464,6 +482,8 @@ struct GemmTable<double> {
         case 9: return launch_v<double, 64, 128, 8, false, 2, 4>(g, st, tr);
         case 10: return launch_v<double, 256, 64, 8, false, 4, 2>(g, st, tr);
         case 11: return launch_v<double, 64, 64, 16, false, 2, 2>(g, st, tr);
+        case 13: return launch_v<double, 128, 64, 8, false, 4, 2, 1>(g, st, tr);
+        case 14: return launch_v<double, 128, 64, 16, false, 4, 2, 1>(g, st, tr);
         default: return launch_v<double, 64, 64, 4, false, 2, 2>(g, st, tr);
         }
     }
@@ -472,7 +492,7 @@ struct GemmTable<double> {
         static const GemmShape t[count] = {{64, 64},  {128, 64},  {64, 128}, {128, 64},
                                            {128, 128}, {128, 128}, {128, 64}, {128, 128},
                                            {128, 64}, {64, 128},  {256, 64},  {64, 64},
-                                           {64, 64}};
+                                           {64, 64},  {128, 64},  {128, 64}};
         return t[v];
     }
     static int tile_m(int v) { return shape(v).bm; }
@@ -480,11 +500,12 @@ struct GemmTable<double> {
 };
 
 // Selected variant for large and small problems, [double, float]. The
-// defaults are the fastest on MI355X at the LU's shapes (K = 128): 128 x 64
-// tiles of 8 waves (32 x 32 per wave; three-multiply for complex64) for large
+// defaults are the fastest on MI355X at the LU's shapes (K = 128 for
+// complex128, 256 for complex64): 128 x 64 tiles of 8 waves (32 x 32 per
+// wave; three-multiply for complex64; A tile row-major in LDS) for large
 // products, 64 x 64 tiles of 4 waves for small ones; tags_autotune may replace
 // them.
-static std::atomic<int> g_big_variant[2] = {6, 14};
+static std::atomic<int> g_big_variant[2] = {13, 19};
 static std::atomic<int> g_small_variant[2] = {0, 5};
 
 template <typename T>
