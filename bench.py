@@ -34,7 +34,8 @@ PEAK = {"fp64": 78.6, "fp32": 157.3}  # dense MFMA TFLOP/s, MI355X
 GRID = (31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)
 SIGMA, EPSR = 1.0 / 500.0, 20.0
 NFREQ = 256
-PLAN_NAMES = ["dense", "structured", "zl_first", "block_pivot", "grounded", "grounded_pivoted"]
+PLAN_NAMES = ["dense", "structured", "zl_first", "block_pivot", "grounded", "grounded_pivoted",
+              "nodal", "hybrid"]
 
 
 def lu_flops(N):

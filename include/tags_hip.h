@@ -194,16 +194,42 @@ int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
  *                       breakdown or stalled refinement falls back to the
  *                       pivoted FP64 factor;
  * TAGS_PLAN_GROUNDED_PIVOTED as GROUNDED, but the complex64 factor of T^T W T
- *                       pivots like ZL_FIRST. */
+ *                       pivots like ZL_FIRST;
+ * TAGS_PLAN_NODAL       stored and factored as ZL_FIRST in FP64. The mixed
+ *                       form's complex64 factor is a block elimination of the
+ *                       currents: Z_L and Z_T are factored with row exchanges
+ *                       among their own rows, X = Z^-1 A for each, and the
+ *                       node block is set to its Schur complement
+ *                       -(A^T X_L + B^T X_T) from the sparse incidence rows,
+ *                       then factored (about half the flops of ZL_FIRST: the
+ *                       incidence blocks are not treated as dense). It needs
+ *                       more refinement steps toward low frequencies, where a
+ *                       stalled refinement falls back to the FP64 factor;
+ * TAGS_PLAN_HYBRID      stored and factored as ZL_FIRST in FP64. In a sweep's
+ *                       mixed form, each sample takes NODAL's complex64 factor
+ *                       at and above the switch frequency
+ *                       (tags_sweep_set_nodal_above) and ZL_FIRST's below it;
+ *                       tags_immittance_solve, which has no frequency, treats
+ *                       it as ZL_FIRST. */
 enum tags_plan {
     TAGS_PLAN_DENSE = 0,
     TAGS_PLAN_STRUCTURED = 1,
     TAGS_PLAN_ZL_FIRST = 2,
     TAGS_PLAN_BLOCK_PIVOT = 3,
     TAGS_PLAN_GROUNDED = 4,
-    TAGS_PLAN_GROUNDED_PIVOTED = 5
+    TAGS_PLAN_GROUNDED_PIVOTED = 5,
+    TAGS_PLAN_NODAL = 6,
+    TAGS_PLAN_HYBRID = 7
 };
 int tags_sweep_set_plan(tags_sweep *sw, int plan);
+
+/* Switch frequency of TAGS_PLAN_HYBRID (default 2e4 Hz): in the mixed form a
+ * sample at |s| / 2 pi >= hz takes the nodal plan's complex64 factor, a lower
+ * one ZL_FIRST's. A nodal sample that falls back to FP64 or needs more than 6
+ * refinement steps raises the switch to 1.5 times its frequency for the rest
+ * of the sweep (until this is called again). No counterpart in the reference,
+ * which factors every sample alike. Returns 0, or TAGS_ERR_ARG for hz < 0. */
+int tags_sweep_set_nodal_above(tags_sweep *sw, double hz);
 
 /* Frequency samples a sweep keeps in flight (1 .. 4, default 2), each with its
  * own HIP stream, W and factor work space (about 11 GB for the 60 x 60 m grid

@@ -255,7 +255,7 @@ def test_c1_sweep_vs_oracle():
     sw.close()
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_c2_sweep_vs_oracle_and_mixed(plan):
     """Config 2 (20 x 20 m grid, N = 1,185), every factorisation plan: FP64 vs
     the oracle at a few frequencies, mixed vs FP64 on 40 frequencies, KCL and
@@ -285,7 +285,7 @@ def test_c2_sweep_vs_oracle_and_mixed(plan):
     sw.close()
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_lu_modes_agree(plan):
     """Config 2 sweep under the factorisation schedules: look-ahead (default;
     the next panel on a second stream with its own row map), strictly in
@@ -309,7 +309,7 @@ def test_lu_modes_agree(plan):
             u64, _ = sw.solve(s, kap, 0, T.PREC_FP64)
             umx, stmx = sw.solve(s, kap, 0, T.PREC_MIXED)
             sw.close()
-            if plan != 3:  # plan 3 pivots within the Z_L block only
+            if plan not in (3, 6, 7):  # these pivot within the Z_L block only
                 assert all(q["precision_used"] == 1 for q in stmx), (mode, stmx)
             res[mode] = (u64, umx)
     finally:
@@ -353,7 +353,7 @@ def test_c3_mixed_vs_fp64_property():
     sw.close()
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5, 6, 7])
 def test_sweep_assembly_vs_oracle(plan):
     """W assembled in HBM by the sweep (pair integrals, images, incidence) vs
     the oracle pipeline, in the reference unknown order, for every storage
@@ -373,7 +373,7 @@ def test_sweep_assembly_vs_oracle(plan):
     sw.close()
 
 
-@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("plan", [0, 1, 2, 3, 4, 5, 6])
 @pytest.mark.parametrize("prec", [0, 1])
 def test_immittance_solve_plans(plan, prec):
     """Structured factorisation plans on matrices with the immittance zero
@@ -398,6 +398,37 @@ def test_immittance_solve_plans(plan, prec):
     x0 = np.linalg.solve(A, b)
     tol = 1e-7 if plan == 3 else 1e-9  # plan 3 pivots within the Z_L block only
     assert np.abs(x - x0).max() <= tol * np.abs(x0).max()
+
+
+@pytest.mark.parametrize("freq", [100.0, 1e3, 3e5, 1e7])
+def test_nodal_vs_lapack(freq):
+    """The oracle's W of a small grid in the [I_L; I_T; U] order, mixed form of
+    the nodal plan (complex64 factor: Z_L and Z_T factored within their own
+    rows, the node block's Schur complement from the incidence rows; FP64
+    refinement on W): no FP64 fallback and few refinement steps at high
+    frequency (a wrong factor would need many), and x equals LAPACK's
+    solution of W. At 100 Hz the complex64 nodal factor's refinement stalls on
+    this grid and the FP64 fallback answers (the node block's Schur complement
+    is dominated by A^T Z_L^-1 A, about 1/|Z_L|, whose complex64 rounding
+    outweighs the grounding terms that fix its smallest modes)."""
+    prec = T.PREC_MIXED
+    e, nodes = _small_grid()
+    m, n = len(e), len(nodes)
+    N = n + 2 * m
+    wo = _assembled(e, nodes, freq, 1e-3, 15.0).reshape(N, N).T
+    Wr = np.tril(wo) + np.tril(wo, -1).T  # reference order [U; I_L; I_T]
+    order = np.concatenate([n + np.arange(2 * m), np.arange(n)])
+    A = np.ascontiguousarray(Wr[np.ix_(order, order)])
+    b = np.zeros(N, complex)
+    b[2 * m + 3] = 1.0  # unit current at node 3
+    rc, x, st = T.immittance_solve(A, b, m, n, prec, T.PLAN_NODAL)
+    assert rc == 0 and st.info == 0
+    if freq >= 3e5:
+        assert st.precision_used == 1 and st.backward_err <= 1e-13, (
+            st.precision_used, st.refine_iters, st.backward_err)
+        assert st.refine_iters <= 4, st.refine_iters
+    x0 = np.linalg.solve(A, b)
+    assert np.abs(x - x0).max() <= 1e-9 * np.abs(x0).max(), (st.precision_used, st.refine_iters)
 
 
 @pytest.mark.parametrize("freq", [100.0, 3e5, 1e7])

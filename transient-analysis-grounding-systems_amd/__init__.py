@@ -35,6 +35,8 @@ INTG_NONE, INTG_DOUBLE, INTG_SINGLE, INTG_MHEM = 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 PLAN_DENSE, PLAN_STRUCTURED, PLAN_ZL_FIRST, PLAN_BLOCK_PIVOT, PLAN_GROUNDED = 0, 1, 2, 3, 4
 PLAN_GROUNDED_PIVOTED = 5
+PLAN_NODAL = 6
+PLAN_HYBRID = 7
 
 # src/auxiliary.c:17-21
 TWO_PI = 6.283185307179586476925286766559
@@ -108,7 +110,7 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_dense_solve", "tags_version", "tags_device_synchronize", "tags_gemm_bench",
             "tags_gemm_variants", "tags_gemm_select", "tags_autotune", "tags_sweep_set_plan",
             "tags_sweep_assemble", "tags_immittance_solve", "tags_sweep_set_inflight",
-            "tags_gemm_run", "tags_set_step_width"]
+            "tags_gemm_run", "tags_set_step_width", "tags_sweep_set_nodal_above"]
 
 _lib = None
 
@@ -153,6 +155,7 @@ def lib():
         "tags_set_lookahead": ([I], I),
         "tags_set_update_chunks": ([I], I),
         "tags_set_step_width": ([I], I),
+        "tags_sweep_set_nodal_above": ([P, D], I),
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
         "tags_sweep_set_inflight": ([P, I], I),
@@ -410,6 +413,12 @@ class Sweep:
         rc = lib().tags_sweep_set_plan(self._h, plan)
         if rc:
             raise RuntimeError("tags_sweep_set_plan failed: %d" % rc)
+
+    def set_nodal_above(self, hz):
+        """Switch frequency of PLAN_HYBRID's mixed form (nodal factor at and above)."""
+        rc = lib().tags_sweep_set_nodal_above(self._h, float(hz))
+        if rc:
+            raise RuntimeError("tags_sweep_set_nodal_above failed: %d" % rc)
 
     def set_inflight(self, samples):
         rc = lib().tags_sweep_set_inflight(self._h, samples)

@@ -1040,6 +1040,13 @@ int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_ele
     case TAGS_PLAN_STRUCTURED: p = LUPlan::nodes_first((int)m, (int)nn); break;
     case TAGS_PLAN_ZL_FIRST: p = LUPlan::zl_first((int)m, (int)nn, false); break;
     case TAGS_PLAN_BLOCK_PIVOT: p = LUPlan::zl_first((int)m, (int)nn, true); break;
+    case TAGS_PLAN_NODAL:
+        p = LUPlan::zl_first((int)m, (int)nn, false);
+        p32 = LUPlan::nodal((int)m, (int)nn);
+        break;
+    case TAGS_PLAN_HYBRID:  // no frequency here: as ZL_FIRST
+        p = LUPlan::zl_first((int)m, (int)nn, false);
+        break;
     case TAGS_PLAN_GROUNDED:
     case TAGS_PLAN_GROUNDED_PIVOTED:
         p = LUPlan::zl_first((int)m, (int)nn, false);
@@ -1050,7 +1057,7 @@ int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_ele
     }
     const bool g = plan == TAGS_PLAN_GROUNDED || plan == TAGS_PLAN_GROUNDED_PIVOTED;
     return dense_solve_host((const c64*)a, (c64*)b, nn + 2 * m, precision, false, stats, &p,
-                            g ? &p32 : nullptr, gr);
+                            g || plan == TAGS_PLAN_NODAL ? &p32 : nullptr, gr);
 }
 
 }  // extern "C"
@@ -1082,6 +1089,8 @@ struct SweepSlot {
     Profiler prof;
     Solver sv;
     long f = -1;  // sample in flight
+    double hz = 0;        // its frequency |s| / 2 pi
+    bool nodal = false;   // its complex64 factor is the nodal plan's
     int init(long N)
     {
         if (!stream.s) TAGS_HIP(hipStreamCreate(&stream.s));
@@ -1094,6 +1103,9 @@ struct SweepSlot {
     }
 };
 
+// refinement steps above which a nodal sample of a hybrid sweep counts as slow
+constexpr int kHybridMaxIters = 6;
+
 struct tags_sweep {
     long m = 0, n = 0, N = 0;
     int plan_kind = TAGS_PLAN_STRUCTURED;
@@ -1101,6 +1113,12 @@ struct tags_sweep {
     int inflight = 2;      // frequency samples in flight
     tags::LUPlan plan;
     tags::LUPlan plan32;
+    // TAGS_PLAN_HYBRID: the mixed form's complex64 factor is the nodal plan's
+    // at and above nodal_above Hz, ZL_FIRST's below; a nodal sample that falls
+    // back to FP64 or needs more than kHybridMaxIters refinement steps raises
+    // the switch to 1.5 times its frequency for the rest of the sweep
+    tags::LUPlan plan_nodal;
+    double nodal_above = 2e4;
     tags::BlockLayout layout{};
     hipStream_t st = nullptr;  // set-up and assembly stream
     tags::DevSegs e, img;
@@ -1150,6 +1168,17 @@ static int sweep_layout(tags_sweep* sw)
                          : LUPlan::zl_first((int)m, (int)n, false);
         p32 = &sw->plan32;
         gr = Grounding{(int)(2 * m), (int)(2 * m), (int)(2 * m + n)};
+        break;
+    case TAGS_PLAN_NODAL:
+        sw->layout = BlockLayout{n, m, 2 * m, 0, m};
+        sw->plan = LUPlan::zl_first((int)m, (int)n, false);
+        sw->plan32 = LUPlan::nodal((int)m, (int)n);
+        p32 = &sw->plan32;
+        break;
+    case TAGS_PLAN_HYBRID:  // complex64 plan chosen per sample (sweep_issue)
+        sw->layout = BlockLayout{n, m, 2 * m, 0, m};
+        sw->plan = LUPlan::zl_first((int)m, (int)n, false);
+        sw->plan_nodal = LUPlan::nodal((int)m, (int)n);
         break;
     default:
         sw->layout = BlockLayout{n, m, 2 * m, 0, m};
@@ -1293,8 +1322,15 @@ int tags_sweep_assemble(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, doub
 
 int tags_sweep_set_plan(tags_sweep* sw, int plan)
 {
-    if (!sw || plan < 0 || plan > TAGS_PLAN_GROUNDED_PIVOTED) return TAGS_ERR_ARG;
+    if (!sw || plan < 0 || plan > TAGS_PLAN_HYBRID) return TAGS_ERR_ARG;
     sw->plan_kind = plan;
+    return 0;
+}
+
+int tags_sweep_set_nodal_above(tags_sweep* sw, double hz)
+{
+    if (!sw || !(hz >= 0.0)) return TAGS_ERR_ARG;
+    sw->nodal_above = hz;
     return 0;
 }
 
@@ -1362,6 +1398,12 @@ static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const TAGS_COMPL
     const BlockLayout& L = sw->layout;
     int rc;
     sl.f = (long)f;
+    {
+        const c64 sf = from_c(s[f]);
+        sl.hz = std::hypot(sf.x, sf.y) / (2.0 * M_PI);
+    }
+    sl.nodal = sw->plan_kind == TAGS_PLAN_HYBRID && mixed && sl.hz >= sw->nodal_above;
+    if (sw->plan_kind == TAGS_PLAN_HYBRID) sl.sv.plan32 = sl.nodal ? &sw->plan_nodal : nullptr;
     if ((rc = sweep_fill(sw, sl.W.p, sl.stream.s, &sl.prof, s[f], kappa[f], mur,
                          gamma ? &gamma[f] : nullptr, ref_l ? &ref_l[f] : nullptr,
                          ref_t ? &ref_t[f] : nullptr)))
@@ -1385,6 +1427,8 @@ static int sweep_finish(tags_sweep* sw, SweepSlot& sl, size_t inj_node, bool mix
     int rc = mixed ? sl.sv.finish_mixed(sl.W.p, sl.rhs.p, &stt) : sl.sv.finish64(&stt);
     if (rc < 0) return rc;
     if (stats) stats[f] = stt;
+    if (sl.nodal && (stt.precision_used != TAGS_PREC_MIXED || stt.refine_iters > kHybridMaxIters))
+        sw->nodal_above = std::max(sw->nodal_above, 1.5 * sl.hz);
     if (rc > 0) {
         if (u_inj) u_inj[f] = to_c(c64{NAN, NAN});
         return 0;

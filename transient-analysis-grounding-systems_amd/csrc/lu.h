@@ -242,6 +242,24 @@ struct LUPlan {
         for (int i = 0; i < m + n; i++) p.orders[i] = m + i;
         return p;
     }
+    // Nodal block elimination of the [I_L; I_T; U] storage order (m > 0: on):
+    // Z_L and Z_T each factored with row exchanges among their own rows, the
+    // node columns carried as extra columns, so that their trailing updates
+    // form L^-1 P A; before the node phase the node columns of those rows are
+    // completed to X = Z^-1 A (backward blocked solves), and the node block is
+    // set to its Schur complement S = -(A^T X_L + B^T X_T) from the non-zeros of
+    // the node rows (the incidence), not by dense products; then S is factored.
+    static LUPlan nodal(int m, int n)
+    {
+        LUPlan p;
+        const int N = n + 2 * m;
+        p.phases.push_back(LUPhase{0, m, m, m, 2 * m, N});
+        p.phases.push_back(LUPhase{m, 2 * m, 2 * m, 2 * m, 2 * m, N});
+        p.phases.push_back(LUPhase{2 * m, N, N, N, 0, 0});
+        p.nodal_m = m;
+        return p;
+    }
+    int nodal_m = 0;
     std::vector<int> orders;  // row orders referenced by phases (order_off)
     bool pivots() const
     {
@@ -275,6 +293,7 @@ struct LUWork {
     std::vector<int> blk;   // start column of every diagonal block, in order
     std::vector<int> h_init, h_orders;  // what d_init / d_orders hold
     double flops = 0;       // real flops of the last factorisation (GEMM + panel)
+    int nodal_m = 0;        // the last factorisation's nodal block size (0: none)
 
     int alloc(int n_, int max_panel_wgs, size_t panel_lds_limit);
     void release();

@@ -353,11 +353,14 @@ __global__ __launch_bounds__(256) void k_diag_block(cx<T>* A, long ld, const int
 // physical pm[..]; jb <= 256 rows in all)
 //     U_t = inv(L_tt) (A_t - sum_{u<t} L_tu U_u),
 // with block t's unit-lower inverse (identity padding past its width) at
-// invl + 4096 t. Work-group b owns the CW columns cb + CW b ..: its U_u stay in
-// LDS for the later rows, L_tu and inv(L_tt) pass through LDS in 64 x KC
-// chunks, and thread (tr, tc) holds rows 4 tr .. 4 tr + 3 at columns
-// tc + 16 q (q < CW / 16) of each 64 x CW product.
-template <typename T, int CW, int KC>
+// invl + 4096 t. With UP the backward solve with the step's upper triangle
+// instead, X_t = inv(U_tt) (A_t - sum_{u>t} U_tu X_u), last block first, with
+// the upper inverses at invl. Work-group b owns the CW columns cb + CW b ..:
+// its solved blocks stay in LDS for the later rows, the off-diagonal blocks
+// and the inverses pass through LDS in 64 x KC chunks, and thread (tr, tc)
+// holds rows 4 tr .. 4 tr + 3 at columns tc + 16 q (q < CW / 16) of each
+// 64 x CW product.
+template <typename T, int CW, int KC, bool UP>
 __global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* pm, int j, int jb,
                                                   const cx<T>* invl, int cb, int cols)
 {
@@ -370,7 +373,8 @@ __global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* 
     const int nsub = (jb + 63) / 64;
     s_pm[tid] = tid < jb ? pm[j + tid] : 0;
     __syncthreads();
-    for (int t = 0; t < nsub; t++) {
+    for (int tt = 0; tt < nsub; tt++) {
+        const int t = UP ? nsub - 1 - tt : tt;
         const int r0 = 64 * t, wt = min(64, jb - r0);
         cx<T> acc[4][CT];
 #pragma unroll
@@ -382,13 +386,14 @@ __global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* 
                 acc[i][q] = r < wt && c < cend ? A[(long)s_pm[r0 + r] * ld + c] : cx<T>{0, 0};
             }
         }
-        // acc -= L_tu U_u for the earlier sub-panels
-        for (int u = 0; u < t; u++) {
+        // acc -= L_tu U_u over the sub-panels solved before this one
+        for (int uu = 0; uu < (UP ? nsub - 1 - t : t); uu++) {
+            const int u = UP ? t + 1 + uu : uu;
             for (int kc = 0; kc < 64; kc += KC) {
                 for (int e = tid; e < 64 * KC; e += 256) {
-                    const int r = e / KC, kk = e % KC;
-                    Ls[r][kk] = r < wt ? A[(long)s_pm[r0 + r] * ld + j + 64 * u + kc + kk]
-                                       : cx<T>{0, 0};
+                    const int r = e / KC, kk = e % KC, k = 64 * u + kc + kk;
+                    Ls[r][kk] = r < wt && k < jb ? A[(long)s_pm[r0 + r] * ld + j + k]
+                                                 : cx<T>{0, 0};
                 }
                 __syncthreads();
 #pragma unroll
@@ -450,6 +455,101 @@ __global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* 
             }
         }
         __syncthreads();
+    }
+}
+
+// Node block of the nodal plan (stored order [I_L; I_T; U], m segments, n
+// nodes): for node row i (logical 2m + i, physical pm[2m + i]),
+//     S[i][k] = W[i][2m + k] - sum_c W[i][c] X[c][k],   c in [0, 2m), k in [0, n),
+// where W[i][c] are the row's incidence entries (A^T and B^T, untouched by the
+// Z_L and Z_T phases) and X[c] = Z^-1 A row of unknown c, at physical row pm[c]
+// in the node columns. One work-group per node row: the row's non-zeros are
+// gathered into LDS in column order (ballot prefix sums, so the sums do not
+// depend on timing), in batches of at most NODAL_NZ_MAX, and after each batch
+// every thread subtracts its output columns' share.
+constexpr int NODAL_NZ_MAX = 512;
+template <typename T>
+__global__ __launch_bounds__(256) void k_nodal_schur(cx<T>* A, long ld, const int* pm, int m,
+                                                     int n)
+{
+    __shared__ long s_row[NODAL_NZ_MAX];
+    __shared__ cx<T> s_cf[NODAL_NZ_MAX];
+    __shared__ int s_wcnt[4];
+    __shared__ int s_total;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    cx<T>* row = A + (long)pm[2 * m + blockIdx.x] * ld;
+    int c_next = 0;
+    for (;;) {
+        if (tid == 0) s_total = 0;
+        __syncthreads();
+        // gather while a whole 256-column pass still fits
+        while (c_next < 2 * m && s_total <= NODAL_NZ_MAX - 256) {
+            const int c = c_next + tid;
+            const cx<T> v = c < 2 * m ? row[c] : cx<T>{0, 0};
+            const bool nz = v.x != T(0) || v.y != T(0);
+            const unsigned long long bal = __ballot(nz);
+            if (lane == 0) s_wcnt[wv] = __popcll(bal);
+            __syncthreads();
+            int pos = s_total + __popcll(bal & ((1ull << lane) - 1ull));
+            for (int q = 0; q < wv; q++) pos += s_wcnt[q];
+            if (nz) {
+                s_row[pos] = (long)pm[c] * ld + 2 * m;
+                s_cf[pos] = v;
+            }
+            __syncthreads();
+            if (tid == 0) s_total += s_wcnt[0] + s_wcnt[1] + s_wcnt[2] + s_wcnt[3];
+            c_next += 256;
+            __syncthreads();
+        }
+        const int cnt = s_total;
+        for (int k = tid; k < n && cnt > 0; k += 256) {
+            T sx = 0, sy = 0;
+            for (int e = 0; e < cnt; e++) {
+                const cx<T> a = s_cf[e], x = A[s_row[e] + k];
+                sx = fma(a.x, x.x, sx);
+                sx = fma(-a.y, x.y, sx);
+                sy = fma(a.x, x.y, sy);
+                sy = fma(a.y, x.x, sy);
+            }
+            const cx<T> o = row[2 * m + k];
+            row[2 * m + k] = cx<T>{o.x - sx, o.y - sy};
+        }
+        __syncthreads();
+        if (c_next >= 2 * m) break;
+    }
+}
+
+// y[i] -= sum_{k<w} A[perm[i]][c0+k] y[c0+k] for logical rows i in [lo, hi), any
+// w (the rows and the columns [c0, c0 + w) are disjoint); one work-group per row
+template <typename T>
+__global__ __launch_bounds__(256) void k_rows_gemv(const cx<T>* A, long ld, const int* perm,
+                                                    cx<T>* y, int lo, int hi, int c0, int w)
+{
+    __shared__ T red[2][4];
+    const int i = lo + blockIdx.x;
+    if (i >= hi) return;
+    const cx<T>* row = A + (long)perm[i] * ld + c0;
+    T sx = 0, sy = 0;
+    for (int k = threadIdx.x; k < w; k += 256) {
+        const cx<T> a = row[k], v = y[c0 + k];
+        sx = fma(a.x, v.x, sx);
+        sx = fma(-a.y, v.y, sx);
+        sy = fma(a.x, v.y, sy);
+        sy = fma(a.y, v.x, sy);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        sx += __shfl_xor(sx, off);
+        sy += __shfl_xor(sy, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        red[0][threadIdx.x >> 6] = sx;
+        red[1][threadIdx.x >> 6] = sy;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const T tx = (red[0][0] + red[0][1]) + (red[0][2] + red[0][3]);
+        const T ty = (red[1][0] + red[1][1]) + (red[1][2] + red[1][3]);
+        y[i] = cx<T>{y[i].x - tx, y[i].y - ty};
     }
 }
 
@@ -1038,7 +1138,7 @@ static int step_u12(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long l
         flops += 8.0 * wt * (double)cols * (r0 + wt);
     }
     Timed tm(pr, FAM_GEMM, st);
-    hipLaunchKernelGGL((k_step_u12<T, CW, KC>), dim3((cols + CW - 1) / CW), dim3(256), 0, st, A,
+    hipLaunchKernelGGL((k_step_u12<T, CW, KC, false>), dim3((cols + CW - 1) / CW), dim3(256), 0, st, A,
                        ld, pm, s.j, s.jb, (const cx<T>*)(w.invL + (size_t)s.slot * 4096), cb,
                        cols);
     TAGS_HIP(hipGetLastError());
@@ -1061,6 +1161,53 @@ static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, lon
     if ((rc = step_u12<T>(w, pr, st, A, ld, pm, s, cb, ce))) return rc;
     return gemm<T>(pr, st, ph.row_end - c1, ce - cb, s.jb, A + j, ld, pm + c1, A + cb, ld, pm + j,
                    A + cb, ld, pm + c1, 1, true, &w.flops);
+}
+
+// Nodal plan, before its node phase: the node columns of the Z_L and Z_T rows
+// hold Y = L^-1 P A (their phases' trailing updates); complete them to
+// X = U^-1 Y by the blocked backward solve over each phase's steps, last step
+// first (the step's own rows in k_step_u12, then the rows above it in the
+// phase by one product with K = the step width), then write the node block's
+// Schur complement (k_nodal_schur). Flops: 4 m^2 n per current block.
+template <typename T>
+static int nodal_prepare(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
+                         const int* pm, const LUPlan& plan, const std::vector<LUStep>& steps)
+{
+    constexpr int CW = sizeof(T) == 4 ? 32 : 16, KC = sizeof(T) == 4 ? 16 : 8;
+    const int m = plan.nodal_m, nn = w.n - 2 * m, x0 = 2 * m;
+    int rc;
+    for (int p = 0; p < 2; p++) {
+        const LUPhase& ph = plan.phases[p];
+        for (int k = (int)steps.size() - 1; k >= 0; k--) {
+            const LUStep& s = steps[k];
+            if (s.phase != p) continue;
+            if (s.jb > 256) return TAGS_ERR_ARG;
+            double flops = 0;
+            for (int r0 = 0; r0 < s.jb; r0 += 64) {
+                const int wt = std::min(64, s.jb - r0);
+                flops += 8.0 * wt * (double)nn * (s.jb - r0);
+            }
+            {
+                Timed tm(pr, FAM_GEMM, st);
+                hipLaunchKernelGGL((k_step_u12<T, CW, KC, true>), dim3((nn + CW - 1) / CW),
+                                   dim3(256), 0, st, A, ld, pm, s.j, s.jb,
+                                   (const cx<T>*)(w.invU + (size_t)s.slot * 4096), x0, nn);
+                TAGS_HIP(hipGetLastError());
+                tm.end(flops);
+                w.flops += flops;
+            }
+            if (s.j > ph.c_begin &&
+                (rc = gemm<T>(pr, st, s.j - ph.c_begin, nn, s.jb, A + s.j, ld, pm + ph.c_begin,
+                              A + x0, ld, pm + s.j, A + x0, ld, pm + ph.c_begin, 1, false,
+                              &w.flops)))
+                return rc;
+        }
+    }
+    Timed tm(pr, FAM_OTHER, st);
+    hipLaunchKernelGGL(k_nodal_schur<T>, dim3(nn), dim3(256), 0, st, A, ld, pm, m, nn);
+    TAGS_HIP(hipGetLastError());
+    tm.end();
+    return 0;
 }
 
 template <typename T>
@@ -1099,6 +1246,11 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
         if (ph.order_off >= 0 &&
             (ph.pivot || ph.order_off + (n - ph.c_begin) > (int)plan->orders.size()))
             return TAGS_ERR_ARG;
+    if (plan->nodal_m > 0 &&
+        (plan->phases.size() != 3 || 2 * plan->nodal_m >= n || plan->phases[2].c_begin !=
+                                                                    2 * plan->nodal_m))
+        return TAGS_ERR_ARG;
+    w.nodal_m = plan->nodal_m;
     hipLaunchKernelGGL(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, st, w.perm, init, n,
                        w.flags);
     TAGS_HIP(hipGetLastError());
@@ -1157,7 +1309,11 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             re[1] = rb[1];
         }
         const LUPhase* nph = k + 1 < steps.size() ? &plan->phases[steps[k + 1].phase] : nullptr;
-        const bool overlap = nph && lookahead && (tournament || !nph->pivot);
+        // the nodal plan's node phase starts after the whole of the current
+        // blocks' work (its Schur complement reads their finished rows)
+        const bool nodal_next =
+            plan->nodal_m > 0 && nph && steps[k + 1].phase == 2 && s.phase == 1;
+        const bool overlap = nph && lookahead && (tournament || !nph->pivot) && !nodal_next;
         if (nph && !overlap) {
             // in order: the whole update of this step, then the next panel (the
             // next panel may permute rows beyond this step's row range, which
@@ -1166,6 +1322,8 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                 if ((rc = step_update<T>(w, pr, st, A, ld, pcur, ph, s, rb[q], re[q])))
                     return rc;
             const LUStep& nx = steps[k + 1];
+            if (nodal_next && (rc = nodal_prepare<T>(w, pr, st, A, ld, pcur, *plan, steps)))
+                return rc;
             if ((rc = set_order(nx, pcur, st))) return rc;
             if ((rc = step_panels<T>(w, pr, st, A, ld, pcur, plan->phases[nx.phase], nx)))
                 return rc;
@@ -1223,12 +1381,65 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
 
 // Solves (L U) x = P b in place on `y` (logical-order input already gathered:
 // y[i] = b[perm[i]]); output x in natural order in `y`.
+// Triangular solves over the diagonal blocks of the columns [c_lo, c_hi) only,
+// the off-diagonal products confined to the rows [c_lo, c_hi).
+template <typename T>
+static void range_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
+                        int c_lo, int c_hi)
+{
+    const int n = w.n;
+    for (int b = 0; b < w.nblk; b++) {
+        const int r0 = w.blk[b];
+        if (r0 < c_lo || r0 >= c_hi) continue;
+        const int bw = std::min(b + 1 < w.nblk ? w.blk[b + 1] : n, c_hi) - r0;
+        hipLaunchKernelGGL(k_block_apply<T>, dim3(1), dim3(64), 0, st,
+                           w.invL + (size_t)b * 4096, y, r0, bw);
+        const int lo = r0 + bw;
+        if (lo < c_hi)
+            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((c_hi - lo + 3) / 4), dim3(256), 0, st, A,
+                               ld, w.perm, y, lo, c_hi, r0, bw);
+    }
+    for (int b = w.nblk - 1; b >= 0; b--) {
+        const int r0 = w.blk[b];
+        if (r0 < c_lo || r0 >= c_hi) continue;
+        const int bw = std::min(b + 1 < w.nblk ? w.blk[b + 1] : n, c_hi) - r0;
+        hipLaunchKernelGGL(k_block_apply<T>, dim3(1), dim3(64), 0, st,
+                           w.invU + (size_t)b * 4096, y, r0, bw);
+        if (r0 > c_lo)
+            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((r0 - c_lo + 3) / 4), dim3(256), 0, st, A,
+                               ld, w.perm, y, c_lo, r0, r0, bw);
+    }
+}
+
+// Solve with the nodal plan's factors (logical order in, unknowns out):
+// z = Z_L^-1 and Z_T^-1 of the current parts, the node right-hand side less
+// A^T z_L + B^T z_T (the node rows' incidence entries), x_U = S^-1 of that,
+// then x_L,T = z - X x_U (X = Z^-1 A in the node columns of the current rows).
+template <typename T>
+static void nodal_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st)
+{
+    const int n = w.n, m = w.nodal_m;
+    range_solve<T>(w, A, ld, y, st, 0, m);
+    range_solve<T>(w, A, ld, y, st, m, 2 * m);
+    hipLaunchKernelGGL(k_rows_gemv<T>, dim3(n - 2 * m), dim3(256), 0, st, A, ld, w.perm, y,
+                       2 * m, n, 0, 2 * m);
+    range_solve<T>(w, A, ld, y, st, 2 * m, n);
+    hipLaunchKernelGGL(k_rows_gemv<T>, dim3(2 * m), dim3(256), 0, st, A, ld, w.perm, y, 0,
+                       2 * m, 2 * m, n - 2 * m);
+}
+
 template <typename T>
 int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
                      Profiler* pr)
 {
     const int n = w.n;
     Timed tm(pr, FAM_SOLVE, st);
+    if (w.nodal_m > 0) {
+        nodal_solve<T>(w, A, ld, y, st);
+        TAGS_HIP(hipGetLastError());
+        tm.end();
+        return 0;
+    }
     for (int b = 0; b < w.nblk; b++) {
         const int r0 = w.blk[b];
         const int bw = (b + 1 < w.nblk ? w.blk[b + 1] : n) - r0;
