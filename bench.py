@@ -123,10 +123,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--plan-mixed", type=int, default=2,
-                    help="LU plan of the mixed form: 0 dense, 1 structured, 2 Z_L first "
-                         "(default), 3 Z_L-block pivoting, 4 grounded without row "
-                         "exchanges, 5 grounded with pivoting")
+    ap.add_argument("--plan-mixed", type=int, default=7,
+                    help="LU plan of the mixed form: 0 dense, 1 structured, 2 Z_L first, "
+                         "3 Z_L-block pivoting, 4 grounded without row exchanges, 5 "
+                         "grounded with pivoting, 6 nodal, 7 hybrid (default: nodal "
+                         "complex64 factor from 20 kHz, Z_L first below)")
     ap.add_argument("--plan-fp64", type=int, default=2,
                     help="LU plan of the FP64 form (default 2, Z_L first)")
     ap.add_argument("--update-chunks", type=int, default=None,
