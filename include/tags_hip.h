@@ -206,8 +206,10 @@ int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
  *                       more refinement steps toward low frequencies, where a
  *                       stalled refinement falls back to the FP64 factor;
  * TAGS_PLAN_HYBRID      stored and factored as ZL_FIRST in FP64. In a sweep's
- *                       mixed form, each sample takes NODAL's complex64 factor
- *                       at and above the switch frequency
+ *                       mixed form, each sample takes NODAL's complex64 factor,
+ *                       without row exchanges (Z_L, Z_T and S are complex
+ *                       symmetric with a definite real or imaginary part), at
+ *                       and above the switch frequency
  *                       (tags_sweep_set_nodal_above) and ZL_FIRST's below it;
  *                       tags_immittance_solve, which has no frequency, treats
  *                       it as ZL_FIRST. */

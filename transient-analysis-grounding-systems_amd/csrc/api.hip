@@ -1178,7 +1178,7 @@ static int sweep_layout(tags_sweep* sw)
     case TAGS_PLAN_HYBRID:  // complex64 plan chosen per sample (sweep_issue)
         sw->layout = BlockLayout{n, m, 2 * m, 0, m};
         sw->plan = LUPlan::zl_first((int)m, (int)n, false);
-        sw->plan_nodal = LUPlan::nodal((int)m, (int)n);
+        sw->plan_nodal = LUPlan::nodal((int)m, (int)n, false);
         break;
     default:
         sw->layout = BlockLayout{n, m, 2 * m, 0, m};

@@ -243,19 +243,21 @@ struct LUPlan {
         return p;
     }
     // Nodal block elimination of the [I_L; I_T; U] storage order (m > 0: on):
-    // Z_L and Z_T each factored with row exchanges among their own rows, the
-    // node columns carried as extra columns, so that their trailing updates
-    // form L^-1 P A; before the node phase the node columns of those rows are
+    // Z_L and Z_T each factored within their own rows (with row exchanges among
+    // them, or with `pivot` false on their diagonals: both are complex
+    // symmetric with a definite real or imaginary part, as is S), the node
+    // columns carried as extra columns, so that their trailing updates form
+    // L^-1 P A; before the node phase the node columns of those rows are
     // completed to X = Z^-1 A (backward blocked solves), and the node block is
     // set to its Schur complement S = -(A^T X_L + B^T X_T) from the non-zeros of
     // the node rows (the incidence), not by dense products; then S is factored.
-    static LUPlan nodal(int m, int n)
+    static LUPlan nodal(int m, int n, bool pivot = true)
     {
         LUPlan p;
         const int N = n + 2 * m;
-        p.phases.push_back(LUPhase{0, m, m, m, 2 * m, N});
-        p.phases.push_back(LUPhase{m, 2 * m, 2 * m, 2 * m, 2 * m, N});
-        p.phases.push_back(LUPhase{2 * m, N, N, N, 0, 0});
+        p.phases.push_back(LUPhase{0, m, m, m, 2 * m, N, pivot});
+        p.phases.push_back(LUPhase{m, 2 * m, 2 * m, 2 * m, 2 * m, N, pivot});
+        p.phases.push_back(LUPhase{2 * m, N, N, N, 0, 0, pivot});
         p.nodal_m = m;
         return p;
     }
