@@ -128,6 +128,8 @@ def main():
                          "3 Z_L-block pivoting, 4 grounded without row exchanges, 5 "
                          "grounded with pivoting, 6 nodal, 7 hybrid (default: nodal "
                          "complex64 factor from 20 kHz, Z_L first below)")
+    ap.add_argument("--nodal-above", type=float, default=2e4,
+                    help="switch frequency of the hybrid plan (Hz, default 2e4)")
     ap.add_argument("--plan-fp64", type=int, default=2,
                     help="LU plan of the FP64 form (default 2, Z_L first)")
     ap.add_argument("--update-chunks", type=int, default=None,
@@ -180,6 +182,7 @@ def main():
         the stream it runs on."""
         mixed = prec == T.PREC_MIXED
         sw.set_plan(args.plan_mixed if mixed else args.plan_fp64)
+        sw.set_nodal_above(args.nodal_above)
         T.set_step_width(args.step_width_mixed if mixed else args.step_width_fp64)
         sw.set_inflight(inflight)
         if W:
@@ -222,6 +225,7 @@ def main():
                    "samples_timed": [float(f) for f in freqs[idx]],
                    "plan": PLAN_NAMES[args.plan_mixed],
                    "step_width": args.step_width_mixed,
+                   "nodal_above_hz": args.nodal_above,
                    "gemm_variants": tuned},
         "mixed": {"refine_iters": [q["refine_iters"] for q in st_mx],
                   "precision_used": [q["precision_used"] for q in st_mx],

@@ -227,7 +227,7 @@ int tags_sweep_set_plan(tags_sweep *sw, int plan);
 
 /* Switch frequency of TAGS_PLAN_HYBRID (default 2e4 Hz): in the mixed form a
  * sample at |s| / 2 pi >= hz takes the nodal plan's complex64 factor, a lower
- * one ZL_FIRST's. A nodal sample that falls back to FP64 or needs more than 6
+ * one ZL_FIRST's. A nodal sample that falls back to FP64 or needs more than 10
  * refinement steps raises the switch to 1.5 times its frequency for the rest
  * of the sweep (until this is called again). No counterpart in the reference,
  * which factors every sample alike. Returns 0, or TAGS_ERR_ARG for hz < 0. */

@@ -117,6 +117,10 @@ struct Solver {
     // backward error at which refinement stops (about 2 FP64 ulps: the mixed
     // solution is then at least as accurate as a pivoted FP64 factor's)
     static constexpr double kConverged = 2e-16;
+    // an iterate is accepted at a backward error <= kAccept (a few FP64 ulps);
+    // refinement goes on while each step gains at least a factor kMinGain
+    static constexpr double kAccept = 1e-15;
+    static constexpr double kMinGain = 0.8;
     int n = 0;
     hipStream_t st = nullptr;
     Profiler* pr = nullptr;
@@ -196,6 +200,9 @@ struct Solver {
 
     const LUPlan* plan = nullptr;  // nullptr: dense partial pivoting
     const LUPlan* plan32 = nullptr;  // plan of the mixed form's factor (nullptr: `plan`)
+    // a second complex64 plan tried, on the intact W, when the refinement with
+    // plan32 fails, before the FP64 factor (nullptr: none)
+    const LUPlan* plan32_retry = nullptr;
     bool equilibrate = false;      // symmetric row-max scaling before the FP32 factor
                                    // (not applied with grounding)
 
@@ -304,10 +311,12 @@ struct Solver {
 
     // Second half: waits and judges the queued steps by the backward error
     // |b - W x| / (|W| |x|) of each iterate; while it is above kConverged and
-    // still falling by at least half per step, refines further one step at a
-    // time (up to kMaxIters). The iterate in x (the last step run) is accepted at
-    // <= 1e-13; otherwise, or after a complex64 breakdown, the FP64 factor of
-    // the intact W gives the answer. Solution returned in `bvec`.
+    // still falling by a factor kMinGain or better per step, refines further
+    // one step at a time (up to kMaxIters). The iterate in x (the last step
+    // run) is accepted at <= kAccept. Otherwise, or after a complex64
+    // breakdown, the solve starts again with plan32_retry's complex64 factor
+    // if one is set, else the FP64 factor of the intact W gives the answer.
+    // Solution returned in `bvec`.
     int finish_mixed(c64* W, c64* bvec, tags_item_stats* stt)
     {
         TAGS_HIP(hipStreamSynchronize(st));
@@ -329,7 +338,7 @@ struct Solver {
         auto keep_going = [&]() {
             if (info32 != 0 || !(berr == berr) || berr <= kConverged || done >= kMaxIters)
                 return false;
-            return done < 2 || berr <= 0.5 * berr_at(done - 1);
+            return done < 2 || berr <= kMinGain * berr_at(done - 1);
         };
         while (keep_going()) {
             int rc = queue_step(W, done + 1, gr);
@@ -341,7 +350,7 @@ struct Solver {
             berr = berr_at(done);
             if (stt && done <= 8) stt->refine_hist[done - 1] = berr;
         }
-        if (info32 == 0 && berr == berr && berr <= 1e-13) {
+        if (info32 == 0 && berr == berr && berr <= kAccept) {
             if (stt) {
                 stt->precision_used = TAGS_PREC_MIXED;
                 stt->refine_iters = done;
@@ -350,6 +359,16 @@ struct Solver {
             }
             TAGS_HIP(hipMemcpyAsync(bvec, x, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
             return 0;
+        }
+        if (plan32_retry) {
+            // the second complex64 plan; its steps add to this one's
+            plan32 = plan32_retry;
+            plan32_retry = nullptr;
+            int rc = issue_mixed(W, bvec);
+            if (rc) return rc;
+            rc = finish_mixed(W, bvec, stt);
+            if (stt) stt->refine_iters += done;
+            return rc;
         }
         // FP64 fallback on the intact W
         int rc = solve64(W, bvec, stt);
@@ -1104,7 +1123,9 @@ struct SweepSlot {
 };
 
 // refinement steps above which a nodal sample of a hybrid sweep counts as slow
-constexpr int kHybridMaxIters = 6;
+// (a step costs a few per cent of a factorisation, a fallback to FP64 several
+// factorisations)
+constexpr int kHybridMaxIters = 10;
 
 struct tags_sweep {
     long m = 0, n = 0, N = 0;
@@ -1403,7 +1424,10 @@ static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const TAGS_COMPL
         sl.hz = std::hypot(sf.x, sf.y) / (2.0 * M_PI);
     }
     sl.nodal = sw->plan_kind == TAGS_PLAN_HYBRID && mixed && sl.hz >= sw->nodal_above;
-    if (sw->plan_kind == TAGS_PLAN_HYBRID) sl.sv.plan32 = sl.nodal ? &sw->plan_nodal : nullptr;
+    if (sw->plan_kind == TAGS_PLAN_HYBRID) {
+        sl.sv.plan32 = sl.nodal ? &sw->plan_nodal : nullptr;
+        sl.sv.plan32_retry = sl.nodal ? &sw->plan : nullptr;
+    }
     if ((rc = sweep_fill(sw, sl.W.p, sl.stream.s, &sl.prof, s[f], kappa[f], mur,
                          gamma ? &gamma[f] : nullptr, ref_l ? &ref_l[f] : nullptr,
                          ref_t ? &ref_t[f] : nullptr)))

@@ -65,14 +65,21 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
     const int tiles_m = (g.M + BM - 1) / BM;
     const int ntiles = tiles_n * tiles_m;
     // XCD-aware order: dispatch id b runs on XCD b % 8; give each XCD a
-    // contiguous range of tiles so neighbouring tiles share its L2.
+    // contiguous range of tiles, walked in windows of 8 tile rows (down the
+    // window's rows, then across its columns), so that the work-groups an XCD
+    // holds at once share a few row panels of A and column panels of B in its
+    // L2 instead of one row panel and many column panels.
     int bid;
     {
         const int per = (ntiles + 7) / 8;
         bid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
     }
     if (bid >= ntiles) return;
-    const int tm = bid / tiles_n, tn = bid % tiles_n;
+    constexpr int GM = 8;
+    const int first_m = bid / (GM * tiles_n) * GM;
+    const int gm = min(tiles_m - first_m, GM);
+    const int rem = bid - first_m * tiles_n;
+    const int tm = first_m + rem % gm, tn = rem / gm;
     const int m0 = tm * BM, n0 = tn * BN;
 
     // global-load slots of this thread (fixed rows of A, fixed columns of B)
