@@ -235,8 +235,11 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
 // the rows' U12 = inv(L11) A12 and the triangular solves use. Work-groups of
 // 256 threads; thread (r, p) holds columns 16p .. 16p+15 of row r in
 // registers. Rows and columns past w are padded with the identity, which the
-// sweeps leave unchanged. Launched with two work-groups (one per inverse)
-// without FACTOR, with one (which also writes the block back) with FACTOR.
+// sweeps leave unchanged. Launched with two work-groups of 256 threads (one
+// per inverse) without FACTOR. With FACTOR, one work-group of 512 threads: two
+// groups of 256 both run the LU on their own register copies (group 0 alone
+// writes LDS and the block back), then group 0 sweeps inv(L) while group 1
+// sweeps inv(U), passing the same barriers.
 //   LU:      at step k row k is final; its entries right of k and the
 //            reciprocal of its pivot are broadcast through LDS and every row
 //            r > k forms l = a[r][k] / u_kk and does a[r] -= l u[k].
@@ -248,19 +251,20 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
 // 1-based, as LAPACK); a caller then falls back to a pivoted factorisation.
 // ---------------------------------------------------------------------------
 template <typename T, bool FACTOR>
-__global__ __launch_bounds__(256) void k_diag_block(cx<T>* A, long ld, const int* pm, int c0,
+__global__ __launch_bounds__(512) void k_diag_block(cx<T>* A, long ld, const int* pm, int c0,
                                                     int w, cx<T>* invL, cx<T>* invU, int* info)
 {
     __shared__ cx<T> s_u[2][64];
-    __shared__ cx<T> s_v[2][64];
+    __shared__ cx<T> s_v[2][2][64];
     __shared__ cx<T> s_rp[2];
     __shared__ int s_pm[64];
     __shared__ int s_zero;
     __builtin_amdgcn_s_setprio(1);  // on the panel's critical path
-    const int t = threadIdx.x, r = t >> 2, p = t & 3, lane = t & 63;
+    const int tid = threadIdx.x, t = tid & 255, grp = tid >> 8;
+    const int r = t >> 2, p = t & 3, lane = t & 63;
     const int quad = lane & ~3;
-    if (t < 64) s_pm[t] = t < w ? pm[c0 + t] : 0;
-    if (t == 0) s_zero = -1;
+    if (tid < 64) s_pm[tid] = tid < w ? pm[c0 + tid] : 0;
+    if (tid == 0) s_zero = -1;
     __syncthreads();
     cx<T> a[16];
     {
@@ -275,7 +279,7 @@ __global__ __launch_bounds__(256) void k_diag_block(cx<T>* A, long ld, const int
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
             const int k = kb * 16 + kk, par = k & 1;
-            if (r == k) {
+            if (r == k && grp == 0) {
 #pragma unroll
                 for (int q = 0; q < 16; q++)
                     s_u[par][16 * p + q] = 16 * p + q > k ? a[q] : cx<T>{0, 0};
@@ -298,17 +302,17 @@ __global__ __launch_bounds__(256) void k_diag_block(cx<T>* A, long ld, const int
         }
     }
     // a: L \ U of row r. Without FACTOR the block is only read: work-group 0
-    // sweeps inv(L) and work-group 1 inv(U). With FACTOR the one work-group
-    // writes the factored block back, so it does both sweeps itself.
-    if (FACTOR && r < w) {
+    // sweeps inv(L) and work-group 1 inv(U). With FACTOR group 0 writes the
+    // factored block back, then sweeps inv(L) while group 1 sweeps inv(U).
+    if (FACTOR && r < w && grp == 0) {
         cx<T>* dst = A + (long)s_pm[r] * ld + c0 + 16 * p;
 #pragma unroll
         for (int q = 0; q < 16; q++)
             if (16 * p + q < w) dst[q] = a[q];
     }
-    if (FACTOR && t == 0 && s_zero >= 0 && *info == 0) *info = c0 + s_zero + 1;
-    const int pass0 = FACTOR ? 0 : (int)blockIdx.x, pass1 = FACTOR ? 2 : pass0 + 1;
-    for (int pass = pass0; pass < pass1; pass++) {
+    if (FACTOR && tid == 0 && s_zero >= 0 && *info == 0) *info = c0 + s_zero + 1;
+    {
+        const int pass = FACTOR ? grp : (int)blockIdx.x;
         const bool lower = pass == 0;
         cx<T> xv[16];
 #pragma unroll
@@ -333,12 +337,13 @@ __global__ __launch_bounds__(256) void k_diag_block(cx<T>* A, long ld, const int
                         for (int q = 0; q < 16; q++) xv[q] = cmul(xv[q], rd);
                     }
 #pragma unroll
-                    for (int q = 0; q < 16; q++) s_v[par][16 * p + q] = xv[q];
+                    for (int q = 0; q < 16; q++) s_v[grp][par][16 * p + q] = xv[q];
                 }
                 __syncthreads();
                 if (lower ? r > k : r < k) {
 #pragma unroll
-                    for (int q = 0; q < 16; q++) xv[q] = cfnma(xv[q], c, s_v[par][16 * p + q]);
+                    for (int q = 0; q < 16; q++)
+                        xv[q] = cfnma(xv[q], c, s_v[grp][par][16 * p + q]);
                 }
             }
         }
@@ -1088,7 +1093,7 @@ static int sub_panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long 
             hipLaunchKernelGGL((k_diag_block<T, false>), dim3(2), dim3(256), 0, st, A, ld, pm, c0,
                                cols, invl, invu, w.flags);
         else
-            hipLaunchKernelGGL((k_diag_block<T, true>), dim3(1), dim3(256), 0, st, A, ld, pm, c0,
+            hipLaunchKernelGGL((k_diag_block<T, true>), dim3(1), dim3(512), 0, st, A, ld, pm, c0,
                                cols, invl, invu, w.flags);
         TAGS_HIP(hipGetLastError());
         tm.end();
