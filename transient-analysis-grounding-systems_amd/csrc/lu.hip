@@ -463,6 +463,112 @@ __global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* 
     }
 }
 
+// complex64 form of k_step_u12 on the MFMA cores: work-group b owns the 32
+// columns cb + 32 b ..; wave w forms rows 16 w .. 16 w + 15 of each 64 x 32
+// block product as two 16 x 16 tiles of v_mfma_f32_16x16x4f32 (lane l supplies
+// A[l % 16][l / 16] and B[l / 16][l % 16]; result value v of lane l is
+// D[4 (l / 16) + v][l % 16]), four real products per complex one. The
+// accumulators start from A_t and take the off-diagonal blocks negated, so they
+// end as A_t - sum L_tu U_u; that is staged in the block's LDS slot and
+// multiplied by the staged inverse.
+typedef float u12v4 __attribute__((ext_vector_type(4)));
+template <bool UP>
+__global__ __launch_bounds__(256) void k_step_u12_mfma(c32* A, long ld, const int* pm, int j,
+                                                       int jb, const c32* invl, int cb, int cols)
+{
+    constexpr int CW = 32, KC = 16;
+    __shared__ c32 Us[4][64][CW];
+    __shared__ c32 Ls[64][KC + 1];
+    __shared__ int s_pm[256];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int lr = lane & 15, lk = lane >> 4;
+    const int c0 = cb + blockIdx.x * CW, cend = cb + cols;
+    const int nsub = (jb + 63) / 64;
+    s_pm[tid] = tid < jb ? pm[j + tid] : 0;
+    __syncthreads();
+    for (int tt = 0; tt < nsub; tt++) {
+        const int t = UP ? nsub - 1 - tt : tt;
+        const int r0 = 64 * t, wt = min(64, jb - r0);
+        u12v4 ar[2], ai[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 16 * wv + 4 * lk + v, c = c0 + 16 * q + lr;
+                const c32 x = row < wt && c < cend ? A[(long)s_pm[r0 + row] * ld + c] : c32{0, 0};
+                ar[q][v] = x.x;
+                ai[q][v] = x.y;
+            }
+        for (int uu = 0; uu < (UP ? nsub - 1 - t : t); uu++) {
+            const int u = UP ? t + 1 + uu : uu;
+            for (int kc = 0; kc < 64; kc += KC) {
+                for (int e = tid; e < 64 * KC; e += 256) {
+                    const int r = e / KC, kk = e % KC, k = 64 * u + kc + kk;
+                    Ls[r][kk] = r < wt && k < jb ? A[(long)s_pm[r0 + r] * ld + j + k] : c32{0, 0};
+                }
+                __syncthreads();
+#pragma unroll
+                for (int kk = 0; kk < KC; kk += 4) {
+                    const c32 l = Ls[16 * wv + lr][kk + lk];
+#pragma unroll
+                    for (int q = 0; q < 2; q++) {
+                        const c32 x = Us[u][kc + kk + lk][16 * q + lr];
+                        // acc -= l x
+                        ar[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-l.x, x.x, ar[q], 0, 0, 0);
+                        ar[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.y, x.y, ar[q], 0, 0, 0);
+                        ai[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-l.x, x.y, ai[q], 0, 0, 0);
+                        ai[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-l.y, x.x, ai[q], 0, 0, 0);
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        // the block's right-hand side into its LDS slot, then inv_t times it
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int v = 0; v < 4; v++)
+                Us[t][16 * wv + 4 * lk + v][16 * q + lr] = c32{ar[q][v], ai[q][v]};
+        u12v4 orr[2], oi[2];
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            orr[q] = u12v4{0, 0, 0, 0};
+            oi[q] = u12v4{0, 0, 0, 0};
+        }
+        const c32* li = invl + (size_t)t * 4096;
+        for (int kc = 0; kc < 64; kc += KC) {
+            for (int e = tid; e < 64 * KC; e += 256) {
+                const int r = e / KC, kk = e % KC;
+                Ls[r][kk] = li[r * 64 + kc + kk];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kk = 0; kk < KC; kk += 4) {
+                const c32 l = Ls[16 * wv + lr][kk + lk];
+#pragma unroll
+                for (int q = 0; q < 2; q++) {
+                    const c32 x = Us[t][kc + kk + lk][16 * q + lr];
+                    orr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.x, x.x, orr[q], 0, 0, 0);
+                    orr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-l.y, x.y, orr[q], 0, 0, 0);
+                    oi[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.x, x.y, oi[q], 0, 0, 0);
+                    oi[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.y, x.x, oi[q], 0, 0, 0);
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int row = 16 * wv + 4 * lk + v, c = c0 + 16 * q + lr;
+                const c32 o = c32{orr[q][v], oi[q][v]};
+                Us[t][row][16 * q + lr] = o;
+                if (row < wt && c < cend) A[(long)s_pm[r0 + row] * ld + c] = o;
+            }
+        __syncthreads();
+    }
+}
+
 // Node block of the nodal plan (stored order [I_L; I_T; U], m segments, n
 // nodes): for node row i (logical 2m + i, physical pm[2m + i]),
 //     S[i][k] = W[i][2m + k] - sum_c W[i][c] X[c][k],   c in [0, 2m), k in [0, n),
@@ -1143,9 +1249,14 @@ static int step_u12(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long l
         flops += 8.0 * wt * (double)cols * (r0 + wt);
     }
     Timed tm(pr, FAM_GEMM, st);
-    hipLaunchKernelGGL((k_step_u12<T, CW, KC, false>), dim3((cols + CW - 1) / CW), dim3(256), 0, st, A,
-                       ld, pm, s.j, s.jb, (const cx<T>*)(w.invL + (size_t)s.slot * 4096), cb,
-                       cols);
+    if constexpr (sizeof(T) == 4)
+        hipLaunchKernelGGL(k_step_u12_mfma<false>, dim3((cols + 31) / 32), dim3(256), 0, st, A,
+                           ld, pm, s.j, s.jb, (const c32*)(w.invL + (size_t)s.slot * 4096), cb,
+                           cols);
+    else
+        hipLaunchKernelGGL((k_step_u12<T, CW, KC, false>), dim3((cols + CW - 1) / CW), dim3(256),
+                           0, st, A, ld, pm, s.j, s.jb,
+                           (const cx<T>*)(w.invL + (size_t)s.slot * 4096), cb, cols);
     TAGS_HIP(hipGetLastError());
     tm.end(flops);
     w.flops += flops;
@@ -1194,9 +1305,14 @@ static int nodal_prepare(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, l
             }
             {
                 Timed tm(pr, FAM_GEMM, st);
-                hipLaunchKernelGGL((k_step_u12<T, CW, KC, true>), dim3((nn + CW - 1) / CW),
-                                   dim3(256), 0, st, A, ld, pm, s.j, s.jb,
-                                   (const cx<T>*)(w.invU + (size_t)s.slot * 4096), x0, nn);
+                if constexpr (sizeof(T) == 4)
+                    hipLaunchKernelGGL(k_step_u12_mfma<true>, dim3((nn + 31) / 32), dim3(256), 0,
+                                       st, A, ld, pm, s.j, s.jb,
+                                       (const c32*)(w.invU + (size_t)s.slot * 4096), x0, nn);
+                else
+                    hipLaunchKernelGGL((k_step_u12<T, CW, KC, true>), dim3((nn + CW - 1) / CW),
+                                       dim3(256), 0, st, A, ld, pm, s.j, s.jb,
+                                       (const cx<T>*)(w.invU + (size_t)s.slot * 4096), x0, nn);
                 TAGS_HIP(hipGetLastError());
                 tm.end(flops);
                 w.flops += flops;
