@@ -425,10 +425,11 @@ static int gemm_bench_t(int M, int N, int K, int variant, int iters, double* ms)
     const long na = (long)M * K, nb = (long)K * N, nc = (long)M * N;
     DevBuf<double> tmp;
     DevBuf<cx<T>> A, B, C;
-    DevBuf<int> rows;
+    DevBuf<int> rows, krows;
     int rc;
     if ((rc = tmp.alloc(2 * std::max(na, std::max(nb, nc)))) || (rc = A.alloc(na)) ||
-        (rc = B.alloc(nb)) || (rc = C.alloc(nc)) || (rc = rows.alloc(M)))
+        (rc = B.alloc(nb)) || (rc = C.alloc(nc)) || (rc = rows.alloc(M)) ||
+        (rc = krows.alloc(K)))
         return rc;
     cx<T>* bufs[3] = {A.p, B.p, C.p};
     const long sz[3] = {na, nb, nc};
@@ -449,7 +450,12 @@ static int gemm_bench_t(int M, int N, int K, int variant, int iters, double* ms)
     if (!perm_ok)
         for (int i = 0; i < M; i++) h[i] = i;
     TAGS_HIP(hipMemcpyAsync(rows.p, h.data(), sizeof(int) * M, hipMemcpyHostToDevice, S.s));
-    GemmArgs<T> g{M, N, K, A.p, K, rows.p, B.p, N, nullptr, C.p, N, rows.p, 1};
+    // B's rows through a reversal of 0 .. K-1 (the LU's U12 rows are addressed
+    // through the row map too)
+    std::vector<int> hk(K);
+    for (int i = 0; i < K; i++) hk[i] = K - 1 - i;
+    TAGS_HIP(hipMemcpyAsync(krows.p, hk.data(), sizeof(int) * K, hipMemcpyHostToDevice, S.s));
+    GemmArgs<T> g{M, N, K, A.p, K, rows.p, B.p, N, krows.p, C.p, N, rows.p, 1};
     if ((rc = launch_gemm_variant<T>(g, variant, S.s))) return rc;  // warm-up
     hipEvent_t e0, e1;
     TAGS_HIP(hipEventCreate(&e0));

@@ -41,6 +41,10 @@ struct Mfma<float> {
     static __device__ inline int row(int lane, int v) { return 4 * (lane >> 4) + v; }
 };
 
+// Largest K of a product with a row map on B, which is staged in LDS (the LU's
+// products have K <= 256)
+constexpr int GEMM_BROW_LDS = 512;
+
 template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN, int AL>
 __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
 {
@@ -58,6 +62,7 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
     static_assert(TM * 16 == WM && TN * 16 == WN, "wave tile");
     __shared__ cx<T> As[2][AL ? BM * SAK : BK * SA];
     __shared__ cx<T> Bs[2][BK * SB];
+    __shared__ int s_brow[GEMM_BROW_LDS];
 
     const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
     const int wm = wave / WGN, wn = wave % WGN;
@@ -117,7 +122,8 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
         {                                                                                 \
             const int gk = (k0_) + brw[q];                                                \
             const bool ok = gk < g.K && bcol[q] >= 0;                                     \
-            const long pr = ok ? (long)(g.brow ? g.brow[gk] : gk) * g.ldb + n0 + bcol[q] : 0; \
+            const int br = g.brow ? s_brow[ok ? gk : 0] : gk;                              \
+            const long pr = ok ? (long)br * g.ldb + n0 + bcol[q] : 0;                      \
             const cx<T> v = g.B[pr];                                                      \
             RB[q] = ok ? v : cx<T>{0, 0};                                                 \
         }                                                                                 \
@@ -185,6 +191,14 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
         }
     };
 
+    // B's row map for the whole K range (K <= GEMM_BROW_LDS, checked at launch)
+    // in LDS: read from HBM inside the K loop, each index would make the next
+    // B address wait for every load in flight (loads complete in issue order),
+    // the next chunk's A prefetch too
+    if (g.brow) {
+        for (int k = tid; k < g.K; k += NT) s_brow[k] = g.brow[k];
+        __syncthreads();
+    }
     TAGS_GLOAD(0, ra, rb);
     TAGS_SSTORE(0, ra, rb);
     __syncthreads();
@@ -525,6 +539,7 @@ template <typename T>
 int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing)
 {
     if (g.M <= 0 || g.N <= 0) return 0;
+    if (g.brow && g.K > GEMM_BROW_LDS) return TAGS_ERR_ARG;
     const int p = prec_index<T>();
     const int big = g_big_variant[p].load(std::memory_order_relaxed);
     const long tiles_big = (long)((g.M + GemmTable<T>::tile_m(big) - 1) / GemmTable<T>::tile_m(big)) *
@@ -538,6 +553,7 @@ int launch_gemm_variant(const GemmArgs<T>& g, int variant, hipStream_t st)
 {
     if (g.M <= 0 || g.N <= 0) return 0;
     if (variant < 0 || variant >= GemmTable<T>::count) return TAGS_ERR_ARG;
+    if (g.brow && g.K > GEMM_BROW_LDS) return TAGS_ERR_ARG;
     return GemmTable<T>::launch(variant, g, st, false);
 }
 
