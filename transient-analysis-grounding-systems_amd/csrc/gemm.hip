@@ -524,10 +524,10 @@ struct GemmTable<double> {
 // defaults are the fastest on MI355X at the LU's shapes (K = 128 for
 // complex128, 256 for complex64): 128 x 64 tiles of 8 waves (32 x 32 per
 // wave; three-multiply for complex64; A tile row-major in LDS) for large
-// products, 64 x 64 tiles of 4 waves for small ones; tags_autotune may replace
-// them.
+// products, 64 x 64 tiles of 4 waves (K chunks of 8) for small ones;
+// tags_autotune may replace them.
 static std::atomic<int> g_big_variant[2] = {13, 19};
-static std::atomic<int> g_small_variant[2] = {0, 5};
+static std::atomic<int> g_small_variant[2] = {0, 17};
 
 template <typename T>
 static constexpr int prec_index()
