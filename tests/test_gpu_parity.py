@@ -431,6 +431,26 @@ def test_nodal_vs_lapack(freq):
     assert np.abs(x - x0).max() <= 1e-9 * np.abs(x0).max(), (st.precision_used, st.refine_iters)
 
 
+def test_hybrid_low_switch_retries():
+    """Hybrid plan with its switch lowered to 1 kHz on the 20 x 20 m grid: the
+    nodal complex64 samples whose refinement stalls start again with the
+    Z_L-first complex64 factor (W is intact) and raise the switch; every sample
+    stays within 1e-8 of the FP64 answer and none needs the FP64 factor."""
+    rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
+    f = O.logspace(2, 7, 400)[::10]
+    s = 2j * np.pi * f
+    kap = 1e-3 + s * 15.0 * T.EPS0
+    sw = T.Sweep(e, nodes)
+    sw.set_plan(T.PLAN_ZL_FIRST)
+    u64, _ = sw.solve(s, kap, 0, T.PREC_FP64)
+    sw.set_plan(T.PLAN_HYBRID)
+    sw.set_nodal_above(1e3)
+    umx, st = sw.solve(s, kap, 0, T.PREC_MIXED)
+    sw.close()
+    assert np.max(np.abs(umx - u64) / np.abs(u64)) <= 1e-8
+    assert all(q["precision_used"] == 1 and q["backward_err"] <= 1e-15 for q in st), st
+
+
 @pytest.mark.parametrize("freq", [100.0, 3e5, 1e7])
 def test_grounded_unpivoted_vs_lapack(freq):
     """The oracle's W of a small grid in the [I_L; I_T; U] order, mixed form of

@@ -470,15 +470,17 @@ __global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* 
 // D[4 (l / 16) + v][l % 16]), four real products per complex one. The
 // accumulators start from A_t and take the off-diagonal blocks negated, so they
 // end as A_t - sum L_tu U_u; that is staged in the block's LDS slot and
-// multiplied by the staged inverse.
+// multiplied by the staged inverse. The left operands of a block pass through
+// LDS in 64 x 8 chunks, double-buffered: the next chunk is fetched into
+// registers while the MFMAs of the current one run.
 typedef float u12v4 __attribute__((ext_vector_type(4)));
 template <bool UP>
 __global__ __launch_bounds__(256) void k_step_u12_mfma(c32* A, long ld, const int* pm, int j,
                                                        int jb, const c32* invl, int cb, int cols)
 {
-    constexpr int CW = 32, KC = 16;
+    constexpr int CW = 32, KC = 8, NCH = 64 / KC, LPT = 64 * KC / 256;
     __shared__ c32 Us[4][64][CW];
-    __shared__ c32 Ls[64][KC + 1];
+    __shared__ c32 Ls[2][64][KC + 1];
     __shared__ int s_pm[256];
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int lr = lane & 15, lk = lane >> 4;
@@ -489,7 +491,25 @@ __global__ __launch_bounds__(256) void k_step_u12_mfma(c32* A, long ld, const in
     for (int tt = 0; tt < nsub; tt++) {
         const int t = UP ? nsub - 1 - tt : tt;
         const int r0 = 64 * t, wt = min(64, jb - r0);
-        u12v4 ar[2], ai[2];
+        const int nu = UP ? nsub - 1 - t : t;  // off-diagonal blocks of this row block
+        const int nch = NCH * (nu + 1);        // their chunks, then the inverse's
+        const c32* li = invl + (size_t)t * 4096;
+        // chunk ci: off-diagonal block u = (ci / NCH)-th, or (last NCH) the inverse
+        auto fetch = [&](int ci, c32 (&reg)[LPT]) {
+            const int kc = (ci % NCH) * KC, b = ci / NCH;
+            const int u = UP ? t + 1 + b : b;
+#pragma unroll
+            for (int q = 0; q < LPT; q++) {
+                const int e = tid + 256 * q, r = e / KC, kk = e % KC;
+                if (b < nu) {
+                    const int k = 64 * u + kc + kk;
+                    reg[q] = r < wt && k < jb ? A[(long)s_pm[r0 + r] * ld + j + k] : c32{0, 0};
+                } else {
+                    reg[q] = li[r * 64 + kc + kk];
+                }
+            }
+        };
+        u12v4 ar[2], ai[2], orr[2], oi[2];
 #pragma unroll
         for (int q = 0; q < 2; q++)
 #pragma unroll
@@ -498,18 +518,34 @@ __global__ __launch_bounds__(256) void k_step_u12_mfma(c32* A, long ld, const in
                 const c32 x = row < wt && c < cend ? A[(long)s_pm[r0 + row] * ld + c] : c32{0, 0};
                 ar[q][v] = x.x;
                 ai[q][v] = x.y;
+                orr[q][v] = 0;
+                oi[q][v] = 0;
             }
-        for (int uu = 0; uu < (UP ? nsub - 1 - t : t); uu++) {
-            const int u = UP ? t + 1 + uu : uu;
-            for (int kc = 0; kc < 64; kc += KC) {
-                for (int e = tid; e < 64 * KC; e += 256) {
-                    const int r = e / KC, kk = e % KC, k = 64 * u + kc + kk;
-                    Ls[r][kk] = r < wt && k < jb ? A[(long)s_pm[r0 + r] * ld + j + k] : c32{0, 0};
-                }
-                __syncthreads();
+        c32 reg[LPT];
+        fetch(0, reg);
+        for (int ci = 0; ci < nch; ci++) {
+            const int buf = ci & 1;
+#pragma unroll
+            for (int q = 0; q < LPT; q++) {
+                const int e = tid + 256 * q;
+                Ls[buf][e / KC][e % KC] = reg[q];
+            }
+            if (ci + 1 < nch) fetch(ci + 1, reg);
+            const int b = ci / NCH, kc = (ci % NCH) * KC;
+            if (ci == NCH * nu) {
+                // the block's right-hand side A_t - sum L_tu U_u into its slot
+#pragma unroll
+                for (int q = 0; q < 2; q++)
+#pragma unroll
+                    for (int v = 0; v < 4; v++)
+                        Us[t][16 * wv + 4 * lk + v][16 * q + lr] = c32{ar[q][v], ai[q][v]};
+            }
+            __syncthreads();
+            if (b < nu) {
+                const int u = UP ? t + 1 + b : b;
 #pragma unroll
                 for (int kk = 0; kk < KC; kk += 4) {
-                    const c32 l = Ls[16 * wv + lr][kk + lk];
+                    const c32 l = Ls[buf][16 * wv + lr][kk + lk];
 #pragma unroll
                     for (int q = 0; q < 2; q++) {
                         const c32 x = Us[u][kc + kk + lk][16 * q + lr];
@@ -520,42 +556,22 @@ __global__ __launch_bounds__(256) void k_step_u12_mfma(c32* A, long ld, const in
                         ai[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-l.y, x.x, ai[q], 0, 0, 0);
                     }
                 }
-                __syncthreads();
-            }
-        }
-        // the block's right-hand side into its LDS slot, then inv_t times it
+            } else {
 #pragma unroll
-        for (int q = 0; q < 2; q++)
+                for (int kk = 0; kk < KC; kk += 4) {
+                    const c32 l = Ls[buf][16 * wv + lr][kk + lk];
 #pragma unroll
-            for (int v = 0; v < 4; v++)
-                Us[t][16 * wv + 4 * lk + v][16 * q + lr] = c32{ar[q][v], ai[q][v]};
-        u12v4 orr[2], oi[2];
-#pragma unroll
-        for (int q = 0; q < 2; q++) {
-            orr[q] = u12v4{0, 0, 0, 0};
-            oi[q] = u12v4{0, 0, 0, 0};
-        }
-        const c32* li = invl + (size_t)t * 4096;
-        for (int kc = 0; kc < 64; kc += KC) {
-            for (int e = tid; e < 64 * KC; e += 256) {
-                const int r = e / KC, kk = e % KC;
-                Ls[r][kk] = li[r * 64 + kc + kk];
-            }
-            __syncthreads();
-#pragma unroll
-            for (int kk = 0; kk < KC; kk += 4) {
-                const c32 l = Ls[16 * wv + lr][kk + lk];
-#pragma unroll
-                for (int q = 0; q < 2; q++) {
-                    const c32 x = Us[t][kc + kk + lk][16 * q + lr];
-                    orr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.x, x.x, orr[q], 0, 0, 0);
-                    orr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-l.y, x.y, orr[q], 0, 0, 0);
-                    oi[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.x, x.y, oi[q], 0, 0, 0);
-                    oi[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.y, x.x, oi[q], 0, 0, 0);
+                    for (int q = 0; q < 2; q++) {
+                        const c32 x = Us[t][kc + kk + lk][16 * q + lr];
+                        orr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.x, x.x, orr[q], 0, 0, 0);
+                        orr[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(-l.y, x.y, orr[q], 0, 0, 0);
+                        oi[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.x, x.y, oi[q], 0, 0, 0);
+                        oi[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(l.y, x.x, oi[q], 0, 0, 0);
+                    }
                 }
             }
-            __syncthreads();
         }
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < 2; q++)
 #pragma unroll
