@@ -293,6 +293,8 @@ struct LUWork {
     int* tcand = nullptr;       // tournament nominee lists
     unsigned* tcnt = nullptr;   // tournament arrival counters (kept at zero)
     std::vector<int> blk;   // start column of every diagonal block, in order
+    // the steps of the last factorisation: first column, width, first block
+    std::vector<int> step_j, step_w, step_slot;
     std::vector<int> h_init, h_orders;  // what d_init / d_orders hold
     double flops = 0;       // real flops of the last factorisation (GEMM + panel)
     int nodal_m = 0;        // the last factorisation's nodal block size (0: none)

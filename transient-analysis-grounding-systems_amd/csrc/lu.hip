@@ -699,21 +699,122 @@ __global__ void k_lu_init(int* perm, const int* init, int n, int* flags)
 // Triangular solves (one right-hand side, logical order) with the inverted
 // diagonal blocks; vector type V (the factor precision).
 // ---------------------------------------------------------------------------
-// y[r0 .. r0+64) = inv * y[r0 .. r0+64)   (inv: 64 x 64 row-major)
-template <typename T>
-__global__ __launch_bounds__(64) void k_block_apply(const cx<T>* inv, cx<T>* y, int r0, int w)
+// The triangular solve within one factorisation step (logical rows j .. j+jb,
+// jb <= 256, sub-blocks t of 64 with inverses at inv + 4096 t): forward,
+//     y_t = inv(L_tt) (y_t - sum_{u<t} L_tu y_u),
+// or with UP backward with the upper triangle, last block first. One
+// work-group of 4 waves; wave w takes rows 16 w .. 16 w + 15 of a block, its
+// lanes reading 64 consecutive columns of a row at a time (wave sums by lane
+// shuffles).
+// Wave sum by DPP (quad swaps, row rotations, then the row broadcasts, whose
+// masked-out rows read 0), read from lane 63: uniform.
+template <int CTRL, int ROW_MASK>
+__device__ inline float dpp_f(float v)
 {
-    __shared__ cx<T> v[64];
-    const int t = threadIdx.x;
-    v[t] = t < w ? y[r0 + t] : cx<T>{0, 0};
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF,
+                                                      false));
+}
+__device__ inline float wave_sum(float v)
+{
+    v += dpp_f<0xB1, 0xF>(v);
+    v += dpp_f<0x4E, 0xF>(v);
+    v += dpp_f<0x124, 0xF>(v);
+    v += dpp_f<0x128, 0xF>(v);
+    v += dpp_f<0x142, 0xA>(v);
+    v += dpp_f<0x143, 0xC>(v);
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+}
+template <int CTRL, int ROW_MASK>
+__device__ inline double dpp_d(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, ROW_MASK, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, ROW_MASK, 0xF,
+                                               false);
+    return __longlong_as_double(
+        (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+__device__ inline double wave_sum(double v)
+{
+    v += dpp_d<0xB1, 0xF>(v);
+    v += dpp_d<0x4E, 0xF>(v);
+    v += dpp_d<0x124, 0xF>(v);
+    v += dpp_d<0x128, 0xF>(v);
+    v += dpp_d<0x142, 0xA>(v);
+    v += dpp_d<0x143, 0xC>(v);
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, 63);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), 63);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+template <typename T>
+__device__ inline cx<T> wave_sum_cx(cx<T> v)
+{
+    return cx<T>{wave_sum(v.x), wave_sum(v.y)};
+}
+template <typename T, bool UP>
+__global__ __launch_bounds__(256) void k_step_trsv(const cx<T>* A, long ld, const int* perm,
+                                                   cx<T>* y, int j, int jb, const cx<T>* inv)
+{
+    __shared__ cx<T> ys[256];
+    __shared__ cx<T> acc[64];
+    __shared__ int s_pm[256];
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int nsub = (jb + 63) / 64;
+    ys[tid] = tid < jb ? y[j + tid] : cx<T>{0, 0};
+    s_pm[tid] = tid < jb ? perm[j + tid] : 0;
     __syncthreads();
-    cx<T> s = cx<T>{0, 0};
-    for (int k = 0; k < 64; k++) s = cadd(s, cmul(inv[t * 64 + k], v[k]));
-    if (t < w) y[r0 + t] = s;
+    for (int tt = 0; tt < nsub; tt++) {
+        const int t = UP ? nsub - 1 - tt : tt;
+        const int r0 = 64 * t, wt = min(64, jb - r0);
+        const int k0 = UP ? r0 + 64 : 0, k1 = UP ? jb : r0;
+        // acc[r] = y_t[r] - sum over the solved blocks of the step
+        // (k1 - k0 <= 192: every load of the 16 rows is issued before any sum)
+        cx<T> part[16];
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int r = 16 * wv + q;
+            const cx<T>* row = A + (long)s_pm[r0 + (r < wt ? r : 0)] * ld + j;
+            cx<T> s = cx<T>{0, 0};
+#pragma unroll
+            for (int kk = 0; kk < 3; kk++) {
+                // loads at clamped addresses, the sum taken by a select, so that
+                // no load sits behind a branch
+                const int k = k0 + lane + 64 * kk;
+                const bool ok = r < wt && k < k1;
+                const cx<T> p = cmul(row[ok ? k : 0], ys[ok ? k : 0]);
+                s = csel(ok, cadd(s, p), s);
+            }
+            part[q] = s;
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const cx<T> s = wave_sum_cx<T>(part[q]);
+            const int r = 16 * wv + q;
+            if (lane == 0) acc[r] = r < wt ? csub(ys[r0 + r], s) : cx<T>{0, 0};
+        }
+        __syncthreads();
+        // y_t = inv_t acc (the inverse is padded with the identity past wt)
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int r = 16 * wv + q;
+            part[q] = cmul(inv[(size_t)t * 4096 + r * 64 + lane], acc[lane]);
+        }
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const cx<T> s = wave_sum_cx<T>(part[q]);
+            const int r = 16 * wv + q;
+            if (lane == 0 && r < wt) ys[r0 + r] = s;
+        }
+        __syncthreads();
+    }
+    if (tid < jb) y[j + tid] = ys[tid];
 }
 
-// y[i] -= sum_{k<w} A[perm[i]][c0+k] * y[c0+k]  for logical rows i in [lo, hi)
-// one wave per row
+// y[i] -= sum_{k<w} A[perm[i]][c0+k] * y[c0+k]  for logical rows i in [lo, hi),
+// w <= 256; one wave per row (the rows and the columns [c0, c0 + w) are
+// disjoint); the loads go out unconditionally (clamped addresses, sums by
+// select), so none waits behind a branch
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_gemv(const cx<T>* A, long ld, const int* perm,
                                                      cx<T>* y, int lo, int hi, int c0, int w)
@@ -722,11 +823,15 @@ __global__ __launch_bounds__(256) void k_panel_gemv(const cx<T>* A, long ld, con
     const int lane = threadIdx.x & 63;
     if (i >= hi) return;
     cx<T> s = cx<T>{0, 0};
-    if (lane < w) s = cmul(A[(long)perm[i] * ld + c0 + lane], y[c0 + lane]);
-    for (int off = 32; off > 0; off >>= 1) {
-        s.x += __shfl_xor(s.x, off);
-        s.y += __shfl_xor(s.y, off);
+    const cx<T>* row = A + (long)perm[i] * ld + c0;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+        const int c = lane + 64 * q;
+        const bool ok = c < w;
+        const cx<T> p = cmul(row[ok ? c : 0], y[c0 + (ok ? c : 0)]);
+        s = csel(ok, cadd(s, p), s);
     }
+    s = wave_sum_cx<T>(s);
     if (lane == 0) y[i] = csub(y[i], s);
 }
 
@@ -1513,38 +1618,46 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     }
     if (pcur != w.perm) std::swap(w.perm, w.perm_alt);
     w.nblk = (int)w.blk.size();
+    w.step_j.clear();
+    w.step_w.clear();
+    w.step_slot.clear();
+    for (const LUStep& sp : steps) {
+        w.step_j.push_back(sp.j);
+        w.step_w.push_back(sp.jb);
+        w.step_slot.push_back(sp.slot);
+    }
     return 0;
 }
 
 // Solves (L U) x = P b in place on `y` (logical-order input already gathered:
 // y[i] = b[perm[i]]); output x in natural order in `y`.
-// Triangular solves over the diagonal blocks of the columns [c_lo, c_hi) only,
-// the off-diagonal products confined to the rows [c_lo, c_hi).
+// Triangular solves over the factorisation steps within the columns
+// [c_lo, c_hi), the off-diagonal products confined to the rows [c_lo, c_hi):
+// per step one k_step_trsv (its diagonal block) and one gemv (the rows beyond
+// it), forward then backward.
 template <typename T>
 static void range_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
                         int c_lo, int c_hi)
 {
-    const int n = w.n;
-    for (int b = 0; b < w.nblk; b++) {
-        const int r0 = w.blk[b];
-        if (r0 < c_lo || r0 >= c_hi) continue;
-        const int bw = std::min(b + 1 < w.nblk ? w.blk[b + 1] : n, c_hi) - r0;
-        hipLaunchKernelGGL(k_block_apply<T>, dim3(1), dim3(64), 0, st,
-                           w.invL + (size_t)b * 4096, y, r0, bw);
-        const int lo = r0 + bw;
+    const int ns = (int)w.step_j.size();
+    for (int k = 0; k < ns; k++) {
+        const int j = w.step_j[k], jb = w.step_w[k];
+        if (j < c_lo || j >= c_hi) continue;
+        hipLaunchKernelGGL((k_step_trsv<T, false>), dim3(1), dim3(256), 0, st, A, ld, w.perm, y,
+                           j, jb, (const cx<T>*)(w.invL + (size_t)w.step_slot[k] * 4096));
+        const int lo = j + jb;
         if (lo < c_hi)
             hipLaunchKernelGGL(k_panel_gemv<T>, dim3((c_hi - lo + 3) / 4), dim3(256), 0, st, A,
-                               ld, w.perm, y, lo, c_hi, r0, bw);
+                               ld, w.perm, y, lo, c_hi, j, jb);
     }
-    for (int b = w.nblk - 1; b >= 0; b--) {
-        const int r0 = w.blk[b];
-        if (r0 < c_lo || r0 >= c_hi) continue;
-        const int bw = std::min(b + 1 < w.nblk ? w.blk[b + 1] : n, c_hi) - r0;
-        hipLaunchKernelGGL(k_block_apply<T>, dim3(1), dim3(64), 0, st,
-                           w.invU + (size_t)b * 4096, y, r0, bw);
-        if (r0 > c_lo)
-            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((r0 - c_lo + 3) / 4), dim3(256), 0, st, A,
-                               ld, w.perm, y, c_lo, r0, r0, bw);
+    for (int k = ns - 1; k >= 0; k--) {
+        const int j = w.step_j[k], jb = w.step_w[k];
+        if (j < c_lo || j >= c_hi) continue;
+        hipLaunchKernelGGL((k_step_trsv<T, true>), dim3(1), dim3(256), 0, st, A, ld, w.perm, y,
+                           j, jb, (const cx<T>*)(w.invU + (size_t)w.step_slot[k] * 4096));
+        if (j > c_lo)
+            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((j - c_lo + 3) / 4), dim3(256), 0, st, A,
+                               ld, w.perm, y, c_lo, j, j, jb);
     }
 }
 
@@ -1577,25 +1690,7 @@ int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_
         tm.end();
         return 0;
     }
-    for (int b = 0; b < w.nblk; b++) {
-        const int r0 = w.blk[b];
-        const int bw = (b + 1 < w.nblk ? w.blk[b + 1] : n) - r0;
-        hipLaunchKernelGGL(k_block_apply<T>, dim3(1), dim3(64), 0, st,
-                           w.invL + (size_t)b * 4096, y, r0, bw);
-        const int lo = r0 + bw;
-        if (lo < n)
-            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((n - lo + 3) / 4), dim3(256), 0, st, A, ld,
-                               w.perm, y, lo, n, r0, bw);
-    }
-    for (int b = w.nblk - 1; b >= 0; b--) {
-        const int r0 = w.blk[b];
-        const int bw = (b + 1 < w.nblk ? w.blk[b + 1] : n) - r0;
-        hipLaunchKernelGGL(k_block_apply<T>, dim3(1), dim3(64), 0, st,
-                           w.invU + (size_t)b * 4096, y, r0, bw);
-        if (r0 > 0)
-            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((r0 + 3) / 4), dim3(256), 0, st, A, ld,
-                               w.perm, y, 0, r0, r0, bw);
-    }
+    range_solve<T>(w, A, ld, y, st, 0, n);
     TAGS_HIP(hipGetLastError());
     tm.end();
     return 0;
