@@ -31,20 +31,29 @@ struct Profiler {
             pool.pop_back();
             return e;
         }
-        hipEvent_t e;
-        hipEventCreate(&e);
+        hipEvent_t e = nullptr;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
         return e;
     }
+    // a timing that cannot get or record its events is skipped
     hipEvent_t begin(hipStream_t st)
     {
         hipEvent_t a = get();
-        hipEventRecord(a, st);
+        if (a && hipEventRecord(a, st) != hipSuccess) {
+            pool.push_back(a);
+            return nullptr;
+        }
         return a;
     }
     void end(hipEvent_t a, hipStream_t st, int fam, double flops, bool trailing)
     {
+        if (!a) return;
         hipEvent_t b = get();
-        hipEventRecord(b, st);
+        if (!b || hipEventRecord(b, st) != hipSuccess) {
+            pool.push_back(a);
+            if (b) pool.push_back(b);
+            return;
+        }
         recs.push_back(Rec{a, b, fam, flops, trailing});
     }
     // call after the stream is synchronised
@@ -52,7 +61,7 @@ struct Profiler {
     {
         for (auto& r : recs) {
             float t = 0.f;
-            hipEventElapsedTime(&t, r.a, r.b);
+            if (hipEventElapsedTime(&t, r.a, r.b) != hipSuccess) t = 0.f;
             ms[r.fam] += t;
             if (r.fam == FAM_GEMM) gemm_flops += r.flops;
             if (r.trailing) {
