@@ -83,10 +83,10 @@ struct Profiler {
     ~Profiler()
     {
         for (auto& r : recs) {
-            hipEventDestroy(r.a);
-            hipEventDestroy(r.b);
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
         }
-        for (auto e : pool) hipEventDestroy(e);
+        for (auto e : pool) (void)hipEventDestroy(e);
     }
 };
 
