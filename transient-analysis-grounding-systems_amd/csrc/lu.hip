@@ -1189,7 +1189,7 @@ static std::atomic<bool> g_lookahead{true};
 void lu_set_lookahead(bool on) { g_lookahead.store(on); }
 
 // two launches per step measured fastest on MI355X for the 60 x 60 m grid
-static std::atomic<int> g_update_chunks{2};
+static std::atomic<int> g_update_chunks{4};
 void lu_set_update_chunks(int n) { g_update_chunks.store(n < 1 ? 1 : n); }
 
 // Columns per step (four sub-panels of 64): the trailing update's K. 256 runs
