@@ -1520,6 +1520,20 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
         }
     }
     if ((int)w.blk.size() > w.nblk_cap) return TAGS_ERR_TOO_LARGE;
+    if (plan->nodal_m > 0) {
+        // the Z_L and Z_T phases share no rows or columns but the node
+        // columns, which each updates in its own rows only: alternate their
+        // steps, so that each phase's next panel runs beside the other's
+        // whole update
+        std::vector<LUStep> a, b, rest;
+        for (const LUStep& sp : steps) (sp.phase == 0 ? a : sp.phase == 1 ? b : rest).push_back(sp);
+        steps.clear();
+        for (size_t q = 0; q < std::max(a.size(), b.size()); q++) {
+            if (q < a.size()) steps.push_back(a[q]);
+            if (q < b.size()) steps.push_back(b[q]);
+        }
+        steps.insert(steps.end(), rest.begin(), rest.end());
+    }
     // Look-ahead: the panel work of step s+1 runs on the second stream with its
     // own copy of the row map while this stream finishes step s's update of
     // the columns the panel does not touch. The update of the next panel's
@@ -1554,7 +1568,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
         // the nodal plan's node phase starts after the whole of the current
         // blocks' work (its Schur complement reads their finished rows)
         const bool nodal_next =
-            plan->nodal_m > 0 && nph && steps[k + 1].phase == 2 && s.phase == 1;
+            plan->nodal_m > 0 && nph && steps[k + 1].phase == 2 && s.phase != 2;
         const bool overlap = nph && lookahead && (tournament || !nph->pivot) && !nodal_next;
         if (nph && !overlap) {
             // in order: the whole update of this step, then the next panel (the
