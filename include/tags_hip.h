@@ -282,9 +282,9 @@ int tags_set_panel(int kind);
 int tags_set_lookahead(int on);
 
 /* Number of launches (split by columns) of the part of each step's trailing
- * update that runs beside the look-ahead panel (default 4). Each launch
- * boundary lets panel work-groups waiting for a CU start. Returns 0, or
- * TAGS_ERR_ARG for n < 1. */
+ * update that runs beside the look-ahead panel (default 4; at most 2 when
+ * that panel has no row exchanges). Each launch boundary lets panel
+ * work-groups waiting for a CU start. Returns 0, or TAGS_ERR_ARG for n < 1. */
 int tags_set_update_chunks(int n);
 
 /* Columns per factorisation step of every LU in the process: 256 (default,

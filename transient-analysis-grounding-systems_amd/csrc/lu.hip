@@ -1607,8 +1607,11 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             if ((rc = step_panels<T>(w, pr, w.s2, A, ld, pnx, *nph, nx))) return rc;
             TAGS_HIP(hipEventRecord(w.ev_panel, w.s2));
             // the remaining columns of this step, in g_update_chunks launches
-            // (each launch boundary lets waiting panel work-groups start)
-            const int nch = std::max(1, g_update_chunks.load(std::memory_order_relaxed));
+            // (each launch boundary lets waiting panel work-groups start: the
+            // tournament's need a whole CU), at most 2 when the next panel has
+            // no row exchanges (one work-group of 512 threads)
+            int nch = std::max(1, g_update_chunks.load(std::memory_order_relaxed));
+            if (!nph->pivot) nch = std::min(nch, 2);
             for (int q = 0; q < 2; q++) {
                 if (re[q] <= rb[q]) continue;
                 const int lo1 = rb[q], hi1 = std::min(re[q], nb);
