@@ -113,6 +113,45 @@ int fill_impedance_imm(TAGS_COMPLEX *wg, const TAGS_COMPLEX *zl, const TAGS_COMP
 int solve_immittance(TAGS_COMPLEX *wg, TAGS_COMPLEX *ie, size_t num_electrodes,
                      size_t num_nodes);
 
+/* Nodal admittance formulation, Y_N U = I_E with Y_N = A^T Z_L^-1 A + B^T Z_T^-1 B.
+ * a, b, yla, ytb: m x n column-major (entry (segment s, node k) at m k + s);
+ * zl, zt: m x m, lower triangle read (the calculate_impedances convention);
+ * yn: n x n column-major. */
+
+/* src/linalg.h:103-106. A = +1 / -1 where the node is a segment's start / end
+ * point (1e-5 per coordinate), B = 0.5 there, 0 elsewhere, for the nodes up to
+ * the first unconnected one. Same return codes as fill_incidence_imm; with
+ * a == b the array ends as A and B is reported wrong, as in the reference. */
+int fill_incidence_adm(TAGS_COMPLEX *a, TAGS_COMPLEX *b, const Electrode *electrodes,
+                       size_t num_electrodes, const double *nodes, size_t num_nodes);
+
+/* src/linalg.h:123-126. Writes all of yn. As in the reference (zsytrf/zsytri
+ * in place) the lower triangles of zl and zt are replaced by those of Z_L^-1
+ * and Z_T^-1; their strict upper triangles are left as they were. Z^-1 and
+ * Z^-1 A come from one complex LU with row exchanges per block on the GPU
+ * (FP64, the identity and A carried as extra columns, then a blocked backward
+ * solve), the products on the MFMA cores. Returns 0, or the first exactly
+ * zero pivot (1-based; zl, zt then unchanged). */
+int fill_impedance_adm(TAGS_COMPLEX *yn, TAGS_COMPLEX *zl, TAGS_COMPLEX *zt, TAGS_COMPLEX *a,
+                       TAGS_COMPLEX *b, size_t num_electrodes, size_t num_nodes);
+
+/* src/linalg.h:145-149. yla = Z_L^-1 A, ytb = Z_T^-1 B; zl and zt as in
+ * fill_impedance_adm. */
+int calculate_yla_ytb(TAGS_COMPLEX *yla, TAGS_COMPLEX *ytb, TAGS_COMPLEX *zl, TAGS_COMPLEX *zt,
+                      TAGS_COMPLEX *a, TAGS_COMPLEX *b, size_t num_electrodes,
+                      size_t num_nodes);
+
+/* src/linalg.h:169-173. yn = A^T yla + B^T ytb (all of yn); the inputs are
+ * not modified. */
+int fill_impedance_adm2(TAGS_COMPLEX *yn, TAGS_COMPLEX *yla, TAGS_COMPLEX *ytb, TAGS_COMPLEX *a,
+                        TAGS_COMPLEX *b, size_t num_electrodes, size_t num_nodes);
+
+/* src/linalg.h:192-193. Solves Y_N x = ic for the symmetric yn whose lower
+ * triangle is stored; x replaces ic. FP64 complex LU on the GPU as in
+ * solve_immittance (the reference uses zsysv); yn is not modified. Returns 0,
+ * or i > 0 when U(i,i) is exactly zero. */
+int solve_admittance(TAGS_COMPLEX *yn, TAGS_COMPLEX *ic, size_t num_nodes);
+
 /* ---------------------------------------------------------------------------
  * Part 2: device-resident sweep (no reference counterpart)
  * ------------------------------------------------------------------------- */

@@ -68,6 +68,8 @@ def lib():
         L.or_impedances_images.restype = I
         L.or_fill_incidence_imm.argtypes = [P, P, S, P, S]
         L.or_fill_incidence_imm.restype = ctypes.c_long
+        L.or_fill_incidence_adm.argtypes = [P, P, P, S, P, S]
+        L.or_fill_incidence_adm.restype = ctypes.c_long
         L.or_fill_impedance_imm.argtypes = [P, P, P, S, S]
         L.or_fill_impedance_imm.restype = I
         L.or_self_integral.argtypes = [P]
@@ -232,6 +234,77 @@ def solve_immittance(wg, ie, m, n):
     W = wg.reshape(N, N).T  # column-major view -> row/col indexable
     _, _, x, info = lapack.zsysv(np.ascontiguousarray(W), ie.reshape(N, 1), lower=1)
     return info, x.reshape(N)
+
+
+# ---------------------------------------------------------------------------
+# nodal admittance formulation (src/linalg.c:142-350)
+# ---------------------------------------------------------------------------
+def fill_incidence_adm(e, nodes, a=None, b=None):
+    """src/linalg.c:142-199 (C restatement). a, b: m*n complex flat arrays,
+    column-major m x n (entry (segment s, node k) at m*k + s). b may be a
+    (aliasing). Returns (rc, a, b)."""
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    nodes = np.ascontiguousarray(nodes, dtype=np.float64)
+    m, n = len(e), len(nodes)
+    a = np.zeros(m * n, np.complex128) if a is None else a
+    b = np.zeros(m * n, np.complex128) if b is None else b
+    rc = lib().or_fill_incidence_adm(_p(a), _p(b), _p(e), m, _p(nodes), n)
+    return rc, a, b
+
+
+def _cm(x, rows, cols):
+    """(rows x cols) view of a column-major flat array."""
+    return x.reshape(cols, rows).T
+
+
+def _sym_lower(z, m):
+    """The symmetric matrix whose lower triangle the column-major z holds."""
+    lo = np.tril(_cm(z, m, m))
+    return lo + np.tril(lo, -1).T
+
+
+def _invert_lower(z, m):
+    """zsytrf + zsytri with uplo='L' (src/linalg.c:226-236): the lower triangle
+    of z (in place) becomes that of Z^-1; the strict upper triangle is not
+    touched. Z^-1 here is LAPACK's zgesv on the identity (numpy), the same
+    matrix up to rounding; zsytri's Bunch-Kaufman route is not in scipy."""
+    inv = np.linalg.inv(_sym_lower(z, m))
+    v = _cm(z, m, m)
+    il = np.tril_indices(m)
+    v[il] = inv[il]
+
+
+def calculate_yla_ytb(zl, zt, a, b, m, n):
+    """src/linalg.c:263-305: zl, zt replaced (lower triangles) by the inverses;
+    yla = symm(inv(Z_L)) A, ytb = symm(inv(Z_T)) B (zsymm, side 'L', uplo 'L').
+    Returns (yla, ytb) as column-major flat arrays."""
+    _invert_lower(zl, m)
+    _invert_lower(zt, m)
+    yla = _sym_lower(zl, m) @ _cm(a, m, n)
+    ytb = _sym_lower(zt, m) @ _cm(b, m, n)
+    return yla.ravel(order="F"), ytb.ravel(order="F")
+
+
+def fill_impedance_adm2(yla, ytb, a, b, m, n):
+    """src/linalg.c:307-330: yn = a^T yla + b^T ytb (zgemm 'T','N'), all of yn,
+    column-major flat."""
+    yn = _cm(a, m, n).T @ _cm(yla, m, n) + _cm(b, m, n).T @ _cm(ytb, m, n)
+    return yn.ravel(order="F")
+
+
+def fill_impedance_adm(zl, zt, a, b, m, n):
+    """src/linalg.c:201-261: the inverses into zl, zt as calculate_yla_ytb, then
+    yn = a^T (symm(inv Z_L) a) + b^T (symm(inv Z_T) b). Returns yn."""
+    yla, ytb = calculate_yla_ytb(zl, zt, a, b, m, n)
+    return fill_impedance_adm2(yla, ytb, a, b, m, n)
+
+
+def solve_admittance(yn, ie, n):
+    """src/linalg.c:332-350: zsysv('L') on the column-major yn. Returns (info, x)."""
+    from scipy.linalg import lapack
+    Y = np.ascontiguousarray(_cm(yn, n, n))
+    _, _, x, info = lapack.zsysv(Y, np.asarray(ie, np.complex128).reshape(n, 1), lower=1)
+    return info, x.reshape(n)
 
 
 def pair_sample(e, images, pairs, gamma, rel_tol=1e-6, threads=1):

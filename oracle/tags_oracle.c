@@ -591,6 +591,40 @@ long or_fill_incidence_imm(double *wg, const or_electrode *e, size_t m,
     return 0;
 }
 
+/* src/linalg.c:142-199: the same matching in the admittance layout, a and b
+ * m x n column-major (entry (s, node) at m * node + s); b is written before a
+ * and the sums read a and b after the writes, so with a == b the array ends as
+ * A and sum(B) = sum(A). Returns as or_fill_incidence_imm. */
+long or_fill_incidence_adm(double *a, double *b, const or_electrode *e, size_t m,
+                           const double *nodes, size_t n)
+{
+    const double tol = 1e-5;
+    long suma = 0;
+    double sumb = 0.0;
+    for (size_t nd = 0; nd < n; nd++) {
+        int none = 1;
+        for (size_t s = 0; s < m; s++) {
+            int cond = 0;
+            if (or_equal_points_tol(e[s].start_point, nodes + 3 * nd, tol))
+                cond = 1;
+            else if (or_equal_points_tol(e[s].end_point, nodes + 3 * nd, tol))
+                cond = 2;
+            const size_t pos = m * nd + s;
+            b[2 * pos] = cond ? 0.5 : 0.0;
+            b[2 * pos + 1] = 0.0;
+            a[2 * pos] = cond == 1 ? 1.0 : (cond == 2 ? -1.0 : 0.0);
+            a[2 * pos + 1] = 0.0;
+            if (cond) none = 0;
+            suma += (long)a[2 * pos];
+            sumb += b[2 * pos];
+        }
+        if (none) return 1;
+    }
+    if (suma != 0) return -suma;
+    if (sumb != (double)m) return (long)(sumb - m);
+    return 0;
+}
+
 /* src/linalg.c:88-110: -Z_L and -Z_T copied into their blocks for all (i, k),
  * zero cross blocks and zero Y_E block. */
 int or_fill_impedance_imm(double *wg, const double *zl, const double *zt, size_t m,

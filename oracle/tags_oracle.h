@@ -86,6 +86,8 @@ int or_impedances_images(double *zl, double *zt, const or_electrode *e,
 /* W: (2m+n)^2 complex, column-major, caller-zeroed. */
 long or_fill_incidence_imm(double *wg, const or_electrode *e, size_t m,
                            const double *nodes, size_t n);
+long or_fill_incidence_adm(double *a, double *b, const or_electrode *e, size_t m,
+                           const double *nodes, size_t n);
 int or_fill_impedance_imm(double *wg, const double *zl, const double *zt, size_t m,
                           size_t n);
 

@@ -110,7 +110,9 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_dense_solve", "tags_version", "tags_device_synchronize", "tags_gemm_bench",
             "tags_gemm_variants", "tags_gemm_select", "tags_autotune", "tags_sweep_set_plan",
             "tags_sweep_assemble", "tags_immittance_solve", "tags_sweep_set_inflight",
-            "tags_gemm_run", "tags_set_step_width", "tags_sweep_set_nodal_above"]
+            "tags_gemm_run", "tags_set_step_width", "tags_sweep_set_nodal_above",
+            "fill_incidence_adm", "fill_impedance_adm", "calculate_yla_ytb",
+            "fill_impedance_adm2", "solve_admittance"]
 
 _lib = None
 
@@ -139,6 +141,11 @@ def lib():
         "fill_incidence_imm": ([P, P, S, P, S], I),
         "fill_impedance_imm": ([P, P, P, S, S], I),
         "solve_immittance": ([P, P, S, S], I),
+        "fill_incidence_adm": ([P, P, P, S, P, S], I),
+        "fill_impedance_adm": ([P, P, P, P, P, S, S], I),
+        "calculate_yla_ytb": ([P, P, P, P, P, P, S, S], I),
+        "fill_impedance_adm2": ([P, P, P, P, P, S, S], I),
+        "solve_admittance": ([P, P, S], I),
         "tags_sweep_create": ([ctypes.POINTER(P), P, P, S, P, S], I),
         "tags_sweep_solve": ([P, S, P, P, D, P, P, P, S, I, P, P, P], I),
         "tags_sweep_set_profiling": ([P, I], I),
@@ -270,6 +277,34 @@ def fill_impedance_imm(wg, zl, zt, num_electrodes, num_nodes):
 def solve_immittance(wg, ie, num_electrodes, num_nodes):
     """ie (complex128, N) is overwritten by [U; I_L; I_T]; returns info."""
     return lib().solve_immittance(_p(wg), _p(ie), num_electrodes, num_nodes)
+
+
+def fill_incidence_adm(a, b, electrodes, nodes):
+    """a, b: complex128 arrays of m*n (column-major m x n), written in place."""
+    e = _electrodes(electrodes)
+    nodes = np.ascontiguousarray(nodes, np.float64)
+    return lib().fill_incidence_adm(_p(a), _p(b), _p(e), len(e), _p(nodes), len(nodes))
+
+
+def fill_impedance_adm(yn, zl, zt, a, b, num_electrodes, num_nodes):
+    """yn (n*n) written; the lower triangles of zl, zt replaced by the inverses'."""
+    return lib().fill_impedance_adm(_p(yn), _p(zl), _p(zt), _p(a), _p(b), num_electrodes,
+                                    num_nodes)
+
+
+def calculate_yla_ytb(yla, ytb, zl, zt, a, b, num_electrodes, num_nodes):
+    return lib().calculate_yla_ytb(_p(yla), _p(ytb), _p(zl), _p(zt), _p(a), _p(b),
+                                   num_electrodes, num_nodes)
+
+
+def fill_impedance_adm2(yn, yla, ytb, a, b, num_electrodes, num_nodes):
+    return lib().fill_impedance_adm2(_p(yn), _p(yla), _p(ytb), _p(a), _p(b), num_electrodes,
+                                     num_nodes)
+
+
+def solve_admittance(yn, ic, num_nodes):
+    """ic (complex128, n) is overwritten by the node potentials; returns info."""
+    return lib().solve_admittance(_p(yn), _p(ic), num_nodes)
 
 
 def dense_solve(a, b, precision=PREC_FP64):

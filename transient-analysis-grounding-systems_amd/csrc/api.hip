@@ -888,12 +888,19 @@ static std::vector<double> seg_points(const Electrode* e, size_t m)
     return sp;
 }
 
-int fill_incidence_imm(TAGS_COMPLEX* wg, const Electrode* electrodes, size_t num_electrodes,
-                       const double* nodes, size_t num_nodes)
+// Node/segment incidence on the GPU, in the reference's node order: nodes up to
+// and including the first one no segment touches (nw of them; bad = its index
+// or -1), as blk1 (2m x nw column-major: A in rows s, B in rows m + s) and blk2
+// (nw x 2m column-major: the transposes); counts = start and end matches.
+struct Incidence {
+    DevBuf<c64> b1, b2;
+    long nw = 0, bad = -1;
+    unsigned long long c[2] = {0, 0};
+};
+
+static int incidence_blocks(Incidence& inc, const Electrode* electrodes, long m,
+                            const double* nodes, long n, hipStream_t st)
 {
-    const long m = (long)num_electrodes, n = (long)num_nodes, N = n + 2 * m;
-    if (m == 0 || n == 0) return 0;
-    Stream S;
     int rc;
     std::vector<double> sp = seg_points(electrodes, m);
     DevBuf<double> dsp, dnodes;
@@ -901,49 +908,46 @@ int fill_incidence_imm(TAGS_COMPLEX* wg, const Electrode* electrodes, size_t num
     if ((rc = dsp.alloc(6 * m)) || (rc = dnodes.alloc(3 * n)) || (rc = dsn.alloc(m)) ||
         (rc = den.alloc(m)) || (rc = dhits.alloc(n + 2 * m)))
         return rc;
-    TAGS_HIP(hipMemcpyAsync(dsp.p, sp.data(), sizeof(double) * 6 * m, hipMemcpyHostToDevice, S.s));
-    TAGS_HIP(hipMemcpyAsync(dnodes.p, nodes, sizeof(double) * 3 * n, hipMemcpyHostToDevice, S.s));
-    TAGS_HIP(hipMemsetAsync(dhits.p, 0, sizeof(int) * (n + 2 * m), S.s));
-    if ((rc = launch_incidence(dsp.p, m, dnodes.p, n, dsn.p, den.p, dhits.p, S.s))) return rc;
+    TAGS_HIP(hipMemcpyAsync(dsp.p, sp.data(), sizeof(double) * 6 * m, hipMemcpyHostToDevice, st));
+    TAGS_HIP(hipMemcpyAsync(dnodes.p, nodes, sizeof(double) * 3 * n, hipMemcpyHostToDevice, st));
+    TAGS_HIP(hipMemsetAsync(dhits.p, 0, sizeof(int) * (n + 2 * m), st));
+    if ((rc = launch_incidence(dsp.p, m, dnodes.p, n, dsn.p, den.p, dhits.p, st))) return rc;
     std::vector<int> hits(n + 2 * m);
-    TAGS_HIP(hipMemcpyAsync(hits.data(), dhits.p, sizeof(int) * n, hipMemcpyDeviceToHost, S.s));
-    TAGS_HIP(hipStreamSynchronize(S.s));
-    // the reference stops after the first node without a segment (:72-75)
-    long nw = n, bad = -1;
+    TAGS_HIP(hipMemcpyAsync(hits.data(), dhits.p, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    TAGS_HIP(hipStreamSynchronize(st));
+    // the reference stops after the first node without a segment (src/linalg.c:72-75)
+    inc.nw = n;
+    inc.bad = -1;
     for (long i = 0; i < n; i++)
         if (hits[i] == 0) {
-            bad = i;
-            nw = i + 1;
+            inc.bad = i;
+            inc.nw = i + 1;
             break;
         }
-    DevBuf<c64> b1, b2;
     DevBuf<unsigned long long> cnt;
-    if ((rc = b1.alloc((size_t)2 * m * nw)) || (rc = b2.alloc((size_t)2 * m * nw)) ||
+    if ((rc = inc.b1.alloc((size_t)2 * m * inc.nw)) || (rc = inc.b2.alloc((size_t)2 * m * inc.nw)) ||
         (rc = cnt.alloc(2)))
         return rc;
-    TAGS_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, S.s));
+    TAGS_HIP(hipMemsetAsync(cnt.p, 0, sizeof(unsigned long long) * 2, st));
     {
-        long tot = nw * m;
+        long tot = inc.nw * m;
         unsigned g = (unsigned)std::min<long>((tot + 255) / 256, 4096);
-        hipLaunchKernelGGL(k_incidence_blocks, dim3(g), dim3(256), 0, S.s, dsp.p, m, dnodes.p, nw,
-                           b1.p, b2.p, cnt.p);
+        hipLaunchKernelGGL(k_incidence_blocks, dim3(g), dim3(256), 0, st, dsp.p, m, dnodes.p,
+                           inc.nw, inc.b1.p, inc.b2.p, cnt.p);
         TAGS_HIP(hipGetLastError());
     }
-    // blk1 -> W rows n..N, columns 0..nw ; blk2 -> W rows 0..nw, columns n..N
-    c64* W = (c64*)wg;
-    TAGS_HIP(hipMemcpy2DAsync(W + n, sizeof(c64) * N, b1.p, sizeof(c64) * 2 * m,
-                              sizeof(c64) * 2 * m, nw, hipMemcpyDeviceToHost, S.s));
-    TAGS_HIP(hipMemcpy2DAsync(W + N * n, sizeof(c64) * N, b2.p, sizeof(c64) * nw,
-                              sizeof(c64) * nw, 2 * m, hipMemcpyDeviceToHost, S.s));
-    unsigned long long c[2];
-    TAGS_HIP(hipMemcpyAsync(c, cnt.p, sizeof(c), hipMemcpyDeviceToHost, S.s));
-    TAGS_HIP(hipStreamSynchronize(S.s));
+    TAGS_HIP(hipMemcpyAsync(inc.c, cnt.p, sizeof(inc.c), hipMemcpyDeviceToHost, st));
+    TAGS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+// the reference's return codes and messages (src/linalg.c:72-84, :186-198)
+static int incidence_verdict(long bad, long suma, double sumb, long m)
+{
     if (bad >= 0) {
         std::printf("No electrode is connected to node[%i]\n", (int)bad);
         return 1;
     }
-    const long suma = (long)c[0] - (long)c[1];
-    const double sumb = 0.5 * (double)(c[0] + c[1]);
     if (suma != 0) {
         std::printf("incidence matrix A is wrong. sum(A) = %li != 0\n", suma);
         return (int)-suma;
@@ -953,6 +957,55 @@ int fill_incidence_imm(TAGS_COMPLEX* wg, const Electrode* electrodes, size_t num
         return (int)(sumb - m);
     }
     return 0;
+}
+
+int fill_incidence_imm(TAGS_COMPLEX* wg, const Electrode* electrodes, size_t num_electrodes,
+                       const double* nodes, size_t num_nodes)
+{
+    const long m = (long)num_electrodes, n = (long)num_nodes, N = n + 2 * m;
+    // no nodes: the sums stay 0 (B is reported wrong unless m = 0); no
+    // segments: node 0 is unconnected
+    if (n == 0) return incidence_verdict(-1, 0, 0.0, m);
+    if (m == 0) return incidence_verdict(0, 0, 0.0, 0);
+    Stream S;
+    Incidence inc;
+    int rc = incidence_blocks(inc, electrodes, m, nodes, n, S.s);
+    if (rc) return rc;
+    const long nw = inc.nw;
+    // blk1 -> W rows n..N, columns 0..nw ; blk2 -> W rows 0..nw, columns n..N
+    c64* W = (c64*)wg;
+    TAGS_HIP(hipMemcpy2DAsync(W + n, sizeof(c64) * N, inc.b1.p, sizeof(c64) * 2 * m,
+                              sizeof(c64) * 2 * m, nw, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipMemcpy2DAsync(W + N * n, sizeof(c64) * N, inc.b2.p, sizeof(c64) * nw,
+                              sizeof(c64) * nw, 2 * m, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    return incidence_verdict(inc.bad, (long)inc.c[0] - (long)inc.c[1],
+                             0.5 * (double)(inc.c[0] + inc.c[1]), m);
+}
+
+// src/linalg.c:143-199: a and b (m x n column-major, entry (s, node) at
+// m * node + s) for the nodes up to the first unconnected one. b is written
+// before a, so with a == b the array ends as A; the reference then also sums
+// B over a's values (sum(B) = sum(A) = 0) and reports B as wrong.
+int fill_incidence_adm(TAGS_COMPLEX* a, TAGS_COMPLEX* b, const Electrode* electrodes,
+                       size_t num_electrodes, const double* nodes, size_t num_nodes)
+{
+    const long m = (long)num_electrodes, n = (long)num_nodes;
+    if (n == 0) return incidence_verdict(-1, 0, 0.0, m);
+    if (m == 0) return incidence_verdict(0, 0, 0.0, 0);
+    Stream S;
+    Incidence inc;
+    int rc = incidence_blocks(inc, electrodes, m, nodes, n, S.s);
+    if (rc) return rc;
+    const long nw = inc.nw;
+    TAGS_HIP(hipMemcpy2DAsync(b, sizeof(c64) * m, inc.b1.p + m, sizeof(c64) * 2 * m,
+                              sizeof(c64) * m, nw, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipMemcpy2DAsync(a, sizeof(c64) * m, inc.b1.p, sizeof(c64) * 2 * m,
+                              sizeof(c64) * m, nw, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    const long suma = (long)inc.c[0] - (long)inc.c[1];
+    const double sumb = a == b ? (double)suma : 0.5 * (double)(inc.c[0] + inc.c[1]);
+    return incidence_verdict(inc.bad, suma, sumb, m);
 }
 
 }  // extern "C"
@@ -1083,6 +1136,237 @@ int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_ele
     const bool g = plan == TAGS_PLAN_GROUNDED || plan == TAGS_PLAN_GROUNDED_PIVOTED;
     return dense_solve_host((const c64*)a, (c64*)b, nn + 2 * m, precision, false, stats, &p,
                             g || plan == TAGS_PLAN_NODAL ? &p32 : nullptr, gr);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// nodal admittance formulation (src/linalg.c:201-350)
+// ---------------------------------------------------------------------------
+namespace tags {
+// dst[c * ldd + r] = scale * src[(map ? map[r] : r) * lds + c] for r < rows,
+// c < cols (lower: r >= c only), through a 32 x 32 LDS tile; grid (cols / 32,
+// rows / 32) rounded up.
+__global__ __launch_bounds__(256) void k_transpose(const c64* __restrict__ src, long lds,
+                                                   const int* __restrict__ map, c64* dst,
+                                                   long ldd, int rows, int cols, double scale,
+                                                   int lower)
+{
+    __shared__ c64 t[32][33];
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    if (lower && r0 + 31 < c0) return;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + tx;
+        if (r < rows && c < cols) t[i][tx] = src[(long)(map ? map[r] : r) * lds + c];
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + tx;
+        if (r < rows && c < cols && (!lower || r >= c)) {
+            const c64 v = t[tx][i];
+            dst[(long)c * ldd + r] = c64{scale * v.x, scale * v.y};
+        }
+    }
+}
+
+// rows [0, m) of the m x m block at column c0 of a row-major matrix set to I
+__global__ __launch_bounds__(256) void k_identity(c64* M, long ld, long c0, long m)
+{
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < m * m;
+         idx += (long)gridDim.x * 256) {
+        const long r = idx / m, c = idx % m;
+        M[r * ld + c0 + c] = c64{r == c ? 1.0 : 0.0, 0.0};
+    }
+}
+
+static int transpose(const c64* src, long lds, const int* map, c64* dst, long ldd, long rows,
+                     long cols, double scale, bool lower, hipStream_t st)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    if ((rows + 31) / 32 > 65535) return TAGS_ERR_TOO_LARGE;
+    hipLaunchKernelGGL(k_transpose, dim3((unsigned)((cols + 31) / 32), (unsigned)((rows + 31) / 32)),
+                       dim3(256), 0, st, src, lds, map, dst, ldd, (int)rows, (int)cols, scale,
+                       lower ? 1 : 0);
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+// One complex symmetric block Z (m x m, its lower triangle in the column-major
+// device array dz, as zsytrf/zsytri read it) and a right-hand block C, crm
+// holding s C (m x n, row-major on the device, ld n): the row-major work
+// matrix [Z | s C | I]
+// (ld m + n + m) is factored on its first m columns with row exchanges, its
+// trailing updates carrying s C and I through L^-1 P, and the extra columns
+// are completed by the backward blocked solve to [s Z^-1 C | Z^-1]. Then
+// Z^-1 C goes to dy (m x n column-major) and the lower triangle of Z^-1 over
+// dz's lower triangle (its strict upper triangle is left as it was). Returns
+// 0, the first zero pivot (1-based) or an error.
+static int adm_block(LUWork<double>& w, c64* dz, const c64* crm, double s, long m, long n,
+                     c64* dy, hipStream_t st)
+{
+    if (m >= 65536 || n >= 65536) return TAGS_ERR_TOO_LARGE;
+    const long ld = 2 * m + n;
+    int rc;
+    DevInfo& di = dev_info();
+    if (!di.ok) return TAGS_ERR_HIP;
+    if ((rc = w.alloc((int)m, di.num_cu, di.lds_limit))) return rc;
+    DevBuf<c64> M;
+    if ((rc = M.alloc((size_t)m * ld))) return rc;
+    TAGS_HIP(hipMemcpy2DAsync(M.p, sizeof(c64) * ld, dz, sizeof(c64) * m, sizeof(c64) * m, m,
+                              hipMemcpyDeviceToDevice, st));
+    if ((rc = sym_from_upper(M.p, ld, (int)m, st))) return rc;
+    if (n > 0)
+        TAGS_HIP(hipMemcpy2DAsync(M.p + m, sizeof(c64) * ld, crm, sizeof(c64) * n,
+                                  sizeof(c64) * n, m, hipMemcpyDeviceToDevice, st));
+    {
+        unsigned g = (unsigned)std::min<long>((m * m + 255) / 256, 8192);
+        hipLaunchKernelGGL(k_identity, dim3(g), dim3(256), 0, st, M.p, ld, m + n, m);
+        TAGS_HIP(hipGetLastError());
+    }
+    LUPlan plan;
+    plan.phases.push_back(LUPhase{0, (int)m, (int)m, (int)m, (int)m, (int)ld});
+    if ((rc = lu_factor<double>(w, M.p, ld, st, nullptr, &plan))) return rc;
+    int flags[2];
+    TAGS_HIP(hipMemcpyAsync(flags, w.flags, sizeof(flags), hipMemcpyDeviceToHost, st));
+    TAGS_HIP(hipStreamSynchronize(st));
+    if (flags[1]) return TAGS_ERR_BARRIER;
+    if (flags[0]) return flags[0];
+    if ((rc = lu_solve_cols<double>(w, M.p, ld, st, nullptr, (int)m, (int)(n + m)))) return rc;
+    if ((rc = transpose(M.p + m, ld, w.perm, dy, m, m, n, 1.0 / s, false, st))) return rc;
+    if ((rc = transpose(M.p + m + n, ld, w.perm, dz, m, m, m, 1.0, true, st))) return rc;
+    TAGS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+}  // namespace tags
+
+extern "C" {
+
+// Device copies of the host arrays of one admittance call.
+struct AdmArrays {
+    DevBuf<c64> zl, zt, arm, brm, yla, ytb, yn;
+};
+
+// Z^-1 (lower triangles, into zl and zt on the host), yla = Z_L^-1 A and
+// ytb = Z_T^-1 B (on the device, column-major m x n) and the row-major A and
+// -B (m x n) for the products.
+static int adm_inverses(AdmArrays& d, TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const TAGS_COMPLEX* a,
+                        const TAGS_COMPLEX* b, long m, long n, hipStream_t st)
+{
+    int rc;
+    if (m == 0) return 0;
+    if ((rc = d.zl.alloc((size_t)m * m)) || (rc = d.zt.alloc((size_t)m * m)) ||
+        (rc = d.arm.alloc((size_t)m * n)) || (rc = d.brm.alloc((size_t)m * n)) ||
+        (rc = d.yla.alloc((size_t)m * n)) || (rc = d.ytb.alloc((size_t)m * n)))
+        return rc;
+    {
+        // a, b (column-major m x n) staged in yla / ytb, transposed to row-major
+        TAGS_HIP(hipMemcpyAsync(d.yla.p, a, sizeof(c64) * m * n, hipMemcpyHostToDevice, st));
+        TAGS_HIP(hipMemcpyAsync(d.ytb.p, b, sizeof(c64) * m * n, hipMemcpyHostToDevice, st));
+        if ((rc = transpose(d.yla.p, m, nullptr, d.arm.p, n, n, m, 1.0, false, st)) ||
+            (rc = transpose(d.ytb.p, m, nullptr, d.brm.p, n, n, m, -1.0, false, st)))
+            return rc;
+    }
+    TAGS_HIP(hipMemcpyAsync(d.zl.p, zl, sizeof(c64) * m * m, hipMemcpyHostToDevice, st));
+    TAGS_HIP(hipMemcpyAsync(d.zt.p, zt, sizeof(c64) * m * m, hipMemcpyHostToDevice, st));
+    LUWork<double> w;
+    if ((rc = adm_block(w, d.zl.p, d.arm.p, 1.0, m, n, d.yla.p, st))) return rc;
+    if ((rc = adm_block(w, d.zt.p, d.brm.p, -1.0, m, n, d.ytb.p, st))) return rc;
+    TAGS_HIP(hipMemcpyAsync(zl, d.zl.p, sizeof(c64) * m * m, hipMemcpyDeviceToHost, st));
+    TAGS_HIP(hipMemcpyAsync(zt, d.zt.p, sizeof(c64) * m * m, hipMemcpyDeviceToHost, st));
+    TAGS_HIP(hipStreamSynchronize(st));
+    return 0;
+}
+
+// yn (n x n column-major) = A^T yla + B^T ytb: as row-major products,
+// yn^T-view(v, u) = sum_i yla(i, v) A(i, u), i.e. C = P Q with P = yla read
+// row-major (n x m) and Q = A row-major (m x n); the B term as C -= ytb (-B).
+static int adm_products(AdmArrays& d, long m, long n, hipStream_t st)
+{
+    int rc;
+    if ((rc = d.yn.alloc((size_t)n * n))) return rc;
+    if (m == 0) {
+        TAGS_HIP(hipMemsetAsync(d.yn.p, 0, sizeof(c64) * n * n, st));
+        return 0;
+    }
+    GemmArgs<double> g{(int)n, (int)n, (int)m, d.yla.p, m, nullptr, d.arm.p, n, nullptr,
+                       d.yn.p, n, nullptr, 0};
+    if ((rc = launch_gemm<double>(g, st))) return rc;
+    g.A = d.ytb.p;
+    g.B = d.brm.p;
+    g.mode = 1;
+    return launch_gemm<double>(g, st);
+}
+
+int fill_impedance_adm(TAGS_COMPLEX* yn, TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, TAGS_COMPLEX* a,
+                       TAGS_COMPLEX* b, size_t num_electrodes, size_t num_nodes)
+{
+    const long m = (long)num_electrodes, n = (long)num_nodes;
+    Stream S;
+    AdmArrays d;
+    int rc = adm_inverses(d, zl, zt, a, b, m, n, S.s);
+    if (rc) return rc;
+    if ((rc = adm_products(d, m, n, S.s))) return rc;
+    TAGS_HIP(hipMemcpyAsync(yn, d.yn.p, sizeof(c64) * n * n, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    return 0;
+}
+
+int calculate_yla_ytb(TAGS_COMPLEX* yla, TAGS_COMPLEX* ytb, TAGS_COMPLEX* zl, TAGS_COMPLEX* zt,
+                      TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_electrodes, size_t num_nodes)
+{
+    const long m = (long)num_electrodes, n = (long)num_nodes;
+    if (m == 0) return 0;
+    Stream S;
+    AdmArrays d;
+    int rc = adm_inverses(d, zl, zt, a, b, m, n, S.s);
+    if (rc) return rc;
+    TAGS_HIP(hipMemcpyAsync(yla, d.yla.p, sizeof(c64) * m * n, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipMemcpyAsync(ytb, d.ytb.p, sizeof(c64) * m * n, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    return 0;
+}
+
+int fill_impedance_adm2(TAGS_COMPLEX* yn, TAGS_COMPLEX* yla, TAGS_COMPLEX* ytb, TAGS_COMPLEX* a,
+                        TAGS_COMPLEX* b, size_t num_electrodes, size_t num_nodes)
+{
+    const long m = (long)num_electrodes, n = (long)num_nodes;
+    if (n == 0) return 0;
+    Stream S;
+    AdmArrays d;
+    int rc;
+    if (m > 0) {
+        if ((rc = d.arm.alloc((size_t)m * n)) || (rc = d.brm.alloc((size_t)m * n)) ||
+            (rc = d.yla.alloc((size_t)m * n)) || (rc = d.ytb.alloc((size_t)m * n)))
+            return rc;
+        // a and b pass through one staging buffer to their row-major copies
+        DevBuf<c64> stage;
+        if ((rc = stage.alloc((size_t)m * n))) return rc;
+        TAGS_HIP(hipMemcpyAsync(stage.p, a, sizeof(c64) * m * n, hipMemcpyHostToDevice, S.s));
+        if ((rc = transpose(stage.p, m, nullptr, d.arm.p, n, n, m, 1.0, false, S.s))) return rc;
+        TAGS_HIP(hipStreamSynchronize(S.s));
+        TAGS_HIP(hipMemcpyAsync(stage.p, b, sizeof(c64) * m * n, hipMemcpyHostToDevice, S.s));
+        if ((rc = transpose(stage.p, m, nullptr, d.brm.p, n, n, m, -1.0, false, S.s))) return rc;
+        TAGS_HIP(hipMemcpyAsync(d.yla.p, yla, sizeof(c64) * m * n, hipMemcpyHostToDevice, S.s));
+        TAGS_HIP(hipMemcpyAsync(d.ytb.p, ytb, sizeof(c64) * m * n, hipMemcpyHostToDevice, S.s));
+        TAGS_HIP(hipStreamSynchronize(S.s));
+    }
+    if ((rc = adm_products(d, m, n, S.s))) return rc;
+    TAGS_HIP(hipMemcpyAsync(yn, d.yn.p, sizeof(c64) * n * n, hipMemcpyDeviceToHost, S.s));
+    TAGS_HIP(hipStreamSynchronize(S.s));
+    return 0;
+}
+
+int solve_admittance(TAGS_COMPLEX* yn, TAGS_COMPLEX* ic, size_t num_nodes)
+{
+    int rc = dense_solve_host((const c64*)yn, (c64*)ic, (long)num_nodes, TAGS_PREC_FP64, true,
+                              nullptr);
+    if (rc > 0) {
+        std::printf("The diagonal element of the triangular factor of YN,\n");
+        std::printf("U(%i,%i) is zero, so that YN is singular;\n", rc, rc);
+        std::printf("the solution could not be computed.\n");
+    }
+    return rc;
 }
 
 }  // extern "C"

@@ -322,6 +322,12 @@ int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_
                      Profiler* pr);
 template <typename T>
 int lu_gather_rhs(LUWork<T>& w, const cx<T>* b, cx<T>* y, hipStream_t st);
+// After a factorisation whose plan carried extra columns [x0, x0 + nx) through
+// its trailing updates (they hold L^-1 P C): completes them to U^-1 L^-1 P C =
+// A^-1 C, in logical rows (physical row w.perm[i] holds unknown i). Not for the
+// nodal plan.
+template <typename T>
+int lu_solve_cols(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, int x0, int nx);
 // trailing selects the kernel name k_trailing_update (same code as k_cgemm)
 template <typename T>
 int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing = false);

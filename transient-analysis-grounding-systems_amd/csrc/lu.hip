@@ -1400,51 +1400,64 @@ static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, lon
                    A + cb, ld, pm + c1, 1, true, &w.flops);
 }
 
+// Backward blocked solve of the extra columns [x0, x0 + nx) in the logical rows
+// of the steps of phase p (p < 0: every step) from c_lo on: those rows hold
+// Y = L^-1 P C (the factorisation's trailing updates of the extra columns) and
+// become X = U^-1 Y. Last step first: the step's own rows in k_step_u12, then
+// the rows above it from c_lo by one product with K = the step width.
+// Flops: 4 r^2 nx for r rows.
+template <typename T>
+static int backward_cols(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
+                         const int* pm, const std::vector<LUStep>& steps, int p, int c_lo,
+                         int x0, int nx)
+{
+    constexpr int CW = sizeof(T) == 4 ? 32 : 16, KC = sizeof(T) == 4 ? 16 : 8;
+    int rc;
+    for (int k = (int)steps.size() - 1; k >= 0; k--) {
+        const LUStep& s = steps[k];
+        if ((p >= 0 && s.phase != p) || s.j < c_lo) continue;
+        if (s.jb > 256) return TAGS_ERR_ARG;
+        double flops = 0;
+        for (int r0 = 0; r0 < s.jb; r0 += 64) {
+            const int wt = std::min(64, s.jb - r0);
+            flops += 8.0 * wt * (double)nx * (s.jb - r0);
+        }
+        {
+            Timed tm(pr, FAM_GEMM, st);
+            if constexpr (sizeof(T) == 4)
+                hipLaunchKernelGGL(k_step_u12_mfma<true>, dim3((nx + 31) / 32), dim3(256), 0, st,
+                                   A, ld, pm, s.j, s.jb,
+                                   (const c32*)(w.invU + (size_t)s.slot * 4096), x0, nx);
+            else
+                hipLaunchKernelGGL((k_step_u12<T, CW, KC, true>), dim3((nx + CW - 1) / CW),
+                                   dim3(256), 0, st, A, ld, pm, s.j, s.jb,
+                                   (const cx<T>*)(w.invU + (size_t)s.slot * 4096), x0, nx);
+            TAGS_HIP(hipGetLastError());
+            tm.end(flops);
+            w.flops += flops;
+        }
+        if (s.j > c_lo &&
+            (rc = gemm<T>(pr, st, s.j - c_lo, nx, s.jb, A + s.j, ld, pm + c_lo, A + x0, ld,
+                          pm + s.j, A + x0, ld, pm + c_lo, 1, false, &w.flops)))
+            return rc;
+    }
+    return 0;
+}
+
 // Nodal plan, before its node phase: the node columns of the Z_L and Z_T rows
 // hold Y = L^-1 P A (their phases' trailing updates); complete them to
-// X = U^-1 Y by the blocked backward solve over each phase's steps, last step
-// first (the step's own rows in k_step_u12, then the rows above it in the
-// phase by one product with K = the step width), then write the node block's
-// Schur complement (k_nodal_schur). Flops: 4 m^2 n per current block.
+// X = U^-1 Y (backward_cols over each phase's steps), then write the node
+// block's Schur complement (k_nodal_schur). Flops: 4 m^2 n per current block.
 template <typename T>
 static int nodal_prepare(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
                          const int* pm, const LUPlan& plan, const std::vector<LUStep>& steps)
 {
-    constexpr int CW = sizeof(T) == 4 ? 32 : 16, KC = sizeof(T) == 4 ? 16 : 8;
     const int m = plan.nodal_m, nn = w.n - 2 * m, x0 = 2 * m;
     int rc;
-    for (int p = 0; p < 2; p++) {
-        const LUPhase& ph = plan.phases[p];
-        for (int k = (int)steps.size() - 1; k >= 0; k--) {
-            const LUStep& s = steps[k];
-            if (s.phase != p) continue;
-            if (s.jb > 256) return TAGS_ERR_ARG;
-            double flops = 0;
-            for (int r0 = 0; r0 < s.jb; r0 += 64) {
-                const int wt = std::min(64, s.jb - r0);
-                flops += 8.0 * wt * (double)nn * (s.jb - r0);
-            }
-            {
-                Timed tm(pr, FAM_GEMM, st);
-                if constexpr (sizeof(T) == 4)
-                    hipLaunchKernelGGL(k_step_u12_mfma<true>, dim3((nn + 31) / 32), dim3(256), 0,
-                                       st, A, ld, pm, s.j, s.jb,
-                                       (const c32*)(w.invU + (size_t)s.slot * 4096), x0, nn);
-                else
-                    hipLaunchKernelGGL((k_step_u12<T, CW, KC, true>), dim3((nn + CW - 1) / CW),
-                                       dim3(256), 0, st, A, ld, pm, s.j, s.jb,
-                                       (const cx<T>*)(w.invU + (size_t)s.slot * 4096), x0, nn);
-                TAGS_HIP(hipGetLastError());
-                tm.end(flops);
-                w.flops += flops;
-            }
-            if (s.j > ph.c_begin &&
-                (rc = gemm<T>(pr, st, s.j - ph.c_begin, nn, s.jb, A + s.j, ld, pm + ph.c_begin,
-                              A + x0, ld, pm + s.j, A + x0, ld, pm + ph.c_begin, 1, false,
-                              &w.flops)))
-                return rc;
-        }
-    }
+    for (int p = 0; p < 2; p++)
+        if ((rc = backward_cols<T>(w, pr, st, A, ld, pm, steps, p, plan.phases[p].c_begin, x0,
+                                   nn)))
+            return rc;
     Timed tm(pr, FAM_OTHER, st);
     hipLaunchKernelGGL(k_nodal_schur<T>, dim3(nn), dim3(256), 0, st, A, ld, pm, m, nn);
     TAGS_HIP(hipGetLastError());
@@ -1714,6 +1727,17 @@ int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_
 }
 
 template <typename T>
+int lu_solve_cols(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, int x0, int nx)
+{
+    if (w.nodal_m > 0 || nx < 0 || x0 < w.n) return TAGS_ERR_ARG;
+    if (nx == 0) return 0;
+    std::vector<LUStep> steps;
+    for (size_t k = 0; k < w.step_j.size(); k++)
+        steps.push_back(LUStep{w.step_j[k], w.step_w[k], 0, w.step_slot[k]});
+    return backward_cols<T>(w, pr, st, A, ld, w.perm, steps, -1, 0, x0, nx);
+}
+
+template <typename T>
 int lu_gather_rhs(LUWork<T>& w, const cx<T>* b, cx<T>* y, hipStream_t st)
 {
     hipLaunchKernelGGL(k_gather_vec<T>, dim3((w.n + 255) / 256), dim3(256), 0, st, b, w.perm,
@@ -1824,6 +1848,8 @@ template int lu_solve_inplace<double>(LUWork<double>&, const c64*, long, c64*, h
 template int lu_solve_inplace<float>(LUWork<float>&, const c32*, long, c32*, hipStream_t,
                                      Profiler*);
 template int lu_gather_rhs<double>(LUWork<double>&, const c64*, c64*, hipStream_t);
+template int lu_solve_cols<double>(LUWork<double>&, c64*, long, hipStream_t, Profiler*, int, int);
+template int lu_solve_cols<float>(LUWork<float>&, c32*, long, hipStream_t, Profiler*, int, int);
 template int lu_gather_rhs<float>(LUWork<float>&, const c32*, c32*, hipStream_t);
 
 }  // namespace tags
