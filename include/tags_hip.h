@@ -71,10 +71,15 @@ TAGS_COMPLEX self_integral(const Electrode *sender);
  * order Gauss-Legendre with analytic treatment of touching segments) and
  * returns L_r L_s times the unit-square integral of exp(-gamma r)/r, the same
  * quantity as the reference; `error` receives the difference to a rule of
- * higher order. INTG_NONE returns the distance between middle points. max_eval,
- * req_abs_error and req_rel_error are accepted; the rule meets a relative
- * error far below any req_rel_error >= 1e-10. Returns 0, -10 for an unknown
- * type, -11 for INTG_SINGLE / INTG_MHEM (not supported by this library). */
+ * higher order. INTG_MHEM returns L_r times the receiver-axis integral of
+ * log((r1 + r2 + L_s)/(r1 + r2 - L_s)) (src/electrode.c:219-235), which is the
+ * double integral of 1/r: the same pair rule at gamma = 0, imaginary part 0,
+ * error per unit receiver length as in the reference. INTG_NONE returns the
+ * distance between middle points. max_eval, req_abs_error and req_rel_error
+ * are accepted; the rule meets a relative error far below any req_rel_error
+ * >= 1e-10. Returns 0, -10 for an unknown type, -11 for INTG_SINGLE (not
+ * supported by this library). calculate_impedances and impedances_images take
+ * INTG_MHEM with the reference's unit scalars. */
 int integral(const Electrode *sender, const Electrode *receiver, TAGS_COMPLEX gamma,
              size_t max_eval, double req_abs_error, double req_rel_error,
              int integration_type, double result[2], double error[2]);
@@ -263,6 +268,24 @@ enum tags_plan {
     TAGS_PLAN_HYBRID = 7
 };
 int tags_sweep_set_plan(tags_sweep *sw, int plan);
+
+/* Impedance formulation of a sweep's samples.
+ * TAGS_FORM_HEM   (default) every sample integrates e^{-gamma r}/r over every
+ *                 segment pair (calculate_impedances + impedances_images with
+ *                 INTG_DOUBLE, src/electrode.c:305-431).
+ * TAGS_FORM_MHEM  the reference examples' modified HEM
+ *                 (examples/harmonic_impedance.c:78-85, :125-140): the
+ *                 gamma-free INTG_MHEM potentials are integrated once per
+ *                 geometry (4 m^2 doubles of device memory), and each sample
+ *                 scales them by exp(-gamma rbar) (rbar: middle-point distance
+ *                 of the pair, of segment and image for the image part), s mu /
+ *                 4 pi, 1 / (4 pi kappa) and the reflection coefficients. The
+ *                 examples then solve the nodal admittance system; the sweep
+ *                 solves the immittance system with the same Z_L, Z_T, whose
+ *                 node potentials are the same (Y_N is its Schur complement).
+ * Returns 0, or TAGS_ERR_ARG for another value. */
+enum tags_formulation { TAGS_FORM_HEM = 0, TAGS_FORM_MHEM = 1 };
+int tags_sweep_set_formulation(tags_sweep *sw, int formulation);
 
 /* Switch frequency of TAGS_PLAN_HYBRID (default 2e4 Hz): in the mixed form a
  * sample at |s| / 2 pi >= hz takes the nodal plan's complex64 factor, a lower

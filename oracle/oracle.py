@@ -76,6 +76,8 @@ def lib():
         L.or_self_integral.restype = D
         L.or_integral.argtypes = [P, P, P, S, D, D, I, P, P]
         L.or_integral.restype = I
+        L.or_gk15_rule.argtypes = [P, P, P]
+        L.or_gk15_rule.restype = None
         L.or_populate_electrode.argtypes = [P, P, P, D]
         L.or_populate_electrode.restype = I
         L.or_segment_electrode.argtypes = [P, P, S, P, P, D]
@@ -174,6 +176,14 @@ def soil_scalars(freq, sigma, epsr, mur=1.0):
 def self_integral(e) -> float:
     e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE).reshape(1)
     return lib().or_self_integral(_p(e))
+
+
+def gk15_rule():
+    """The Gauss-Kronrod (7, 15) abscissae and weights on [-1, 1] that the 1-D
+    adaptive rule uses: (x[8], kronrod weights[8], gauss weights[4])."""
+    x, wk, wg = np.zeros(8), np.zeros(8), np.zeros(4)
+    lib().or_gk15_rule(_p(x), _p(wk), _p(wg))
+    return x, wk, wg
 
 
 def integral(sender, receiver, gamma, rel_tol=1e-6, abs_tol=0.0, max_eval=0,
@@ -305,6 +315,77 @@ def solve_admittance(yn, ie, n):
     Y = np.ascontiguousarray(_cm(yn, n, n))
     _, _, x, info = lapack.zsysv(Y, np.asarray(ie, np.complex128).reshape(n, 1), lower=1)
     return info, x.reshape(n)
+
+
+# ---------------------------------------------------------------------------
+# modified HEM (examples/harmonic_impedance.c:66-164)
+# ---------------------------------------------------------------------------
+def mhem_potentials(e, images, rel_tol=1e-6, threads=1):
+    """The gamma-free potentials of examples/harmonic_impedance.c:78-85:
+    calculate_impedances and impedances_images with INTG_MHEM (unit scalars).
+    Returns (potzl, potzt, potzli, potzti), flat m*m, lower-half convention."""
+    m = len(e)
+    rc, pl, pt = calculate_impedances(e, 0.0, 0.0, 0.0, 0.0, rel_tol=rel_tol,
+                                      integration_type=INTG_MHEM, threads=threads)
+    if rc:
+        raise RuntimeError("calculate_impedances (mHEM) failed: %d" % rc)
+    pli = np.zeros(m * m, np.complex128)
+    pti = np.zeros(m * m, np.complex128)
+    rc = impedances_images(pli, pti, e, images, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, rel_tol=rel_tol,
+                           integration_type=INTG_MHEM, threads=threads)
+    if rc:
+        raise RuntimeError("impedances_images (mHEM) failed: %d" % rc)
+    return pl, pt, pli, pti
+
+
+def mhem_impedances(pots, e, images, sc):
+    """examples/harmonic_impedance.c:116-140: for k >= m (flat m*ne + k),
+    zl = (e^{-gamma rbar} potzl + ref_l e^{-gamma rbar_i} potzli) iwu_4pi and
+    zt likewise with ref_t and one_4pik; rbar = |mid(e[k]) - mid(e[m])|,
+    rbar_i = |mid(e[k]) - mid(images[m])|. Returns (zl, zt)."""
+    pl, pt, pli, pti = pots
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    images = np.ascontiguousarray(images, dtype=ELECTRODE_DTYPE)
+    ne = len(e)
+    rows, cols = np.triu_indices(ne)  # reference loop m (rows) <= k (cols)
+    flat = rows * ne + cols
+    mid, imid = e["middle_point"], images["middle_point"]
+    rbar = np.sqrt(((mid[cols] - mid[rows]) ** 2).sum(1))
+    rbi = np.sqrt(((mid[cols] - imid[rows]) ** 2).sum(1))
+    g = sc["gamma"]
+    iwu_4pi = sc["s"] * sc["mur"] * MU0 / FOUR_PI
+    one_4pik = 1.0 / (FOUR_PI * sc["kappa"])
+    zl = np.zeros(ne * ne, np.complex128)
+    zt = np.zeros(ne * ne, np.complex128)
+    ed, ei = np.exp(-g * rbar), np.exp(-g * rbi)
+    zl[flat] = (ed * pl[flat] + sc["ref_l"] * ei * pli[flat]) * iwu_4pi
+    zt[flat] = (ed * pt[flat] + sc["ref_t"] * ei * pti[flat]) * one_4pik
+    return zl, zt
+
+
+def mhem_harmonic_impedance(e, nodes, freqs, sigma, epsr, inj_node=0, mur=1.0, rel_tol=1e-6,
+                            threads=1, pots=None):
+    """Z_h(f) = U[inj] for a unit current by the examples' route: mHEM
+    impedances, fill_impedance_adm, zsysv on Y_N (harmonic_impedance.c:141-164)."""
+    images = images_of(e)
+    m, n = len(e), len(nodes)
+    rc, a, b = fill_incidence_adm(e, nodes)
+    if rc:
+        raise RuntimeError("fill_incidence_adm failed: %d" % rc)
+    if pots is None:
+        pots = mhem_potentials(e, images, rel_tol=rel_tol, threads=threads)
+    zh = np.zeros(len(freqs), np.complex128)
+    for i, f in enumerate(freqs):
+        sc = soil_scalars(f, sigma, epsr, mur)
+        zl, zt = mhem_impedances(pots, e, images, sc)
+        yn = fill_impedance_adm(zl, zt, a, b, m, n)
+        ie = np.zeros(n, np.complex128)
+        ie[inj_node] = 1.0
+        info, u = solve_admittance(yn, ie, n)
+        if info != 0:
+            raise RuntimeError("zsysv info=%d at f=%g" % (info, f))
+        zh[i] = u[inj_node]
+    return zh
 
 
 def pair_sample(e, images, pairs, gamma, rel_tol=1e-6, threads=1):

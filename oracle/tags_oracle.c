@@ -373,6 +373,141 @@ static void or_integrand_cb(const double t[2], void *data, double out[2])
     or_integrand_double(t, p->sender, p->receiver, p->g, out);
 }
 
+/* ------------------------------------------------------------------------- */
+/* Adaptive 1-D Gauss-Kronrod (7, 15) on [0, 1], one component: the route    */
+/* hcubature takes for a 1-D integrand (src/electrode.c:288-290, fdim 1,      */
+/* ERROR_INDIVIDUAL). Cubature (stevengj/cubature, version not pinned) uses   */
+/* QUADPACK's QK15 rule with QUADPACK's error heuristic; the region with the  */
+/* largest error is bisected until err <= abs_tol or err <= rel_tol |val|.    */
+/* ------------------------------------------------------------------------- */
+/* QUADPACK dqk15 abscissae / weights (Piessens et al. 1983); the Gauss       */
+/* 7-point rule uses the odd-indexed abscissae.                               */
+static const double gk_x[8] = {
+    0.991455371120812639206854697526329, 0.949107912342758524526189684047851,
+    0.864864423359769072789712788640926, 0.741531185599394439863864773280788,
+    0.586087235467691130294144845693013, 0.405845151377397166906606412076961,
+    0.207784955007898467600689403773245, 0.0};
+static const double gk_wk[8] = {
+    0.022935322010529224963732008058970, 0.063092092629978553290700663189204,
+    0.104790010322250183839876322541518, 0.140653259715525918745189590510238,
+    0.169004726639267902826583426598550, 0.190350578064785409913256402421014,
+    0.204432940075298892414161999234649, 0.209482141084727828012999174891714};
+static const double gk_wg[4] = {
+    0.129484966168869693270611432679082, 0.279705391489276667901467771423780,
+    0.381830050505118944950369775488975, 0.417959183673469387755102040816327};
+
+void or_gk15_rule(double x[8], double wk[8], double wg[4])
+{
+    memcpy(x, gk_x, sizeof(gk_x));
+    memcpy(wk, gk_wk, sizeof(gk_wk));
+    memcpy(wg, gk_wg, sizeof(gk_wg));
+}
+
+typedef double (*or_integrand1)(double t, void *data);
+
+static void gk15(or_integrand1 f, void *data, double a, double b, double *val, double *err)
+{
+    const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+    double fv1[7], fv2[7];
+    const double fc = f(c, data);
+    double rg = fc * gk_wg[3], rk = fc * gk_wk[7], ra = fabs(rk);
+    for (int j = 0; j < 3; j++) {
+        const int jj = 2 * j + 1;
+        const double x = h * gk_x[jj];
+        const double f1 = f(c - x, data), f2 = f(c + x, data);
+        fv1[jj] = f1;
+        fv2[jj] = f2;
+        rg += gk_wg[j] * (f1 + f2);
+        rk += gk_wk[jj] * (f1 + f2);
+        ra += gk_wk[jj] * (fabs(f1) + fabs(f2));
+    }
+    for (int j = 0; j < 4; j++) {
+        const int jj = 2 * j;
+        const double x = h * gk_x[jj];
+        const double f1 = f(c - x, data), f2 = f(c + x, data);
+        fv1[jj] = f1;
+        fv2[jj] = f2;
+        rk += gk_wk[jj] * (f1 + f2);
+        ra += gk_wk[jj] * (fabs(f1) + fabs(f2));
+    }
+    const double mean = 0.5 * rk;
+    double asc = gk_wk[7] * fabs(fc - mean);
+    for (int j = 0; j < 7; j++) asc += gk_wk[j] * (fabs(fv1[j] - mean) + fabs(fv2[j] - mean));
+    double e = fabs((rk - rg) * h);
+    asc *= fabs(h);
+    ra *= fabs(h);
+    if (asc != 0.0 && e != 0.0) {
+        const double q = pow(200.0 * e / asc, 1.5);
+        e = asc * (q < 1.0 ? q : 1.0);
+    }
+    if (ra > DBL_MIN / (50.0 * DBL_EPSILON)) {
+        const double emin = 50.0 * DBL_EPSILON * ra;
+        if (emin > e) e = emin;
+    }
+    *val = rk * h;
+    *err = e;
+}
+
+int or_quad1(or_integrand1 f, void *data, size_t max_eval, double abs_tol, double rel_tol,
+             double *val, double *err)
+{
+    size_t cap = 64, n = 1, ne = 15;
+    double *ra = malloc(cap * sizeof(double)), *rb = malloc(cap * sizeof(double));
+    double *rv = malloc(cap * sizeof(double)), *re = malloc(cap * sizeof(double));
+    ra[0] = 0.0;
+    rb[0] = 1.0;
+    gk15(f, data, 0.0, 1.0, &rv[0], &re[0]);
+    for (;;) {
+        double tv = 0, te = 0;
+        size_t worst = 0;
+        for (size_t i = 0; i < n; i++) {
+            tv += rv[i];
+            te += re[i];
+            if (re[i] > re[worst]) worst = i;
+        }
+        *val = tv;
+        *err = te;
+        if (te <= abs_tol || te <= rel_tol * fabs(tv)) break;
+        if (max_eval && ne + 30 > max_eval) break;
+        if (n >= ((size_t)1 << 20)) break; /* memory guard */
+        if (n == cap) {
+            cap *= 2;
+            ra = realloc(ra, cap * sizeof(double));
+            rb = realloc(rb, cap * sizeof(double));
+            rv = realloc(rv, cap * sizeof(double));
+            re = realloc(re, cap * sizeof(double));
+        }
+        const double a = ra[worst], b = rb[worst], mid = 0.5 * (a + b);
+        rb[worst] = mid;
+        gk15(f, data, a, mid, &rv[worst], &re[worst]);
+        ra[n] = mid;
+        rb[n] = b;
+        gk15(f, data, mid, b, &rv[n], &re[n]);
+        n++;
+        ne += 30;
+    }
+    free(ra);
+    free(rb);
+    free(rv);
+    free(re);
+    return 0;
+}
+
+/* src/electrode.c:219-235: at the receiver-axis point t, the sender-axis
+ * integral of 1/r in closed form, log((r1 + r2 + L_s) / (r1 + r2 - L_s)). */
+static double or_log_nf(double t, void *data)
+{
+    const or_idata *p = (const or_idata *)data;
+    double pr[3];
+    for (int i = 0; i < 3; i++)
+        pr[i] = t * (p->receiver->end_point[i] - p->receiver->start_point[i]) +
+                p->receiver->start_point[i];
+    const double r1 = or_vector_length(pr, p->sender->start_point);
+    const double r2 = or_vector_length(pr, p->sender->end_point);
+    const double L = p->sender->length;
+    return log((r1 + r2 + L) / (r1 + r2 - L));
+}
+
 /* src/electrode.c:238-245: closed form of the double integral of 1/r between two
  * parallel filaments of length L one radius apart; independent of gamma. */
 double or_self_integral(const or_electrode *e)
@@ -383,8 +518,10 @@ double or_self_integral(const or_electrode *e)
     return 2.0 * ls * (log((k2 + 1.0) / k1) - k2 + k1);
 }
 
-/* src/electrode.c:247-302 (INTG_NONE and INTG_DOUBLE branches; -10 otherwise;
- * INTG_SINGLE and INTG_MHEM are outside the restated path and also give -10). */
+/* src/electrode.c:247-302 (INTG_NONE, INTG_DOUBLE and INTG_MHEM branches; -10
+ * otherwise: INTG_SINGLE is outside the restated path and also gives -10).
+ * INTG_MHEM: result[0] = L_r times the 1-D integral, result[1] = 0; error[0]
+ * is not scaled (as in the reference). */
 int or_integral(const or_electrode *sender, const or_electrode *receiver,
                 const double gamma[2], size_t max_eval, double abs_tol, double rel_tol,
                 int integration_type, double result[2], double error[2])
@@ -400,6 +537,12 @@ int or_integral(const or_electrode *sender, const or_electrode *receiver,
                                  error, NULL);
         result[0] *= receiver->length * sender->length;
         result[1] *= receiver->length * sender->length;
+        return fail;
+    }
+    case OR_INTG_MHEM: {
+        or_idata d = {sender, receiver, {0.0, 0.0}};
+        int fail = or_quad1(or_log_nf, &d, max_eval, abs_tol, rel_tol, &result[0], &error[0]);
+        result[0] *= receiver->length;
         return fail;
     }
     default:

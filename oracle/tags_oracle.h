@@ -84,6 +84,7 @@ int or_impedances_images(double *zl, double *zt, const or_electrode *e,
                          int nthreads);
 
 /* W: (2m+n)^2 complex, column-major, caller-zeroed. */
+void or_gk15_rule(double x[8], double wk[8], double wg[4]);
 long or_fill_incidence_imm(double *wg, const or_electrode *e, size_t m,
                            const double *nodes, size_t n);
 long or_fill_incidence_adm(double *a, double *b, const or_electrode *e, size_t m,

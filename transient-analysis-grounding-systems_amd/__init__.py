@@ -37,6 +37,7 @@ PLAN_DENSE, PLAN_STRUCTURED, PLAN_ZL_FIRST, PLAN_BLOCK_PIVOT, PLAN_GROUNDED = 0,
 PLAN_GROUNDED_PIVOTED = 5
 PLAN_NODAL = 6
 PLAN_HYBRID = 7
+FORM_HEM, FORM_MHEM = 0, 1  # tags_formulation
 
 # src/auxiliary.c:17-21
 TWO_PI = 6.283185307179586476925286766559
@@ -112,7 +113,7 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_sweep_assemble", "tags_immittance_solve", "tags_sweep_set_inflight",
             "tags_gemm_run", "tags_set_step_width", "tags_sweep_set_nodal_above",
             "fill_incidence_adm", "fill_impedance_adm", "calculate_yla_ytb",
-            "fill_impedance_adm2", "solve_admittance"]
+            "fill_impedance_adm2", "solve_admittance", "tags_sweep_set_formulation"]
 
 _lib = None
 
@@ -146,6 +147,7 @@ def lib():
         "calculate_yla_ytb": ([P, P, P, P, P, P, S, S], I),
         "fill_impedance_adm2": ([P, P, P, P, P, S, S], I),
         "solve_admittance": ([P, P, S], I),
+        "tags_sweep_set_formulation": ([P, I], I),
         "tags_sweep_create": ([ctypes.POINTER(P), P, P, S, P, S], I),
         "tags_sweep_solve": ([P, S, P, P, D, P, P, P, S, I, P, P, P], I),
         "tags_sweep_set_profiling": ([P, I], I),
@@ -454,6 +456,13 @@ class Sweep:
         rc = lib().tags_sweep_set_nodal_above(self._h, float(hz))
         if rc:
             raise RuntimeError("tags_sweep_set_nodal_above failed: %d" % rc)
+
+    def set_formulation(self, formulation):
+        """FORM_HEM (pair integrals per sample) or FORM_MHEM (gamma-free
+        potentials once, exp(-gamma rbar) scaling per sample)."""
+        rc = lib().tags_sweep_set_formulation(self._h, int(formulation))
+        if rc:
+            raise RuntimeError("tags_sweep_set_formulation failed: %d" % rc)
 
     def set_inflight(self, samples):
         rc = lib().tags_sweep_set_inflight(self._h, samples)

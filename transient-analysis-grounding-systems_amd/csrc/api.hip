@@ -731,14 +731,15 @@ int integral(const Electrode* sender, const Electrode* receiver, TAGS_COMPLEX ga
         result[0] = vlen(sender->middle_point, receiver->middle_point);
         return 0;
     }
-    if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
+    if (integration_type == INTG_SINGLE) {
         std::printf("integration type %d is not supported by this library\n", integration_type);
         return TAGS_ERR_UNSUPPORTED;
     }
-    if (integration_type != INTG_DOUBLE) {
+    if (integration_type != INTG_DOUBLE && integration_type != INTG_MHEM) {
         std::printf("integration failed with code: %i\n", -10);
         return -10;
     }
+    const bool mhem = integration_type == INTG_MHEM;
     Stream S;
     DevSegs a, b;
     int rc;
@@ -746,10 +747,19 @@ int integral(const Electrode* sender, const Electrode* receiver, TAGS_COMPLEX ga
     if ((rc = b.upload(receiver, 1, S.s))) return rc;
     DevBuf<double> out;
     if ((rc = out.alloc(4))) return rc;
-    if ((rc = launch_single_pair(a.soa(), b.soa(), from_c(gamma), out.p, S.s))) return rc;
+    if ((rc = launch_single_pair(a.soa(), b.soa(), mhem ? c64{0.0, 0.0} : from_c(gamma), out.p,
+                                 S.s)))
+        return rc;
     double h[4];
     TAGS_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, S.s));
     TAGS_HIP(hipStreamSynchronize(S.s));
+    if (mhem) {
+        // the gamma = 0 double integral (real); the reference scales only
+        // result[0] by the receiver length, its error stays per unit length
+        result[0] = h[0];
+        error[0] = receiver->length > 0 ? h[2] / receiver->length : h[2];
+        return 0;
+    }
     result[0] = h[0];
     result[1] = h[1];
     error[0] = h[2];
@@ -795,11 +805,12 @@ int calculate_impedances(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* el
     (void)max_eval;
     (void)req_abs_error;
     (void)req_rel_error;
-    if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
+    if (integration_type == INTG_SINGLE) {
         std::printf("integration type %d is not supported by this library\n", integration_type);
         return TAGS_ERR_UNSUPPORTED;
     }
-    if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE) {
+    if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE &&
+        integration_type != INTG_MHEM) {
         std::printf("integration failed with code: %i\n", -10);
         return -10;
     }
@@ -817,11 +828,12 @@ int impedances_images(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* elect
     (void)max_eval;
     (void)req_abs_error;
     (void)req_rel_error;
-    if (integration_type == INTG_SINGLE || integration_type == INTG_MHEM) {
+    if (integration_type == INTG_SINGLE) {
         std::printf("integration type %d is not supported by this library\n", integration_type);
         return TAGS_ERR_UNSUPPORTED;
     }
-    if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE) {
+    if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE &&
+        integration_type != INTG_MHEM) {
         std::printf("integration failed with code: %i\n", -10);
         return -10;
     }
@@ -1437,6 +1449,10 @@ struct tags_sweep {
     int* en = nullptr;
     std::vector<std::unique_ptr<SweepSlot>> slots;
     bool profiling = false;
+    // TAGS_FORM_MHEM: the gamma-free mHEM potentials (direct Z_L, Z_T, image
+    // Z_L, Z_T; real m x m each), computed at the first sample after the switch
+    int formulation = TAGS_FORM_HEM;
+    double* pot = nullptr;
 };
 
 // The first k slots exist and are allocated for order N (precision `mixed`).
@@ -1534,9 +1550,31 @@ static int sweep_fill(tags_sweep* sw, c64* W, hipStream_t st, Profiler* prof, TA
     Timed tm(prof, FAM_FILL, st);
     int rc;
     if ((rc = launch_fill_static(W, N, sw->layout, sw->sn, sw->en, st))) return rc;
-    if ((rc = launch_fill_zblocks(W, N, sw->layout, sw->e.soa(), sw->img.soa(), sc, st)))
+    if (sw->formulation == TAGS_FORM_MHEM) {
+        if (!sw->pot) return TAGS_ERR_ARG;
+        const size_t mm = (size_t)sw->m * sw->m;
+        if ((rc = launch_fill_mhem(W, N, sw->layout, sw->e.soa(), sw->img.soa(), sw->pot,
+                                   sw->pot + mm, sw->pot + 2 * mm, sw->pot + 3 * mm, sc, st)))
+            return rc;
+    } else if ((rc = launch_fill_zblocks(W, N, sw->layout, sw->e.soa(), sw->img.soa(), sc,
+                                         st))) {
         return rc;
+    }
     tm.end();
+    return 0;
+}
+
+// The mHEM potentials, once per geometry (on the set-up stream, finished
+// before any sample is queued).
+static int sweep_potentials(tags_sweep* sw)
+{
+    if (sw->formulation != TAGS_FORM_MHEM || sw->pot) return 0;
+    const size_t mm = (size_t)sw->m * sw->m;
+    TAGS_HIP(hipMalloc(&sw->pot, sizeof(double) * 4 * mm));
+    int rc = launch_mhem_potentials(sw->pot, sw->pot + mm, sw->pot + 2 * mm, sw->pot + 3 * mm,
+                                    sw->m, sw->e.soa(), sw->img.soa(), sw->st);
+    if (rc) return rc;
+    TAGS_HIP(hipStreamSynchronize(sw->st));
     return 0;
 }
 
@@ -1612,8 +1650,8 @@ int tags_sweep_assemble(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, doub
     const long N = sw->N;
     sweep_layout(sw);
     DevBuf<c64> W;
-    int rc = W.alloc((size_t)N * N);
-    if (rc) return rc;
+    int rc = sweep_potentials(sw);
+    if (rc || (rc = W.alloc((size_t)N * N))) return rc;
     if ((rc = sweep_fill(sw, W.p, sw->st, nullptr, s, kappa, mur, nullptr, nullptr, nullptr)))
         return rc;
     std::vector<c64> h((size_t)N * N);
@@ -1686,6 +1724,14 @@ int tags_sweep_get_timers(tags_sweep* sw, tags_timers* t)
     return 0;
 }
 
+int tags_sweep_set_formulation(tags_sweep* sw, int formulation)
+{
+    if (!sw || (formulation != TAGS_FORM_HEM && formulation != TAGS_FORM_MHEM))
+        return TAGS_ERR_ARG;
+    sw->formulation = formulation;
+    return 0;
+}
+
 void tags_sweep_destroy(tags_sweep* sw)
 {
     if (!sw) return;
@@ -1693,6 +1739,7 @@ void tags_sweep_destroy(tags_sweep* sw)
     sw->slots.clear();
     hipFree(sw->sn);
     hipFree(sw->en);
+    hipFree(sw->pot);
     hipStream_t st = sw->st;
     delete sw;
     hipStreamDestroy(st);
@@ -1778,7 +1825,7 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     // every configured slot is allocated up front (a warm-up call sets all of
     // them up)
     int rc = sweep_slots(sw, sw->inflight, mixed);
-    if (rc) return rc;
+    if (rc || (rc = sweep_potentials(sw))) return rc;
     sweep_layout(sw);
     const int k = (int)std::min<size_t>((size_t)sw->inflight, std::max<size_t>(nfreq, 1));
     size_t next = 0;

@@ -329,15 +329,23 @@ __device__ inline DSeg load_seg(const SegSoA& s, long i)
 }
 
 // Reference's `integral` result (L_r L_s times the unit-square integral, or the
-// middle-point distance for INTG_NONE).
+// middle-point distance for INTG_NONE). INTG_MHEM (src/electrode.c:219-235,
+// :288-293) is L_r times the receiver-axis integral of
+// log((r1 + r2 + L_s) / (r1 + r2 - L_s)), which is the sender-axis integral of
+// 1/r in closed form: the same quantity as the double integral of 1/r, i.e.
+// the gamma = 0 value of the pair rule, with a zero imaginary part.
 __device__ inline c64 pair_result(const DSeg& S, const DSeg& R, c64 g, int type)
 {
     if (type == INTG_NONE) {
         V3 ms = vmul(vadd(S.a, S.b), 0.5), mr = vmul(vadd(R.a, R.b), 0.5);
         return c64{vnorm(vsub(ms, mr)), 0.0};
     }
+    const double ll = R.len * S.len;
+    if (type == INTG_MHEM) {
+        const c64 J = pair_integral_unit(S, R, c64{0.0, 0.0});
+        return c64{J.x * ll, 0.0};
+    }
     c64 J = pair_integral_unit(S, R, g);
-    double ll = R.len * S.len;
     return c64{J.x * ll, J.y * ll};
 }
 
@@ -525,6 +533,148 @@ int launch_fill_zblocks(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA
     const long nt = T * (T + 1) / 2;
     hipLaunchKernelGGL(k_fill_zblocks, dim3((unsigned)nt), dim3(256), 0, st, W, ldw, L.zl_off,
                        L.zt_off, m, e, img, sc);
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// mHEM (examples/harmonic_impedance.c:78-85, :125-140): the gamma-free
+// potentials, once per geometry, are the reference's calculate_impedances and
+// impedances_images with INTG_MHEM (unit scalars, src/electrode.c:312-316,
+// :391-395); every frequency sample scales them by exp(-gamma rbar) with rbar
+// the middle-point distance of the pair (of segment and image for the image
+// part) and by s mu / 4 pi, 1 / (4 pi kappa) and the reflection coefficients.
+// Potentials: real row-major m x m arrays, entries (i, k <= i) only, for the
+// reference pair (sender e[k], receiver e[i] or image[i]).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_mhem_potentials(double* pl, double* pt, double* pli,
+                                                          double* pti, long m, SegSoA e,
+                                                          SegSoA img)
+{
+    const long k = blockIdx.x * 16 + threadIdx.x % 16;
+    const long i = blockIdx.y * 16 + threadIdx.x / 16;
+    if (i >= m || k > i) return;
+    const long idx = i * m + k;
+    const DSeg Si = load_seg(e, i), Sk = load_seg(e, k), Ii = load_seg(img, i);
+    const double li = Si.len, lk = Sk.len;
+    if (k == i) {
+        // src/electrode.c:346-348
+        const double intg = self_integral_dev(li, e.rad[i]);
+        pl[idx] = intg;
+        pt[idx] = intg / (li * li);
+    } else {
+        // src/electrode.c:352-372
+        const double intg = pair_result(Sk, Si, c64{0.0, 0.0}, INTG_MHEM).x;
+        pt[idx] = intg / (lk * li);
+        const double cost = abs_dot(Sk, Si);
+        pl[idx] = cost < 2.220446049250313e-16 ? 0.0 : intg * cost / (lk * li);
+    }
+    // src/electrode.c:403-428 (k <= i, the image of the receiver)
+    const double intg = pair_result(Sk, Ii, c64{0.0, 0.0}, INTG_MHEM).x;
+    pti[idx] = intg / (lk * Ii.len);
+    const double cost = abs_dot(Sk, Ii);
+    pli[idx] = cost > 2.220446049250313e-16 ? intg * cost / (lk * Ii.len) : 0.0;
+}
+
+int launch_mhem_potentials(double* pl, double* pt, double* pli, double* pti, long m, SegSoA e,
+                           SegSoA img, hipStream_t st)
+{
+    if (m <= 0) return 0;
+    if ((m + 15) / 16 > 65535) return TAGS_ERR_TOO_LARGE;
+    int rc = gl_tables_init();
+    if (rc) return rc;
+    dim3 grid((unsigned)((m + 15) / 16), (unsigned)((m + 15) / 16));
+    hipLaunchKernelGGL(k_mhem_potentials, grid, dim3(256), 0, st, pl, pt, pli, pti, m, e, img);
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+// One frequency sample of the mHEM impedance blocks, -Z_L and -Z_T straight
+// into W (the tiling of k_fill_zblocks):
+//   Z_L(i, k) = s mu/4pi (exp(-gamma rbar) pl + ref_l exp(-gamma rbar_img) pli),
+//   Z_T(i, k) = 1/(4 pi kappa) (exp(-gamma rbar) pt + ref_t exp(-gamma rbar_img) pti),
+// rbar = |mid(e[i]) - mid(e[k])|, rbar_img = |mid(e[i]) - mid(image[k])|.
+__global__ __launch_bounds__(256) void k_fill_mhem(c64* W, long ldw, long zl_off, long zt_off,
+                                                    long m, SegSoA e, SegSoA img,
+                                                    const double* __restrict__ pl,
+                                                    const double* __restrict__ pt,
+                                                    const double* __restrict__ pli,
+                                                    const double* __restrict__ pti,
+                                                    PairScalars sc)
+{
+    const long b = blockIdx.x;
+    long ti = (long)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
+    while ((ti + 1) * (ti + 2) / 2 <= b) ti++;
+    while (ti * (ti + 1) / 2 > b) ti--;
+    const long tk = b - ti * (ti + 1) / 2;
+    const long i0 = ti * kFT, k0 = tk * kFT;
+    __shared__ double mid[3][3][kFT];  // [e rows, e cols, image cols][xyz][seg]
+    __shared__ c64 tl[kFT][kFT + 1];
+    __shared__ c64 tt[kFT][kFT + 1];
+    const int t = threadIdx.x;
+    if (t < 3 * kFT) {
+        const int side = t / kFT, s = t % kFT;
+        const long idx = (side == 0 ? i0 : k0) + s;
+        const SegSoA& g = side == 2 ? img : e;
+        if (idx < m) {
+            mid[side][0][s] = 0.5 * (g.ax[idx] + g.bx[idx]);
+            mid[side][1][s] = 0.5 * (g.ay[idx] + g.by[idx]);
+            mid[side][2][s] = 0.5 * (g.az[idx] + g.bz[idx]);
+        }
+    }
+    __syncthreads();
+    for (int p = t; p < kFT * kFT; p += 256) {
+        const int a = p / kFT, c = p % kFT;
+        const long i = i0 + a, k = k0 + c;
+        c64 vl = c64{0, 0}, vt = c64{0, 0};
+        if (i < m && k < m && k <= i) {
+            const long idx = i * m + k;
+            const V3 mi = V3{mid[0][0][a], mid[0][1][a], mid[0][2][a]};
+            const double rb = vnorm(vsub(mi, V3{mid[1][0][c], mid[1][1][c], mid[1][2][c]}));
+            const double ri = vnorm(vsub(mi, V3{mid[2][0][c], mid[2][1][c], mid[2][2][c]}));
+            // exp(-gamma rbar), exp(-gamma rbar_img)
+            double sn, cs;
+            sincos(-sc.gamma.y * rb, &sn, &cs);
+            const double md = exp(-sc.gamma.x * rb);
+            const c64 xd = c64{md * cs, md * sn};
+            sincos(-sc.gamma.y * ri, &sn, &cs);
+            const double mg = exp(-sc.gamma.x * ri);
+            const c64 xi = c64{mg * cs, mg * sn};
+            vl = cadd(cmul(sc.zl_dir, cscale(xd, pl[idx])), cmul(sc.zl_img, cscale(xi, pli[idx])));
+            vt = cadd(cmul(sc.zt_dir, cscale(xd, pt[idx])), cmul(sc.zt_img, cscale(xi, pti[idx])));
+        }
+        tl[a][c] = c64{-vl.x, -vl.y};
+        tt[a][c] = c64{-vt.x, -vt.y};
+    }
+    __syncthreads();
+    for (int p = t; p < kFT * kFT; p += 256) {
+        const int a = p / kFT, c = p % kFT;
+        const long i = i0 + a, k = k0 + c;
+        if (i < m && k < m && k <= i) {
+            W[(zl_off + i) * ldw + (zl_off + k)] = tl[a][c];
+            W[(zt_off + i) * ldw + (zt_off + k)] = tt[a][c];
+        }
+    }
+    for (int p = t; p < kFT * kFT; p += 256) {
+        const int c = p / kFT, a = p % kFT;
+        const long i = i0 + a, k = k0 + c;
+        if (i < m && k < m && k < i) {
+            W[(zl_off + k) * ldw + (zl_off + i)] = tl[a][c];
+            W[(zt_off + k) * ldw + (zt_off + i)] = tt[a][c];
+        }
+    }
+}
+
+int launch_fill_mhem(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
+                     const double* pl, const double* pt, const double* pli, const double* pti,
+                     const PairScalars& sc, hipStream_t st)
+{
+    const long m = L.m;
+    if (m <= 0) return 0;
+    const long T = (m + kFT - 1) / kFT;
+    const long nt = T * (T + 1) / 2;
+    hipLaunchKernelGGL(k_fill_mhem, dim3((unsigned)nt), dim3(256), 0, st, W, ldw, L.zl_off,
+                       L.zt_off, m, e, img, pl, pt, pli, pti, sc);
     TAGS_HIP(hipGetLastError());
     return 0;
 }

@@ -146,7 +146,7 @@ struct PairScalars {
     c64 gamma;
     c64 zl_dir, zt_dir;   // s mu / 4 pi and 1 / (4 pi kappa)
     c64 zl_img, zt_img;   // ref_l s mu / 4 pi and ref_t / (4 pi kappa)
-    int intg_type;        // INTG_DOUBLE or INTG_NONE
+    int intg_type;        // INTG_DOUBLE, INTG_MHEM or INTG_NONE
 };
 
 // Placement of the three unknown blocks of the immittance system (row and
@@ -161,6 +161,11 @@ struct BlockLayout {
 int gl_tables_init();
 int launch_pairs_dropin(double* zl, double* zt, bool alias, SegSoA e, SegSoA img,
                         const long m, const PairScalars& sc, bool images, hipStream_t st);
+int launch_mhem_potentials(double* pl, double* pt, double* pli, double* pti, long m, SegSoA e,
+                           SegSoA img, hipStream_t st);
+int launch_fill_mhem(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
+                     const double* pl, const double* pt, const double* pli, const double* pti,
+                     const PairScalars& sc, hipStream_t st);
 int launch_fill_zblocks(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
                         const PairScalars& sc, hipStream_t st);
 int launch_fill_static(c64* W, long ldw, const BlockLayout& L, const int* seg_start_node,
