@@ -157,6 +157,44 @@ int fill_impedance_adm2(TAGS_COMPLEX *yn, TAGS_COMPLEX *yla, TAGS_COMPLEX *ytb, 
  * or i > 0 when U(i,i) is exactly zero. */
 int solve_admittance(TAGS_COMPLEX *yn, TAGS_COMPLEX *ic, size_t num_nodes);
 
+/* Transient wrap: frequency-dependent soils and the numerical Laplace
+ * transform pair (Gomez & Uribe 2009), used on the s_k = c + j k 2 pi / tmax
+ * grid of laplace_trans. */
+
+/* src/auxiliary.h:101-103. Alipio-Visacro soil: sigma(s) and eps_r(s) with
+ * f = s / (2 pi j) (complex), principal-branch complex powers. Host code. */
+int alipio_soil(TAGS_COMPLEX *sigma, TAGS_COMPLEX *epsr, double sigma0, TAGS_COMPLEX s, double h,
+                double g, double eps_ratio);
+
+/* src/auxiliary.h:119-121. Smith-Longmire soil (13 terms). Host code. */
+int smith_longmire_soil(TAGS_COMPLEX *sigma, TAGS_COMPLEX *epsr, double sigma0, TAGS_COMPLEX s,
+                        double erinf);
+
+/* src/auxiliary.h:320-321. g[k] = sum_j f[j] dt e^{-c j dt} e^{-2 pi i jk/nt},
+ * k < nt/2 + 1, with c = log(nt^2)/tmax, dt = tmax/(nt - 1), and
+ * s[k] = c + j k 2 pi / tmax. The transform runs on the GPU (radix-2 Stockham
+ * passes for a power-of-two nt, the direct sum otherwise); the reference uses
+ * FFTW's r2c plan. Returns 0, or TAGS_ERR_ARG for nt < 2 or tmax <= 0. */
+int laplace_trans(double *f, TAGS_COMPLEX *g, TAGS_COMPLEX *s, double tmax, size_t nt);
+
+/* src/auxiliary.h:329-346 */
+enum INLT_Filter {
+    FILTER_NONE = 0,
+    FILTER_BLACKMAN = 1,
+    FILTER_HANNING = 2,
+    FILTER_LANCZOS = 3,
+    FILTER_RIESZ = 4
+};
+
+/* src/auxiliary.h:369-370. f[k] = e^{c k dt} / (nt dt) times the real inverse
+ * DFT of the Hermitian extension of sigma(w_k) g[k] (the imaginary parts of the
+ * DC and, for even nt, Nyquist terms do not enter, as in a c2r transform);
+ * sigma is the data window `filter` with Omega = Im s[nt/2]. As in the
+ * reference, FILTER_LANCZOS is 0/0 at w = 0, so its output is NaN. GPU
+ * transform as in laplace_trans. */
+int inv_laplace_trans(double *f, TAGS_COMPLEX *g, TAGS_COMPLEX *s, double tmax, size_t nt,
+                      int filter);
+
 /* ---------------------------------------------------------------------------
  * Part 2: device-resident sweep (no reference counterpart)
  * ------------------------------------------------------------------------- */
@@ -268,6 +306,15 @@ enum tags_plan {
     TAGS_PLAN_HYBRID = 7
 };
 int tags_sweep_set_plan(tags_sweep *sw, int plan);
+
+/* Batched laplace_trans / inv_laplace_trans over `howmany` signals (the
+ * examples' fftw_plan_many layouts, e.g. examples/sunjerga173powsys.c:56-90):
+ * signal i at f[i nt ..] and g[i (nt/2 + 1) ..]; s as in the single calls.
+ * No reference counterpart as a function. */
+int tags_laplace_batch(const double *f, TAGS_COMPLEX *g, TAGS_COMPLEX *s, double tmax, size_t nt,
+                       size_t howmany);
+int tags_inv_laplace_batch(double *f, const TAGS_COMPLEX *g, const TAGS_COMPLEX *s, double tmax,
+                           size_t nt, int filter, size_t howmany);
 
 /* Impedance formulation of a sweep's samples.
  * TAGS_FORM_HEM   (default) every sample integrates e^{-gamma r}/r over every

@@ -388,6 +388,79 @@ def mhem_harmonic_impedance(e, nodes, freqs, sigma, epsr, inj_node=0, mur=1.0, r
     return zh
 
 
+# ---------------------------------------------------------------------------
+# transient wrap (src/auxiliary.c:61-93, :240-329)
+# ---------------------------------------------------------------------------
+def alipio_soil(sigma0, s, h, g, eps_ratio):
+    """src/auxiliary.c:61-70 (complex f = s / (2 pi j), principal branches)."""
+    f = complex(s) / (TWO_PI * 1j)
+    sigma = sigma0 + sigma0 * h * (f / 1e6) ** g
+    t = np.tan(PI * g / 2) / (TWO_PI * EPS0 * 1e6 ** g)
+    epsr = eps_ratio + t * sigma0 * h * f ** (g - 1.0)
+    return complex(sigma), complex(epsr)
+
+
+def smith_longmire_soil(sigma0, s, erinf):
+    """src/auxiliary.c:72-93."""
+    a = [3.4e6, 2.74e5, 2.58e4, 3.38e3, 5.26e2, 1.33e2, 2.72e1, 1.25e1, 4.8e0, 2.17e0, 9.8e-1,
+         3.92e-1, 1.73e-1]
+    fdc = complex(125.0 * sigma0) ** 0.8312
+    se = 0j
+    ss = 0j
+    for i, ai in enumerate(a):
+        F = fdc * 10.0 ** i
+        r2 = (complex(s) / (1j * TWO_PI * F)) ** 2.0
+        den = 1.0 + r2
+        se += ai / den
+        ss += ai * F * (r2 / den)
+    return complex(sigma0 + TWO_PI * EPS0 * ss), complex(erinf + se)
+
+
+def laplace_trans(f, tmax):
+    """src/auxiliary.c:240-263 with numpy's real FFT for FFTW's r2c plan:
+    rows of f -> (g, s)."""
+    f = np.asarray(f, np.float64)
+    nt = f.shape[-1]
+    ns = nt // 2 + 1
+    c = np.log(float(nt) ** 2.0) / tmax
+    dt = tmax / (nt - 1)
+    dw = TWO_PI / tmax
+    s = c + 1j * dw * np.arange(ns)
+    k = np.arange(nt)
+    g = np.fft.rfft(f * (dt * np.exp(-c * k * dt)), axis=-1)
+    return g, s
+
+
+def inlt_window(filter, w, omega):
+    """The data windows of src/auxiliary.c:275-305."""
+    with np.errstate(invalid="ignore", divide="ignore"):
+        if filter == 1:
+            a = PI * w / omega
+            return 0.42 + 0.5 * np.cos(a) + 0.08 * np.cos(2 * a)
+        if filter == 2:
+            return (1 + np.cos(PI * w / omega)) / 2
+        if filter == 3:
+            a = PI * w / omega
+            return np.sin(a) / a
+        if filter == 4:
+            return 1.0 - np.abs(w / omega) ** 2.0
+    return np.ones_like(w)
+
+
+def inv_laplace_trans(g, s, tmax, nt, filter=0):
+    """src/auxiliary.c:266-329 with numpy's irfft for FFTW's c2r plan (which is
+    unnormalised: irfft times nt)."""
+    g = np.asarray(g, np.complex128)
+    s = np.asarray(s, np.complex128)
+    ns = nt // 2 + 1
+    dt = tmax / (nt - 1)
+    sig = inlt_window(filter, s.imag, s[ns - 1].imag)
+    out = np.fft.irfft(sig * g, n=nt, axis=-1) * nt
+    c = s[0].real
+    k = np.arange(nt)
+    return out * np.exp(c * k * dt) / (nt * dt)
+
+
 def pair_sample(e, images, pairs, gamma, rel_tol=1e-6, threads=1):
     e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
     images = np.ascontiguousarray(images, dtype=ELECTRODE_DTYPE)

@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libtags_hip.so")
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "api.hip"]
+SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "api.hip", "nlt.hip"]
 
 ELECTRODE_DTYPE = np.dtype(
     [("start_point", "<f8", 3), ("end_point", "<f8", 3), ("middle_point", "<f8", 3),
@@ -113,7 +113,9 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_sweep_assemble", "tags_immittance_solve", "tags_sweep_set_inflight",
             "tags_gemm_run", "tags_set_step_width", "tags_sweep_set_nodal_above",
             "fill_incidence_adm", "fill_impedance_adm", "calculate_yla_ytb",
-            "fill_impedance_adm2", "solve_admittance", "tags_sweep_set_formulation"]
+            "fill_impedance_adm2", "solve_admittance", "tags_sweep_set_formulation",
+            "alipio_soil", "smith_longmire_soil", "laplace_trans", "inv_laplace_trans",
+            "tags_laplace_batch", "tags_inv_laplace_batch"]
 
 _lib = None
 
@@ -148,6 +150,12 @@ def lib():
         "fill_impedance_adm2": ([P, P, P, P, P, S, S], I),
         "solve_admittance": ([P, P, S], I),
         "tags_sweep_set_formulation": ([P, I], I),
+        "alipio_soil": ([P, P, D, C, D, D, D], I),
+        "smith_longmire_soil": ([P, P, D, C, D], I),
+        "laplace_trans": ([P, P, P, D, S], I),
+        "inv_laplace_trans": ([P, P, P, D, S, I], I),
+        "tags_laplace_batch": ([P, P, P, D, S, S], I),
+        "tags_inv_laplace_batch": ([P, P, P, D, S, I, S], I),
         "tags_sweep_create": ([ctypes.POINTER(P), P, P, S, P, S], I),
         "tags_sweep_solve": ([P, S, P, P, D, P, P, P, S, I, P, P, P], I),
         "tags_sweep_set_profiling": ([P, I], I),
@@ -307,6 +315,50 @@ def fill_impedance_adm2(yn, yla, ytb, a, b, num_electrodes, num_nodes):
 def solve_admittance(yn, ic, num_nodes):
     """ic (complex128, n) is overwritten by the node potentials; returns info."""
     return lib().solve_admittance(_p(yn), _p(ic), num_nodes)
+
+
+FILTER_NONE, FILTER_BLACKMAN, FILTER_HANNING, FILTER_LANCZOS, FILTER_RIESZ = 0, 1, 2, 3, 4
+
+
+def alipio_soil(sigma0, s, h, g, eps_ratio):
+    """(sigma(s), eps_r(s)) of the Alipio-Visacro soil model."""
+    sg, er = CDouble(), CDouble()
+    lib().alipio_soil(ctypes.byref(sg), ctypes.byref(er), float(sigma0), CDouble.of(s),
+                      float(h), float(g), float(eps_ratio))
+    return complex(sg.re, sg.im), complex(er.re, er.im)
+
+
+def smith_longmire_soil(sigma0, s, erinf):
+    sg, er = CDouble(), CDouble()
+    lib().smith_longmire_soil(ctypes.byref(sg), ctypes.byref(er), float(sigma0), CDouble.of(s),
+                              float(erinf))
+    return complex(sg.re, sg.im), complex(er.re, er.im)
+
+
+def laplace_trans(f, tmax):
+    """Numerical Laplace transform of the rows of f (nt samples on [0, tmax]):
+    returns (g, s), g of shape (..., nt // 2 + 1)."""
+    f = np.ascontiguousarray(f, np.float64)
+    nt = f.shape[-1]
+    ns = nt // 2 + 1
+    batch = int(np.prod(f.shape[:-1], dtype=np.int64)) if f.ndim > 1 else 1
+    g = np.zeros(f.shape[:-1] + (ns,), np.complex128)
+    s = np.zeros(ns, np.complex128)
+    rc = lib().tags_laplace_batch(_p(f), _p(g), _p(s), float(tmax), nt, batch)
+    if rc:
+        raise RuntimeError("tags_laplace_batch failed: %d" % rc)
+    return g, s
+
+
+def inv_laplace_trans(g, s, tmax, nt, filter=FILTER_NONE):
+    g = np.ascontiguousarray(g, np.complex128)
+    s = np.ascontiguousarray(s, np.complex128)
+    batch = int(np.prod(g.shape[:-1], dtype=np.int64)) if g.ndim > 1 else 1
+    f = np.zeros(g.shape[:-1] + (nt,), np.float64)
+    rc = lib().tags_inv_laplace_batch(_p(f), _p(g), _p(s), float(tmax), nt, int(filter), batch)
+    if rc:
+        raise RuntimeError("tags_inv_laplace_batch failed: %d" % rc)
+    return f
 
 
 def dense_solve(a, b, precision=PREC_FP64):
