@@ -157,6 +157,18 @@ int fill_impedance_adm2(TAGS_COMPLEX *yn, TAGS_COMPLEX *yla, TAGS_COMPLEX *ytb, 
  * or i > 0 when U(i,i) is exactly zero. */
 int solve_admittance(TAGS_COMPLEX *yn, TAGS_COMPLEX *ic, size_t num_nodes);
 
+/* Case files (host code). */
+
+/* src/electrode.h:127-128 (src/electrode.c:45-80). Reads num_electrodes lines
+ * "x0, y0, z0, x1, y1, z1, radius" and populates the electrodes. Returns 0, -10
+ * when the file cannot be opened, fscanf's count - 7 at the first short line, or
+ * populate_electrode's code for a bad record; messages as the reference. */
+int electrodes_file(const char file_name[], Electrode *electrodes, size_t num_electrodes);
+
+/* src/electrode.h:140-141 (src/electrode.c:82-101). Reads num_nodes lines
+ * "x, y, z". Returns 0, -10, or fscanf's count - 3 at the first short line. */
+int nodes_file(const char file_name[], double *nodes, size_t num_nodes);
+
 /* Transient wrap: frequency-dependent soils and the numerical Laplace
  * transform pair (Gomez & Uribe 2009), used on the s_k = c + j k 2 pi / tmax
  * grid of laplace_trans. */
@@ -306,6 +318,15 @@ enum tags_plan {
     TAGS_PLAN_HYBRID = 7
 };
 int tags_sweep_set_plan(tags_sweep *sw, int plan);
+
+/* The examples' case-file helpers (examples/harmonic_impedance.c:178-257), no
+ * library counterpart in the reference: the line count the examples size
+ * their arrays with (newlines + 1; -1 if the file cannot be opened), a list of
+ * n frequencies (one "%lf" each; returns the number not read, or -10), and a
+ * result file with one "%e%+eim" line per value (0, or -10). */
+long tags_count_lines(const char file_name[]);
+int tags_frequencies_file(const char file_name[], double *freq, size_t n);
+int tags_write_complex_file(const char file_name[], const TAGS_COMPLEX *z, size_t n);
 
 /* Batched laplace_trans / inv_laplace_trans over `howmany` signals (the
  * examples' fftw_plan_many layouts, e.g. examples/sunjerga173powsys.c:56-90):

@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libtags_hip.so")
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "api.hip", "nlt.hip"]
+SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "api.hip", "nlt.hip", "io.hip"]
 
 ELECTRODE_DTYPE = np.dtype(
     [("start_point", "<f8", 3), ("end_point", "<f8", 3), ("middle_point", "<f8", 3),
@@ -115,7 +115,8 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "fill_incidence_adm", "fill_impedance_adm", "calculate_yla_ytb",
             "fill_impedance_adm2", "solve_admittance", "tags_sweep_set_formulation",
             "alipio_soil", "smith_longmire_soil", "laplace_trans", "inv_laplace_trans",
-            "tags_laplace_batch", "tags_inv_laplace_batch"]
+            "tags_laplace_batch", "tags_inv_laplace_batch", "electrodes_file", "nodes_file",
+            "tags_count_lines", "tags_frequencies_file", "tags_write_complex_file"]
 
 _lib = None
 
@@ -156,6 +157,11 @@ def lib():
         "inv_laplace_trans": ([P, P, P, D, S, I], I),
         "tags_laplace_batch": ([P, P, P, D, S, S], I),
         "tags_inv_laplace_batch": ([P, P, P, D, S, I, S], I),
+        "electrodes_file": ([ctypes.c_char_p, P, S], I),
+        "nodes_file": ([ctypes.c_char_p, P, S], I),
+        "tags_count_lines": ([ctypes.c_char_p], ctypes.c_long),
+        "tags_frequencies_file": ([ctypes.c_char_p, P, S], I),
+        "tags_write_complex_file": ([ctypes.c_char_p, P, S], I),
         "tags_sweep_create": ([ctypes.POINTER(P), P, P, S, P, S], I),
         "tags_sweep_solve": ([P, S, P, P, D, P, P, P, S, I, P, P, P], I),
         "tags_sweep_set_profiling": ([P, I], I),
@@ -315,6 +321,22 @@ def fill_impedance_adm2(yn, yla, ytb, a, b, num_electrodes, num_nodes):
 def solve_admittance(yn, ic, num_nodes):
     """ic (complex128, n) is overwritten by the node potentials; returns info."""
     return lib().solve_admittance(_p(yn), _p(ic), num_nodes)
+
+
+def read_case_files(electrodes_csv, nodes_csv):
+    """Electrodes and nodes from the reference's case files, sized as the
+    examples size them (line counts). Returns (rc, electrodes, nodes)."""
+    L = lib()
+    ne = L.tags_count_lines(os.fsencode(electrodes_csv))
+    nn = L.tags_count_lines(os.fsencode(nodes_csv))
+    if ne < 0 or nn < 0:
+        return -10, None, None
+    e = np.zeros(ne, ELECTRODE_DTYPE)
+    nodes = np.zeros((nn, 3))
+    rc = L.electrodes_file(os.fsencode(electrodes_csv), _p(e), ne)
+    if rc == 0:
+        rc = L.nodes_file(os.fsencode(nodes_csv), _p(nodes), nn)
+    return rc, e, nodes
 
 
 FILTER_NONE, FILTER_BLACKMAN, FILTER_HANNING, FILTER_LANCZOS, FILTER_RIESZ = 0, 1, 2, 3, 4
