@@ -157,6 +157,45 @@ int fill_impedance_adm2(TAGS_COMPLEX *yn, TAGS_COMPLEX *yla, TAGS_COMPLEX *ytb, 
  * or i > 0 when U(i,i) is exactly zero. */
 int solve_admittance(TAGS_COMPLEX *yn, TAGS_COMPLEX *ic, size_t num_nodes);
 
+/* Potentials, fields and voltages of given segment currents (src/electrode.c:
+ * 433-668). The line integrals over each segment run on the GPU with a
+ * composite 16-point Gauss-Legendre rule graded towards the point's closest
+ * axis parameter (relative error ~1e-14 for points off the segment axes);
+ * max_eval, req_abs_error and req_rel_error are accepted. il, it: the m
+ * longitudinal / transversal segment currents. */
+
+/* src/electrode.h:333-337. sum_m it[m] int e^{-gamma r}/r dl / L_m / (4 pi kappa)
+ * (INTG_SINGLE from each segment axis to `point`). On an error the error code
+ * is returned as the (real) value, as in the reference. */
+TAGS_COMPLEX electric_potential(const double *point, const Electrode *electrodes,
+                                size_t num_electrodes, const TAGS_COMPLEX *it, TAGS_COMPLEX gamma,
+                                TAGS_COMPLEX kappa, size_t max_eval, double req_abs_error,
+                                double req_rel_error);
+
+/* src/electrode.h:355-360. va (3 complex) = mu_r mu0 / 4 pi sum_m il[m]
+ * int e^{-gamma r}/r dl / L_m (end - start)_m. Returns 0. */
+int magnetic_potential(const double *point, const Electrode *electrodes, size_t num_electrodes,
+                       const TAGS_COMPLEX *il, TAGS_COMPLEX gamma, double mur, size_t max_eval,
+                       double req_abs_error, double req_rel_error, TAGS_COMPLEX *va);
+
+/* src/electrode.h:398-404. E = -grad u - s A at `point`: for |s| > DBL_EPSILON
+ * ve is overwritten with -s A and the conservative part added; otherwise the
+ * conservative part is added to what ve holds (as in the reference). */
+int electric_field(const double *point, const Electrode *electrodes, size_t num_electrodes,
+                   const TAGS_COMPLEX *il, const TAGS_COMPLEX *it, TAGS_COMPLEX gamma,
+                   TAGS_COMPLEX s, double mur, TAGS_COMPLEX kappa, size_t max_eval,
+                   double req_abs_error, double req_rel_error, TAGS_COMPLEX *ve);
+
+/* src/electrode.h:440-445. The unit-interval integral of E(p(t)) . u along
+ * p(t) = point1 + t (point2 - point1), u the unit direction: the voltage
+ * divided by the path length, as the reference computes it. Composite
+ * Gauss-Legendre along the path, panels doubled until two results agree to
+ * 1e-13. */
+TAGS_COMPLEX voltage(const double *point1, const double *point2, const Electrode *electrodes,
+                     size_t num_electrodes, const TAGS_COMPLEX *il, const TAGS_COMPLEX *it,
+                     TAGS_COMPLEX gamma, TAGS_COMPLEX s, double mur, TAGS_COMPLEX kappa,
+                     size_t max_eval, double req_abs_error, double req_rel_error);
+
 /* Case files (host code). */
 
 /* src/electrode.h:127-128 (src/electrode.c:45-80). Reads num_electrodes lines
@@ -318,6 +357,26 @@ enum tags_plan {
     TAGS_PLAN_HYBRID = 7
 };
 int tags_sweep_set_plan(tags_sweep *sw, int plan);
+
+/* Batched fields, no reference counterpart as a function: for every point p
+ * and right-hand side k (currents il[k m ..], it[k m ..]; either may be NULL for
+ * zero), pot[p nrhs + k] (electric_potential), va[3 (p nrhs + k) ..]
+ * (magnetic_potential) and ve[3 (p nrhs + k) ..] = -grad u - s A (the
+ * electric_field of a zeroed ve, -s A only when |s| > DBL_EPSILON); outputs may
+ * be NULL. */
+int tags_fields_batch(const double *points, size_t num_points, const Electrode *electrodes,
+                      size_t num_electrodes, const TAGS_COMPLEX *il, const TAGS_COMPLEX *it,
+                      size_t nrhs, TAGS_COMPLEX gamma, TAGS_COMPLEX s, double mur,
+                      TAGS_COMPLEX kappa, TAGS_COMPLEX *pot, TAGS_COMPLEX *va, TAGS_COMPLEX *ve);
+
+/* The mHEM ground potential of the examples (examples/visacro57emc01.c:406-432):
+ * out[p nrhs + k] = sum_m it[k m + m'] / (4 pi kappa) e^{-gamma rbar}
+ * log((r1 + r2 + L)/(r1 + r2 - L)) / L, rbar the distance to the segment's
+ * middle point, r1, r2 to its ends (the caller applies the (1 + ref_t) image
+ * factor to it, as the examples do). */
+int tags_mhem_gpd(const double *points, size_t num_points, const Electrode *electrodes,
+                  size_t num_electrodes, const TAGS_COMPLEX *it, size_t nrhs, TAGS_COMPLEX gamma,
+                  TAGS_COMPLEX kappa, TAGS_COMPLEX *out);
 
 /* The examples' case-file helpers (examples/harmonic_impedance.c:178-257), no
  * library counterpart in the reference: the line count the examples size

@@ -78,6 +78,14 @@ def lib():
         L.or_integral.restype = I
         L.or_gk15_rule.argtypes = [P, P, P]
         L.or_gk15_rule.restype = None
+        L.or_electric_potential.argtypes = [P, P, S, P, P, P, S, D, D, P]
+        L.or_electric_potential.restype = None
+        L.or_magnetic_potential.argtypes = [P, P, S, P, P, D, S, D, D, P]
+        L.or_magnetic_potential.restype = None
+        L.or_electric_field.argtypes = [P, P, S, P, P, P, P, D, P, S, D, D, P]
+        L.or_electric_field.restype = I
+        L.or_voltage.argtypes = [P, P, P, S, P, P, P, P, D, P, S, D, D, P]
+        L.or_voltage.restype = None
         L.or_populate_electrode.argtypes = [P, P, P, D]
         L.or_populate_electrode.restype = I
         L.or_segment_electrode.argtypes = [P, P, S, P, P, D]
@@ -459,6 +467,71 @@ def inv_laplace_trans(g, s, tmax, nt, filter=0):
     c = s[0].real
     k = np.arange(nt)
     return out * np.exp(c * k * dt) / (nt * dt)
+
+
+# ---------------------------------------------------------------------------
+# potentials, fields, voltages (src/electrode.c:433-668)
+# ---------------------------------------------------------------------------
+def _cur(c, m):
+    return np.ascontiguousarray(np.zeros(m) if c is None else c, np.complex128)
+
+
+def electric_potential(point, e, it, gamma, kappa, rel_tol=1e-10, abs_tol=0.0, max_eval=0):
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    out = np.zeros(2)
+    lib().or_electric_potential(_p(np.ascontiguousarray(point, np.float64)), _p(e), len(e),
+                                _p(_cur(it, len(e))), _p(_c2(gamma)), _p(_c2(kappa)), max_eval,
+                                abs_tol, rel_tol, _p(out))
+    return complex(out[0], out[1])
+
+
+def magnetic_potential(point, e, il, gamma, mur=1.0, rel_tol=1e-10, abs_tol=0.0, max_eval=0):
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    va = np.zeros(6)
+    lib().or_magnetic_potential(_p(np.ascontiguousarray(point, np.float64)), _p(e), len(e),
+                                _p(_cur(il, len(e))), _p(_c2(gamma)), float(mur), max_eval,
+                                abs_tol, rel_tol, _p(va))
+    return va[0::2] + 1j * va[1::2]
+
+
+def electric_field(point, e, il, it, gamma, s, kappa, mur=1.0, rel_tol=1e-10, abs_tol=0.0,
+                   max_eval=0):
+    """ve of a zeroed input vector (src/electrode.c:517-558)."""
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    ve = np.zeros(6)
+    lib().or_electric_field(_p(np.ascontiguousarray(point, np.float64)), _p(e), len(e),
+                            _p(_cur(il, len(e))), _p(_cur(it, len(e))), _p(_c2(gamma)),
+                            _p(_c2(s)), float(mur), _p(_c2(kappa)), max_eval, abs_tol, rel_tol,
+                            _p(ve))
+    return ve[0::2] + 1j * ve[1::2]
+
+
+def voltage(p1, p2, e, il, it, gamma, s, kappa, mur=1.0, rel_tol=1e-10, abs_tol=0.0,
+            max_eval=0):
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    out = np.zeros(2)
+    lib().or_voltage(_p(np.ascontiguousarray(p1, np.float64)),
+                     _p(np.ascontiguousarray(p2, np.float64)), _p(e), len(e),
+                     _p(_cur(il, len(e))), _p(_cur(it, len(e))), _p(_c2(gamma)), _p(_c2(s)),
+                     float(mur), _p(_c2(kappa)), max_eval, abs_tol, rel_tol, _p(out))
+    return complex(out[0], out[1])
+
+
+def mhem_gpd(points, e, it, gamma, kappa):
+    """examples/visacro57emc01.c:418-430 for one right-hand side: sum_m
+    one_4pik it[m] e^{-gamma rbar} log(r0) / L_m at each point."""
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    pts = np.asarray(points, np.float64).reshape(-1, 3)
+    one_4pik = 1.0 / (FOUR_PI * kappa)
+    out = np.zeros(len(pts), np.complex128)
+    for p, P in enumerate(pts):
+        rbar = np.sqrt(((P - e["middle_point"]) ** 2).sum(1))
+        r1 = np.sqrt(((P - e["start_point"]) ** 2).sum(1))
+        r2 = np.sqrt(((P - e["end_point"]) ** 2).sum(1))
+        r0 = (r1 + r2 + e["length"]) / (r1 + r2 - e["length"])
+        out[p] = np.sum(one_4pik * np.asarray(it) * np.exp(-gamma * rbar) * np.log(r0) /
+                        e["length"])
+    return out
 
 
 def pair_sample(e, images, pairs, gamma, rel_tol=1e-6, threads=1):

@@ -814,3 +814,299 @@ int or_pair_sample(const or_electrode *e, const or_electrode *images, const long
     }
     return 0;
 }
+
+/* ------------------------------------------------------------------------- */
+/* Fields and voltages (src/electrode.c:177-216 integrand_single,             */
+/* :433-668). INTG_SINGLE follows the reference's call, hcubature over the    */
+/* unit square of an integrand of t[0] only (src/electrode.c:279-284), here   */
+/* or_hcubature2; the field integrand is a 6-component 1-D hcubature with     */
+/* ERROR_PAIRED (:538-546), here the multi-component form of or_quad1.        */
+/* ------------------------------------------------------------------------- */
+typedef void (*or_integrandn)(double t, void *data, double *out);
+
+/* GK15 on [a, b] for fdim components: values and QUADPACK error per component */
+static void gk15n(or_integrandn f, void *data, unsigned fdim, double a, double b, double *val,
+                  double *err)
+{
+    const double c = 0.5 * (a + b), h = 0.5 * (b - a);
+    double fv[15][8];
+    double xs[15];
+    xs[7] = c;
+    for (int j = 0; j < 7; j++) {
+        xs[j] = c - h * gk_x[j];
+        xs[14 - j] = c + h * gk_x[j];
+    }
+    for (int q = 0; q < 15; q++) f(xs[q], data, fv[q]);
+    for (unsigned d = 0; d < fdim; d++) {
+        double f1[7], f2[7];
+        for (int j = 0; j < 7; j++) {
+            f1[j] = fv[j][d];
+            f2[j] = fv[14 - j][d];
+        }
+        const double fc = fv[7][d];
+        double rg = fc * gk_wg[3], rk = fc * gk_wk[7], ra = fabs(rk);
+        for (int j = 0; j < 3; j++) {
+            const int jj = 2 * j + 1;
+            rg += gk_wg[j] * (f1[jj] + f2[jj]);
+        }
+        for (int j = 0; j < 7; j++) {
+            rk += gk_wk[j] * (f1[j] + f2[j]);
+            ra += gk_wk[j] * (fabs(f1[j]) + fabs(f2[j]));
+        }
+        const double mean = 0.5 * rk;
+        double asc = gk_wk[7] * fabs(fc - mean);
+        for (int j = 0; j < 7; j++) asc += gk_wk[j] * (fabs(f1[j] - mean) + fabs(f2[j] - mean));
+        double e = fabs((rk - rg) * h);
+        asc *= fabs(h);
+        ra *= fabs(h);
+        if (asc != 0.0 && e != 0.0) {
+            const double q = pow(200.0 * e / asc, 1.5);
+            e = asc * (q < 1.0 ? q : 1.0);
+        }
+        if (ra > DBL_MIN / (50.0 * DBL_EPSILON)) {
+            const double emin = 50.0 * DBL_EPSILON * ra;
+            if (emin > e) e = emin;
+        }
+        val[d] = rk * h;
+        err[d] = e;
+    }
+}
+
+/* ERROR_PAIRED convergence: every (re, im) pair's error norm within tolerance */
+static int paired_ok(unsigned fdim, const double *v, const double *e, double abs_tol,
+                     double rel_tol)
+{
+    for (unsigned j = 0; j + 1 < fdim; j += 2) {
+        const double en = hypot(e[j], e[j + 1]), vn = hypot(v[j], v[j + 1]);
+        if (en > abs_tol && en > rel_tol * vn) return 0;
+    }
+    if (fdim % 2) {
+        const double en = e[fdim - 1], vn = fabs(v[fdim - 1]);
+        if (en > abs_tol && en > rel_tol * vn) return 0;
+    }
+    return 1;
+}
+
+/* adaptive 1-D integration of fdim <= 8 components over [0, 1]: the region
+ * with the largest component error is bisected until paired_ok */
+int or_quadn(or_integrandn f, void *data, unsigned fdim, size_t max_eval, double abs_tol,
+             double rel_tol, double *val, double *err)
+{
+    size_t cap = 64, n = 1, ne = 15;
+    double *ra = malloc(cap * sizeof(double)), *rb = malloc(cap * sizeof(double));
+    double *rv = malloc(cap * 8 * sizeof(double)), *re = malloc(cap * 8 * sizeof(double));
+    ra[0] = 0.0;
+    rb[0] = 1.0;
+    gk15n(f, data, fdim, 0.0, 1.0, rv, re);
+    for (;;) {
+        double tv[8] = {0}, te[8] = {0}, worst_e = -1.0;
+        size_t worst = 0;
+        for (size_t i = 0; i < n; i++) {
+            double emax = 0.0;
+            for (unsigned d = 0; d < fdim; d++) {
+                tv[d] += rv[8 * i + d];
+                te[d] += re[8 * i + d];
+                if (re[8 * i + d] > emax) emax = re[8 * i + d];
+            }
+            if (emax > worst_e) {
+                worst_e = emax;
+                worst = i;
+            }
+        }
+        memcpy(val, tv, fdim * sizeof(double));
+        memcpy(err, te, fdim * sizeof(double));
+        if (paired_ok(fdim, tv, te, abs_tol, rel_tol)) break;
+        if (max_eval && ne + 30 > max_eval) break;
+        if (n >= ((size_t)1 << 14)) break; /* region guard */
+        if (n == cap) {
+            cap *= 2;
+            ra = realloc(ra, cap * sizeof(double));
+            rb = realloc(rb, cap * sizeof(double));
+            rv = realloc(rv, cap * 8 * sizeof(double));
+            re = realloc(re, cap * 8 * sizeof(double));
+        }
+        const double a = ra[worst], b = rb[worst], mid = 0.5 * (a + b);
+        rb[worst] = mid;
+        gk15n(f, data, fdim, a, mid, rv + 8 * worst, re + 8 * worst);
+        ra[n] = mid;
+        rb[n] = b;
+        gk15n(f, data, fdim, mid, b, rv + 8 * n, re + 8 * n);
+        n++;
+        ne += 30;
+    }
+    free(ra);
+    free(rb);
+    free(rv);
+    free(re);
+    return 0;
+}
+
+typedef struct {
+    const or_electrode *e;
+    const double *point;
+    double g[2];
+} or_fdata;
+
+/* src/electrode.c:199-216: e^{-gamma r}/r from the sender axis point at t[0]
+ * to the receiver's middle point (here: the field point), r >= DBL_EPSILON */
+static void or_single_cb(const double t[2], void *data, double out[2])
+{
+    const or_fdata *p = (const or_fdata *)data;
+    double ps[3];
+    for (int i = 0; i < 3; i++)
+        ps[i] = t[0] * (p->e->end_point[i] - p->e->start_point[i]) + p->e->start_point[i];
+    double r = or_vector_length(ps, p->point);
+    if (r < DBL_EPSILON) r = DBL_EPSILON;
+    const double mag = exp(-p->g[0] * r) / r, ph = -p->g[1] * r;
+    out[0] = mag * cos(ph);
+    out[1] = mag * sin(ph);
+}
+
+/* INTG_SINGLE (src/electrode.c:279-284): L_s times the unit-square integral */
+static void or_single(const or_electrode *e, const double *point, const double g[2],
+                      size_t max_eval, double abs_tol, double rel_tol, double res[2])
+{
+    or_fdata d = {e, point, {g[0], g[1]}};
+    double err[2];
+    or_hcubature2(or_single_cb, &d, max_eval, abs_tol, rel_tol, res, err, NULL);
+    res[0] *= e->length;
+    res[1] *= e->length;
+}
+
+/* src/electrode.c:435-456: sum_m I_single(m) it[m] / L_m / (4 pi kappa) */
+void or_electric_potential(const double *point, const or_electrode *e, size_t m,
+                           const double *it, const double g[2], const double kappa[2],
+                           size_t max_eval, double abs_tol, double rel_tol, double out[2])
+{
+    double pr = 0.0, pi = 0.0, res[2];
+    for (size_t k = 0; k < m; k++) {
+        or_single(&e[k], point, g, max_eval, abs_tol, rel_tol, res);
+        const double ar = res[0] / e[k].length, ai = res[1] / e[k].length;
+        pr += ar * it[2 * k] - ai * it[2 * k + 1];
+        pi += ar * it[2 * k + 1] + ai * it[2 * k];
+    }
+    const double dr = OR_FOUR_PI * kappa[0], di = OR_FOUR_PI * kappa[1], den = dr * dr + di * di;
+    out[0] = (pr * dr + pi * di) / den;
+    out[1] = (pi * dr - pr * di) / den;
+}
+
+/* src/electrode.c:458-490: va = mu/4pi sum_m il[m] I_single(m)/L_m (end - start),
+ * electrodes with |il| < DBL_EPSILON skipped */
+void or_magnetic_potential(const double *point, const or_electrode *e, size_t m,
+                           const double *il, const double g[2], double mur, size_t max_eval,
+                           double abs_tol, double rel_tol, double va[6])
+{
+    for (int i = 0; i < 6; i++) va[i] = 0.0;
+    double res[2];
+    for (size_t k = 0; k < m; k++) {
+        if (hypot(il[2 * k], il[2 * k + 1]) < DBL_EPSILON) continue;
+        or_single(&e[k], point, g, max_eval, abs_tol, rel_tol, res);
+        const double ar = res[0] / e[k].length, ai = res[1] / e[k].length;
+        const double pr = ar * il[2 * k] - ai * il[2 * k + 1];
+        const double pi = ar * il[2 * k + 1] + ai * il[2 * k];
+        for (int i = 0; i < 3; i++) {
+            const double dx = e[k].end_point[i] - e[k].start_point[i];
+            va[2 * i] += pr * dx;
+            va[2 * i + 1] += pi * dx;
+        }
+    }
+    const double u = mur * OR_MU0 / OR_FOUR_PI;
+    for (int i = 0; i < 6; i++) va[i] *= u;
+}
+
+/* src/electrode.c:492-515 */
+static void or_efield_cb(double t, void *data, double *out)
+{
+    const or_fdata *p = (const or_fdata *)data;
+    double dx[3], r2 = 0.0;
+    for (int i = 0; i < 3; i++) {
+        const double q = t * (p->e->end_point[i] - p->e->start_point[i]) + p->e->start_point[i];
+        dx[i] = p->point[i] - q;
+        r2 += dx[i] * dx[i];
+    }
+    const double r1 = sqrt(r2);
+    if (r2 < DBL_EPSILON) r2 = DBL_EPSILON;
+    /* (1 + gamma r) e^{-gamma r} / r^2 */
+    const double er = exp(-p->g[0] * r1), ph = -p->g[1] * r1;
+    const double xr = er * cos(ph), xi = er * sin(ph);
+    const double ar = 1.0 + p->g[0] * r1, ai = p->g[1] * r1;
+    const double fr = (ar * xr - ai * xi) / r2, fi = (ar * xi + ai * xr) / r2;
+    for (int i = 0; i < 3; i++) {
+        out[2 * i] = fr * dx[i] / r1;
+        out[2 * i + 1] = fi * dx[i] / r1;
+    }
+}
+
+/* src/electrode.c:517-558: with |s| > DBL_EPSILON ve = -s A (overwritten);
+ * then ve += sum_m it[m]/(4 pi kappa) times the 6-component field integral */
+int or_electric_field(const double *point, const or_electrode *e, size_t m, const double *il,
+                      const double *it, const double g[2], const double s[2], double mur,
+                      const double kappa[2], size_t max_eval, double abs_tol, double rel_tol,
+                      double ve[6])
+{
+    if (hypot(s[0], s[1]) > DBL_EPSILON) {
+        or_magnetic_potential(point, e, m, il, g, mur, max_eval, abs_tol, rel_tol, ve);
+        for (int i = 0; i < 3; i++) {
+            const double vr = ve[2 * i], vi = ve[2 * i + 1];
+            ve[2 * i] = -(s[0] * vr - s[1] * vi);
+            ve[2 * i + 1] = -(s[0] * vi + s[1] * vr);
+        }
+    }
+    const double dr = OR_FOUR_PI * kappa[0], di = OR_FOUR_PI * kappa[1], den = dr * dr + di * di;
+    double res[6], err[6];
+    for (size_t k = 0; k < m; k++) {
+        if (hypot(it[2 * k], it[2 * k + 1]) < DBL_EPSILON) continue;
+        or_fdata d = {&e[k], point, {g[0], g[1]}};
+        or_quadn(or_efield_cb, &d, 6, max_eval, abs_tol, rel_tol, res, err);
+        /* it / (4 pi kappa) */
+        const double cr = (it[2 * k] * dr + it[2 * k + 1] * di) / den;
+        const double ci = (it[2 * k + 1] * dr - it[2 * k] * di) / den;
+        for (int i = 0; i < 3; i++) {
+            ve[2 * i] += res[2 * i] * cr - res[2 * i + 1] * ci;
+            ve[2 * i + 1] += res[2 * i] * ci + res[2 * i + 1] * cr;
+        }
+    }
+    return 0;
+}
+
+typedef struct {
+    const double *p1, *p2;
+    const or_electrode *e;
+    size_t m;
+    const double *il, *it;
+    double g[2], s[2], kappa[2], mur;
+    size_t max_eval;
+    double abs_tol, rel_tol;
+} or_vdata;
+
+/* src/electrode.c:594-624 (ve starts at zero here) */
+static void or_voltage_cb(double t, void *data, double *out)
+{
+    const or_vdata *p = (const or_vdata *)data;
+    double pt[3], dx[3], len = 0.0;
+    for (int i = 0; i < 3; i++) {
+        dx[i] = p->p2[i] - p->p1[i];
+        pt[i] = t * dx[i] + p->p1[i];
+        len += dx[i] * dx[i];
+    }
+    len = sqrt(len);
+    double ve[6] = {0, 0, 0, 0, 0, 0};
+    or_electric_field(pt, p->e, p->m, p->il, p->it, p->g, p->s, p->mur, p->kappa, p->max_eval,
+                      p->abs_tol, p->rel_tol, ve);
+    out[0] = (ve[0] * dx[0] + ve[2] * dx[1] + ve[4] * dx[2]) / len;
+    out[1] = (ve[1] * dx[0] + ve[3] * dx[1] + ve[5] * dx[2]) / len;
+}
+
+/* src/electrode.c:626-660: the unit-interval integral of E . (p2 - p1)/|p2 - p1|
+ * along the straight path (not multiplied by the path length, as in the
+ * reference) */
+void or_voltage(const double *p1, const double *p2, const or_electrode *e, size_t m,
+                const double *il, const double *it, const double g[2], const double s[2],
+                double mur, const double kappa[2], size_t max_eval, double abs_tol,
+                double rel_tol, double out[2])
+{
+    or_vdata d = {p1, p2, e, m, il, it, {g[0], g[1]}, {s[0], s[1]}, {kappa[0], kappa[1]}, mur,
+                  max_eval, abs_tol, rel_tol};
+    double err[2];
+    or_quadn(or_voltage_cb, &d, 2, max_eval, abs_tol, rel_tol, out, err);
+}

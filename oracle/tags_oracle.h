@@ -101,4 +101,20 @@ int or_pair_sample(const or_electrode *e, const or_electrode *images, const long
 #ifdef __cplusplus
 }
 #endif
+
+void or_electric_potential(const double *point, const or_electrode *e, size_t m,
+                           const double *it, const double g[2], const double kappa[2],
+                           size_t max_eval, double abs_tol, double rel_tol, double out[2]);
+void or_magnetic_potential(const double *point, const or_electrode *e, size_t m,
+                           const double *il, const double g[2], double mur, size_t max_eval,
+                           double abs_tol, double rel_tol, double va[6]);
+int or_electric_field(const double *point, const or_electrode *e, size_t m, const double *il,
+                      const double *it, const double g[2], const double s[2], double mur,
+                      const double kappa[2], size_t max_eval, double abs_tol, double rel_tol,
+                      double ve[6]);
+void or_voltage(const double *p1, const double *p2, const or_electrode *e, size_t m,
+                const double *il, const double *it, const double g[2], const double s[2],
+                double mur, const double kappa[2], size_t max_eval, double abs_tol,
+                double rel_tol, double out[2]);
+
 #endif

@@ -24,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libtags_hip.so")
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "api.hip", "nlt.hip", "io.hip"]
+SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "api.hip", "nlt.hip", "io.hip", "field.hip"]
 
 ELECTRODE_DTYPE = np.dtype(
     [("start_point", "<f8", 3), ("end_point", "<f8", 3), ("middle_point", "<f8", 3),
@@ -116,7 +116,9 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "fill_impedance_adm2", "solve_admittance", "tags_sweep_set_formulation",
             "alipio_soil", "smith_longmire_soil", "laplace_trans", "inv_laplace_trans",
             "tags_laplace_batch", "tags_inv_laplace_batch", "electrodes_file", "nodes_file",
-            "tags_count_lines", "tags_frequencies_file", "tags_write_complex_file"]
+            "tags_count_lines", "tags_frequencies_file", "tags_write_complex_file",
+            "electric_potential", "magnetic_potential", "electric_field", "voltage",
+            "tags_fields_batch", "tags_mhem_gpd"]
 
 _lib = None
 
@@ -162,6 +164,12 @@ def lib():
         "tags_count_lines": ([ctypes.c_char_p], ctypes.c_long),
         "tags_frequencies_file": ([ctypes.c_char_p, P, S], I),
         "tags_write_complex_file": ([ctypes.c_char_p, P, S], I),
+        "electric_potential": ([P, P, S, P, C, C, S, D, D], C),
+        "magnetic_potential": ([P, P, S, P, C, D, S, D, D, P], I),
+        "electric_field": ([P, P, S, P, P, C, C, D, C, S, D, D, P], I),
+        "voltage": ([P, P, P, S, P, P, C, C, D, C, S, D, D], C),
+        "tags_fields_batch": ([P, S, P, S, P, P, S, C, C, D, C, P, P, P], I),
+        "tags_mhem_gpd": ([P, S, P, S, P, S, C, C, P], I),
         "tags_sweep_create": ([ctypes.POINTER(P), P, P, S, P, S], I),
         "tags_sweep_solve": ([P, S, P, P, D, P, P, P, S, I, P, P, P], I),
         "tags_sweep_set_profiling": ([P, I], I),
@@ -321,6 +329,49 @@ def fill_impedance_adm2(yn, yla, ytb, a, b, num_electrodes, num_nodes):
 def solve_admittance(yn, ic, num_nodes):
     """ic (complex128, n) is overwritten by the node potentials; returns info."""
     return lib().solve_admittance(_p(yn), _p(ic), num_nodes)
+
+
+def fields(points, electrodes, il, it, gamma, s, kappa, mur=1.0):
+    """Batched potentials and fields: points (np, 3); il, it (nrhs, m) or None.
+    Returns (pot (np, nrhs), va (np, nrhs, 3), ve (np, nrhs, 3))."""
+    e = _electrodes(electrodes)
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    cur = [None if c is None else np.ascontiguousarray(np.atleast_2d(c), np.complex128)
+           for c in (il, it)]
+    nrhs = next(c.shape[0] for c in cur if c is not None)
+    np_ = len(pts)
+    pot = np.zeros((np_, nrhs), np.complex128)
+    va = np.zeros((np_, nrhs, 3), np.complex128)
+    ve = np.zeros((np_, nrhs, 3), np.complex128)
+    rc = lib().tags_fields_batch(_p(pts), np_, _p(e), len(e), _p(cur[0]), _p(cur[1]), nrhs,
+                                 CDouble.of(gamma), CDouble.of(s), float(mur), CDouble.of(kappa),
+                                 _p(pot), _p(va), _p(ve))
+    if rc:
+        raise RuntimeError("tags_fields_batch failed: %d" % rc)
+    return pot, va, ve
+
+
+def mhem_gpd(points, electrodes, it, gamma, kappa):
+    e = _electrodes(electrodes)
+    pts = np.ascontiguousarray(points, np.float64).reshape(-1, 3)
+    it = np.ascontiguousarray(np.atleast_2d(it), np.complex128)
+    out = np.zeros((len(pts), it.shape[0]), np.complex128)
+    rc = lib().tags_mhem_gpd(_p(pts), len(pts), _p(e), len(e), _p(it), it.shape[0],
+                             CDouble.of(gamma), CDouble.of(kappa), _p(out))
+    if rc:
+        raise RuntimeError("tags_mhem_gpd failed: %d" % rc)
+    return out
+
+
+def voltage(p1, p2, electrodes, il, it, gamma, s, kappa, mur=1.0):
+    e = _electrodes(electrodes)
+    p1 = np.ascontiguousarray(p1, np.float64)
+    p2 = np.ascontiguousarray(p2, np.float64)
+    il = np.ascontiguousarray(il, np.complex128)
+    it = np.ascontiguousarray(it, np.complex128)
+    v = lib().voltage(_p(p1), _p(p2), _p(e), len(e), _p(il), _p(it), CDouble.of(gamma),
+                      CDouble.of(s), float(mur), CDouble.of(kappa), 0, 0.0, 1e-6)
+    return v.value()
 
 
 def read_case_files(electrodes_csv, nodes_csv):

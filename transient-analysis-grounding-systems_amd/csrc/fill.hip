@@ -37,7 +37,7 @@ static bool g_gl_ready = false;
 static std::mutex g_gl_mutex;
 
 // Legendre nodes by Newton iteration on P_n from the Chebyshev-like guess.
-static void gauss_legendre(int n, double* x, double* w)
+void gauss_legendre(int n, double* x, double* w)
 {
     for (int i = 0; i < n; i++) {
         double z = std::cos(kPi * (i + 0.75) / (n + 0.5));

@@ -161,6 +161,8 @@ struct BlockLayout {
 int gl_tables_init();
 int launch_pairs_dropin(double* zl, double* zt, bool alias, SegSoA e, SegSoA img,
                         const long m, const PairScalars& sc, bool images, hipStream_t st);
+// Gauss-Legendre nodes and weights of order n on [0, 1] (host; Newton on P_n)
+void gauss_legendre(int n, double* x, double* w);
 int launch_mhem_potentials(double* pl, double* pt, double* pli, double* pti, long m, SegSoA e,
                            SegSoA img, hipStream_t st);
 int launch_fill_mhem(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
