@@ -75,11 +75,14 @@ TAGS_COMPLEX self_integral(const Electrode *sender);
  * log((r1 + r2 + L_s)/(r1 + r2 - L_s)) (src/electrode.c:219-235), which is the
  * double integral of 1/r: the same pair rule at gamma = 0, imaginary part 0,
  * error per unit receiver length as in the reference. INTG_NONE returns the
- * distance between middle points. max_eval, req_abs_error and req_rel_error
- * are accepted; the rule meets a relative error far below any req_rel_error
- * >= 1e-10. Returns 0, -10 for an unknown type, -11 for INTG_SINGLE (not
- * supported by this library). calculate_impedances and impedances_images take
- * INTG_MHEM with the reference's unit scalars. */
+ * distance between middle points. INTG_SINGLE returns L_s times the
+ * sender-axis integral of exp(-gamma r)/r to the receiver's middle point
+ * (src/electrode.c:199-216; 16-point Gauss-Legendre panels graded towards the
+ * closest axis point). max_eval, req_abs_error and req_rel_error are accepted;
+ * the rules meet a relative error far below any req_rel_error >= 1e-10.
+ * Returns 0 or -10 for an unknown type. calculate_impedances and
+ * impedances_images take every type, INTG_MHEM and INTG_NONE with the
+ * reference's unit scalars. */
 int integral(const Electrode *sender, const Electrode *receiver, TAGS_COMPLEX gamma,
              size_t max_eval, double req_abs_error, double req_rel_error,
              int integration_type, double result[2], double error[2]);

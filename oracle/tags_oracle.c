@@ -518,8 +518,12 @@ double or_self_integral(const or_electrode *e)
     return 2.0 * ls * (log((k2 + 1.0) / k1) - k2 + k1);
 }
 
-/* src/electrode.c:247-302 (INTG_NONE, INTG_DOUBLE and INTG_MHEM branches; -10
- * otherwise: INTG_SINGLE is outside the restated path and also gives -10).
+int or_single_pair(const or_electrode *sender, const or_electrode *receiver,
+                   const double gamma[2], size_t max_eval, double abs_tol, double rel_tol,
+                   double result[2], double error[2]);
+
+/* src/electrode.c:247-302 (INTG_NONE, INTG_DOUBLE, INTG_SINGLE and INTG_MHEM
+ * branches; -10 otherwise).
  * INTG_MHEM: result[0] = L_r times the 1-D integral, result[1] = 0; error[0]
  * is not scaled (as in the reference). */
 int or_integral(const or_electrode *sender, const or_electrode *receiver,
@@ -539,6 +543,9 @@ int or_integral(const or_electrode *sender, const or_electrode *receiver,
         result[1] *= receiver->length * sender->length;
         return fail;
     }
+    case OR_INTG_SINGLE:
+        return or_single_pair(sender, receiver, gamma, max_eval, abs_tol, rel_tol, result,
+                              error);
     case OR_INTG_MHEM: {
         or_idata d = {sender, receiver, {0.0, 0.0}};
         int fail = or_quad1(or_log_nf, &d, max_eval, abs_tol, rel_tol, &result[0], &error[0]);
@@ -971,6 +978,19 @@ static void or_single(const or_electrode *e, const double *point, const double g
     or_hcubature2(or_single_cb, &d, max_eval, abs_tol, rel_tol, res, err, NULL);
     res[0] *= e->length;
     res[1] *= e->length;
+}
+
+/* INTG_SINGLE of `integral` (src/electrode.c:279-284): the receiver's middle
+ * point as the field point */
+int or_single_pair(const or_electrode *sender, const or_electrode *receiver,
+                   const double gamma[2], size_t max_eval, double abs_tol, double rel_tol,
+                   double result[2], double error[2])
+{
+    or_fdata d = {sender, receiver->middle_point, {gamma[0], gamma[1]}};
+    or_hcubature2(or_single_cb, &d, max_eval, abs_tol, rel_tol, result, error, NULL);
+    result[0] *= sender->length;
+    result[1] *= sender->length;
+    return 0;
 }
 
 /* src/electrode.c:435-456: sum_m I_single(m) it[m] / L_m / (4 pi kappa) */

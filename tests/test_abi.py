@@ -79,7 +79,6 @@ def test_integral_none_and_unknown_types_host():
     rc, v, _ = T.integral(a, b, 0.1 + 0.2j, integration_type=T.INTG_NONE)
     assert rc == 0 and v == 2.0
     assert T.integral(a, b, 0.1, integration_type=17)[0] == -10
-    assert T.integral(a, b, 0.1, integration_type=T.INTG_SINGLE)[0] == -11
     zl = np.zeros(4, complex)
     assert T.calculate_impedances(zl, zl, np.concatenate([a, b]), 0.1, 1j, 1.0, 1.0,
                                   integration_type=17) == -10

@@ -95,3 +95,17 @@ def test_mhem_gpd_is_static_potential_at_zero_gamma():
     g = O.mhem_gpd(pts, e, it, 0.0, kappa)
     ref = [O.electric_potential(P, e, it, 0.0, kappa, rel_tol=1e-13) for P in pts]
     assert np.max(np.abs(g - ref)) <= 1e-11 * np.max(np.abs(ref))
+
+
+def test_intg_single_closed_form_at_zero_gamma():
+    """INTG_SINGLE (src/electrode.c:279-284): L_s times the sender-axis integral
+    of e^{-gamma r}/r to the receiver's middle point; at gamma = 0 that is
+    log((r1 + r2 + L)/(r1 + r2 - L)) with r1, r2 from the middle point."""
+    s = O.segment_electrode([0.0, 0.0, -0.5], [2.0, 0.5, -0.5], 1, 5e-3)[0]
+    r = O.segment_electrode([1.0, 2.0, -0.5], [3.0, 2.0, -0.5], 1, 5e-3)[0]
+    rc, v, err = O.integral(s, r, 0.0, rel_tol=1e-13, integration_type=O.INTG_SINGLE)
+    M = r[0]["middle_point"]
+    r1 = np.linalg.norm(M - s[0]["start_point"])
+    r2 = np.linalg.norm(M - s[0]["end_point"])
+    L = s[0]["length"]
+    assert rc == 0 and abs(v - np.log((r1 + r2 + L) / (r1 + r2 - L))) <= 1e-12 * abs(v)

@@ -97,3 +97,28 @@ def test_mhem_gpd_vs_oracle():
     for k in range(2):
         ref = O.mhem_gpd(pts, e, it[k], sc["gamma"], sc["kappa"])
         assert np.max(np.abs(out[:, k] - ref)) <= 1e-12 * np.max(np.abs(ref))
+
+
+@pytest.mark.parametrize("gamma", [0.0, 0.05 + 0.04j, 1.5 + 1.2j])
+def test_intg_single_vs_oracle(gamma):
+    e, it, il, pts = _case()
+    img = O.images_of(e)
+    for k in (1, 5, len(e) - 1):
+        rc, v, _ = T.integral(e[0], img[k], gamma, integration_type=T.INTG_SINGLE)
+        rco, vo, _ = O.integral(e[0], img[k], gamma, rel_tol=1e-13,
+                                integration_type=O.INTG_SINGLE)
+        assert rc == rco == 0 and abs(v - vo) <= 1e-11 * abs(vo)
+    m = len(e)
+    sc = O.soil_scalars(2e5, 0.002, 20.0)
+    zl = np.zeros(m * m, complex)
+    zt = np.zeros(m * m, complex)
+    assert T.calculate_impedances(zl, zt, e, sc["gamma"], sc["s"], 1.0, sc["kappa"],
+                                  integration_type=T.INTG_SINGLE) == 0
+    rc, zlo, zto = O.calculate_impedances(e, sc["gamma"], sc["s"], 1.0, sc["kappa"],
+                                          rel_tol=1e-12, integration_type=O.INTG_SINGLE,
+                                          threads=8)
+    rows, cols = np.triu_indices(m)
+    flat = rows * m + cols
+    assert rc == 0
+    assert np.abs(zl[flat] - zlo[flat]).max() <= 1e-10 * np.abs(zlo[flat]).max()
+    assert np.abs(zt[flat] - zto[flat]).max() <= 1e-10 * np.abs(zto[flat]).max()

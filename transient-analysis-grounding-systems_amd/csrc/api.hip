@@ -731,11 +731,8 @@ int integral(const Electrode* sender, const Electrode* receiver, TAGS_COMPLEX ga
         result[0] = vlen(sender->middle_point, receiver->middle_point);
         return 0;
     }
-    if (integration_type == INTG_SINGLE) {
-        std::printf("integration type %d is not supported by this library\n", integration_type);
-        return TAGS_ERR_UNSUPPORTED;
-    }
-    if (integration_type != INTG_DOUBLE && integration_type != INTG_MHEM) {
+    if (integration_type != INTG_DOUBLE && integration_type != INTG_MHEM &&
+        integration_type != INTG_SINGLE) {
         std::printf("integration failed with code: %i\n", -10);
         return -10;
     }
@@ -747,8 +744,8 @@ int integral(const Electrode* sender, const Electrode* receiver, TAGS_COMPLEX ga
     if ((rc = b.upload(receiver, 1, S.s))) return rc;
     DevBuf<double> out;
     if ((rc = out.alloc(4))) return rc;
-    if ((rc = launch_single_pair(a.soa(), b.soa(), mhem ? c64{0.0, 0.0} : from_c(gamma), out.p,
-                                 S.s)))
+    if ((rc = launch_single_pair(a.soa(), b.soa(), mhem ? c64{0.0, 0.0} : from_c(gamma),
+                                 mhem ? INTG_DOUBLE : integration_type, out.p, S.s)))
         return rc;
     double h[4];
     TAGS_HIP(hipMemcpyAsync(h, out.p, sizeof(h), hipMemcpyDeviceToHost, S.s));
@@ -805,12 +802,8 @@ int calculate_impedances(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* el
     (void)max_eval;
     (void)req_abs_error;
     (void)req_rel_error;
-    if (integration_type == INTG_SINGLE) {
-        std::printf("integration type %d is not supported by this library\n", integration_type);
-        return TAGS_ERR_UNSUPPORTED;
-    }
     if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE &&
-        integration_type != INTG_MHEM) {
+        integration_type != INTG_MHEM && integration_type != INTG_SINGLE) {
         std::printf("integration failed with code: %i\n", -10);
         return -10;
     }
@@ -828,12 +821,8 @@ int impedances_images(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* elect
     (void)max_eval;
     (void)req_abs_error;
     (void)req_rel_error;
-    if (integration_type == INTG_SINGLE) {
-        std::printf("integration type %d is not supported by this library\n", integration_type);
-        return TAGS_ERR_UNSUPPORTED;
-    }
     if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE &&
-        integration_type != INTG_MHEM) {
+        integration_type != INTG_MHEM && integration_type != INTG_SINGLE) {
         std::printf("integration failed with code: %i\n", -10);
         return -10;
     }

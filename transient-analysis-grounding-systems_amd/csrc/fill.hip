@@ -328,6 +328,45 @@ __device__ inline DSeg load_seg(const SegSoA& s, long i)
     return DSeg{V3{s.ax[i], s.ay[i], s.az[i]}, V3{s.bx[i], s.by[i], s.bz[i]}, s.len[i]};
 }
 
+// INTG_SINGLE (src/electrode.c:199-216, :279-284): the sender-axis integral of
+// e^{-gamma r}/r to the receiver's middle point M over t in [0, 1] (r floored at
+// DBL_EPSILON), by 16-point Gauss-Legendre panels graded away from the axis
+// parameter closest to M: the first panel d/L wide (d: distance from M to the
+// segment), each next one as wide as its distance from that parameter and at
+// most 2/(|gamma| L) wide.
+__device__ c64 single_unit(const DSeg& S, V3 M, c64 g)
+{
+    const int off = g_gloff[5];  // order 16
+    const V3 d = vsub(S.b, S.a);
+    const double dd = vdot(d, d);
+    double t0 = dd > 0.0 ? vdot(vsub(M, S.a), d) / dd : 0.0;
+    t0 = fmin(fmax(t0, 0.0), 1.0);
+    const double L = dd > 0.0 ? sqrt(dd) : S.len;
+    const double wmin = fmax(vnorm(vsub(vadd(S.a, vmul(d, t0)), M)) / L, 1e-14);
+    const double ga = sqrt(g.x * g.x + g.y * g.y);
+    const double wmax = ga * L > 2.0 ? 2.0 / (ga * L) : 1.0;
+    double sx = 0.0, sy = 0.0;
+    for (int side = 0; side < 2; side++) {
+        const double span = side ? 1.0 - t0 : t0;
+        double u0 = 0.0;
+        while (u0 < span) {
+            const double u1 = fmin(u0 + (u0 == 0.0 ? wmin : fmin(u0, wmax)), span);
+            const double h = u1 - u0;
+            for (int q = 0; q < 16; q++) {
+                const double u = u0 + h * g_glx[off + q];
+                const double t = side ? t0 + u : t0 - u;
+                double r = vnorm(vsub(vadd(S.a, vmul(d, t)), M));
+                if (r < 2.220446049250313e-16) r = 2.220446049250313e-16;
+                const c64 f = green(g, r);
+                sx += h * g_glw[off + q] * f.x;
+                sy += h * g_glw[off + q] * f.y;
+            }
+            u0 = u1;
+        }
+    }
+    return c64{sx, sy};
+}
+
 // Reference's `integral` result (L_r L_s times the unit-square integral, or the
 // middle-point distance for INTG_NONE). INTG_MHEM (src/electrode.c:219-235,
 // :288-293) is L_r times the receiver-axis integral of
@@ -341,6 +380,10 @@ __device__ inline c64 pair_result(const DSeg& S, const DSeg& R, c64 g, int type)
         return c64{vnorm(vsub(ms, mr)), 0.0};
     }
     const double ll = R.len * S.len;
+    if (type == INTG_SINGLE) {
+        const c64 J = single_unit(S, vmul(vadd(R.a, R.b), 0.5), g);
+        return c64{J.x * S.len, J.y * S.len};
+    }
     if (type == INTG_MHEM) {
         const c64 J = pair_integral_unit(S, R, c64{0.0, 0.0});
         return c64{J.x * ll, 0.0};
@@ -730,11 +773,29 @@ int launch_fill_static(c64* W, long ldw, const BlockLayout& L, const int* seg_st
 }
 
 // Single pair for the drop-in `integral`: out = {re, im, err_re, err_im}; the
-// error is the difference to the same pair evaluated with a 3x finer split.
-__global__ void k_single_pair(SegSoA e, SegSoA r, c64 g, double* out)
+// error is the difference to the same pair evaluated with a 3x finer split
+// (INTG_SINGLE: the sender split in 3).
+__global__ void k_single_pair(SegSoA e, SegSoA r, c64 g, int type, double* out)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     DSeg S = load_seg(e, 0), R = load_seg(r, 0);
+    if (type == INTG_SINGLE) {
+        const V3 M = vmul(vadd(R.a, R.b), 0.5);
+        const c64 J = single_unit(S, M, g);
+        const V3 ds = vmul(vsub(S.b, S.a), 1.0 / 3);
+        double sx = 0, sy = 0;
+        for (int i = 0; i < 3; i++) {
+            DSeg a{vadd(S.a, vmul(ds, i)), vadd(S.a, vmul(ds, i + 1)), S.len / 3};
+            const c64 v = single_unit(a, M, g);
+            sx += v.x / 3.0;
+            sy += v.y / 3.0;
+        }
+        out[0] = J.x * S.len;
+        out[1] = J.y * S.len;
+        out[2] = fabs(J.x - sx) * S.len;
+        out[3] = fabs(J.y - sy) * S.len;
+        return;
+    }
     c64 J = pair_integral_unit(S, R, g);
     // reference: split both segments in 3 and sum the 9 sub-pairs
     V3 ds = vmul(vsub(S.b, S.a), 1.0 / 3), dr = vmul(vsub(R.b, R.a), 1.0 / 3);
@@ -754,11 +815,11 @@ __global__ void k_single_pair(SegSoA e, SegSoA r, c64 g, double* out)
     out[3] = fabs(J.y - sy) * ll;
 }
 
-int launch_single_pair(SegSoA e, SegSoA r, c64 gamma, double* out4, hipStream_t st)
+int launch_single_pair(SegSoA e, SegSoA r, c64 gamma, int type, double* out4, hipStream_t st)
 {
     int rc = gl_tables_init();
     if (rc) return rc;
-    hipLaunchKernelGGL(k_single_pair, dim3(1), dim3(64), 0, st, e, r, gamma, out4);
+    hipLaunchKernelGGL(k_single_pair, dim3(1), dim3(64), 0, st, e, r, gamma, type, out4);
     TAGS_HIP(hipGetLastError());
     return 0;
 }

@@ -172,7 +172,7 @@ int launch_fill_zblocks(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA
                         const PairScalars& sc, hipStream_t st);
 int launch_fill_static(c64* W, long ldw, const BlockLayout& L, const int* seg_start_node,
                        const int* seg_end_node, hipStream_t st);
-int launch_single_pair(SegSoA e, SegSoA r, c64 gamma, double* out4, hipStream_t st);
+int launch_single_pair(SegSoA e, SegSoA r, c64 gamma, int type, double* out4, hipStream_t st);
 int launch_incidence(const double* seg_pts, long m, const double* nodes, long n,
                      int* seg_start_node, int* seg_end_node, int* node_hits,
                      hipStream_t st);
