@@ -26,22 +26,30 @@ def oracle():
     return O
 
 
-C_DRIVER_SRC = os.path.join(ROOT, "tests", "c_driver", "harmonic_c1.c")
-C_DRIVER_BIN = os.path.join(ROOT, "tests", "c_driver", "harmonic_c1")
+C_DRIVER_DIR = os.path.join(ROOT, "tests", "c_driver")
+C_DRIVER_SRC = os.path.join(C_DRIVER_DIR, "harmonic_c1.c")
+C_DRIVER_BIN = os.path.join(C_DRIVER_DIR, "harmonic_c1")
+C_ADM_DRIVER_BIN = os.path.join(C_DRIVER_DIR, "admittance_c1")
 
 
-def c_driver_command(out=C_DRIVER_BIN):
-    """gcc line of the C driver: plain C11 against include/tags_hip.h, linked
+def c_driver_command(out=C_DRIVER_BIN, src=C_DRIVER_SRC):
+    """gcc line of a C driver: plain C11 against include/tags_hip.h, linked
     to the in-tree libtags_hip.so (found through an $ORIGIN-relative rpath)."""
     return ["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", "-Werror",
-            "-I" + os.path.join(ROOT, "include"), C_DRIVER_SRC, "-o", out,
+            "-I" + os.path.join(ROOT, "include"), src, "-o", out,
             "-L" + PKG_DIR, "-ltags_hip", "-lm",
             "-Wl,-rpath,$ORIGIN/../../transient-analysis-grounding-systems_amd"]
 
 
-def build_c_driver(out=C_DRIVER_BIN):
+def build_c_driver(out=C_DRIVER_BIN, src=C_DRIVER_SRC):
+    """Builds a C driver (by default harmonic_c1; build() builds every one)."""
     import subprocess
-    r = subprocess.run(c_driver_command(out), capture_output=True, text=True)
+    r = subprocess.run(c_driver_command(out, src), capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("gcc failed on the C driver:\n" + r.stdout + r.stderr)
     return out
+
+
+def build_c_drivers():
+    build_c_driver()
+    build_c_driver(C_ADM_DRIVER_BIN, os.path.join(C_DRIVER_DIR, "admittance_c1.c"))

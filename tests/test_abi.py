@@ -106,3 +106,6 @@ def test_c_driver_compiles_and_links(tmp_path):
     out = str(tmp_path / "harmonic_c1")
     tags_pkg.build_c_driver(out)
     assert os.path.getsize(out) > 0
+    out = str(tmp_path / "admittance_c1")
+    tags_pkg.build_c_driver(out, os.path.join(tags_pkg.C_DRIVER_DIR, "admittance_c1.c"))
+    assert os.path.getsize(out) > 0

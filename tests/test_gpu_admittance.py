@@ -11,6 +11,8 @@ Tolerances:
 * the admittance potentials vs the immittance solve of the same physics:
   1e-9 (the two formulations' rounding differs by about cond(W) eps).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -172,3 +174,28 @@ def test_adm_singular_reports_info():
     yn = np.zeros(n * n, complex)
     assert T.fill_impedance_adm(yn, zl, zt, a, b, m, n) == m
     assert np.array_equal(zl, zl_keep)
+
+
+@pytest.mark.parametrize("mode", ["adm", "adm2"])
+def test_c_admittance_driver_vs_oracle(mode):
+    """The plain-C driver tests/c_driver/admittance_c1.c (built by build()):
+    the examples' mHEM admittance route through the reference API gives the
+    oracle's mHEM harmonic impedance of the 15 m conductor within 1e-8, and
+    (adm2) the leakage currents ytb U add up to the unit injection."""
+    import subprocess
+    assert os.path.exists(tags_pkg.C_ADM_DRIVER_BIN), "run __graft_entry__.build() first"
+    print("case: c_driver admittance_c1 6 freqs %s" % mode, flush=True)
+    r = subprocess.run([tags_pkg.C_ADM_DRIVER_BIN, "6", mode], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    rows = np.array([[float(v) for v in line.split()] for line in r.stdout.split("\n")
+                     if line.strip()])
+    f = rows[:, 0]
+    z = rows[:, 1] + 1j * rows[:, 2]
+    assert np.allclose(f, np.logspace(2, 7, 6), rtol=1e-10)
+    e, nodes = O.segment_electrode([0, 0, -0.5], [15, 0, -0.5], 30, 7e-3)
+    zo = O.mhem_harmonic_impedance(e, nodes, f, 0.01, 10.0, rel_tol=1e-11, threads=8)
+    assert np.max(np.abs(z - zo) / np.abs(zo)) <= 1e-8
+    if mode == "adm2":
+        kcl = rows[:, 3] + 1j * rows[:, 4]
+        assert np.max(np.abs(kcl - 1.0)) <= 1e-9
