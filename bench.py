@@ -16,7 +16,9 @@ between the two. The per-kernel figures (roofline, timers) come from a second
 timed pass over the same samples with one in flight, so that each launch's
 HIP-event duration is its own.
 
-Prints one JSON line. Single GPU only.
+Prints one JSON line. Single GPU only. `--rows` instead times the calls next
+to the path (nodal admittance, the Laplace transform pair, fields) and prints
+their own JSON line.
 """
 import argparse
 import json
@@ -116,6 +118,69 @@ def measured_traffic():
     return None
 
 
+def rows_bench(T):
+    """Timings of the rows next to the path, one JSON line: every figure is a
+    host-pointer call end to end (PCIe copies included) unless named device."""
+    out = {"mode": "rows", "n_gpus": 1}
+    L = T.lib()
+    rng = np.random.default_rng(1)
+
+    def timed(fn, reps=3):
+        fn()
+        L.tags_device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        L.tags_device_synchronize()
+        return (time.perf_counter() - t0) / reps
+
+    # nodal admittance at the 30 x 30 m grid, 5 m mesh (m = 1320, n = 1221),
+    # synthetic complex symmetric Z (the LU's work does not depend on values)
+    _, e, nodes = T.electrode_grid(11, 11, 30.0, 30.0, 6, 6, 5e-3, -0.5)
+    m, n = len(e), len(nodes)
+    a = np.zeros(m * n, complex)
+    b = np.zeros(m * n, complex)
+    T.fill_incidence_adm(a, b, e, nodes)
+    zs = []
+    for _ in range(2):
+        r = rng.standard_normal((m, m)) + 1j * rng.standard_normal((m, m))
+        zs.append(((r + r.T) / 2 + 4 * np.sqrt(m) * np.eye(m)).T.ravel().copy())
+    yn = np.zeros(n * n, complex)
+    dt = timed(lambda: T.fill_impedance_adm(yn, zs[0].copy(), zs[1].copy(), a, b, m, n))
+    fl = 2 * (8 / 3 * m ** 3 + 12.0 * m * m * (n + m)) + 2 * 8.0 * m * n * n
+    out["fill_impedance_adm"] = {"m": m, "n": n, "ms": 1e3 * dt,
+                                 "tflops_incl_pcie": fl / dt / 1e12,
+                                 "flops": fl}
+    ie = np.zeros(n, complex)
+    ie[0] = 1
+    dt = timed(lambda: T.solve_admittance(yn, ie.copy(), n))
+    out["solve_admittance"] = {"n": n, "ms": 1e3 * dt}
+    # numerical Laplace transform pair: nt = 2^14, 512 signals
+    nt, B, tmax = 2 ** 14, 512, 1e-4
+    f = rng.standard_normal((B, nt))
+    g, sgrid = T.laplace_trans(f, tmax)
+    dt_f = timed(lambda: T.laplace_trans(f, tmax))
+    dt_i = timed(lambda: T.inv_laplace_trans(g, sgrid, tmax, nt))
+    passes = int(np.log2(nt))
+    out["laplace_batch"] = {"nt": nt, "signals": B, "ms_forward": 1e3 * dt_f,
+                            "ms_inverse": 1e3 * dt_i,
+                            "device_pass_bytes": passes * 2 * 16 * nt * B}
+    # fields: 256 ground points around the 60 x 60 m grid (m = 7440), 1 rhs
+    _, e3, _ = T.electrode_grid(*GRID)
+    m3 = len(e3)
+    pts = np.column_stack([rng.uniform(-5, 65, 256), rng.uniform(-5, 65, 256), np.zeros(256)])
+    it = rng.standard_normal((1, m3)) + 1j * rng.standard_normal((1, m3))
+    il = rng.standard_normal((1, m3)) + 1j * rng.standard_normal((1, m3))
+    sc = T.soil_scalars(1e5, SIGMA, EPSR)
+    dt = timed(lambda: T.fields(pts, e3, il, it, sc["gamma"], sc["s"], sc["kappa"]))
+    out["fields_batch"] = {"points": 256, "segments": m3, "ms": 1e3 * dt,
+                           "pairs_per_s": 256 * m3 / dt}
+    dt = timed(lambda: T.mhem_gpd(pts, e3, it, sc["gamma"], sc["kappa"]))
+    out["mhem_gpd"] = {"points": 256, "segments": m3, "ms": 1e3 * dt,
+                       "pairs_per_s": 256 * m3 / dt}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +210,9 @@ def main():
                     help="hem: pair integrals of e^{-gamma r}/r every sample (the headline); "
                          "mhem: the examples' modified HEM, gamma-free potentials once per "
                          "geometry (in the warm-up call) and exp(-gamma rbar) scaling per sample")
+    ap.add_argument("--rows", action="store_true",
+                    help="time the rows next to the path (admittance, Laplace transform "
+                         "pair, fields) instead of the sweep; prints its own JSON line")
     ap.add_argument("--no-lookahead", action="store_true",
                     help="factor the steps strictly in order on one stream")
     ap.add_argument("--autotune", action="store_true",
@@ -161,6 +229,9 @@ def main():
         copy_calibration(args.copy_calibration)
         return
     T = load_pkg()
+    if args.rows:
+        rows_bench(T)
+        return
     rc, e, nodes = T.electrode_grid(*GRID)
     m, n = len(e), len(nodes)
     N = n + 2 * m

@@ -68,6 +68,45 @@ __global__ __launch_bounds__(256) void k_dft(const c64* __restrict__ x, c64* __r
     }
 }
 
+// a[b n + k] = (f[b n + k] w[k], 0): the reference's weighted samples
+__global__ __launch_bounds__(256) void k_weight_real(const double* __restrict__ f,
+                                                      const double* __restrict__ w, c64* a,
+                                                      long n, long batch)
+{
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < n * batch;
+         idx += (long)gridDim.x * 256)
+        a[idx] = c64{f[idx] * w[idx % n], 0.0};
+}
+
+// x[b n + k]: the windowed half spectrum g[b ns + k] win[k] for k < ns (the
+// imaginary parts of the DC and, for even n, Nyquist terms dropped, as a real
+// c2r transform does), its conjugate mirror above
+__global__ __launch_bounds__(256) void k_hermitian(const c64* __restrict__ g,
+                                                    const double* __restrict__ win, c64* x,
+                                                    long n, long ns, long batch)
+{
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < n * batch;
+         idx += (long)gridDim.x * 256) {
+        const long b = idx / n, k = idx % n;
+        const long kk = k < ns ? k : n - k;
+        const c64 v = g[b * ns + kk];
+        c64 r = c64{win[kk] * v.x, win[kk] * v.y};
+        if (kk == 0 || 2 * kk == n) r.y = 0.0;
+        if (k >= ns) r.y = -r.y;
+        x[idx] = r;
+    }
+}
+
+// out[b n + k] = Re x[b n + k] sc[k]
+__global__ __launch_bounds__(256) void k_real_scale(const c64* __restrict__ x,
+                                                     const double* __restrict__ sc, double* out,
+                                                     long n, long batch)
+{
+    for (long idx = (long)blockIdx.x * 256 + threadIdx.x; idx < n * batch;
+         idx += (long)gridDim.x * 256)
+        out[idx] = x[idx].x * sc[idx % n];
+}
+
 static unsigned grid_for(long work)
 {
     return (unsigned)std::min<long>((work + 255) / 256, 16384);
@@ -112,9 +151,10 @@ static std::vector<c64> twiddles(long n)
 }
 
 struct NltBuffers {
-    c64* w = nullptr;
-    c64* a = nullptr;
+    c64* w = nullptr;       // twiddles
+    c64* a = nullptr;       // transform buffers
     c64* b = nullptr;
+    double* v = nullptr;    // per-sample weights (n)
     hipStream_t st = nullptr;
     ~NltBuffers()
     {
@@ -122,6 +162,7 @@ struct NltBuffers {
         hipFree(w);
         hipFree(a);
         hipFree(b);
+        hipFree(v);
         if (st) hipStreamDestroy(st);
     }
     int init(long n, long batch)
@@ -130,6 +171,7 @@ struct NltBuffers {
         TAGS_HIP(hipMalloc(&w, sizeof(c64) * n));
         TAGS_HIP(hipMalloc(&a, sizeof(c64) * n * batch));
         TAGS_HIP(hipMalloc(&b, sizeof(c64) * n * batch));
+        TAGS_HIP(hipMalloc(&v, sizeof(double) * n));
         std::vector<c64> tw = twiddles(n);
         TAGS_HIP(hipMemcpyAsync(w, tw.data(), sizeof(c64) * n, hipMemcpyHostToDevice, st));
         return 0;
@@ -191,15 +233,18 @@ int tags_laplace_batch(const double* f, TAGS_COMPLEX* g, TAGS_COMPLEX* s, double
     c64* sg = (c64*)s;
     for (long k = 0; k < ns; k++) sg[k] = c64{c, dw * (double)k};
     if (B == 0) return 0;
-    std::vector<c64> in((size_t)n * B);
-    for (long k = 0; k < n; k++) {
-        const double wk = dt * std::exp(-c * (double)k * dt);
-        for (long i = 0; i < B; i++) in[(size_t)i * n + k] = c64{f[i * n + k] * wk, 0.0};
-    }
+    std::vector<double> wk(n);
+    for (long k = 0; k < n; k++) wk[k] = dt * std::exp(-c * (double)k * dt);
     NltBuffers nb;
     int rc = nb.init(n, B);
     if (rc) return rc;
-    TAGS_HIP(hipMemcpyAsync(nb.a, in.data(), sizeof(c64) * n * B, hipMemcpyHostToDevice, nb.st));
+    // the real samples travel as doubles (staged in the second buffer)
+    double* fd = (double*)nb.b;
+    TAGS_HIP(hipMemcpyAsync(fd, f, sizeof(double) * n * B, hipMemcpyHostToDevice, nb.st));
+    TAGS_HIP(hipMemcpyAsync(nb.v, wk.data(), sizeof(double) * n, hipMemcpyHostToDevice, nb.st));
+    hipLaunchKernelGGL(k_weight_real, dim3(grid_for(n * B)), dim3(256), 0, nb.st, fd, nb.v, nb.a,
+                       n, B);
+    TAGS_HIP(hipGetLastError());
     if ((rc = dft_batch(nb.a, nb.b, nb.w, n, B, false, nb.st))) return rc;
     TAGS_HIP(hipMemcpy2DAsync(g, sizeof(c64) * ns, nb.a, sizeof(c64) * n, sizeof(c64) * ns, B,
                               hipMemcpyDeviceToHost, nb.st));
@@ -236,31 +281,33 @@ int tags_inv_laplace_batch(double* f, const TAGS_COMPLEX* g, const TAGS_COMPLEX*
     const c64* sv = (const c64*)s;
     const double dt = tmax / (double)(nt - 1);
     const double omega = sv[ns - 1].y;
-    std::vector<double> win(ns);
+    // window (ns) and output scale (n), by the reference's formulas on the host
+    std::vector<double> win(ns), sc(n);
     for (long k = 0; k < ns; k++) win[k] = inlt_window(filter, sv[k].y, omega);
-    // Hermitian extension of the windowed half spectrum; the imaginary parts
-    // of the DC and (even n) Nyquist terms do not enter a real c2r transform
-    std::vector<c64> in((size_t)n * B);
-    for (long i = 0; i < B; i++) {
-        c64* x = in.data() + (size_t)i * n;
-        const c64* gi = gg + i * ns;
-        for (long k = 0; k < ns; k++) x[k] = c64{win[k] * gi[k].x, win[k] * gi[k].y};
-        x[0].y = 0.0;
-        if (n % 2 == 0) x[n / 2].y = 0.0;
-        for (long k = ns; k < n; k++) x[k] = c64{x[n - k].x, -x[n - k].y};
-    }
+    const double c = sv[0].x;
+    for (long k = 0; k < n; k++) sc[k] = std::exp(c * (double)k * dt) / ((double)nt * dt);
     NltBuffers nb;
     int rc = nb.init(n, B);
     if (rc) return rc;
-    TAGS_HIP(hipMemcpyAsync(nb.a, in.data(), sizeof(c64) * n * B, hipMemcpyHostToDevice, nb.st));
+    // the half spectra staged in the second buffer and the window after the
+    // output scale in nb.v; the Hermitian extension built on the device
+    TAGS_HIP(hipFree(nb.v));
+    nb.v = nullptr;
+    TAGS_HIP(hipMalloc(&nb.v, sizeof(double) * (n + ns)));
+    double* dw = nb.v + n;
+    TAGS_HIP(hipMemcpyAsync(nb.b, gg, sizeof(c64) * ns * B, hipMemcpyHostToDevice, nb.st));
+    TAGS_HIP(hipMemcpyAsync(dw, win.data(), sizeof(double) * ns, hipMemcpyHostToDevice, nb.st));
+    TAGS_HIP(hipMemcpyAsync(nb.v, sc.data(), sizeof(double) * n, hipMemcpyHostToDevice, nb.st));
+    hipLaunchKernelGGL(k_hermitian, dim3(grid_for(n * B)), dim3(256), 0, nb.st, nb.b, dw, nb.a, n,
+                       ns, B);
+    TAGS_HIP(hipGetLastError());
     if ((rc = dft_batch(nb.a, nb.b, nb.w, n, B, true, nb.st))) return rc;
-    TAGS_HIP(hipMemcpyAsync(in.data(), nb.a, sizeof(c64) * n * B, hipMemcpyDeviceToHost, nb.st));
+    double* fd = (double*)nb.b;
+    hipLaunchKernelGGL(k_real_scale, dim3(grid_for(n * B)), dim3(256), 0, nb.st, nb.a, nb.v, fd,
+                       n, B);
+    TAGS_HIP(hipGetLastError());
+    TAGS_HIP(hipMemcpyAsync(f, fd, sizeof(double) * n * B, hipMemcpyDeviceToHost, nb.st));
     TAGS_HIP(hipStreamSynchronize(nb.st));
-    const double c = sv[0].x;
-    for (long k = 0; k < n; k++) {
-        const double sc = std::exp(c * (double)k * dt) / ((double)nt * dt);
-        for (long i = 0; i < B; i++) f[i * n + k] = in[(size_t)i * n + k].x * sc;
-    }
     return 0;
 }
 
