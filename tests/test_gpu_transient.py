@@ -97,7 +97,7 @@ def test_gpr_waveform_end_to_end():
     assert np.max(np.abs(zh - zo) / np.abs(zo)) <= 1e-8
     vo = O.inv_laplace_trans(zo * igo, so, tmax, nt, 1)
     # the last samples of an NLT inversion are not physical (the e^{c t}
-    # factor reaches nt^2 at tmax); the waveform is judged before 0.9 tmax
+    # factor reaches nt^2 at tmax); the waveform is compared before 0.9 tmax
     body = slice(0, int(0.9 * nt))
     peak = np.max(np.abs(vo[body]))
     assert np.max(np.abs(v - vo)[body]) <= 1e-7 * peak

@@ -309,7 +309,7 @@ struct Solver {
         return 0;
     }
 
-    // Second half: waits and judges the queued steps by the backward error
+    // Second half: waits and checks the queued steps by the backward error
     // |b - W x| / (|W| |x|) of each iterate; while it is above kConverged and
     // still falling by a factor kMinGain or better per step, refines further
     // one step at a time (up to kMaxIters). The iterate in x (the last step
