@@ -17,8 +17,9 @@ timed pass over the same samples with one in flight, so that each launch's
 HIP-event duration is its own.
 
 Prints one JSON line. Single GPU only. `--rows` instead times the calls next
-to the path (nodal admittance, the Laplace transform pair, fields) and prints
-their own JSON line.
+to the path (nodal admittance, the Laplace transform pair, fields), and
+`--transient` runs the lightning transient on the 30 x 30 m grid end to end;
+each prints its own JSON line.
 """
 import argparse
 import json
@@ -181,6 +182,66 @@ def rows_bench(T):
     print(json.dumps(out))
 
 
+def transient_bench(T, nt=2 ** 13):
+    """The lightning-transient case end to end, one JSON line: the 30 x 30 m
+    grid with 5 m mesh (11 + 11 conductors, 1 m segments: m = 1320, n = 1221,
+    N = 3861) in an Alipio-Visacro "mean" soil (sigma0 = 5e-4 S/m,
+    h = 1.26 (1000 sigma0)^-0.73, g = 0.54, eps_inf'/eps0 = 12), a 1 kA
+    double-exponential current (alpha = 1.4659e4 /s, beta = 2.4689e6 /s,
+    k = 1.0373) injected at the centre node, tmax = 100 us, nt samples: NLT of
+    the current, the sweep over the nt/2 + 1 complex frequencies in both
+    precisions, INLT (Blackman window) of Z_h I. Reports the sweeps' rates and
+    the GPR waveform difference between them before 0.9 tmax."""
+    _, e, nodes = T.electrode_grid(11, 11, 30.0, 30.0, 6, 6, 5e-3, -0.5)
+    inj = int(np.argmin(np.sum((nodes - [15.0, 15.0, -0.5]) ** 2, axis=1)))
+    tmax = 1e-4
+    t = np.linspace(0.0, tmax, nt)
+    cur = 1e3 * 1.0373 * (np.exp(-1.4659e4 * t) - np.exp(-2.4689e6 * t))
+    ig, s = T.laplace_trans(cur, tmax)
+    sigma0 = 5e-4
+    h = 1.26 * (1000 * sigma0) ** -0.73
+    kap = np.array([sg + sk * er * T.EPS0 for sk in s
+                    for sg, er in [T.alipio_soil(sigma0, sk, h, 0.54, 12.0)]])
+    sw = T.Sweep(e, nodes)
+    sw.set_plan(T.PLAN_HYBRID)
+    L = T.lib()
+    out = {"mode": "transient", "n_gpus": 1, "m": len(e), "n": len(nodes),
+           "N": len(nodes) + 2 * len(e), "inj_node": inj, "nt": nt, "samples": len(s)}
+    res = {}
+    for name, prec in (("fp64", T.PREC_FP64), ("mixed", T.PREC_MIXED)):
+        sw.set_plan(T.PLAN_HYBRID if prec == T.PREC_MIXED else 2)
+        sw.solve(s[:4], kap[:4], inj, prec)
+        L.tags_device_synchronize()
+        t0 = time.perf_counter()
+        zh, st = sw.solve(s, kap, inj, prec)
+        L.tags_device_synchronize()
+        dt = time.perf_counter() - t0
+        v = T.inv_laplace_trans(zh * ig, s, tmax, nt, T.FILTER_BLACKMAN)
+        res[name] = (zh, v)
+        out[name] = {"samples_per_s": len(s) / dt, "s": dt}
+        if prec == T.PREC_MIXED:
+            out[name]["fp64_fallbacks"] = int(sum(q["precision_used"] == 0 for q in st))
+            out[name]["refine_iters_mean"] = float(np.mean([q["refine_iters"] for q in st]))
+    sw.close()
+    # the INLT multiplies by e^{c t}, which reaches nt^2 at tmax: a relative
+    # difference d of the spectra shows as up to d nt^(2 t / tmax) in the
+    # waveform, so the waveforms are compared over the first half of the
+    # window and, over all of it, with that factor taken out
+    v64, vmx = res["fp64"][1], res["mixed"][1]
+    half = slice(0, nt // 2)
+    damp = np.exp(-s[0].real * t)
+    out["gpr_peak_v"] = float(np.max(np.abs(v64[half])))
+    out["gpr_max_rel_diff_first_half"] = float(np.max(np.abs(vmx - v64)[half]) /
+                                               np.max(np.abs(v64[half])))
+    out["gpr_max_rel_diff_damped"] = float(np.max(np.abs((vmx - v64) * damp)) /
+                                           np.max(np.abs(v64 * damp)))
+    dz = np.abs(res["mixed"][0] - res["fp64"][0]) / np.abs(res["fp64"][0])
+    out["zh_rel_diff_mixed_vs_fp64"] = {"max": float(dz.max()), "median": float(np.median(dz)),
+                                        "worst_sample_hz": float(abs(s[int(np.argmax(dz))]) /
+                                                                 (2 * np.pi))}
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,6 +271,9 @@ def main():
                     help="hem: pair integrals of e^{-gamma r}/r every sample (the headline); "
                          "mhem: the examples' modified HEM, gamma-free potentials once per "
                          "geometry (in the warm-up call) and exp(-gamma rbar) scaling per sample")
+    ap.add_argument("--transient", action="store_true",
+                    help="the lightning transient on the 30 x 30 m grid end to end (NLT, "
+                         "both sweeps, INLT); prints its own JSON line")
     ap.add_argument("--rows", action="store_true",
                     help="time the rows next to the path (admittance, Laplace transform "
                          "pair, fields) instead of the sweep; prints its own JSON line")
@@ -231,6 +295,9 @@ def main():
     T = load_pkg()
     if args.rows:
         rows_bench(T)
+        return
+    if args.transient:
+        transient_bench(T)
         return
     rc, e, nodes = T.electrode_grid(*GRID)
     m, n = len(e), len(nodes)
