@@ -139,7 +139,9 @@ int fill_incidence_adm(TAGS_COMPLEX *a, TAGS_COMPLEX *b, const Electrode *electr
  * Z^-1 A come from one complex LU with row exchanges per block on the GPU
  * (FP64, the identity and A carried as extra columns, then a blocked backward
  * solve), the products on the MFMA cores. Returns 0, or the first exactly
- * zero pivot (1-based; zl, zt then unchanged). */
+ * zero pivot (1-based). On a singular Z_L neither zl nor zt changes; on a
+ * singular Z_T zl holds its inverse (as after the reference's zsytri) and zt
+ * is unchanged (the reference leaves zsytrf's partial factor there). */
 int fill_impedance_adm(TAGS_COMPLEX *yn, TAGS_COMPLEX *zl, TAGS_COMPLEX *zt, TAGS_COMPLEX *a,
                        TAGS_COMPLEX *b, size_t num_electrodes, size_t num_nodes);
 

@@ -174,6 +174,13 @@ def test_adm_singular_reports_info():
     yn = np.zeros(n * n, complex)
     assert T.fill_impedance_adm(yn, zl, zt, a, b, m, n) == m
     assert np.array_equal(zl, zl_keep)
+    # singular Z_T: zl already holds its (here identity) inverse, zt unchanged
+    zl2 = np.eye(m, dtype=complex).ravel() * 2.0
+    zt2 = z.T.ravel().copy()
+    zt2_keep = zt2.copy()
+    assert T.fill_impedance_adm(yn, zl2, zt2, a, b, m, n) == m
+    assert np.allclose(zl2, np.eye(m).ravel() * 0.5, rtol=0, atol=1e-15)
+    assert np.array_equal(zt2, zt2_keep)
 
 
 @pytest.mark.parametrize("mode", ["adm", "adm2"])

@@ -1271,9 +1271,12 @@ static int adm_inverses(AdmArrays& d, TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const 
     TAGS_HIP(hipMemcpyAsync(d.zl.p, zl, sizeof(c64) * m * m, hipMemcpyHostToDevice, st));
     TAGS_HIP(hipMemcpyAsync(d.zt.p, zt, sizeof(c64) * m * m, hipMemcpyHostToDevice, st));
     LUWork<double> w;
+    // Z_L first, written back before Z_T is factored: as with zsytrf/zsytri in
+    // the reference, a singular Z_T leaves zl already inverted
     if ((rc = adm_block(w, d.zl.p, d.arm.p, 1.0, m, n, d.yla.p, st))) return rc;
-    if ((rc = adm_block(w, d.zt.p, d.brm.p, -1.0, m, n, d.ytb.p, st))) return rc;
     TAGS_HIP(hipMemcpyAsync(zl, d.zl.p, sizeof(c64) * m * m, hipMemcpyDeviceToHost, st));
+    TAGS_HIP(hipStreamSynchronize(st));
+    if ((rc = adm_block(w, d.zt.p, d.brm.p, -1.0, m, n, d.ytb.p, st))) return rc;
     TAGS_HIP(hipMemcpyAsync(zt, d.zt.p, sizeof(c64) * m * m, hipMemcpyDeviceToHost, st));
     TAGS_HIP(hipStreamSynchronize(st));
     return 0;
