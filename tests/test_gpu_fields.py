@@ -122,3 +122,24 @@ def test_intg_single_vs_oracle(gamma):
     assert rc == 0
     assert np.abs(zl[flat] - zlo[flat]).max() <= 1e-10 * np.abs(zlo[flat]).max()
     assert np.abs(zt[flat] - zto[flat]).max() <= 1e-10 * np.abs(zto[flat]).max()
+
+
+def test_fields_many_right_hand_sides():
+    """Five right-hand sides (two work-group rows of at most four) against
+    the same sides one at a time, and linearity in the currents."""
+    e, it, il, pts = _case()
+    rng = np.random.default_rng(3)
+    m = len(e)
+    it5 = rng.standard_normal((5, m)) + 1j * rng.standard_normal((5, m))
+    il5 = rng.standard_normal((5, m)) + 1j * rng.standard_normal((5, m))
+    sc = O.soil_scalars(7e4, 0.002, 20.0)
+    g, s, kap = sc["gamma"], sc["s"], sc["kappa"]
+    pot, va, ve = T.fields(pts, e, il5, it5, g, s, kap)
+    for k in range(5):
+        p1, a1, e1 = T.fields(pts, e, il5[k], it5[k], g, s, kap)
+        assert np.max(np.abs(pot[:, k] - p1[:, 0])) <= 1e-13 * np.max(np.abs(p1))
+        assert np.max(np.abs(va[:, k] - a1[:, 0])) <= 1e-13 * np.max(np.abs(a1))
+        assert np.max(np.abs(ve[:, k] - e1[:, 0])) <= 1e-13 * np.max(np.abs(e1))
+    p2, a2, e2 = T.fields(pts, e, il5[0] + 2 * il5[1], it5[0] + 2 * it5[1], g, s, kap)
+    assert np.max(np.abs(p2[:, 0] - (pot[:, 0] + 2 * pot[:, 1]))) <= 1e-12 * np.max(np.abs(p2))
+    assert np.max(np.abs(e2[:, 0] - (ve[:, 0] + 2 * ve[:, 1]))) <= 1e-12 * np.max(np.abs(e2))
