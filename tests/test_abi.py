@@ -109,3 +109,27 @@ def test_c_driver_compiles_and_links(tmp_path):
     out = str(tmp_path / "admittance_c1")
     tags_pkg.build_c_driver(out, os.path.join(tags_pkg.C_DRIVER_DIR, "admittance_c1.c"))
     assert os.path.getsize(out) > 0
+
+
+def test_argument_checks_without_gpu():
+    """Entry points that reject their arguments (or have nothing to do) before
+    any device call, as the reference returns its codes."""
+    L = T.lib()
+    P = T._p
+    f = np.zeros(8)
+    g = np.zeros(5, complex)
+    s = np.zeros(5, complex)
+    assert L.laplace_trans(P(f), P(g), P(s), 1e-5, 1) == -12          # nt < 2
+    assert L.inv_laplace_trans(P(f), P(g), P(s), 0.0, 8, 0) == -12     # tmax <= 0
+    assert L.tags_fields_batch(None, 3, None, 1, None, None, 1, T.CDouble.of(0),
+                               T.CDouble.of(0), 1.0, T.CDouble.of(1), None, None, None) == -12
+    assert L.tags_fields_batch(None, 0, None, 1, None, None, 1, T.CDouble.of(0),
+                               T.CDouble.of(0), 1.0, T.CDouble.of(1), None, None, None) == 0
+    assert L.tags_mhem_gpd(None, 2, None, 1, None, 1, T.CDouble.of(0), T.CDouble.of(1),
+                           None) == -12
+    e, nodes = O.segment_electrode([0, 0, -0.5], [3, 0, -0.5], 3, 5e-3)
+    a = np.zeros(3 * 4, complex)
+    # no nodes: sum(B) = 0 != m -> -m; no segments: node 0 unconnected -> 1
+    assert T.fill_incidence_adm(a, a, e, nodes[:0]) == -3
+    assert T.fill_incidence_adm(a, a, e[:0], nodes) == 1
+    assert L.solve_admittance(None, None, 0) == 0
