@@ -182,7 +182,7 @@ def rows_bench(T):
     print(json.dumps(out))
 
 
-def transient_bench(T, nt=2 ** 13):
+def transient_bench(T, nt=2 ** 14):
     """The lightning-transient case end to end, one JSON line: the 30 x 30 m
     grid with 5 m mesh (11 + 11 conductors, 1 m segments: m = 1320, n = 1221,
     N = 3861) in an Alipio-Visacro "mean" soil (sigma0 = 5e-4 S/m,

@@ -265,8 +265,13 @@ typedef struct {
     int precision_used;   /* tags_precision actually used for this sample */
     int refine_iters;     /* refinement steps (mixed) */
     int info;             /* 0, or first zero pivot (1-based) */
-    double backward_err;  /* ||b - W x||_inf / (||W||_inf ||x||_inf) in FP64 */
+    double backward_err;  /* ||b - W x||_inf / (||W||_inf ||x||_inf) in FP64 (-1 when
+                             not computed: the FP64 form of a sweep without
+                             tags_sweep_set_verify) */
     double refine_hist[8];/* backward error after each refinement step (mixed) */
+    double forward_est;   /* mixed: ||dx||_inf / ||x||_inf of the last refinement
+                             step, an estimate of the error left in x (accepted at
+                             <= 1e-11); -1 for the FP64 form */
 } tags_item_stats;
 
 typedef struct {
@@ -298,14 +303,17 @@ int tags_sweep_create(tags_sweep **out, const Electrode *electrodes,
  * ref_l = 1, ref_t = (kappa - s eps0)/(kappa + s eps0) (the reference caller's
  * formulas). u_inj[f] receives U[inj_node] (= Z_h for unit current).
  * x_out (nullable): nfreq * N complex, the full [U; I_L; I_T] per sample.
- * stats (nullable): nfreq entries. */
+ * stats (nullable): nfreq entries. A sample whose factor has an exactly zero
+ * pivot gets stats info > 0 and NaN in u_inj[f] and in its block of x_out. */
 int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
                      const TAGS_COMPLEX *kappa, double mur, const TAGS_COMPLEX *gamma,
                      const TAGS_COMPLEX *ref_l, const TAGS_COMPLEX *ref_t, size_t inj_node,
                      int precision, TAGS_COMPLEX *u_inj, TAGS_COMPLEX *x_out,
                      tags_item_stats *stats);
 
-/* Factorisation plan of the sweep (default TAGS_PLAN_STRUCTURED). Every plan
+/* Factorisation plan of the sweep (default TAGS_PLAN_HYBRID, the fastest
+ * measured: ZL_FIRST's FP64 factor, the nodal complex64 factor from the switch
+ * frequency up). Every plan
  * but the mixed factor of GROUNDED pivots with the process's panel algorithm
  * (tags_set_panel); the structured
  * ones skip rows and columns that are exactly zero for the columns being
@@ -423,16 +431,27 @@ int tags_sweep_set_formulation(tags_sweep *sw, int formulation);
  * sample at |s| / 2 pi >= hz takes the nodal plan's complex64 factor, a lower
  * one ZL_FIRST's. A nodal sample that falls back to FP64 or needs more than 10
  * refinement steps raises the switch to 1.5 times its frequency for the rest
- * of the sweep (until this is called again). No counterpart in the reference,
- * which factors every sample alike. Returns 0, or TAGS_ERR_ARG for hz < 0. */
+ * of that tags_sweep_solve call (each call starts from the configured switch).
+ * No counterpart in the reference, which factors every sample alike. Returns 0,
+ * or TAGS_ERR_ARG for hz < 0. */
 int tags_sweep_set_nodal_above(tags_sweep *sw, double hz);
+
+/* FP64 form only: with on != 0, W is assembled again after each sample's
+ * factorisation (which replaced it) and the backward error
+ * ||e_inj - W x||_inf / (||W||_inf ||x||_inf) goes to the item statistics
+ * (one more fill and one FP64 residual per sample; off by default, when
+ * backward_err is -1). No counterpart in the reference. */
+int tags_sweep_set_verify(tags_sweep *sw, int on);
 
 /* Frequency samples a sweep keeps in flight (1 .. 4, default 2), each with its
  * own HIP stream, W and factor work space (about 11 GB for the 60 x 60 m grid
  * in the mixed form). The calling thread queues a sample on every slot, then
  * finishes them in issue order, queuing the next sample on each slot as it
  * frees up, so that one sample's trailing updates keep the GPU busy while
- * another waits on its panel chain. Results do not depend on the count. */
+ * another waits on its panel chain. Results do not depend on the count, except
+ * where TAGS_PLAN_HYBRID raises its switch during a call: samples already in
+ * flight keep the factor they were issued with (both answers meet the same
+ * acceptance rule). */
 int tags_sweep_set_inflight(tags_sweep *sw, int samples);
 
 /* Assembles W for one frequency sample on the GPU exactly as tags_sweep_solve
@@ -440,6 +459,14 @@ int tags_sweep_set_inflight(tags_sweep *sw, int samples);
  * column-major, reference unknown order [U; I_L; I_T], both triangles). */
 int tags_sweep_assemble(tags_sweep *sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, double mur,
                         TAGS_COMPLEX *wg);
+
+/* As tags_sweep_assemble, but copies only the `count` entries W(rows[i],
+ * cols[i]) (reference unknown order, 0-based) to out[i]: spot checks of large
+ * systems without moving N^2 values. Returns TAGS_ERR_ARG for an index out of
+ * range. */
+int tags_sweep_assemble_entries(tags_sweep *sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, double mur,
+                                size_t count, const long *rows, const long *cols,
+                                TAGS_COMPLEX *out);
 
 /* Accumulates HIP-event timings of every kernel family on the sweep's stream. */
 int tags_sweep_set_profiling(tags_sweep *sw, int on);
@@ -481,8 +508,9 @@ int tags_set_lookahead(int on);
  * work-groups waiting for a CU start. Returns 0, or TAGS_ERR_ARG for n < 1. */
 int tags_set_update_chunks(int n);
 
-/* Columns per factorisation step of every LU in the process: 256 (default,
- * four sub-panels of 64; the trailing update is a K = 256 product) or 128. */
+/* Columns per factorisation step of every LU in the process, both precisions:
+ * 256 (four sub-panels of 64; the trailing update is a K = 256 product) or 128;
+ * 0 restores the defaults (128 for complex128 factors, 256 for complex64). */
 int tags_set_step_width(int columns);
 
 /* Library version string. */

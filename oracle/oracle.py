@@ -96,6 +96,8 @@ def lib():
         L.or_number_nodes.restype = I
         L.or_electrode_grid.argtypes = [_Grid, P, P]
         L.or_electrode_grid.restype = I
+        L.or_z_entries.argtypes = [P, P, S, P, S, P, P, D, P, P, P, D, P, P, I]
+        L.or_z_entries.restype = I
         L.or_pair_sample.argtypes = [P, P, P, S, P, D, P, I]
         L.or_pair_sample.restype = I
         L.or_logspace.argtypes = [D, D, S, P]
@@ -542,6 +544,24 @@ def pair_sample(e, images, pairs, gamma, rel_tol=1e-6, threads=1):
     lib().or_pair_sample(_p(e), _p(images), _p(pairs), len(pairs), _p(_c2(gamma)), rel_tol,
                          _p(out), threads)
     return out[:, 0] + 1j * out[:, 1], out[:, 2] + 1j * out[:, 3]
+
+
+def z_entries(e, images, pairs, sc, rel_tol=1e-6, threads=1):
+    """Z_L(i, k), Z_T(i, k) after calculate_impedances + impedances_images for
+    each pair (i, k) (symmetric: the lower-half entry of min, max), with the
+    same per-entry arithmetic as the full-matrix routines."""
+    e = np.ascontiguousarray(e, dtype=ELECTRODE_DTYPE)
+    images = np.ascontiguousarray(images, dtype=ELECTRODE_DTYPE)
+    pairs = np.ascontiguousarray(pairs, dtype=np.int64).reshape(-1, 2)
+    zl = np.zeros(len(pairs), np.complex128)
+    zt = np.zeros(len(pairs), np.complex128)
+    rc = lib().or_z_entries(_p(e), _p(images), len(e), _p(pairs), len(pairs),
+                            _p(_c2(sc["gamma"])), _p(_c2(sc["s"])), sc["mur"],
+                            _p(_c2(sc["kappa"])), _p(_c2(sc["ref_l"])), _p(_c2(sc["ref_t"])),
+                            rel_tol, _p(zl), _p(zt), threads)
+    if rc:
+        raise RuntimeError("or_z_entries failed: %d" % rc)
+    return zl, zt
 
 
 def assemble(e, nodes, images, sc, rel_tol=1e-6, threads=1):

@@ -577,61 +577,124 @@ static inline void cdiv(const double a[2], const double b[2], double o[2])
 /* src/electrode.c:304-377. Diagonal from the closed-form self integral; for
  * k > i: Z_T = one_4pik/(Li Lk) * I, Z_L = iwu_4pi * I * |d_i . d_k|/(Li Lk), or 0
  * when the un-normalised dot product is below DBL_EPSILON. */
+/* One off-diagonal entry (k > i) of src/electrode.c:350-372: I = integral of
+ * the pair, Z_T = one_4pik I / (L_i L_k), Z_L = iwu_4pi I |d_i . d_k| / (L_i L_k)
+ * or 0 when the un-normalised |d_i . d_k| < DBL_EPSILON. Returns the integral's
+ * failure code (entries untouched then). */
+static int z_direct_entry(const or_electrode *e, size_t i, size_t k, const double gamma[2],
+                          const double iwu_4pi[2], const double one_4pik[2], size_t max_eval,
+                          double abs_tol, double rel_tol, int integration_type, double zl[2],
+                          double zt[2])
+{
+    double ls = e[i].length, lr = e[k].length, cost = 0.0, res[2], err[2], tmp[2];
+    for (int c = 0; c < 3; c++)
+        cost += (e[i].end_point[c] - e[i].start_point[c]) *
+                (e[k].end_point[c] - e[k].start_point[c]);
+    cost = fabs(cost);
+    int f = or_integral(&e[i], &e[k], gamma, max_eval, abs_tol, rel_tol, integration_type, res,
+                        err);
+    if (f) return f;
+    tmp[0] = one_4pik[0] / (ls * lr);
+    tmp[1] = one_4pik[1] / (ls * lr);
+    cmul(tmp, res, zt);
+    if (cost < DBL_EPSILON) {
+        zl[0] = 0.0;
+        zl[1] = 0.0;
+    } else {
+        cost = cost / (ls * lr);
+        cmul(iwu_4pi, res, tmp);
+        zl[0] = tmp[0] * cost;
+        zl[1] = tmp[1] * cost;
+    }
+    return 0;
+}
+
+/* One image entry (k >= i) of src/electrode.c:403-428, added to zl, zt: I = the
+ * integral between segment i and image k, Z_T += one_4pik I / (L_i L_k) and, if
+ * |d_i . d'_k| > DBL_EPSILON, Z_L += iwu_4pi I |d_i . d'_k| / (L_i L_k), the
+ * scalars already carrying ref_t and ref_l. */
+static int z_image_entry(const or_electrode *e, const or_electrode *images, size_t i, size_t k,
+                         const double gamma[2], const double iwu_4pi[2],
+                         const double one_4pik[2], size_t max_eval, double abs_tol,
+                         double rel_tol, int integration_type, double zl[2], double zt[2])
+{
+    double ls = e[i].length, lr = images[k].length, cost = 0.0, res[2], err[2], tmp[2];
+    for (int c = 0; c < 3; c++)
+        cost += (e[i].end_point[c] - e[i].start_point[c]) *
+                (images[k].end_point[c] - images[k].start_point[c]);
+    cost = fabs(cost);
+    int f = or_integral(&e[i], &images[k], gamma, max_eval, abs_tol, rel_tol, integration_type,
+                        res, err);
+    if (f) return f;
+    tmp[0] = one_4pik[0] / (ls * lr);
+    tmp[1] = one_4pik[1] / (ls * lr);
+    double add[2];
+    cmul(tmp, res, add);
+    zt[0] += add[0];
+    zt[1] += add[1];
+    if (cost > DBL_EPSILON) {
+        cost = cost / (ls * lr);
+        cmul(iwu_4pi, res, add);
+        zl[0] += add[0] * cost;
+        zl[1] += add[1] * cost;
+    }
+    return 0;
+}
+
+/* Scalars of src/electrode.c:313-319 (direct) and :391-399 (images, with the
+ * reflection coefficients); unit scalars for INTG_MHEM and INTG_NONE. */
+static void z_scalars(const double s[2], double mur, const double kappa[2],
+                      const double ref_l[2], const double ref_t[2], int integration_type,
+                      double iwu_4pi[2], double one_4pik[2])
+{
+    if (integration_type == OR_INTG_MHEM || integration_type == OR_INTG_NONE) {
+        iwu_4pi[0] = 1.0;
+        iwu_4pi[1] = 0.0;
+        one_4pik[0] = 1.0;
+        one_4pik[1] = 0.0;
+        return;
+    }
+    double t[2] = {s[0] * mur * OR_MU0 / OR_FOUR_PI, s[1] * mur * OR_MU0 / OR_FOUR_PI};
+    double fk[2] = {OR_FOUR_PI * kappa[0], OR_FOUR_PI * kappa[1]};
+    if (ref_l) {
+        cmul(ref_l, t, iwu_4pi);
+        cdiv(ref_t, fk, one_4pik);
+    } else {
+        double one[2] = {1.0, 0.0};
+        iwu_4pi[0] = t[0];
+        iwu_4pi[1] = t[1];
+        cdiv(one, fk, one_4pik);
+    }
+}
+
 int or_calculate_impedances(double *zl, double *zt, const or_electrode *e, size_t m,
                             const double gamma[2], const double s[2], double mur,
                             const double kappa[2], size_t max_eval, double abs_tol,
                             double rel_tol, int integration_type, int nthreads)
 {
     double iwu_4pi[2], one_4pik[2];
-    if (integration_type == OR_INTG_MHEM || integration_type == OR_INTG_NONE) {
-        iwu_4pi[0] = 1.0;
-        iwu_4pi[1] = 0.0;
-        one_4pik[0] = 1.0;
-        one_4pik[1] = 0.0;
-    } else {
-        iwu_4pi[0] = s[0] * mur * OR_MU0 / OR_FOUR_PI;
-        iwu_4pi[1] = s[1] * mur * OR_MU0 / OR_FOUR_PI;
-        double one[2] = {1.0, 0.0}, fk[2] = {OR_FOUR_PI * kappa[0], OR_FOUR_PI * kappa[1]};
-        cdiv(one, fk, one_4pik);
-    }
+    z_scalars(s, mur, kappa, NULL, NULL, integration_type, iwu_4pi, one_4pik);
     int failure = 0;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
 #endif
     for (long ii = 0; ii < (long)m; ii++) {
         size_t i = (size_t)ii;
-        double ls = e[i].length, k1[3], res[2], err[2], tmp[2];
+        double ls = e[i].length, tmp[2];
         double intg[2] = {or_self_integral(&e[i]), 0.0};
         cmul(iwu_4pi, intg, zl + 2 * (i * m + i));
         tmp[0] = one_4pik[0] / (ls * ls);
         tmp[1] = one_4pik[1] / (ls * ls);
         cmul(tmp, intg, zt + 2 * (i * m + i));
-        for (int c = 0; c < 3; c++) k1[c] = e[i].end_point[c] - e[i].start_point[c];
         for (size_t k = i + 1; k < m; k++) {
-            double lr = e[k].length, cost = 0.0;
-            for (int c = 0; c < 3; c++)
-                cost += k1[c] * (e[k].end_point[c] - e[k].start_point[c]);
-            cost = fabs(cost);
-            int f = or_integral(&e[i], &e[k], gamma, max_eval, abs_tol, rel_tol,
-                                integration_type, res, err);
+            int f = z_direct_entry(e, i, k, gamma, iwu_4pi, one_4pik, max_eval, abs_tol,
+                                   rel_tol, integration_type, zl + 2 * (i * m + k),
+                                   zt + 2 * (i * m + k));
             if (f) {
 #ifdef _OPENMP
 #pragma omp atomic write
 #endif
                 failure = f;
-                continue;
-            }
-            tmp[0] = one_4pik[0] / (ls * lr);
-            tmp[1] = one_4pik[1] / (ls * lr);
-            cmul(tmp, res, zt + 2 * (i * m + k));
-            if (cost < DBL_EPSILON) {
-                zl[2 * (i * m + k)] = 0.0;
-                zl[2 * (i * m + k) + 1] = 0.0;
-            } else {
-                cost = cost / (ls * lr);
-                cmul(iwu_4pi, res, tmp);
-                zl[2 * (i * m + k)] = tmp[0] * cost;
-                zl[2 * (i * m + k) + 1] = tmp[1] * cost;
             }
         }
     }
@@ -648,51 +711,74 @@ int or_impedances_images(double *zl, double *zt, const or_electrode *e,
                          int integration_type, int nthreads)
 {
     double iwu_4pi[2], one_4pik[2];
-    if (integration_type == OR_INTG_MHEM || integration_type == OR_INTG_NONE) {
-        iwu_4pi[0] = 1.0;
-        iwu_4pi[1] = 0.0;
-        one_4pik[0] = 1.0;
-        one_4pik[1] = 0.0;
-    } else {
-        double t[2] = {s[0] * mur * OR_MU0 / OR_FOUR_PI, s[1] * mur * OR_MU0 / OR_FOUR_PI};
-        cmul(ref_l_in, t, iwu_4pi);
-        double fk[2] = {OR_FOUR_PI * kappa[0], OR_FOUR_PI * kappa[1]};
-        cdiv(ref_t_in, fk, one_4pik);
-    }
+    z_scalars(s, mur, kappa, ref_l_in, ref_t_in, integration_type, iwu_4pi, one_4pik);
     int failure = 0;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads > 0 ? nthreads : 1)
 #endif
     for (long ii = 0; ii < (long)m; ii++) {
         size_t i = (size_t)ii;
-        double ls = e[i].length, k1[3], res[2], err[2], tmp[2];
-        for (int c = 0; c < 3; c++) k1[c] = e[i].end_point[c] - e[i].start_point[c];
         for (size_t k = i; k < m; k++) {
-            double lr = images[k].length, cost = 0.0;
-            for (int c = 0; c < 3; c++)
-                cost += k1[c] * (images[k].end_point[c] - images[k].start_point[c]);
-            cost = fabs(cost);
-            int f = or_integral(&e[i], &images[k], gamma, max_eval, abs_tol, rel_tol,
-                                integration_type, res, err);
+            int f = z_image_entry(e, images, i, k, gamma, iwu_4pi, one_4pik, max_eval, abs_tol,
+                                  rel_tol, integration_type, zl + 2 * (i * m + k),
+                                  zt + 2 * (i * m + k));
             if (f) {
 #ifdef _OPENMP
 #pragma omp atomic write
 #endif
                 failure = f;
-                continue;
             }
-            tmp[0] = one_4pik[0] / (ls * lr);
-            tmp[1] = one_4pik[1] / (ls * lr);
-            double add[2];
-            cmul(tmp, res, add);
-            zt[2 * (i * m + k)] += add[0];
-            zt[2 * (i * m + k) + 1] += add[1];
-            if (cost > DBL_EPSILON) {
-                cost = cost / (ls * lr);
-                cmul(iwu_4pi, res, add);
-                zl[2 * (i * m + k)] += add[0] * cost;
-                zl[2 * (i * m + k) + 1] += add[1] * cost;
-            }
+        }
+    }
+    return failure;
+}
+
+/* Lower-half entries (i <= k; a pair given as i > k is swapped) of Z_L and Z_T
+ * after calculate_impedances then impedances_images, for a list of pairs:
+ * the same per-entry arithmetic as the two full-matrix routines above (the
+ * diagonal from the closed form), so each entry equals theirs bit for bit. */
+int or_z_entries(const or_electrode *e, const or_electrode *images, size_t m, const long *pairs,
+                 size_t npairs, const double gamma[2], const double s[2], double mur,
+                 const double kappa[2], const double ref_l[2], const double ref_t[2],
+                 double rel_tol, double *zl, double *zt, int nthreads)
+{
+    double iwu_d[2], fk_d[2], iwu_i[2], fk_i[2];
+    z_scalars(s, mur, kappa, NULL, NULL, OR_INTG_DOUBLE, iwu_d, fk_d);
+    z_scalars(s, mur, kappa, ref_l, ref_t, OR_INTG_DOUBLE, iwu_i, fk_i);
+    int failure = 0;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 16) num_threads(nthreads > 0 ? nthreads : 1)
+#endif
+    for (long p = 0; p < (long)npairs; p++) {
+        size_t i = (size_t)pairs[2 * p], k = (size_t)pairs[2 * p + 1];
+        if (i > k) {
+            size_t t = i;
+            i = k;
+            k = t;
+        }
+        double *l = zl + 2 * p, *t = zt + 2 * p;
+        int f = 0;
+        if (i >= m || k >= m) {
+            f = -1;
+        } else if (i == k) {
+            double ls = e[i].length, tmp[2];
+            double intg[2] = {or_self_integral(&e[i]), 0.0};
+            cmul(iwu_d, intg, l);
+            tmp[0] = fk_d[0] / (ls * ls);
+            tmp[1] = fk_d[1] / (ls * ls);
+            cmul(tmp, intg, t);
+        } else {
+            f = z_direct_entry(e, i, k, gamma, iwu_d, fk_d, 0, 0.0, rel_tol, OR_INTG_DOUBLE, l,
+                               t);
+        }
+        if (!f)
+            f = z_image_entry(e, images, i, k, gamma, iwu_i, fk_i, 0, 0.0, rel_tol,
+                              OR_INTG_DOUBLE, l, t);
+        if (f) {
+#ifdef _OPENMP
+#pragma omp atomic write
+#endif
+            failure = f;
         }
     }
     return failure;

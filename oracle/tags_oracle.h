@@ -94,6 +94,11 @@ int or_fill_impedance_imm(double *wg, const double *zl, const double *zt, size_t
 
 /* Direct + image integrals of a fixed list of pairs, for the CPU timing sample:
  * pairs[2*p] = i, pairs[2*p+1] = k; out[4*p..] = (direct re, im, image re, im). */
+/* lower-half Z_L, Z_T entries (direct + image) of a list of pairs */
+int or_z_entries(const or_electrode *e, const or_electrode *images, size_t m, const long *pairs,
+                 size_t npairs, const double gamma[2], const double s[2], double mur,
+                 const double kappa[2], const double ref_l[2], const double ref_t[2],
+                 double rel_tol, double *zl, double *zt, int nthreads);
 int or_pair_sample(const or_electrode *e, const or_electrode *images, const long *pairs,
                    size_t npairs, const double gamma[2], double rel_tol, double *out,
                    int nthreads);

@@ -237,3 +237,26 @@ def test_sunde_dc_resistance():
     R = rho / (math.pi * L) * (math.log(2 * L / math.sqrt(2 * a * d)) - 1)
     assert abs(zh.real - R) < 0.02 * R
     assert abs(zh.imag) < 0.01 * R
+
+
+def test_z_entries_match_full_matrices():
+    """The pair sampler gives exactly the entries of calculate_impedances +
+    impedances_images (lower half, k >= i, pairs given either way round),
+    including diagonals, touching and perpendicular pairs, on a small grid and
+    a vertical rod (images flip the segment direction)."""
+    for e in (_grid_small()[0], O.segment_electrode([0, 0, -0.5], [0, 0, -6.5], 12, 8e-3)[0]):
+        m = len(e)
+        img = O.images_of(e)
+        sc = O.soil_scalars(2e5, 1e-3, 15.0)
+        rc, zl, zt = O.calculate_impedances(e, sc["gamma"], sc["s"], 1.0, sc["kappa"],
+                                            rel_tol=1e-8, threads=4)
+        assert rc == 0
+        O.impedances_images(zl, zt, e, img, sc["gamma"], sc["s"], 1.0, sc["kappa"], sc["ref_l"],
+                            sc["ref_t"], rel_tol=1e-8, threads=4)
+        rng = np.random.default_rng(5)
+        pairs = np.vstack([np.column_stack([np.arange(m), np.arange(m)]),
+                           rng.integers(0, m, (200, 2))])
+        ezl, ezt = O.z_entries(e, img, pairs, sc, rel_tol=1e-8, threads=4)
+        lo, hi = pairs.min(1), pairs.max(1)
+        assert np.array_equal(ezl, zl[lo * m + hi])
+        assert np.array_equal(ezt, zt[lo * m + hi])

@@ -14,9 +14,12 @@ missing, and every call runs the HIP path.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes
 import os
 import subprocess
+import sys
+import weakref
 
 import numpy as np
 
@@ -46,22 +49,45 @@ MU0 = 1.256637061435917e-6
 EPS0 = 8.854187817620e-12
 
 
+HIPCC_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
+               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+               "-Wno-unused-result", "-Wno-unused-value"]
+OBJ_DIR = os.path.join(HERE, "_build")
+
+
 def hipcc_command(out=LIB_PATH):
-    return (["hipcc", "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
-             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-             "-Wno-unused-result", "-Wno-unused-value",
-             "-I" + os.path.join(ROOT, "include"), "-I" + CSRC, "-o", out] +
+    """One hipcc line for the whole library (what build() runs per source)."""
+    return (["hipcc"] + HIPCC_FLAGS + ["-shared", "-I" + os.path.join(ROOT, "include"),
+                                       "-I" + CSRC, "-o", out] +
             [os.path.join(CSRC, s) for s in SOURCES])
 
 
 def build(verbose=False) -> str:
-    """Compile libtags_hip.so for gfx950 in-tree (cross-compiles without a GPU)."""
+    """Compile libtags_hip.so for gfx950 in-tree (cross-compiles without a GPU):
+    every source to an object in parallel, then one link."""
+    from concurrent.futures import ThreadPoolExecutor
+    os.makedirs(OBJ_DIR, exist_ok=True)
+    inc = ["-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
+
+    def compile_one(src):
+        obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".o")
+        r = subprocess.run(["hipcc"] + HIPCC_FLAGS + inc + ["-c", os.path.join(CSRC, src),
+                                                            "-o", obj],
+                           capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError("hipcc failed on %s:\n%s%s" % (src, r.stdout, r.stderr))
+        return obj, r.stderr
+
+    workers = max(1, min(len(SOURCES), os.cpu_count() or 1, 8))
+    with ThreadPoolExecutor(workers) as ex:
+        res = list(ex.map(compile_one, SOURCES))
     tmp = LIB_PATH + ".tmp"
-    r = subprocess.run(hipcc_command(tmp), capture_output=True, text=True)
+    r = subprocess.run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] +
+                       [o for o, _ in res], capture_output=True, text=True)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError("hipcc link failed:\n" + r.stdout + r.stderr)
     if verbose:
-        print(r.stderr)
+        print("".join(e for _, e in res) + r.stderr)
     os.replace(tmp, LIB_PATH)
     return LIB_PATH
 
@@ -89,7 +115,7 @@ class Grid(ctypes.Structure):
 class ItemStats(ctypes.Structure):
     _fields_ = [("precision_used", ctypes.c_int), ("refine_iters", ctypes.c_int),
                 ("info", ctypes.c_int), ("backward_err", ctypes.c_double),
-                ("refine_hist", ctypes.c_double * 8)]
+                ("refine_hist", ctypes.c_double * 8), ("forward_est", ctypes.c_double)]
 
 
 class Timers(ctypes.Structure):
@@ -118,7 +144,8 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_laplace_batch", "tags_inv_laplace_batch", "electrodes_file", "nodes_file",
             "tags_count_lines", "tags_frequencies_file", "tags_write_complex_file",
             "electric_potential", "magnetic_potential", "electric_field", "voltage",
-            "tags_fields_batch", "tags_mhem_gpd"]
+            "tags_fields_batch", "tags_mhem_gpd", "tags_sweep_set_verify",
+            "tags_sweep_assemble_entries"]
 
 _lib = None
 
@@ -190,6 +217,8 @@ def lib():
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
         "tags_sweep_set_inflight": ([P, I], I),
+        "tags_sweep_set_verify": ([P, I], I),
+        "tags_sweep_assemble_entries": ([P, C, C, D, S, P, P, P], I),
         "tags_sweep_assemble": ([P, C, C, D, P], I),
         "tags_immittance_solve": ([P, P, S, S, I, I, P], I),
     }
@@ -528,6 +557,20 @@ def soil_scalars(freq, sigma, epsr, mur=1.0):
 # ---------------------------------------------------------------------------
 # device-resident sweep
 # ---------------------------------------------------------------------------
+_live = weakref.WeakSet()
+
+
+@atexit.register
+def _close_live():
+    """Destroys the sweeps still open at interpreter exit, while the HIP
+    runtime is still up (atexit hooks run before module teardown)."""
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
 class Sweep:
     """Device-resident frequency sweep of one grounding geometry."""
 
@@ -542,6 +585,7 @@ class Sweep:
             raise RuntimeError("tags_sweep_create failed: %d" % rc)
         self.m, self.n = len(e), len(nodes)
         self.N = self.n + 2 * self.m
+        _live.add(self)
 
     def solve(self, s, kappa, inj_node=0, precision=PREC_FP64, mur=1.0, gamma=None,
               ref_l=None, ref_t=None, want_x=False):
@@ -559,7 +603,7 @@ class Sweep:
         if rc != 0:
             raise RuntimeError("tags_sweep_solve failed: %d" % rc)
         st = [dict(precision_used=q.precision_used, refine_iters=q.refine_iters, info=q.info,
-                   backward_err=q.backward_err,
+                   backward_err=q.backward_err, forward_est=q.forward_est,
                    refine_hist=[v for v in q.refine_hist if v >= 0]) for q in stats]
         return (u, st, x) if want_x else (u, st)
 
@@ -570,6 +614,19 @@ class Sweep:
         if rc:
             raise RuntimeError("tags_sweep_assemble failed: %d" % rc)
         return w
+
+    def assemble_entries(self, s, kappa, rows, cols, mur=1.0):
+        """W(rows[i], cols[i]) of one sample as assembled on the GPU (reference
+        unknown order)."""
+        rows = np.ascontiguousarray(rows, np.int64)
+        cols = np.ascontiguousarray(cols, np.int64)
+        out = np.zeros(len(rows), np.complex128)
+        rc = lib().tags_sweep_assemble_entries(self._h, CDouble.of(s), CDouble.of(kappa),
+                                               float(mur), len(rows), _p(rows), _p(cols),
+                                               _p(out))
+        if rc:
+            raise RuntimeError("tags_sweep_assemble_entries failed: %d" % rc)
+        return out
 
     def set_plan(self, plan):
         rc = lib().tags_sweep_set_plan(self._h, plan)
@@ -588,6 +645,12 @@ class Sweep:
         rc = lib().tags_sweep_set_formulation(self._h, int(formulation))
         if rc:
             raise RuntimeError("tags_sweep_set_formulation failed: %d" % rc)
+
+    def set_verify(self, on=True):
+        """FP64 form: report each sample's backward error (W assembled again)."""
+        rc = lib().tags_sweep_set_verify(self._h, 1 if on else 0)
+        if rc:
+            raise RuntimeError("tags_sweep_set_verify failed: %d" % rc)
 
     def set_inflight(self, samples):
         rc = lib().tags_sweep_set_inflight(self._h, samples)
@@ -608,6 +671,10 @@ class Sweep:
             self._h = ctypes.c_void_p()
 
     def __del__(self):
+        # no HIP calls during interpreter finalization (the runtime's exit
+        # handlers may already have run); _close_live runs before that
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:

@@ -113,14 +113,26 @@ static PairScalars make_scalars(c64 gamma, c64 s, double mur, c64 kappa, c64 ref
 // ---------------------------------------------------------------------------
 struct Solver {
     static constexpr int kSpecIters = 3;   // refinement steps queued before the first check
-    static constexpr int kMaxIters = 12;   // refinement steps in all
-    // backward error at which refinement stops (about 2 FP64 ulps: the mixed
-    // solution is then at least as accurate as a pivoted FP64 factor's)
+    static constexpr int kMaxIters = 20;   // refinement steps in all
+    // Acceptance needs both a small backward error |b - W x| / (|W| |x|) and a
+    // small forward-error estimate |dx| / |x|, dx the last correction. The
+    // backward error alone does not bound the error of x for an ill-conditioned
+    // W (cond_1 ~ 6e8 on the 30 x 30 m grid at 31 kHz: a backward error of
+    // 1e-15 still allows ~1e-7). The last correction is the error the previous
+    // iterate had; while the refinement contracts, the accepted iterate's error
+    // is smaller than that.
+    static constexpr double kAccept = 1e-15;      // backward error, a few FP64 ulps
+    static constexpr double kFwdAccept = 1e-11;   // |dx| / |x|
+    // refinement stops at a backward error <= kConverged (about 2 FP64 ulps)
+    // with |dx| / |x| <= kFwdAccept, or at a backward error <= kAccept with
+    // |dx| / |x| <= kFwdStop, or when a step gains less than a factor kMinGain
+    // in both measures
     static constexpr double kConverged = 2e-16;
-    // an iterate is accepted at a backward error <= kAccept (a few FP64 ulps);
-    // refinement goes on while each step gains at least a factor kMinGain
-    static constexpr double kAccept = 1e-15;
+    static constexpr double kFwdStop = 1e-13;
     static constexpr double kMinGain = 0.8;
+    // norms per refinement step k at norms[kNS k ..]: |r|, |x|, |W| (k = 0 only)
+    // and |dx| (the correction of step k), as bit patterns of non-negative doubles
+    static constexpr int kNS = 4;
     int n = 0;
     hipStream_t st = nullptr;
     Profiler* pr = nullptr;
@@ -132,8 +144,8 @@ struct Solver {
     c64* r = nullptr;
     c64* b = nullptr;
     c32* t32 = nullptr;
-    // residual norms of refinement step k at norms[3k ..]: |r|, |x| and (k = 0) |W|,
-    // as the bit patterns of non-negative doubles; hnorms is its pinned copy
+    // residual and correction norms (kNS per refinement step); hnorms is its
+    // pinned copy
     unsigned long long* norms = nullptr;
     unsigned long long* hnorms = nullptr;
     int* hflags = nullptr;  // pinned copy of a factorisation's (info, barrier error)
@@ -160,8 +172,9 @@ struct Solver {
             TAGS_HIP(hipMalloc(&r, sizeof(c64) * n));
             TAGS_HIP(hipMalloc(&b, sizeof(c64) * n));
             TAGS_HIP(hipMalloc(&t32, sizeof(c32) * n));
-            TAGS_HIP(hipMalloc(&norms, sizeof(unsigned long long) * 3 * (kMaxIters + 1)));
-            TAGS_HIP(hipHostMalloc(&hnorms, sizeof(unsigned long long) * 3 * (kMaxIters + 1), 0));
+            TAGS_HIP(hipMalloc(&norms, sizeof(unsigned long long) * kNS * (kMaxIters + 1)));
+            TAGS_HIP(hipHostMalloc(&hnorms, sizeof(unsigned long long) * kNS * (kMaxIters + 1),
+                                   0));
             TAGS_HIP(hipHostMalloc(&hflags, sizeof(int) * 2, 0));
         }
         int rc = w64.alloc(n, di.num_cu, di.lds_limit);
@@ -209,7 +222,7 @@ struct Solver {
     double norm_at(int k, int i) const
     {
         double v;
-        std::memcpy(&v, &hnorms[3 * k + i], 8);
+        std::memcpy(&v, &hnorms[kNS * k + i], 8);
         return v;
     }
 
@@ -236,7 +249,8 @@ struct Solver {
             stt->precision_used = TAGS_PREC_FP64;
             stt->refine_iters = 0;
             stt->info = info;
-            stt->backward_err = -1.0;
+            stt->backward_err = -1.0;  // W was factored in place (a sweep can verify)
+            stt->forward_est = -1.0;
         }
         return info;
     }
@@ -248,17 +262,18 @@ struct Solver {
 
     // One refinement step k >= 1 queued: correction from the complex64 factor
     // with the right-hand side scaled by |r_{k-1}| (read on the device),
-    // x += correction, r_k = b - W x and its norms into slot k.
+    // x += correction (its norm into slot k), r_k = b - W x and its norms into
+    // slot k (zeroed when the solve was issued).
     int queue_step(c64* W, int k, const Grounding* gr)
     {
         int rc;
-        const unsigned long long* rn = norms + 3 * (k - 1);
+        const unsigned long long* rn = norms + kNS * (k - 1);
         if ((rc = prep_rhs32(r, dscale, w32.perm, n, 1.0, t32, st, gr,
                              gr ? gsum + 1 : nullptr, rn)))
             return rc;
         if ((rc = lu_solve_inplace<float>(w32, W32, n, t32, st, pr))) return rc;
-        if ((rc = update_x(x, t32, dscale, n, 1.0, st, gr, rn))) return rc;
-        return residual64(W, n, n, x, b, r, norms + 3 * k, false, st, pr);
+        if ((rc = update_x(x, t32, dscale, n, 1.0, st, gr, rn, norms + kNS * k + 3))) return rc;
+        return residual64(W, n, n, x, b, r, norms + kNS * k, false, st, pr, false);
     }
 
     // Mixed form, first half: W (FP64, kept intact unless the FP64 fallback
@@ -297,26 +312,29 @@ struct Solver {
         if ((rc = lu_factor<float>(w32, W32, n, st, pr, plan32 ? plan32 : plan))) return rc;
         TAGS_HIP(hipMemcpyAsync(hflags, w32.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
         TAGS_HIP(hipMemcpyAsync(b, bvec, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
-        TAGS_HIP(hipMemsetAsync(x, 0, sizeof(c64) * n, st));
+        if ((rc = fill_c64(x, n, c64{0.0, 0.0}, st))) return rc;
+        // every step's norms start at zero (max-reductions)
+        if ((rc = fill_c64((c64*)norms, kNS * (kMaxIters + 1) / 2, c64{0.0, 0.0}, st)))
+            return rc;
         // r_0 = b (x = 0), with |W|
-        if ((rc = residual64(W, n, n, x, b, r, norms, true, st, pr))) return rc;
+        if ((rc = residual64(W, n, n, x, b, r, norms, true, st, pr, false))) return rc;
         const Grounding* gr = grounded ? &ground : nullptr;
         for (int k = 1; k <= kSpecIters; k++)
             if ((rc = queue_step(W, k, gr))) return rc;
         queued = kSpecIters;
-        TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * 3 * (queued + 1),
+        TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * kNS * (queued + 1),
                                 hipMemcpyDeviceToHost, st));
         return 0;
     }
 
     // Second half: waits and checks the queued steps by the backward error
-    // |b - W x| / (|W| |x|) of each iterate; while it is above kConverged and
-    // still falling by a factor kMinGain or better per step, refines further
-    // one step at a time (up to kMaxIters). The iterate in x (the last step
-    // run) is accepted at <= kAccept. Otherwise, or after a complex64
-    // breakdown, the solve starts again with plan32_retry's complex64 factor
-    // if one is set, else the FP64 factor of the intact W gives the answer.
-    // Solution returned in `bvec`.
+    // |b - W x| / (|W| |x|) and the forward-error estimate |dx| / |x| of each
+    // iterate, refines further one step at a time while the stop rule (see
+    // kConverged) does not hold (up to kMaxIters), and accepts the last iterate
+    // at a backward error <= kAccept and |dx| / |x| <= kFwdAccept. Otherwise,
+    // or after a complex64 breakdown, the solve starts again with
+    // plan32_retry's complex64 factor if one is set, else the FP64 factor of
+    // the intact W gives the answer. Solution returned in `bvec`.
     int finish_mixed(c64* W, c64* bvec, tags_item_stats* stt)
     {
         TAGS_HIP(hipStreamSynchronize(st));
@@ -332,30 +350,40 @@ struct Solver {
             if (rn == 0.0) return 0.0;
             return (wnorm > 0 && xn > 0) ? rn / (wnorm * xn) : 1.0;
         };
+        auto fwd_at = [&](int k) {
+            const double dn = norm_at(k, 3), xn = norm_at(k, 1);
+            if (dn == 0.0) return 0.0;
+            return xn > 0 ? dn / xn : 1.0;
+        };
         for (int k = 1; k <= queued && stt && k <= 8; k++) stt->refine_hist[k - 1] = berr_at(k);
         int done = queued;
-        double berr = berr_at(done);
+        double berr = berr_at(done), fwd = fwd_at(done);
         auto keep_going = [&]() {
-            if (info32 != 0 || !(berr == berr) || berr <= kConverged || done >= kMaxIters)
+            if (info32 != 0 || !(berr == berr) || !(fwd == fwd) || done >= kMaxIters)
                 return false;
-            return done < 2 || berr <= kMinGain * berr_at(done - 1);
+            if (berr <= kConverged && fwd <= kFwdAccept) return false;
+            if (berr <= kAccept && fwd <= kFwdStop) return false;
+            return done < 2 || berr <= kMinGain * berr_at(done - 1) ||
+                   fwd <= kMinGain * fwd_at(done - 1);
         };
         while (keep_going()) {
             int rc = queue_step(W, done + 1, gr);
             if (rc) return rc;
-            TAGS_HIP(hipMemcpyAsync(hnorms + 3 * (done + 1), norms + 3 * (done + 1),
-                                    sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, st));
+            TAGS_HIP(hipMemcpyAsync(hnorms + kNS * (done + 1), norms + kNS * (done + 1),
+                                    sizeof(unsigned long long) * kNS, hipMemcpyDeviceToHost, st));
             TAGS_HIP(hipStreamSynchronize(st));
             done++;
             berr = berr_at(done);
+            fwd = fwd_at(done);
             if (stt && done <= 8) stt->refine_hist[done - 1] = berr;
         }
-        if (info32 == 0 && berr == berr && berr <= kAccept) {
+        if (info32 == 0 && berr == berr && berr <= kAccept && fwd <= kFwdAccept) {
             if (stt) {
                 stt->precision_used = TAGS_PREC_MIXED;
                 stt->refine_iters = done;
                 stt->info = 0;
                 stt->backward_err = berr;
+                stt->forward_est = fwd;
             }
             TAGS_HIP(hipMemcpyAsync(bvec, x, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
             return 0;
@@ -558,7 +586,7 @@ int tags_set_update_chunks(int n)
 
 int tags_set_step_width(int columns)
 {
-    if (columns != 128 && columns != 256) return TAGS_ERR_ARG;
+    if (columns != 0 && columns != 128 && columns != 256) return TAGS_ERR_ARG;
     lu_set_step_width(columns);
     return 0;
 }
@@ -1423,7 +1451,7 @@ constexpr int kHybridMaxIters = 10;
 
 struct tags_sweep {
     long m = 0, n = 0, N = 0;
-    int plan_kind = TAGS_PLAN_STRUCTURED;
+    int plan_kind = TAGS_PLAN_HYBRID;
     int layout_kind = -1;  // plan kind the current layout and grounding sums belong to
     int inflight = 2;      // frequency samples in flight
     tags::LUPlan plan;
@@ -1445,6 +1473,9 @@ struct tags_sweep {
     // Z_L, Z_T; real m x m each), computed at the first sample after the switch
     int formulation = TAGS_FORM_HEM;
     double* pot = nullptr;
+    // FP64 form: after each solve, assemble W again (the factor replaced it)
+    // and report the backward error of x in the item statistics
+    bool verify = false;
 };
 
 // The first k slots exist and are allocated for order N (precision `mixed`).
@@ -1661,6 +1692,59 @@ int tags_sweep_assemble(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, doub
     return 0;
 }
 
+}  // extern "C"
+
+namespace tags {
+// out[i] = W[idx[i]] (flat storage index)
+__global__ void k_gather_entries(const c64* W, const long* idx, long count, c64* out)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < count) out[i] = W[idx[i]];
+}
+}  // namespace tags
+
+extern "C" {
+
+int tags_sweep_assemble_entries(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, double mur,
+                                size_t count, const long* rows, const long* cols,
+                                TAGS_COMPLEX* out)
+{
+    if (!sw || (count && (!rows || !cols || !out))) return TAGS_ERR_ARG;
+    const long N = sw->N;
+    const BlockLayout& L = sw->layout;
+    sweep_layout(sw);
+    // unknown index in the reference order [U; I_L; I_T] -> storage index
+    auto pos = [&](long u) {
+        if (u < sw->n) return L.node_off + u;
+        if (u < sw->n + sw->m) return L.zl_off + (u - sw->n);
+        return L.zt_off + (u - sw->n - sw->m);
+    };
+    std::vector<long> idx(count ? count : 1);
+    for (size_t i = 0; i < count; i++) {
+        if (rows[i] < 0 || rows[i] >= N || cols[i] < 0 || cols[i] >= N) return TAGS_ERR_ARG;
+        idx[i] = pos(rows[i]) * N + pos(cols[i]);
+    }
+    DevBuf<c64> W, dout;
+    DevBuf<long> didx;
+    int rc = sweep_potentials(sw);
+    if (rc || (rc = W.alloc((size_t)N * N)) || (rc = dout.alloc(count)) ||
+        (rc = didx.alloc(count)))
+        return rc;
+    if ((rc = sweep_fill(sw, W.p, sw->st, nullptr, s, kappa, mur, nullptr, nullptr, nullptr)))
+        return rc;
+    if (count) {
+        TAGS_HIP(hipMemcpyAsync(didx.p, idx.data(), sizeof(long) * count, hipMemcpyHostToDevice,
+                                sw->st));
+        hipLaunchKernelGGL(k_gather_entries, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                           sw->st, W.p, didx.p, (long)count, dout.p);
+        TAGS_HIP(hipGetLastError());
+        TAGS_HIP(hipMemcpyAsync(out, dout.p, sizeof(c64) * count, hipMemcpyDeviceToHost,
+                                sw->st));
+    }
+    TAGS_HIP(hipStreamSynchronize(sw->st));
+    return 0;
+}
+
 int tags_sweep_set_plan(tags_sweep* sw, int plan)
 {
     if (!sw || plan < 0 || plan > TAGS_PLAN_HYBRID) return TAGS_ERR_ARG;
@@ -1768,9 +1852,42 @@ static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const TAGS_COMPL
     return mixed ? sl.sv.issue_mixed(sl.W.p, sl.rhs.p) : sl.sv.issue64(sl.W.p, sl.rhs.p);
 }
 
-// Finishes the sample in flight on slot `sl` and writes its results.
+// Backward error of the FP64 form's solution of sample f (in sl.rhs): W is
+// assembled again (the factor replaced it) and r = e_inj - W x taken in FP64.
+static int sweep_verify64(tags_sweep* sw, SweepSlot& sl, size_t f, const TAGS_COMPLEX* s,
+                          const TAGS_COMPLEX* kappa, double mur, const TAGS_COMPLEX* gamma,
+                          const TAGS_COMPLEX* ref_l, const TAGS_COMPLEX* ref_t,
+                          size_t inj_node, tags_item_stats* stt)
+{
+    const long N = sw->N;
+    hipStream_t st = sl.stream.s;
+    Solver& sv = sl.sv;
+    int rc;
+    if ((rc = sweep_fill(sw, sl.W.p, st, nullptr, s[f], kappa[f], mur,
+                         gamma ? &gamma[f] : nullptr, ref_l ? &ref_l[f] : nullptr,
+                         ref_t ? &ref_t[f] : nullptr)))
+        return rc;
+    if ((rc = fill_c64(sv.b, N, c64{0.0, 0.0}, st)) ||
+        (rc = fill_c64(sv.b + sw->layout.node_off + inj_node, 1, c64{1.0, 0.0}, st)) ||
+        (rc = fill_c64((c64*)sv.norms, 2, c64{0.0, 0.0}, st)) ||
+        (rc = residual64(sl.W.p, N, (int)N, sl.rhs.p, sv.b, sv.r, sv.norms, true, st, nullptr,
+                         false)))
+        return rc;
+    TAGS_HIP(hipMemcpyAsync(sv.hnorms, sv.norms, sizeof(unsigned long long) * 3,
+                            hipMemcpyDeviceToHost, st));
+    TAGS_HIP(hipStreamSynchronize(st));
+    const double rn = sv.norm_at(0, 0), xn = sv.norm_at(0, 1), wn = sv.norm_at(0, 2);
+    stt->backward_err = rn == 0.0 ? 0.0 : (wn > 0 && xn > 0 ? rn / (wn * xn) : 1.0);
+    return 0;
+}
+
+// Finishes the sample in flight on slot `sl` and writes its results. A
+// singular sample (info > 0) gets NaN in u_inj and in its block of x_out.
 static int sweep_finish(tags_sweep* sw, SweepSlot& sl, size_t inj_node, bool mixed,
-                        TAGS_COMPLEX* u_inj, TAGS_COMPLEX* x_out, tags_item_stats* stats)
+                        TAGS_COMPLEX* u_inj, TAGS_COMPLEX* x_out, tags_item_stats* stats,
+                        const TAGS_COMPLEX* s, const TAGS_COMPLEX* kappa, double mur,
+                        const TAGS_COMPLEX* gamma, const TAGS_COMPLEX* ref_l,
+                        const TAGS_COMPLEX* ref_t)
 {
     const long N = sw->N, n = sw->n, m = sw->m;
     const BlockLayout& L = sw->layout;
@@ -1779,11 +1896,18 @@ static int sweep_finish(tags_sweep* sw, SweepSlot& sl, size_t inj_node, bool mix
     tags_item_stats stt{};
     int rc = mixed ? sl.sv.finish_mixed(sl.W.p, sl.rhs.p, &stt) : sl.sv.finish64(&stt);
     if (rc < 0) return rc;
+    if (rc == 0 && !mixed && sw->verify) {
+        int vr = sweep_verify64(sw, sl, f, s, kappa, mur, gamma, ref_l, ref_t, inj_node, &stt);
+        if (vr) return vr;
+    }
     if (stats) stats[f] = stt;
     if (sl.nodal && (stt.precision_used != TAGS_PREC_MIXED || stt.refine_iters > kHybridMaxIters))
         sw->nodal_above = std::max(sw->nodal_above, 1.5 * sl.hz);
     if (rc > 0) {
-        if (u_inj) u_inj[f] = to_c(c64{NAN, NAN});
+        const c64 nan = c64{NAN, NAN};
+        if (u_inj) u_inj[f] = to_c(nan);
+        if (x_out)
+            for (long i = 0; i < N; i++) x_out[f * N + i] = to_c(nan);
         return 0;
     }
     c64 u;
@@ -1819,6 +1943,9 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     int rc = sweep_slots(sw, sw->inflight, mixed);
     if (rc || (rc = sweep_potentials(sw))) return rc;
     sweep_layout(sw);
+    // the hybrid plan's switch may rise during this call; it is restored at
+    // the end, so that every call starts from the configured switch
+    const double nodal_above0 = sw->nodal_above;
     const int k = (int)std::min<size_t>((size_t)sw->inflight, std::max<size_t>(nfreq, 1));
     size_t next = 0;
     for (int i = 0; i < k && next < nfreq; i++)
@@ -1829,18 +1956,28 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     for (int i = 0; !rc; i = (i + 1) % k) {
         SweepSlot& sl = *sw->slots[i];
         if (sl.f < 0) break;  // the oldest slot is idle: every sample is done
-        if ((rc = sweep_finish(sw, sl, inj_node, mixed, u_inj, x_out, stats))) break;
+        if ((rc = sweep_finish(sw, sl, inj_node, mixed, u_inj, x_out, stats, s, kappa, mur,
+                               gamma, ref_l, ref_t)))
+            break;
         if (next < nfreq &&
             (rc = sweep_issue(sw, sl, next++, s, kappa, mur, gamma, ref_l, ref_t, inj_node,
                               mixed)))
             break;
     }
     // after an error: let every queued sample end before returning
+    sw->nodal_above = nodal_above0;
     for (int i = 0; i < k; i++) {
         TAGS_HIP(hipStreamSynchronize(sw->slots[i]->stream.s));
         sw->slots[i]->f = -1;
     }
     return rc;
+}
+
+int tags_sweep_set_verify(tags_sweep* sw, int on)
+{
+    if (!sw) return TAGS_ERR_ARG;
+    sw->verify = on != 0;
+    return 0;
 }
 
 }  // extern "C"

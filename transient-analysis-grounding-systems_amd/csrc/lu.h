@@ -169,9 +169,10 @@ int lu_panel_kind();
 void lu_set_lookahead(bool on);
 // Launches the look-ahead trailing update of a step is split into (by columns)
 void lu_set_update_chunks(int n);
-// Columns per factorisation step: 128 or 256 (default)
+// Columns per factorisation step: 128 or 256 for both precisions; w <= 0
+// restores the defaults (128 for complex128 factors, 256 for complex64)
 void lu_set_step_width(int w);
-int lu_step_width();
+int lu_step_width(bool c64);
 
 // Column phases of a factorisation (see lu.hip).
 struct LUPhase {
@@ -336,8 +337,11 @@ int launch_gemm_variant(const GemmArgs<T>& g, int variant, hipStream_t st);
 int gemm_set_variants(int precision, int big, int small);
 int gemm_variant_count(int precision);
 
+// norms[0..1] (and [2] with want_wnorm) receive |r|, |x| (|W|) as max-reductions;
+// zero_norms false: the caller has already zeroed them (no memset node)
 int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
-               unsigned long long* norms, bool want_wnorm, hipStream_t st, Profiler* pr);
+               unsigned long long* norms, bool want_wnorm, hipStream_t st, Profiler* pr,
+               bool zero_norms = true);
 int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t st);
 int to_c32_scaled(const c64* W, long ld, int n, const double* d, c32* W32, long ld32,
                   hipStream_t st);
@@ -358,8 +362,10 @@ int to_c32_grounded(const c64* W, long ld, int n, const Grounding& gr, const c64
 int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv_scale, c32* t,
                hipStream_t st, const Grounding* gr = nullptr, c64* rg = nullptr,
                const unsigned long long* rn = nullptr);
+// dxn (optional, zeroed beforehand): receives the max-norm of the correction
 int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st,
-             const Grounding* gr = nullptr, const unsigned long long* rn = nullptr);
+             const Grounding* gr = nullptr, const unsigned long long* rn = nullptr,
+             unsigned long long* dxn = nullptr);
 int fill_c64(c64* p, long n, c64 v, hipStream_t st);
 int sym_from_upper(c64* W, long ld, int n, hipStream_t st);
 

@@ -1032,20 +1032,31 @@ __global__ void k_prep_rhs32(const c64* r, const double* d, const int* perm, int
 }
 
 // x[j] += scale * d[j] * z'[j] (scale = |r| from rn when given); with grounding
-// (g >= 0) z'[j] = z[j] + z[g] on the node range except g itself (x = T y)
-__global__ void k_update_x(c64* x, const c32* z, const double* d, int n, double scale,
-                           const unsigned long long* rn, int g, int nb, int ne)
+// (g >= 0) z'[j] = z[j] + z[g] on the node range except g itself (x = T y).
+// dxn (optional, zeroed by the caller): max_j |scale d[j] z'[j]|, the size of
+// the correction, as the bit pattern of a non-negative double.
+__global__ __launch_bounds__(256) void k_update_x(c64* x, const c32* z, const double* d, int n,
+                                                   double scale, const unsigned long long* rn,
+                                                   int g, int nb, int ne,
+                                                   unsigned long long* dxn)
 {
     const int j = blockIdx.x * 256 + threadIdx.x;
-    if (j >= n) return;
-    if (rn) scale = rnorm_of(rn);
-    const double s = scale * d[j];
-    double zx = (double)z[j].x, zy = (double)z[j].y;
-    if (g >= 0 && j != g && j >= nb && j < ne) {
-        zx += (double)z[g].x;
-        zy += (double)z[g].y;
+    double a = 0.0;
+    if (j < n) {
+        if (rn) scale = rnorm_of(rn);
+        const double s = scale * d[j];
+        double zx = (double)z[j].x, zy = (double)z[j].y;
+        if (g >= 0 && j != g && j >= nb && j < ne) {
+            zx += (double)z[g].x;
+            zy += (double)z[g].y;
+        }
+        const double dx = s * zx, dy = s * zy;
+        x[j] = c64{x[j].x + dx, x[j].y + dy};
+        a = sqrt(dx * dx + dy * dy);
     }
-    x[j] = c64{x[j].x + s * zx, x[j].y + s * zy};
+    if (!dxn) return;
+    for (int off = 32; off > 0; off >>= 1) a = fmax(a, __shfl_xor(a, off));
+    if ((threadIdx.x & 63) == 0) atomicMax(dxn, (unsigned long long)__double_as_longlong(a));
 }
 
 // t[i] = b[perm[i]]
@@ -1195,9 +1206,15 @@ void lu_set_update_chunks(int n) { g_update_chunks.store(n < 1 ? 1 : n); }
 // Columns per step (four sub-panels of 64): the trailing update's K. 256 runs
 // the complex64 product ~26 % faster than 128 (the A22 read-modify-write is
 // shared by twice the flops).
-static std::atomic<int> g_step_width{256};
-void lu_set_step_width(int w) { g_step_width.store(w <= 128 ? 128 : 256); }
-int lu_step_width() { return g_step_width.load(); }
+// By default complex128 factors take 128-column steps (measured faster for
+// the FP64 form on the 60 x 60 m grid) and complex64 factors 256.
+static std::atomic<int> g_step_width[2] = {{128}, {256}};
+void lu_set_step_width(int w)
+{
+    g_step_width[0].store(w <= 0 ? 128 : w <= 128 ? 128 : 256);
+    g_step_width[1].store(w <= 0 ? 256 : w <= 128 ? 128 : 256);
+}
+int lu_step_width(bool c64) { return g_step_width[c64 ? 1 : 0].load(); }
 
 template <typename T>
 static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
@@ -1523,7 +1540,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     std::vector<LUStep> steps;
     w.blk.clear();
     w.flops = 0;
-    const int width = g_step_width.load(std::memory_order_relaxed);
+    const int width = g_step_width[sizeof(T) == 4 ? 1 : 0].load(std::memory_order_relaxed);
     for (int p = 0; p < (int)plan->phases.size(); p++) {
         const LUPhase& ph = plan->phases[p];
         for (int j = ph.c_begin; j < ph.c_end; j += width) {
@@ -1747,10 +1764,12 @@ int lu_gather_rhs(LUWork<T>& w, const cx<T>* b, cx<T>* y, hipStream_t st)
 }
 
 int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
-               unsigned long long* norms, bool want_wnorm, hipStream_t st, Profiler* pr)
+               unsigned long long* norms, bool want_wnorm, hipStream_t st, Profiler* pr,
+               bool zero_norms)
 {
     Timed tm(pr, FAM_RESID, st);
-    TAGS_HIP(hipMemsetAsync(norms, 0, sizeof(unsigned long long) * (want_wnorm ? 3 : 2), st));
+    if (zero_norms)
+        TAGS_HIP(hipMemsetAsync(norms, 0, sizeof(unsigned long long) * (want_wnorm ? 3 : 2), st));
     hipLaunchKernelGGL(k_residual, dim3(n), dim3(256), 0, st, W, ld, n, x, b, r, norms,
                        want_wnorm ? 1 : 0);
     TAGS_HIP(hipGetLastError());
@@ -1789,11 +1808,11 @@ int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv
 }
 
 int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st,
-             const Grounding* gr, const unsigned long long* rn)
+             const Grounding* gr, const unsigned long long* rn, unsigned long long* dxn)
 {
     const bool on = gr && gr->g >= 0;
     hipLaunchKernelGGL(k_update_x, dim3((n + 255) / 256), dim3(256), 0, st, x, z, d, n, scale,
-                       rn, on ? gr->g : -1, on ? gr->nb : 0, on ? gr->ne : 0);
+                       rn, on ? gr->g : -1, on ? gr->nb : 0, on ? gr->ne : 0, dxn);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
