@@ -327,9 +327,10 @@ int tags_sweep_solve(tags_sweep *sw, size_t nfreq, const TAGS_COMPLEX *s,
  *                       zero in the Z_L columns and are skipped there (fewer
  *                       flops; its complex64 factor is less accurate at the
  *                       lowest frequencies);
- * TAGS_PLAN_BLOCK_PIVOT as ZL_FIRST but the Z_L columns pivot among Z_L rows
- *                       only, which also skips the Z_T columns (fewest flops,
- *                       weakest pivoting);
+ * TAGS_PLAN_BLOCK_PIVOT stored and factored as ZL_FIRST in FP64; the mixed
+ *                       form's complex64 factor lets the Z_L columns pivot among
+ *                       Z_L rows only, which also skips the Z_T columns (fewest
+ *                       flops, weakest pivoting: refinement answers for it);
  * TAGS_PLAN_GROUNDED    stored as ZL_FIRST and factored as ZL_FIRST in FP64.
  *                       The mixed form's complex64 factor is of the grounded
  *                       matrix T^T W T (node voltages replaced by the first
@@ -436,6 +437,19 @@ int tags_sweep_set_formulation(tags_sweep *sw, int formulation);
  * or TAGS_ERR_ARG for hz < 0. */
 int tags_sweep_set_nodal_above(tags_sweep *sw, double hz);
 
+/* Frequency samples per group (0 .. 4096; 0, the default, chooses from N: one
+ * sample from N = 8192 up, else as many as keep a group's matrices within
+ * 16 GiB, at most 64). A group's samples are assembled, factored, solved and
+ * refined together: every kernel launch covers all of them (one work-group
+ * grid per sample, side by side), so that mid-size systems fill the device.
+ * Consecutive samples form a group; the hybrid plan starts a new group where
+ * its choice of complex64 factor changes. Each sample's arithmetic is its own:
+ * results agree with one-sample groups to rounding (the GEMM tile shape is
+ * chosen by the size of the whole launch), and are independent of the
+ * samples in flight. Grounded plans in the mixed form and the partial-
+ * pivoting panel take one sample at a time. No counterpart in the reference. */
+int tags_sweep_set_batch(tags_sweep *sw, int samples);
+
 /* FP64 form only: with on != 0, W is assembled again after each sample's
  * factorisation (which replaced it) and the backward error
  * ||e_inj - W x||_inf / (||W||_inf ||x||_inf) goes to the item statistics
@@ -443,12 +457,12 @@ int tags_sweep_set_nodal_above(tags_sweep *sw, double hz);
  * backward_err is -1). No counterpart in the reference. */
 int tags_sweep_set_verify(tags_sweep *sw, int on);
 
-/* Frequency samples a sweep keeps in flight (1 .. 4, default 2), each with its
- * own HIP stream, W and factor work space (about 11 GB for the 60 x 60 m grid
- * in the mixed form). The calling thread queues a sample on every slot, then
- * finishes them in issue order, queuing the next sample on each slot as it
- * frees up, so that one sample's trailing updates keep the GPU busy while
- * another waits on its panel chain. Results do not depend on the count, except
+/* Groups of frequency samples a sweep keeps in flight (1 .. 4, default 2),
+ * each with its own HIP stream, W and factor work space (about 11 GB for a
+ * one-sample group of the 60 x 60 m grid in the mixed form). The calling
+ * thread queues a group on every slot, then finishes them in issue order,
+ * queuing the next group on each slot as it frees up, so that one group's
+ * trailing updates keep the GPU busy while another waits on its panel chain. Results do not depend on the count, except
  * where TAGS_PLAN_HYBRID raises its switch during a call: samples already in
  * flight keep the factor they were issued with (both answers meet the same
  * acceptance rule). */

@@ -145,7 +145,7 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_count_lines", "tags_frequencies_file", "tags_write_complex_file",
             "electric_potential", "magnetic_potential", "electric_field", "voltage",
             "tags_fields_batch", "tags_mhem_gpd", "tags_sweep_set_verify",
-            "tags_sweep_assemble_entries"]
+            "tags_sweep_assemble_entries", "tags_sweep_set_batch"]
 
 _lib = None
 
@@ -218,6 +218,7 @@ def lib():
         "tags_sweep_set_plan": ([P, I], I),
         "tags_sweep_set_inflight": ([P, I], I),
         "tags_sweep_set_verify": ([P, I], I),
+        "tags_sweep_set_batch": ([P, I], I),
         "tags_sweep_assemble_entries": ([P, C, C, D, S, P, P, P], I),
         "tags_sweep_assemble": ([P, C, C, D, P], I),
         "tags_immittance_solve": ([P, P, S, S, I, I, P], I),
@@ -651,6 +652,12 @@ class Sweep:
         rc = lib().tags_sweep_set_verify(self._h, 1 if on else 0)
         if rc:
             raise RuntimeError("tags_sweep_set_verify failed: %d" % rc)
+
+    def set_batch(self, samples):
+        """Frequency samples per group (0: chosen from N and memory)."""
+        rc = lib().tags_sweep_set_batch(self._h, int(samples))
+        if rc:
+            raise RuntimeError("tags_sweep_set_batch failed: %d" % rc)
 
     def set_inflight(self, samples):
         rc = lib().tags_sweep_set_inflight(self._h, samples)

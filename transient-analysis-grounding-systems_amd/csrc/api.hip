@@ -110,6 +110,12 @@ static PairScalars make_scalars(c64 gamma, c64 s, double mur, c64 kappa, c64 ref
 
 // ---------------------------------------------------------------------------
 // Dense solver shared by solve_immittance, tags_dense_solve and the sweep.
+// It solves nb items at once (the frequency samples of a sweep batch): item z
+// has its matrix at W + z sW, right-hand side and solution at bvec + z n, and
+// every launch covers all items (blockIdx.z). Each item's arithmetic is that
+// of a one-item solve: refinement steps past the first kSpecIters run only on
+// the items still refining (an active mask), and an item that misses the
+// acceptance rule is retried or refactored alone.
 // ---------------------------------------------------------------------------
 struct Solver {
     static constexpr int kSpecIters = 3;   // refinement steps queued before the first check
@@ -131,9 +137,13 @@ struct Solver {
     static constexpr double kFwdStop = 1e-13;
     static constexpr double kMinGain = 0.8;
     // norms per refinement step k at norms[kNS k ..]: |r|, |x|, |W| (k = 0 only)
-    // and |dx| (the correction of step k), as bit patterns of non-negative doubles
+    // and |dx| (the correction of step k), as bit patterns of non-negative doubles;
+    // an item's norms take kNormStride
     static constexpr int kNS = 4;
+    static constexpr long kNormStride = kNS * (kMaxIters + 1);
     int n = 0;
+    int cap = 0;      // items the buffers hold
+    bool cap_mixed = false;
     hipStream_t st = nullptr;
     Profiler* pr = nullptr;
     LUWork<double> w64;
@@ -144,12 +154,14 @@ struct Solver {
     c64* r = nullptr;
     c64* b = nullptr;
     c32* t32 = nullptr;
-    // residual and correction norms (kNS per refinement step); hnorms is its
-    // pinned copy
+    // residual and correction norms; hnorms is their pinned copy
     unsigned long long* norms = nullptr;
     unsigned long long* hnorms = nullptr;
-    int* hflags = nullptr;  // pinned copy of a factorisation's (info, barrier error)
-    // grounded node unknowns of the mixed factor (ground.g < 0: off)
+    int* hflags = nullptr;  // pinned copy of the factorisations' flags (4 per item:
+                            // info, barrier error)
+    int* active = nullptr;  // items still refining (device, and pinned host copy)
+    int* hactive = nullptr;
+    // grounded node unknowns of the mixed factor (ground.g < 0: off; one item only)
     Grounding ground;
     bool ground_valid = false;  // gcol / grow / gsum hold the sums of the current W's node part
     c64* gcol = nullptr;
@@ -159,29 +171,60 @@ struct Solver {
     // the solve in flight (issue_* then finish)
     bool pending_mixed = false;
     int queued = 0;  // refinement steps queued by issue_mixed
+    int nb = 1;      // its items
+    long sW = 0;     // and their matrix stride
 
-    int ensure(int n_, bool mixed)
+    // buffers for `batch` items of order n_ (the FP64 work space for all of them
+    // in the plain form, for one in the mixed form, where it is the fallback's);
+    // kept while large enough, so that one solver serves systems of different
+    // orders in turn
+    size_t vec_cap = 0, w32_cap = 0;
+    int item_cap = 0;
+    int ensure(int n_, bool mixed, int batch = 1)
     {
         DevInfo& di = dev_info();
         if (!di.ok) return TAGS_ERR_HIP;
-        if (n_ != n) {
+        if (batch < 1 || n_ < 1) return TAGS_ERR_ARG;
+        const size_t need = (size_t)n_ * batch;
+        if (need > vec_cap || batch > item_cap) {
             free_vecs();
-            n = n_;
-            TAGS_HIP(hipMalloc(&dscale, sizeof(double) * n));
-            TAGS_HIP(hipMalloc(&x, sizeof(c64) * n));
-            TAGS_HIP(hipMalloc(&r, sizeof(c64) * n));
-            TAGS_HIP(hipMalloc(&b, sizeof(c64) * n));
-            TAGS_HIP(hipMalloc(&t32, sizeof(c32) * n));
-            TAGS_HIP(hipMalloc(&norms, sizeof(unsigned long long) * kNS * (kMaxIters + 1)));
-            TAGS_HIP(hipHostMalloc(&hnorms, sizeof(unsigned long long) * kNS * (kMaxIters + 1),
+            vec_cap = need;
+            item_cap = batch;
+            TAGS_HIP(hipMalloc(&dscale, sizeof(double) * vec_cap));
+            TAGS_HIP(hipMalloc(&x, sizeof(c64) * vec_cap));
+            TAGS_HIP(hipMalloc(&r, sizeof(c64) * vec_cap));
+            TAGS_HIP(hipMalloc(&b, sizeof(c64) * vec_cap));
+            TAGS_HIP(hipMalloc(&t32, sizeof(c32) * vec_cap));
+            TAGS_HIP(hipMalloc(&norms, sizeof(unsigned long long) * kNormStride * item_cap));
+            TAGS_HIP(hipHostMalloc(&hnorms, sizeof(unsigned long long) * kNormStride * item_cap,
                                    0));
-            TAGS_HIP(hipHostMalloc(&hflags, sizeof(int) * 2, 0));
+            TAGS_HIP(hipHostMalloc(&hflags, sizeof(int) * 4 * item_cap, 0));
+            TAGS_HIP(hipMalloc(&active, sizeof(int) * item_cap));
+            TAGS_HIP(hipHostMalloc(&hactive, sizeof(int) * item_cap, 0));
         }
-        int rc = w64.alloc(n, di.num_cu, di.lds_limit);
+        if (n_ != n) {
+            // the grounding sums belong to one system
+            hipFree(gcol);
+            hipFree(grow);
+            hipFree(gsum);
+            hipFree(gpart);
+            gcol = grow = gsum = gpart = nullptr;
+            ground_valid = false;
+        }
+        n = n_;
+        cap = batch;
+        int rc = w64.alloc(n, di.num_cu, di.lds_limit, mixed ? 1 : batch);
         if (rc) return rc;
         if (mixed) {
-            if ((rc = w32.alloc(n, di.num_cu, di.lds_limit))) return rc;
-            if (!W32) TAGS_HIP(hipMalloc(&W32, sizeof(c32) * (size_t)n * n));
+            if ((rc = w32.alloc(n, di.num_cu, di.lds_limit, batch))) return rc;
+            const size_t need32 = (size_t)n * n * batch;
+            if (need32 > w32_cap) {
+                hipFree(W32);
+                W32 = nullptr;
+                w32_cap = 0;
+                TAGS_HIP(hipMalloc(&W32, sizeof(c32) * need32));
+                w32_cap = need32;
+            }
         }
         return 0;
     }
@@ -195,6 +238,8 @@ struct Solver {
         hipFree(norms);
         if (hnorms) hipHostFree(hnorms);
         if (hflags) hipHostFree(hflags);
+        hipFree(active);
+        if (hactive) hipHostFree(hactive);
         hipFree(W32);
         hipFree(gcol);
         hipFree(grow);
@@ -208,6 +253,10 @@ struct Solver {
         t32 = nullptr;
         norms = hnorms = nullptr;
         hflags = nullptr;
+        active = hactive = nullptr;
+        cap = 0;
+        vec_cap = w32_cap = 0;
+        item_cap = 0;
     }
     ~Solver() { free_vecs(); }
 
@@ -219,40 +268,62 @@ struct Solver {
     bool equilibrate = false;      // symmetric row-max scaling before the FP32 factor
                                    // (not applied with grounding)
 
-    double norm_at(int k, int i) const
+    double norm_at(int z, int k, int i) const
     {
         double v;
-        std::memcpy(&v, &hnorms[kNS * k + i], 8);
+        std::memcpy(&v, &hnorms[kNormStride * z + kNS * k + i], 8);
+        return v;
+    }
+    double norm_at(int k, int i) const { return norm_at(0, k, i); }
+    VecBatch vb(bool masked) const
+    {
+        VecBatch v;
+        v.nb = nb;
+        v.sw = sW;
+        v.sw32 = (long)n * n;
+        v.sn = kNormStride;
+        v.active = masked ? active : nullptr;
         return v;
     }
 
-    // Plain form, first half: factor W (destroyed) in FP64, solve into w64.vec
-    // and copy x to `bvec`, all queued on the stream.
-    int issue64(c64* W, c64* bvec)
+    // Plain form, first half: factor the nb items of W (destroyed) in FP64, solve
+    // into w64.vec and copy x to `bvec`, all queued on the stream.
+    int issue64(c64* W, c64* bvec, int nb_ = 1, long sW_ = 0)
     {
         pending_mixed = false;
-        int rc = lu_factor<double>(w64, W, n, st, pr, plan);
+        if (nb_ < 1 || nb_ > w64.batch) return TAGS_ERR_ARG;
+        nb = nb_;
+        sW = nb_ > 1 ? sW_ : 0;
+        int rc = lu_factor<double>(w64, W, n, st, pr, plan, nb, sW);
         if (rc) return rc;
-        TAGS_HIP(hipMemcpyAsync(hflags, w64.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
+        TAGS_HIP(hipMemcpyAsync(hflags, w64.flags, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost,
+                                st));
         if ((rc = lu_gather_rhs<double>(w64, bvec, w64.vec, st))) return rc;
         if ((rc = lu_solve_inplace<double>(w64, W, n, w64.vec, st, pr))) return rc;
-        TAGS_HIP(hipMemcpyAsync(bvec, w64.vec, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
+        TAGS_HIP(hipMemcpyAsync(bvec, w64.vec, sizeof(c64) * n * nb, hipMemcpyDeviceToDevice,
+                                st));
         return 0;
     }
-    // Second half: waits, returns 0, info > 0 (x not valid) or an error.
-    int finish64(tags_item_stats* stt)
+    // Second half: waits; stats (nullable) and info (nullable) receive each
+    // item's; returns 0, an error, or for one item its info (> 0: x not valid).
+    int finish64(tags_item_stats* stt, int* info = nullptr)
     {
         TAGS_HIP(hipStreamSynchronize(st));
-        if (hflags[1]) return TAGS_ERR_BARRIER;
-        const int info = hflags[0];
-        if (stt) {
-            stt->precision_used = TAGS_PREC_FP64;
-            stt->refine_iters = 0;
-            stt->info = info;
-            stt->backward_err = -1.0;  // W was factored in place (a sweep can verify)
-            stt->forward_est = -1.0;
+        int first = 0;
+        for (int z = 0; z < nb; z++) {
+            if (hflags[4 * z + 1]) return TAGS_ERR_BARRIER;
+            const int inf = hflags[4 * z];
+            if (info) info[z] = inf;
+            if (!first) first = inf;
+            if (stt) {
+                stt[z].precision_used = TAGS_PREC_FP64;
+                stt[z].refine_iters = 0;
+                stt[z].info = inf;
+                stt[z].backward_err = -1.0;  // W was factored in place (a sweep can verify)
+                stt[z].forward_est = -1.0;
+            }
         }
-        return info;
+        return nb == 1 ? first : 0;
     }
     int solve64(c64* W, c64* bvec, tags_item_stats* stt)
     {
@@ -260,31 +331,38 @@ struct Solver {
         return rc ? rc : finish64(stt);
     }
 
-    // One refinement step k >= 1 queued: correction from the complex64 factor
-    // with the right-hand side scaled by |r_{k-1}| (read on the device),
-    // x += correction (its norm into slot k), r_k = b - W x and its norms into
-    // slot k (zeroed when the solve was issued).
-    int queue_step(c64* W, int k, const Grounding* gr)
+    // One refinement step k >= 1 queued for the items (the active ones when
+    // masked): correction from the complex64 factor with the right-hand side
+    // scaled by |r_{k-1}| (read on the device), x += correction (its norm into
+    // slot k), r_k = b - W x and its norms into slot k (zeroed when the solve
+    // was issued).
+    int queue_step(c64* W, int k, const Grounding* gr, bool masked)
     {
         int rc;
+        const VecBatch v = vb(masked);
         const unsigned long long* rn = norms + kNS * (k - 1);
         if ((rc = prep_rhs32(r, dscale, w32.perm, n, 1.0, t32, st, gr,
-                             gr ? gsum + 1 : nullptr, rn)))
+                             gr ? gsum + 1 : nullptr, rn, v)))
             return rc;
         if ((rc = lu_solve_inplace<float>(w32, W32, n, t32, st, pr))) return rc;
-        if ((rc = update_x(x, t32, dscale, n, 1.0, st, gr, rn, norms + kNS * k + 3))) return rc;
-        return residual64(W, n, n, x, b, r, norms + kNS * k, false, st, pr, false);
+        if ((rc = update_x(x, t32, dscale, n, 1.0, st, gr, rn, norms + kNS * k + 3, v)))
+            return rc;
+        return residual64(W, n, n, x, b, r, norms + kNS * k, false, st, pr, false, v);
     }
 
     // Mixed form, first half: W (FP64, kept intact unless the FP64 fallback
     // runs) to complex64 (grounded or scaled), its factor, r_0 = b and the
     // first kSpecIters refinement steps, all queued on the stream without a
     // host wait; `bvec` holds b.
-    int issue_mixed(c64* W, c64* bvec)
+    int issue_mixed(c64* W, c64* bvec, int nb_ = 1, long sW_ = 0)
     {
         int rc;
         pending_mixed = true;
+        if (nb_ < 1 || nb_ > w32.batch) return TAGS_ERR_ARG;
+        nb = nb_;
+        sW = nb_ > 1 ? sW_ : 0;
         const bool grounded = ground.g >= 0;
+        if (grounded && nb > 1) return TAGS_ERR_ARG;
         {
             Timed tm(pr, FAM_OTHER, st);
             if (grounded) {
@@ -304,104 +382,169 @@ struct Solver {
                 if ((rc = to_c32_grounded(W, n, n, ground, gcol, grow, gsum, W32, n, st)))
                     return rc;
             } else {
-                if ((rc = row_scale(W, n, n, dscale, equilibrate, st))) return rc;
-                if ((rc = to_c32_scaled(W, n, n, dscale, W32, n, st))) return rc;
+                if ((rc = row_scale(W, n, n, dscale, equilibrate, st, vb(false)))) return rc;
+                if ((rc = to_c32_scaled(W, n, n, dscale, W32, n, st, vb(false)))) return rc;
             }
             tm.end();
         }
-        if ((rc = lu_factor<float>(w32, W32, n, st, pr, plan32 ? plan32 : plan))) return rc;
-        TAGS_HIP(hipMemcpyAsync(hflags, w32.flags, sizeof(int) * 2, hipMemcpyDeviceToHost, st));
-        TAGS_HIP(hipMemcpyAsync(b, bvec, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
-        if ((rc = fill_c64(x, n, c64{0.0, 0.0}, st))) return rc;
-        // every step's norms start at zero (max-reductions)
-        if ((rc = fill_c64((c64*)norms, kNS * (kMaxIters + 1) / 2, c64{0.0, 0.0}, st)))
+        if ((rc = lu_factor<float>(w32, W32, n, st, pr, plan32 ? plan32 : plan, nb,
+                                   (long)n * n)))
             return rc;
+        TAGS_HIP(hipMemcpyAsync(hflags, w32.flags, sizeof(int) * 4 * nb, hipMemcpyDeviceToHost,
+                                st));
+        TAGS_HIP(hipMemcpyAsync(b, bvec, sizeof(c64) * n * nb, hipMemcpyDeviceToDevice, st));
+        if ((rc = fill_c64(x, (long)n * nb, c64{0.0, 0.0}, st))) return rc;
+        // every step's norms start at zero (max-reductions)
+        if ((rc = fill_c64((c64*)norms, kNormStride * nb / 2, c64{0.0, 0.0}, st))) return rc;
         // r_0 = b (x = 0), with |W|
-        if ((rc = residual64(W, n, n, x, b, r, norms, true, st, pr, false))) return rc;
+        if ((rc = residual64(W, n, n, x, b, r, norms, true, st, pr, false, vb(false)))) return rc;
         const Grounding* gr = grounded ? &ground : nullptr;
         for (int k = 1; k <= kSpecIters; k++)
-            if ((rc = queue_step(W, k, gr))) return rc;
+            if ((rc = queue_step(W, k, gr, false))) return rc;
         queued = kSpecIters;
-        TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * kNS * (queued + 1),
+        TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * kNormStride * nb,
                                 hipMemcpyDeviceToHost, st));
         return 0;
     }
 
-    // Second half: waits and checks the queued steps by the backward error
-    // |b - W x| / (|W| |x|) and the forward-error estimate |dx| / |x| of each
-    // iterate, refines further one step at a time while the stop rule (see
-    // kConverged) does not hold (up to kMaxIters), and accepts the last iterate
-    // at a backward error <= kAccept and |dx| / |x| <= kFwdAccept. Otherwise,
-    // or after a complex64 breakdown, the solve starts again with
-    // plan32_retry's complex64 factor if one is set, else the FP64 factor of
-    // the intact W gives the answer. Solution returned in `bvec`.
-    int finish_mixed(c64* W, c64* bvec, tags_item_stats* stt)
+    // Second half: waits and checks each item's queued steps by the backward
+    // error |b - W x| / (|W| |x|) and the forward-error estimate |dx| / |x| of
+    // each iterate, refines the items for which the stop rule (see kConverged)
+    // does not hold further, one step at a time (up to kMaxIters), and accepts an
+    // item's last iterate at a backward error <= kAccept and |dx| / |x| <=
+    // kFwdAccept. An item that fails, or whose complex64 factor broke down,
+    // starts again alone with plan32_retry's complex64 factor if one is set,
+    // else the FP64 factor of its intact W gives its answer. Solutions returned
+    // in `bvec`; stt (nullable) and info (nullable) receive each item's. Returns
+    // 0, an error, or for one item its info.
+    int finish_mixed(c64* W, c64* bvec, tags_item_stats* stt, int* info = nullptr)
     {
         TAGS_HIP(hipStreamSynchronize(st));
         pending_mixed = false;
-        if (hflags[1]) return TAGS_ERR_BARRIER;
-        if (stt)
-            for (int i = 0; i < 8; i++) stt->refine_hist[i] = -1.0;
-        const int info32 = hflags[0];
-        const double wnorm = norm_at(0, 2);
+        const int nbi = nb;
+        const long sWi = sW;
+        std::vector<int> info32(nbi), done(nbi, queued), go(nbi, 0);
+        std::vector<double> berr(nbi), fwd(nbi);
+        for (int z = 0; z < nbi; z++) {
+            if (hflags[4 * z + 1]) return TAGS_ERR_BARRIER;
+            info32[z] = hflags[4 * z];
+        }
         const Grounding* gr = ground.g >= 0 ? &ground : nullptr;
-        auto berr_at = [&](int k) {
-            const double rn = norm_at(k, 0), xn = norm_at(k, 1);
+        auto berr_at = [&](int z, int k) {
+            const double wnorm = norm_at(z, 0, 2);
+            const double rn = norm_at(z, k, 0), xn = norm_at(z, k, 1);
             if (rn == 0.0) return 0.0;
             return (wnorm > 0 && xn > 0) ? rn / (wnorm * xn) : 1.0;
         };
-        auto fwd_at = [&](int k) {
-            const double dn = norm_at(k, 3), xn = norm_at(k, 1);
+        auto fwd_at = [&](int z, int k) {
+            const double dn = norm_at(z, k, 3), xn = norm_at(z, k, 1);
             if (dn == 0.0) return 0.0;
             return xn > 0 ? dn / xn : 1.0;
         };
-        for (int k = 1; k <= queued && stt && k <= 8; k++) stt->refine_hist[k - 1] = berr_at(k);
-        int done = queued;
-        double berr = berr_at(done), fwd = fwd_at(done);
-        auto keep_going = [&]() {
-            if (info32 != 0 || !(berr == berr) || !(fwd == fwd) || done >= kMaxIters)
-                return false;
-            if (berr <= kConverged && fwd <= kFwdAccept) return false;
-            if (berr <= kAccept && fwd <= kFwdStop) return false;
-            return done < 2 || berr <= kMinGain * berr_at(done - 1) ||
-                   fwd <= kMinGain * fwd_at(done - 1);
+        auto keep_going = [&](int z) {
+            const int d = done[z];
+            const double be = berr[z], fe = fwd[z];
+            if (info32[z] != 0 || !(be == be) || !(fe == fe) || d >= kMaxIters) return false;
+            if (be <= kConverged && fe <= kFwdAccept) return false;
+            if (be <= kAccept && fe <= kFwdStop) return false;
+            return d < 2 || be <= kMinGain * berr_at(z, d - 1) || fe <= kMinGain * fwd_at(z, d - 1);
         };
-        while (keep_going()) {
-            int rc = queue_step(W, done + 1, gr);
-            if (rc) return rc;
-            TAGS_HIP(hipMemcpyAsync(hnorms + kNS * (done + 1), norms + kNS * (done + 1),
-                                    sizeof(unsigned long long) * kNS, hipMemcpyDeviceToHost, st));
-            TAGS_HIP(hipStreamSynchronize(st));
-            done++;
-            berr = berr_at(done);
-            fwd = fwd_at(done);
-            if (stt && done <= 8) stt->refine_hist[done - 1] = berr;
-        }
-        if (info32 == 0 && berr == berr && berr <= kAccept && fwd <= kFwdAccept) {
+        int ngo = 0;
+        for (int z = 0; z < nbi; z++) {
             if (stt) {
-                stt->precision_used = TAGS_PREC_MIXED;
-                stt->refine_iters = done;
-                stt->info = 0;
-                stt->backward_err = berr;
-                stt->forward_est = fwd;
+                for (int i = 0; i < 8; i++) stt[z].refine_hist[i] = -1.0;
+                for (int k = 1; k <= queued && k <= 8; k++)
+                    stt[z].refine_hist[k - 1] = berr_at(z, k);
             }
-            TAGS_HIP(hipMemcpyAsync(bvec, x, sizeof(c64) * n, hipMemcpyDeviceToDevice, st));
-            return 0;
+            berr[z] = berr_at(z, queued);
+            fwd[z] = fwd_at(z, queued);
+            go[z] = keep_going(z);
+            ngo += go[z];
         }
-        if (plan32_retry) {
-            // the second complex64 plan; its steps add to this one's
-            plan32 = plan32_retry;
-            plan32_retry = nullptr;
-            int rc = issue_mixed(W, bvec);
+        // further steps, batch-wide but masked to the items still refining (all of
+        // which have taken the same number of steps)
+        int step = queued;
+        while (ngo > 0) {
+            if (nbi > 1) {
+                for (int z = 0; z < nbi; z++) hactive[z] = go[z];
+                TAGS_HIP(hipMemcpyAsync(active, hactive, sizeof(int) * nbi,
+                                        hipMemcpyHostToDevice, st));
+            }
+            int rc = queue_step(W, step + 1, gr, nbi > 1);
             if (rc) return rc;
-            rc = finish_mixed(W, bvec, stt);
-            if (stt) stt->refine_iters += done;
-            return rc;
+            for (int z = 0; z < nbi; z++)
+                if (go[z])
+                    TAGS_HIP(hipMemcpyAsync(hnorms + kNormStride * z + kNS * (step + 1),
+                                            norms + kNormStride * z + kNS * (step + 1),
+                                            sizeof(unsigned long long) * kNS,
+                                            hipMemcpyDeviceToHost, st));
+            TAGS_HIP(hipStreamSynchronize(st));
+            step++;
+            ngo = 0;
+            for (int z = 0; z < nbi; z++) {
+                if (!go[z]) continue;
+                done[z] = step;
+                berr[z] = berr_at(z, step);
+                fwd[z] = fwd_at(z, step);
+                if (stt && step <= 8) stt[z].refine_hist[step - 1] = berr[z];
+                go[z] = keep_going(z);
+                ngo += go[z];
+            }
         }
-        // FP64 fallback on the intact W
-        int rc = solve64(W, bvec, stt);
-        if (stt) stt->refine_iters = done;
-        return rc;
+        TAGS_HIP(hipMemcpyAsync(bvec, x, sizeof(c64) * n * nbi, hipMemcpyDeviceToDevice, st));
+        std::vector<int> failed;
+        for (int z = 0; z < nbi; z++) {
+            const bool ok = info32[z] == 0 && berr[z] == berr[z] && berr[z] <= kAccept &&
+                            fwd[z] <= kFwdAccept;
+            if (ok) {
+                if (stt) {
+                    stt[z].precision_used = TAGS_PREC_MIXED;
+                    stt[z].refine_iters = done[z];
+                    stt[z].info = 0;
+                    stt[z].backward_err = berr[z];
+                    stt[z].forward_est = fwd[z];
+                }
+                if (info) info[z] = 0;
+            } else {
+                failed.push_back(z);
+            }
+        }
+        int first = 0;
+        // items that failed, one at a time on the intact W of each
+        const LUPlan* retry = plan32_retry;
+        const LUPlan* p32 = plan32;
+        for (int z : failed) {
+            c64* Wz = W + (long)z * sWi;
+            c64* bz = bvec + (long)z * n;
+            tags_item_stats one{};
+            tags_item_stats* sz = stt ? &stt[z] : &one;
+            int rc;
+            // b_z again (bvec holds the failed iterate)
+            TAGS_HIP(hipMemcpyAsync(bz, b + (long)z * n, sizeof(c64) * n, hipMemcpyDeviceToDevice,
+                                    st));
+            if (retry) {
+                // the second complex64 plan; its steps add to this one's
+                plan32 = retry;
+                plan32_retry = nullptr;
+                rc = issue_mixed(Wz, bz);
+                if (!rc) rc = finish_mixed(Wz, bz, sz);
+                plan32 = p32;
+                plan32_retry = retry;
+                if (rc >= 0) sz->refine_iters += done[z];
+            } else {
+                // FP64 fallback on the intact W
+                rc = issue64(Wz, bz);
+                if (!rc) rc = finish64(sz);
+                if (rc >= 0) sz->refine_iters = done[z];
+            }
+            if (rc < 0) return rc;
+            if (info) info[z] = rc;
+            if (!first) first = rc;
+            TAGS_HIP(hipStreamSynchronize(st));
+        }
+        nb = nbi;
+        sW = sWi;
+        return nbi == 1 ? first : 0;
     }
     int solve_mixed(c64* W, c64* bvec, tags_item_stats* stt)
     {
@@ -1146,7 +1289,10 @@ int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_ele
     case TAGS_PLAN_DENSE: p = LUPlan::dense((int)(nn + 2 * m)); break;
     case TAGS_PLAN_STRUCTURED: p = LUPlan::nodes_first((int)m, (int)nn); break;
     case TAGS_PLAN_ZL_FIRST: p = LUPlan::zl_first((int)m, (int)nn, false); break;
-    case TAGS_PLAN_BLOCK_PIVOT: p = LUPlan::zl_first((int)m, (int)nn, true); break;
+    case TAGS_PLAN_BLOCK_PIVOT:  // the weak pivoting for the complex64 factor only
+        p = LUPlan::zl_first((int)m, (int)nn, false);
+        p32 = LUPlan::zl_first((int)m, (int)nn, true);
+        break;
     case TAGS_PLAN_NODAL:
         p = LUPlan::zl_first((int)m, (int)nn, false);
         p32 = LUPlan::nodal((int)m, (int)nn);
@@ -1164,7 +1310,10 @@ int tags_immittance_solve(const TAGS_COMPLEX* a, TAGS_COMPLEX* b, size_t num_ele
     }
     const bool g = plan == TAGS_PLAN_GROUNDED || plan == TAGS_PLAN_GROUNDED_PIVOTED;
     return dense_solve_host((const c64*)a, (c64*)b, nn + 2 * m, precision, false, stats, &p,
-                            g || plan == TAGS_PLAN_NODAL ? &p32 : nullptr, gr);
+                            g || plan == TAGS_PLAN_NODAL || plan == TAGS_PLAN_BLOCK_PIVOT
+                                ? &p32
+                                : nullptr,
+                            gr);
 }
 
 }  // extern "C"
@@ -1426,21 +1575,48 @@ struct OwnedStream {
 // updates keep the GPU busy while another waits on its panel chain.
 struct SweepSlot {
     OwnedStream stream;
-    DevBuf<c64> W, rhs;
+    DevBuf<c64> W, rhs;        // cap items: W at stride N^2, rhs at stride N
+    DevBuf<PairScalars> dsc;   // the group's per-sample scalars
+    PairScalars* hsc = nullptr;  // pinned staging of dsc
     Profiler prof;
     Solver sv;
-    long f = -1;  // sample in flight
-    double hz = 0;        // its frequency |s| / 2 pi
-    bool nodal = false;   // its complex64 factor is the nodal plan's
-    int init(long N)
+    int cap = 0;
+    long f = -1;            // first sample of the group in flight (-1: idle)
+    int cnt = 0;            // its samples f .. f + cnt - 1
+    std::vector<double> hz; // their frequencies |s| / 2 pi
+    bool nodal = false;     // their complex64 factor is the nodal plan's
+    size_t w_cap = 0, rhs_cap = 0;  // allocated elements of W and rhs
+    int init(long N, int batch)
     {
         if (!stream.s) TAGS_HIP(hipStreamCreate(&stream.s));
         int rc;
-        if (!W.p && (rc = W.alloc((size_t)N * N))) return rc;
-        if (!rhs.p && (rc = rhs.alloc((size_t)N))) return rc;
+        const size_t nw = (size_t)N * N * batch, nr = (size_t)N * batch;
+        if (nw > w_cap || nr > rhs_cap || batch > cap) {
+            hipFree(W.p);
+            hipFree(rhs.p);
+            hipFree(dsc.p);
+            if (hsc) hipHostFree(hsc);
+            W.p = rhs.p = nullptr;
+            dsc.p = nullptr;
+            hsc = nullptr;
+            cap = 0;
+            w_cap = rhs_cap = 0;
+            const int items = std::max(batch, cap);
+            if ((rc = W.alloc(nw)) || (rc = rhs.alloc(nr)) || (rc = dsc.alloc((size_t)items)))
+                return rc;
+            TAGS_HIP(hipHostMalloc(&hsc, sizeof(PairScalars) * items, 0));
+            cap = items;
+            w_cap = nw;
+            rhs_cap = nr;
+        }
         sv.st = stream.s;
         sv.pr = &prof;
         return 0;
+    }
+    ~SweepSlot()
+    {
+        if (stream.s) hipStreamSynchronize(stream.s);
+        if (hsc) hipHostFree(hsc);
     }
 };
 
@@ -1453,7 +1629,9 @@ struct tags_sweep {
     long m = 0, n = 0, N = 0;
     int plan_kind = TAGS_PLAN_HYBRID;
     int layout_kind = -1;  // plan kind the current layout and grounding sums belong to
-    int inflight = 2;      // frequency samples in flight
+    int inflight = 2;      // groups of frequency samples in flight
+    int batch_req = 0;     // samples per group as set (0: chosen from N and memory)
+    int batch = 1;         // samples per group of the current call
     tags::LUPlan plan;
     tags::LUPlan plan32;
     // TAGS_PLAN_HYBRID: the mixed form's complex64 factor is the nodal plan's
@@ -1479,14 +1657,34 @@ struct tags_sweep {
 };
 
 // The first k slots exist and are allocated for order N (precision `mixed`).
+// Samples per group when none is set: one from order kBatchOneAbove up (a
+// sample's own launches fill the device), else as many as keep a slot's
+// matrices (complex128 W and, mixed, its complex64 copy) within kBatchBytes,
+// at most kBatchMax: 44 (mixed) or 64 for the 30 x 30 m transient grid
+// (N = 3861), 64 below.
+constexpr double kBatchBytes = 16.0 * (1 << 30);
+constexpr int kBatchMax = 64;
+constexpr long kBatchOneAbove = 8192;
+static int sweep_batch(const tags_sweep* sw, bool mixed)
+{
+    if (sw->batch_req > 0) return sw->batch_req;
+    if (sw->N >= kBatchOneAbove) return 1;
+    const double N = (double)sw->N;
+    const double per = N * N * (mixed ? 24.0 : 16.0) + N * 64.0 * 2 * (mixed ? 8 : 16) + N * 96.0;
+    const double b = std::floor(kBatchBytes / per);
+    return (int)std::max(1.0, std::min((double)kBatchMax, b));
+}
+
+// The first k slots exist and are allocated for groups of sw->batch samples of
+// order N (precision `mixed`).
 static int sweep_slots(tags_sweep* sw, int k, bool mixed)
 {
     while ((int)sw->slots.size() < k) sw->slots.emplace_back(new SweepSlot());
     for (int i = 0; i < k; i++) {
         SweepSlot& sl = *sw->slots[i];
-        int rc = sl.init(sw->N);
+        int rc = sl.init(sw->N, sw->batch);
         if (rc) return rc;
-        if ((rc = sl.sv.ensure((int)sw->N, mixed))) return rc;
+        if ((rc = sl.sv.ensure((int)sw->N, mixed, sw->batch))) return rc;
         sl.prof.on = sw->profiling;
     }
     return 0;
@@ -1530,9 +1728,17 @@ static int sweep_layout(tags_sweep* sw)
         sw->plan = LUPlan::zl_first((int)m, (int)n, false);
         sw->plan_nodal = LUPlan::nodal((int)m, (int)n, false);
         break;
+    case TAGS_PLAN_BLOCK_PIVOT:
+        // the FP64 form takes the fully pivoted factor: the Z_L block's weak
+        // pivoting is for the complex64 factor, whose refinement answers for it
+        sw->layout = BlockLayout{n, m, 2 * m, 0, m};
+        sw->plan = LUPlan::zl_first((int)m, (int)n, false);
+        sw->plan32 = LUPlan::zl_first((int)m, (int)n, true);
+        p32 = &sw->plan32;
+        break;
     default:
         sw->layout = BlockLayout{n, m, 2 * m, 0, m};
-        sw->plan = LUPlan::zl_first((int)m, (int)n, sw->plan_kind == TAGS_PLAN_BLOCK_PIVOT);
+        sw->plan = LUPlan::zl_first((int)m, (int)n, false);
         break;
     }
     for (auto& sl : sw->slots) {
@@ -1546,11 +1752,10 @@ static int sweep_layout(tags_sweep* sw)
 
 // Per-frequency scalars (reference caller formulas) and the assembly of W in
 // the sweep's block layout.
-static int sweep_fill(tags_sweep* sw, c64* W, hipStream_t st, Profiler* prof, TAGS_COMPLEX s_,
-                      TAGS_COMPLEX kappa_, double mur, const TAGS_COMPLEX* gamma_,
-                      const TAGS_COMPLEX* ref_l_, const TAGS_COMPLEX* ref_t_)
+static PairScalars sample_scalars(TAGS_COMPLEX s_, TAGS_COMPLEX kappa_, double mur,
+                                  const TAGS_COMPLEX* gamma_, const TAGS_COMPLEX* ref_l_,
+                                  const TAGS_COMPLEX* ref_t_)
 {
-    const long N = sw->N;
     const c64 sf = from_c(s_), kf = from_c(kappa_);
     c64 gf, rl, rt;
     if (gamma_) {
@@ -1569,22 +1774,40 @@ static int sweep_fill(tags_sweep* sw, c64* W, hipStream_t st, Profiler* prof, TA
         const c64 se = c64{sf.x * kEps0, sf.y * kEps0};
         rt = cdiv(csub(kf, se), cadd(kf, se));
     }
-    PairScalars sc = make_scalars(gf, sf, mur, kf, rl, rt, INTG_DOUBLE);
+    return make_scalars(gf, sf, mur, kf, rl, rt, INTG_DOUBLE);
+}
+
+// Assembly of nb samples' W (item z at W + z N^2) in the sweep's block layout:
+// one sample from `sc`, several from their scalars `dsc` in device memory.
+static int sweep_fill_items(tags_sweep* sw, c64* W, hipStream_t st, Profiler* prof, int nb,
+                            const PairScalars& sc, const PairScalars* dsc)
+{
+    const long N = sw->N, sW = N * N;
     Timed tm(prof, FAM_FILL, st);
     int rc;
-    if ((rc = launch_fill_static(W, N, sw->layout, sw->sn, sw->en, st))) return rc;
+    const PairScalars* scs = nb > 1 ? dsc : nullptr;
+    if ((rc = launch_fill_static(W, N, sw->layout, sw->sn, sw->en, st, nb, sW))) return rc;
     if (sw->formulation == TAGS_FORM_MHEM) {
         if (!sw->pot) return TAGS_ERR_ARG;
         const size_t mm = (size_t)sw->m * sw->m;
         if ((rc = launch_fill_mhem(W, N, sw->layout, sw->e.soa(), sw->img.soa(), sw->pot,
-                                   sw->pot + mm, sw->pot + 2 * mm, sw->pot + 3 * mm, sc, st)))
+                                   sw->pot + mm, sw->pot + 2 * mm, sw->pot + 3 * mm, sc, st, nb,
+                                   sW, scs)))
             return rc;
-    } else if ((rc = launch_fill_zblocks(W, N, sw->layout, sw->e.soa(), sw->img.soa(), sc,
-                                         st))) {
+    } else if ((rc = launch_fill_zblocks(W, N, sw->layout, sw->e.soa(), sw->img.soa(), sc, st, nb,
+                                         sW, scs))) {
         return rc;
     }
     tm.end();
     return 0;
+}
+
+static int sweep_fill(tags_sweep* sw, c64* W, hipStream_t st, Profiler* prof, TAGS_COMPLEX s_,
+                      TAGS_COMPLEX kappa_, double mur, const TAGS_COMPLEX* gamma_,
+                      const TAGS_COMPLEX* ref_l_, const TAGS_COMPLEX* ref_t_)
+{
+    const PairScalars sc = sample_scalars(s_, kappa_, mur, gamma_, ref_l_, ref_t_);
+    return sweep_fill_items(sw, W, st, prof, 1, sc, nullptr);
 }
 
 // The mHEM potentials, once per geometry (on the set-up stream, finished
@@ -1821,57 +2044,100 @@ void tags_sweep_destroy(tags_sweep* sw)
     hipStreamDestroy(st);
 }
 
-// Queues sample f on slot `sl`: assembly, right-hand side and the solve's
-// first half.
-static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const TAGS_COMPLEX* s,
-                       const TAGS_COMPLEX* kappa, double mur, const TAGS_COMPLEX* gamma,
-                       const TAGS_COMPLEX* ref_l, const TAGS_COMPLEX* ref_t, size_t inj_node,
-                       bool mixed)
+}  // extern "C"
+
+namespace tags {
+// b[z N + i] = (i == pos), i < N, for the nb items of a group
+__global__ void k_unit_rhs(c64* b, long N, long pos)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i < N) b[(long)blockIdx.z * N + i] = c64{i == pos ? 1.0 : 0.0, 0.0};
+}
+}  // namespace tags
+
+extern "C" {
+
+// Per-call arguments of tags_sweep_solve shared by the issue and finish steps.
+struct SweepCall {
+    size_t nfreq;
+    const TAGS_COMPLEX *s, *kappa, *gamma, *ref_l, *ref_t;
+    double mur;
+    size_t inj_node;
+    bool mixed;
+    TAGS_COMPLEX *u_inj, *x_out;
+    tags_item_stats* stats;
+};
+
+// Queues the group of samples starting at f on slot `sl`: up to sw->batch
+// consecutive samples that take the same complex64 factor (the hybrid plan
+// chooses per frequency); assembly, right-hand sides and the solve's first
+// half, one set of launches for the group. Returns the samples taken (> 0) or
+// an error.
+static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const SweepCall& c)
 {
     const long N = sw->N;
     const BlockLayout& L = sw->layout;
     int rc;
-    sl.f = (long)f;
-    {
-        const c64 sf = from_c(s[f]);
-        sl.hz = std::hypot(sf.x, sf.y) / (2.0 * M_PI);
+    auto hz_of = [&](size_t q) {
+        const c64 sf = from_c(c.s[q]);
+        return std::hypot(sf.x, sf.y) / (2.0 * M_PI);
+    };
+    const bool hybrid = sw->plan_kind == TAGS_PLAN_HYBRID && c.mixed;
+    sl.nodal = hybrid && hz_of(f) >= sw->nodal_above;
+    int cnt = 0;
+    sl.hz.clear();
+    while (cnt < sw->batch && f + cnt < c.nfreq &&
+           (!hybrid || (hz_of(f + cnt) >= sw->nodal_above) == sl.nodal)) {
+        sl.hz.push_back(hz_of(f + cnt));
+        cnt++;
     }
-    sl.nodal = sw->plan_kind == TAGS_PLAN_HYBRID && mixed && sl.hz >= sw->nodal_above;
+    sl.f = (long)f;
+    sl.cnt = cnt;
     if (sw->plan_kind == TAGS_PLAN_HYBRID) {
         sl.sv.plan32 = sl.nodal ? &sw->plan_nodal : nullptr;
         sl.sv.plan32_retry = sl.nodal ? &sw->plan : nullptr;
     }
-    if ((rc = sweep_fill(sw, sl.W.p, sl.stream.s, &sl.prof, s[f], kappa[f], mur,
-                         gamma ? &gamma[f] : nullptr, ref_l ? &ref_l[f] : nullptr,
-                         ref_t ? &ref_t[f] : nullptr)))
-        return rc;
+    hipStream_t st = sl.stream.s;
+    PairScalars sc0{};
+    for (int z = 0; z < cnt; z++) {
+        const size_t q = f + z;
+        const PairScalars sc = sample_scalars(c.s[q], c.kappa[q], c.mur,
+                                              c.gamma ? &c.gamma[q] : nullptr,
+                                              c.ref_l ? &c.ref_l[q] : nullptr,
+                                              c.ref_t ? &c.ref_t[q] : nullptr);
+        if (z == 0) sc0 = sc;
+        sl.hsc[z] = sc;
+    }
+    if (cnt > 1)
+        TAGS_HIP(hipMemcpyAsync(sl.dsc.p, sl.hsc, sizeof(PairScalars) * cnt,
+                                hipMemcpyHostToDevice, st));
+    if ((rc = sweep_fill_items(sw, sl.W.p, st, &sl.prof, cnt, sc0, sl.dsc.p))) return rc;
     // b = unit current at the injection node
-    if ((rc = fill_c64(sl.rhs.p, N, c64{0.0, 0.0}, sl.stream.s))) return rc;
-    if ((rc = fill_c64(sl.rhs.p + L.node_off + inj_node, 1, c64{1.0, 0.0}, sl.stream.s)))
-        return rc;
-    return mixed ? sl.sv.issue_mixed(sl.W.p, sl.rhs.p) : sl.sv.issue64(sl.W.p, sl.rhs.p);
+    hipLaunchKernelGGL(k_unit_rhs, dim3((unsigned)((N + 255) / 256), 1, (unsigned)cnt), dim3(256),
+                       0, st, sl.rhs.p, N, L.node_off + (long)c.inj_node);
+    TAGS_HIP(hipGetLastError());
+    rc = c.mixed ? sl.sv.issue_mixed(sl.W.p, sl.rhs.p, cnt, N * N)
+                 : sl.sv.issue64(sl.W.p, sl.rhs.p, cnt, N * N);
+    return rc ? rc : cnt;
 }
 
-// Backward error of the FP64 form's solution of sample f (in sl.rhs): W is
-// assembled again (the factor replaced it) and r = e_inj - W x taken in FP64.
-static int sweep_verify64(tags_sweep* sw, SweepSlot& sl, size_t f, const TAGS_COMPLEX* s,
-                          const TAGS_COMPLEX* kappa, double mur, const TAGS_COMPLEX* gamma,
-                          const TAGS_COMPLEX* ref_l, const TAGS_COMPLEX* ref_t,
-                          size_t inj_node, tags_item_stats* stt)
+// Backward error of the FP64 form's solution x of sample f (W assembled again
+// at Wz, which the factor replaced; r = e_inj - W x in FP64).
+static int sweep_verify64(tags_sweep* sw, SweepSlot& sl, c64* Wz, const c64* xz, size_t f,
+                          const SweepCall& c, tags_item_stats* stt)
 {
     const long N = sw->N;
     hipStream_t st = sl.stream.s;
     Solver& sv = sl.sv;
     int rc;
-    if ((rc = sweep_fill(sw, sl.W.p, st, nullptr, s[f], kappa[f], mur,
-                         gamma ? &gamma[f] : nullptr, ref_l ? &ref_l[f] : nullptr,
-                         ref_t ? &ref_t[f] : nullptr)))
+    if ((rc = sweep_fill(sw, Wz, st, nullptr, c.s[f], c.kappa[f], c.mur,
+                         c.gamma ? &c.gamma[f] : nullptr, c.ref_l ? &c.ref_l[f] : nullptr,
+                         c.ref_t ? &c.ref_t[f] : nullptr)))
         return rc;
     if ((rc = fill_c64(sv.b, N, c64{0.0, 0.0}, st)) ||
-        (rc = fill_c64(sv.b + sw->layout.node_off + inj_node, 1, c64{1.0, 0.0}, st)) ||
+        (rc = fill_c64(sv.b + sw->layout.node_off + c.inj_node, 1, c64{1.0, 0.0}, st)) ||
         (rc = fill_c64((c64*)sv.norms, 2, c64{0.0, 0.0}, st)) ||
-        (rc = residual64(sl.W.p, N, (int)N, sl.rhs.p, sv.b, sv.r, sv.norms, true, st, nullptr,
-                         false)))
+        (rc = residual64(Wz, N, (int)N, xz, sv.b, sv.r, sv.norms, true, st, nullptr, false)))
         return rc;
     TAGS_HIP(hipMemcpyAsync(sv.hnorms, sv.norms, sizeof(unsigned long long) * 3,
                             hipMemcpyDeviceToHost, st));
@@ -1881,51 +2147,56 @@ static int sweep_verify64(tags_sweep* sw, SweepSlot& sl, size_t f, const TAGS_CO
     return 0;
 }
 
-// Finishes the sample in flight on slot `sl` and writes its results. A
+// Finishes the group in flight on slot `sl` and writes its results. A
 // singular sample (info > 0) gets NaN in u_inj and in its block of x_out.
-static int sweep_finish(tags_sweep* sw, SweepSlot& sl, size_t inj_node, bool mixed,
-                        TAGS_COMPLEX* u_inj, TAGS_COMPLEX* x_out, tags_item_stats* stats,
-                        const TAGS_COMPLEX* s, const TAGS_COMPLEX* kappa, double mur,
-                        const TAGS_COMPLEX* gamma, const TAGS_COMPLEX* ref_l,
-                        const TAGS_COMPLEX* ref_t)
+static int sweep_finish(tags_sweep* sw, SweepSlot& sl, const SweepCall& c)
 {
     const long N = sw->N, n = sw->n, m = sw->m;
     const BlockLayout& L = sw->layout;
-    const size_t f = (size_t)sl.f;
+    const size_t f0 = (size_t)sl.f;
+    const int cnt = sl.cnt;
     sl.f = -1;
-    tags_item_stats stt{};
-    int rc = mixed ? sl.sv.finish_mixed(sl.W.p, sl.rhs.p, &stt) : sl.sv.finish64(&stt);
+    std::vector<tags_item_stats> stt(cnt);
+    for (auto& q : stt)
+        for (int i = 0; i < 8; i++) q.refine_hist[i] = -1.0;
+    std::vector<int> info(cnt, 0);
+    int rc = c.mixed ? sl.sv.finish_mixed(sl.W.p, sl.rhs.p, stt.data(), info.data())
+                     : sl.sv.finish64(stt.data(), info.data());
     if (rc < 0) return rc;
-    if (rc == 0 && !mixed && sw->verify) {
-        int vr = sweep_verify64(sw, sl, f, s, kappa, mur, gamma, ref_l, ref_t, inj_node, &stt);
-        if (vr) return vr;
-    }
-    if (stats) stats[f] = stt;
-    if (sl.nodal && (stt.precision_used != TAGS_PREC_MIXED || stt.refine_iters > kHybridMaxIters))
-        sw->nodal_above = std::max(sw->nodal_above, 1.5 * sl.hz);
-    if (rc > 0) {
-        const c64 nan = c64{NAN, NAN};
-        if (u_inj) u_inj[f] = to_c(nan);
-        if (x_out)
-            for (long i = 0; i < N; i++) x_out[f * N + i] = to_c(nan);
-        return 0;
-    }
-    c64 u;
     hipStream_t st = sl.stream.s;
-    TAGS_HIP(hipMemcpyAsync(&u, sl.rhs.p + L.node_off + inj_node, sizeof(c64),
-                            hipMemcpyDeviceToHost, st));
-    if (x_out) {
-        // reference order [U; I_L; I_T]
-        c64* xo = (c64*)(x_out + f * N);
-        TAGS_HIP(hipMemcpyAsync(xo, sl.rhs.p + L.node_off, sizeof(c64) * n,
-                                hipMemcpyDeviceToHost, st));
-        TAGS_HIP(hipMemcpyAsync(xo + n, sl.rhs.p + L.zl_off, sizeof(c64) * m,
-                                hipMemcpyDeviceToHost, st));
-        TAGS_HIP(hipMemcpyAsync(xo + n + m, sl.rhs.p + L.zt_off, sizeof(c64) * m,
-                                hipMemcpyDeviceToHost, st));
-    }
+    if (!c.mixed && sw->verify)
+        for (int z = 0; z < cnt; z++)
+            if (info[z] == 0 &&
+                (rc = sweep_verify64(sw, sl, sl.W.p + (long)z * N * N, sl.rhs.p + (long)z * N,
+                                     f0 + z, c, &stt[z])))
+                return rc;
+    std::vector<c64> xs((size_t)N * cnt);
+    TAGS_HIP(hipMemcpyAsync(xs.data(), sl.rhs.p, sizeof(c64) * N * cnt, hipMemcpyDeviceToHost,
+                            st));
     TAGS_HIP(hipStreamSynchronize(st));
-    if (u_inj) u_inj[f] = to_c(u);
+    for (int z = 0; z < cnt; z++) {
+        const size_t f = f0 + z;
+        if (c.stats) c.stats[f] = stt[z];
+        if (sl.nodal &&
+            (stt[z].precision_used != TAGS_PREC_MIXED || stt[z].refine_iters > kHybridMaxIters))
+            sw->nodal_above = std::max(sw->nodal_above, 1.5 * sl.hz[z]);
+        const c64* xz = xs.data() + (size_t)z * N;
+        if (info[z] > 0) {
+            const c64 nan = c64{NAN, NAN};
+            if (c.u_inj) c.u_inj[f] = to_c(nan);
+            if (c.x_out)
+                for (long i = 0; i < N; i++) c.x_out[f * N + i] = to_c(nan);
+            continue;
+        }
+        if (c.u_inj) c.u_inj[f] = to_c(xz[L.node_off + (long)c.inj_node]);
+        if (c.x_out) {
+            // reference order [U; I_L; I_T]
+            TAGS_COMPLEX* xo = c.x_out + f * N;
+            for (long i = 0; i < n; i++) xo[i] = to_c(xz[L.node_off + i]);
+            for (long i = 0; i < m; i++) xo[n + i] = to_c(xz[L.zl_off + i]);
+            for (long i = 0; i < m; i++) xo[n + m + i] = to_c(xz[L.zt_off + i]);
+        }
+    }
     return 0;
 }
 
@@ -1938,31 +2209,47 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     if (!sw || !s || !kappa) return TAGS_ERR_ARG;
     if ((long)inj_node >= sw->n) return TAGS_ERR_ARG;
     const bool mixed = precision == TAGS_PREC_MIXED;
+    // groups of samples share launches; grounded plans take one sample at a time
+    sw->batch = (sw->plan_kind == TAGS_PLAN_GROUNDED ||
+                 sw->plan_kind == TAGS_PLAN_GROUNDED_PIVOTED) && mixed
+                    ? 1
+                    : sweep_batch(sw, mixed);
+    if (lu_panel_kind() == PANEL_PARTIAL) sw->batch = 1;
     // every configured slot is allocated up front (a warm-up call sets all of
     // them up)
     int rc = sweep_slots(sw, sw->inflight, mixed);
     if (rc || (rc = sweep_potentials(sw))) return rc;
+    // after the slots exist: the layout sets every slot's plans
     sweep_layout(sw);
+    const SweepCall c{nfreq, s, kappa, gamma, ref_l, ref_t, mur, inj_node, mixed, u_inj, x_out,
+                      stats};
     // the hybrid plan's switch may rise during this call; it is restored at
     // the end, so that every call starts from the configured switch
     const double nodal_above0 = sw->nodal_above;
-    const int k = (int)std::min<size_t>((size_t)sw->inflight, std::max<size_t>(nfreq, 1));
+    const int k = sw->inflight;
     size_t next = 0;
-    for (int i = 0; i < k && next < nfreq; i++)
-        if ((rc = sweep_issue(sw, *sw->slots[i], next++, s, kappa, mur, gamma, ref_l, ref_t,
-                              inj_node, mixed)))
+    int used = 0;
+    for (int i = 0; i < k && next < nfreq; i++, used++) {
+        const int got = sweep_issue(sw, *sw->slots[i], next, c);
+        if (got < 0) {
+            rc = got;
             break;
-    // finish the slots in turn (samples in issue order), refilling each one
-    for (int i = 0; !rc; i = (i + 1) % k) {
+        }
+        next += (size_t)got;
+    }
+    // finish the slots in turn (groups in issue order), refilling each one
+    for (int i = 0; !rc && used > 0; i = (i + 1) % used) {
         SweepSlot& sl = *sw->slots[i];
         if (sl.f < 0) break;  // the oldest slot is idle: every sample is done
-        if ((rc = sweep_finish(sw, sl, inj_node, mixed, u_inj, x_out, stats, s, kappa, mur,
-                               gamma, ref_l, ref_t)))
-            break;
-        if (next < nfreq &&
-            (rc = sweep_issue(sw, sl, next++, s, kappa, mur, gamma, ref_l, ref_t, inj_node,
-                              mixed)))
-            break;
+        if ((rc = sweep_finish(sw, sl, c))) break;
+        if (next < nfreq) {
+            const int got = sweep_issue(sw, sl, next, c);
+            if (got < 0) {
+                rc = got;
+                break;
+            }
+            next += (size_t)got;
+        }
     }
     // after an error: let every queued sample end before returning
     sw->nodal_above = nodal_above0;
@@ -1980,4 +2267,12 @@ int tags_sweep_set_verify(tags_sweep* sw, int on)
     return 0;
 }
 
+int tags_sweep_set_batch(tags_sweep* sw, int samples)
+{
+    if (!sw || samples < 0 || samples > 4096) return TAGS_ERR_ARG;
+    sw->batch_req = samples;
+    return 0;
+}
+
 }  // extern "C"
+

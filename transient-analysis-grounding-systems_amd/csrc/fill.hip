@@ -473,8 +473,14 @@ int launch_pairs_dropin(double* zl, double* zt, bool alias, SegSoA e, SegSoA img
 constexpr int kFT = 32;
 
 __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long zl_off, long zt_off,
-                                                       long m, SegSoA e, SegSoA img, PairScalars sc)
+                                                       long m, SegSoA e, SegSoA img, PairScalars sc,
+                                                       long sw, const PairScalars* scs)
 {
+    // item blockIdx.z of a batch: its own W and frequency-sample scalars
+    if (scs) {
+        sc = scs[blockIdx.z];
+        W += (long)blockIdx.z * sw;
+    }
     // triangular tile index -> (ti >= tk)
     const long b = blockIdx.x;
     long ti = (long)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
@@ -566,16 +572,18 @@ __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long zl_
 }
 
 int launch_fill_zblocks(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
-                        const PairScalars& sc, hipStream_t st)
+                        const PairScalars& sc, hipStream_t st, int nb, long sw,
+                        const PairScalars* scs)
 {
     const long m = L.m;
-    if (m <= 0) return 0;
+    if (m <= 0 || nb <= 0) return 0;
+    if (nb > 1 && !scs) return TAGS_ERR_ARG;
     int rc = gl_tables_init();
     if (rc) return rc;
     const long T = (m + kFT - 1) / kFT;
     const long nt = T * (T + 1) / 2;
-    hipLaunchKernelGGL(k_fill_zblocks, dim3((unsigned)nt), dim3(256), 0, st, W, ldw, L.zl_off,
-                       L.zt_off, m, e, img, sc);
+    hipLaunchKernelGGL(k_fill_zblocks, dim3((unsigned)nt, 1, (unsigned)nb), dim3(256), 0, st, W,
+                       ldw, L.zl_off, L.zt_off, m, e, img, sc, sw, scs);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
@@ -643,8 +651,13 @@ __global__ __launch_bounds__(256) void k_fill_mhem(c64* W, long ldw, long zl_off
                                                     const double* __restrict__ pt,
                                                     const double* __restrict__ pli,
                                                     const double* __restrict__ pti,
-                                                    PairScalars sc)
+                                                    PairScalars sc, long sw,
+                                                    const PairScalars* scs)
 {
+    if (scs) {
+        sc = scs[blockIdx.z];
+        W += (long)blockIdx.z * sw;
+    }
     const long b = blockIdx.x;
     long ti = (long)((sqrt(8.0 * (double)b + 1.0) - 1.0) * 0.5);
     while ((ti + 1) * (ti + 2) / 2 <= b) ti++;
@@ -710,14 +723,16 @@ __global__ __launch_bounds__(256) void k_fill_mhem(c64* W, long ldw, long zl_off
 
 int launch_fill_mhem(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
                      const double* pl, const double* pt, const double* pli, const double* pti,
-                     const PairScalars& sc, hipStream_t st)
+                     const PairScalars& sc, hipStream_t st, int nb, long sw,
+                     const PairScalars* scs)
 {
     const long m = L.m;
-    if (m <= 0) return 0;
+    if (m <= 0 || nb <= 0) return 0;
+    if (nb > 1 && !scs) return TAGS_ERR_ARG;
     const long T = (m + kFT - 1) / kFT;
     const long nt = T * (T + 1) / 2;
-    hipLaunchKernelGGL(k_fill_mhem, dim3((unsigned)nt), dim3(256), 0, st, W, ldw, L.zl_off,
-                       L.zt_off, m, e, img, pl, pt, pli, pti, sc);
+    hipLaunchKernelGGL(k_fill_mhem, dim3((unsigned)nt, 1, (unsigned)nb), dim3(256), 0, st, W, ldw,
+                       L.zl_off, L.zt_off, m, e, img, pl, pt, pli, pti, sc, sw, scs);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
@@ -735,8 +750,9 @@ __device__ inline int block_of(long i, const BlockLayout& L)
 
 __global__ __launch_bounds__(256) void k_fill_static(c64* W, long ldw, BlockLayout L,
                                                       const int* __restrict__ sn,
-                                                      const int* __restrict__ en)
+                                                      const int* __restrict__ en, long sw)
 {
+    W += (long)blockIdx.z * sw;
     const long N = L.n + 2 * L.m;
     for (long r = blockIdx.y; r < N; r += gridDim.y) {
     const int rb = block_of(r, L);
@@ -759,15 +775,15 @@ __global__ __launch_bounds__(256) void k_fill_static(c64* W, long ldw, BlockLayo
 }
 
 int launch_fill_static(c64* W, long ldw, const BlockLayout& L, const int* seg_start_node,
-                       const int* seg_end_node, hipStream_t st)
+                       const int* seg_end_node, hipStream_t st, int nb, long sw)
 {
     const long N = L.n + 2 * L.m;
-    if (N <= 0) return 0;
+    if (N <= 0 || nb <= 0) return 0;
     unsigned gx = (unsigned)((N + 255) / 256);
     if (gx > 16) gx = 16;
     const unsigned gy = (unsigned)(N < 65535 ? N : 65535);
-    hipLaunchKernelGGL(k_fill_static, dim3(gx, gy), dim3(256), 0, st, W, ldw, L, seg_start_node,
-                       seg_end_node);
+    hipLaunchKernelGGL(k_fill_static, dim3(gx, gy, (unsigned)nb), dim3(256), 0, st, W, ldw, L,
+                       seg_start_node, seg_end_node, sw);
     TAGS_HIP(hipGetLastError());
     return 0;
 }

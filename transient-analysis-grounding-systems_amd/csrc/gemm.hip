@@ -243,10 +243,29 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
         }
 }
 
+// The operands of item blockIdx.z of a batched launch. The grid's x extent is
+// a multiple of 8, so a work-group's XCD (its linear dispatch id mod 8) is
+// still blockIdx.x mod 8 for every item.
+template <typename T>
+__device__ __forceinline__ GemmArgs<T> gemm_item(const GemmArgs<T>& g0)
+{
+    GemmArgs<T> g = g0;
+    const long z = blockIdx.z;
+    if (z) {
+        g.A += z * g.sa;
+        g.B += z * g.sb;
+        g.C += z * g.sc;
+        if (g.arow) g.arow += z * g.sar;
+        if (g.brow) g.brow += z * g.sbr;
+        if (g.crow) g.crow += z * g.scr;
+    }
+    return g;
+}
+
 template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN, int AL>
 __global__ __launch_bounds__(64 * WGM * WGN) void k_cgemm(GemmArgs<T> g)
 {
-    cgemm_block<T, BM, BN, BK, M3, WGM, WGN, AL>(g);
+    cgemm_block<T, BM, BN, BK, M3, WGM, WGN, AL>(gemm_item(g));
 }
 
 // The same product under its own name for the LU's trailing update
@@ -255,7 +274,7 @@ __global__ __launch_bounds__(64 * WGM * WGN) void k_cgemm(GemmArgs<T> g)
 template <typename T, int BM, int BN, int BK, bool M3, int WGM, int WGN, int AL>
 __global__ __launch_bounds__(64 * WGM * WGN) void k_trailing_update(GemmArgs<T> g)
 {
-    cgemm_block<T, BM, BN, BK, M3, WGM, WGN, AL>(g);
+    cgemm_block<T, BM, BN, BK, M3, WGM, WGN, AL>(gemm_item(g));
 }
 
 // ---------------------------------------------------------------------------
@@ -397,21 +416,25 @@ __device__ __forceinline__ void cgemm32_block(const GemmArgs<float>& g)
         }
 }
 
-__global__ __launch_bounds__(G32_NT, 2) void k_cgemm32(GemmArgs<float> g) { cgemm32_block(g); }
+__global__ __launch_bounds__(G32_NT, 2) void k_cgemm32(GemmArgs<float> g)
+{
+    cgemm32_block(gemm_item(g));
+}
 // the same product under the trailing update's name (see k_trailing_update)
 __global__ __launch_bounds__(G32_NT, 2) void k_trailing_update32(GemmArgs<float> g)
 {
-    cgemm32_block(g);
+    cgemm32_block(gemm_item(g));
 }
 
 static int launch_v32(const GemmArgs<float>& g, hipStream_t st, bool trailing)
 {
     const long tiles = (long)((g.M + G32_BM - 1) / G32_BM) * ((g.N + G32_BN - 1) / G32_BN);
     const long grid = (tiles + 7) / 8 * 8;
+    const dim3 gd((unsigned)grid, 1, (unsigned)g.nb);
     if (trailing)
-        hipLaunchKernelGGL(k_trailing_update32, dim3((unsigned)grid), dim3(G32_NT), 0, st, g);
+        hipLaunchKernelGGL(k_trailing_update32, gd, dim3(G32_NT), 0, st, g);
     else
-        hipLaunchKernelGGL(k_cgemm32, dim3((unsigned)grid), dim3(G32_NT), 0, st, g);
+        hipLaunchKernelGGL(k_cgemm32, gd, dim3(G32_NT), 0, st, g);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
@@ -425,12 +448,12 @@ static int launch_v(const GemmArgs<T>& g, hipStream_t st, bool trailing)
     const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
     const long grid = (tiles + 7) / 8 * 8;
     const dim3 block(64 * WGM * WGN);
+    const dim3 gd((unsigned)grid, 1, (unsigned)g.nb);
     if (trailing)
-        hipLaunchKernelGGL((k_trailing_update<T, BM, BN, BK, M3, WGM, WGN, AL>),
-                           dim3((unsigned)grid), block, 0, st, g);
+        hipLaunchKernelGGL((k_trailing_update<T, BM, BN, BK, M3, WGM, WGN, AL>), gd, block, 0, st,
+                           g);
     else
-        hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3, WGM, WGN, AL>), dim3((unsigned)grid),
-                           block, 0, st, g);
+        hipLaunchKernelGGL((k_cgemm<T, BM, BN, BK, M3, WGM, WGN, AL>), gd, block, 0, st, g);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
@@ -538,12 +561,14 @@ static constexpr int prec_index()
 template <typename T>
 int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing)
 {
-    if (g.M <= 0 || g.N <= 0) return 0;
+    if (g.M <= 0 || g.N <= 0 || g.nb <= 0) return 0;
+    if (g.nb > 65535) return TAGS_ERR_ARG;
     if (g.brow && g.K > GEMM_BROW_LDS) return TAGS_ERR_ARG;
     const int p = prec_index<T>();
     const int big = g_big_variant[p].load(std::memory_order_relaxed);
     const long tiles_big = (long)((g.M + GemmTable<T>::tile_m(big) - 1) / GemmTable<T>::tile_m(big)) *
-                           ((g.N + GemmTable<T>::tile_n(big) - 1) / GemmTable<T>::tile_n(big));
+                           ((g.N + GemmTable<T>::tile_n(big) - 1) / GemmTable<T>::tile_n(big)) *
+                           (g.nb > 0 ? g.nb : 1);
     const int v = tiles_big >= 512 ? big : g_small_variant[p].load(std::memory_order_relaxed);
     return GemmTable<T>::launch(v, g, st, trailing);
 }

@@ -120,6 +120,10 @@ struct GemmArgs {
     long ldc;
     const int* crow;
     int mode;  // 0: C = A B; 1: C -= A B
+    // batched launch (blockIdx.z = item z): every operand and row map of item z
+    // starts z times its stride (elements) after item 0's
+    int nb = 1;
+    long sa = 0, sar = 0, sb = 0, sbr = 0, sc = 0, scr = 0;
 };
 
 // One work-group's pivot candidate for one column of the panel.
@@ -156,9 +160,23 @@ struct TsluTree {
 };
 int tslu_tree(int cand_rows, int leaf_rows, TsluTree* tr);
 int tslu_min_leaf_rows();  // smallest leaf of any precision (largest tree)
+// Strides (elements) between the items of a batched factorisation or solve:
+// blockIdx.z is the item; its matrix, row map, block inverses, vectors, flags
+// and tournament lists start z times these after item 0's.
+struct BatchStrides {
+    int nb = 1;       // items in the launch
+    long a = 0;       // matrix
+    long p = 0;       // row map (ints)
+    long inv = 0;     // 64 x 64 block inverses
+    long v = 0;       // vectors
+    long f = 0;       // flags (ints)
+    long cand = 0;    // tournament nominee lists (ints)
+    long cnt = 0;     // tournament arrival counters
+};
 template <typename T>
 int launch_tslu(cx<T>* A, long ld, int* perm, int c0, int w, int row_end, int cand_end,
-                int* cand, unsigned* cnt, int* info, hipStream_t st);
+                int* cand, unsigned* cnt, int* info, hipStream_t st,
+                const BatchStrides& bs = BatchStrides());
 
 // Panel algorithm: tournament pivoting (default) or partial pivoting over the
 // whole panel with a cooperative launch.
@@ -281,10 +299,18 @@ struct LUPlan {
     }
 };
 
-// Device work space of one LU of order n in precision T.
+// Device work space of `batch` LUs of order n in precision T (the items of a
+// batched factorisation, each with its own row maps, inverses, vectors, flags
+// and tournament lists at the strides of bs()).
 template <typename T>
 struct LUWork {
     int n = 0, nblk = 0, nblk_cap = 0;
+    int batch = 1;   // items the work space holds
+    int nb = 1;      // items of the current factorisation (<= batch)
+    long sA = 0;     // matrix stride (elements) of the current factorisation
+    long cand_stride = 0, cnt_stride = 0;
+    // allocated element counts (kept while large enough for later orders)
+    size_t cap_perm = 0, cap_inv = 0, cap_items = 0, cap_cand = 0, cap_cnt = 0;
     int max_wgs = 256;
     int num_cu = 256;
     size_t lds_limit = 65536;
@@ -309,15 +335,31 @@ struct LUWork {
     double flops = 0;       // real flops of the last factorisation (GEMM + panel)
     int nodal_m = 0;        // the last factorisation's nodal block size (0: none)
 
-    int alloc(int n_, int max_panel_wgs, size_t panel_lds_limit);
+    int alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batch_ = 1);
     void release();
     int setup_kernel_attrs();
+    // strides of the current call (nb items, matrix stride sA)
+    BatchStrides bs() const
+    {
+        BatchStrides b;
+        b.nb = nb;
+        b.a = sA;
+        b.p = n;
+        b.inv = 4096L * nblk_cap;
+        b.v = n;
+        b.f = 4;
+        b.cand = cand_stride;
+        b.cnt = cnt_stride;
+        return b;
+    }
     ~LUWork() { release(); }
 };
 
+// Factors nb items (matrices A + z sA, z < nb <= w.batch) alike; the solves
+// below then work on all nb items (vectors at stride n).
 template <typename T>
 int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr,
-              const LUPlan* plan = nullptr);
+              const LUPlan* plan = nullptr, int nb = 1, long sA = 0);
 template <typename T>
 int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
                      Profiler* pr);
@@ -339,12 +381,22 @@ int gemm_variant_count(int precision);
 
 // norms[0..1] (and [2] with want_wnorm) receive |r|, |x| (|W|) as max-reductions;
 // zero_norms false: the caller has already zeroed them (no memset node)
+// Batched forms (nb items): W of item z at W + z sw, vectors at stride n, the
+// norms of item z at norms + z sn.
+struct VecBatch {
+    int nb = 1;
+    long sw = 0;   // c64 matrix stride
+    long sw32 = 0; // c32 matrix stride
+    long sn = 0;   // norms stride (u64)
+    const int* active = nullptr;  // item z skipped where active[z] == 0 (nullable)
+};
 int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
                unsigned long long* norms, bool want_wnorm, hipStream_t st, Profiler* pr,
-               bool zero_norms = true);
-int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t st);
+               bool zero_norms = true, const VecBatch& vb = VecBatch());
+int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t st,
+              const VecBatch& vb = VecBatch());
 int to_c32_scaled(const c64* W, long ld, int n, const double* d, c32* W32, long ld32,
-                  hipStream_t st);
+                  hipStream_t st, const VecBatch& vb = VecBatch());
 // Grounded node unknowns of the mixed form (lu.hip): node rows and columns
 // [nb, ne) of the physical order, reference node g (g < 0: off).
 struct Grounding {
@@ -361,11 +413,11 @@ int to_c32_grounded(const c64* W, long ld, int n, const Grounding& gr, const c64
 // (max norm as left by residual64 in device memory) replaces the host scale
 int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv_scale, c32* t,
                hipStream_t st, const Grounding* gr = nullptr, c64* rg = nullptr,
-               const unsigned long long* rn = nullptr);
+               const unsigned long long* rn = nullptr, const VecBatch& vb = VecBatch());
 // dxn (optional, zeroed beforehand): receives the max-norm of the correction
 int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st,
              const Grounding* gr = nullptr, const unsigned long long* rn = nullptr,
-             unsigned long long* dxn = nullptr);
+             unsigned long long* dxn = nullptr, const VecBatch& vb = VecBatch());
 int fill_c64(c64* p, long n, c64 v, hipStream_t st);
 int sym_from_upper(c64* W, long ld, int n, hipStream_t st);
 

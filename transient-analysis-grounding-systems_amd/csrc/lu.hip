@@ -252,8 +252,15 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
 // ---------------------------------------------------------------------------
 template <typename T, bool FACTOR>
 __global__ __launch_bounds__(512) void k_diag_block(cx<T>* A, long ld, const int* pm, int c0,
-                                                    int w, cx<T>* invL, cx<T>* invU, int* info)
+                                                    int w, cx<T>* invL, cx<T>* invU, int* info,
+                                                    long sa, long sp, long sinv, long sf)
 {
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    pm += z_ * sp;
+    invL += z_ * sinv;
+    invU += z_ * sinv;
+    info += z_ * sf;
     __shared__ cx<T> s_u[2][64];
     __shared__ cx<T> s_v[2][2][64];
     __shared__ cx<T> s_rp[2];
@@ -367,8 +374,13 @@ __global__ __launch_bounds__(512) void k_diag_block(cx<T>* A, long ld, const int
 // 64 x CW product.
 template <typename T, int CW, int KC, bool UP>
 __global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* pm, int j, int jb,
-                                                  const cx<T>* invl, int cb, int cols)
+                                                  const cx<T>* invl, int cb, int cols, long sa,
+                                                  long sp, long sinv)
 {
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    pm += z_ * sp;
+    invl += z_ * sinv;
     constexpr int CT = CW / 16;
     __shared__ cx<T> Us[4][64][CW];
     __shared__ cx<T> Ls[64][KC + 1];
@@ -476,8 +488,13 @@ __global__ __launch_bounds__(256) void k_step_u12(cx<T>* A, long ld, const int* 
 typedef float u12v4 __attribute__((ext_vector_type(4)));
 template <bool UP>
 __global__ __launch_bounds__(256) void k_step_u12_mfma(c32* A, long ld, const int* pm, int j,
-                                                       int jb, const c32* invl, int cb, int cols)
+                                                       int jb, const c32* invl, int cb, int cols,
+                                                       long sa, long sp, long sinv)
 {
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    pm += z_ * sp;
+    invl += z_ * sinv;
     constexpr int CW = 32, KC = 8, NCH = 64 / KC, LPT = 64 * KC / 256;
     __shared__ c32 Us[4][64][CW];
     __shared__ c32 Ls[2][64][KC + 1];
@@ -597,8 +614,11 @@ __global__ __launch_bounds__(256) void k_step_u12_mfma(c32* A, long ld, const in
 constexpr int NODAL_NZ_MAX = 512;
 template <typename T>
 __global__ __launch_bounds__(256) void k_nodal_schur(cx<T>* A, long ld, const int* pm, int m,
-                                                     int n)
+                                                     int n, long sa, long sp)
 {
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    pm += z_ * sp;
     __shared__ long s_row[NODAL_NZ_MAX];
     __shared__ cx<T> s_cf[NODAL_NZ_MAX];
     __shared__ int s_wcnt[4];
@@ -650,8 +670,13 @@ __global__ __launch_bounds__(256) void k_nodal_schur(cx<T>* A, long ld, const in
 // w (the rows and the columns [c0, c0 + w) are disjoint); one work-group per row
 template <typename T>
 __global__ __launch_bounds__(256) void k_rows_gemv(const cx<T>* A, long ld, const int* perm,
-                                                    cx<T>* y, int lo, int hi, int c0, int w)
+                                                    cx<T>* y, int lo, int hi, int c0, int w,
+                                                    long sa, long sp, long sv)
 {
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    perm += z_ * sp;
+    y += z_ * sv;
     __shared__ T red[2][4];
     const int i = lo + blockIdx.x;
     if (i >= hi) return;
@@ -681,15 +706,18 @@ __global__ __launch_bounds__(256) void k_rows_gemv(const cx<T>* A, long ld, cons
 }
 
 // perm[c0 + i] = order[i] for i < len (a phase's row order)
-__global__ void k_set_order(int* perm, int c0, int len, const int* order)
+__global__ void k_set_order(int* perm, int c0, int len, const int* order, long sp)
 {
+    perm += (long)blockIdx.z * sp;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < len) perm[c0 + i] = order[i];
 }
 
 // perm = init (or identity); flags (info, barrier error) cleared
-__global__ void k_lu_init(int* perm, const int* init, int n, int* flags)
+__global__ void k_lu_init(int* perm, const int* init, int n, int* flags, long sp, long sf)
 {
+    perm += (long)blockIdx.z * sp;
+    flags += (long)blockIdx.z * sf;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) perm[i] = init ? init[i] : i;
     if (i < 2) flags[i] = 0;
@@ -754,8 +782,14 @@ __device__ inline cx<T> wave_sum_cx(cx<T> v)
 }
 template <typename T, bool UP>
 __global__ __launch_bounds__(256) void k_step_trsv(const cx<T>* A, long ld, const int* perm,
-                                                   cx<T>* y, int j, int jb, const cx<T>* inv)
+                                                   cx<T>* y, int j, int jb, const cx<T>* inv,
+                                                   long sa, long sp, long sv, long sinv)
 {
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    perm += z_ * sp;
+    y += z_ * sv;
+    inv += z_ * sinv;
     __shared__ cx<T> ys[256];
     __shared__ cx<T> acc[64];
     __shared__ int s_pm[256];
@@ -817,8 +851,13 @@ __global__ __launch_bounds__(256) void k_step_trsv(const cx<T>* A, long ld, cons
 // select), so none waits behind a branch
 template <typename T>
 __global__ __launch_bounds__(256) void k_panel_gemv(const cx<T>* A, long ld, const int* perm,
-                                                     cx<T>* y, int lo, int hi, int c0, int w)
+                                                     cx<T>* y, int lo, int hi, int c0, int w,
+                                                     long sa, long sp, long sv)
 {
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    perm += z_ * sp;
+    y += z_ * sv;
     const int i = lo + blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
     if (i >= hi) return;
@@ -841,8 +880,16 @@ __global__ __launch_bounds__(256) void k_panel_gemv(const cx<T>* A, long ld, con
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(256) void k_residual(const c64* W, long ld, int n, const c64* x,
                                                    const c64* b, c64* r,
-                                                   unsigned long long* norms, int want_wnorm)
+                                                   unsigned long long* norms, int want_wnorm,
+                                                   long sw, long sn, const int* active)
 {
+    const long z_ = blockIdx.z;
+    if (active && !active[z_]) return;
+    W += z_ * sw;
+    x += z_ * n;
+    b += z_ * n;
+    r += z_ * n;
+    norms += z_ * sn;
     const int row = blockIdx.x;
     const c64* Wr = W + (long)row * ld;
     double sx = 0, sy = 0, an = 0;
@@ -885,8 +932,11 @@ __global__ __launch_bounds__(256) void k_residual(const c64* W, long ld, int n, 
 // row maxima of |W_ij| (for symmetric equilibration): out[i] = 1/sqrt(max_j |W_ij|),
 // or 1 when equilibration is off
 __global__ __launch_bounds__(256) void k_row_scale(const c64* W, long ld, int n, double* d,
-                                                    int enabled)
+                                                    int enabled, long sw)
 {
+    const long z_ = blockIdx.z;
+    W += z_ * sw;
+    d += z_ * n;
     const int row = blockIdx.x;
     if (!enabled) {
         if (threadIdx.x == 0) d[row] = 1.0;
@@ -909,8 +959,13 @@ __global__ __launch_bounds__(256) void k_row_scale(const c64* W, long ld, int n,
 
 // W32[i][j] = d_i W64[i][j] d_j
 __global__ __launch_bounds__(256) void k_to_c32_scaled(const c64* W, long ld, int n,
-                                                        const double* d, c32* W32, long ld32)
+                                                        const double* d, c32* W32, long ld32,
+                                                        long sw, long sw32)
 {
+    const long z_ = blockIdx.z;
+    W += z_ * sw;
+    d += z_ * n;
+    W32 += z_ * sw32;
     const int row = blockIdx.y;
     const double di = d[row];
     for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) {
@@ -1017,8 +1072,15 @@ __device__ inline double rnorm_of(const unsigned long long* rn)
 // (g >= 0) r'[g] = *rg (the sum of the node entries) and r'[p] = r[p] otherwise
 __global__ void k_prep_rhs32(const c64* r, const double* d, const int* perm, int n,
                              double inv_scale, const unsigned long long* rn, int g,
-                             const c64* rg, c32* t)
+                             const c64* rg, c32* t, long sn, const int* active)
 {
+    const long z_ = blockIdx.z;
+    if (active && !active[z_]) return;
+    r += z_ * n;
+    d += z_ * n;
+    perm += z_ * n;
+    t += z_ * n;
+    if (rn) rn += z_ * sn;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i >= n) return;
     if (rn) {
@@ -1038,8 +1100,16 @@ __global__ void k_prep_rhs32(const c64* r, const double* d, const int* perm, int
 __global__ __launch_bounds__(256) void k_update_x(c64* x, const c32* z, const double* d, int n,
                                                    double scale, const unsigned long long* rn,
                                                    int g, int nb, int ne,
-                                                   unsigned long long* dxn)
+                                                   unsigned long long* dxn, long sn,
+                                                   const int* active)
 {
+    const long z_ = blockIdx.z;
+    if (active && !active[z_]) return;
+    x += z_ * n;
+    z += z_ * n;
+    d += z_ * n;
+    if (rn) rn += z_ * sn;
+    if (dxn) dxn += z_ * sn;
     const int j = blockIdx.x * 256 + threadIdx.x;
     double a = 0.0;
     if (j < n) {
@@ -1063,6 +1133,9 @@ __global__ __launch_bounds__(256) void k_update_x(c64* x, const c32* z, const do
 template <typename T>
 __global__ void k_gather_vec(const cx<T>* b, const int* perm, int n, cx<T>* t)
 {
+    b += (long)blockIdx.z * n;
+    perm += (long)blockIdx.z * n;
+    t += (long)blockIdx.z * n;
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < n) t[i] = b[perm[i]];
 }
@@ -1096,38 +1169,80 @@ __global__ __launch_bounds__(256) void k_sym_from_upper(c64* W, long ld, int n)
 // host side
 // ---------------------------------------------------------------------------
 template <typename T>
-int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit)
+int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batch_)
 {
     num_cu = max_panel_wgs;
     max_wgs = max_panel_wgs;
     lds_limit = panel_lds_limit;
-    if (n_ <= 0) return TAGS_ERR_ARG;
-    if (n_ >= 65536) return TAGS_ERR_TOO_LARGE;
-    if (n == n_) return 0;
-    release();
-    n = n_;
-    nblk_cap = (n + 63) / 64 + 8;
-    TAGS_HIP(hipMalloc(&perm, sizeof(int) * n));
-    TAGS_HIP(hipMalloc(&perm_alt, sizeof(int) * n));
-    TAGS_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-    TAGS_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
-    TAGS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
-    TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * 4096 * (size_t)nblk_cap));
-    TAGS_HIP(hipMalloc(&invU, sizeof(cx<T>) * 4096 * (size_t)nblk_cap));
-    TAGS_HIP(hipMalloc(&slots, sizeof(PanelSlot<T>) * 2 * 512));
-    TAGS_HIP(hipMalloc(&bar, sizeof(unsigned) * 64));
-    TAGS_HIP(hipMalloc(&flags, sizeof(int) * 4));
-    TAGS_HIP(hipMalloc(&vec, sizeof(cx<T>) * n));
-    TAGS_HIP(hipMalloc(&d_init, sizeof(int) * n));
-    TAGS_HIP(hipMalloc(&d_orders, sizeof(int) * n));
-    TAGS_HIP(hipMemset(bar, 0, sizeof(unsigned) * 64));
-    TAGS_HIP(hipMemset(flags, 0, sizeof(int) * 4));
+    if (n_ <= 0 || batch_ <= 0) return TAGS_ERR_ARG;
+    if (n_ >= 65536 || batch_ > 65535) return TAGS_ERR_TOO_LARGE;
+    if (n == n_ && batch >= batch_) return 0;
+    // element counts for (n_, batch_); arrays are kept while they are large
+    // enough (a work space serves systems of different orders in turn)
+    const int nblk_ = (n_ + 63) / 64 + 8;
     TsluTree tr;
-    int rc = tslu_tree(n, tslu_min_leaf_rows(), &tr);
+    int rc = tslu_tree(n_, tslu_min_leaf_rows(), &tr);
     if (rc) return rc;
-    TAGS_HIP(hipMalloc(&tcand, sizeof(int) * (size_t)tr.cand_total));
-    TAGS_HIP(hipMalloc(&tcnt, sizeof(unsigned) * (size_t)std::max(1, tr.cnt_total)));
-    TAGS_HIP(hipMemset(tcnt, 0, sizeof(unsigned) * (size_t)std::max(1, tr.cnt_total)));
+    const size_t B = (size_t)batch_;
+    const size_t need_perm = (size_t)n_ * B, need_inv = 4096 * (size_t)nblk_ * B;
+    const size_t need_cand = (size_t)tr.cand_total * B;
+    const size_t need_cnt = (size_t)std::max(1, tr.cnt_total) * B;
+    if (!s2) {
+        TAGS_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        TAGS_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
+        TAGS_HIP(hipEventCreateWithFlags(&ev_panel, hipEventDisableTiming));
+        TAGS_HIP(hipMalloc(&slots, sizeof(PanelSlot<T>) * 2 * 512));
+        TAGS_HIP(hipMalloc(&bar, sizeof(unsigned) * 64));
+        TAGS_HIP(hipMemset(bar, 0, sizeof(unsigned) * 64));
+    }
+    if (need_perm > cap_perm || need_inv > cap_inv || B > cap_items || need_cand > cap_cand ||
+        need_cnt > cap_cnt) {
+        (void)hipStreamSynchronize(s2);
+        (void)hipFree(perm);
+        (void)hipFree(perm_alt);
+        (void)hipFree(invL);
+        (void)hipFree(invU);
+        (void)hipFree(flags);
+        (void)hipFree(vec);
+        (void)hipFree(d_init);
+        (void)hipFree(d_orders);
+        (void)hipFree(tcand);
+        (void)hipFree(tcnt);
+        perm = perm_alt = flags = d_init = d_orders = tcand = nullptr;
+        invL = invU = vec = nullptr;
+        tcnt = nullptr;
+        cap_perm = cap_inv = cap_items = cap_cand = cap_cnt = 0;
+        TAGS_HIP(hipMalloc(&perm, sizeof(int) * need_perm));
+        TAGS_HIP(hipMalloc(&perm_alt, sizeof(int) * need_perm));
+        TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * need_inv));
+        TAGS_HIP(hipMalloc(&invU, sizeof(cx<T>) * need_inv));
+        TAGS_HIP(hipMalloc(&flags, sizeof(int) * 4 * B));
+        TAGS_HIP(hipMalloc(&vec, sizeof(cx<T>) * need_perm));
+        TAGS_HIP(hipMalloc(&d_init, sizeof(int) * need_perm));
+        TAGS_HIP(hipMalloc(&d_orders, sizeof(int) * need_perm));
+        TAGS_HIP(hipMemset(flags, 0, sizeof(int) * 4 * B));
+        TAGS_HIP(hipMalloc(&tcand, sizeof(int) * need_cand));
+        TAGS_HIP(hipMalloc(&tcnt, sizeof(unsigned) * need_cnt));
+        // arrival counters start at zero; every tournament's root resets the
+        // ones it used, so they stay zero between launches whatever the order
+        TAGS_HIP(hipMemset(tcnt, 0, sizeof(unsigned) * need_cnt));
+        cap_perm = need_perm;
+        cap_inv = need_inv;
+        cap_items = B;
+        cap_cand = need_cand;
+        cap_cnt = need_cnt;
+    }
+    n = n_;
+    batch = batch_;
+    nb = 1;
+    sA = 0;
+    nblk_cap = nblk_;
+    cand_stride = tr.cand_total;
+    cnt_stride = std::max(1, tr.cnt_total);
+    h_init.clear();
+    h_orders.clear();
+    blk.clear();
+    nblk = 0;
     return 0;
 }
 
@@ -1166,19 +1281,32 @@ void LUWork<T>::release()
     flags = nullptr;
     blk.clear();
     n = nblk = 0;
+    batch = nb = 1;
+    sA = 0;
+    cap_perm = cap_inv = cap_items = cap_cand = cap_cnt = 0;
 }
 
+// One product per item of the factorisation's batch: the matrix operands at the
+// batch's matrix stride and row maps at its row-map stride; the operand named
+// by inv_op (1: A, 2: B) is a block inverse at the inverse stride instead.
+// *flops counts one item's flops, the profiler the launch's.
 template <typename T>
 static int gemm(Profiler* pr, hipStream_t st, int M, int N, int K, const cx<T>* A, long lda,
                 const int* arow, const cx<T>* B, long ldb, const int* brow, cx<T>* C, long ldc,
-                const int* crow, int mode, bool trailing, double* flops)
+                const int* crow, int mode, bool trailing, double* flops, const BatchStrides& bs,
+                int inv_op = 0)
 {
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     GemmArgs<T> g{M, N, K, A, lda, arow, B, ldb, brow, C, ldc, crow, mode};
+    g.nb = bs.nb;
+    g.sa = inv_op == 1 ? bs.inv : bs.a;
+    g.sb = inv_op == 2 ? bs.inv : bs.a;
+    g.sc = bs.a;
+    g.sar = g.sbr = g.scr = bs.p;
     Timed tm(pr, FAM_GEMM, st);
     int rc = launch_gemm<T>(g, st, trailing);
     const double f = 8.0 * M * N * (double)K;
-    tm.end(f, trailing);
+    tm.end(f * bs.nb, trailing);
     if (flops) *flops += f;
     return rc;
 }
@@ -1224,11 +1352,13 @@ static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, 
     if (g_panel_kind.load(std::memory_order_relaxed) == PANEL_TOURNAMENT) {
         Timed tm(pr, FAM_PANEL, st);
         const int rc = launch_tslu<T>(A, ld, pm, c0, cols, row_end, cand_end, w.tcand,
-                                      w.tcnt, w.flags, st);
+                                      w.tcnt, w.flags, st, w.bs());
         tm.end();
         w.flops += 8.0 * (double)M * cols * cols / 2;
         return rc;
     }
+    // the cooperative panel holds the whole device: one item at a time only
+    if (w.nb != 1) return TAGS_ERR_ARG;
     const size_t per_row = 64 * sizeof(cx<T>) + 2 * sizeof(int);
     const int rmax = panel_rows_max<T>(w.lds_limit);
     // few work-groups keep the grid barrier cheap; ~160 rows each
@@ -1290,7 +1420,7 @@ static int apply_linv_rows(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A,
     if (cols <= 0) return 0;
     if (h > 64) return TAGS_ERR_ARG;
     return gemm<T>(pr, st, h, cols, h, inv, 64, nullptr, A + cb, ld, pm + r0, A + cb, ld, pm + r0,
-                   0, false, &w.flops);
+                   0, false, &w.flops, w.bs(), 1);
 }
 
 // One step of the factorisation (up to the step width in columns).
@@ -1311,7 +1441,7 @@ static int l21_gemm(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long l
     const int rb = c0 + cols;
     if (row_end <= rb) return 0;
     return gemm<T>(pr, st, row_end - rb, cols, cols, A + c0, ld, pm + rb, invu, 64, nullptr,
-                   A + c0, ld, pm + rb, 0, false, &w.flops);
+                   A + c0, ld, pm + rb, 0, false, &w.flops, w.bs(), 2);
 }
 
 // One sub-panel of `cols` columns at c0 whose diagonal block is block `slot`:
@@ -1333,12 +1463,13 @@ static int sub_panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long 
         return rc;
     {
         Timed tm(pr, ph.pivot ? FAM_OTHER : FAM_PANEL, st);
+        const BatchStrides b = w.bs();
         if (ph.pivot)
-            hipLaunchKernelGGL((k_diag_block<T, false>), dim3(2), dim3(256), 0, st, A, ld, pm, c0,
-                               cols, invl, invu, w.flags);
+            hipLaunchKernelGGL((k_diag_block<T, false>), dim3(2, 1, b.nb), dim3(256), 0, st, A,
+                               ld, pm, c0, cols, invl, invu, w.flags, b.a, b.p, b.inv, b.f);
         else
-            hipLaunchKernelGGL((k_diag_block<T, true>), dim3(1), dim3(512), 0, st, A, ld, pm, c0,
-                               cols, invl, invu, w.flags);
+            hipLaunchKernelGGL((k_diag_block<T, true>), dim3(1, 1, b.nb), dim3(512), 0, st, A,
+                               ld, pm, c0, cols, invl, invu, w.flags, b.a, b.p, b.inv, b.f);
         TAGS_HIP(hipGetLastError());
         tm.end();
     }
@@ -1365,7 +1496,7 @@ static int step_panels(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, lon
                                      c0, wt, c1, end)))
             return rc;
         if ((rc = gemm<T>(pr, st, ph.row_end - c1, end - c1, wt, A + c0, ld, pm + c1, A + c1, ld,
-                          pm + c0, A + c1, ld, pm + c1, 1, false, &w.flops)))
+                          pm + c0, A + c1, ld, pm + c1, 1, false, &w.flops, w.bs())))
             return rc;
     }
     return 0;
@@ -1387,16 +1518,18 @@ static int step_u12(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long l
         flops += 8.0 * wt * (double)cols * (r0 + wt);
     }
     Timed tm(pr, FAM_GEMM, st);
+    const BatchStrides b = w.bs();
     if constexpr (sizeof(T) == 4)
-        hipLaunchKernelGGL(k_step_u12_mfma<false>, dim3((cols + 31) / 32), dim3(256), 0, st, A,
-                           ld, pm, s.j, s.jb, (const c32*)(w.invL + (size_t)s.slot * 4096), cb,
-                           cols);
+        hipLaunchKernelGGL(k_step_u12_mfma<false>, dim3((cols + 31) / 32, 1, b.nb), dim3(256), 0,
+                           st, A, ld, pm, s.j, s.jb, (const c32*)(w.invL + (size_t)s.slot * 4096),
+                           cb, cols, b.a, b.p, b.inv);
     else
-        hipLaunchKernelGGL((k_step_u12<T, CW, KC, false>), dim3((cols + CW - 1) / CW), dim3(256),
-                           0, st, A, ld, pm, s.j, s.jb,
-                           (const cx<T>*)(w.invL + (size_t)s.slot * 4096), cb, cols);
+        hipLaunchKernelGGL((k_step_u12<T, CW, KC, false>), dim3((cols + CW - 1) / CW, 1, b.nb),
+                           dim3(256), 0, st, A, ld, pm, s.j, s.jb,
+                           (const cx<T>*)(w.invL + (size_t)s.slot * 4096), cb, cols, b.a, b.p,
+                           b.inv);
     TAGS_HIP(hipGetLastError());
-    tm.end(flops);
+    tm.end(flops * b.nb);
     w.flops += flops;
     return 0;
 }
@@ -1414,7 +1547,7 @@ static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, lon
     int rc;
     if ((rc = step_u12<T>(w, pr, st, A, ld, pm, s, cb, ce))) return rc;
     return gemm<T>(pr, st, ph.row_end - c1, ce - cb, s.jb, A + j, ld, pm + c1, A + cb, ld, pm + j,
-                   A + cb, ld, pm + c1, 1, true, &w.flops);
+                   A + cb, ld, pm + c1, 1, true, &w.flops, w.bs());
 }
 
 // Backward blocked solve of the extra columns [x0, x0 + nx) in the logical rows
@@ -1441,21 +1574,24 @@ static int backward_cols(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, l
         }
         {
             Timed tm(pr, FAM_GEMM, st);
+            const BatchStrides b = w.bs();
             if constexpr (sizeof(T) == 4)
-                hipLaunchKernelGGL(k_step_u12_mfma<true>, dim3((nx + 31) / 32), dim3(256), 0, st,
-                                   A, ld, pm, s.j, s.jb,
-                                   (const c32*)(w.invU + (size_t)s.slot * 4096), x0, nx);
-            else
-                hipLaunchKernelGGL((k_step_u12<T, CW, KC, true>), dim3((nx + CW - 1) / CW),
+                hipLaunchKernelGGL(k_step_u12_mfma<true>, dim3((nx + 31) / 32, 1, b.nb),
                                    dim3(256), 0, st, A, ld, pm, s.j, s.jb,
-                                   (const cx<T>*)(w.invU + (size_t)s.slot * 4096), x0, nx);
+                                   (const c32*)(w.invU + (size_t)s.slot * 4096), x0, nx, b.a,
+                                   b.p, b.inv);
+            else
+                hipLaunchKernelGGL((k_step_u12<T, CW, KC, true>),
+                                   dim3((nx + CW - 1) / CW, 1, b.nb), dim3(256), 0, st, A, ld, pm,
+                                   s.j, s.jb, (const cx<T>*)(w.invU + (size_t)s.slot * 4096), x0,
+                                   nx, b.a, b.p, b.inv);
             TAGS_HIP(hipGetLastError());
-            tm.end(flops);
+            tm.end(flops * b.nb);
             w.flops += flops;
         }
         if (s.j > c_lo &&
             (rc = gemm<T>(pr, st, s.j - c_lo, nx, s.jb, A + s.j, ld, pm + c_lo, A + x0, ld,
-                          pm + s.j, A + x0, ld, pm + c_lo, 1, false, &w.flops)))
+                          pm + s.j, A + x0, ld, pm + c_lo, 1, false, &w.flops, w.bs())))
             return rc;
     }
     return 0;
@@ -1476,16 +1612,21 @@ static int nodal_prepare(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, l
                                    nn)))
             return rc;
     Timed tm(pr, FAM_OTHER, st);
-    hipLaunchKernelGGL(k_nodal_schur<T>, dim3(nn), dim3(256), 0, st, A, ld, pm, m, nn);
+    hipLaunchKernelGGL(k_nodal_schur<T>, dim3(nn, 1, w.nb), dim3(256), 0, st, A, ld, pm, m, nn,
+                       w.sA, (long)w.n);
     TAGS_HIP(hipGetLastError());
     tm.end();
     return 0;
 }
 
 template <typename T>
-int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, const LUPlan* plan)
+int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, const LUPlan* plan,
+              int nb, long sA)
 {
     const int n = w.n;
+    if (nb < 1 || nb > w.batch || (nb > 1 && sA < (long)n * ld)) return TAGS_ERR_ARG;
+    w.nb = nb;
+    w.sA = nb > 1 ? sA : 0;
     int rc = w.setup_kernel_attrs();
     if (rc) return rc;
     LUPlan dense;
@@ -1523,16 +1664,16 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
                                                                     2 * plan->nodal_m))
         return TAGS_ERR_ARG;
     w.nodal_m = plan->nodal_m;
-    hipLaunchKernelGGL(k_lu_init, dim3((n + 255) / 256), dim3(256), 0, st, w.perm, init, n,
-                       w.flags);
+    hipLaunchKernelGGL(k_lu_init, dim3((n + 255) / 256, 1, w.nb), dim3(256), 0, st, w.perm,
+                       init, n, w.flags, (long)n, w.bs().f);
     TAGS_HIP(hipGetLastError());
     // a phase's row order, set at its first step on the row map that step uses
     auto set_order = [&](const LUStep& sp, int* pm, hipStream_t sq) -> int {
         const LUPhase& ph = plan->phases[sp.phase];
         if (ph.order_off < 0 || sp.j != ph.c_begin) return 0;
         const int len = n - ph.c_begin;
-        hipLaunchKernelGGL(k_set_order, dim3((len + 255) / 256), dim3(256), 0, sq, pm,
-                           ph.c_begin, len, w.d_orders + ph.order_off);
+        hipLaunchKernelGGL(k_set_order, dim3((len + 255) / 256, 1, w.nb), dim3(256), 0, sq, pm,
+                           ph.c_begin, len, w.d_orders + ph.order_off, (long)n);
         TAGS_HIP(hipGetLastError());
         return 0;
     };
@@ -1629,8 +1770,8 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             TAGS_HIP(hipStreamWaitEvent(w.s2, w.ev_left, 0));
             int* pnx = pcur;
             if (own_map) {
-                TAGS_HIP(hipMemcpyAsync(palt, pcur, sizeof(int) * n, hipMemcpyDeviceToDevice,
-                                        w.s2));
+                TAGS_HIP(hipMemcpyAsync(palt, pcur, sizeof(int) * n * (size_t)w.nb,
+                                        hipMemcpyDeviceToDevice, w.s2));
                 if ((rc = set_order(nx, palt, w.s2))) return rc;
                 pnx = palt;
             }
@@ -1686,25 +1827,30 @@ template <typename T>
 static void range_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
                         int c_lo, int c_hi)
 {
+    const BatchStrides b = w.bs();
     const int ns = (int)w.step_j.size();
     for (int k = 0; k < ns; k++) {
         const int j = w.step_j[k], jb = w.step_w[k];
         if (j < c_lo || j >= c_hi) continue;
-        hipLaunchKernelGGL((k_step_trsv<T, false>), dim3(1), dim3(256), 0, st, A, ld, w.perm, y,
-                           j, jb, (const cx<T>*)(w.invL + (size_t)w.step_slot[k] * 4096));
+        hipLaunchKernelGGL((k_step_trsv<T, false>), dim3(1, 1, b.nb), dim3(256), 0, st, A, ld,
+                           w.perm, y, j, jb,
+                           (const cx<T>*)(w.invL + (size_t)w.step_slot[k] * 4096), b.a, b.p, b.v,
+                           b.inv);
         const int lo = j + jb;
         if (lo < c_hi)
-            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((c_hi - lo + 3) / 4), dim3(256), 0, st, A,
-                               ld, w.perm, y, lo, c_hi, j, jb);
+            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((c_hi - lo + 3) / 4, 1, b.nb), dim3(256), 0,
+                               st, A, ld, w.perm, y, lo, c_hi, j, jb, b.a, b.p, b.v);
     }
     for (int k = ns - 1; k >= 0; k--) {
         const int j = w.step_j[k], jb = w.step_w[k];
         if (j < c_lo || j >= c_hi) continue;
-        hipLaunchKernelGGL((k_step_trsv<T, true>), dim3(1), dim3(256), 0, st, A, ld, w.perm, y,
-                           j, jb, (const cx<T>*)(w.invU + (size_t)w.step_slot[k] * 4096));
+        hipLaunchKernelGGL((k_step_trsv<T, true>), dim3(1, 1, b.nb), dim3(256), 0, st, A, ld,
+                           w.perm, y, j, jb,
+                           (const cx<T>*)(w.invU + (size_t)w.step_slot[k] * 4096), b.a, b.p, b.v,
+                           b.inv);
         if (j > c_lo)
-            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((j - c_lo + 3) / 4), dim3(256), 0, st, A,
-                               ld, w.perm, y, c_lo, j, j, jb);
+            hipLaunchKernelGGL(k_panel_gemv<T>, dim3((j - c_lo + 3) / 4, 1, b.nb), dim3(256), 0,
+                               st, A, ld, w.perm, y, c_lo, j, j, jb, b.a, b.p, b.v);
     }
 }
 
@@ -1718,11 +1864,12 @@ static void nodal_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStre
     const int n = w.n, m = w.nodal_m;
     range_solve<T>(w, A, ld, y, st, 0, m);
     range_solve<T>(w, A, ld, y, st, m, 2 * m);
-    hipLaunchKernelGGL(k_rows_gemv<T>, dim3(n - 2 * m), dim3(256), 0, st, A, ld, w.perm, y,
-                       2 * m, n, 0, 2 * m);
+    const BatchStrides b = w.bs();
+    hipLaunchKernelGGL(k_rows_gemv<T>, dim3(n - 2 * m, 1, b.nb), dim3(256), 0, st, A, ld, w.perm,
+                       y, 2 * m, n, 0, 2 * m, b.a, b.p, b.v);
     range_solve<T>(w, A, ld, y, st, 2 * m, n);
-    hipLaunchKernelGGL(k_rows_gemv<T>, dim3(2 * m), dim3(256), 0, st, A, ld, w.perm, y, 0,
-                       2 * m, 2 * m, n - 2 * m);
+    hipLaunchKernelGGL(k_rows_gemv<T>, dim3(2 * m, 1, b.nb), dim3(256), 0, st, A, ld, w.perm, y,
+                       0, 2 * m, 2 * m, n - 2 * m, b.a, b.p, b.v);
 }
 
 template <typename T>
@@ -1746,7 +1893,7 @@ int lu_solve_inplace(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_
 template <typename T>
 int lu_solve_cols(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, int x0, int nx)
 {
-    if (w.nodal_m > 0 || nx < 0 || x0 < w.n) return TAGS_ERR_ARG;
+    if (w.nodal_m > 0 || nx < 0 || x0 < w.n || w.nb != 1) return TAGS_ERR_ARG;
     if (nx == 0) return 0;
     std::vector<LUStep> steps;
     for (size_t k = 0; k < w.step_j.size(); k++)
@@ -1757,62 +1904,72 @@ int lu_solve_cols(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr,
 template <typename T>
 int lu_gather_rhs(LUWork<T>& w, const cx<T>* b, cx<T>* y, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_gather_vec<T>, dim3((w.n + 255) / 256), dim3(256), 0, st, b, w.perm,
-                       w.n, y);
+    hipLaunchKernelGGL(k_gather_vec<T>, dim3((w.n + 255) / 256, 1, w.nb), dim3(256), 0, st, b,
+                       w.perm, w.n, y);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
 int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
                unsigned long long* norms, bool want_wnorm, hipStream_t st, Profiler* pr,
-               bool zero_norms)
+               bool zero_norms, const VecBatch& vb)
 {
     Timed tm(pr, FAM_RESID, st);
-    if (zero_norms)
+    if (zero_norms) {
+        if (vb.nb != 1) return TAGS_ERR_ARG;
         TAGS_HIP(hipMemsetAsync(norms, 0, sizeof(unsigned long long) * (want_wnorm ? 3 : 2), st));
-    hipLaunchKernelGGL(k_residual, dim3(n), dim3(256), 0, st, W, ld, n, x, b, r, norms,
-                       want_wnorm ? 1 : 0);
+    }
+    hipLaunchKernelGGL(k_residual, dim3(n, 1, vb.nb), dim3(256), 0, st, W, ld, n, x, b, r, norms,
+                       want_wnorm ? 1 : 0, vb.sw, vb.sn, vb.active);
     TAGS_HIP(hipGetLastError());
     tm.end();
     return 0;
 }
 
-int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t st)
+int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t st,
+              const VecBatch& vb)
 {
-    hipLaunchKernelGGL(k_row_scale, dim3(n), dim3(256), 0, st, W, ld, n, d, enabled ? 1 : 0);
+    hipLaunchKernelGGL(k_row_scale, dim3(n, 1, vb.nb), dim3(256), 0, st, W, ld, n, d,
+                       enabled ? 1 : 0, vb.sw);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
 int to_c32_scaled(const c64* W, long ld, int n, const double* d, c32* W32, long ld32,
-                  hipStream_t st)
+                  hipStream_t st, const VecBatch& vb)
 {
     unsigned gx = (unsigned)std::min(16, (n + 255) / 256);
-    hipLaunchKernelGGL(k_to_c32_scaled, dim3(gx, n), dim3(256), 0, st, W, ld, n, d, W32, ld32);
+    hipLaunchKernelGGL(k_to_c32_scaled, dim3(gx, n, vb.nb), dim3(256), 0, st, W, ld, n, d, W32,
+                       ld32, vb.sw, vb.sw32);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
 int prep_rhs32(const c64* r, const double* d, const int* perm, int n, double inv_scale, c32* t,
-               hipStream_t st, const Grounding* gr, c64* rg, const unsigned long long* rn)
+               hipStream_t st, const Grounding* gr, c64* rg, const unsigned long long* rn,
+               const VecBatch& vb)
 {
     const bool on = gr && gr->g >= 0;
+    if (on && vb.nb != 1) return TAGS_ERR_ARG;
     if (on) {
         hipLaunchKernelGGL(k_ground_rhs, dim3(1), dim3(256), 0, st, r, gr->nb, gr->ne, rg);
         TAGS_HIP(hipGetLastError());
     }
-    hipLaunchKernelGGL(k_prep_rhs32, dim3((n + 255) / 256), dim3(256), 0, st, r, d, perm, n,
-                       inv_scale, rn, on ? gr->g : -1, rg, t);
+    hipLaunchKernelGGL(k_prep_rhs32, dim3((n + 255) / 256, 1, vb.nb), dim3(256), 0, st, r, d,
+                       perm, n, inv_scale, rn, on ? gr->g : -1, rg, t, vb.sn, vb.active);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
 int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStream_t st,
-             const Grounding* gr, const unsigned long long* rn, unsigned long long* dxn)
+             const Grounding* gr, const unsigned long long* rn, unsigned long long* dxn,
+             const VecBatch& vb)
 {
     const bool on = gr && gr->g >= 0;
-    hipLaunchKernelGGL(k_update_x, dim3((n + 255) / 256), dim3(256), 0, st, x, z, d, n, scale,
-                       rn, on ? gr->g : -1, on ? gr->nb : 0, on ? gr->ne : 0, dxn);
+    if (on && vb.nb != 1) return TAGS_ERR_ARG;
+    hipLaunchKernelGGL(k_update_x, dim3((n + 255) / 256, 1, vb.nb), dim3(256), 0, st, x, z, d, n,
+                       scale, rn, on ? gr->g : -1, on ? gr->nb : 0, on ? gr->ne : 0, dxn, vb.sn,
+                       vb.active);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
@@ -1860,8 +2017,9 @@ int sym_from_upper(c64* W, long ld, int n, hipStream_t st)
 template struct LUWork<double>;
 template struct LUWork<float>;
 template int lu_factor<double>(LUWork<double>&, c64*, long, hipStream_t, Profiler*,
-                               const LUPlan*);
-template int lu_factor<float>(LUWork<float>&, c32*, long, hipStream_t, Profiler*, const LUPlan*);
+                               const LUPlan*, int, long);
+template int lu_factor<float>(LUWork<float>&, c32*, long, hipStream_t, Profiler*, const LUPlan*,
+                              int, long);
 template int lu_solve_inplace<double>(LUWork<double>&, const c64*, long, c64*, hipStream_t,
                                       Profiler*);
 template int lu_solve_inplace<float>(LUWork<float>&, const c32*, long, c32*, hipStream_t,

@@ -165,13 +165,17 @@ int launch_pairs_dropin(double* zl, double* zt, bool alias, SegSoA e, SegSoA img
 void gauss_legendre(int n, double* x, double* w);
 int launch_mhem_potentials(double* pl, double* pt, double* pli, double* pti, long m, SegSoA e,
                            SegSoA img, hipStream_t st);
+// Fills of nb frequency samples at once (nb > 1: W of item z at W + z sw, its
+// scalars scs[z] in device memory; nb == 1 with scs null: `sc`).
 int launch_fill_mhem(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
                      const double* pl, const double* pt, const double* pli, const double* pti,
-                     const PairScalars& sc, hipStream_t st);
+                     const PairScalars& sc, hipStream_t st, int nb = 1, long sw = 0,
+                     const PairScalars* scs = nullptr);
 int launch_fill_zblocks(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA img,
-                        const PairScalars& sc, hipStream_t st);
+                        const PairScalars& sc, hipStream_t st, int nb = 1, long sw = 0,
+                        const PairScalars* scs = nullptr);
 int launch_fill_static(c64* W, long ldw, const BlockLayout& L, const int* seg_start_node,
-                       const int* seg_end_node, hipStream_t st);
+                       const int* seg_end_node, hipStream_t st, int nb = 1, long sw = 0);
 int launch_single_pair(SegSoA e, SegSoA r, c64 gamma, int type, double* out4, hipStream_t st);
 int launch_incidence(const double* seg_pts, long m, const double* nodes, long n,
                      int* seg_start_node, int* seg_end_node, int* node_hits,

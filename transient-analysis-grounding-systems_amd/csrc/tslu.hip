@@ -56,6 +56,8 @@ struct TsluArgs {
     unsigned* cnt;  // arrival counters of the tree nodes (zero between launches)
     int* info;
     TsluTree tree;
+    // batched launch: item blockIdx.z at these strides (elements)
+    long sa, sp, scand, scnt, sf;
 };
 
 __device__ inline cx<float> readlane_cx(cx<float> v, int l)
@@ -333,6 +335,14 @@ __device__ __forceinline__ int tslu_gepp(StackArr<T>& a, const int* s_lrow, int 
 template <typename T>
 __global__ __launch_bounds__(TS_NT) void k_tslu(TsluArgs<T> p)
 {
+    {
+        const long z = blockIdx.z;
+        p.A += z * p.sa;
+        p.perm += z * p.sp;
+        p.cand += z * p.scand;
+        p.cnt += z * p.scnt;
+        p.info += z * p.sf;
+    }
     constexpr int TS_RPT = TsShape<T>::RPT, TS_S = TsShape<T>::S, TS_F = TsShape<T>::F;
     __shared__ int s_lrow[TS_S];
     __shared__ int s_piv[64];
@@ -487,8 +497,10 @@ int tslu_min_leaf_rows() { return TsShape<double>::S; }
 
 template <typename T>
 int launch_tslu(cx<T>* A, long ld, int* perm, int c0, int w, int row_end, int cand_end,
-                int* cand, unsigned* cnt, int* info, hipStream_t st)
+                int* cand, unsigned* cnt, int* info, hipStream_t st, const BatchStrides& bs)
 {
+    if (bs.nb <= 0) return 0;
+    if (bs.nb > 65535) return TAGS_ERR_ARG;
     if (w <= 0 || w > 64 || cand_end - c0 < w || row_end < cand_end) return TAGS_ERR_ARG;
     TsluArgs<T> p;
     p.A = A;
@@ -500,16 +512,21 @@ int launch_tslu(cx<T>* A, long ld, int* perm, int c0, int w, int row_end, int ca
     p.cand = cand;
     p.cnt = cnt;
     p.info = info;
+    p.sa = bs.a;
+    p.sp = bs.p;
+    p.scand = bs.cand;
+    p.scnt = bs.cnt;
+    p.sf = bs.f;
     int rc = tslu_tree(cand_end - c0, TsShape<T>::S, &p.tree);
     if (rc) return rc;
-    hipLaunchKernelGGL(k_tslu<T>, dim3(p.tree.nodes[0]), dim3(TS_NT), 0, st, p);
+    hipLaunchKernelGGL(k_tslu<T>, dim3(p.tree.nodes[0], 1, bs.nb), dim3(TS_NT), 0, st, p);
     TAGS_HIP(hipGetLastError());
     return 0;
 }
 
 template int launch_tslu<double>(c64*, long, int*, int, int, int, int, int*, unsigned*, int*,
-                                 hipStream_t);
+                                 hipStream_t, const BatchStrides&);
 template int launch_tslu<float>(c32*, long, int*, int, int, int, int, int*, unsigned*, int*,
-                                hipStream_t);
+                                hipStream_t, const BatchStrides&);
 
 }  // namespace tags
