@@ -16,6 +16,17 @@ between the two. The per-kernel figures (roofline, timers) come from a second
 timed pass over the same samples with one in flight, so that each launch's
 HIP-event duration is its own.
 
+The same line carries the other configurations of the workload family as
+keys, each timed over all its samples in both precisions (the library groups
+samples per launch), with the trailing update's roofline from a profiled pass
+and the oracle on the host as CPU baseline:
+  c1  the 15 m conductor (N = 91), 200 frequencies;
+  c2  the 20 x 20 m grid (N = 1185), 400 frequencies;
+  c5  the 30 x 30 m grid in Alipio-Visacro soil (N = 3861), every 8th of the
+      8193 numerical-Laplace samples;
+  c4  512 seeded cases (N = 61 .. 4246), every 8th of their frequencies, in
+      one multi-case solve per precision.
+
 Prints one JSON line. Single GPU only. `--rows` instead times the calls next
 to the path (nodal admittance, the Laplace transform pair, fields), and
 `--transient` runs the lightning transient on the 30 x 30 m grid end to end;
@@ -242,6 +253,207 @@ def transient_bench(T, nt=2 ** 14):
     print(json.dumps(out))
 
 
+def _sync(T):
+    T.lib().tags_device_synchronize()
+
+
+def _timed_solve(T, obj, prec, *args):
+    _sync(T)
+    t0 = time.perf_counter()
+    r = obj.solve(*args, precision=prec) if args else obj.solve(prec)
+    _sync(T)
+    return r, time.perf_counter() - t0
+
+
+def _roofline(tm, peak, kernel):
+    """Trailing-update GEMM of a profiled pass: algorithmic 8MNK flops over the
+    HIP-event time of its launches."""
+    if not tm or tm["trailing_launches"] <= 0 or tm["trailing_ms"] <= 0:
+        return None
+    ach = tm["trailing_flops"] / tm["trailing_ms"] / 1e9
+    return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+            "frac": ach / peak, "traffic": None, "kernel": kernel,
+            "launches": tm["trailing_launches"],
+            "avg_launch_ms": tm["trailing_ms"] / tm["trailing_launches"]}
+
+
+def sweep_config(T, e, nodes, inj, s, kap, plan=7):
+    """Both precisions of one sweep over all its samples (groups of samples per
+    launch as the library chooses), one unprofiled timed pass each, then a
+    profiled pass for the trailing-update figures."""
+    sw = T.Sweep(e, nodes)
+    sw.set_plan(plan)
+    out = {}
+    us = {}
+    for name, prec in (("mixed", T.PREC_MIXED), ("fp64", T.PREC_FP64)):
+        sw.solve(s[:2], kap[:2], inj, prec)  # allocations
+        (u, st), dt = _timed_solve(T, sw, prec, s, kap, inj)
+        sw.set_profiling(True)
+        sw.solve(s, kap, inj, prec)
+        tm = sw.timers()
+        sw.set_profiling(False)
+        us[name] = u
+        r = {"samples_per_s": len(s) / dt, "s": dt,
+             "roofline": _roofline(tm, PEAK["fp32" if prec == T.PREC_MIXED else "fp64"],
+                                   "k_trailing_update<%s>" % ("float, 3-multiply"
+                                                              if prec == T.PREC_MIXED
+                                                              else "double")),
+             "executed_gemm_flops_per_sample": tm["gemm_flops"] / len(s)}
+        if prec == T.PREC_MIXED:
+            r["fp64_fallbacks"] = int(sum(q["precision_used"] == 0 for q in st))
+            r["refine_iters_mean"] = float(np.mean([q["refine_iters"] for q in st]))
+            r["forward_est_max"] = float(max(q["forward_est"] for q in st))
+        out[name] = r
+    sw.close()
+    d = np.abs(us["mixed"] - us["fp64"]) / np.abs(us["fp64"])
+    out["mixed_over_fp64"] = out["mixed"]["samples_per_s"] / out["fp64"]["samples_per_s"]
+    out["zh_rel_diff_mixed_vs_fp64"] = {"max": float(d.max()), "median": float(np.median(d))}
+    return out, us
+
+
+def cpu_oracle_samples(e, nodes, freqs, sigma, epsr, threads, inj=0, soil=None):
+    """The oracle's pipeline (Genz-Malik cubature at the reference's rel tol 1e-6,
+    LAPACK zsysv) over the given samples, timed end to end; soil(s) -> kappa
+    overrides the constant soil."""
+    import tags_pkg
+    O = tags_pkg.oracle()
+    images = O.images_of(e)
+    m, n = len(e), len(nodes)
+    t0 = time.perf_counter()
+    for f in freqs:
+        if soil is None:
+            sc = O.soil_scalars(f, sigma, epsr)
+        else:
+            sk = complex(f)
+            kk = soil(sk)
+            sc = dict(s=sk, kappa=kk, gamma=complex(np.sqrt(sk * O.MU0 * kk + 0j)),
+                      ref_l=1.0 + 0j, ref_t=(kk - sk * O.EPS0) / (kk + sk * O.EPS0), mur=1.0)
+        wg, _, _ = O.assemble(e, nodes, images, sc, rel_tol=1e-6, threads=threads)
+        ie = np.zeros(n + 2 * m, complex)
+        ie[inj] = 1.0
+        O.solve_immittance(wg, ie, m, n)
+    return time.perf_counter() - t0
+
+
+def more_configs(T, threads):
+    """The other configurations of the workload family, each one key."""
+    res = {}
+    # c1: one 15 m conductor, 30 segments, 200 frequencies 100 Hz - 10 MHz
+    rc, e, nodes = T.segment_electrode([0, 0, -0.5], [15, 0, -0.5], 30, 7e-3)
+    f = np.logspace(2, 7, 200)
+    s = 2j * np.pi * f
+    kap = 0.01 + s * 10.0 * T.EPS0
+    r, _ = sweep_config(T, e, nodes, 0, s, kap)
+    t = cpu_oracle_samples(e, nodes, f, 0.01, 10.0, threads)
+    r["cpu_baseline"] = {"value": len(f) / t, "unit": "freq samples/s", "cores": threads,
+                         "kind": "port",
+                         "sample": "oracle end to end over all 200 frequencies (cubature rel tol "
+                                   "1e-6, LAPACK zsysv), %d threads: %.2f s" % (threads, t)}
+    r["workload"] = "15 m conductor, m=30, N=91, rho=100, eps_r=10, 200 freqs 100 Hz-10 MHz"
+    res["c1"] = r
+    # c2: 20 x 20 m grid, 5 m mesh, N = 1185, 400 frequencies
+    rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
+    f = np.logspace(2, 7, 400)
+    s = 2j * np.pi * f
+    kap = 1e-3 + s * 15.0 * T.EPS0
+    r, _ = sweep_config(T, e, nodes, 0, s, kap)
+    fs = f[[0, 133, 266, 399]]
+    t = cpu_oracle_samples(e, nodes, fs, 1e-3, 15.0, threads)
+    r["cpu_baseline"] = {"value": len(fs) / t, "unit": "freq samples/s", "cores": threads,
+                         "kind": "port",
+                         "sample": "oracle end to end at 4 of the 400 frequencies, %d threads: "
+                                   "%.2f s" % (threads, t)}
+    r["workload"] = ("20x20 m grid, 5 m mesh (m=400, n=385, N=1185), rho=1000, eps_r=15, "
+                     "400 freqs 100 Hz-10 MHz, corner injection")
+    res["c2"] = r
+    # c5: 30 x 30 m grid in Alipio-Visacro soil, every 8th of the 8193 s-samples
+    _, e, nodes = T.electrode_grid(11, 11, 30.0, 30.0, 6, 6, 5e-3, -0.5)
+    inj = int(np.argmin(np.sum((nodes - [15.0, 15.0, -0.5]) ** 2, axis=1)))
+    nt, tmax = 2 ** 14, 1e-4
+    k = np.arange(0, nt // 2 + 1, 8)
+    s = np.log(nt ** 2) / tmax + 2j * np.pi * k / tmax
+    sigma0 = 5e-4
+    h = 1.26 * (1000 * sigma0) ** -0.73
+
+    def soil(sk):
+        sg, er = T.alipio_soil(sigma0, sk, h, 0.54, 12.0)
+        return sg + sk * er * T.EPS0
+    kap = np.array([soil(x) for x in s])
+    r, _ = sweep_config(T, e, nodes, inj, s, kap)
+    t = cpu_oracle_samples(e, nodes, s[:1], None, None, threads, inj=inj, soil=soil)
+    r["cpu_baseline"] = {"value": 1.0 / t, "unit": "freq samples/s", "cores": threads,
+                         "kind": "port",
+                         "sample": "oracle end to end at the lowest |s| sample, %d threads: "
+                                   "%.2f s" % (threads, t)}
+    r["workload"] = ("30x30 m grid, 5 m mesh (m=1320, n=1221, N=3861), Alipio-Visacro mean soil, "
+                     "every 8th of the 8193 Laplace samples (nt=2^14, tmax=100 us), centre "
+                     "injection")
+    res["c5"] = r
+    res["c4"] = cases_config(T, threads)
+    return res
+
+
+def cases_config(T, threads, step=8):
+    """512 seeded cases (rods, counterpoises, grids; N = 61 .. 4246), every
+    step-th frequency of each, one multi-case solve per precision."""
+    import tags_pkg
+    C = tags_pkg.cases()
+    fam = C.case_family(T, freq_step=step)
+    n = sum(len(c["s"]) for c in fam)
+    cs = T.Cases(fam)
+    out = {"workload": "512 seeded cases (seed 2024), every %dth of their 64-512 frequencies: "
+                       "%d samples" % (step, n)}
+    us = {}
+    for name, prec in (("mixed", T.PREC_MIXED), ("fp64", T.PREC_FP64)):
+        cs.solve(prec)  # allocations
+        (u, st), dt = _timed_solve(T, cs, prec)
+        cs.set_profiling(True)
+        cs.solve(prec)
+        tm = cs.timers()
+        cs.set_profiling(False)
+        us[name] = u
+        r = {"samples_per_s": n / dt, "s": dt,
+             "roofline": _roofline(tm, PEAK["fp32" if prec == T.PREC_MIXED else "fp64"],
+                                   "k_trailing_update"),
+             "dense_equivalent_tflops": sum(len(c["s"]) * lu_flops(c["N"]) for c in fam) / dt
+             / 1e12}
+        if prec == T.PREC_MIXED:
+            r["fp64_fallbacks"] = int(sum(q["precision_used"] == 0 for s_ in st for q in s_))
+        out[name] = r
+    cs.close()
+    out["mixed_over_fp64"] = out["mixed"]["samples_per_s"] / out["fp64"]["samples_per_s"]
+    out["zh_rel_diff_mixed_vs_fp64_max"] = max(float(np.max(np.abs(a - b) / np.abs(b)))
+                                               for a, b in zip(us["mixed"], us["fp64"]))
+    # CPU: one sample of each of 6 cases spread over the size range, timed; the
+    # whole workload's CPU time modelled per case as t(N) interpolated in
+    # log-log between those measurements
+    Ns = np.array([c["N"] for c in fam])
+    pick = [int(np.argmin(np.abs(Ns - q))) for q in (61, 181, 401, 801, 1501, 2201)]
+    tn = []
+    for i in pick:
+        c = fam[i]
+        t = cpu_oracle_samples(c["electrodes"], c["nodes"], c["f"][len(c["f"]) // 2:][:1],
+                               c["sigma"], c["epsr"], threads)
+        tn.append((c["N"], t))
+    tn.sort()
+    lx, ly = np.log([a for a, _ in tn]), np.log([b for _, b in tn])
+    slope = (ly[-1] - ly[-2]) / (lx[-1] - lx[-2])
+
+    def tmodel(N):
+        x = np.log(N)
+        return float(np.exp(np.interp(x, lx, ly) if x <= lx[-1] else ly[-1] + slope * (x - lx[-1])))
+    total = sum(len(c["s"]) * tmodel(c["N"]) for c in fam)
+    out["cpu_baseline"] = {"value": n / total, "unit": "freq samples/s", "cores": threads,
+                           "kind": "port",
+                           "sample": "oracle end to end, one sample each of 6 cases (N = %s: "
+                                     "%s s), %d threads; the workload's CPU time modelled as "
+                                     "t(N) interpolated in log-log (extrapolated as the last "
+                                     "slope above N = %d): %.0f s"
+                                     % ([a for a, _ in tn], ["%.3g" % b for _, b in tn],
+                                        threads, tn[-1][0], total)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -249,6 +461,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="only the 60 x 60 m grid line (no c1, c2, c5, c4 keys)")
     ap.add_argument("--plan-mixed", type=int, default=7,
                     help="LU plan of the mixed form: 0 dense, 1 structured, 2 Z_L first, "
                          "3 Z_L-block pivoting, 4 grounded without row exchanges, 5 "
@@ -389,6 +603,7 @@ def main():
             _, _, dt641, tm64 = run(T.PREC_FP64, 1, True)
         rel = float(np.max(np.abs(u_mx - u64) / np.abs(u64)))
         out["fp64"] = {"value": K / dt64, "ms_per_step": 1e3 * dt64 / K,
+                       "executed_gemm_flops_per_sample": tm64["gemm_flops"] / K,
                        "plan": PLAN_NAMES[args.plan_fp64],
                        "step_width": args.step_width_fp64,
                        "inflight": args.inflight,
@@ -396,7 +611,7 @@ def main():
                        "timers_ms_one_in_flight": {k: v for k, v in tm64.items()
                                                    if k.startswith("ms_")},
                        "trailing_gemm_tflops": tm64["trailing_flops"] / tm64["trailing_ms"] / 1e9,
-                       "lu_tflops_per_sample": lu_flops(N) * K / dt64 / 1e12}
+                       "dense_equivalent_tflops": lu_flops(N) * K / dt64 / 1e12}
         out["mixed_over_fp64"] = (K / dt_mx) / (K / dt64)
         out["max_rel_err_zh_vs_fp64"] = rel
     ach = tm_mx["trailing_flops"] / tm_mx["trailing_ms"] / 1e9
@@ -409,11 +624,19 @@ def main():
                        "avg_launch_ms": tm_mx["trailing_ms"] / max(1, tm_mx["trailing_launches"]),
                        "measured_in": ("the timed samples" if one else
                                        "the same %d samples timed one at a time" % K)}
-    out["lu_algorithmic_tflops"] = lu_flops(N) * K / dt_mx / 1e12
+    # F(N) = (8/3) N^3 + 8 N^2 per sample counts the dense LU, including the
+    # exactly-zero blocks the plans skip: a dense-equivalent rate, not a
+    # hardware rate; the executed one counts the products the plans ran
+    out["dense_equivalent_tflops"] = lu_flops(N) * K / dt_mx / 1e12
+    out["mixed"]["executed_gemm_flops_per_sample"] = tm_mx["gemm_flops"] / K
+    out["mixed"]["executed_gemm_tflops"] = tm_mx["gemm_flops"] / K * (K / dt_mx) / 1e12
+    out["mixed"]["forward_est_max"] = max(q["forward_est"] for q in st_mx)
+    threads = min(16, os.cpu_count() or 1)
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
         out["cpu_baseline"] = cpu_baseline(T, e, nodes, N, threads)
     sw.close()
+    if not args.headline_only:
+        out.update(more_configs(T, threads))
     print(json.dumps(out))
 
 

@@ -269,9 +269,11 @@ typedef struct {
                              not computed: the FP64 form of a sweep without
                              tags_sweep_set_verify) */
     double refine_hist[8];/* backward error after each refinement step (mixed) */
-    double forward_est;   /* mixed: ||dx||_inf / ||x||_inf of the last refinement
-                             step, an estimate of the error left in x (accepted at
-                             <= 1e-11); -1 for the FP64 form */
+    double forward_est;   /* mixed: estimated error left in x relative to ||x||_inf,
+                             rho / (1 - rho) ||dx||_inf / ||x||_inf with dx the last
+                             correction and rho the observed contraction per step
+                             (||dx|| itself where rho >= 1/2); accepted at <= 1e-10;
+                             -1 for the FP64 form */
 } tags_item_stats;
 
 typedef struct {
@@ -486,6 +488,57 @@ int tags_sweep_assemble_entries(tags_sweep *sw, TAGS_COMPLEX s, TAGS_COMPLEX kap
 int tags_sweep_set_profiling(tags_sweep *sw, int on);
 int tags_sweep_get_timers(tags_sweep *sw, tags_timers *t);
 void tags_sweep_destroy(tags_sweep *sw);
+
+/* ---------------------------------------------------------------------------
+ * Many grounding cases at once (no reference counterpart: the reference's
+ * drivers run one case per process, its frequencies in an OpenMP loop,
+ * examples/harmonic_impedance.c:87-176). Each case is a geometry with its own
+ * frequency samples; every (case, frequency) sample of every case is solved
+ * for a unit current at the case's injection node, with the case's
+ * reference-caller scalars (gamma = csqrt(s mu0 mur kappa), ref_l = 1,
+ * ref_t = (kappa - s eps0)/(kappa + s eps0)).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    const Electrode *electrodes;  /* m segments */
+    const Electrode *images;      /* m images, or NULL for z -> -z images */
+    size_t num_electrodes;
+    const double *nodes;          /* n nodes, 3 doubles each */
+    size_t num_nodes;
+    size_t inj_node;              /* unit current injected here */
+    size_t nfreq;                 /* frequency samples of the case */
+    const TAGS_COMPLEX *s;        /* nfreq complex frequencies */
+    const TAGS_COMPLEX *kappa;    /* nfreq soil complex conductivities */
+    double mur;
+    TAGS_COMPLEX *u_inj;          /* out: nfreq values U[inj_node] (Z_h) */
+    tags_item_stats *stats;       /* out, nullable: nfreq entries */
+} tags_case;
+
+typedef struct tags_cases tags_cases;
+
+/* Uploads every case's geometry (one tags_sweep each; the incidence is matched
+ * on the GPU). The descriptors' arrays are read again by tags_cases_solve and
+ * must stay valid until tags_cases_destroy. Returns 0, TAGS_ERR_ARG, or the
+ * first case's tags_sweep_create code. */
+int tags_cases_create(tags_cases **out, const tags_case *cases, size_t ncases);
+
+/* Solves every sample of every case in `precision` (tags_precision). The
+ * samples are grouped per case (tags_sweep_set_batch's automatic size), the
+ * groups of all cases queued largest first (per-sample cost
+ * (8/3) N^3 + 6400 m^2) and taken by the lanes (tags_cases_set_lanes) as each
+ * finishes its last group. Results as tags_sweep_solve's for each case. */
+int tags_cases_solve(tags_cases *cs, int precision);
+
+/* Factorisation plan of every case (tags_plan, default TAGS_PLAN_HYBRID). */
+int tags_cases_set_plan(tags_cases *cs, int plan);
+
+/* Lanes (1 .. 4, default 3): groups in flight, each with its own stream, W and
+ * factor work space, sized for the largest group it has taken. */
+int tags_cases_set_lanes(tags_cases *cs, int lanes);
+
+/* HIP-event timings of every lane's kernels, as tags_sweep_get_timers. */
+int tags_cases_set_profiling(tags_cases *cs, int on);
+int tags_cases_get_timers(tags_cases *cs, tags_timers *t);
+void tags_cases_destroy(tags_cases *cs);
 
 /* Dense solve of a device-resident or host complex matrix, for tests and
  * library use. `a` is row-major n x n (for the symmetric W the same as column-

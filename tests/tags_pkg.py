@@ -53,3 +53,10 @@ def build_c_driver(out=C_DRIVER_BIN, src=C_DRIVER_SRC):
 def build_c_drivers():
     build_c_driver()
     build_c_driver(C_ADM_DRIVER_BIN, os.path.join(C_DRIVER_DIR, "admittance_c1.c"))
+
+
+def cases():
+    """The package's case-family module (many grounding cases from a seed)."""
+    import importlib
+    load()
+    return importlib.import_module("tags_amd.cases")

@@ -4,7 +4,7 @@ covers every sample of the group). Each sample's arithmetic is its own, so
 groups of any size give the one-sample results to rounding (the GEMM tile
 shape follows the size of the whole launch): FP64 within 1e-12, the mixed form
 within 1e-9 (both far inside the 1e-8 bar; the mixed form accepts an iterate
-at a forward-error estimate <= 1e-11). Ragged last groups, groups larger than
+at an estimated error <= 1e-10). Ragged last groups, groups larger than
 the sweep, and the hybrid plan's change of complex64 factor inside a group's
 frequency range are covered, on the 20 x 20 m grid (N = 1185) for every
 plan, and the result of a batched sweep against the oracle at three
@@ -51,7 +51,7 @@ def test_batch_sizes_agree(plan):
         assert np.max(np.abs(umx - u64) / np.abs(u64)) <= 1e-8
         for q in stmx:
             if q["precision_used"] == 1:
-                assert q["backward_err"] <= 1e-15 and q["forward_est"] <= 1e-11, q
+                assert q["backward_err"] <= 1e-15 and q["forward_est"] <= 1e-10, q
 
 
 def test_batched_sweep_vs_oracle():

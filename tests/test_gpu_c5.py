@@ -127,7 +127,7 @@ def test_c5_solve_vs_lapack(k):
     assert 0 <= st64[0]["backward_err"] <= 1e-14
     q = stmx[0]
     if q["precision_used"] == 1:
-        assert q["backward_err"] <= 1e-15 and 0 <= q["forward_est"] <= 1e-11
+        assert q["backward_err"] <= 1e-15 and 0 <= q["forward_est"] <= 1e-10
 
 
 def test_c5_sweep_mixed_vs_fp64_every_16th():
@@ -147,5 +147,5 @@ def test_c5_sweep_mixed_vs_fp64_every_16th():
     assert d.max() <= 1e-8
     for q in st:
         if q["precision_used"] == 1:
-            assert q["backward_err"] <= 1e-15 and q["forward_est"] <= 1e-11, q
+            assert q["backward_err"] <= 1e-15 and q["forward_est"] <= 1e-10, q
     assert np.all(u64.real > 0)

@@ -60,3 +60,74 @@ def test_case_family_sweep_vs_oracle(seed):
     assert np.max(np.abs(u64 - zo) / np.abs(zo)) <= 1e-7
     assert np.max(np.abs(umx - u64) / np.abs(u64)) <= 1e-8
     assert np.all(u64.real > 0)
+
+
+# ---------------------------------------------------------------------------
+# many cases at once (tags_cases_*): the seeded 512-case workload
+# ---------------------------------------------------------------------------
+C = tags_pkg.cases()
+
+
+def test_cases_subset_vs_oracle():
+    """The first 14 cases of the 512-case family with m <= 120 (rods,
+    counterpoises, small grids; random radii and soils), four frequencies each
+    across 100 Hz - 10 MHz, solved together through tags_cases in both
+    precisions: FP64 against the oracle's pipeline (Genz-Malik cubature at rel
+    tol 1e-9, LAPACK zsysv) within 1e-7, mixed against FP64 within 1e-8."""
+    fam = [c for c in C.case_family(T) if c["m"] <= 120][:14]
+    cases = []
+    for c in fam:
+        sel = np.unique([0, c["nf"] // 3, 2 * c["nf"] // 3, c["nf"] - 1])
+        cases.append(dict(c, s=c["s"][sel], kappa=c["kappa"][sel], f=c["f"][sel]))
+    print("cases: %s" % [(c["index"], c["kind"], c["m"]) for c in cases], flush=True)
+    cs = T.Cases(cases)
+    u64, st64 = cs.solve(T.PREC_FP64)
+    umx, stmx = cs.solve(T.PREC_MIXED)
+    cs.close()
+    for c, a, b, st in zip(cases, u64, umx, stmx):
+        zo = O.harmonic_impedance(c["electrodes"], c["nodes"], c["f"], c["sigma"], c["epsr"], 0,
+                                  rel_tol=1e-9, threads=8)
+        assert np.max(np.abs(a - zo) / np.abs(zo)) <= 1e-7, c["index"]
+        assert np.max(np.abs(b - a) / np.abs(a)) <= 1e-8, c["index"]
+        assert all(q["precision_used"] in (0, 1) for q in st)
+
+
+def test_cases_full_family_mixed_vs_fp64():
+    """All 512 cases (N = 61 .. 4246), every 16th frequency of each (9,300
+    samples), through one multi-case solve per precision: every mixed sample
+    within 1e-8 of its FP64 value, and the mixed samples meet the acceptance
+    rule."""
+    fam = C.case_family(T, freq_step=16)
+    cs = T.Cases(fam)
+    u64, _ = cs.solve(T.PREC_FP64)
+    umx, stmx = cs.solve(T.PREC_MIXED)
+    cs.close()
+    worst = max(float(np.max(np.abs(b - a) / np.abs(a))) for a, b in zip(u64, umx))
+    nfb = sum(q["precision_used"] == 0 for st in stmx for q in st)
+    print("full family: %d samples, worst mixed vs fp64 %.3g, fp64 fallbacks %d" %
+          (sum(len(u) for u in u64), worst, nfb), flush=True)
+    assert worst <= 1e-8
+    for st in stmx:
+        for q in st:
+            if q["precision_used"] == 1:
+                assert q["backward_err"] <= 1e-15 and q["forward_est"] <= 1e-10, q
+    assert all(np.all(np.isfinite(u)) for u in u64)
+
+
+def test_cases_match_one_sweep_each():
+    """A multi-case solve gives each case what its own sweep gives (the same
+    groups and kernels: FP64 bit for bit; the mixed form within 1e-9, since the
+    hybrid plan's switch may rise at other points of the lanes' order)."""
+    fam = C.case_family(T, freq_step=8)
+    pick = [fam[i] for i in (6, 11, 23, 40)]
+    cs = T.Cases(pick)
+    u64, _ = cs.solve(T.PREC_FP64)
+    umx, _ = cs.solve(T.PREC_MIXED)
+    cs.close()
+    for c, a, b in zip(pick, u64, umx):
+        sw = T.Sweep(c["electrodes"], c["nodes"])
+        r64, _ = sw.solve(c["s"], c["kappa"], 0, T.PREC_FP64)
+        rmx, _ = sw.solve(c["s"], c["kappa"], 0, T.PREC_MIXED)
+        sw.close()
+        assert np.array_equal(a, r64), c["index"]
+        assert np.max(np.abs(b - rmx) / np.abs(rmx)) <= 1e-9, c["index"]

@@ -359,7 +359,7 @@ def test_c3_hybrid_vs_zl_first():
     switch, the nodal one from it up) against the Z_L-first FP64 factor, at the
     band edges and on both sides of the switch, each within 1e-8; the mixed
     samples meet the acceptance rule (backward error <= 1e-15, forward-error
-    estimate <= 1e-11) and the FP64 ones report a backward error below 1e-13
+    estimate <= 1e-10) and the FP64 ones report a backward error below 1e-13
     (measured 1e-19 .. 2.7e-14; the normwise bound of a stable LU grows like
     N eps, 2.4e-12 at this order)."""
     rc, e, nodes = T.electrode_grid(31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)
@@ -377,7 +377,7 @@ def test_c3_hybrid_vs_zl_first():
           flush=True)
     assert d.max() <= 1e-8
     assert all(q["precision_used"] == 1 and q["backward_err"] <= 1e-15 and
-               q["forward_est"] <= 1e-11 for q in st), st
+               q["forward_est"] <= 1e-10 for q in st), st
     assert all(0 <= q["backward_err"] <= 1e-13 for q in st64), st64
 
 

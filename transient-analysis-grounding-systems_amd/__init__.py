@@ -118,6 +118,15 @@ class ItemStats(ctypes.Structure):
                 ("refine_hist", ctypes.c_double * 8), ("forward_est", ctypes.c_double)]
 
 
+class CaseDesc(ctypes.Structure):
+    """tags_case: one grounding case of a multi-case solve."""
+    _fields_ = [("electrodes", ctypes.c_void_p), ("images", ctypes.c_void_p),
+                ("num_electrodes", ctypes.c_size_t), ("nodes", ctypes.c_void_p),
+                ("num_nodes", ctypes.c_size_t), ("inj_node", ctypes.c_size_t),
+                ("nfreq", ctypes.c_size_t), ("s", ctypes.c_void_p), ("kappa", ctypes.c_void_p),
+                ("mur", ctypes.c_double), ("u_inj", ctypes.c_void_p), ("stats", ctypes.c_void_p)]
+
+
 class Timers(ctypes.Structure):
     _fields_ = [("ms_fill", ctypes.c_double), ("ms_panel", ctypes.c_double),
                 ("ms_gemm", ctypes.c_double), ("ms_other", ctypes.c_double),
@@ -145,7 +154,9 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_count_lines", "tags_frequencies_file", "tags_write_complex_file",
             "electric_potential", "magnetic_potential", "electric_field", "voltage",
             "tags_fields_batch", "tags_mhem_gpd", "tags_sweep_set_verify",
-            "tags_sweep_assemble_entries", "tags_sweep_set_batch"]
+            "tags_sweep_assemble_entries", "tags_sweep_set_batch", "tags_cases_create",
+            "tags_cases_solve", "tags_cases_set_plan", "tags_cases_set_lanes",
+            "tags_cases_set_profiling", "tags_cases_get_timers", "tags_cases_destroy"]
 
 _lib = None
 
@@ -219,6 +230,13 @@ def lib():
         "tags_sweep_set_inflight": ([P, I], I),
         "tags_sweep_set_verify": ([P, I], I),
         "tags_sweep_set_batch": ([P, I], I),
+        "tags_cases_create": ([P, P, S], I),
+        "tags_cases_solve": ([P, I], I),
+        "tags_cases_set_plan": ([P, I], I),
+        "tags_cases_set_lanes": ([P, I], I),
+        "tags_cases_set_profiling": ([P, I], I),
+        "tags_cases_get_timers": ([P, P], I),
+        "tags_cases_destroy": ([P], None),
         "tags_sweep_assemble_entries": ([P, C, C, D, S, P, P, P], I),
         "tags_sweep_assemble": ([P, C, C, D, P], I),
         "tags_immittance_solve": ([P, P, S, S, I, I, P], I),
@@ -686,3 +704,89 @@ class Sweep:
             self.close()
         except Exception:
             pass
+
+
+class Cases:
+    """Many grounding cases solved together (tags_cases_*): `cases` is a list
+    of dicts with electrodes, nodes, inj_node, s, kappa (and optionally
+    images, mur). solve() returns one array of U[inj] per case and the
+    per-sample statistics."""
+
+    def __init__(self, cases):
+        self._keep = []
+        self._u = []
+        self._stats = []
+        descs = (CaseDesc * len(cases))()
+        for i, c in enumerate(cases):
+            e = _electrodes(c["electrodes"])
+            nodes = np.ascontiguousarray(c["nodes"], np.float64)
+            im = None if c.get("images") is None else _electrodes(c["images"])
+            s = np.ascontiguousarray(c["s"], np.complex128)
+            kap = np.ascontiguousarray(c["kappa"], np.complex128)
+            u = np.zeros(len(s), np.complex128)
+            st = (ItemStats * max(1, len(s)))()
+            self._keep += [e, nodes, im, s, kap]
+            self._u.append(u)
+            self._stats.append(st)
+            d = descs[i]
+            d.electrodes = e.ctypes.data
+            d.images = None if im is None else im.ctypes.data
+            d.num_electrodes = len(e)
+            d.nodes = nodes.ctypes.data
+            d.num_nodes = len(nodes)
+            d.inj_node = int(c.get("inj_node", 0))
+            d.nfreq = len(s)
+            d.s = s.ctypes.data
+            d.kappa = kap.ctypes.data
+            d.mur = float(c.get("mur", 1.0))
+            d.u_inj = u.ctypes.data
+            d.stats = ctypes.cast(st, ctypes.c_void_p)
+        self._descs = descs
+        self._h = ctypes.c_void_p()
+        rc = lib().tags_cases_create(ctypes.byref(self._h), ctypes.cast(descs, ctypes.c_void_p),
+                                     len(cases))
+        if rc != 0:
+            raise RuntimeError("tags_cases_create failed: %d" % rc)
+        _live.add(self)
+
+    def solve(self, precision=PREC_MIXED):
+        rc = lib().tags_cases_solve(self._h, precision)
+        if rc != 0:
+            raise RuntimeError("tags_cases_solve failed: %d" % rc)
+        us = [u.copy() for u in self._u]
+        sts = [[dict(precision_used=q.precision_used, refine_iters=q.refine_iters, info=q.info,
+                     backward_err=q.backward_err, forward_est=q.forward_est)
+                for q in st[:len(u)]] for st, u in zip(self._stats, self._u)]
+        return us, sts
+
+    def set_plan(self, plan):
+        rc = lib().tags_cases_set_plan(self._h, plan)
+        if rc:
+            raise RuntimeError("tags_cases_set_plan failed: %d" % rc)
+
+    def set_lanes(self, lanes):
+        rc = lib().tags_cases_set_lanes(self._h, lanes)
+        if rc:
+            raise RuntimeError("tags_cases_set_lanes failed: %d" % rc)
+
+    def set_profiling(self, on=True):
+        lib().tags_cases_set_profiling(self._h, 1 if on else 0)
+
+    def timers(self):
+        t = Timers()
+        lib().tags_cases_get_timers(self._h, ctypes.byref(t))
+        return t.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().tags_cases_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        if sys.is_finalizing():
+            return
+        try:
+            self.close()
+        except Exception:
+            pass
+

@@ -6,7 +6,9 @@
 #include <complex>
 #include <cstring>
 #include <memory>
+#include <chrono>
 #include <mutex>
+#include <thread>
 #include <vector>
 
 #include "lu.h"
@@ -121,20 +123,17 @@ struct Solver {
     static constexpr int kSpecIters = 3;   // refinement steps queued before the first check
     static constexpr int kMaxIters = 20;   // refinement steps in all
     // Acceptance needs both a small backward error |b - W x| / (|W| |x|) and a
-    // small forward-error estimate |dx| / |x|, dx the last correction. The
-    // backward error alone does not bound the error of x for an ill-conditioned
-    // W (cond_1 ~ 6e8 on the 30 x 30 m grid at 31 kHz: a backward error of
-    // 1e-15 still allows ~1e-7). The last correction is the error the previous
-    // iterate had; while the refinement contracts, the accepted iterate's error
-    // is smaller than that.
+    // small estimate of the error left in x. The backward error alone does not
+    // bound the error of x for an ill-conditioned W (cond_1 ~ 6e8 on the
+    // 30 x 30 m grid at 31 kHz: a backward error of 1e-15 still allows ~1e-7).
+    // The correction dx_k of step k is about (1 - rho) times the error of
+    // x_{k-1}, rho the contraction per step, so the error of x_k is about
+    // rho / (1 - rho) |dx_k|; rho is taken as |dx_k| / |dx_{k-1}| when that is
+    // below 1/2, otherwise (or at the first step) the estimate is |dx_k| itself.
     static constexpr double kAccept = 1e-15;      // backward error, a few FP64 ulps
-    static constexpr double kFwdAccept = 1e-11;   // |dx| / |x|
-    // refinement stops at a backward error <= kConverged (about 2 FP64 ulps)
-    // with |dx| / |x| <= kFwdAccept, or at a backward error <= kAccept with
-    // |dx| / |x| <= kFwdStop, or when a step gains less than a factor kMinGain
-    // in both measures
-    static constexpr double kConverged = 2e-16;
-    static constexpr double kFwdStop = 1e-13;
+    static constexpr double kFwdAccept = 1e-10;   // estimated error / |x|, 100 x below 1e-8
+    // refinement stops at the first acceptable iterate, or when a step gains
+    // less than a factor kMinGain in both measures
     static constexpr double kMinGain = 0.8;
     // norms per refinement step k at norms[kNS k ..]: |r|, |x|, |W| (k = 0 only)
     // and |dx| (the correction of step k), as bit patterns of non-negative doubles;
@@ -402,107 +401,139 @@ struct Solver {
         for (int k = 1; k <= kSpecIters; k++)
             if ((rc = queue_step(W, k, gr, false))) return rc;
         queued = kSpecIters;
+        m_first = true;
         TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * kNormStride * nb,
                                 hipMemcpyDeviceToHost, st));
         return 0;
     }
 
-    // Second half: waits and checks each item's queued steps by the backward
-    // error |b - W x| / (|W| |x|) and the forward-error estimate |dx| / |x| of
-    // each iterate, refines the items for which the stop rule (see kConverged)
-    // does not hold further, one step at a time (up to kMaxIters), and accepts an
-    // item's last iterate at a backward error <= kAccept and |dx| / |x| <=
-    // kFwdAccept. An item that fails, or whose complex64 factor broke down,
-    // starts again alone with plan32_retry's complex64 factor if one is set,
-    // else the FP64 factor of its intact W gives its answer. Solutions returned
-    // in `bvec`; stt (nullable) and info (nullable) receive each item's. Returns
-    // 0, an error, or for one item its info.
-    int finish_mixed(c64* W, c64* bvec, tags_item_stats* stt, int* info = nullptr)
+    // Second half, in steps that never wait: advance_mixed is called when the
+    // stream has finished what was queued (the caller syncs or polls). It
+    // checks each item's steps so far by the backward error |b - W x| / (|W|
+    // |x|) and the forward-error estimate |dx| / |x|; while the stop rule (see
+    // kMinGain) does not hold for some items it queues one more step for
+    // them (masked to those items) and returns with *complete false; then it
+    // accepts an item's last iterate at a backward error <= kAccept and
+    // |dx| / |x| <= kFwdAccept. An item that fails, or whose complex64 factor
+    // broke down, starts again alone with plan32_retry's complex64 factor if
+    // one is set, else the FP64 factor of its intact W gives its answer (these
+    // wait). Solutions in `bvec`; stt (nullable) and info (nullable) receive
+    // each item's. Returns 0, an error, or for one item its info.
+    std::vector<int> m_info32, m_done, m_go;
+    std::vector<double> m_berr, m_fwd;
+    int m_step = 0;
+    bool m_first = true;
+    c64* m_W = nullptr;
+    c64* m_bvec = nullptr;
+    double berr_at(int z, int k) const
     {
-        TAGS_HIP(hipStreamSynchronize(st));
-        pending_mixed = false;
+        const double wnorm = norm_at(z, 0, 2);
+        const double rn = norm_at(z, k, 0), xn = norm_at(z, k, 1);
+        if (rn == 0.0) return 0.0;
+        return (wnorm > 0 && xn > 0) ? rn / (wnorm * xn) : 1.0;
+    }
+    // |dx_k| / |x_k|
+    double dx_at(int z, int k) const
+    {
+        const double dn = norm_at(z, k, 3), xn = norm_at(z, k, 1);
+        if (dn == 0.0) return 0.0;
+        return xn > 0 ? dn / xn : 1.0;
+    }
+    // estimated error of x_k relative to |x_k| (see kFwdAccept)
+    double fwd_at(int z, int k) const
+    {
+        const double d = dx_at(z, k);
+        if (k < 2 || d == 0.0) return d;
+        const double dp = dx_at(z, k - 1);
+        const double rho = dp > 0.0 ? d / dp : 1.0;
+        return rho < 0.5 ? d * rho / (1.0 - rho) : d;
+    }
+    bool keep_going(int z) const
+    {
+        const int d = m_done[z];
+        const double be = m_berr[z], fe = m_fwd[z];
+        if (m_info32[z] != 0 || !(be == be) || !(fe == fe) || d >= kMaxIters) return false;
+        if (be <= kAccept && fe <= kFwdAccept) return false;
+        return d < 2 || be <= kMinGain * berr_at(z, d - 1) || dx_at(z, d) <= kMinGain * dx_at(z, d - 1);
+    }
+    int advance_mixed(c64* W, c64* bvec, tags_item_stats* stt, int* info, bool* complete)
+    {
+        *complete = false;
         const int nbi = nb;
-        const long sWi = sW;
-        std::vector<int> info32(nbi), done(nbi, queued), go(nbi, 0);
-        std::vector<double> berr(nbi), fwd(nbi);
-        for (int z = 0; z < nbi; z++) {
-            if (hflags[4 * z + 1]) return TAGS_ERR_BARRIER;
-            info32[z] = hflags[4 * z];
-        }
         const Grounding* gr = ground.g >= 0 ? &ground : nullptr;
-        auto berr_at = [&](int z, int k) {
-            const double wnorm = norm_at(z, 0, 2);
-            const double rn = norm_at(z, k, 0), xn = norm_at(z, k, 1);
-            if (rn == 0.0) return 0.0;
-            return (wnorm > 0 && xn > 0) ? rn / (wnorm * xn) : 1.0;
-        };
-        auto fwd_at = [&](int z, int k) {
-            const double dn = norm_at(z, k, 3), xn = norm_at(z, k, 1);
-            if (dn == 0.0) return 0.0;
-            return xn > 0 ? dn / xn : 1.0;
-        };
-        auto keep_going = [&](int z) {
-            const int d = done[z];
-            const double be = berr[z], fe = fwd[z];
-            if (info32[z] != 0 || !(be == be) || !(fe == fe) || d >= kMaxIters) return false;
-            if (be <= kConverged && fe <= kFwdAccept) return false;
-            if (be <= kAccept && fe <= kFwdStop) return false;
-            return d < 2 || be <= kMinGain * berr_at(z, d - 1) || fe <= kMinGain * fwd_at(z, d - 1);
-        };
         int ngo = 0;
-        for (int z = 0; z < nbi; z++) {
-            if (stt) {
-                for (int i = 0; i < 8; i++) stt[z].refine_hist[i] = -1.0;
-                for (int k = 1; k <= queued && k <= 8; k++)
-                    stt[z].refine_hist[k - 1] = berr_at(z, k);
+        if (m_first) {
+            // the queued steps
+            m_first = false;
+            m_W = W;
+            m_bvec = bvec;
+            m_info32.assign(nbi, 0);
+            m_done.assign(nbi, queued);
+            m_go.assign(nbi, 0);
+            m_berr.assign(nbi, 0.0);
+            m_fwd.assign(nbi, 0.0);
+            m_step = queued;
+            for (int z = 0; z < nbi; z++) {
+                if (hflags[4 * z + 1]) return TAGS_ERR_BARRIER;
+                m_info32[z] = hflags[4 * z];
             }
-            berr[z] = berr_at(z, queued);
-            fwd[z] = fwd_at(z, queued);
-            go[z] = keep_going(z);
-            ngo += go[z];
+            for (int z = 0; z < nbi; z++) {
+                if (stt) {
+                    for (int i = 0; i < 8; i++) stt[z].refine_hist[i] = -1.0;
+                    for (int k = 1; k <= queued && k <= 8; k++)
+                        stt[z].refine_hist[k - 1] = berr_at(z, k);
+                }
+                m_berr[z] = berr_at(z, queued);
+                m_fwd[z] = fwd_at(z, queued);
+                m_go[z] = keep_going(z);
+                ngo += m_go[z];
+            }
+        } else {
+            // the step queued by the last call, for the items still refining
+            m_step++;
+            for (int z = 0; z < nbi; z++) {
+                if (!m_go[z]) continue;
+                m_done[z] = m_step;
+                m_berr[z] = berr_at(z, m_step);
+                m_fwd[z] = fwd_at(z, m_step);
+                if (stt && m_step <= 8) stt[z].refine_hist[m_step - 1] = m_berr[z];
+                m_go[z] = keep_going(z);
+                ngo += m_go[z];
+            }
         }
-        // further steps, batch-wide but masked to the items still refining (all of
-        // which have taken the same number of steps)
-        int step = queued;
-        while (ngo > 0) {
+        if (ngo > 0) {
+            // one more step, batch-wide but masked to the items still refining
+            // (all of which have taken the same number of steps)
             if (nbi > 1) {
-                for (int z = 0; z < nbi; z++) hactive[z] = go[z];
+                for (int z = 0; z < nbi; z++) hactive[z] = m_go[z];
                 TAGS_HIP(hipMemcpyAsync(active, hactive, sizeof(int) * nbi,
                                         hipMemcpyHostToDevice, st));
             }
-            int rc = queue_step(W, step + 1, gr, nbi > 1);
+            int rc = queue_step(W, m_step + 1, gr, nbi > 1);
             if (rc) return rc;
             for (int z = 0; z < nbi; z++)
-                if (go[z])
-                    TAGS_HIP(hipMemcpyAsync(hnorms + kNormStride * z + kNS * (step + 1),
-                                            norms + kNormStride * z + kNS * (step + 1),
+                if (m_go[z])
+                    TAGS_HIP(hipMemcpyAsync(hnorms + kNormStride * z + kNS * (m_step + 1),
+                                            norms + kNormStride * z + kNS * (m_step + 1),
                                             sizeof(unsigned long long) * kNS,
                                             hipMemcpyDeviceToHost, st));
-            TAGS_HIP(hipStreamSynchronize(st));
-            step++;
-            ngo = 0;
-            for (int z = 0; z < nbi; z++) {
-                if (!go[z]) continue;
-                done[z] = step;
-                berr[z] = berr_at(z, step);
-                fwd[z] = fwd_at(z, step);
-                if (stt && step <= 8) stt[z].refine_hist[step - 1] = berr[z];
-                go[z] = keep_going(z);
-                ngo += go[z];
-            }
+            return 0;
         }
+        pending_mixed = false;
+        const long sWi = sW;
+        const std::vector<int> done = m_done;
         TAGS_HIP(hipMemcpyAsync(bvec, x, sizeof(c64) * n * nbi, hipMemcpyDeviceToDevice, st));
         std::vector<int> failed;
         for (int z = 0; z < nbi; z++) {
-            const bool ok = info32[z] == 0 && berr[z] == berr[z] && berr[z] <= kAccept &&
-                            fwd[z] <= kFwdAccept;
+            const bool ok = m_info32[z] == 0 && m_berr[z] == m_berr[z] && m_berr[z] <= kAccept &&
+                            m_fwd[z] <= kFwdAccept;
             if (ok) {
                 if (stt) {
                     stt[z].precision_used = TAGS_PREC_MIXED;
                     stt[z].refine_iters = done[z];
                     stt[z].info = 0;
-                    stt[z].backward_err = berr[z];
-                    stt[z].forward_est = fwd[z];
+                    stt[z].backward_err = m_berr[z];
+                    stt[z].forward_est = m_fwd[z];
                 }
                 if (info) info[z] = 0;
             } else {
@@ -544,7 +575,19 @@ struct Solver {
         }
         nb = nbi;
         sW = sWi;
+        *complete = true;
         return nbi == 1 ? first : 0;
+    }
+    // The whole second half, waiting for each step.
+    int finish_mixed(c64* W, c64* bvec, tags_item_stats* stt, int* info = nullptr)
+    {
+        bool complete = false;
+        int rc = 0;
+        while (!complete) {
+            TAGS_HIP(hipStreamSynchronize(st));
+            if ((rc = advance_mixed(W, bvec, stt, info, &complete)) < 0) return rc;
+        }
+        return rc;
     }
     int solve_mixed(c64* W, c64* bvec, tags_item_stats* stt)
     {
@@ -1581,8 +1624,12 @@ struct SweepSlot {
     Profiler prof;
     Solver sv;
     int cap = 0;
+    const tags_sweep* bound = nullptr;  // the sweep whose plans the solver holds
+    int bound_kind = -1;                // and its plan kind then
     long f = -1;            // first sample of the group in flight (-1: idle)
     int cnt = 0;            // its samples f .. f + cnt - 1
+    std::vector<tags_item_stats> stt;  // their statistics and infos, filled as they finish
+    std::vector<int> info;
     std::vector<double> hz; // their frequencies |s| / 2 pi
     bool nodal = false;     // their complex64 factor is the nodal plan's
     size_t w_cap = 0, rhs_cap = 0;  // allocated elements of W and rhs
@@ -1654,7 +1701,25 @@ struct tags_sweep {
     // FP64 form: after each solve, assemble W again (the factor replaced it)
     // and report the backward error of x in the item statistics
     bool verify = false;
+    // the plans a slot's solver takes for this sweep (set by sweep_layout)
+    const tags::LUPlan* bind64 = nullptr;
+    const tags::LUPlan* bind32 = nullptr;
+    tags::Grounding bindgr{};
 };
+
+// Points a slot's solver at the sweep's plans (a slot of a multi-case solve
+// serves many sweeps in turn); grounding sums are kept only while the slot
+// stays with the same sweep and plan kind.
+static void slot_bind(tags_sweep* sw, SweepSlot& sl)
+{
+    if (sl.bound != sw || sl.bound_kind != sw->layout_kind) sl.sv.ground_valid = false;
+    sl.bound = sw;
+    sl.bound_kind = sw->layout_kind;
+    sl.sv.plan = sw->bind64;
+    sl.sv.plan32 = sw->bind32;
+    sl.sv.plan32_retry = nullptr;
+    sl.sv.ground = sw->bindgr;
+}
 
 // The first k slots exist and are allocated for order N (precision `mixed`).
 // Samples per group when none is set: one from order kBatchOneAbove up (a
@@ -1741,11 +1806,12 @@ static int sweep_layout(tags_sweep* sw)
         sw->plan = LUPlan::zl_first((int)m, (int)n, false);
         break;
     }
+    sw->bind64 = p64;
+    sw->bind32 = p32;
+    sw->bindgr = gr;
     for (auto& sl : sw->slots) {
         if (changed) sl->sv.ground_valid = false;
-        sl->sv.plan = p64;
-        sl->sv.plan32 = p32;
-        sl->sv.ground = gr;
+        slot_bind(sw, *sl);
     }
     return 0;
 }
@@ -2093,6 +2159,10 @@ static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const SweepCall&
     }
     sl.f = (long)f;
     sl.cnt = cnt;
+    sl.stt.assign(cnt, tags_item_stats{});
+    for (auto& q : sl.stt)
+        for (int i = 0; i < 8; i++) q.refine_hist[i] = -1.0;
+    sl.info.assign(cnt, 0);
     if (sw->plan_kind == TAGS_PLAN_HYBRID) {
         sl.sv.plan32 = sl.nodal ? &sw->plan_nodal : nullptr;
         sl.sv.plan32_retry = sl.nodal ? &sw->plan : nullptr;
@@ -2147,22 +2217,25 @@ static int sweep_verify64(tags_sweep* sw, SweepSlot& sl, c64* Wz, const c64* xz,
     return 0;
 }
 
-// Finishes the group in flight on slot `sl` and writes its results. A
-// singular sample (info > 0) gets NaN in u_inj and in its block of x_out.
-static int sweep_finish(tags_sweep* sw, SweepSlot& sl, const SweepCall& c)
+// Advances the group in flight on slot `sl`, whose stream has finished what
+// was queued: the mixed form checks its refinement and may queue another step
+// (*complete false); once complete the results are written. A singular sample
+// (info > 0) gets NaN in u_inj and in its block of x_out.
+static int sweep_advance(tags_sweep* sw, SweepSlot& sl, const SweepCall& c, bool* complete)
 {
     const long N = sw->N, n = sw->n, m = sw->m;
     const BlockLayout& L = sw->layout;
     const size_t f0 = (size_t)sl.f;
     const int cnt = sl.cnt;
-    sl.f = -1;
-    std::vector<tags_item_stats> stt(cnt);
-    for (auto& q : stt)
-        for (int i = 0; i < 8; i++) q.refine_hist[i] = -1.0;
-    std::vector<int> info(cnt, 0);
-    int rc = c.mixed ? sl.sv.finish_mixed(sl.W.p, sl.rhs.p, stt.data(), info.data())
+    std::vector<tags_item_stats>& stt = sl.stt;
+    std::vector<int>& info = sl.info;
+    *complete = false;
+    int rc = c.mixed ? sl.sv.advance_mixed(sl.W.p, sl.rhs.p, stt.data(), info.data(), complete)
                      : sl.sv.finish64(stt.data(), info.data());
     if (rc < 0) return rc;
+    if (!c.mixed) *complete = true;
+    if (!*complete) return 0;
+    sl.f = -1;
     hipStream_t st = sl.stream.s;
     if (!c.mixed && sw->verify)
         for (int z = 0; z < cnt; z++)
@@ -2228,8 +2301,7 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     const double nodal_above0 = sw->nodal_above;
     const int k = sw->inflight;
     size_t next = 0;
-    int used = 0;
-    for (int i = 0; i < k && next < nfreq; i++, used++) {
+    for (int i = 0; i < k && next < nfreq; i++) {
         const int got = sweep_issue(sw, *sw->slots[i], next, c);
         if (got < 0) {
             rc = got;
@@ -2237,12 +2309,28 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
         }
         next += (size_t)got;
     }
-    // finish the slots in turn (groups in issue order), refilling each one
-    for (int i = 0; !rc && used > 0; i = (i + 1) % used) {
-        SweepSlot& sl = *sw->slots[i];
-        if (sl.f < 0) break;  // the oldest slot is idle: every sample is done
-        if ((rc = sweep_finish(sw, sl, c))) break;
-        if (next < nfreq) {
+    // advance whichever slot's stream is idle; refill a slot whose group is done
+    while (!rc) {
+        int busy = 0, ready = -1;
+        for (int i = 0; i < k; i++) {
+            if (sw->slots[i]->f < 0) continue;
+            busy++;
+            const hipError_t q = hipStreamQuery(sw->slots[i]->stream.s);
+            if (q == hipSuccess) {
+                ready = i;
+                break;
+            }
+            if (q != hipErrorNotReady) TAGS_HIP(q);
+        }
+        if (!busy) break;
+        if (ready < 0) {
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            continue;
+        }
+        SweepSlot& sl = *sw->slots[ready];
+        bool complete = false;
+        if ((rc = sweep_advance(sw, sl, c, &complete))) break;
+        if (complete && next < nfreq) {
             const int got = sweep_issue(sw, sl, next, c);
             if (got < 0) {
                 rc = got;
@@ -2276,3 +2364,194 @@ int tags_sweep_set_batch(tags_sweep* sw, int samples)
 
 }  // extern "C"
 
+
+// ---------------------------------------------------------------------------
+// Many grounding cases: every (case, frequency) sample of every case, in
+// groups of one case's consecutive samples, scheduled over a pool of lanes.
+// The reference runs one case per driver run, its frequencies in an OpenMP
+// loop with per-thread buffers (examples/harmonic_impedance.c:87-176,
+// examples/grcev12pwrd01.c:88-97); here the groups of all cases form one
+// queue, largest first (longest-processing-time order by the per-sample cost
+// (8/3) N^3 + 6400 m^2: the factorisation and the pair quadrature), and each
+// lane (own stream, buffers and solver) takes the next group as soon as its
+// last one is finished.
+// ---------------------------------------------------------------------------
+struct tags_cases {
+    std::vector<tags_sweep*> sw;
+    std::vector<tags_case> cs;
+    std::vector<std::unique_ptr<SweepSlot>> lanes;
+    int nlanes = 3;
+    int plan_kind = TAGS_PLAN_HYBRID;
+    bool profiling = false;
+    ~tags_cases()
+    {
+        lanes.clear();
+        for (tags_sweep* p : sw) tags_sweep_destroy(p);
+    }
+};
+
+extern "C" {
+
+int tags_cases_create(tags_cases** out, const tags_case* cases, size_t ncases)
+{
+    if (!out || (ncases && !cases)) return TAGS_ERR_ARG;
+    std::unique_ptr<tags_cases> cs(new tags_cases());
+    for (size_t c = 0; c < ncases; c++) {
+        const tags_case& d = cases[c];
+        if (d.inj_node >= d.num_nodes || (d.nfreq && (!d.s || !d.kappa || !d.u_inj)))
+            return TAGS_ERR_ARG;
+        tags_sweep* p = nullptr;
+        int rc = tags_sweep_create(&p, d.electrodes, d.images, d.num_electrodes, d.nodes,
+                                   d.num_nodes);
+        if (rc) return rc;
+        cs->sw.push_back(p);
+        cs->cs.push_back(d);
+    }
+    *out = cs.release();
+    return 0;
+}
+
+int tags_cases_set_plan(tags_cases* cs, int plan)
+{
+    if (!cs || plan < 0 || plan > TAGS_PLAN_HYBRID) return TAGS_ERR_ARG;
+    cs->plan_kind = plan;
+    return 0;
+}
+
+int tags_cases_set_lanes(tags_cases* cs, int lanes)
+{
+    if (!cs || lanes < 1 || lanes > 4) return TAGS_ERR_ARG;
+    cs->nlanes = lanes;
+    return 0;
+}
+
+int tags_cases_set_profiling(tags_cases* cs, int on)
+{
+    if (!cs) return TAGS_ERR_ARG;
+    cs->profiling = on != 0;
+    for (auto& l : cs->lanes) {
+        TAGS_HIP(hipStreamSynchronize(l->stream.s));
+        l->prof.on = cs->profiling;
+        l->prof.reset();
+    }
+    return 0;
+}
+
+int tags_cases_get_timers(tags_cases* cs, tags_timers* t)
+{
+    if (!cs || !t) return TAGS_ERR_ARG;
+    std::memset(t, 0, sizeof(*t));
+    for (auto& l : cs->lanes) {
+        TAGS_HIP(hipStreamSynchronize(l->stream.s));
+        Profiler& p = l->prof;
+        p.resolve();
+        t->ms_fill += p.ms[FAM_FILL];
+        t->ms_panel += p.ms[FAM_PANEL];
+        t->ms_gemm += p.ms[FAM_GEMM];
+        t->ms_other += p.ms[FAM_OTHER];
+        t->ms_solve += p.ms[FAM_SOLVE];
+        t->ms_residual += p.ms[FAM_RESID];
+        t->gemm_flops += p.gemm_flops;
+        t->trailing_ms += p.trailing_ms;
+        t->trailing_flops += p.trailing_flops;
+        t->trailing_launches += p.trailing_launches;
+    }
+    return 0;
+}
+
+int tags_cases_solve(tags_cases* cs, int precision)
+{
+    if (!cs) return TAGS_ERR_ARG;
+    const bool mixed = precision == TAGS_PREC_MIXED;
+    const size_t nc = cs->cs.size();
+    int rc;
+    // per case: plans, layout and group size
+    std::vector<SweepCall> call(nc);
+    std::vector<double> cost(nc), nodal0(nc);
+    std::vector<size_t> next(nc, 0);
+    for (size_t c = 0; c < nc; c++) {
+        tags_sweep* sw = cs->sw[c];
+        const tags_case& d = cs->cs[c];
+        sw->plan_kind = cs->plan_kind;
+        sweep_layout(sw);
+        sw->batch = (sw->plan_kind == TAGS_PLAN_GROUNDED ||
+                     sw->plan_kind == TAGS_PLAN_GROUNDED_PIVOTED) && mixed
+                        ? 1
+                        : sweep_batch(sw, mixed);
+        if (lu_panel_kind() == PANEL_PARTIAL) sw->batch = 1;
+        if ((rc = sweep_potentials(sw))) return rc;
+        call[c] = SweepCall{d.nfreq, d.s, d.kappa, nullptr, nullptr, nullptr, d.mur, d.inj_node,
+                            mixed, d.u_inj, nullptr, d.stats};
+        const double N = (double)sw->N, m = (double)sw->m;
+        cost[c] = 8.0 / 3.0 * N * N * N + 6400.0 * m * m;
+        nodal0[c] = sw->nodal_above;
+    }
+    // the queue: cases by per-sample cost, largest first; a case's groups are
+    // taken in frequency order
+    std::vector<size_t> order(nc);
+    for (size_t c = 0; c < nc; c++) order[c] = c;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](size_t a, size_t b) { return cost[a] > cost[b]; });
+    size_t qi = 0;  // first case in `order` with samples left
+    while ((int)cs->lanes.size() < cs->nlanes) cs->lanes.emplace_back(new SweepSlot());
+    std::vector<long> lane_case(cs->nlanes, -1);
+    auto issue = [&](int li) -> int {
+        while (qi < nc && next[order[qi]] >= cs->cs[order[qi]].nfreq) qi++;
+        if (qi >= nc) return 0;
+        const size_t c = order[qi];
+        tags_sweep* sw = cs->sw[c];
+        SweepSlot& L = *cs->lanes[li];
+        int r = L.init(sw->N, sw->batch);
+        if (r || (r = L.sv.ensure((int)sw->N, mixed, sw->batch))) return r;
+        L.prof.on = cs->profiling;
+        slot_bind(sw, L);
+        const int got = sweep_issue(sw, L, next[c], call[c]);
+        if (got < 0) return got;
+        next[c] += (size_t)got;
+        lane_case[li] = (long)c;
+        return 0;
+    };
+    rc = 0;
+    for (int li = 0; li < cs->nlanes && !rc; li++) rc = issue(li);
+    // finish whichever lane is done first and refill it
+    while (!rc) {
+        int busy = 0, done = -1;
+        for (int li = 0; li < cs->nlanes; li++) {
+            if (cs->lanes[li]->f < 0) continue;
+            busy++;
+            const hipError_t q = hipStreamQuery(cs->lanes[li]->stream.s);
+            if (q == hipSuccess) {
+                done = li;
+                break;
+            }
+            if (q != hipErrorNotReady) {
+                TAGS_HIP(q);
+            }
+        }
+        if (!busy) break;
+        if (done < 0) {
+            std::this_thread::sleep_for(std::chrono::microseconds(20));
+            continue;
+        }
+        SweepSlot& L = *cs->lanes[done];
+        const size_t c = (size_t)lane_case[done];
+        bool complete = false;
+        if ((rc = sweep_advance(cs->sw[c], L, call[c], &complete))) break;
+        if (!complete) continue;
+        lane_case[done] = -1;
+        rc = issue(done);
+    }
+    for (size_t c = 0; c < nc; c++) cs->sw[c]->nodal_above = nodal0[c];
+    for (auto& l : cs->lanes) {
+        TAGS_HIP(hipStreamSynchronize(l->stream.s));
+        l->f = -1;
+    }
+    return rc;
+}
+
+void tags_cases_destroy(tags_cases* cs)
+{
+    delete cs;
+}
+
+}  // extern "C"
