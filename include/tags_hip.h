@@ -540,6 +540,22 @@ int tags_cases_set_profiling(tags_cases *cs, int on);
 int tags_cases_get_timers(tags_cases *cs, tags_timers *t);
 void tags_cases_destroy(tags_cases *cs);
 
+/* calculate_impedances (src/electrode.h:276-281) for callers that cannot
+ * allocate device memory (work captured in HIP graphs, tight serving loops); no
+ * counterpart in the reference. Same arguments and results (the lower half of
+ * zl and zt, aliasing rule included), but zl, zt and electrodes are DEVICE
+ * pointers and the work is queued on `stream` (a hipStream_t; NULL: the null
+ * stream) without waiting. The library allocates nothing: the segment
+ * geometry is staged in a fixed device array of the library (one per device,
+ * 4 MiB, set up at the first call on that device), so num_electrodes is at
+ * most 65536 (TAGS_ERR_TOO_LARGE above, before any device work). Calls that
+ * overlap on one device race on that array: queue them on one stream. */
+int calculate_impedances_noalloc(TAGS_COMPLEX *zl, TAGS_COMPLEX *zt, const Electrode *electrodes,
+                                 size_t num_electrodes, TAGS_COMPLEX gamma, TAGS_COMPLEX s,
+                                 double mur, TAGS_COMPLEX kappa, size_t max_eval,
+                                 double req_abs_error, double req_rel_error,
+                                 int integration_type, void *stream);
+
 /* Dense solve of a device-resident or host complex matrix, for tests and
  * library use. `a` is row-major n x n (for the symmetric W the same as column-
  * major), `b` is n; on return `b` holds x. precision: tags_precision.

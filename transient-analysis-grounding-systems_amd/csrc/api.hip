@@ -2555,3 +2555,99 @@ void tags_cases_destroy(tags_cases* cs)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// calculate_impedances without allocation, for callers that cannot allocate
+// device memory (work captured in HIP graphs, tight serving loops): device
+// pointers, the caller's stream, and the segment geometry staged as structure
+// of arrays in a fixed device array instead of a per-call buffer.
+// ---------------------------------------------------------------------------
+namespace tags {
+// the staging array: 8 doubles per segment (start xyz, end xyz, length,
+// radius), for at most kNoAllocMaxSegments segments (4 MiB per device)
+constexpr long kNoAllocMaxSegments = 65536;
+static __device__ __align__(256) double scratch[8 * kNoAllocMaxSegments];
+
+// device address of `scratch` per device, taken once with that device current
+constexpr int kNoAllocMaxDevices = 64;
+static double* g_scratch_ptr[kNoAllocMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_ptr(double** out)
+{
+    int dev = 0;
+    TAGS_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kNoAllocMaxDevices) return TAGS_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        TAGS_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return TAGS_ERR_HIP;
+        // set-up of the first call on this device: the quadrature tables too
+        int rc = gl_tables_init();
+        if (rc) return rc;
+        g_scratch_ptr[dev] = (double*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
+// soa[k m + i] = field k of segment i (start xyz, end xyz, length, radius)
+__global__ void k_segments_soa(const Electrode* __restrict__ e, long m, double* soa)
+{
+    const long i = (long)blockIdx.x * 256 + threadIdx.x;
+    if (i >= m) return;
+    const Electrode s = e[i];
+    soa[0 * m + i] = s.start_point[0];
+    soa[1 * m + i] = s.start_point[1];
+    soa[2 * m + i] = s.start_point[2];
+    soa[3 * m + i] = s.end_point[0];
+    soa[4 * m + i] = s.end_point[1];
+    soa[5 * m + i] = s.end_point[2];
+    soa[6 * m + i] = s.length;
+    soa[7 * m + i] = s.radius;
+}
+}  // namespace tags
+
+extern "C" {
+
+int calculate_impedances_noalloc(TAGS_COMPLEX* zl, TAGS_COMPLEX* zt, const Electrode* electrodes,
+                                 size_t num_electrodes, TAGS_COMPLEX gamma, TAGS_COMPLEX s,
+                                 double mur, TAGS_COMPLEX kappa, size_t max_eval,
+                                 double req_abs_error, double req_rel_error,
+                                 int integration_type, void* stream)
+{
+    (void)max_eval;
+    (void)req_abs_error;
+    (void)req_rel_error;
+    if (integration_type != INTG_DOUBLE && integration_type != INTG_NONE &&
+        integration_type != INTG_MHEM && integration_type != INTG_SINGLE) {
+        std::printf("integration failed with code: %i\n", -10);
+        return -10;
+    }
+    const long m = (long)num_electrodes;
+    if (m == 0) return 0;
+    if (!zl || !zt || !electrodes) return TAGS_ERR_ARG;
+    if (m > kNoAllocMaxSegments) {
+        std::fprintf(stderr,
+                     "calculate_impedances_noalloc: %ld segments exceed the staging array's "
+                     "maximum of %ld\n",
+                     m, kNoAllocMaxSegments);
+        return TAGS_ERR_TOO_LARGE;
+    }
+    double* soa = nullptr;
+    int rc = scratch_ptr(&soa);
+    if (rc) return rc;
+    hipStream_t st = (hipStream_t)stream;
+    hipLaunchKernelGGL(k_segments_soa, dim3((unsigned)((m + 255) / 256)), dim3(256), 0, st,
+                       electrodes, m, soa);
+    TAGS_HIP(hipGetLastError());
+    const SegSoA g{soa, soa + m, soa + 2 * m, soa + 3 * m, soa + 4 * m, soa + 5 * m, soa + 6 * m,
+                   soa + 7 * m};
+    const PairScalars sc = make_scalars(from_c(gamma), from_c(s), mur, from_c(kappa), c64{1, 0},
+                                        c64{1, 0}, integration_type);
+    return launch_pairs_dropin((double*)zl, (double*)zt, (void*)zl == (void*)zt, g, g, m, sc,
+                               false, st);
+}
+
+}  // extern "C"
