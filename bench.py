@@ -467,9 +467,15 @@ def main():
                     help="LU plan of the mixed form: 0 dense, 1 structured, 2 Z_L first, "
                          "3 Z_L-block pivoting, 4 grounded without row exchanges, 5 "
                          "grounded with pivoting, 6 nodal, 7 hybrid (default: nodal "
-                         "complex64 factor from 20 kHz, Z_L first below)")
-    ap.add_argument("--nodal-above", type=float, default=2e4,
-                    help="switch frequency of the hybrid plan (Hz, default 2e4)")
+                         "complex64 factor from --nodal-above, Z_L first below)")
+    ap.add_argument("--nodal-above", type=float, default=0.0,
+                    help="switch frequency of the hybrid plan (Hz, default 0: the nodal "
+                         "complex64 factor at every frequency)")
+    ap.add_argument("--gmres-below", type=float, default=2e4,
+                    help="nodal factors below this frequency refine by FGMRES (Hz, "
+                         "default 2e4)")
+    ap.add_argument("--gmres-steps", type=int, default=6,
+                    help="Arnoldi steps per FGMRES refinement step (default 6)")
     ap.add_argument("--plan-fp64", type=int, default=2,
                     help="LU plan of the FP64 form (default 2, Z_L first)")
     ap.add_argument("--update-chunks", type=int, default=None,
@@ -541,6 +547,7 @@ def main():
         mixed = prec == T.PREC_MIXED
         sw.set_plan(args.plan_mixed if mixed else args.plan_fp64)
         sw.set_nodal_above(args.nodal_above)
+        sw.set_gmres(args.gmres_below, args.gmres_steps)
         T.set_step_width(args.step_width_mixed if mixed else args.step_width_fp64)
         sw.set_inflight(inflight)
         if W:
@@ -585,6 +592,8 @@ def main():
                    "plan": PLAN_NAMES[args.plan_mixed],
                    "step_width": args.step_width_mixed,
                    "nodal_above_hz": args.nodal_above,
+                   "gmres_below_hz": args.gmres_below,
+                   "gmres_steps": args.gmres_steps,
                    "gemm_variants": tuned},
         "mixed": {"refine_iters": [q["refine_iters"] for q in st_mx],
                   "precision_used": [q["precision_used"] for q in st_mx],

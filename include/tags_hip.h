@@ -430,7 +430,19 @@ int tags_inv_laplace_batch(double *f, const TAGS_COMPLEX *g, const TAGS_COMPLEX 
 enum tags_formulation { TAGS_FORM_HEM = 0, TAGS_FORM_MHEM = 1 };
 int tags_sweep_set_formulation(tags_sweep *sw, int formulation);
 
-/* Switch frequency of TAGS_PLAN_HYBRID (default 2e4 Hz): in the mixed form a
+/* Refinement of nodal complex64 factors (TAGS_PLAN_HYBRID and TAGS_PLAN_NODAL,
+ * mixed form): a sample at |s| / 2 pi < below_hz (default 2e4 Hz) refines by
+ * FGMRES in FP64 right-preconditioned by the complex64 factor, `steps`
+ * (1 .. 32, default 6) Arnoldi steps per refinement step (GMRES-based
+ * refinement), where the classic correction from the factor alone converges
+ * slowly or not at all (the node block's Schur complement is dominated by
+ * A^T Z_L^-1 A, whose complex64 rounding swamps the grounding terms at low
+ * frequency); a higher one refines classically. No counterpart in the
+ * reference. Returns 0, or TAGS_ERR_ARG. */
+int tags_sweep_set_gmres(tags_sweep *sw, double below_hz, int steps);
+
+/* Switch frequency of TAGS_PLAN_HYBRID (default 0 Hz: the nodal factor at
+ * every frequency, refined as tags_sweep_set_gmres sets): in the mixed form a
  * sample at |s| / 2 pi >= hz takes the nodal plan's complex64 factor, a lower
  * one ZL_FIRST's. A nodal sample that falls back to FP64 or needs more than 10
  * refinement steps raises the switch to 1.5 times its frequency for the rest

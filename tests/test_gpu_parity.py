@@ -355,9 +355,10 @@ def test_c3_mixed_vs_fp64_property():
 
 def test_c3_hybrid_vs_zl_first():
     """Full 60 x 60 m grid (N = 21,421) with the plans the bench runs: the
-    hybrid plan's mixed form (Z_L-first complex64 factor below the 20 kHz
-    switch, the nodal one from it up) against the Z_L-first FP64 factor, at the
-    band edges and on both sides of the switch, each within 1e-8; the mixed
+    hybrid plan's mixed form (the nodal complex64 factor, refined by FGMRES
+    below 20 kHz and classically from it up) against the Z_L-first FP64
+    factor, at the band edges and on both sides of 20 kHz, each within 1e-8;
+    the mixed
     samples meet the acceptance rule (backward error <= 1e-15, forward-error
     estimate <= 1e-10) and the FP64 ones report a backward error below 1e-13
     (measured 1e-19 .. 2.7e-14; the normwise bound of a stable LU grows like
@@ -488,10 +489,11 @@ def test_nodal_vs_lapack(freq):
 
 
 def test_hybrid_low_switch_retries():
-    """Hybrid plan with its switch lowered to 1 kHz on the 20 x 20 m grid: the
-    nodal complex64 samples whose refinement stalls start again with the
-    Z_L-first complex64 factor (W is intact) and raise the switch; every sample
-    stays within 1e-8 of the FP64 answer and none needs the FP64 factor."""
+    """Hybrid plan with its switch at 1 kHz and classic refinement at every
+    frequency (no FGMRES) on the 20 x 20 m grid: the nodal complex64 samples
+    whose refinement stalls start again with the Z_L-first complex64 factor (W
+    is intact) and raise the switch; every sample stays within 1e-8 of the FP64
+    answer and none needs the FP64 factor."""
     rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
     f = O.logspace(2, 7, 400)[::10]
     s = 2j * np.pi * f
@@ -501,6 +503,7 @@ def test_hybrid_low_switch_retries():
     u64, _ = sw.solve(s, kap, 0, T.PREC_FP64)
     sw.set_plan(T.PLAN_HYBRID)
     sw.set_nodal_above(1e3)
+    sw.set_gmres(0.0, 6)
     umx, st = sw.solve(s, kap, 0, T.PREC_MIXED)
     sw.close()
     assert np.max(np.abs(umx - u64) / np.abs(u64)) <= 1e-8

@@ -27,7 +27,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB_PATH = os.path.join(HERE, "libtags_hip.so")
 CSRC = os.path.join(HERE, "csrc")
-SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "api.hip", "nlt.hip", "io.hip", "field.hip"]
+SOURCES = ["fill.hip", "gemm.hip", "lu.hip", "tslu.hip", "gmres.hip", "api.hip", "nlt.hip",
+           "io.hip", "field.hip"]
 
 ELECTRODE_DTYPE = np.dtype(
     [("start_point", "<f8", 3), ("end_point", "<f8", 3), ("middle_point", "<f8", 3),
@@ -156,7 +157,8 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_fields_batch", "tags_mhem_gpd", "tags_sweep_set_verify",
             "tags_sweep_assemble_entries", "tags_sweep_set_batch", "tags_cases_create",
             "tags_cases_solve", "tags_cases_set_plan", "tags_cases_set_lanes",
-            "tags_cases_set_profiling", "tags_cases_get_timers", "tags_cases_destroy"]
+            "tags_cases_set_profiling", "tags_cases_get_timers", "tags_cases_destroy",
+            "tags_sweep_set_gmres"]
 
 _lib = None
 
@@ -230,6 +232,7 @@ def lib():
         "tags_sweep_set_inflight": ([P, I], I),
         "tags_sweep_set_verify": ([P, I], I),
         "tags_sweep_set_batch": ([P, I], I),
+        "tags_sweep_set_gmres": ([P, D, I], I),
         "tags_cases_create": ([P, P, S], I),
         "tags_cases_solve": ([P, I], I),
         "tags_cases_set_plan": ([P, I], I),
@@ -670,6 +673,13 @@ class Sweep:
         rc = lib().tags_sweep_set_verify(self._h, 1 if on else 0)
         if rc:
             raise RuntimeError("tags_sweep_set_verify failed: %d" % rc)
+
+    def set_gmres(self, below_hz=2e4, steps=6):
+        """Nodal complex64 factors below below_hz refine by FGMRES (steps Arnoldi
+        steps per refinement step)."""
+        rc = lib().tags_sweep_set_gmres(self._h, float(below_hz), int(steps))
+        if rc:
+            raise RuntimeError("tags_sweep_set_gmres failed: %d" % rc)
 
     def set_batch(self, samples):
         """Frequency samples per group (0: chosen from N and memory)."""

@@ -122,6 +122,11 @@ static PairScalars make_scalars(c64 gamma, c64 s, double mur, c64 kappa, c64 ref
 struct Solver {
     static constexpr int kSpecIters = 3;   // refinement steps queued before the first check
     static constexpr int kMaxIters = 20;   // refinement steps in all
+    // GMRES-based refinement (gmres_k > 0): each step is gmres_k Arnoldi steps
+    // preconditioned by the complex64 factor; kGmresSpec steps are queued
+    // before the first check, kGmresMaxIters in all
+    static constexpr int kGmresSpec = 2;
+    static constexpr int kGmresMaxIters = 6;
     // Acceptance needs both a small backward error |b - W x| / (|W| |x|) and a
     // small estimate of the error left in x. The backward error alone does not
     // bound the error of x for an ill-conditioned W (cond_1 ~ 6e8 on the
@@ -172,6 +177,10 @@ struct Solver {
     int queued = 0;  // refinement steps queued by issue_mixed
     int nb = 1;      // its items
     long sW = 0;     // and their matrix stride
+    // Arnoldi steps per refinement step (0: classic refinement, one correction
+    // from the complex64 factor per step); set before issue_mixed
+    int gmres_k = 0;
+    GmresWork gm;
 
     // buffers for `batch` items of order n_ (the FP64 work space for all of them
     // in the plain form, for one in the mixed form, where it is the fallback's);
@@ -339,6 +348,14 @@ struct Solver {
     {
         int rc;
         const VecBatch v = vb(masked);
+        if (gmres_k > 0) {
+            // the correction from FGMRES on W d = r, then r = b - W x
+            if ((rc = gmres_step(gm, gmres_k, nb, w32, W32, t32, dscale, W, sW, r, x,
+                                 norms + kNS * k + 3, kNormStride, masked ? active : nullptr, st,
+                                 pr)))
+                return rc;
+            return residual64(W, n, n, x, b, r, norms + kNS * k, false, st, pr, false, v);
+        }
         const unsigned long long* rn = norms + kNS * (k - 1);
         if ((rc = prep_rhs32(r, dscale, w32.perm, n, 1.0, t32, st, gr,
                              gr ? gsum + 1 : nullptr, rn, v)))
@@ -362,6 +379,8 @@ struct Solver {
         sW = nb_ > 1 ? sW_ : 0;
         const bool grounded = ground.g >= 0;
         if (grounded && nb > 1) return TAGS_ERR_ARG;
+        if (grounded) gmres_k = 0;
+        if (gmres_k > 0 && (rc = gm.alloc(n, gmres_k, cap))) return rc;
         {
             Timed tm(pr, FAM_OTHER, st);
             if (grounded) {
@@ -398,9 +417,10 @@ struct Solver {
         // r_0 = b (x = 0), with |W|
         if ((rc = residual64(W, n, n, x, b, r, norms, true, st, pr, false, vb(false)))) return rc;
         const Grounding* gr = grounded ? &ground : nullptr;
-        for (int k = 1; k <= kSpecIters; k++)
+        const int spec = gmres_k > 0 ? kGmresSpec : kSpecIters;
+        for (int k = 1; k <= spec; k++)
             if ((rc = queue_step(W, k, gr, false))) return rc;
-        queued = kSpecIters;
+        queued = spec;
         m_first = true;
         TAGS_HIP(hipMemcpyAsync(hnorms, norms, sizeof(unsigned long long) * kNormStride * nb,
                                 hipMemcpyDeviceToHost, st));
@@ -452,7 +472,9 @@ struct Solver {
     {
         const int d = m_done[z];
         const double be = m_berr[z], fe = m_fwd[z];
-        if (m_info32[z] != 0 || !(be == be) || !(fe == fe) || d >= kMaxIters) return false;
+        if (m_info32[z] != 0 || !(be == be) || !(fe == fe) ||
+            d >= (gmres_k > 0 ? kGmresMaxIters : kMaxIters))
+            return false;
         if (be <= kAccept && fe <= kFwdAccept) return false;
         return d < 2 || be <= kMinGain * berr_at(z, d - 1) || dx_at(z, d) <= kMinGain * dx_at(z, d - 1);
     }
@@ -554,13 +576,17 @@ struct Solver {
             TAGS_HIP(hipMemcpyAsync(bz, b + (long)z * n, sizeof(c64) * n, hipMemcpyDeviceToDevice,
                                     st));
             if (retry) {
-                // the second complex64 plan; its steps add to this one's
+                // the second complex64 plan, with classic refinement; its steps
+                // add to this one's
+                const int gk = gmres_k;
                 plan32 = retry;
                 plan32_retry = nullptr;
+                gmres_k = 0;
                 rc = issue_mixed(Wz, bz);
                 if (!rc) rc = finish_mixed(Wz, bz, sz);
                 plan32 = p32;
                 plan32_retry = retry;
+                gmres_k = gk;
                 if (rc >= 0) sz->refine_iters += done[z];
             } else {
                 // FP64 fallback on the intact W
@@ -1686,7 +1712,11 @@ struct tags_sweep {
     // back to FP64 or needs more than kHybridMaxIters refinement steps raises
     // the switch to 1.5 times its frequency for the rest of the sweep
     tags::LUPlan plan_nodal;
-    double nodal_above = 2e4;
+    double nodal_above = 0.0;
+    // nodal complex64 factors below gmres_below Hz refine by FGMRES
+    // (gmres_steps Arnoldi steps per refinement step), above it classically
+    double gmres_below = 2e4;
+    int gmres_steps = 6;
     tags::BlockLayout layout{};
     hipStream_t st = nullptr;  // set-up and assembly stream
     tags::DevSegs e, img;
@@ -1719,6 +1749,7 @@ static void slot_bind(tags_sweep* sw, SweepSlot& sl)
     sl.sv.plan32 = sw->bind32;
     sl.sv.plan32_retry = nullptr;
     sl.sv.ground = sw->bindgr;
+    sl.sv.gmres_k = 0;
 }
 
 // The first k slots exist and are allocated for order N (precision `mixed`).
@@ -2149,14 +2180,25 @@ static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const SweepCall&
         return std::hypot(sf.x, sf.y) / (2.0 * M_PI);
     };
     const bool hybrid = sw->plan_kind == TAGS_PLAN_HYBRID && c.mixed;
-    sl.nodal = hybrid && hz_of(f) >= sw->nodal_above;
+    const bool nodal_plan = sw->plan_kind == TAGS_PLAN_NODAL && c.mixed;
+    // the sample's class: 0 the plan's own complex64 factor with classic
+    // refinement (the hybrid plan's Z_L-first one below its switch), 1 a nodal
+    // factor refined by FGMRES, 2 a nodal factor refined classically
+    auto cls = [&](size_t q) {
+        if (!hybrid && !nodal_plan) return 0;
+        const double h = hz_of(q);
+        if (hybrid && h < sw->nodal_above) return 0;
+        return h < sw->gmres_below ? 1 : 2;
+    };
+    const int c0 = cls(f);
+    sl.nodal = hybrid && c0 > 0;
     int cnt = 0;
     sl.hz.clear();
-    while (cnt < sw->batch && f + cnt < c.nfreq &&
-           (!hybrid || (hz_of(f + cnt) >= sw->nodal_above) == sl.nodal)) {
+    while (cnt < sw->batch && f + cnt < c.nfreq && cls(f + cnt) == c0) {
         sl.hz.push_back(hz_of(f + cnt));
         cnt++;
     }
+    sl.sv.gmres_k = c0 == 1 ? sw->gmres_steps : 0;
     sl.f = (long)f;
     sl.cnt = cnt;
     sl.stt.assign(cnt, tags_item_stats{});
@@ -2352,6 +2394,14 @@ int tags_sweep_set_verify(tags_sweep* sw, int on)
 {
     if (!sw) return TAGS_ERR_ARG;
     sw->verify = on != 0;
+    return 0;
+}
+
+int tags_sweep_set_gmres(tags_sweep* sw, double below_hz, int steps)
+{
+    if (!sw || !(below_hz >= 0.0) || steps < 1 || steps > 32) return TAGS_ERR_ARG;
+    sw->gmres_below = below_hz;
+    sw->gmres_steps = steps;
     return 0;
 }
 

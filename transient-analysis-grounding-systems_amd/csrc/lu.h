@@ -419,6 +419,27 @@ int update_x(c64* x, const c32* z, const double* d, int n, double scale, hipStre
              const Grounding* gr = nullptr, const unsigned long long* rn = nullptr,
              unsigned long long* dxn = nullptr, const VecBatch& vb = VecBatch());
 int fill_c64(c64* p, long n, c64 v, hipStream_t st);
+
+// Work space of the FP64 flexible GMRES correction solve (gmres.hip) for cap
+// items of order n with up to kmax Arnoldi steps.
+struct GmresWork {
+    int n = 0, kmax = 0, cap = 0, nchunk = 0;
+    long n_cap = 0, part_cap = 0;
+    int kmax_cap = 0, item_cap = 0;
+    c64 *V = nullptr, *Z = nullptr, *w = nullptr, *H = nullptr, *g = nullptr, *y = nullptr;
+    c64 *h1 = nullptr, *h2 = nullptr, *nrm = nullptr, *sn = nullptr, *part = nullptr;
+    double *cs = nullptr, *inv = nullptr;
+    int alloc(int n_, int kmax_, int cap_);
+    void release();
+    ~GmresWork() { release(); }
+};
+// One outer step of GMRES-based refinement for nb items: d ~ W^-1 r by k
+// Arnoldi steps right-preconditioned with the complex64 factor (w32, W32,
+// row scaling d), then x += d with max |d| into dxn (item stride sn; items
+// with active[z] == 0 keep x). W at stride sW; r, x, t32 at stride n.
+int gmres_step(GmresWork& g, int k, int nb, LUWork<float>& w32, const c32* W32, c32* t32,
+               const double* d, const c64* W, long sW, const c64* r, c64* x,
+               unsigned long long* dxn, long sn, const int* active, hipStream_t st, Profiler* pr);
 int sym_from_upper(c64* W, long ld, int n, hipStream_t st);
 
 }  // namespace tags
