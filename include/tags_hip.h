@@ -471,7 +471,7 @@ int tags_sweep_set_batch(tags_sweep *sw, int samples);
  * backward_err is -1). No counterpart in the reference. */
 int tags_sweep_set_verify(tags_sweep *sw, int on);
 
-/* Groups of frequency samples a sweep keeps in flight (1 .. 4, default 2),
+/* Groups of frequency samples a sweep keeps in flight (1 .. 4, default 4),
  * each with its own HIP stream, W and factor work space (about 11 GB for a
  * one-sample group of the 60 x 60 m grid in the mixed form). The calling
  * thread queues a group on every slot, then finishes them in issue order,

@@ -1702,7 +1702,7 @@ struct tags_sweep {
     long m = 0, n = 0, N = 0;
     int plan_kind = TAGS_PLAN_HYBRID;
     int layout_kind = -1;  // plan kind the current layout and grounding sums belong to
-    int inflight = 2;      // groups of frequency samples in flight
+    int inflight = 4;      // groups of frequency samples in flight
     int batch_req = 0;     // samples per group as set (0: chosen from N and memory)
     int batch = 1;         // samples per group of the current call
     tags::LUPlan plan;
