@@ -181,6 +181,9 @@ struct Solver {
     // from the complex64 factor per step); set before issue_mixed
     int gmres_k = 0;
     GmresWork gm;
+    // the structure of W when it is an immittance matrix of the HEM (nullptr:
+    // dense): the refinement's FP64 products then read its impedance blocks only
+    const HemOp* hem = nullptr;
 
     // buffers for `batch` items of order n_ (the FP64 work space for all of them
     // in the plain form, for one in the mixed form, where it is the fallback's);
@@ -291,6 +294,7 @@ struct Solver {
         v.sw32 = (long)n * n;
         v.sn = kNormStride;
         v.active = masked ? active : nullptr;
+        v.hem = hem;
         return v;
     }
 
@@ -352,7 +356,7 @@ struct Solver {
             // the correction from FGMRES on W d = r, then r = b - W x
             if ((rc = gmres_step(gm, gmres_k, nb, w32, W32, t32, dscale, W, sW, r, x,
                                  norms + kNS * k + 3, kNormStride, masked ? active : nullptr, st,
-                                 pr)))
+                                 pr, hem)))
                 return rc;
             return residual64(W, n, n, x, b, r, norms + kNS * k, false, st, pr, false, v);
         }
@@ -1722,6 +1726,11 @@ struct tags_sweep {
     tags::DevSegs e, img;
     int* sn = nullptr;
     int* en = nullptr;
+    // each node's segment ends (node_ptr: n + 1, node_seg: 2 m at most) and the
+    // structure of W in the current layout, for the refinement's products
+    int* node_ptr = nullptr;
+    int* node_seg = nullptr;
+    tags::HemOp hem{};
     std::vector<std::unique_ptr<SweepSlot>> slots;
     bool profiling = false;
     // TAGS_FORM_MHEM: the gamma-free mHEM potentials (direct Z_L, Z_T, image
@@ -1750,6 +1759,7 @@ static void slot_bind(tags_sweep* sw, SweepSlot& sl)
     sl.sv.plan32_retry = nullptr;
     sl.sv.ground = sw->bindgr;
     sl.sv.gmres_k = 0;
+    sl.sv.hem = &sw->hem;
 }
 
 // The first k slots exist and are allocated for order N (precision `mixed`).
@@ -1840,6 +1850,15 @@ static int sweep_layout(tags_sweep* sw)
     sw->bind64 = p64;
     sw->bind32 = p32;
     sw->bindgr = gr;
+    sw->hem.m = m;
+    sw->hem.n = n;
+    sw->hem.node_off = sw->layout.node_off;
+    sw->hem.zl_off = sw->layout.zl_off;
+    sw->hem.zt_off = sw->layout.zt_off;
+    sw->hem.sn = sw->sn;
+    sw->hem.en = sw->en;
+    sw->hem.node_ptr = sw->node_ptr;
+    sw->hem.node_seg = sw->node_seg;
     for (auto& sl : sw->slots) {
         if (changed) sl->sv.ground_valid = false;
         slot_bind(sw, *sl);
@@ -1982,6 +2001,32 @@ int tags_sweep_create(tags_sweep** out, const Electrode* electrodes, const Elect
             return TAGS_ERR_ARG;
         }
     }
+    // each node's segment ends in segment order (a segment whose two ends
+    // share a node has its start entry only, as W has one entry there)
+    std::vector<int> hsn(m), hen(m);
+    TAGS_HIP(hipMemcpy(hsn.data(), sw->sn, sizeof(int) * m, hipMemcpyDeviceToHost));
+    TAGS_HIP(hipMemcpy(hen.data(), sw->en, sizeof(int) * m, hipMemcpyDeviceToHost));
+    std::vector<int> ptr(n + 1, 0), seg;
+    for (long s = 0; s < m; s++) {
+        if (hsn[s] < 0 || hsn[s] >= n || hen[s] < 0 || hen[s] >= n) return TAGS_ERR_ARG;
+        ptr[hsn[s] + 1]++;
+        if (hen[s] != hsn[s]) ptr[hen[s] + 1]++;
+    }
+    for (long u = 0; u < n; u++) ptr[u + 1] += ptr[u];
+    seg.assign(ptr[n], 0);
+    {
+        std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+        for (long s = 0; s < m; s++) {
+            seg[fill[hsn[s]]++] = (int)(2 * s);
+            if (hen[s] != hsn[s]) seg[fill[hen[s]]++] = (int)(2 * s + 1);
+        }
+    }
+    TAGS_HIP(hipMalloc(&sw->node_ptr, sizeof(int) * (n + 1)));
+    TAGS_HIP(hipMalloc(&sw->node_seg, sizeof(int) * std::max<size_t>(1, seg.size())));
+    TAGS_HIP(hipMemcpy(sw->node_ptr, ptr.data(), sizeof(int) * (n + 1), hipMemcpyHostToDevice));
+    if (!seg.empty())
+        TAGS_HIP(hipMemcpy(sw->node_seg, seg.data(), sizeof(int) * seg.size(),
+                           hipMemcpyHostToDevice));
     *out = sw.release();
     return 0;
 }
@@ -2135,6 +2180,8 @@ void tags_sweep_destroy(tags_sweep* sw)
     sw->slots.clear();
     hipFree(sw->sn);
     hipFree(sw->en);
+    hipFree(sw->node_ptr);
+    hipFree(sw->node_seg);
     hipFree(sw->pot);
     hipStream_t st = sw->st;
     delete sw;

@@ -327,7 +327,8 @@ static int gm_dots(GmresWork& g, int nb, const c64* V, long sV, int cnt, const c
 
 int gmres_step(GmresWork& g, int k, int nb, LUWork<float>& w32, const c32* W32, c32* t32,
                const double* d, const c64* W, long sW, const c64* r, c64* x,
-               unsigned long long* dxn, long sn, const int* active, hipStream_t st, Profiler* pr)
+               unsigned long long* dxn, long sn, const int* active, hipStream_t st, Profiler* pr,
+               const HemOp* hem)
 {
     const long n = g.n;
     if (k < 1 || k > g.kmax || nb < 1 || nb > g.cap) return TAGS_ERR_ARG;
@@ -350,8 +351,14 @@ int gmres_step(GmresWork& g, int k, int nb, LUWork<float>& w32, const c32* W32, 
         if ((rc = lu_solve_inplace<float>(w32, W32, (long)n, t32, st, nullptr))) return rc;
         hipLaunchKernelGGL(k_gm_storez, vg, dim3(256), 0, st, zj, sZ, t32, d, n);
         // w = W z_j
-        hipLaunchKernelGGL(k_gm_matvec, dim3((unsigned)n, 1, (unsigned)nb), dim3(256), 0, st, W,
-                           n, n, sW, zj, sZ, g.w, n);
+        if (hem) {
+            if ((rc = hem_apply(W, n, *hem, zj, sZ, nullptr, g.w, n, nullptr, 0, 0, nb, sW,
+                                nullptr, st)))
+                return rc;
+        } else {
+            hipLaunchKernelGGL(k_gm_matvec, dim3((unsigned)n, 1, (unsigned)nb), dim3(256), 0, st,
+                               W, n, n, sW, zj, sZ, g.w, n);
+        }
         // two Gram-Schmidt passes against v_1 .. v_j
         if ((rc = gm_dots(g, nb, g.V, sV, j + 1, g.w, n, g.h1, sh, st))) return rc;
         hipLaunchKernelGGL(k_gm_axpy, vg, dim3(256), 0, st, g.w, n, g.V, sV, g.h1, sh, j + 1, n);

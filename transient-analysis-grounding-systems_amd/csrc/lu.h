@@ -379,6 +379,22 @@ int launch_gemm_variant(const GemmArgs<T>& g, int variant, hipStream_t st);
 int gemm_set_variants(int precision, int big, int small);
 int gemm_variant_count(int precision);
 
+// Structure of an immittance matrix W of the HEM: the dense impedance blocks
+// at rows and columns [zl_off, zl_off + m) and [zt_off, zt_off + m) of W, the
+// incidence entries from the segments' nodes (A: +1 at the start node, -1 at
+// the end node; B: 1/2 at both; one entry where the two nodes coincide), zero
+// elsewhere. Products with W through it read only the impedance blocks of W
+// (2 m^2 of its N^2 entries). node_ptr / node_seg list each node's segment
+// ends in segment order: 2 s for the start of segment s, 2 s + 1 for its end.
+struct HemOp {
+    long m = 0, n = 0;  // segments, nodes
+    long node_off = 0, zl_off = 0, zt_off = 0;
+    const int* sn = nullptr;        // start node of each segment (device, m)
+    const int* en = nullptr;        // end node (device, m)
+    const int* node_ptr = nullptr;  // device, n + 1
+    const int* node_seg = nullptr;  // device, node_ptr[n]
+};
+
 // norms[0..1] (and [2] with want_wnorm) receive |r|, |x| (|W|) as max-reductions;
 // zero_norms false: the caller has already zeroed them (no memset node)
 // Batched forms (nb items): W of item z at W + z sw, vectors at stride n, the
@@ -389,6 +405,7 @@ struct VecBatch {
     long sw32 = 0; // c32 matrix stride
     long sn = 0;   // norms stride (u64)
     const int* active = nullptr;  // item z skipped where active[z] == 0 (nullable)
+    const HemOp* hem = nullptr;   // W has HEM structure: products read its impedance blocks only
 };
 int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
                unsigned long long* norms, bool want_wnorm, hipStream_t st, Profiler* pr,
@@ -436,10 +453,18 @@ struct GmresWork {
 // One outer step of GMRES-based refinement for nb items: d ~ W^-1 r by k
 // Arnoldi steps right-preconditioned with the complex64 factor (w32, W32,
 // row scaling d), then x += d with max |d| into dxn (item stride sn; items
-// with active[z] == 0 keep x). W at stride sW; r, x, t32 at stride n.
+// with active[z] == 0 keep x). W at stride sW (HEM structure when hem is
+// given); r, x, t32 at stride n.
 int gmres_step(GmresWork& g, int k, int nb, LUWork<float>& w32, const c32* W32, c32* t32,
                const double* d, const c64* W, long sW, const c64* r, c64* x,
-               unsigned long long* dxn, long sn, const int* active, hipStream_t st, Profiler* pr);
+               unsigned long long* dxn, long sn, const int* active, hipStream_t st, Profiler* pr,
+               const HemOp* hem = nullptr);
+// y = W x (mode 0), or r = b - W x with |r|, |x| (mode 1) and |W| (mode 2) as
+// max-reductions into norms, for a W of HEM structure; item z at W + z sw,
+// x + z sx, b and r + z sr, norms + z sn (items with active[z] == 0 skipped)
+int hem_apply(const c64* W, long ld, const HemOp& op, const c64* x, long sx, const c64* b,
+              c64* r, long sr, unsigned long long* norms, long sn, int mode, int nb, long sw,
+              const int* active, hipStream_t st);
 int sym_from_upper(c64* W, long ld, int n, hipStream_t st);
 
 }  // namespace tags

@@ -929,6 +929,126 @@ __global__ __launch_bounds__(256) void k_residual(const c64* W, long ld, int n, 
     }
 }
 
+// Products with a W of HEM structure (HemOp). Work-groups [0, 2m) take one
+// impedance row each (the Z_L block's rows, then Z_T's): the row's m entries
+// of the block from W, strided over the threads and summed in the order of
+// k_residual, then its one or two incidence entries. The remaining
+// work-groups take 256 node rows each, one thread per row, summing over the
+// node's segment ends in list order. mode 0: r = W x; 1: r = b - W x with
+// |r| and |x| max-reductions into norms[0], norms[1]; 2: also |W| (row sums
+// of |W_ij|) into norms[2].
+__global__ __launch_bounds__(256) void k_hem_apply(const c64* W, long ld, HemOp op, const c64* x,
+                                                    long sx, const c64* b, c64* r, long sr,
+                                                    unsigned long long* norms, long sn, int mode,
+                                                    long sw, const int* active)
+{
+    const long z_ = blockIdx.z;
+    if (active && !active[z_]) return;
+    W += z_ * sw;
+    x += z_ * sx;
+    if (b) b += z_ * sr;
+    r += z_ * sr;
+    if (norms) norms += z_ * sn;
+    const bool want_w = mode == 2;
+    const long m = op.m;
+    if ((long)blockIdx.x < 2 * m) {
+        const bool tblk = (long)blockIdx.x >= m;
+        const long i = tblk ? (long)blockIdx.x - m : (long)blockIdx.x;
+        const long off = tblk ? op.zt_off : op.zl_off;
+        const long row = off + i;
+        const c64* Wr = W + row * ld + off;
+        const c64* xz = x + off;
+        double sx_ = 0, sy_ = 0, an = 0;
+        for (long c = threadIdx.x; c < m; c += 256) {
+            const c64 a = Wr[c], v = xz[c];
+            sx_ += a.x * v.x - a.y * v.y;
+            sy_ += a.x * v.y + a.y * v.x;
+            if (want_w) an += sqrt(a.x * a.x + a.y * a.y);
+        }
+        __shared__ double red[3][4];
+        for (int o = 32; o > 0; o >>= 1) {
+            sx_ += __shfl_xor(sx_, o);
+            sy_ += __shfl_xor(sy_, o);
+            an += __shfl_xor(an, o);
+        }
+        const int wv = threadIdx.x >> 6;
+        if ((threadIdx.x & 63) == 0) {
+            red[0][wv] = sx_;
+            red[1][wv] = sy_;
+            red[2][wv] = an;
+        }
+        __syncthreads();
+        if (threadIdx.x != 0) return;
+        double tx = 0, ty = 0, ta = 0;
+        for (int q = 0; q < 4; q++) {
+            tx += red[0][q];
+            ty += red[1][q];
+            ta += red[2][q];
+        }
+        // incidence: A (+1 start, -1 end) on Z_L rows, B (1/2, 1/2) on Z_T rows
+        const int s = op.sn[i], e = op.en[i];
+        const double cs = tblk ? 0.5 : 1.0, ce = tblk ? 0.5 : -1.0;
+        const c64 xs = x[op.node_off + s];
+        tx += cs * xs.x;
+        ty += cs * xs.y;
+        ta += fabs(cs);
+        if (e != s) {
+            const c64 xe = x[op.node_off + e];
+            tx += ce * xe.x;
+            ty += ce * xe.y;
+            ta += fabs(ce);
+        }
+        if (mode == 0) {
+            r[row] = c64{tx, ty};
+            return;
+        }
+        const c64 rr = c64{b[row].x - tx, b[row].y - ty};
+        r[row] = rr;
+        const double ra = sqrt(rr.x * rr.x + rr.y * rr.y);
+        const double xa = sqrt(x[row].x * x[row].x + x[row].y * x[row].y);
+        atomicMax(&norms[0], (unsigned long long)__double_as_longlong(ra));
+        atomicMax(&norms[1], (unsigned long long)__double_as_longlong(xa));
+        if (want_w) atomicMax(&norms[2], (unsigned long long)__double_as_longlong(ta));
+        return;
+    }
+    // node rows: sum over the node's segment ends of A^T (x_L) and B^T (x_T)
+    const long u = ((long)blockIdx.x - 2 * m) * 256 + threadIdx.x;
+    double ra = 0, xa = 0, ta = 0;
+    if (u < op.n) {
+        double tx = 0, ty = 0;
+        const int p1 = op.node_ptr[u + 1];
+        for (int p = op.node_ptr[u]; p < p1; p++) {
+            const int se = op.node_seg[p];
+            const long s = se >> 1;
+            const double cl = (se & 1) ? -1.0 : 1.0;
+            const c64 xl = x[op.zl_off + s], xt = x[op.zt_off + s];
+            tx += cl * xl.x + 0.5 * xt.x;
+            ty += cl * xl.y + 0.5 * xt.y;
+            ta += 1.5;
+        }
+        const long row = op.node_off + u;
+        if (mode == 0) {
+            r[row] = c64{tx, ty};
+        } else {
+            const c64 rr = c64{b[row].x - tx, b[row].y - ty};
+            r[row] = rr;
+            ra = sqrt(rr.x * rr.x + rr.y * rr.y);
+            xa = sqrt(x[row].x * x[row].x + x[row].y * x[row].y);
+        }
+    }
+    if (mode == 0) return;
+    for (int o = 32; o > 0; o >>= 1) {
+        ra = fmax(ra, __shfl_xor(ra, o));
+        xa = fmax(xa, __shfl_xor(xa, o));
+        ta = fmax(ta, __shfl_xor(ta, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&norms[0], (unsigned long long)__double_as_longlong(ra));
+        atomicMax(&norms[1], (unsigned long long)__double_as_longlong(xa));
+        if (want_w) atomicMax(&norms[2], (unsigned long long)__double_as_longlong(ta));
+    }
+}
+
 // row maxima of |W_ij| (for symmetric equilibration): out[i] = 1/sqrt(max_j |W_ij|),
 // or 1 when equilibration is off
 __global__ __launch_bounds__(256) void k_row_scale(const c64* W, long ld, int n, double* d,
@@ -1919,10 +2039,30 @@ int residual64(const c64* W, long ld, int n, const c64* x, const c64* b, c64* r,
         if (vb.nb != 1) return TAGS_ERR_ARG;
         TAGS_HIP(hipMemsetAsync(norms, 0, sizeof(unsigned long long) * (want_wnorm ? 3 : 2), st));
     }
-    hipLaunchKernelGGL(k_residual, dim3(n, 1, vb.nb), dim3(256), 0, st, W, ld, n, x, b, r, norms,
-                       want_wnorm ? 1 : 0, vb.sw, vb.sn, vb.active);
-    TAGS_HIP(hipGetLastError());
+    if (vb.hem) {
+        const int rc = hem_apply(W, ld, *vb.hem, x, n, b, r, n, norms, vb.sn, want_wnorm ? 2 : 1,
+                                 vb.nb, vb.sw, vb.active, st);
+        if (rc) return rc;
+    } else {
+        hipLaunchKernelGGL(k_residual, dim3(n, 1, vb.nb), dim3(256), 0, st, W, ld, n, x, b, r,
+                           norms, want_wnorm ? 1 : 0, vb.sw, vb.sn, vb.active);
+        TAGS_HIP(hipGetLastError());
+    }
     tm.end();
+    return 0;
+}
+
+int hem_apply(const c64* W, long ld, const HemOp& op, const c64* x, long sx, const c64* b,
+              c64* r, long sr, unsigned long long* norms, long sn, int mode, int nb, long sw,
+              const int* active, hipStream_t st)
+{
+    if (op.m < 1 || op.n < 1 || nb < 1 || !op.sn || !op.en || !op.node_ptr || !op.node_seg)
+        return TAGS_ERR_ARG;
+    if (mode != 0 && (!b || !norms)) return TAGS_ERR_ARG;
+    const long g = 2 * op.m + (op.n + 255) / 256;
+    hipLaunchKernelGGL(k_hem_apply, dim3((unsigned)g, 1, (unsigned)nb), dim3(256), 0, st, W, ld, op,
+                       x, sx, b, r, sr, norms, sn, mode, sw, active);
+    TAGS_HIP(hipGetLastError());
     return 0;
 }
 
