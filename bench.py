@@ -488,6 +488,10 @@ def main():
                          "update's K; default 256)")
     ap.add_argument("--step-width-fp64", type=int, default=128, choices=(128, 256),
                     help="columns per factorisation step of the FP64 form (default 128)")
+    ap.add_argument("--solve-rows", type=int, default=2, choices=(0, 1, 2),
+                    help="triangular solves: one launch per range and direction with work-"
+                         "groups of 64 (1) or 128 (2, default) rows, or the per-step "
+                         "launches (0)")
     ap.add_argument("--inflight", type=int, default=4,
                     help="frequency samples in flight in the timed sweep (default 4)")
     ap.add_argument("--formulation", choices=("hem", "mhem"), default="hem",
@@ -539,6 +543,7 @@ def main():
         T.set_lookahead(False)
     if args.update_chunks is not None:
         T.set_update_chunks(args.update_chunks)
+    T.set_solve_rows(args.solve_rows)
     tuned = "defaults"
     if args.autotune:
         tuned = {"fp32": T.autotune(T.PREC_MIXED), "fp64": T.autotune(T.PREC_FP64)}
@@ -597,6 +602,7 @@ def main():
                    "nodal_above_hz": args.nodal_above,
                    "gmres_below_hz": args.gmres_below,
                    "gmres_steps": args.gmres_steps,
+                   "solve_rows": args.solve_rows,
                    "gemm_variants": tuned},
         "mixed": {"refine_iters": [q["refine_iters"] for q in st_mx],
                   "precision_used": [q["precision_used"] for q in st_mx],

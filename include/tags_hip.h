@@ -464,6 +464,48 @@ int tags_sweep_set_nodal_above(tags_sweep *sw, double hz);
  * pivoting panel take one sample at a time. No counterpart in the reference. */
 int tags_sweep_set_batch(tags_sweep *sw, int samples);
 
+/* Tile shapes and schedule of one sweep and precision (no counterpart in the
+ * reference). Fields < 0 (0 for step_width and inflight) leave the
+ * process-wide settings (tags_gemm_select, tags_set_step_width,
+ * tags_set_update_chunks, tags_set_solve_rows) or the sweep's own
+ * (tags_sweep_set_inflight, tags_sweep_set_batch, tags_sweep_set_nodal_above)
+ * as they are. */
+typedef struct {
+    int gemm_big;          /* MFMA tile variant of large products (tags_gemm_variants) */
+    int gemm_small;        /* of small products */
+    int step_width;        /* columns per factorisation step: 128 or 256 */
+    int update_chunks;     /* trailing-update launches beside the look-ahead panel */
+    int solve_rows;        /* triangular solves: 0, 1 or 2 (tags_set_solve_rows) */
+    int inflight;          /* groups of samples in flight, 1..4 */
+    int batch;             /* samples per group (0: chosen from N) */
+    double nodal_above;    /* hybrid plan's switch (Hz), mixed form only */
+    double ms_per_sample;  /* out: time per probe sample with these values */
+    int from_cache;        /* out: 1 read from the cache file, 0 measured */
+    int evaluations;       /* out: timed probe solves */
+} tags_tuning;
+
+/* Autotuner: picks the tile shapes and the schedule of `precision` for this
+ * sweep and installs them (tags_sweep_set_tuning). First every MFMA tile
+ * variant is timed on the sweep's own trailing-update shapes (the largest
+ * update of the plan's widest phase, and a 1024 x 1024 tail product) at both
+ * step widths; then, by coordinate descent from the current settings, step
+ * width (with its fastest variants), update launches, solve rows, samples in
+ * flight, samples per group (orders below 8192) and, for the mixed form of
+ * the hybrid plan, the nodal switch (0 or 20 kHz) are each kept when a
+ * timed solve of the `nprobe` probe samples (s, kappa, mur, inj_node as in
+ * tags_sweep_solve; one untimed solve first for the allocations) runs more
+ * than 3 % faster. With cache_path (nullable) an entry for the same device,
+ * formulation, plan, precision, N and m is used instead when present (unless
+ * force != 0), and the result is written there (one line per key). `out`
+ * (nullable) receives the values. Returns 0 or an error code. */
+int tags_sweep_autotune(tags_sweep *sw, int precision, size_t nprobe, const TAGS_COMPLEX *s,
+                        const TAGS_COMPLEX *kappa, double mur, size_t inj_node,
+                        const char *cache_path, int force, tags_tuning *out);
+/* Installs (set) or reads (get) the tuning of `precision`; set with NULL
+ * removes it. Returns 0, or TAGS_ERR_ARG for values out of range. */
+int tags_sweep_set_tuning(tags_sweep *sw, int precision, const tags_tuning *t);
+int tags_sweep_get_tuning(const tags_sweep *sw, int precision, tags_tuning *t);
+
 /* FP64 form only: with on != 0, W is assembled again after each sample's
  * factorisation (which replaced it) and the backward error
  * ||e_inj - W x||_inf / (||W||_inf ||x||_inf) goes to the item statistics
@@ -607,6 +649,17 @@ int tags_set_update_chunks(int n);
  * 256 (four sub-panels of 64; the trailing update is a K = 256 product) or 128;
  * 0 restores the defaults (128 for complex128 factors, 256 for complex64). */
 int tags_set_step_width(int columns);
+
+/* Triangular solves of every LU in the process (no reference counterpart;
+ * part of the solve that replaces zsysv, src/linalg.c:113-139): rows = 1 or 2
+ * runs each range of the solve forward and backward as one launch each,
+ * work-groups of 64 rows (1) or 128 rows (2; complex128 factors take 64)
+ * taking their blocks in dependency order and streaming the off-diagonal
+ * blocks while earlier blocks resolve (default 2); 0 launches one
+ * diagonal-block solve and one
+ * product per factorisation step. Results agree to rounding. Returns 0, or
+ * TAGS_ERR_ARG for other values. */
+int tags_set_solve_rows(int rows);
 
 /* Library version string. */
 const char *tags_version(void);

@@ -133,3 +133,17 @@ def test_argument_checks_without_gpu():
     assert T.fill_incidence_adm(a, a, e, nodes[:0]) == -3
     assert T.fill_incidence_adm(a, a, e[:0], nodes) == 1
     assert L.solve_admittance(None, None, 0) == 0
+
+
+def test_tuning_entry_points_check_arguments():
+    """The autotuner's and the solve knob's entry points reject bad arguments
+    before any device call."""
+    L = T.lib()
+    t = T.Tuning()
+    assert L.tags_sweep_autotune(None, 1, 1, None, None, 1.0, 0, None, 0, None) == -12
+    assert L.tags_sweep_set_tuning(None, 1, ctypes.byref(t)) == -12
+    assert L.tags_sweep_get_tuning(None, 0, ctypes.byref(t)) == -12
+    assert L.tags_set_solve_rows(3) == -12
+    assert L.tags_set_solve_rows(-1) == -12
+    assert L.tags_set_solve_rows(2) == 0
+    assert ctypes.sizeof(T.Tuning) == 7 * 4 + 4 + 2 * 8 + 2 * 4

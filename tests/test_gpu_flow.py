@@ -1,0 +1,107 @@
+"""GPU parity of the flow triangular solves: each range of a solve runs as
+one launch per direction, work-groups of 64 or 128 rows taking their blocks
+in dependency order and streaming the off-diagonal blocks while earlier
+blocks resolve. They solve with the same factor as the per-step launches
+(one diagonal-block solve and one product per factorisation step), so the
+two agree to rounding: the FP64 form within 1e-12, the mixed form within
+1e-9 (its refinement accepts at an estimated error <= 1e-10). Covered: the
+20 x 20 m grid (N = 1185) for the dense, Z_L-first, nodal and hybrid plans
+with batched groups of samples, the dense solve entry at an order that is
+not a multiple of 64 against LAPACK's answer, and the 30 x 30 m grid
+(N = 3861) at three Laplace samples."""
+import numpy as np
+import pytest
+
+import tags_pkg
+
+T = tags_pkg.load()
+O = tags_pkg.oracle()
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(autouse=True)
+def _restore_default():
+    yield
+    T.set_solve_rows(2)
+
+
+def _c2():
+    rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
+    f = O.logspace(2, 7, 400)[::19]  # 22 samples, 100 Hz .. 10 MHz
+    s = 2j * np.pi * f
+    kap = 1e-3 + s * 15.0 * T.EPS0
+    return e, nodes, s, kap
+
+
+@pytest.mark.parametrize("plan", [0, 2, 6, 7])
+def test_flow_matches_step_solves(plan):
+    e, nodes, s, kap = _c2()
+    sw = T.Sweep(e, nodes)
+    sw.set_plan(plan)
+    res = {}
+    for rows in (0, 1, 2):
+        T.set_solve_rows(rows)
+        u64, _ = sw.solve(s, kap, 0, T.PREC_FP64)
+        umx, stmx = sw.solve(s, kap, 0, T.PREC_MIXED)
+        res[rows] = (u64, umx, stmx)
+    sw.close()
+    ref64, refmx, _ = res[0]
+    for rows, (u64, umx, stmx) in res.items():
+        d64 = np.max(np.abs(u64 - ref64) / np.abs(ref64))
+        dmx = np.max(np.abs(umx - refmx) / np.abs(refmx))
+        print("plan %d rows %d: fp64 %.2e mixed %.2e steps %s" %
+              (plan, rows, d64, dmx, [q["refine_iters"] for q in stmx]), flush=True)
+        assert d64 <= 1e-12, (rows, d64)
+        assert dmx <= 1e-9, (rows, dmx)
+        assert all(q["precision_used"] == 1 for q in stmx)
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_flow_dense_solve_vs_lapack(precision):
+    from scipy.linalg import lapack
+    rng = np.random.default_rng(11)
+    n = 777
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    a += n ** 0.5 * np.eye(n)
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    _, _, xr, info = lapack.zgesv(a, b.reshape(-1, 1))
+    assert info == 0
+    xr = xr.ravel()
+    xs = {}
+    for rows in (0, 1, 2):
+        T.set_solve_rows(rows)
+        rc, x, _ = T.dense_solve(a, b, precision)
+        assert rc == 0, rc
+        xs[rows] = x
+        d = np.max(np.abs(x - xr)) / np.max(np.abs(xr))
+        print("precision %d rows %d: vs LAPACK %.2e" % (precision, rows, d), flush=True)
+        assert d <= 1e-10, (rows, d)
+    for rows in (1, 2):
+        assert np.max(np.abs(xs[rows] - xs[0])) / np.max(np.abs(xs[0])) <= 1e-12
+
+
+def test_flow_grid30_laplace_samples():
+    _, e, nodes = T.electrode_grid(11, 11, 30.0, 30.0, 6, 6, 5e-3, -0.5)
+    inj = int(np.argmin(np.sum((nodes - [15.0, 15.0, -0.5]) ** 2, axis=1)))
+    nt, tmax = 2 ** 14, 1e-4
+    k = np.array([1, 300, 4000])
+    s = np.log(nt ** 2) / tmax + 2j * np.pi * k / tmax
+    sigma0 = 5e-4
+    h = 1.26 * (1000 * sigma0) ** -0.73
+    kap = np.array([sg + sk * er * T.EPS0 for sk in s
+                    for sg, er in [T.alipio_soil(sigma0, sk, h, 0.54, 12.0)]])
+    sw = T.Sweep(e, nodes)
+    res = {}
+    for rows in (0, 2):
+        T.set_solve_rows(rows)
+        u64, _ = sw.solve(s, kap, inj, T.PREC_FP64)
+        umx, stmx = sw.solve(s, kap, inj, T.PREC_MIXED)
+        res[rows] = (u64, umx)
+        assert all(q["precision_used"] == 1 for q in stmx)
+    sw.close()
+    d64 = np.max(np.abs(res[2][0] - res[0][0]) / np.abs(res[0][0]))
+    dmx = np.max(np.abs(res[2][1] - res[0][1]) / np.abs(res[0][1]))
+    dx = np.max(np.abs(res[2][1] - res[2][0]) / np.abs(res[2][0]))
+    print("grid30: fp64 %.2e mixed %.2e mixed vs fp64 %.2e" % (d64, dmx, dx), flush=True)
+    assert d64 <= 1e-12 and dmx <= 1e-9 and dx <= 1e-8

@@ -1,0 +1,73 @@
+"""GPU tests of the autotuner (tags_sweep_autotune): on the 20 x 20 m grid
+(N = 1185) it times the MFMA tile variants on the sweep's own update shapes
+and the schedule knobs on probe solves, installs the choice, writes it to the
+cache file and reads it back for the same system. A tuned sweep answers as an
+untuned one to rounding (the tile shapes, step width, launch split, solve rows
+and grouping change no sample's arithmetic beyond rounding): FP64 within
+1e-12, the mixed form within 1e-9, both within 1e-8 of each other."""
+import numpy as np
+import pytest
+
+import tags_pkg
+
+T = tags_pkg.load()
+O = tags_pkg.oracle()
+
+pytestmark = pytest.mark.gpu
+
+
+def _c2():
+    rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
+    f = O.logspace(2, 7, 400)[::23]  # 18 samples
+    s = 2j * np.pi * f
+    kap = 1e-3 + s * 15.0 * T.EPS0
+    return e, nodes, s, kap
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_autotune_installs_and_caches(precision, tmp_path):
+    e, nodes, s, kap = _c2()
+    cache = str(tmp_path / "tuning.txt")
+    sw = T.Sweep(e, nodes)
+    u0, _ = sw.solve(s, kap, 0, precision)
+    t = sw.autotune(precision, s[::3], kap[::3], 0, cache=cache)
+    print("tuned:", t, flush=True)
+    assert t["from_cache"] == 0 and t["evaluations"] >= 5
+    assert t["step_width"] in (128, 256) and t["inflight"] in (1, 2, 3, 4)
+    assert 0 < t["ms_per_sample"] < 1e4
+    assert sw.tuning(precision)["gemm_big"] == t["gemm_big"]
+    u1, st1 = sw.solve(s, kap, 0, precision)
+    d = np.max(np.abs(u1 - u0) / np.abs(u0))
+    print("tuned vs default: %.2e" % d, flush=True)
+    assert d <= (1e-12 if precision == 0 else 1e-9)
+    if precision == 1:
+        assert all(q["precision_used"] == 1 for q in st1)
+    sw.close()
+    with open(cache) as f:
+        lines = [l for l in f if " | " in l]
+    assert len(lines) == 1 and ("prec=%s" % ("mixed" if precision else "fp64")) in lines[0]
+    # a second sweep of the same system reads the entry instead of timing
+    sw2 = T.Sweep(e, nodes)
+    t2 = sw2.autotune(precision, s[::3], kap[::3], 0, cache=cache)
+    assert t2["from_cache"] == 1 and t2["evaluations"] == 0
+    for k in ("gemm_big", "gemm_small", "step_width", "update_chunks", "solve_rows",
+              "inflight", "batch"):
+        assert t2[k] == t[k], k
+    u2, _ = sw2.solve(s, kap, 0, precision)
+    assert np.max(np.abs(u2 - u1) / np.abs(u1)) <= 1e-12
+    sw2.set_tuning(precision, None)
+    assert sw2.tuning(precision) is None
+    sw2.close()
+
+
+def test_tuning_rejects_bad_values():
+    e, nodes, s, kap = _c2()
+    sw = T.Sweep(e, nodes)
+    bad = dict(gemm_big=999, gemm_small=0, step_width=256, update_chunks=-1, solve_rows=-1,
+               inflight=0, batch=-1, nodal_above=-1.0)
+    with pytest.raises(RuntimeError):
+        sw.set_tuning(1, bad)
+    bad.update(gemm_big=0, step_width=100)
+    with pytest.raises(RuntimeError):
+        sw.set_tuning(1, bad)
+    sw.close()

@@ -119,6 +119,19 @@ class ItemStats(ctypes.Structure):
                 ("refine_hist", ctypes.c_double * 8), ("forward_est", ctypes.c_double)]
 
 
+class Tuning(ctypes.Structure):
+    """tags_tuning: tile shapes and schedule of one sweep and precision."""
+    _fields_ = [("gemm_big", ctypes.c_int), ("gemm_small", ctypes.c_int),
+                ("step_width", ctypes.c_int), ("update_chunks", ctypes.c_int),
+                ("solve_rows", ctypes.c_int), ("inflight", ctypes.c_int),
+                ("batch", ctypes.c_int), ("nodal_above", ctypes.c_double),
+                ("ms_per_sample", ctypes.c_double), ("from_cache", ctypes.c_int),
+                ("evaluations", ctypes.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class CaseDesc(ctypes.Structure):
     """tags_case: one grounding case of a multi-case solve."""
     _fields_ = [("electrodes", ctypes.c_void_p), ("images", ctypes.c_void_p),
@@ -147,7 +160,7 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_dense_solve", "tags_version", "tags_device_synchronize", "tags_gemm_bench",
             "tags_gemm_variants", "tags_gemm_select", "tags_autotune", "tags_sweep_set_plan",
             "tags_sweep_assemble", "tags_immittance_solve", "tags_sweep_set_inflight",
-            "tags_gemm_run", "tags_set_step_width", "tags_sweep_set_nodal_above",
+            "tags_gemm_run", "tags_set_step_width", "tags_set_solve_rows", "tags_sweep_set_nodal_above",
             "fill_incidence_adm", "fill_impedance_adm", "calculate_yla_ytb",
             "fill_impedance_adm2", "solve_admittance", "tags_sweep_set_formulation",
             "alipio_soil", "smith_longmire_soil", "laplace_trans", "inv_laplace_trans",
@@ -158,6 +171,7 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_sweep_assemble_entries", "tags_sweep_set_batch", "tags_cases_create",
             "tags_cases_solve", "tags_cases_set_plan", "tags_cases_set_lanes",
             "tags_cases_set_profiling", "tags_cases_get_timers", "tags_cases_destroy",
+            "tags_sweep_autotune", "tags_sweep_set_tuning", "tags_sweep_get_tuning",
             "tags_sweep_set_gmres"]
 
 _lib = None
@@ -226,6 +240,10 @@ def lib():
         "tags_set_lookahead": ([I], I),
         "tags_set_update_chunks": ([I], I),
         "tags_set_step_width": ([I], I),
+        "tags_set_solve_rows": ([I], I),
+        "tags_sweep_autotune": ([P, I, S, P, P, D, S, ctypes.c_char_p, I, P], I),
+        "tags_sweep_set_tuning": ([P, I, P], I),
+        "tags_sweep_get_tuning": ([P, I, P], I),
         "tags_sweep_set_nodal_above": ([P, D], I),
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
@@ -539,6 +557,15 @@ def set_step_width(columns):
         raise RuntimeError("tags_set_step_width failed: %d" % rc)
 
 
+def set_solve_rows(rows):
+    """Triangular solves of every later LU: 1 or 2 (64 or 128 rows per
+    work-group) for one launch per range and direction, 0 for the per-step
+    launches."""
+    rc = lib().tags_set_solve_rows(int(rows))
+    if rc:
+        raise RuntimeError("tags_set_solve_rows failed: %d" % rc)
+
+
 def set_update_chunks(n):
     """Launches each step's look-ahead trailing update is split into."""
     rc = lib().tags_set_update_chunks(int(n))
@@ -691,6 +718,39 @@ class Sweep:
         rc = lib().tags_sweep_set_inflight(self._h, samples)
         if rc:
             raise RuntimeError("tags_sweep_set_inflight failed: %d" % rc)
+
+    def autotune(self, precision, s, kappa, inj_node=0, mur=1.0, cache=None, force=False):
+        """Picks and installs this sweep's tile shapes and schedule for
+        `precision` (tags_sweep_autotune) from timed solves of the probe
+        samples s, kappa; with `cache` (a file path) a stored entry for the
+        same device and system is used instead unless force, and the result is
+        stored. Returns the values as a dict."""
+        s = np.ascontiguousarray(s, np.complex128)
+        kappa = np.ascontiguousarray(kappa, np.complex128)
+        t = Tuning()
+        rc = lib().tags_sweep_autotune(self._h, int(precision), len(s), _p(s), _p(kappa),
+                                       float(mur), int(inj_node),
+                                       None if cache is None else os.fsencode(cache),
+                                       1 if force else 0, ctypes.byref(t))
+        if rc:
+            raise RuntimeError("tags_sweep_autotune failed: %d" % rc)
+        return t.as_dict()
+
+    def set_tuning(self, precision, values=None):
+        """Installs tuned values (a dict as autotune returns; None removes)."""
+        if values is None:
+            rc = lib().tags_sweep_set_tuning(self._h, int(precision), None)
+        else:
+            t = Tuning(**{k: values[k] for k, _ in Tuning._fields_ if k in values})
+            rc = lib().tags_sweep_set_tuning(self._h, int(precision), ctypes.byref(t))
+        if rc:
+            raise RuntimeError("tags_sweep_set_tuning failed: %d" % rc)
+
+    def tuning(self, precision):
+        """The installed tuning of `precision` as a dict, or None."""
+        t = Tuning()
+        rc = lib().tags_sweep_get_tuning(self._h, int(precision), ctypes.byref(t))
+        return t.as_dict() if rc == 0 else None
 
     def set_profiling(self, on=True):
         lib().tags_sweep_set_profiling(self._h, 1 if on else 0)

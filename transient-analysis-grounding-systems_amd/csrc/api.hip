@@ -4,11 +4,13 @@
 #include <cfloat>
 #include <cmath>
 #include <complex>
+#include <cstdio>
 #include <cstring>
 #include <memory>
 #include <chrono>
 #include <mutex>
 #include <thread>
+#include <string>
 #include <vector>
 
 #include "lu.h"
@@ -804,6 +806,13 @@ int tags_set_step_width(int columns)
 {
     if (columns != 0 && columns != 128 && columns != 256) return TAGS_ERR_ARG;
     lu_set_step_width(columns);
+    return 0;
+}
+
+int tags_set_solve_rows(int rows)
+{
+    if (rows != 0 && rows != 1 && rows != 2) return TAGS_ERR_ARG;
+    lu_set_solve_rows(rows);
     return 0;
 }
 
@@ -1740,6 +1749,13 @@ struct tags_sweep {
     // FP64 form: after each solve, assemble W again (the factor replaced it)
     // and report the backward error of x in the item statistics
     bool verify = false;
+    // tuned tile shapes and schedule per precision (tags_sweep_set_tuning):
+    // the kernel-level part is installed for the calling thread during a
+    // solve, the sweep-level part (in flight, group size, switch) replaces the
+    // sweep's settings for the call
+    tags_tuning tune[2] = {};
+    bool tuned[2] = {false, false};
+    tags::Tuning ktune[2];
     // the plans a slot's solver takes for this sweep (set by sweep_layout)
     const tags::LUPlan* bind64 = nullptr;
     const tags::LUPlan* bind32 = nullptr;
@@ -2371,15 +2387,22 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     if (!sw || !s || !kappa) return TAGS_ERR_ARG;
     if ((long)inj_node >= sw->n) return TAGS_ERR_ARG;
     const bool mixed = precision == TAGS_PREC_MIXED;
+    // the tuned values of this precision, if any, for this call
+    const tags_tuning* tu = sw->tuned[mixed ? 1 : 0] ? &sw->tune[mixed ? 1 : 0] : nullptr;
+    TuningScope scope(tu ? &sw->ktune[mixed ? 1 : 0] : tuning_current());
+    const int batch_req0 = sw->batch_req;
+    if (tu && tu->batch >= 0) sw->batch_req = tu->batch;
     // groups of samples share launches; grounded plans take one sample at a time
     sw->batch = (sw->plan_kind == TAGS_PLAN_GROUNDED ||
                  sw->plan_kind == TAGS_PLAN_GROUNDED_PIVOTED) && mixed
                     ? 1
                     : sweep_batch(sw, mixed);
+    sw->batch_req = batch_req0;
     if (lu_panel_kind() == PANEL_PARTIAL) sw->batch = 1;
+    const int k = tu && tu->inflight > 0 ? tu->inflight : sw->inflight;
     // every configured slot is allocated up front (a warm-up call sets all of
     // them up)
-    int rc = sweep_slots(sw, sw->inflight, mixed);
+    int rc = sweep_slots(sw, k, mixed);
     if (rc || (rc = sweep_potentials(sw))) return rc;
     // after the slots exist: the layout sets every slot's plans
     sweep_layout(sw);
@@ -2388,7 +2411,7 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     // the hybrid plan's switch may rise during this call; it is restored at
     // the end, so that every call starts from the configured switch
     const double nodal_above0 = sw->nodal_above;
-    const int k = sw->inflight;
+    if (tu && mixed && tu->nodal_above >= 0.0) sw->nodal_above = tu->nodal_above;
     size_t next = 0;
     for (int i = 0; i < k && next < nfreq; i++) {
         const int got = sweep_issue(sw, *sw->slots[i], next, c);
@@ -2461,6 +2484,320 @@ int tags_sweep_set_batch(tags_sweep* sw, int samples)
 
 }  // extern "C"
 
+
+// ---------------------------------------------------------------------------
+// Autotuner: tile shapes on the sweep's own update shapes, then the schedule by
+// coordinate descent over timed probe solves, with an on-disk cache.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr double kTuneGain = 0.97;  // a candidate must be 3 % faster to replace
+
+bool tuning_valid(const tags_tuning& t, int precision)
+{
+    const int nv = gemm_variant_count(precision == TAGS_PREC_MIXED ? 1 : 0);
+    if (t.gemm_big >= nv || t.gemm_small >= nv) return false;
+    if (t.step_width != 0 && t.step_width != 128 && t.step_width != 256) return false;
+    if (t.update_chunks == 0 || t.update_chunks > 64) return false;
+    if (t.solve_rows > 2) return false;
+    if (t.inflight < 0 || t.inflight > 4) return false;
+    if (t.batch > 4096) return false;
+    return true;
+}
+
+void tuning_kernel_part(const tags_tuning& t, int precision, Tuning* k)
+{
+    const int p = precision == TAGS_PREC_MIXED ? 1 : 0;
+    *k = Tuning();
+    k->gemm_big[p] = t.gemm_big;
+    k->gemm_small[p] = t.gemm_small;
+    k->step_width[p] = t.step_width;
+    k->update_chunks = t.update_chunks;
+    k->solve_rows = t.solve_rows;
+}
+
+std::string tune_key(const tags_sweep* sw, int precision)
+{
+    int dev = 0;
+    hipDeviceProp_t p;
+    std::string arch = "gpu";
+    int cus = 0;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&p, dev) == hipSuccess) {
+        arch = p.gcnArchName;
+        const size_t colon = arch.find(':');
+        if (colon != std::string::npos) arch.resize(colon);
+        cus = p.multiProcessorCount;
+    }
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s cu=%d form=%s plan=%d prec=%s N=%ld m=%ld", arch.c_str(),
+                  cus, sw->formulation == TAGS_FORM_MHEM ? "mhem" : "hem", sw->plan_kind,
+                  precision == TAGS_PREC_MIXED ? "mixed" : "fp64", sw->N, sw->m);
+    return buf;
+}
+
+std::string tune_value(const tags_tuning& t)
+{
+    char buf[256];
+    std::snprintf(buf, sizeof(buf),
+                  "big=%d small=%d width=%d chunks=%d rows=%d inflight=%d batch=%d "
+                  "nodal_above=%.6g ms=%.4f",
+                  t.gemm_big, t.gemm_small, t.step_width, t.update_chunks, t.solve_rows,
+                  t.inflight, t.batch, t.nodal_above, t.ms_per_sample);
+    return buf;
+}
+
+// "<key> | <value>" lines; '#' starts a comment line
+bool tune_cache_read(const char* path, const std::string& key, tags_tuning* t)
+{
+    FILE* f = std::fopen(path, "r");
+    if (!f) return false;
+    char line[1024];
+    bool found = false;
+    while (std::fgets(line, sizeof(line), f)) {
+        std::string l(line);
+        while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+        if (l.empty() || l[0] == '#') continue;
+        const size_t bar = l.find(" | ");
+        if (bar == std::string::npos || l.compare(0, bar, key) != 0 || bar != key.size()) continue;
+        tags_tuning v{};
+        if (std::sscanf(l.c_str() + bar + 3,
+                        "big=%d small=%d width=%d chunks=%d rows=%d inflight=%d batch=%d "
+                        "nodal_above=%lf ms=%lf",
+                        &v.gemm_big, &v.gemm_small, &v.step_width, &v.update_chunks,
+                        &v.solve_rows, &v.inflight, &v.batch, &v.nodal_above,
+                        &v.ms_per_sample) == 9) {
+            *t = v;
+            found = true;
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+int tune_cache_write(const char* path, const std::string& key, const tags_tuning& t)
+{
+    std::vector<std::string> keep;
+    if (FILE* f = std::fopen(path, "r")) {
+        char line[1024];
+        while (std::fgets(line, sizeof(line), f)) {
+            std::string l(line);
+            while (!l.empty() && (l.back() == '\n' || l.back() == '\r')) l.pop_back();
+            const size_t bar = l.find(" | ");
+            if (!l.empty() && l[0] != '#' && bar == key.size() && l.compare(0, bar, key) == 0)
+                continue;
+            keep.push_back(l);
+        }
+        std::fclose(f);
+    }
+    keep.push_back(key + " | " + tune_value(t));
+    const std::string tmp = std::string(path) + ".tmp";
+    FILE* f = std::fopen(tmp.c_str(), "w");
+    if (!f) return TAGS_ERR_ARG;
+    for (const std::string& l : keep) std::fprintf(f, "%s\n", l.c_str());
+    if (std::fclose(f) != 0 || std::rename(tmp.c_str(), path) != 0) return TAGS_ERR_ARG;
+    return 0;
+}
+
+// Mean ms per launch of the fastest variant for C -= A B (M x N x K) in the
+// given precision; *best = its index (-1 when none ran).
+int tune_gemm(int precision, int M, int N, int K, int iters, int* best, double* best_ms)
+{
+    *best = -1;
+    *best_ms = 1e30;
+    const int nv = gemm_variant_count(precision == TAGS_PREC_MIXED ? 1 : 0);
+    for (int v = 0; v < nv; v++) {
+        double ms = 0;
+        const int rc = precision == TAGS_PREC_MIXED ? gemm_bench_t<float>(M, N, K, v, iters, &ms)
+                                                    : gemm_bench_t<double>(M, N, K, v, iters, &ms);
+        if (rc == TAGS_ERR_ARG) continue;  // a variant that does not take this shape
+        if (rc) return rc;
+        if (ms < *best_ms) {
+            *best_ms = ms;
+            *best = v;
+        }
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int tags_sweep_set_tuning(tags_sweep* sw, int precision, const tags_tuning* t)
+{
+    if (!sw || (precision != TAGS_PREC_FP64 && precision != TAGS_PREC_MIXED)) return TAGS_ERR_ARG;
+    const int p = precision == TAGS_PREC_MIXED ? 1 : 0;
+    if (!t) {
+        sw->tuned[p] = false;
+        return 0;
+    }
+    if (!tuning_valid(*t, precision)) return TAGS_ERR_ARG;
+    sw->tune[p] = *t;
+    tuning_kernel_part(*t, precision, &sw->ktune[p]);
+    sw->tuned[p] = true;
+    return 0;
+}
+
+int tags_sweep_get_tuning(const tags_sweep* sw, int precision, tags_tuning* t)
+{
+    if (!sw || !t || (precision != TAGS_PREC_FP64 && precision != TAGS_PREC_MIXED))
+        return TAGS_ERR_ARG;
+    const int p = precision == TAGS_PREC_MIXED ? 1 : 0;
+    if (!sw->tuned[p]) return TAGS_ERR_ARG;
+    *t = sw->tune[p];
+    return 0;
+}
+
+int tags_sweep_autotune(tags_sweep* sw, int precision, size_t nprobe, const TAGS_COMPLEX* s,
+                        const TAGS_COMPLEX* kappa, double mur, size_t inj_node,
+                        const char* cache_path, int force, tags_tuning* out)
+{
+    if (!sw || !s || !kappa || nprobe < 1 || (long)inj_node >= sw->n ||
+        (precision != TAGS_PREC_FP64 && precision != TAGS_PREC_MIXED))
+        return TAGS_ERR_ARG;
+    const bool mixed = precision == TAGS_PREC_MIXED;
+    const int p = mixed ? 1 : 0;
+    const std::string key = tune_key(sw, precision);
+    if (cache_path && !force) {
+        tags_tuning t{};
+        if (tune_cache_read(cache_path, key, &t) && tuning_valid(t, precision)) {
+            t.from_cache = 1;
+            t.evaluations = 0;
+            int rc = tags_sweep_set_tuning(sw, precision, &t);
+            if (rc) return rc;
+            if (out) *out = t;
+            return 0;
+        }
+    }
+    // 1. tile variants on the plan's own update shapes, per step width
+    int rc = sweep_layout(sw);
+    if (rc) return rc;
+    const LUPlan* plan = mixed ? (sw->plan_kind == TAGS_PLAN_HYBRID ? &sw->plan_nodal
+                                  : sw->bind32                      ? sw->bind32
+                                                                    : sw->bind64)
+                               : sw->bind64;
+    if (!plan || plan->phases.empty()) return TAGS_ERR_ARG;
+    const LUPhase* wide = &plan->phases[0];
+    auto cols = [](const LUPhase& ph) {
+        return (ph.c_end - ph.c_begin) + std::max(0, ph.x_end - ph.x_begin);
+    };
+    for (const LUPhase& ph : plan->phases)
+        if (cols(ph) > cols(*wide)) wide = &ph;
+    const int widths[2] = {128, 256};
+    int big[2] = {-1, -1}, small[2] = {-1, -1};
+    for (int wi = 0; wi < 2; wi++) {
+        const int w = widths[wi];
+        const int M = std::min(12288, std::max(64, wide->row_end - wide->c_begin - w));
+        const int N = std::min(12288, std::max(64, cols(*wide) - w));
+        double ms = 0;
+        if ((rc = tune_gemm(precision, M, N, w, 3, &big[wi], &ms))) return rc;
+        const int Ms = std::min(1024, M), Ns = std::min(1024, N);
+        if ((rc = tune_gemm(precision, Ms, Ns, w, 10, &small[wi], &ms))) return rc;
+        if (big[wi] < 0 || small[wi] < 0) return TAGS_ERR_ARG;
+    }
+    // 2. the schedule, by coordinate descent over timed probe solves
+    const bool had = sw->tuned[p];
+    const tags_tuning saved = sw->tune[p];
+    std::vector<TAGS_COMPLEX> u(nprobe);
+    int evals = 0;
+    auto eval = [&](const tags_tuning& t, double* ms) -> int {
+        int r = tags_sweep_set_tuning(sw, precision, &t);
+        if (r) return r;
+        const int k = t.inflight > 0 ? t.inflight : sw->inflight;
+        const size_t nw = std::min(nprobe, (size_t)k);
+        r = tags_sweep_solve(sw, nw, s, kappa, mur, nullptr, nullptr, nullptr, inj_node,
+                             precision, u.data(), nullptr, nullptr);
+        if (r) return r;
+        const auto t0 = std::chrono::steady_clock::now();
+        r = tags_sweep_solve(sw, nprobe, s, kappa, mur, nullptr, nullptr, nullptr, inj_node,
+                             precision, u.data(), nullptr, nullptr);
+        const auto t1 = std::chrono::steady_clock::now();
+        if (r) return r;
+        evals++;
+        *ms = std::chrono::duration<double, std::milli>(t1 - t0).count() / (double)nprobe;
+        for (size_t i = 0; i < nprobe; i++)
+            if (!std::isfinite(__real__ u[i]) || !std::isfinite(__imag__ u[i])) *ms = 1e30;
+        return 0;
+    };
+    const int w0 = lu_step_width(mixed) == 128 ? 0 : 1;
+    tags_tuning cur{};
+    cur.gemm_big = big[w0];
+    cur.gemm_small = small[w0];
+    cur.step_width = widths[w0];
+    cur.update_chunks = -1;
+    cur.solve_rows = -1;
+    cur.inflight = sw->inflight;
+    cur.batch = -1;
+    cur.nodal_above = mixed ? sw->nodal_above : -1.0;
+    double best = 0;
+    auto fail = [&](int r) {
+        if (had) tags_sweep_set_tuning(sw, precision, &saved);
+        else sw->tuned[p] = false;
+        return r;
+    };
+    if ((rc = eval(cur, &best))) return fail(rc);
+    auto consider = [&](tags_tuning cand) -> int {
+        double ms = 0;
+        const int r = eval(cand, &ms);
+        if (r == TAGS_ERR_HIP) {
+            // this candidate did not fit (memory): release what it took
+            (void)hipGetLastError();
+            return 0;
+        }
+        if (r) return r;
+        if (ms < kTuneGain * best) {
+            best = ms;
+            cur = cand;
+        }
+        return 0;
+    };
+    {
+        tags_tuning c = cur;
+        c.step_width = widths[1 - w0];
+        c.gemm_big = big[1 - w0];
+        c.gemm_small = small[1 - w0];
+        if ((rc = consider(c))) return fail(rc);
+    }
+    for (int ch : {1, 2, 8}) {
+        tags_tuning c = cur;
+        c.update_chunks = ch;
+        if ((rc = consider(c))) return fail(rc);
+    }
+    for (int rows : {0, 1}) {
+        tags_tuning c = cur;
+        c.solve_rows = rows;
+        if ((rc = consider(c))) return fail(rc);
+    }
+    for (int k = 1; k <= 4; k++) {
+        if (k == cur.inflight) continue;
+        tags_tuning c = cur;
+        c.inflight = k;
+        if ((rc = consider(c))) return fail(rc);
+    }
+    if (sw->N < kBatchOneAbove) {
+        const int b0 = sweep_batch(sw, mixed);
+        for (int b : {b0 / 2, b0 / 4}) {
+            if (b < 1 || b == b0) continue;
+            tags_tuning c = cur;
+            c.batch = b;
+            if ((rc = consider(c))) return fail(rc);
+        }
+    }
+    if (mixed && sw->plan_kind == TAGS_PLAN_HYBRID) {
+        tags_tuning c = cur;
+        c.nodal_above = cur.nodal_above > 0.0 ? 0.0 : 2e4;
+        if ((rc = consider(c))) return fail(rc);
+    }
+    cur.ms_per_sample = best;
+    cur.from_cache = 0;
+    cur.evaluations = evals;
+    if ((rc = tags_sweep_set_tuning(sw, precision, &cur))) return rc;
+    if (cache_path && (rc = tune_cache_write(cache_path, key, cur))) return rc;
+    if (out) *out = cur;
+    return 0;
+}
+
+}  // extern "C"
 
 // ---------------------------------------------------------------------------
 // Many grounding cases: every (case, frequency) sample of every case, in
@@ -2571,10 +2908,14 @@ int tags_cases_solve(tags_cases* cs, int precision)
         const tags_case& d = cs->cs[c];
         sw->plan_kind = cs->plan_kind;
         sweep_layout(sw);
+        const tags_tuning* tu = sw->tuned[mixed ? 1 : 0] ? &sw->tune[mixed ? 1 : 0] : nullptr;
+        const int batch_req0 = sw->batch_req;
+        if (tu && tu->batch >= 0) sw->batch_req = tu->batch;
         sw->batch = (sw->plan_kind == TAGS_PLAN_GROUNDED ||
                      sw->plan_kind == TAGS_PLAN_GROUNDED_PIVOTED) && mixed
                         ? 1
                         : sweep_batch(sw, mixed);
+        sw->batch_req = batch_req0;
         if (lu_panel_kind() == PANEL_PARTIAL) sw->batch = 1;
         if ((rc = sweep_potentials(sw))) return rc;
         call[c] = SweepCall{d.nfreq, d.s, d.kappa, nullptr, nullptr, nullptr, d.mur, d.inj_node,
@@ -2602,6 +2943,8 @@ int tags_cases_solve(tags_cases* cs, int precision)
         if (r || (r = L.sv.ensure((int)sw->N, mixed, sw->batch))) return r;
         L.prof.on = cs->profiling;
         slot_bind(sw, L);
+        // the case's tuned tile shapes and schedule, if any
+        TuningScope scope(sw->tuned[mixed ? 1 : 0] ? &sw->ktune[mixed ? 1 : 0] : tuning_current());
         const int got = sweep_issue(sw, L, next[c], call[c]);
         if (got < 0) return got;
         next[c] += (size_t)got;
@@ -2633,7 +2976,12 @@ int tags_cases_solve(tags_cases* cs, int precision)
         SweepSlot& L = *cs->lanes[done];
         const size_t c = (size_t)lane_case[done];
         bool complete = false;
-        if ((rc = sweep_advance(cs->sw[c], L, call[c], &complete))) break;
+        {
+            tags_sweep* sw = cs->sw[c];
+            TuningScope scope(sw->tuned[mixed ? 1 : 0] ? &sw->ktune[mixed ? 1 : 0]
+                                                       : tuning_current());
+            if ((rc = sweep_advance(sw, L, call[c], &complete))) break;
+        }
         if (!complete) continue;
         lane_case[done] = -1;
         rc = issue(done);

@@ -565,11 +565,17 @@ int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing)
     if (g.nb > 65535) return TAGS_ERR_ARG;
     if (g.brow && g.K > GEMM_BROW_LDS) return TAGS_ERR_ARG;
     const int p = prec_index<T>();
-    const int big = g_big_variant[p].load(std::memory_order_relaxed);
+    // the calling thread's tuned variants, else the process-wide ones
+    const Tuning* tu = tuning_current();
+    const int big = tu && tu->gemm_big[p] >= 0 ? tu->gemm_big[p]
+                                               : g_big_variant[p].load(std::memory_order_relaxed);
     const long tiles_big = (long)((g.M + GemmTable<T>::tile_m(big) - 1) / GemmTable<T>::tile_m(big)) *
                            ((g.N + GemmTable<T>::tile_n(big) - 1) / GemmTable<T>::tile_n(big)) *
                            (g.nb > 0 ? g.nb : 1);
-    const int v = tiles_big >= 512 ? big : g_small_variant[p].load(std::memory_order_relaxed);
+    const int small = tu && tu->gemm_small[p] >= 0
+                          ? tu->gemm_small[p]
+                          : g_small_variant[p].load(std::memory_order_relaxed);
+    const int v = tiles_big >= 512 ? big : small;
     return GemmTable<T>::launch(v, g, st, trailing);
 }
 

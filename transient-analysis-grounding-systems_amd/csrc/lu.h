@@ -192,6 +192,28 @@ void lu_set_update_chunks(int n);
 void lu_set_step_width(int w);
 int lu_step_width(bool c64);
 
+// Tile and schedule choices for the factorisations queued from one host
+// thread (a sweep's tuned values): while a TuningScope is alive on a thread,
+// its fields override the process-wide settings above and the GEMM variants
+// (gemm_set_variants) for the launches that thread queues; a field < 0 (0 for
+// step_width) leaves the process-wide value. Index [0] is complex128, [1]
+// complex64.
+struct Tuning {
+    int gemm_big[2] = {-1, -1};
+    int gemm_small[2] = {-1, -1};
+    int step_width[2] = {0, 0};
+    int update_chunks = -1;
+    int solve_rows = -1;  // lu_set_solve_rows
+};
+const Tuning* tuning_current();
+struct TuningScope {
+    const Tuning* prev;
+    explicit TuningScope(const Tuning* t);
+    ~TuningScope();
+    TuningScope(const TuningScope&) = delete;
+    TuningScope& operator=(const TuningScope&) = delete;
+};
+
 // Column phases of a factorisation (see lu.hip).
 struct LUPhase {
     int c_begin, c_end;   // columns factored in this phase
@@ -302,6 +324,37 @@ struct LUPlan {
 // Device work space of `batch` LUs of order n in precision T (the items of a
 // batched factorisation, each with its own row maps, inverses, vectors, flags
 // and tournament lists at the strides of bs()).
+// The 64-row diagonal blocks of a factorisation, as the flow solves address
+// them: the phases' column ranges [pb, pe) in order, steps of `width` columns
+// from each phase's start, sub-blocks of 64 from each step's start. Block q
+// (its global index, which is also the slot of its 64 x 64 inverses) starts
+// at row0 and is wq <= 64 rows wide. ok = 0 when the phases do not tile the
+// columns in order (the solves then take the per-step launches).
+constexpr int kTrsvMaxPhases = 8;
+struct TrsvGeom {
+    int ok = 0;
+    int np = 0, width = 256;
+    int pb[kTrsvMaxPhases] = {}, pe[kTrsvMaxPhases] = {};
+    int first[kTrsvMaxPhases + 1] = {};  // first block of each phase; first[np] = all
+};
+__host__ __device__ inline void trsv_block(const TrsvGeom& g, int q, int& row0, int& wq)
+{
+    int p = 0;
+    while (p + 1 < g.np && q >= g.first[p + 1]) p++;
+    const int l = q - g.first[p];
+    const int spb = g.width >> 6;
+    const int k = l / spb, t = l - k * spb;
+    const int j = g.pb[p] + g.width * k;
+    const int jb = g.pe[p] - j < g.width ? g.pe[p] - j : g.width;
+    row0 = j + 64 * t;
+    wq = jb - 64 * t < 64 ? jb - 64 * t : 64;
+}
+// Triangular solves: flow (one launch per range and direction, work-groups
+// of 64 rpb rows in dependency order; rpb 1 or 2) or the per-step launches
+// (0)
+void lu_set_solve_rows(int rpb);
+int lu_solve_rows();
+
 template <typename T>
 struct LUWork {
     int n = 0, nblk = 0, nblk_cap = 0;
@@ -334,6 +387,15 @@ struct LUWork {
     std::vector<int> h_init, h_orders;  // what d_init / d_orders hold
     double flops = 0;       // real flops of the last factorisation (GEMM + panel)
     int nodal_m = 0;        // the last factorisation's nodal block size (0: none)
+    // flow solves: the last factorisation's block geometry; per item a ticket
+    // counter, a spare word and one completion word per group of blocks (each
+    // launch tags its completions with its own epoch, so nothing is cleared
+    // between launches)
+    TrsvGeom geom;
+    unsigned* tctl = nullptr;
+    size_t cap_tctl = 0;
+    long tctl_stride = 0;
+    unsigned epoch = 0;
 
     int alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batch_ = 1);
     void release();

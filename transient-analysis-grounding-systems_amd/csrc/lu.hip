@@ -875,6 +875,236 @@ __global__ __launch_bounds__(256) void k_panel_gemv(const cx<T>* A, long ld, con
 }
 
 // ---------------------------------------------------------------------------
+// Flow triangular solve: the whole forward (unit L) or backward (UP: U) solve
+// over the diagonal blocks [b_lo, b_hi) of a factorisation in one launch.
+// Work-groups take groups of RB consecutive blocks (64 RB rows) in
+// dependency order from a ticket counter (the first block first forward, the
+// last first backward), so a group only waits on groups that hold earlier
+// tickets and are therefore running or done. A group
+//   1. streams the off-diagonal part of its rows, block column by block
+//      column in dependency order, from HBM: the loads of a chunk of KU block
+//      columns go out first; then it waits until the group that owns the
+//      chunk's latest block has published (the chain of the solve makes all
+//      earlier blocks final too) and reads their solved values;
+//   2. solves its own blocks in order with their 64 x 64 inverses, the
+//      second block (RB = 2) taking the first one's contribution; the
+//      inverse rows and the coupling entries sit in registers, loaded before
+//      the first wait, so that only the hand-off and register arithmetic
+//      lie on the chain;
+//   3. publishes its solved values (wave 0: agent-scope stores, then one
+//      release store of the launch's epoch into its completion word).
+// Waits spin on relaxed loads with a bounded count (an acquire fence once the
+// word reads done); past the bound the group flags *err, writes NaN and
+// publishes, so that every group ends. Sums run in a fixed order: the result
+// does not depend on timing. Wave v holds rows 16 (v % 4) .. + 15 of the
+// group's block v / 4, one block column per lane.
+// ---------------------------------------------------------------------------
+template <typename T>
+__device__ inline cx<T> flow_load(const cx<T>* p)
+{
+    return cx<T>{__hip_atomic_load(&p->x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT),
+                 __hip_atomic_load(&p->y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)};
+}
+template <typename T>
+__device__ inline void flow_store(cx<T>* p, cx<T> v)
+{
+    __hip_atomic_store(&p->x, v.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&p->y, v.y, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+constexpr long kFlowSpinLimit = 1L << 22;
+
+template <typename T, bool UP, int RB>
+__global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A, long ld, const int* perm,
+                                                        cx<T>* y, const cx<T>* inv, TrsvGeom g,
+                                                        int b_lo, int b_hi, unsigned* ctl,
+                                                        int* err, unsigned epoch, long sa,
+                                                        long sp, long sv, long sinv, long sctl,
+                                                        long sf)
+{
+    static_assert(RB == 1 || RB == 2, "groups of one or two blocks");
+    // block columns per chunk of loads (a lane holds 16 KU values of A)
+    constexpr int KU = sizeof(T) == 4 ? 2 : 1;
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    perm += z_ * sp;
+    y += z_ * sv;
+    inv += z_ * sinv;
+    ctl += z_ * sctl;
+    err += z_ * sf;
+    unsigned* done = ctl + 2;
+    __shared__ cx<T> acc[64 * RB];
+    __shared__ cx<T> xs[64 * RB];
+    __shared__ int s_ticket, s_fail;
+    const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
+    const int nbk = b_hi - b_lo;
+    const int ngrp = (nbk + RB - 1) / RB;
+    if (tid == 0) {
+        const unsigned t =
+            __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        // the last ticket of the launch: every other work-group has drawn
+        if ((int)t == ngrp - 1)
+            __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        s_ticket = (int)t;
+        s_fail = 0;
+    }
+    __syncthreads();
+    const int gi = s_ticket;
+    if (gi >= ngrp) return;  // not reached: the grid has ngrp work-groups per item
+    // this group's blocks [q0, q1) and rows [r_lo, r_hi)
+    int q0, q1;
+    if (!UP) {
+        q0 = b_lo + RB * gi;
+        q1 = min(q0 + RB, b_hi);
+    } else {
+        q1 = b_hi - RB * gi;
+        q0 = max(q1 - RB, b_lo);
+    }
+    const int nsb = q1 - q0;
+    int r_lo, w0, r_last, w_last;
+    trsv_block(g, q0, r_lo, w0);
+    trsv_block(g, q1 - 1, r_last, w_last);
+    const int r_hi = r_last + w_last;
+    // this wave's rows
+    const int sb = wv >> 2;  // block of the group
+    int sb_row0 = 0, sb_w = 0;
+    if (sb < nsb) trsv_block(g, q0 + sb, sb_row0, sb_w);
+    const int rb0 = 16 * (wv & 3);
+    const cx<T>* rowp[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const bool ok = sb < nsb && rb0 + i < sb_w;
+        rowp[i] = A + (long)perm[ok ? sb_row0 + rb0 + i : r_lo] * ld;
+    }
+    // what the group's own solve needs from memory, loaded before any wait:
+    // the rows of this wave's block inverse, and (RB = 2) this wave's entries
+    // in the columns of the group's other block when that block comes first
+    cx<T> ivr[16], lw[16];
+    const int ob = UP ? nsb - 1 : 0;  // the group's first block in dependency order
+    int ob_r0 = 0, ob_w = 0;
+    if (nsb > 1) trsv_block(g, q0 + ob, ob_r0, ob_w);
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        ivr[i] = sb < nsb ? inv[(size_t)(q0 + sb) * 4096 + (rb0 + i) * 64 + lane] : cx<T>{0, 0};
+        const bool okc = sb < nsb && sb != ob && lane < ob_w;
+        lw[i] = okc ? rowp[i][ob_r0 + lane] : cx<T>{0, 0};
+    }
+    cx<T> part[16];
+#pragma unroll
+    for (int i = 0; i < 16; i++) part[i] = cx<T>{0, 0};
+    // 1. off-diagonal block columns: forward b_lo .. q0 - 1, backward q1 .. b_hi - 1
+    const int u_first = UP ? b_hi - 1 : b_lo;
+    const int nu = UP ? b_hi - q1 : q0 - b_lo;
+    int known = -1;  // highest group index seen published
+    for (int c0 = 0; c0 < nu; c0 += KU) {
+        cx<T> a[KU][16];
+        int ucol[KU], uw[KU];
+#pragma unroll
+        for (int c = 0; c < KU; c++) {
+            const int u = UP ? u_first - (c0 + c) : u_first + (c0 + c);
+            const bool in = c0 + c < nu;
+            int r0u = 0, wu = 0;
+            if (in) trsv_block(g, u, r0u, wu);
+            ucol[c] = r0u;
+            uw[c] = wu;
+            const bool okc = lane < wu;
+#pragma unroll
+            for (int i = 0; i < 16; i++) a[c][i] = rowp[i][r0u + (okc ? lane : 0)];
+        }
+        // wait for the group owning the chunk's latest block
+        const int last = min(c0 + KU, nu) - 1;
+        const int need = last / RB;
+        if (need > known) {
+            long spins = 0;
+            while (__hip_atomic_load(&done[need], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                   epoch) {
+                __builtin_amdgcn_s_sleep(1);
+                if (++spins > kFlowSpinLimit) {
+                    if (lane == 0) {
+                        atomicExch(err, 1);
+                        s_fail = 1;
+                    }
+                    break;
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+            known = need;
+        }
+#pragma unroll
+        for (int c = 0; c < KU; c++) {
+            const bool okc = c0 + c < nu && lane < uw[c];
+            const cx<T> xv = flow_load(y + ucol[c] + (okc ? lane : 0));
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const cx<T> p = cmul(a[c][i], xv);
+                part[i] = csel(okc, cadd(part[i], p), part[i]);
+            }
+        }
+    }
+    // right-hand side less the off-diagonal sums, into LDS
+#pragma unroll
+    for (int i = 0; i < 16; i++) {
+        const cx<T> s = wave_sum_cx<T>(part[i]);
+        const int r = rb0 + i;
+        if (lane == 0 && sb < nsb)
+            acc[64 * sb + r] = r < sb_w ? csub(y[sb_row0 + r], s) : cx<T>{0, 0};
+    }
+    // 2. the group's own blocks, in dependency order (RB <= 2: the later
+    // block takes the first one's contribution)
+    for (int ss = 0; ss < nsb; ss++) {
+        const int tb = UP ? nsb - 1 - ss : ss;
+        __syncthreads();
+        // x_tb = inv_tb acc_tb (identity padding past the block's width), by
+        // the block's own waves
+        if (sb == tb) {
+            cx<T> pr[16];
+#pragma unroll
+            for (int i = 0; i < 16; i++) pr[i] = cmul(ivr[i], acc[64 * tb + lane]);
+#pragma unroll
+            for (int i = 0; i < 16; i++) {
+                const cx<T> s = wave_sum_cx<T>(pr[i]);
+                if (lane == 0) xs[64 * tb + rb0 + i] = s;
+            }
+        }
+        if (ss + 1 < nsb) {
+            __syncthreads();
+            if (sb < nsb && sb != ob) {
+                const bool okc = lane < ob_w;
+                const cx<T> xv = xs[64 * ob + (okc ? lane : 0)];
+                cx<T> pr[16];
+#pragma unroll
+                for (int i = 0; i < 16; i++) pr[i] = csel(okc, cmul(lw[i], xv), cx<T>{0, 0});
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const cx<T> s = wave_sum_cx<T>(pr[i]);
+                    if (lane == 0) acc[64 * sb + rb0 + i] = csub(acc[64 * sb + rb0 + i], s);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // 3. publish
+    if (wv == 0) {
+        const bool bad = s_fail != 0;
+        for (int r = lane; r < r_hi - r_lo; r += 64) {
+            // rows of the group are contiguous: find the block of row r_lo + r
+            int rr = r, tb = 0, tw = w0;
+            while (tb + 1 < nsb && rr >= tw) {
+                rr -= tw;
+                tb++;
+                int t0;
+                trsv_block(g, q0 + tb, t0, tw);
+            }
+            cx<T> v = xs[64 * tb + rr];
+            if (bad) v = cx<T>{(T)NAN, (T)NAN};
+            flow_store(y + r_lo + r, v);
+        }
+        if (lane == 0)
+            __hip_atomic_store(&done[gi], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // FP64 residual r = b - W x (W row-major n x n) and |r|_inf, |x|_inf, |W|_inf
 // reductions (non-negative doubles compare as unsigned 64-bit integers).
 // ---------------------------------------------------------------------------
@@ -1307,6 +1537,10 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
     const size_t need_perm = (size_t)n_ * B, need_inv = 4096 * (size_t)nblk_ * B;
     const size_t need_cand = (size_t)tr.cand_total * B;
     const size_t need_cnt = (size_t)std::max(1, tr.cnt_total) * B;
+    // flow-solve control words per item: ticket, spare, one per block group,
+    // padded to 64 words so that items do not share cache lines
+    const long tstride = ((long)nblk_ + 2 + 63) / 64 * 64;
+    const size_t need_tctl = (size_t)tstride * B;
     if (!s2) {
         TAGS_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         TAGS_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
@@ -1316,7 +1550,7 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         TAGS_HIP(hipMemset(bar, 0, sizeof(unsigned) * 64));
     }
     if (need_perm > cap_perm || need_inv > cap_inv || B > cap_items || need_cand > cap_cand ||
-        need_cnt > cap_cnt) {
+        need_cnt > cap_cnt || need_tctl > cap_tctl) {
         (void)hipStreamSynchronize(s2);
         (void)hipFree(perm);
         (void)hipFree(perm_alt);
@@ -1328,10 +1562,11 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         (void)hipFree(d_orders);
         (void)hipFree(tcand);
         (void)hipFree(tcnt);
+        (void)hipFree(tctl);
         perm = perm_alt = flags = d_init = d_orders = tcand = nullptr;
         invL = invU = vec = nullptr;
-        tcnt = nullptr;
-        cap_perm = cap_inv = cap_items = cap_cand = cap_cnt = 0;
+        tcnt = tctl = nullptr;
+        cap_perm = cap_inv = cap_items = cap_cand = cap_cnt = cap_tctl = 0;
         TAGS_HIP(hipMalloc(&perm, sizeof(int) * need_perm));
         TAGS_HIP(hipMalloc(&perm_alt, sizeof(int) * need_perm));
         TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * need_inv));
@@ -1346,11 +1581,24 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         // arrival counters start at zero; every tournament's root resets the
         // ones it used, so they stay zero between launches whatever the order
         TAGS_HIP(hipMemset(tcnt, 0, sizeof(unsigned) * need_cnt));
+        // tickets start at zero (each launch's last ticket resets its counter);
+        // completion words start below every epoch a launch uses
+        TAGS_HIP(hipMalloc(&tctl, sizeof(unsigned) * need_tctl));
+        TAGS_HIP(hipMemset(tctl, 0, sizeof(unsigned) * need_tctl));
+        epoch = 0;
         cap_perm = need_perm;
         cap_inv = need_inv;
         cap_items = B;
         cap_cand = need_cand;
         cap_cnt = need_cnt;
+        cap_tctl = need_tctl;
+        tctl_stride = tstride;
+    }
+    if (tstride != tctl_stride) {
+        // the items' control words move: their tickets must read zero again
+        TAGS_HIP(hipMemset(tctl, 0, sizeof(unsigned) * need_tctl));
+        epoch = 0;
+        tctl_stride = tstride;
     }
     n = n_;
     batch = batch_;
@@ -1388,8 +1636,11 @@ void LUWork<T>::release()
     (void)hipFree(d_orders);
     (void)hipFree(tcand);
     (void)hipFree(tcnt);
+    (void)hipFree(tctl);
     tcand = nullptr;
     tcnt = nullptr;
+    tctl = nullptr;
+    cap_tctl = 0;
     d_init = nullptr;
     d_orders = nullptr;
     h_init.clear();
@@ -1463,6 +1714,26 @@ void lu_set_step_width(int w)
     g_step_width[1].store(w <= 0 ? 256 : w <= 128 ? 128 : 256);
 }
 int lu_step_width(bool c64) { return g_step_width[c64 ? 1 : 0].load(); }
+
+static thread_local const Tuning* t_tuning = nullptr;
+const Tuning* tuning_current() { return t_tuning; }
+TuningScope::TuningScope(const Tuning* t) : prev(t_tuning) { t_tuning = t; }
+TuningScope::~TuningScope() { t_tuning = prev; }
+
+// the step width and update launches of the calling thread's factorisations
+static int step_width_now(bool c64)
+{
+    const Tuning* t = t_tuning;
+    const int p = c64 ? 1 : 0;
+    if (t && t->step_width[p] > 0) return t->step_width[p] <= 128 ? 128 : 256;
+    return g_step_width[p].load(std::memory_order_relaxed);
+}
+static int update_chunks_now()
+{
+    const Tuning* t = t_tuning;
+    if (t && t->update_chunks > 0) return t->update_chunks;
+    return std::max(1, g_update_chunks.load(std::memory_order_relaxed));
+}
 
 template <typename T>
 static int panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld, int* pm,
@@ -1801,7 +2072,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     std::vector<LUStep> steps;
     w.blk.clear();
     w.flops = 0;
-    const int width = g_step_width[sizeof(T) == 4 ? 1 : 0].load(std::memory_order_relaxed);
+    const int width = step_width_now(sizeof(T) == 4);
     for (int p = 0; p < (int)plan->phases.size(); p++) {
         const LUPhase& ph = plan->phases[p];
         for (int j = ph.c_begin; j < ph.c_end; j += width) {
@@ -1897,11 +2168,11 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
             }
             if ((rc = step_panels<T>(w, pr, w.s2, A, ld, pnx, *nph, nx))) return rc;
             TAGS_HIP(hipEventRecord(w.ev_panel, w.s2));
-            // the remaining columns of this step, in g_update_chunks launches
+            // the remaining columns of this step, in update_chunks_now() launches
             // (each launch boundary lets waiting panel work-groups start: the
             // tournament's need a whole CU), at most 2 when the next panel has
             // no row exchanges (one work-group of 512 threads)
-            int nch = std::max(1, g_update_chunks.load(std::memory_order_relaxed));
+            int nch = update_chunks_now();
             if (!nph->pivot) nch = std::min(nch, 2);
             for (int q = 0; q < 2; q++) {
                 if (re[q] <= rb[q]) continue;
@@ -1934,7 +2205,97 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
         w.step_w.push_back(sp.jb);
         w.step_slot.push_back(sp.slot);
     }
+    // the flow solves' view of the blocks: usable when the phases tile the
+    // columns in order and reproduce the block list exactly
+    TrsvGeom& g = w.geom;
+    g = TrsvGeom();
+    const int np = (int)plan->phases.size();
+    if (np >= 1 && np <= kTrsvMaxPhases && width % 64 == 0) {
+        g.np = np;
+        g.width = width;
+        int nbk = 0;
+        bool ok = true;
+        for (int p = 0; p < np; p++) {
+            const LUPhase& ph = plan->phases[p];
+            g.pb[p] = ph.c_begin;
+            g.pe[p] = ph.c_end;
+            g.first[p] = nbk;
+            const int len = ph.c_end - ph.c_begin;
+            if (len < 0 || (p > 0 && ph.c_begin != plan->phases[p - 1].c_end)) ok = false;
+            nbk += len / width * (width / 64) + (len % width + 63) / 64;
+        }
+        g.first[np] = nbk;
+        ok = ok && nbk == (int)w.blk.size();
+        for (int q = 0; ok && q < nbk; q++) {
+            int r0, wq;
+            trsv_block(g, q, r0, wq);
+            ok = r0 == w.blk[q] && wq >= 1 && wq <= 64;
+        }
+        g.ok = ok ? 1 : 0;
+    }
     return 0;
+}
+
+static std::atomic<int> g_solve_rows{2};
+void lu_set_solve_rows(int rpb) { g_solve_rows.store(rpb == 1 || rpb == 2 ? rpb : 0); }
+int lu_solve_rows() { return g_solve_rows.load(); }
+
+template <typename T, bool UP, int RB>
+static void flow_launch(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
+                        int b_lo, int b_hi, unsigned epoch)
+{
+    const BatchStrides b = w.bs();
+    const int ngrp = (b_hi - b_lo + RB - 1) / RB;
+    hipLaunchKernelGGL((k_trsv_flow<T, UP, RB>), dim3(ngrp, 1, b.nb), dim3(256 * RB), 0, st, A,
+                       ld, (const int*)w.perm, y, (const cx<T>*)(UP ? w.invU : w.invL), w.geom,
+                       b_lo, b_hi, w.tctl, w.flags + 1, epoch, b.a, b.p, b.v, b.inv,
+                       w.tctl_stride, b.f);
+}
+
+// Flow solves over the blocks of the rows [c_lo, c_hi), forward then
+// backward; false when the factorisation's blocks do not allow them.
+template <typename T>
+static bool flow_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
+                       int c_lo, int c_hi)
+{
+    const Tuning* tu = tuning_current();
+    int rb = tu && tu->solve_rows >= 0 ? tu->solve_rows
+                                       : g_solve_rows.load(std::memory_order_relaxed);
+    // complex128 groups take one block (two would not fit the registers)
+    if (sizeof(T) == 8 && rb > 1) rb = 1;
+    if (!rb || !w.geom.ok || !w.tctl) return false;
+    const TrsvGeom& g = w.geom;
+    const int nbk = g.first[g.np];
+    int b_lo = nbk, b_hi = nbk;
+    for (int q = 0; q < nbk; q++) {
+        int r0, wq;
+        trsv_block(g, q, r0, wq);
+        if (b_lo == nbk && r0 >= c_lo) b_lo = q;
+        if (r0 >= c_hi) {
+            b_hi = q;
+            break;
+        }
+    }
+    if (b_lo >= b_hi) return true;
+    // the range must start and end on block boundaries
+    int r0, wq;
+    trsv_block(g, b_lo, r0, wq);
+    if (r0 != c_lo) return false;
+    trsv_block(g, b_hi - 1, r0, wq);
+    if (r0 + wq != c_hi) return false;
+    for (int dir = 0; dir < 2; dir++) {
+        if (++w.epoch == 0) ++w.epoch;
+        const unsigned ep = w.epoch;
+        constexpr int R2 = sizeof(T) == 8 ? 1 : 2;
+        if (dir == 0) {
+            if (rb == 1) flow_launch<T, false, 1>(w, A, ld, y, st, b_lo, b_hi, ep);
+            else flow_launch<T, false, R2>(w, A, ld, y, st, b_lo, b_hi, ep);
+        } else {
+            if (rb == 1) flow_launch<T, true, 1>(w, A, ld, y, st, b_lo, b_hi, ep);
+            else flow_launch<T, true, R2>(w, A, ld, y, st, b_lo, b_hi, ep);
+        }
+    }
+    return true;
 }
 
 // Solves (L U) x = P b in place on `y` (logical-order input already gathered:
@@ -1947,6 +2308,7 @@ template <typename T>
 static void range_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
                         int c_lo, int c_hi)
 {
+    if (flow_solve<T>(w, A, ld, y, st, c_lo, c_hi)) return;
     const BatchStrides b = w.bs();
     const int ns = (int)w.step_j.size();
     for (int k = 0; k < ns; k++) {
