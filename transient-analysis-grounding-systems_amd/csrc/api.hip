@@ -1672,6 +1672,13 @@ struct SweepSlot {
     std::vector<double> hz; // their frequencies |s| / 2 pi
     bool nodal = false;     // their complex64 factor is the nodal plan's
     size_t w_cap = 0, rhs_cap = 0;  // allocated elements of W and rhs
+    // items [0, static_items) of W hold the frequency-independent part (zero
+    // and incidence blocks) of static_sw's layout static_kind: the mixed form
+    // factors a complex64 copy and leaves W as assembled, so the next group
+    // of the same sweep writes only the impedance blocks
+    const tags_sweep* static_sw = nullptr;
+    int static_kind = -1;
+    int static_items = 0;
     int init(long N, int batch)
     {
         if (!stream.s) TAGS_HIP(hipStreamCreate(&stream.s));
@@ -1687,6 +1694,7 @@ struct SweepSlot {
             hsc = nullptr;
             cap = 0;
             w_cap = rhs_cap = 0;
+            static_items = 0;
             const int items = std::max(batch, cap);
             if ((rc = W.alloc(nw)) || (rc = rhs.alloc(nr)) || (rc = dsc.alloc((size_t)items)))
                 return rc;
@@ -1703,6 +1711,60 @@ struct SweepSlot {
     {
         if (stream.s) hipStreamSynchronize(stream.s);
         if (hsc) hipHostFree(hsc);
+    }
+};
+
+// Device copies of one plan's row orders, owned by the sweep that owns the
+// plan: uploaded (a blocking copy) only when the plan's content changes, at
+// layout time, before any of the sweep's work is queued; factorisations then
+// read them in place. Without them a work space that serves many plans in
+// turn (a lane of a multi-case solve) would upload, with a blocking copy
+// that waits for every stream, at each change of case.
+struct PlanDev {
+    int* init = nullptr;
+    int* orders = nullptr;
+    size_t init_cap = 0, orders_cap = 0;
+    std::vector<int> h_init, h_orders;
+    int attach(LUPlan& p)
+    {
+        p.dev_init = nullptr;
+        p.dev_orders = nullptr;
+        if (!p.init_perm.empty()) {
+            if (p.init_perm != h_init) {
+                if (p.init_perm.size() > init_cap) {
+                    hipFree(init);
+                    init = nullptr;
+                    init_cap = 0;
+                    TAGS_HIP(hipMalloc(&init, sizeof(int) * p.init_perm.size()));
+                    init_cap = p.init_perm.size();
+                }
+                TAGS_HIP(hipMemcpy(init, p.init_perm.data(), sizeof(int) * p.init_perm.size(),
+                                   hipMemcpyHostToDevice));
+                h_init = p.init_perm;
+            }
+            p.dev_init = init;
+        }
+        if (!p.orders.empty()) {
+            if (p.orders != h_orders) {
+                if (p.orders.size() > orders_cap) {
+                    hipFree(orders);
+                    orders = nullptr;
+                    orders_cap = 0;
+                    TAGS_HIP(hipMalloc(&orders, sizeof(int) * p.orders.size()));
+                    orders_cap = p.orders.size();
+                }
+                TAGS_HIP(hipMemcpy(orders, p.orders.data(), sizeof(int) * p.orders.size(),
+                                   hipMemcpyHostToDevice));
+                h_orders = p.orders;
+            }
+            p.dev_orders = orders;
+        }
+        return 0;
+    }
+    ~PlanDev()
+    {
+        hipFree(init);
+        hipFree(orders);
     }
 };
 
@@ -1725,6 +1787,8 @@ struct tags_sweep {
     // back to FP64 or needs more than kHybridMaxIters refinement steps raises
     // the switch to 1.5 times its frequency for the rest of the sweep
     tags::LUPlan plan_nodal;
+    // device copies of plan, plan32 and plan_nodal's row orders
+    PlanDev plan_dev[3];
     double nodal_above = 0.0;
     // nodal complex64 factors below gmres_below Hz refine by FGMRES
     // (gmres_steps Arnoldi steps per refinement step), above it classically
@@ -1863,6 +1927,10 @@ static int sweep_layout(tags_sweep* sw)
         sw->plan = LUPlan::zl_first((int)m, (int)n, false);
         break;
     }
+    int rc;
+    if ((rc = sw->plan_dev[0].attach(sw->plan)) || (rc = sw->plan_dev[1].attach(sw->plan32)) ||
+        (rc = sw->plan_dev[2].attach(sw->plan_nodal)))
+        return rc;
     sw->bind64 = p64;
     sw->bind32 = p32;
     sw->bindgr = gr;
@@ -1912,13 +1980,15 @@ static PairScalars sample_scalars(TAGS_COMPLEX s_, TAGS_COMPLEX kappa_, double m
 // Assembly of nb samples' W (item z at W + z N^2) in the sweep's block layout:
 // one sample from `sc`, several from their scalars `dsc` in device memory.
 static int sweep_fill_items(tags_sweep* sw, c64* W, hipStream_t st, Profiler* prof, int nb,
-                            const PairScalars& sc, const PairScalars* dsc)
+                            const PairScalars& sc, const PairScalars* dsc,
+                            bool fill_static = true)
 {
     const long N = sw->N, sW = N * N;
     Timed tm(prof, FAM_FILL, st);
     int rc;
     const PairScalars* scs = nb > 1 ? dsc : nullptr;
-    if ((rc = launch_fill_static(W, N, sw->layout, sw->sn, sw->en, st, nb, sW))) return rc;
+    if (fill_static && (rc = launch_fill_static(W, N, sw->layout, sw->sn, sw->en, st, nb, sW)))
+        return rc;
     if (sw->formulation == TAGS_FORM_MHEM) {
         if (!sw->pot) return TAGS_ERR_ARG;
         const size_t mm = (size_t)sw->m * sw->m;
@@ -2052,7 +2122,8 @@ int tags_sweep_assemble(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kappa, doub
 {
     if (!sw || !wg) return TAGS_ERR_ARG;
     const long N = sw->N;
-    sweep_layout(sw);
+    int lrc = sweep_layout(sw);
+    if (lrc) return lrc;
     DevBuf<c64> W;
     int rc = sweep_potentials(sw);
     if (rc || (rc = W.alloc((size_t)N * N))) return rc;
@@ -2093,7 +2164,8 @@ int tags_sweep_assemble_entries(tags_sweep* sw, TAGS_COMPLEX s, TAGS_COMPLEX kap
     if (!sw || (count && (!rows || !cols || !out))) return TAGS_ERR_ARG;
     const long N = sw->N;
     const BlockLayout& L = sw->layout;
-    sweep_layout(sw);
+    int lrc = sweep_layout(sw);
+    if (lrc) return lrc;
     // unknown index in the reference order [U; I_L; I_T] -> storage index
     auto pos = [&](long u) {
         if (u < sw->n) return L.node_off + u;
@@ -2286,7 +2358,18 @@ static int sweep_issue(tags_sweep* sw, SweepSlot& sl, size_t f, const SweepCall&
     if (cnt > 1)
         TAGS_HIP(hipMemcpyAsync(sl.dsc.p, sl.hsc, sizeof(PairScalars) * cnt,
                                 hipMemcpyHostToDevice, st));
-    if ((rc = sweep_fill_items(sw, sl.W.p, st, &sl.prof, cnt, sc0, sl.dsc.p))) return rc;
+    // the zero and incidence blocks stay from the last mixed group of this
+    // sweep on this slot unless a factor in FP64 replaced W since
+    const bool keep = c.mixed && sl.static_sw == sw && sl.static_kind == sw->layout_kind &&
+                      sl.static_items >= cnt;
+    if ((rc = sweep_fill_items(sw, sl.W.p, st, &sl.prof, cnt, sc0, sl.dsc.p, !keep))) return rc;
+    if (!c.mixed) {
+        sl.static_items = 0;  // the FP64 factor replaces W
+    } else if (!keep) {
+        sl.static_sw = sw;
+        sl.static_kind = sw->layout_kind;
+        sl.static_items = cnt;
+    }
     // b = unit current at the injection node
     hipLaunchKernelGGL(k_unit_rhs, dim3((unsigned)((N + 255) / 256), 1, (unsigned)cnt), dim3(256),
                        0, st, sl.rhs.p, N, L.node_off + (long)c.inj_node);
@@ -2355,6 +2438,8 @@ static int sweep_advance(tags_sweep* sw, SweepSlot& sl, const SweepCall& c, bool
     for (int z = 0; z < cnt; z++) {
         const size_t f = f0 + z;
         if (c.stats) c.stats[f] = stt[z];
+        // a fallback factored this sample's W in FP64
+        if (c.mixed && stt[z].precision_used != TAGS_PREC_MIXED) sl.static_items = 0;
         if (sl.nodal &&
             (stt[z].precision_used != TAGS_PREC_MIXED || stt[z].refine_iters > kHybridMaxIters))
             sw->nodal_above = std::max(sw->nodal_above, 1.5 * sl.hz[z]);
@@ -2405,7 +2490,7 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     int rc = sweep_slots(sw, k, mixed);
     if (rc || (rc = sweep_potentials(sw))) return rc;
     // after the slots exist: the layout sets every slot's plans
-    sweep_layout(sw);
+    if ((rc = sweep_layout(sw))) return rc;
     const SweepCall c{nfreq, s, kappa, gamma, ref_l, ref_t, mur, inj_node, mixed, u_inj, x_out,
                       stats};
     // the hybrid plan's switch may rise during this call; it is restored at
@@ -2907,7 +2992,7 @@ int tags_cases_solve(tags_cases* cs, int precision)
         tags_sweep* sw = cs->sw[c];
         const tags_case& d = cs->cs[c];
         sw->plan_kind = cs->plan_kind;
-        sweep_layout(sw);
+        if ((rc = sweep_layout(sw))) return rc;
         const tags_tuning* tu = sw->tuned[mixed ? 1 : 0] ? &sw->tune[mixed ? 1 : 0] : nullptr;
         const int batch_req0 = sw->batch_req;
         if (tu && tu->batch >= 0) sw->batch_req = tu->batch;

@@ -228,6 +228,12 @@ struct LUPhase {
 struct LUPlan {
     std::vector<LUPhase> phases;
     std::vector<int> init_perm;  // logical -> physical row order at start (empty: identity)
+    // device copies of init_perm and orders made by the plan's owner (not
+    // owned here); when set, a factorisation reads them instead of uploading
+    // its own copies, which takes a blocking host copy whenever the plan
+    // differs from the work space's last one
+    const int* dev_init = nullptr;
+    const int* dev_orders = nullptr;
     static LUPlan dense(int n)
     {
         LUPlan p;

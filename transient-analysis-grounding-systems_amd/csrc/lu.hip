@@ -878,9 +878,10 @@ __global__ __launch_bounds__(256) void k_panel_gemv(const cx<T>* A, long ld, con
 // Flow triangular solve: the whole forward (unit L) or backward (UP: U) solve
 // over the diagonal blocks [b_lo, b_hi) of a factorisation in one launch.
 // Work-groups take groups of RB consecutive blocks (64 RB rows) in
-// dependency order from a ticket counter (the first block first forward, the
-// last first backward), so a group only waits on groups that hold earlier
-// tickets and are therefore running or done. A group
+// dependency order from one ticket counter per launch (the first block first
+// forward, the last first backward; group-major over the items of a batched
+// launch), so a group only waits on groups that hold earlier tickets and are
+// therefore running or done. A group
 //   1. streams the off-diagonal part of its rows, block column by block
 //      column in dependency order, from HBM: the loads of a chunk of KU block
 //      columns go out first; then it waits until the group that owns the
@@ -925,32 +926,35 @@ __global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A, long ld,
     static_assert(RB == 1 || RB == 2, "groups of one or two blocks");
     // block columns per chunk of loads (a lane holds 16 KU values of A)
     constexpr int KU = sizeof(T) == 4 ? 2 : 1;
-    const long z_ = blockIdx.z;
-    A += z_ * sa;
-    perm += z_ * sp;
-    y += z_ * sv;
-    inv += z_ * sinv;
-    ctl += z_ * sctl;
-    err += z_ * sf;
-    unsigned* done = ctl + 2;
     __shared__ cx<T> acc[64 * RB];
     __shared__ cx<T> xs[64 * RB];
     __shared__ int s_ticket, s_fail;
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int nbk = b_hi - b_lo;
     const int ngrp = (nbk + RB - 1) / RB;
+    const int nitems = (int)gridDim.x / ngrp;
+    // one ticket counter for the launch (item 0's): tickets run group-major
+    // over the items, so the groups that a work-group waits for were handed
+    // out nitems tickets or more before it
     if (tid == 0) {
         const unsigned t =
             __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // the last ticket of the launch: every other work-group has drawn
-        if ((int)t == ngrp - 1)
+        if ((int)t == ngrp * nitems - 1)
             __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         s_ticket = (int)t;
         s_fail = 0;
     }
     __syncthreads();
-    const int gi = s_ticket;
-    if (gi >= ngrp) return;  // not reached: the grid has ngrp work-groups per item
+    if (s_ticket >= ngrp * nitems) return;  // not reached: the grid has ngrp x nitems
+    const int gi = s_ticket / nitems;
+    const long z_ = s_ticket - (long)gi * nitems;
+    A += z_ * sa;
+    perm += z_ * sp;
+    y += z_ * sv;
+    inv += z_ * sinv;
+    err += z_ * sf;
+    unsigned* done = ctl + z_ * sctl + 2;
     // this group's blocks [q0, q1) and rows [r_lo, r_hi)
     int q0, q1;
     if (!UP) {
@@ -1312,16 +1316,30 @@ __global__ __launch_bounds__(256) void k_to_c32_scaled(const c64* W, long ld, in
                                                         const double* d, c32* W32, long ld32,
                                                         long sw, long sw32)
 {
+    // four loads per thread go out before any store (HBM-bound: bytes in flight)
+    constexpr int U = 4;
     const long z_ = blockIdx.z;
     W += z_ * sw;
     d += z_ * n;
     W32 += z_ * sw32;
     const int row = blockIdx.y;
     const double di = d[row];
-    for (int c = blockIdx.x * 256 + threadIdx.x; c < n; c += gridDim.x * 256) {
-        const c64 a = W[(long)row * ld + c];
-        const double s = di * d[c];
-        W32[(long)row * ld32 + c] = c32{(float)(a.x * s), (float)(a.y * s)};
+    const c64* src = W + (long)row * ld;
+    c32* dst = W32 + (long)row * ld32;
+    for (int c0 = blockIdx.x * 256 * U + threadIdx.x; c0 < n; c0 += gridDim.x * 256 * U) {
+        c64 a[U];
+        double sc[U];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c0 + 256 * u;
+            a[u] = c < n ? src[c] : c64{0.0, 0.0};
+            sc[u] = c < n ? di * d[c] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int c = c0 + 256 * u;
+            if (c < n) dst[c] = c32{(float)(a[u].x * sc[u]), (float)(a[u].y * sc[u])};
+        }
     }
 }
 
@@ -2029,7 +2047,11 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
     // stream: the cooperative panel launches are ordered after kernels, and the
     // initial order is uploaded once with a blocking copy when it changes.
     const int* init = nullptr;
-    if (!plan->init_perm.empty()) {
+    const int* orders = w.d_orders;
+    if (!plan->init_perm.empty() && plan->dev_init) {
+        if ((int)plan->init_perm.size() != n) return TAGS_ERR_ARG;
+        init = plan->dev_init;
+    } else if (!plan->init_perm.empty()) {
         if ((int)plan->init_perm.size() != n) return TAGS_ERR_ARG;
         if (w.h_init != plan->init_perm) {
             TAGS_HIP(hipStreamSynchronize(st));
@@ -2039,7 +2061,10 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
         }
         init = w.d_init;
     }
-    if (!plan->orders.empty() && w.h_orders != plan->orders) {
+    if (!plan->orders.empty() && plan->dev_orders) {
+        if ((int)plan->orders.size() > n) return TAGS_ERR_ARG;
+        orders = plan->dev_orders;
+    } else if (!plan->orders.empty() && w.h_orders != plan->orders) {
         if ((int)plan->orders.size() > n) return TAGS_ERR_ARG;
         TAGS_HIP(hipStreamSynchronize(st));
         TAGS_HIP(hipMemcpy(w.d_orders, plan->orders.data(), sizeof(int) * plan->orders.size(),
@@ -2064,7 +2089,7 @@ int lu_factor(LUWork<T>& w, cx<T>* A, long ld, hipStream_t st, Profiler* pr, con
         if (ph.order_off < 0 || sp.j != ph.c_begin) return 0;
         const int len = n - ph.c_begin;
         hipLaunchKernelGGL(k_set_order, dim3((len + 255) / 256, 1, w.nb), dim3(256), 0, sq, pm,
-                           ph.c_begin, len, w.d_orders + ph.order_off, (long)n);
+                           ph.c_begin, len, orders + ph.order_off, (long)n);
         TAGS_HIP(hipGetLastError());
         return 0;
     };
@@ -2246,7 +2271,7 @@ static void flow_launch(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStre
 {
     const BatchStrides b = w.bs();
     const int ngrp = (b_hi - b_lo + RB - 1) / RB;
-    hipLaunchKernelGGL((k_trsv_flow<T, UP, RB>), dim3(ngrp, 1, b.nb), dim3(256 * RB), 0, st, A,
+    hipLaunchKernelGGL((k_trsv_flow<T, UP, RB>), dim3(ngrp * b.nb), dim3(256 * RB), 0, st, A,
                        ld, (const int*)w.perm, y, (const cx<T>*)(UP ? w.invU : w.invL), w.geom,
                        b_lo, b_hi, w.tctl, w.flags + 1, epoch, b.a, b.p, b.v, b.inv,
                        w.tctl_stride, b.f);
@@ -2440,7 +2465,7 @@ int row_scale(const c64* W, long ld, int n, double* d, bool enabled, hipStream_t
 int to_c32_scaled(const c64* W, long ld, int n, const double* d, c32* W32, long ld32,
                   hipStream_t st, const VecBatch& vb)
 {
-    unsigned gx = (unsigned)std::min(16, (n + 255) / 256);
+    unsigned gx = (unsigned)std::min(8, (n + 1023) / 1024);
     hipLaunchKernelGGL(k_to_c32_scaled, dim3(gx, n, vb.nb), dim3(256), 0, st, W, ld, n, d, W32,
                        ld32, vb.sw, vb.sw32);
     TAGS_HIP(hipGetLastError());
