@@ -192,6 +192,7 @@ struct Solver {
     // kept while large enough, so that one solver serves systems of different
     // orders in turn
     size_t vec_cap = 0, w32_cap = 0;
+    size_t w32_hi = 0;  // largest complex64 copy asked for so far
     int item_cap = 0;
     int ensure(int n_, bool mixed, int batch = 1)
     {
@@ -200,9 +201,15 @@ struct Solver {
         if (batch < 1 || n_ < 1) return TAGS_ERR_ARG;
         const size_t need = (size_t)n_ * batch;
         if (need > vec_cap || batch > item_cap) {
+            // grow to cover every earlier request as well: a solver that
+            // serves systems of different orders in turn (a lane of a
+            // multi-case solve) then reallocates, which waits for the whole
+            // device, only while its largest request still grows
+            const size_t vc = std::max(vec_cap, need);
+            const int ic = std::max(item_cap, batch);
             free_vecs();
-            vec_cap = need;
-            item_cap = batch;
+            vec_cap = vc;
+            item_cap = ic;
             TAGS_HIP(hipMalloc(&dscale, sizeof(double) * vec_cap));
             TAGS_HIP(hipMalloc(&x, sizeof(c64) * vec_cap));
             TAGS_HIP(hipMalloc(&r, sizeof(c64) * vec_cap));
@@ -231,12 +238,13 @@ struct Solver {
         if (mixed) {
             if ((rc = w32.alloc(n, di.num_cu, di.lds_limit, batch))) return rc;
             const size_t need32 = (size_t)n * n * batch;
+            w32_hi = std::max(w32_hi, need32);
             if (need32 > w32_cap) {
                 hipFree(W32);
                 W32 = nullptr;
                 w32_cap = 0;
-                TAGS_HIP(hipMalloc(&W32, sizeof(c32) * need32));
-                w32_cap = need32;
+                TAGS_HIP(hipMalloc(&W32, sizeof(c32) * w32_hi));
+                w32_cap = w32_hi;
             }
         }
         return 0;
@@ -1685,6 +1693,10 @@ struct SweepSlot {
         int rc;
         const size_t nw = (size_t)N * N * batch, nr = (size_t)N * batch;
         if (nw > w_cap || nr > rhs_cap || batch > cap) {
+            // grow to cover every earlier request as well (a lane of a
+            // multi-case solve serves systems of different orders in turn)
+            const size_t aw = std::max(nw, w_cap), ar = std::max(nr, rhs_cap);
+            const int items = std::max(batch, cap);
             hipFree(W.p);
             hipFree(rhs.p);
             hipFree(dsc.p);
@@ -1695,13 +1707,12 @@ struct SweepSlot {
             cap = 0;
             w_cap = rhs_cap = 0;
             static_items = 0;
-            const int items = std::max(batch, cap);
-            if ((rc = W.alloc(nw)) || (rc = rhs.alloc(nr)) || (rc = dsc.alloc((size_t)items)))
+            if ((rc = W.alloc(aw)) || (rc = rhs.alloc(ar)) || (rc = dsc.alloc((size_t)items)))
                 return rc;
             TAGS_HIP(hipHostMalloc(&hsc, sizeof(PairScalars) * items, 0));
             cap = items;
-            w_cap = nw;
-            rhs_cap = nr;
+            w_cap = aw;
+            rhs_cap = ar;
         }
         sv.st = stream.s;
         sv.pr = &prof;
@@ -3017,6 +3028,39 @@ int tags_cases_solve(tags_cases* cs, int precision)
                      [&](size_t a, size_t b) { return cost[a] > cost[b]; });
     size_t qi = 0;  // first case in `order` with samples left
     while ((int)cs->lanes.size() < cs->nlanes) cs->lanes.emplace_back(new SweepSlot());
+    // every lane's buffers sized for all cases before any work is queued: a
+    // group size chosen by an N^2 budget makes N times the group grow as N
+    // falls, and a reallocation between cases would wait for the whole
+    // device. The cases with the largest matrix, vector and group needs come
+    // first (buffers only grow), then the rest.
+    {
+        std::vector<size_t> prep(order);
+        auto pick = [&](auto key) {
+            size_t best = 0;
+            for (size_t c = 1; c < nc; c++)
+                if (key(c) > key(best)) best = c;
+            return best;
+        };
+        if (nc) {
+            auto nn = [&](size_t c) { return (double)cs->sw[c]->N; };
+            auto bb = [&](size_t c) { return (double)cs->sw[c]->batch; };
+            prep.insert(prep.begin(), {pick([&](size_t c) { return nn(c) * nn(c) * bb(c); }),
+                                       pick([&](size_t c) { return nn(c) * bb(c); }),
+                                       pick([&](size_t c) { return bb(c) * 1e6 + nn(c); })});
+        }
+        for (auto& lp : cs->lanes) {
+            SweepSlot& L = *lp;
+            for (size_t c : prep) {
+                tags_sweep* sw = cs->sw[c];
+                if ((rc = L.init(sw->N, sw->batch)) ||
+                    (rc = L.sv.ensure((int)sw->N, mixed, sw->batch)))
+                    return rc;
+                if (mixed && (sw->plan_kind == TAGS_PLAN_HYBRID || sw->plan_kind == TAGS_PLAN_NODAL) &&
+                    (rc = L.sv.gm.alloc((int)sw->N, sw->gmres_steps, sw->batch)))
+                    return rc;
+            }
+        }
+    }
     std::vector<long> lane_case(cs->nlanes, -1);
     auto issue = [&](int li) -> int {
         while (qi < nc && next[order[qi]] >= cs->cs[order[qi]].nfreq) qi++;

@@ -265,15 +265,21 @@ int GmresWork::alloc(int n_, int kmax_, int cap_)
         nchunk = (int)nch_;
         return 0;
     }
+    // grow to cover every earlier request as well (a solver that serves
+    // systems of different orders in turn reallocates, which waits for the
+    // whole device, only while its largest request grows)
+    const size_t NB = std::max((size_t)n_ * (size_t)cap_, (size_t)n_cap);
+    const size_t K = (size_t)std::max(kmax_, kmax_cap);
+    const size_t B = (size_t)std::max(cap_, item_cap);
+    const size_t PB = std::max((size_t)(kmax_ + 1) * nch_ * cap_, (size_t)part_cap);
     release();
     n = n_;
     kmax = kmax_;
     cap = cap_;
-    const size_t N = (size_t)n, B = (size_t)cap, K = (size_t)kmax;
-    nchunk = (int)((n + GM_CHUNK - 1) / GM_CHUNK);
-    TAGS_HIP(hipMalloc(&V, sizeof(c64) * N * (K + 1) * B));
-    TAGS_HIP(hipMalloc(&Z, sizeof(c64) * N * K * B));
-    TAGS_HIP(hipMalloc(&w, sizeof(c64) * N * B));
+    nchunk = (int)nch_;
+    TAGS_HIP(hipMalloc(&V, sizeof(c64) * NB * (K + 1)));
+    TAGS_HIP(hipMalloc(&Z, sizeof(c64) * NB * K));
+    TAGS_HIP(hipMalloc(&w, sizeof(c64) * NB));
     TAGS_HIP(hipMalloc(&H, sizeof(c64) * (K + 1) * K * B));
     TAGS_HIP(hipMalloc(&g, sizeof(c64) * (K + 1) * B));
     TAGS_HIP(hipMalloc(&y, sizeof(c64) * K * B));
@@ -283,11 +289,11 @@ int GmresWork::alloc(int n_, int kmax_, int cap_)
     TAGS_HIP(hipMalloc(&sn, sizeof(c64) * K * B));
     TAGS_HIP(hipMalloc(&cs, sizeof(double) * K * B));
     TAGS_HIP(hipMalloc(&inv, sizeof(double) * B));
-    TAGS_HIP(hipMalloc(&part, sizeof(c64) * (K + 1) * (size_t)nchunk * B));
-    n_cap = (long)N * B;
-    kmax_cap = kmax;
-    item_cap = cap;
-    part_cap = (long)(K + 1) * nchunk * (long)B;
+    TAGS_HIP(hipMalloc(&part, sizeof(c64) * PB));
+    n_cap = (long)NB;
+    kmax_cap = (int)K;
+    item_cap = (int)B;
+    part_cap = (long)PB;
     return 0;
 }
 

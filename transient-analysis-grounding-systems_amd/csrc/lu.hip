@@ -1552,13 +1552,13 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
     int rc = tslu_tree(n_, tslu_min_leaf_rows(), &tr);
     if (rc) return rc;
     const size_t B = (size_t)batch_;
-    const size_t need_perm = (size_t)n_ * B, need_inv = 4096 * (size_t)nblk_ * B;
-    const size_t need_cand = (size_t)tr.cand_total * B;
-    const size_t need_cnt = (size_t)std::max(1, tr.cnt_total) * B;
+    size_t need_perm = (size_t)n_ * B, need_inv = 4096 * (size_t)nblk_ * B;
+    size_t need_cand = (size_t)tr.cand_total * B;
+    size_t need_cnt = (size_t)std::max(1, tr.cnt_total) * B;
     // flow-solve control words per item: ticket, spare, one per block group,
     // padded to 64 words so that items do not share cache lines
     const long tstride = ((long)nblk_ + 2 + 63) / 64 * 64;
-    const size_t need_tctl = (size_t)tstride * B;
+    size_t need_tctl = (size_t)tstride * B;
     if (!s2) {
         TAGS_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
         TAGS_HIP(hipEventCreateWithFlags(&ev_left, hipEventDisableTiming));
@@ -1569,6 +1569,15 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
     }
     if (need_perm > cap_perm || need_inv > cap_inv || B > cap_items || need_cand > cap_cand ||
         need_cnt > cap_cnt || need_tctl > cap_tctl) {
+        // grow to cover every earlier request as well: a work space that
+        // serves systems of different orders in turn then reallocates (which
+        // waits for the whole device) only while its largest request grows
+        need_perm = std::max(need_perm, cap_perm);
+        need_inv = std::max(need_inv, cap_inv);
+        need_cand = std::max(need_cand, cap_cand);
+        need_cnt = std::max(need_cnt, cap_cnt);
+        need_tctl = std::max(need_tctl, cap_tctl);
+        const size_t items = std::max(B, cap_items);
         (void)hipStreamSynchronize(s2);
         (void)hipFree(perm);
         (void)hipFree(perm_alt);
@@ -1589,11 +1598,11 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         TAGS_HIP(hipMalloc(&perm_alt, sizeof(int) * need_perm));
         TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * need_inv));
         TAGS_HIP(hipMalloc(&invU, sizeof(cx<T>) * need_inv));
-        TAGS_HIP(hipMalloc(&flags, sizeof(int) * 4 * B));
+        TAGS_HIP(hipMalloc(&flags, sizeof(int) * 4 * items));
         TAGS_HIP(hipMalloc(&vec, sizeof(cx<T>) * need_perm));
         TAGS_HIP(hipMalloc(&d_init, sizeof(int) * need_perm));
         TAGS_HIP(hipMalloc(&d_orders, sizeof(int) * need_perm));
-        TAGS_HIP(hipMemset(flags, 0, sizeof(int) * 4 * B));
+        TAGS_HIP(hipMemset(flags, 0, sizeof(int) * 4 * items));
         TAGS_HIP(hipMalloc(&tcand, sizeof(int) * need_cand));
         TAGS_HIP(hipMalloc(&tcnt, sizeof(unsigned) * need_cnt));
         // arrival counters start at zero; every tournament's root resets the
@@ -1606,7 +1615,7 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         epoch = 0;
         cap_perm = need_perm;
         cap_inv = need_inv;
-        cap_items = B;
+        cap_items = items;
         cap_cand = need_cand;
         cap_cnt = need_cnt;
         cap_tctl = need_tctl;
@@ -1614,7 +1623,7 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
     }
     if (tstride != tctl_stride) {
         // the items' control words move: their tickets must read zero again
-        TAGS_HIP(hipMemset(tctl, 0, sizeof(unsigned) * need_tctl));
+        TAGS_HIP(hipMemset(tctl, 0, sizeof(unsigned) * cap_tctl));
         epoch = 0;
         tctl_stride = tstride;
     }
