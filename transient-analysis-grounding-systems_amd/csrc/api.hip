@@ -473,11 +473,14 @@ struct Solver {
         if (dn == 0.0) return 0.0;
         return xn > 0 ? dn / xn : 1.0;
     }
-    // estimated error of x_k relative to |x_k| (see kFwdAccept)
+    // estimated error of x_k relative to |x_k| (see kFwdAccept). FGMRES steps
+    // take the correction itself: their contraction varies from one restart
+    // to the next (a fast first step says little about the second), so the
+    // last step's ratio does not bound the error left after it.
     double fwd_at(int z, int k) const
     {
         const double d = dx_at(z, k);
-        if (k < 2 || d == 0.0) return d;
+        if (k < 2 || d == 0.0 || gmres_k > 0) return d;
         const double dp = dx_at(z, k - 1);
         const double rho = dp > 0.0 ? d / dp : 1.0;
         return rho < 0.5 ? d * rho / (1.0 - rho) : d;

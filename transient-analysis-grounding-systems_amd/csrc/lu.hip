@@ -1112,6 +1112,58 @@ __global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A, long ld,
 // FP64 residual r = b - W x (W row-major n x n) and |r|_inf, |x|_inf, |W|_inf
 // reductions (non-negative doubles compare as unsigned 64-bit integers).
 // ---------------------------------------------------------------------------
+// Double-double accumulation for the refinement residuals: a sum kept as
+// hi + lo with every product and addition error carried (Dekker/Knuth exact
+// transformations: a b = p + fma(a, b, -p), s = hi + p + e by TwoSum), so that
+// b - W x is accurate to about one FP64 rounding of its own size even where
+// b and W x cancel. With FP64 residuals alone, refinement stops at a forward
+// error of about cond(W) eps; with these it converges to working accuracy.
+// Contraction is off here: a fused multiply-add in place of the separate
+// product and sum would break the error terms.
+struct DD {
+    double hi, lo;
+};
+__device__ inline void dd_mac(DD& d, double a, double b)
+{
+#pragma clang fp contract(off)
+    const double p = a * b;
+    const double pe = fma(a, b, -p);
+    const double s = d.hi + p;
+    const double bb = s - d.hi;
+    const double se = (d.hi - (s - bb)) + (p - bb);
+    d.hi = s;
+    d.lo += pe + se;
+}
+__device__ inline void dd_add(DD& d, double p)
+{
+#pragma clang fp contract(off)
+    const double s = d.hi + p;
+    const double bb = s - d.hi;
+    const double se = (d.hi - (s - bb)) + (p - bb);
+    d.hi = s;
+    d.lo += se;
+}
+__device__ inline DD dd_plus(DD a, DD b)
+{
+#pragma clang fp contract(off)
+    DD r = a;
+    dd_add(r, b.hi);
+    r.lo += b.lo;
+    return r;
+}
+__device__ inline DD dd_shfl_xor(DD a, int o)
+{
+    return DD{__shfl_xor(a.hi, o), __shfl_xor(a.lo, o)};
+}
+// b - (hi + lo), rounded once
+__device__ inline double dd_sub_from(double b, DD d)
+{
+#pragma clang fp contract(off)
+    DD t{b, 0.0};
+    dd_add(t, -d.hi);
+    return t.hi + (t.lo - d.lo);
+}
+
 __global__ __launch_bounds__(256) void k_residual(const c64* W, long ld, int n, const c64* x,
                                                    const c64* b, c64* r,
                                                    unsigned long long* norms, int want_wnorm,
@@ -1126,34 +1178,39 @@ __global__ __launch_bounds__(256) void k_residual(const c64* W, long ld, int n, 
     norms += z_ * sn;
     const int row = blockIdx.x;
     const c64* Wr = W + (long)row * ld;
-    double sx = 0, sy = 0, an = 0;
+    DD sx{0, 0}, sy{0, 0};
+    double an = 0;
     for (int c = threadIdx.x; c < n; c += 256) {
         const c64 a = Wr[c], v = x[c];
-        sx += a.x * v.x - a.y * v.y;
-        sy += a.x * v.y + a.y * v.x;
+        dd_mac(sx, a.x, v.x);
+        dd_mac(sx, -a.y, v.y);
+        dd_mac(sy, a.x, v.y);
+        dd_mac(sy, a.y, v.x);
         if (want_wnorm) an += sqrt(a.x * a.x + a.y * a.y);
     }
-    __shared__ double red[3][4];
+    __shared__ DD red[2][4];
+    __shared__ double reda[4];
     for (int off = 32; off > 0; off >>= 1) {
-        sx += __shfl_xor(sx, off);
-        sy += __shfl_xor(sy, off);
+        sx = dd_plus(sx, dd_shfl_xor(sx, off));
+        sy = dd_plus(sy, dd_shfl_xor(sy, off));
         an += __shfl_xor(an, off);
     }
     const int wv = threadIdx.x >> 6;
     if ((threadIdx.x & 63) == 0) {
         red[0][wv] = sx;
         red[1][wv] = sy;
-        red[2][wv] = an;
+        reda[wv] = an;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double tx = 0, ty = 0, ta = 0;
+        DD tx{0, 0}, ty{0, 0};
+        double ta = 0;
         for (int i = 0; i < 4; i++) {
-            tx += red[0][i];
-            ty += red[1][i];
-            ta += red[2][i];
+            tx = dd_plus(tx, red[0][i]);
+            ty = dd_plus(ty, red[1][i]);
+            ta += reda[i];
         }
-        const c64 rr = c64{b[row].x - tx, b[row].y - ty};
+        const c64 rr = c64{dd_sub_from(b[row].x, tx), dd_sub_from(b[row].y, ty)};
         r[row] = rr;
         const double ra = sqrt(rr.x * rr.x + rr.y * rr.y);
         const double xa = sqrt(x[row].x * x[row].x + x[row].y * x[row].y);
@@ -1170,7 +1227,7 @@ __global__ __launch_bounds__(256) void k_residual(const c64* W, long ld, int n, 
 // work-groups take 256 node rows each, one thread per row, summing over the
 // node's segment ends in list order. mode 0: r = W x; 1: r = b - W x with
 // |r| and |x| max-reductions into norms[0], norms[1]; 2: also |W| (row sums
-// of |W_ij|) into norms[2].
+// of |W_ij|) into norms[2]. The residual modes sum in double-double.
 __global__ __launch_bounds__(256) void k_hem_apply(const c64* W, long ld, HemOp op, const c64* x,
                                                     long sx, const c64* b, c64* r, long sr,
                                                     unsigned long long* norms, long sn, int mode,
@@ -1184,6 +1241,8 @@ __global__ __launch_bounds__(256) void k_hem_apply(const c64* W, long ld, HemOp 
     r += z_ * sr;
     if (norms) norms += z_ * sn;
     const bool want_w = mode == 2;
+    // residuals (modes 1 and 2) in double-double; plain products (mode 0) in FP64
+    const bool comp = mode != 0;
     const long m = op.m;
     if ((long)blockIdx.x < 2 * m) {
         const bool tblk = (long)blockIdx.x >= m;
@@ -1192,51 +1251,65 @@ __global__ __launch_bounds__(256) void k_hem_apply(const c64* W, long ld, HemOp 
         const long row = off + i;
         const c64* Wr = W + row * ld + off;
         const c64* xz = x + off;
-        double sx_ = 0, sy_ = 0, an = 0;
-        for (long c = threadIdx.x; c < m; c += 256) {
-            const c64 a = Wr[c], v = xz[c];
-            sx_ += a.x * v.x - a.y * v.y;
-            sy_ += a.x * v.y + a.y * v.x;
-            if (want_w) an += sqrt(a.x * a.x + a.y * a.y);
+        DD sx_{0, 0}, sy_{0, 0};
+        double an = 0;
+        if (comp) {
+            for (long c = threadIdx.x; c < m; c += 256) {
+                const c64 a = Wr[c], v = xz[c];
+                dd_mac(sx_, a.x, v.x);
+                dd_mac(sx_, -a.y, v.y);
+                dd_mac(sy_, a.x, v.y);
+                dd_mac(sy_, a.y, v.x);
+                if (want_w) an += sqrt(a.x * a.x + a.y * a.y);
+            }
+        } else {
+            for (long c = threadIdx.x; c < m; c += 256) {
+                const c64 a = Wr[c], v = xz[c];
+                sx_.hi += a.x * v.x - a.y * v.y;
+                sy_.hi += a.x * v.y + a.y * v.x;
+            }
         }
-        __shared__ double red[3][4];
+        __shared__ DD red[2][4];
+        __shared__ double reda[4];
         for (int o = 32; o > 0; o >>= 1) {
-            sx_ += __shfl_xor(sx_, o);
-            sy_ += __shfl_xor(sy_, o);
+            sx_ = dd_plus(sx_, dd_shfl_xor(sx_, o));
+            sy_ = dd_plus(sy_, dd_shfl_xor(sy_, o));
             an += __shfl_xor(an, o);
         }
         const int wv = threadIdx.x >> 6;
         if ((threadIdx.x & 63) == 0) {
             red[0][wv] = sx_;
             red[1][wv] = sy_;
-            red[2][wv] = an;
+            reda[wv] = an;
         }
         __syncthreads();
         if (threadIdx.x != 0) return;
-        double tx = 0, ty = 0, ta = 0;
+        DD tx{0, 0}, ty{0, 0};
+        double ta = 0;
         for (int q = 0; q < 4; q++) {
-            tx += red[0][q];
-            ty += red[1][q];
-            ta += red[2][q];
+            tx = dd_plus(tx, red[0][q]);
+            ty = dd_plus(ty, red[1][q]);
+            ta += reda[q];
         }
         // incidence: A (+1 start, -1 end) on Z_L rows, B (1/2, 1/2) on Z_T rows
+        // (products by +-1 and 1/2 are exact)
         const int s = op.sn[i], e = op.en[i];
         const double cs = tblk ? 0.5 : 1.0, ce = tblk ? 0.5 : -1.0;
         const c64 xs = x[op.node_off + s];
-        tx += cs * xs.x;
-        ty += cs * xs.y;
+        dd_add(tx, cs * xs.x);
+        dd_add(ty, cs * xs.y);
         ta += fabs(cs);
         if (e != s) {
             const c64 xe = x[op.node_off + e];
-            tx += ce * xe.x;
-            ty += ce * xe.y;
+            dd_add(tx, ce * xe.x);
+            dd_add(ty, ce * xe.y);
             ta += fabs(ce);
         }
         if (mode == 0) {
-            r[row] = c64{tx, ty};
+            r[row] = c64{tx.hi + tx.lo, ty.hi + ty.lo};
             return;
         }
-        const c64 rr = c64{b[row].x - tx, b[row].y - ty};
+        const c64 rr = c64{dd_sub_from(b[row].x, tx), dd_sub_from(b[row].y, ty)};
         r[row] = rr;
         const double ra = sqrt(rr.x * rr.x + rr.y * rr.y);
         const double xa = sqrt(x[row].x * x[row].x + x[row].y * x[row].y);
@@ -1249,22 +1322,24 @@ __global__ __launch_bounds__(256) void k_hem_apply(const c64* W, long ld, HemOp 
     const long u = ((long)blockIdx.x - 2 * m) * 256 + threadIdx.x;
     double ra = 0, xa = 0, ta = 0;
     if (u < op.n) {
-        double tx = 0, ty = 0;
+        DD tx{0, 0}, ty{0, 0};
         const int p1 = op.node_ptr[u + 1];
         for (int p = op.node_ptr[u]; p < p1; p++) {
             const int se = op.node_seg[p];
             const long s = se >> 1;
             const double cl = (se & 1) ? -1.0 : 1.0;
             const c64 xl = x[op.zl_off + s], xt = x[op.zt_off + s];
-            tx += cl * xl.x + 0.5 * xt.x;
-            ty += cl * xl.y + 0.5 * xt.y;
+            dd_add(tx, cl * xl.x);
+            dd_add(tx, 0.5 * xt.x);
+            dd_add(ty, cl * xl.y);
+            dd_add(ty, 0.5 * xt.y);
             ta += 1.5;
         }
         const long row = op.node_off + u;
         if (mode == 0) {
-            r[row] = c64{tx, ty};
+            r[row] = c64{tx.hi + tx.lo, ty.hi + ty.lo};
         } else {
-            const c64 rr = c64{b[row].x - tx, b[row].y - ty};
+            const c64 rr = c64{dd_sub_from(b[row].x, tx), dd_sub_from(b[row].y, ty)};
             r[row] = rr;
             ra = sqrt(rr.x * rr.x + rr.y * rr.y);
             xa = sqrt(x[row].x * x[row].x + x[row].y * x[row].y);
