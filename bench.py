@@ -426,13 +426,16 @@ def more_configs(T, threads, tuning):
                      "injection")
     res["c5"] = r
     note("c4 (512 seeded cases)")
-    res["c4"] = cases_config(T, threads)
+    res["c4"] = cases_config(T, threads, tuning=tuning)
     return res
 
 
-def cases_config(T, threads, step=8):
+def cases_config(T, threads, step=8, tuning=("table", TUNING_TABLE)):
     """512 seeded cases (rods, counterpoises, grids; N = 61 .. 4246), every
-    step-th frequency of each, one multi-case solve per precision."""
+    step-th frequency of each, one multi-case solve per precision, with the
+    tile shapes and schedule of each size class from the tuning table (one
+    case per class tuned, tags_cases_autotune) unless `tuning` says
+    otherwise."""
     import tags_pkg
     C = tags_pkg.cases()
     fam = C.case_family(T, freq_step=step)
@@ -442,6 +445,9 @@ def cases_config(T, threads, step=8):
                        "%d samples" % (step, n)}
     us = {}
     for name, prec in (("mixed", T.PREC_MIXED), ("fp64", T.PREC_FP64)):
+        tu = "defaults"
+        if tuning[0] != "off":
+            tu = cs.autotune(prec, cache=tuning[1], force=(tuning[0] == "measure"))
         cs.solve(prec)  # allocations
         (u, st), dt = _timed_solve(T, cs, prec)
         cs.set_profiling(True)
@@ -453,7 +459,8 @@ def cases_config(T, threads, step=8):
              "roofline": _roofline(tm, PEAK["fp32" if prec == T.PREC_MIXED else "fp64"],
                                    "k_trailing_update"),
              "dense_equivalent_tflops": sum(len(c["s"]) * lu_flops(c["N"]) for c in fam) / dt
-             / 1e12}
+             / 1e12,
+             "tuning_classes": tu}
         if prec == T.PREC_MIXED:
             r["fp64_fallbacks"] = int(sum(q["precision_used"] == 0 for s_ in st for q in s_))
         out[name] = r

@@ -479,6 +479,8 @@ typedef struct {
     int inflight;          /* groups of samples in flight, 1..4 */
     int batch;             /* samples per group (0: chosen from N) */
     double nodal_above;    /* hybrid plan's switch (Hz), mixed form only */
+    int gmres_steps;       /* Arnoldi steps per FGMRES refinement step, mixed form
+                              only (1..32; 0 leaves the sweep's) */
     double ms_per_sample;  /* out: time per probe sample with these values */
     int from_cache;        /* out: 1 read from the cache file, 0 measured */
     int evaluations;       /* out: timed probe solves */
@@ -491,7 +493,8 @@ typedef struct {
  * step widths; then, by coordinate descent from the current settings, step
  * width (with its fastest variants), update launches, solve rows, samples in
  * flight, samples per group (orders below 8192) and, for the mixed form of
- * the hybrid plan, the nodal switch (0 or 20 kHz) are each kept when a
+ * the hybrid plan, the nodal switch (0 or 20 kHz) and the Arnoldi steps per
+ * FGMRES refinement step (6, 8, 10) are each kept when a
  * timed solve of the `nprobe` probe samples (s, kappa, mur, inj_node as in
  * tags_sweep_solve; one untimed solve first for the allocations) runs more
  * than 3 % faster. With cache_path (nullable) an entry for the same device,
@@ -588,6 +591,18 @@ int tags_cases_set_plan(tags_cases *cs, int plan);
 /* Lanes (1 .. 4, default 3): groups in flight, each with its own stream, W and
  * factor work space, sized for the largest group it has taken. */
 int tags_cases_set_lanes(tags_cases *cs, int lanes);
+
+/* Autotuner over the cases (no counterpart in the reference): the cases are
+ * split into size classes by N (below 256, then doubling up to 8192, then
+ * above), the largest case of each class is tuned as tags_sweep_autotune does
+ * (its own samples as probe, up to 64 spread over them; cache_path and force
+ * as there, the entry keyed by that case), and its tile shapes, group size,
+ * nodal switch and FGMRES steps are installed on every case of the class
+ * (samples in flight do not apply: the lanes take groups as they free up).
+ * `classes` (nullable) receives up to 16 entries, one per class tuned, and
+ * *nclasses their number. Returns 0 or an error code. */
+int tags_cases_autotune(tags_cases *cs, int precision, const char *cache_path, int force,
+                        tags_tuning *classes, int *nclasses);
 
 /* HIP-event timings of every lane's kernels, as tags_sweep_get_timers. */
 int tags_cases_set_profiling(tags_cases *cs, int on);

@@ -146,4 +146,5 @@ def test_tuning_entry_points_check_arguments():
     assert L.tags_set_solve_rows(3) == -12
     assert L.tags_set_solve_rows(-1) == -12
     assert L.tags_set_solve_rows(2) == 0
-    assert ctypes.sizeof(T.Tuning) == 7 * 4 + 4 + 2 * 8 + 2 * 4
+    # 7 ints, padding, nodal_above, gmres_steps + padding, ms_per_sample, 2 ints
+    assert ctypes.sizeof(T.Tuning) == 7 * 4 + 4 + 8 + 8 + 8 + 2 * 4

@@ -71,3 +71,30 @@ def test_tuning_rejects_bad_values():
     with pytest.raises(RuntimeError):
         sw.set_tuning(1, bad)
     sw.close()
+
+
+def test_cases_autotune_by_size_class(tmp_path):
+    """tags_cases_autotune tunes the largest case of each size class and
+    installs its choice on the class; the answers agree with the untuned
+    solve to rounding."""
+    C = tags_pkg.cases()
+    fam = C.case_family(T, freq_step=16)
+    Ns = np.array([c["N"] for c in fam])
+    pick = sorted({int(np.argmin(np.abs(Ns - q))) for q in (70, 200, 300, 600, 900, 1400)})
+    sub = [fam[i] for i in pick]
+    cache = str(tmp_path / "tuning.txt")
+    cs = T.Cases(sub)
+    u0, _ = cs.solve(T.PREC_MIXED)
+    cl = cs.autotune(T.PREC_MIXED, cache=cache)
+    print("classes:", [(t["gemm_big"], t["step_width"], t["nodal_above"], t["gmres_steps"])
+                       for t in cl], flush=True)
+    assert 3 <= len(cl) <= len(sub)
+    assert all(t["from_cache"] == 0 for t in cl)
+    u1, st1 = cs.solve(T.PREC_MIXED)
+    for a, b, st in zip(u1, u0, st1):
+        assert np.max(np.abs(a - b) / np.abs(b)) <= 1e-9
+        assert all(q["precision_used"] == 1 for q in st)
+    # a second autotune reads every class from the cache
+    cl2 = cs.autotune(T.PREC_MIXED, cache=cache)
+    assert [t["from_cache"] for t in cl2] == [1] * len(cl)
+    cs.close()

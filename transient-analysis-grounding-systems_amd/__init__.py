@@ -125,6 +125,7 @@ class Tuning(ctypes.Structure):
                 ("step_width", ctypes.c_int), ("update_chunks", ctypes.c_int),
                 ("solve_rows", ctypes.c_int), ("inflight", ctypes.c_int),
                 ("batch", ctypes.c_int), ("nodal_above", ctypes.c_double),
+                ("gmres_steps", ctypes.c_int),
                 ("ms_per_sample", ctypes.c_double), ("from_cache", ctypes.c_int),
                 ("evaluations", ctypes.c_int)]
 
@@ -172,6 +173,7 @@ EXPORTED = ["populate_electrode", "segment_electrode", "number_segments", "numbe
             "tags_cases_solve", "tags_cases_set_plan", "tags_cases_set_lanes",
             "tags_cases_set_profiling", "tags_cases_get_timers", "tags_cases_destroy",
             "tags_sweep_autotune", "tags_sweep_set_tuning", "tags_sweep_get_tuning",
+            "tags_cases_autotune",
             "tags_sweep_set_gmres"]
 
 _lib = None
@@ -244,6 +246,7 @@ def lib():
         "tags_sweep_autotune": ([P, I, S, P, P, D, S, ctypes.c_char_p, I, P], I),
         "tags_sweep_set_tuning": ([P, I, P], I),
         "tags_sweep_get_tuning": ([P, I, P], I),
+        "tags_cases_autotune": ([P, I, ctypes.c_char_p, I, P, P], I),
         "tags_sweep_set_nodal_above": ([P, D], I),
         "tags_autotune": ([I, ctypes.POINTER(I), ctypes.POINTER(I)], I),
         "tags_sweep_set_plan": ([P, I], I),
@@ -838,6 +841,19 @@ class Cases:
         rc = lib().tags_cases_set_lanes(self._h, lanes)
         if rc:
             raise RuntimeError("tags_cases_set_lanes failed: %d" % rc)
+
+    def autotune(self, precision, cache=None, force=False):
+        """Tunes one case per size class and installs its choice on the class
+        (tags_cases_autotune); returns the classes' values as dicts."""
+        out = (Tuning * 16)()
+        nk = ctypes.c_int(0)
+        rc = lib().tags_cases_autotune(self._h, int(precision),
+                                       None if cache is None else os.fsencode(cache),
+                                       1 if force else 0, ctypes.cast(out, ctypes.c_void_p),
+                                       ctypes.byref(nk))
+        if rc:
+            raise RuntimeError("tags_cases_autotune failed: %d" % rc)
+        return [out[i].as_dict() for i in range(min(16, nk.value))]
 
     def set_profiling(self, on=True):
         lib().tags_cases_set_profiling(self._h, 1 if on else 0)

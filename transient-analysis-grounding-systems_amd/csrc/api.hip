@@ -2510,7 +2510,9 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     // the hybrid plan's switch may rise during this call; it is restored at
     // the end, so that every call starts from the configured switch
     const double nodal_above0 = sw->nodal_above;
+    const int gmres_steps0 = sw->gmres_steps;
     if (tu && mixed && tu->nodal_above >= 0.0) sw->nodal_above = tu->nodal_above;
+    if (tu && mixed && tu->gmres_steps > 0) sw->gmres_steps = tu->gmres_steps;
     size_t next = 0;
     for (int i = 0; i < k && next < nfreq; i++) {
         const int got = sweep_issue(sw, *sw->slots[i], next, c);
@@ -2552,6 +2554,7 @@ int tags_sweep_solve(tags_sweep* sw, size_t nfreq, const TAGS_COMPLEX* s,
     }
     // after an error: let every queued sample end before returning
     sw->nodal_above = nodal_above0;
+    sw->gmres_steps = gmres_steps0;
     for (int i = 0; i < k; i++) {
         TAGS_HIP(hipStreamSynchronize(sw->slots[i]->stream.s));
         sw->slots[i]->f = -1;
@@ -2601,6 +2604,7 @@ bool tuning_valid(const tags_tuning& t, int precision)
     if (t.solve_rows > 2) return false;
     if (t.inflight < 0 || t.inflight > 4) return false;
     if (t.batch > 4096) return false;
+    if (t.gmres_steps < 0 || t.gmres_steps > 32) return false;
     return true;
 }
 
@@ -2639,9 +2643,9 @@ std::string tune_value(const tags_tuning& t)
     char buf[256];
     std::snprintf(buf, sizeof(buf),
                   "big=%d small=%d width=%d chunks=%d rows=%d inflight=%d batch=%d "
-                  "nodal_above=%.6g ms=%.4f",
+                  "nodal_above=%.6g gmres=%d ms=%.4f",
                   t.gemm_big, t.gemm_small, t.step_width, t.update_chunks, t.solve_rows,
-                  t.inflight, t.batch, t.nodal_above, t.ms_per_sample);
+                  t.inflight, t.batch, t.nodal_above, t.gmres_steps, t.ms_per_sample);
     return buf;
 }
 
@@ -2661,10 +2665,10 @@ bool tune_cache_read(const char* path, const std::string& key, tags_tuning* t)
         tags_tuning v{};
         if (std::sscanf(l.c_str() + bar + 3,
                         "big=%d small=%d width=%d chunks=%d rows=%d inflight=%d batch=%d "
-                        "nodal_above=%lf ms=%lf",
+                        "nodal_above=%lf gmres=%d ms=%lf",
                         &v.gemm_big, &v.gemm_small, &v.step_width, &v.update_chunks,
                         &v.solve_rows, &v.inflight, &v.batch, &v.nodal_above,
-                        &v.ms_per_sample) == 9) {
+                        &v.gmres_steps, &v.ms_per_sample) == 10) {
             *t = v;
             found = true;
         }
@@ -2828,6 +2832,7 @@ int tags_sweep_autotune(tags_sweep* sw, int precision, size_t nprobe, const TAGS
     cur.inflight = sw->inflight;
     cur.batch = -1;
     cur.nodal_above = mixed ? sw->nodal_above : -1.0;
+    cur.gmres_steps = 0;
     double best = 0;
     auto fail = [&](int r) {
         if (had) tags_sweep_set_tuning(sw, precision, &saved);
@@ -2886,6 +2891,14 @@ int tags_sweep_autotune(tags_sweep* sw, int precision, size_t nprobe, const TAGS
         tags_tuning c = cur;
         c.nodal_above = cur.nodal_above > 0.0 ? 0.0 : 2e4;
         if ((rc = consider(c))) return fail(rc);
+    }
+    if (mixed && (sw->plan_kind == TAGS_PLAN_HYBRID || sw->plan_kind == TAGS_PLAN_NODAL)) {
+        for (int k : {6, 8, 10}) {
+            if (k == (cur.gmres_steps > 0 ? cur.gmres_steps : sw->gmres_steps)) continue;
+            tags_tuning c = cur;
+            c.gmres_steps = k;
+            if ((rc = consider(c))) return fail(rc);
+        }
     }
     cur.ms_per_sample = best;
     cur.from_cache = 0;
@@ -2958,6 +2971,56 @@ int tags_cases_set_lanes(tags_cases* cs, int lanes)
     return 0;
 }
 
+int tags_cases_autotune(tags_cases* cs, int precision, const char* cache_path, int force,
+                        tags_tuning* classes, int* nclasses)
+{
+    if (!cs || (precision != TAGS_PREC_FP64 && precision != TAGS_PREC_MIXED)) return TAGS_ERR_ARG;
+    const size_t nc = cs->cs.size();
+    // class of order N: 0 below 256, then one per doubling, the last from 8192
+    auto class_of = [](long N) {
+        int k = 0;
+        for (long lim = 256; N >= lim && k < 6; lim *= 2) k++;
+        return k;
+    };
+    std::vector<long> rep(7, -1);
+    for (size_t c = 0; c < nc; c++) {
+        const int k = class_of(cs->sw[c]->N);
+        if (cs->cs[c].nfreq == 0) continue;
+        if (rep[k] < 0 || cs->sw[c]->N > cs->sw[rep[k]]->N) rep[k] = (long)c;
+    }
+    int nout = 0;
+    for (int k = 0; k < 7; k++) {
+        if (rep[k] < 0) continue;
+        const size_t c = (size_t)rep[k];
+        tags_sweep* sw = cs->sw[c];
+        const tags_case& d = cs->cs[c];
+        sw->plan_kind = cs->plan_kind;
+        // probe: up to 64 of the case's samples, spread over them
+        const size_t step = (d.nfreq + 63) / 64;
+        std::vector<TAGS_COMPLEX> ps, pk;
+        for (size_t f = 0; f < d.nfreq; f += step) {
+            ps.push_back(d.s[f]);
+            pk.push_back(d.kappa[f]);
+        }
+        tags_tuning t{};
+        int rc = tags_sweep_autotune(sw, precision, ps.size(), ps.data(), pk.data(), d.mur,
+                                     d.inj_node, cache_path, force, &t);
+        // the case's own slots served the timing only: the lanes solve it
+        sw->slots.clear();
+        if (rc) return rc;
+        tags_tuning inst = t;
+        inst.inflight = 0;  // lanes, not slots
+        for (size_t q = 0; q < nc; q++)
+            if (class_of(cs->sw[q]->N) == k && (rc = tags_sweep_set_tuning(cs->sw[q], precision,
+                                                                           &inst)))
+                return rc;
+        if (classes && nout < 16) classes[nout] = t;
+        nout++;
+    }
+    if (nclasses) *nclasses = nout;
+    return 0;
+}
+
 int tags_cases_set_profiling(tags_cases* cs, int on)
 {
     if (!cs) return TAGS_ERR_ARG;
@@ -3001,6 +3064,7 @@ int tags_cases_solve(tags_cases* cs, int precision)
     // per case: plans, layout and group size
     std::vector<SweepCall> call(nc);
     std::vector<double> cost(nc), nodal0(nc);
+    std::vector<int> gmres0(nc);
     std::vector<size_t> next(nc, 0);
     for (size_t c = 0; c < nc; c++) {
         tags_sweep* sw = cs->sw[c];
@@ -3022,6 +3086,9 @@ int tags_cases_solve(tags_cases* cs, int precision)
         const double N = (double)sw->N, m = (double)sw->m;
         cost[c] = 8.0 / 3.0 * N * N * N + 6400.0 * m * m;
         nodal0[c] = sw->nodal_above;
+        gmres0[c] = sw->gmres_steps;
+        if (tu && mixed && tu->nodal_above >= 0.0) sw->nodal_above = tu->nodal_above;
+        if (tu && mixed && tu->gmres_steps > 0) sw->gmres_steps = tu->gmres_steps;
     }
     // the queue: cases by per-sample cost, largest first; a case's groups are
     // taken in frequency order
@@ -3118,7 +3185,10 @@ int tags_cases_solve(tags_cases* cs, int precision)
         lane_case[done] = -1;
         rc = issue(done);
     }
-    for (size_t c = 0; c < nc; c++) cs->sw[c]->nodal_above = nodal0[c];
+    for (size_t c = 0; c < nc; c++) {
+        cs->sw[c]->nodal_above = nodal0[c];
+        cs->sw[c]->gmres_steps = gmres0[c];
+    }
     for (auto& l : cs->lanes) {
         TAGS_HIP(hipStreamSynchronize(l->stream.s));
         l->f = -1;
