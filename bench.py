@@ -644,6 +644,14 @@ def main():
         sw.set_profiling(False)
         return u, st, dt, tm, inflight
 
+    def effective(key, default, lowest):
+        """The mixed form's value of a setting: the tuned one where the tuning
+        sets it (>= lowest), else the command line's."""
+        tu = tuned.get(T.PREC_MIXED)
+        if isinstance(tu, dict) and tu.get(key, -1) >= lowest:
+            return tu[key]
+        return default
+
     def width_of(prec):
         tu = tuned.get(prec)
         if isinstance(tu, dict) and tu["step_width"] > 0:
@@ -680,10 +688,10 @@ def main():
                    "formulation": args.formulation,
                    "plan": PLAN_NAMES[args.plan_mixed],
                    "step_width": width_of(T.PREC_MIXED),
-                   "nodal_above_hz": args.nodal_above,
+                   "nodal_above_hz": effective("nodal_above", args.nodal_above, 0.0),
                    "gmres_below_hz": args.gmres_below,
-                   "gmres_steps": args.gmres_steps,
-                   "solve_rows": args.solve_rows,
+                   "gmres_steps": effective("gmres_steps", args.gmres_steps, 1),
+                   "solve_rows": effective("solve_rows", args.solve_rows, 0),
                    "tuning": tuned[T.PREC_MIXED]},
         "mixed": {"refine_iters": [q["refine_iters"] for q in st_mx],
                   "precision_used": [q["precision_used"] for q in st_mx],

@@ -470,8 +470,13 @@ int launch_pairs_dropin(double* zl, double* zt, bool alias, SegSoA e, SegSoA img
 // tile is written once as (i, k) and once transposed as (k, i), both passes
 // coalesced through an LDS staging tile.
 // ---------------------------------------------------------------------------
-constexpr int kFT = 32;
+constexpr int kFT = 32;  // tile edge of the mHEM fill
 
+// Tiles are FT x FT pairs: 32 (four pairs per thread), or 16 (one pair per
+// thread) when a launch would otherwise have too few work-groups to fill the
+// device (few segments per sample: the 15 m conductor's 30 segments make one
+// 32 x 32 tile, three 16 x 16 ones).
+template <int FT>
 __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long zl_off, long zt_off,
                                                        long m, SegSoA e, SegSoA img, PairScalars sc,
                                                        long sw, const PairScalars* scs)
@@ -487,14 +492,14 @@ __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long zl_
     while ((ti + 1) * (ti + 2) / 2 <= b) ti++;
     while (ti * (ti + 1) / 2 > b) ti--;
     const long tk = b - ti * (ti + 1) / 2;
-    const long i0 = ti * kFT, k0 = tk * kFT;
+    const long i0 = ti * FT, k0 = tk * FT;
 
-    __shared__ double geo[2][8][kFT];   // [row/col edge][ax ay az bx by bz len rad][seg]
-    __shared__ c64 tl[kFT][kFT + 1];
-    __shared__ c64 tt[kFT][kFT + 1];
+    __shared__ double geo[2][8][FT];   // [row/col edge][ax ay az bx by bz len rad][seg]
+    __shared__ c64 tl[FT][FT + 1];
+    __shared__ c64 tt[FT][FT + 1];
     const int t = threadIdx.x;
-    if (t < 2 * kFT) {
-        const int side = t / kFT, s = t % kFT;
+    if (t < 2 * FT) {
+        const int side = t / FT, s = t % FT;
         const long idx = (side == 0 ? i0 : k0) + s;
         if (idx < m) {
             geo[side][0][s] = e.ax[idx];
@@ -508,8 +513,8 @@ __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long zl_
         }
     }
     __syncthreads();
-    for (int p = t; p < kFT * kFT; p += 256) {
-        const int a = p / kFT, c = p % kFT;  // a: row in tile (i), c: column (k)
+    for (int p = t; p < FT * FT; p += 256) {
+        const int a = p / FT, c = p % FT;  // a: row in tile (i), c: column (k)
         const long i = i0 + a, k = k0 + c;
         c64 vl = c64{0, 0}, vt = c64{0, 0};
         if (i < m && k < m && k <= i) {
@@ -552,8 +557,8 @@ __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long zl_
     }
     __syncthreads();
     // pass 1: entries (i, k), k <= i of the Z_L and Z_T blocks
-    for (int p = t; p < kFT * kFT; p += 256) {
-        const int a = p / kFT, c = p % kFT;
+    for (int p = t; p < FT * FT; p += 256) {
+        const int a = p / FT, c = p % FT;
         const long i = i0 + a, k = k0 + c;
         if (i < m && k < m && k <= i) {
             W[(zl_off + i) * ldw + (zl_off + k)] = tl[a][c];
@@ -561,8 +566,8 @@ __global__ __launch_bounds__(256) void k_fill_zblocks(c64* W, long ldw, long zl_
         }
     }
     // pass 2: transposed entries (k, i), k < i
-    for (int p = t; p < kFT * kFT; p += 256) {
-        const int c = p / kFT, a = p % kFT;  // consecutive threads walk i
+    for (int p = t; p < FT * FT; p += 256) {
+        const int c = p / FT, a = p % FT;  // consecutive threads walk i
         const long i = i0 + a, k = k0 + c;
         if (i < m && k < m && k < i) {
             W[(zl_off + k) * ldw + (zl_off + i)] = tl[a][c];
@@ -580,10 +585,17 @@ int launch_fill_zblocks(c64* W, long ldw, const BlockLayout& L, SegSoA e, SegSoA
     if (nb > 1 && !scs) return TAGS_ERR_ARG;
     int rc = gl_tables_init();
     if (rc) return rc;
-    const long T = (m + kFT - 1) / kFT;
-    const long nt = T * (T + 1) / 2;
-    hipLaunchKernelGGL(k_fill_zblocks, dim3((unsigned)nt, 1, (unsigned)nb), dim3(256), 0, st, W,
-                       ldw, L.zl_off, L.zt_off, m, e, img, sc, sw, scs);
+    const long T32 = (m + 31) / 32;
+    if (T32 * (T32 + 1) / 2 * nb >= 512) {
+        hipLaunchKernelGGL(k_fill_zblocks<32>, dim3((unsigned)(T32 * (T32 + 1) / 2), 1,
+                                                    (unsigned)nb),
+                           dim3(256), 0, st, W, ldw, L.zl_off, L.zt_off, m, e, img, sc, sw, scs);
+    } else {
+        const long T16 = (m + 15) / 16;
+        hipLaunchKernelGGL(k_fill_zblocks<16>, dim3((unsigned)(T16 * (T16 + 1) / 2), 1,
+                                                    (unsigned)nb),
+                           dim3(256), 0, st, W, ldw, L.zl_off, L.zt_off, m, e, img, sc, sw, scs);
+    }
     TAGS_HIP(hipGetLastError());
     return 0;
 }

@@ -282,10 +282,13 @@ __global__ __launch_bounds__(512) void k_diag_block(cx<T>* A, long ld, const int
             a[q] = src ? (c < w ? src[q] : cx<T>{0, 0}) : cx<T>{c == r ? T(1) : T(0), T(0)};
         }
     }
-    for (int kb = 0; FACTOR && kb < 4; kb++) {
+    // steps past the block's width would act on identity padding only: skipped
+    // (the width is uniform, so whole steps, barriers included, are skipped)
+    for (int kb = 0; FACTOR && kb < 4 && 16 * kb < w; kb++) {
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
             const int k = kb * 16 + kk, par = k & 1;
+            if (k >= w) break;
             if (r == k && grp == 0) {
 #pragma unroll
                 for (int q = 0; q < 16; q++)
@@ -330,6 +333,7 @@ __global__ __launch_bounds__(512) void k_diag_block(cx<T>* A, long ld, const int
                 // inv(L): row k = 16 kb + kk (top-down); inv(U): row 63 - k (bottom-up)
                 const int kbx = lower ? kb : 3 - kb;
                 const int k = lower ? kb * 16 + kk : 63 - (kb * 16 + kk), par = kk & 1;
+                if (k >= w) continue;  // identity padding: no change
                 // column k of this row (the multiplier L[r][k], or U[r][k] and for
                 // row k itself its pivot) from the quad's thread kbx
                 const cx<T> ak = lower ? a[kk] : a[15 - kk];
