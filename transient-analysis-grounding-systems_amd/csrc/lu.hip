@@ -920,12 +920,12 @@ __device__ inline void flow_store(cx<T>* p, cx<T> v)
 constexpr long kFlowSpinLimit = 1L << 22;
 
 template <typename T, bool UP, int RB>
-__global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A, long ld, const int* perm,
-                                                        cx<T>* y, const cx<T>* inv, TrsvGeom g,
-                                                        int b_lo, int b_hi, unsigned* ctl,
-                                                        int* err, unsigned epoch, long sa,
-                                                        long sp, long sv, long sinv, long sctl,
-                                                        long sf)
+__global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A_, long ld, const int* perm_,
+                                                        cx<T>* y_, const cx<T>* inv_, TrsvGeom g,
+                                                        int b_lo, int b_hi, int nitems,
+                                                        unsigned* ctl, int* err_, unsigned epoch,
+                                                        long sa, long sp, long sv, long sinv,
+                                                        long sctl, long sf)
 {
     static_assert(RB == 1 || RB == 2, "groups of one or two blocks");
     // block columns per chunk of loads (a lane holds 16 KU values of A)
@@ -936,179 +936,188 @@ __global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A, long ld,
     const int tid = threadIdx.x, wv = tid >> 6, lane = tid & 63;
     const int nbk = b_hi - b_lo;
     const int ngrp = (nbk + RB - 1) / RB;
-    const int nitems = (int)gridDim.x / ngrp;
-    // one ticket counter for the launch (item 0's): tickets run group-major
-    // over the items, so the groups that a work-group waits for were handed
-    // out nitems tickets or more before it
-    if (tid == 0) {
-        const unsigned t =
-            __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // the last ticket of the launch: every other work-group has drawn
-        if ((int)t == ngrp * nitems - 1)
-            __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        s_ticket = (int)t;
-        s_fail = 0;
-    }
-    __syncthreads();
-    if (s_ticket >= ngrp * nitems) return;  // not reached: the grid has ngrp x nitems
-    const int gi = s_ticket / nitems;
-    const long z_ = s_ticket - (long)gi * nitems;
-    A += z_ * sa;
-    perm += z_ * sp;
-    y += z_ * sv;
-    inv += z_ * sinv;
-    err += z_ * sf;
-    unsigned* done = ctl + z_ * sctl + 2;
-    // this group's blocks [q0, q1) and rows [r_lo, r_hi)
-    int q0, q1;
-    if (!UP) {
-        q0 = b_lo + RB * gi;
-        q1 = min(q0 + RB, b_hi);
-    } else {
-        q1 = b_hi - RB * gi;
-        q0 = max(q1 - RB, b_lo);
-    }
-    const int nsb = q1 - q0;
-    int r_lo, w0, r_last, w_last;
-    trsv_block(g, q0, r_lo, w0);
-    trsv_block(g, q1 - 1, r_last, w_last);
-    const int r_hi = r_last + w_last;
-    // this wave's rows
-    const int sb = wv >> 2;  // block of the group
-    int sb_row0 = 0, sb_w = 0;
-    if (sb < nsb) trsv_block(g, q0 + sb, sb_row0, sb_w);
-    const int rb0 = 16 * (wv & 3);
-    const cx<T>* rowp[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const bool ok = sb < nsb && rb0 + i < sb_w;
-        rowp[i] = A + (long)perm[ok ? sb_row0 + rb0 + i : r_lo] * ld;
-    }
-    // what the group's own solve needs from memory, loaded before any wait:
-    // the rows of this wave's block inverse, and (RB = 2) this wave's entries
-    // in the columns of the group's other block when that block comes first
-    cx<T> ivr[16], lw[16];
-    const int ob = UP ? nsb - 1 : 0;  // the group's first block in dependency order
-    int ob_r0 = 0, ob_w = 0;
-    if (nsb > 1) trsv_block(g, q0 + ob, ob_r0, ob_w);
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        ivr[i] = sb < nsb ? inv[(size_t)(q0 + sb) * 4096 + (rb0 + i) * 64 + lane] : cx<T>{0, 0};
-        const bool okc = sb < nsb && sb != ob && lane < ob_w;
-        lw[i] = okc ? rowp[i][ob_r0 + lane] : cx<T>{0, 0};
-    }
-    cx<T> part[16];
-#pragma unroll
-    for (int i = 0; i < 16; i++) part[i] = cx<T>{0, 0};
-    // 1. off-diagonal block columns: forward b_lo .. q0 - 1, backward q1 .. b_hi - 1
-    const int u_first = UP ? b_hi - 1 : b_lo;
-    const int nu = UP ? b_hi - q1 : q0 - b_lo;
-    int known = -1;  // highest group index seen published
-    for (int c0 = 0; c0 < nu; c0 += KU) {
-        cx<T> a[KU][16];
-        int ucol[KU], uw[KU];
-#pragma unroll
-        for (int c = 0; c < KU; c++) {
-            const int u = UP ? u_first - (c0 + c) : u_first + (c0 + c);
-            const bool in = c0 + c < nu;
-            int r0u = 0, wu = 0;
-            if (in) trsv_block(g, u, r0u, wu);
-            ucol[c] = r0u;
-            uw[c] = wu;
-            const bool okc = lane < wu;
-#pragma unroll
-            for (int i = 0; i < 16; i++) a[c][i] = rowp[i][r0u + (okc ? lane : 0)];
+    const int total = ngrp * nitems;
+    const int nwg = (int)gridDim.x;
+    // One ticket counter for the launch (item 0's), tickets group-major over
+    // the items, so the groups a work-group waits for were handed out nitems
+    // tickets or more before. Each work-group takes groups in turn until it
+    // draws a ticket past the last: a few work-groups stream the later
+    // groups' blocks while the chain moves on, instead of one resident
+    // work-group per group waiting for its turn.
+    for (;;) {
+        if (tid == 0) {
+            const unsigned t =
+                __hip_atomic_fetch_add(&ctl[0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            // every work-group draws exactly one ticket past the last; the last of
+            // those draws resets the counter for the next launch
+            if ((int)t == total + nwg - 1)
+                __hip_atomic_store(&ctl[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            s_ticket = (int)t;
+            s_fail = 0;
         }
-        // wait for the group owning the chunk's latest block
-        const int last = min(c0 + KU, nu) - 1;
-        const int need = last / RB;
-        if (need > known) {
-            long spins = 0;
-            while (__hip_atomic_load(&done[need], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
-                   epoch) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > kFlowSpinLimit) {
-                    if (lane == 0) {
-                        atomicExch(err, 1);
-                        s_fail = 1;
+        __syncthreads();
+        if (s_ticket >= total) return;
+        const int gi = s_ticket / nitems;
+        const long z_ = s_ticket - (long)gi * nitems;
+        const cx<T>* A = A_ + z_ * sa;
+        const int* perm = perm_ + z_ * sp;
+        cx<T>* y = y_ + z_ * sv;
+        const cx<T>* inv = inv_ + z_ * sinv;
+        int* err = err_ + z_ * sf;
+        unsigned* done = ctl + z_ * sctl + 2;
+        // this group's blocks [q0, q1) and rows [r_lo, r_hi)
+        int q0, q1;
+        if (!UP) {
+            q0 = b_lo + RB * gi;
+            q1 = min(q0 + RB, b_hi);
+        } else {
+            q1 = b_hi - RB * gi;
+            q0 = max(q1 - RB, b_lo);
+        }
+        const int nsb = q1 - q0;
+        int r_lo, w0, r_last, w_last;
+        trsv_block(g, q0, r_lo, w0);
+        trsv_block(g, q1 - 1, r_last, w_last);
+        const int r_hi = r_last + w_last;
+        // this wave's rows
+        const int sb = wv >> 2;  // block of the group
+        int sb_row0 = 0, sb_w = 0;
+        if (sb < nsb) trsv_block(g, q0 + sb, sb_row0, sb_w);
+        const int rb0 = 16 * (wv & 3);
+        const cx<T>* rowp[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            const bool ok = sb < nsb && rb0 + i < sb_w;
+            rowp[i] = A + (long)perm[ok ? sb_row0 + rb0 + i : r_lo] * ld;
+        }
+        // what the group's own solve needs from memory, loaded before any wait:
+        // the rows of this wave's block inverse, and (RB = 2) this wave's entries
+        // in the columns of the group's other block when that block comes first
+        cx<T> ivr[16], lw[16];
+        const int ob = UP ? nsb - 1 : 0;  // the group's first block in dependency order
+        int ob_r0 = 0, ob_w = 0;
+        if (nsb > 1) trsv_block(g, q0 + ob, ob_r0, ob_w);
+#pragma unroll
+        for (int i = 0; i < 16; i++) {
+            ivr[i] = sb < nsb ? inv[(size_t)(q0 + sb) * 4096 + (rb0 + i) * 64 + lane] : cx<T>{0, 0};
+            const bool okc = sb < nsb && sb != ob && lane < ob_w;
+            lw[i] = okc ? rowp[i][ob_r0 + lane] : cx<T>{0, 0};
+        }
+        cx<T> part[16];
+#pragma unroll
+        for (int i = 0; i < 16; i++) part[i] = cx<T>{0, 0};
+        // 1. off-diagonal block columns: forward b_lo .. q0 - 1, backward q1 .. b_hi - 1
+        const int u_first = UP ? b_hi - 1 : b_lo;
+        const int nu = UP ? b_hi - q1 : q0 - b_lo;
+        int known = -1;  // highest group index seen published
+        for (int c0 = 0; c0 < nu; c0 += KU) {
+            cx<T> a[KU][16];
+            int ucol[KU], uw[KU];
+#pragma unroll
+            for (int c = 0; c < KU; c++) {
+                const int u = UP ? u_first - (c0 + c) : u_first + (c0 + c);
+                const bool in = c0 + c < nu;
+                int r0u = 0, wu = 0;
+                if (in) trsv_block(g, u, r0u, wu);
+                ucol[c] = r0u;
+                uw[c] = wu;
+                const bool okc = lane < wu;
+#pragma unroll
+                for (int i = 0; i < 16; i++) a[c][i] = rowp[i][r0u + (okc ? lane : 0)];
+            }
+            // wait for the group owning the chunk's latest block
+            const int last = min(c0 + KU, nu) - 1;
+            const int need = last / RB;
+            if (need > known) {
+                long spins = 0;
+                while (__hip_atomic_load(&done[need], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) !=
+                       epoch) {
+                    __builtin_amdgcn_s_sleep(1);
+                    if (++spins > kFlowSpinLimit) {
+                        if (lane == 0) {
+                            atomicExch(err, 1);
+                            s_fail = 1;
+                        }
+                        break;
                     }
-                    break;
+                }
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+                known = need;
+            }
+#pragma unroll
+            for (int c = 0; c < KU; c++) {
+                const bool okc = c0 + c < nu && lane < uw[c];
+                const cx<T> xv = flow_load(y + ucol[c] + (okc ? lane : 0));
+#pragma unroll
+                for (int i = 0; i < 16; i++) {
+                    const cx<T> p = cmul(a[c][i], xv);
+                    part[i] = csel(okc, cadd(part[i], p), part[i]);
                 }
             }
-            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-            known = need;
         }
+        // right-hand side less the off-diagonal sums, into LDS
 #pragma unroll
-        for (int c = 0; c < KU; c++) {
-            const bool okc = c0 + c < nu && lane < uw[c];
-            const cx<T> xv = flow_load(y + ucol[c] + (okc ? lane : 0));
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const cx<T> p = cmul(a[c][i], xv);
-                part[i] = csel(okc, cadd(part[i], p), part[i]);
-            }
+        for (int i = 0; i < 16; i++) {
+            const cx<T> s = wave_sum_cx<T>(part[i]);
+            const int r = rb0 + i;
+            if (lane == 0 && sb < nsb)
+                acc[64 * sb + r] = r < sb_w ? csub(y[sb_row0 + r], s) : cx<T>{0, 0};
         }
-    }
-    // right-hand side less the off-diagonal sums, into LDS
-#pragma unroll
-    for (int i = 0; i < 16; i++) {
-        const cx<T> s = wave_sum_cx<T>(part[i]);
-        const int r = rb0 + i;
-        if (lane == 0 && sb < nsb)
-            acc[64 * sb + r] = r < sb_w ? csub(y[sb_row0 + r], s) : cx<T>{0, 0};
-    }
-    // 2. the group's own blocks, in dependency order (RB <= 2: the later
-    // block takes the first one's contribution)
-    for (int ss = 0; ss < nsb; ss++) {
-        const int tb = UP ? nsb - 1 - ss : ss;
-        __syncthreads();
-        // x_tb = inv_tb acc_tb (identity padding past the block's width), by
-        // the block's own waves
-        if (sb == tb) {
-            cx<T> pr[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) pr[i] = cmul(ivr[i], acc[64 * tb + lane]);
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                const cx<T> s = wave_sum_cx<T>(pr[i]);
-                if (lane == 0) xs[64 * tb + rb0 + i] = s;
-            }
-        }
-        if (ss + 1 < nsb) {
+        // 2. the group's own blocks, in dependency order (RB <= 2: the later
+        // block takes the first one's contribution)
+        for (int ss = 0; ss < nsb; ss++) {
+            const int tb = UP ? nsb - 1 - ss : ss;
             __syncthreads();
-            if (sb < nsb && sb != ob) {
-                const bool okc = lane < ob_w;
-                const cx<T> xv = xs[64 * ob + (okc ? lane : 0)];
+            // x_tb = inv_tb acc_tb (identity padding past the block's width), by
+            // the block's own waves
+            if (sb == tb) {
                 cx<T> pr[16];
 #pragma unroll
-                for (int i = 0; i < 16; i++) pr[i] = csel(okc, cmul(lw[i], xv), cx<T>{0, 0});
+                for (int i = 0; i < 16; i++) pr[i] = cmul(ivr[i], acc[64 * tb + lane]);
 #pragma unroll
                 for (int i = 0; i < 16; i++) {
                     const cx<T> s = wave_sum_cx<T>(pr[i]);
-                    if (lane == 0) acc[64 * sb + rb0 + i] = csub(acc[64 * sb + rb0 + i], s);
+                    if (lane == 0) xs[64 * tb + rb0 + i] = s;
+                }
+            }
+            if (ss + 1 < nsb) {
+                __syncthreads();
+                if (sb < nsb && sb != ob) {
+                    const bool okc = lane < ob_w;
+                    const cx<T> xv = xs[64 * ob + (okc ? lane : 0)];
+                    cx<T> pr[16];
+#pragma unroll
+                    for (int i = 0; i < 16; i++) pr[i] = csel(okc, cmul(lw[i], xv), cx<T>{0, 0});
+#pragma unroll
+                    for (int i = 0; i < 16; i++) {
+                        const cx<T> s = wave_sum_cx<T>(pr[i]);
+                        if (lane == 0) acc[64 * sb + rb0 + i] = csub(acc[64 * sb + rb0 + i], s);
+                    }
                 }
             }
         }
-    }
-    __syncthreads();
-    // 3. publish
-    if (wv == 0) {
-        const bool bad = s_fail != 0;
-        for (int r = lane; r < r_hi - r_lo; r += 64) {
-            // rows of the group are contiguous: find the block of row r_lo + r
-            int rr = r, tb = 0, tw = w0;
-            while (tb + 1 < nsb && rr >= tw) {
-                rr -= tw;
-                tb++;
-                int t0;
-                trsv_block(g, q0 + tb, t0, tw);
+        __syncthreads();
+        // 3. publish
+        if (wv == 0) {
+            const bool bad = s_fail != 0;
+            for (int r = lane; r < r_hi - r_lo; r += 64) {
+                // rows of the group are contiguous: find the block of row r_lo + r
+                int rr = r, tb = 0, tw = w0;
+                while (tb + 1 < nsb && rr >= tw) {
+                    rr -= tw;
+                    tb++;
+                    int t0;
+                    trsv_block(g, q0 + tb, t0, tw);
+                }
+                cx<T> v = xs[64 * tb + rr];
+                if (bad) v = cx<T>{(T)NAN, (T)NAN};
+                flow_store(y + r_lo + r, v);
             }
-            cx<T> v = xs[64 * tb + rr];
-            if (bad) v = cx<T>{(T)NAN, (T)NAN};
-            flow_store(y + r_lo + r, v);
+            if (lane == 0)
+                __hip_atomic_store(&done[gi], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
         }
-        if (lane == 0)
-            __hip_atomic_store(&done[gi], epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // the group's LDS and ticket are free again only after every wave is done
+        __syncthreads();
     }
 }
 
@@ -2359,9 +2368,12 @@ static void flow_launch(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStre
 {
     const BatchStrides b = w.bs();
     const int ngrp = (b_hi - b_lo + RB - 1) / RB;
-    hipLaunchKernelGGL((k_trsv_flow<T, UP, RB>), dim3(ngrp * b.nb), dim3(256 * RB), 0, st, A,
-                       ld, (const int*)w.perm, y, (const cx<T>*)(UP ? w.invU : w.invL), w.geom,
-                       b_lo, b_hi, w.tctl, w.flags + 1, epoch, b.a, b.p, b.v, b.inv,
+    // 16 work-groups per item take the groups in turn (enough to stream ahead
+    // of the chain), at most one per group
+    const int nwg = std::min(ngrp * b.nb, 16 * b.nb);
+    hipLaunchKernelGGL((k_trsv_flow<T, UP, RB>), dim3(nwg), dim3(256 * RB), 0, st, A, ld,
+                       (const int*)w.perm, y, (const cx<T>*)(UP ? w.invU : w.invL), w.geom, b_lo,
+                       b_hi, b.nb, w.tctl, w.flags + 1, epoch, b.a, b.p, b.v, b.inv,
                        w.tctl_stride, b.f);
 }
 
