@@ -724,6 +724,9 @@ def main():
                        "dense_equivalent_tflops": lu_flops(N) * K / dt64 / 1e12}
         out["mixed_over_fp64"] = (K / dt_mx) / (K / dt64)
         out["max_rel_err_zh_vs_fp64"] = rel
+        # the error of every timed sample, not only the largest
+        out["rel_err_zh_vs_fp64_per_sample"] = [float(v) for v in
+                                                np.abs(u_mx - u64) / np.abs(u64)]
     ach = tm_mx["trailing_flops"] / tm_mx["trailing_ms"] / 1e9 if tm_mx["trailing_ms"] > 0 else 0.0
     out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"],
                        "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],

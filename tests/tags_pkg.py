@@ -30,6 +30,7 @@ C_DRIVER_DIR = os.path.join(ROOT, "tests", "c_driver")
 C_DRIVER_SRC = os.path.join(C_DRIVER_DIR, "harmonic_c1.c")
 C_DRIVER_BIN = os.path.join(C_DRIVER_DIR, "harmonic_c1")
 C_ADM_DRIVER_BIN = os.path.join(C_DRIVER_DIR, "admittance_c1")
+C_SWEEP_DRIVER_BIN = os.path.join(C_DRIVER_DIR, "sweep_c2")
 
 
 def c_driver_command(out=C_DRIVER_BIN, src=C_DRIVER_SRC):
@@ -53,6 +54,7 @@ def build_c_driver(out=C_DRIVER_BIN, src=C_DRIVER_SRC):
 def build_c_drivers():
     build_c_driver()
     build_c_driver(C_ADM_DRIVER_BIN, os.path.join(C_DRIVER_DIR, "admittance_c1.c"))
+    build_c_driver(C_SWEEP_DRIVER_BIN, os.path.join(C_DRIVER_DIR, "sweep_c2.c"))
 
 
 def cases():

@@ -98,3 +98,35 @@ def test_cases_autotune_by_size_class(tmp_path):
     cl2 = cs.autotune(T.PREC_MIXED, cache=cache)
     assert [t["from_cache"] for t in cl2] == [1] * len(cl)
     cs.close()
+
+
+def test_c_driver_sweep_with_autotune(tmp_path):
+    """The sweep and its autotuner from plain C (tests/c_driver/sweep_c2.c,
+    built by build()): 40 frequencies of the 20 x 20 m grid in both
+    precisions agree within 1e-8 and with the Python wrapper's FP64 sweep to
+    rounding; a second run reads both tunings from the cache it wrote."""
+    import subprocess
+    exe = tags_pkg.C_SWEEP_DRIVER_BIN
+    cache = str(tmp_path / "tuning.txt")
+    outs = []
+    for run in range(2):
+        r = subprocess.run([exe, cache, "40"], capture_output=True, text=True, timeout=240)
+        assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-2000:])
+        outs.append(r.stdout.splitlines())
+    for run, lines in enumerate(outs):
+        tl = [l.split() for l in lines if l.startswith("tuning")]
+        assert len(tl) == 2 and all(int(t[5]) == run for t in tl), tl
+    rows = np.array([[float(v) for v in l.split()] for l in outs[1] if not l.startswith("tuning")])
+    f = rows[:, 0]
+    zm = rows[:, 1] + 1j * rows[:, 2]
+    z64 = rows[:, 3] + 1j * rows[:, 4]
+    assert np.all(rows[:, 5] == 1)
+    d = np.max(np.abs(zm - z64) / np.abs(z64))
+    rc, e, nodes = T.electrode_grid(5, 5, 20.0, 20.0, 10, 10, 6e-3, -0.5)
+    s = 2j * np.pi * f
+    sw = T.Sweep(e, nodes)
+    u64, _ = sw.solve(s, 1e-3 + s * 15.0 * T.EPS0, 0, T.PREC_FP64)
+    sw.close()
+    dp = np.max(np.abs(z64 - u64) / np.abs(u64))
+    print("C driver: mixed vs FP64 %.2e, FP64 vs Python %.2e" % (d, dp), flush=True)
+    assert d <= 1e-8 and dp <= 1e-12
