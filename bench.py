@@ -44,7 +44,13 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 METRIC = "freq samples/s, 60×60 m grid: FP64 vs FP32+refine at ≤1e-8 rel GPR err"
-PEAK = {"fp64": 78.6, "fp32": 157.3}  # dense MFMA TFLOP/s, MI355X
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI355X
+# complex64 GEMM variants that run on the bf16 MFMA cores (every fp32 operand
+# as three bf16 parts, six bf16 part products per real product), and the
+# library's default large-product variants
+X6_VARIANTS = range(22, 29)
+X6_PARTS_PRODUCTS = 6
+DEFAULT_BIG = {"mixed": 24, "fp64": 13}
 GRID = (31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)
 SIGMA, EPSR = 1.0 / 500.0, 20.0
 NFREQ = 256
@@ -135,14 +141,16 @@ def copy_calibration(mib):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_update_f32_grid60_traffic.txt")
+TRAFFIC_FILE_X6 = os.path.join(ROOT, "profiles", "trailing_update_x6_grid60_traffic.txt")
 
 
-def measured_traffic():
-    """HBM bytes per launch of the trailing complex64 GEMM, from the committed
-    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (scaled by a copy kernel), or
-    None when that summary is absent."""
+def measured_traffic(x6=False):
+    """HBM bytes per launch of the trailing complex64 GEMM (x6: the split-bf16
+    product, its two operand packs included), from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summary (scaled by a copy kernel), or None when
+    that summary is absent."""
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(TRAFFIC_FILE_X6 if x6 else TRAFFIC_FILE) as f:
             for line in f:
                 if line.startswith("traffic_bytes_per_launch"):
                     return float(line.split("=")[1])
@@ -286,16 +294,38 @@ def _timed_solve(T, obj, prec, *args):
     return r, time.perf_counter() - t0
 
 
-def _roofline(tm, peak, kernel):
+def gemm_basis(mixed, tuning):
+    """(kernel label, peak TFLOP/s, what the peak is) of the trailing update's
+    large products under a tuning (a dict with gemm_big, or the defaults)."""
+    big = tuning.get("gemm_big", -1) if isinstance(tuning, dict) else -1
+    if big < 0:
+        big = DEFAULT_BIG["mixed" if mixed else "fp64"]
+    if mixed and big in X6_VARIANTS:
+        return ("k_trailing_update_x6 (complex64 as bf16 parts, 24 bf16 MFMAs per complex "
+                "product), variant %d" % big, PEAK["bf16"] / X6_PARTS_PRODUCTS,
+                "bf16 dense MFMA peak / 6 (bf16 part products per fp32-accurate real product)")
+    if mixed:
+        return ("k_trailing_update<float, 3-multiply>, variant %d" % big, PEAK["fp32"],
+                "fp32 dense MFMA peak")
+    return "k_trailing_update<double>, variant %d" % big, PEAK["fp64"], "fp64 dense MFMA peak"
+
+
+def _roofline(tm, basis):
     """Trailing-update GEMM of a profiled pass: algorithmic 8MNK flops over the
-    HIP-event time of its launches."""
+    HIP-event time of its launches, against the peak of the arithmetic the
+    large products run on (gemm_basis)."""
     if not tm or tm["trailing_launches"] <= 0 or tm["trailing_ms"] <= 0:
         return None
+    kernel, peak, what = basis
     ach = tm["trailing_flops"] / tm["trailing_ms"] / 1e9
-    return {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
-            "frac": ach / peak, "traffic": None, "kernel": kernel,
-            "launches": tm["trailing_launches"],
-            "avg_launch_ms": tm["trailing_ms"] / tm["trailing_launches"]}
+    r = {"bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s",
+         "frac": ach / peak, "traffic": None, "kernel": kernel, "peak_basis": what,
+         "launches": tm["trailing_launches"],
+         "avg_launch_ms": tm["trailing_ms"] / tm["trailing_launches"]}
+    if peak != PEAK["fp32"] and "bf16" in what:
+        # the same rate against the fp32 MFMA peak the complex64 product had
+        r["frac_of_fp32_mfma_peak"] = ach / PEAK["fp32"]
+    return r
 
 
 def probe_step(N):
@@ -326,10 +356,7 @@ def sweep_config(T, e, nodes, inj, s, kap, plan=7, tuning=("table", TUNING_TABLE
         sw.set_profiling(False)
         us[name] = u
         r = {"samples_per_s": len(s) / dt, "s": dt,
-             "roofline": _roofline(tm, PEAK["fp32" if prec == T.PREC_MIXED else "fp64"],
-                                   "k_trailing_update<%s>" % ("float, 3-multiply"
-                                                              if prec == T.PREC_MIXED
-                                                              else "double")),
+             "roofline": _roofline(tm, gemm_basis(prec == T.PREC_MIXED, tu)),
              "executed_gemm_flops_per_sample": tm["gemm_flops"] / len(s),
              "tuning": tu}
         if prec == T.PREC_MIXED:
@@ -456,8 +483,9 @@ def cases_config(T, threads, step=8, tuning=("table", TUNING_TABLE)):
         cs.set_profiling(False)
         us[name] = u
         r = {"samples_per_s": n / dt, "s": dt,
-             "roofline": _roofline(tm, PEAK["fp32" if prec == T.PREC_MIXED else "fp64"],
-                                   "k_trailing_update"),
+             "roofline": _roofline(tm, gemm_basis(prec == T.PREC_MIXED,
+                                                  tu[-1] if isinstance(tu, list) and tu
+                                                  else tu)),
              "dense_equivalent_tflops": sum(len(c["s"]) * lu_flops(c["N"]) for c in fam) / dt
              / 1e12,
              "tuning_classes": tu}
@@ -550,6 +578,9 @@ def main():
                          "pair, fields) instead of the sweep; prints its own JSON line")
     ap.add_argument("--no-lookahead", action="store_true",
                     help="factor the steps strictly in order on one stream")
+    ap.add_argument("--gemm-mixed", type=int, default=None, metavar="VARIANT",
+                    help="complex64 GEMM variant for large products in place of the tuned "
+                         "one (22-24: split-bf16 MFMA)")
     ap.add_argument("--tuning", choices=("table", "measure", "off"), default="table",
                     help="tile shapes and schedule per sweep and precision: from the "
                          "tuning table shipped with the package (default; a missing entry "
@@ -621,6 +652,11 @@ def main():
         sw.set_inflight(args.inflight)
         if prec not in tuned:
             tuned[prec] = apply_tuning(T, sw, prec, s_probe, kap_probe, 0, args.tuning, cache)
+        if mixed and args.gemm_mixed is not None:
+            base = tuned[prec] if isinstance(tuned[prec], dict) else sw.tuning(prec)
+            if base is None:
+                raise SystemExit("--gemm-mixed needs a tuning (table or measure)")
+            tuned[prec] = dict(base, gemm_big=args.gemm_mixed)
         tu = tuned[prec]
         inflight = args.inflight
         if isinstance(tu, dict):
@@ -728,18 +764,19 @@ def main():
         out["rel_err_zh_vs_fp64_per_sample"] = [float(v) for v in
                                                 np.abs(u_mx - u64) / np.abs(u64)]
     ach = tm_mx["trailing_flops"] / tm_mx["trailing_ms"] / 1e9 if tm_mx["trailing_ms"] > 0 else 0.0
-    out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": PEAK["fp32"],
-                       "unit": "TFLOP/s", "frac": ach / PEAK["fp32"],
-                       "traffic": measured_traffic(),
-                       "kernel": "k_trailing_update<float, 3-multiply> (A22 -= L21 U12, K=%d), tile "
-                                 "variant %s" % (width_of(T.PREC_MIXED),
-                                                 tuned[T.PREC_MIXED].get("gemm_big", "default")
-                                                 if isinstance(tuned[T.PREC_MIXED], dict)
-                                                 else "default"),
+    kernel, peak, what = gemm_basis(True, tuned[T.PREC_MIXED])
+    x6 = "bf16" in what
+    out["roofline"] = {"bound": "mfma", "achieved": ach, "peak": peak,
+                       "unit": "TFLOP/s", "frac": ach / peak, "peak_basis": what,
+                       "traffic": measured_traffic(x6),
+                       "kernel": "%s (A22 -= L21 U12, K=%d); products under 512 tiles: the "
+                                 "small variant" % (kernel, width_of(T.PREC_MIXED)),
                        "launches": tm_mx["trailing_launches"],
                        "avg_launch_ms": tm_mx["trailing_ms"] / max(1, tm_mx["trailing_launches"]),
                        "measured_in": ("the timed samples" if one else
                                        "the same %d samples timed one at a time" % K)}
+    if x6:
+        out["roofline"]["frac_of_fp32_mfma_peak"] = ach / PEAK["fp32"]
     # F(N) = (8/3) N^3 + 8 N^2 per sample counts the dense LU, including the
     # exactly-zero blocks the plans skip: a dense-equivalent rate, not a
     # hardware rate; the executed one counts the products the plans ran

@@ -572,6 +572,39 @@ def test_sweep_inflight_agree(plan):
     assert np.array_equal(again, res[3][1][0])
 
 
+def test_gemm_split_bf16_as_accurate_as_f32():
+    """The complex64 products on the bf16 MFMA cores (variants 22-28: every
+    fp32 value as three bf16 parts, six part products per real product) carry
+    the accuracy of the fp32 three-multiply tiles (variant 19): at K = 256 (the
+    LU's step) and entries spread over six decades, the largest error against
+    complex128 numpy on the same fp32 operands, relative to |A| |B| + |C|, is
+    within twice variant 19's (about 1e-6 here, K fp32 roundings); ragged M,
+    N, K too."""
+    rng = np.random.default_rng(11)
+
+    def spread(shape):
+        mag = 10.0 ** rng.uniform(-3, 3, shape)
+        return ((rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) * mag
+                ).astype(np.complex64)
+
+    for (M, N, K) in [(300, 200, 256), (129, 65, 29), (640, 384, 256)]:
+        a, b, c = spread((M, K)), spread((K, N)), spread((M, N))
+        ar = rng.permutation(M)
+        br = rng.permutation(K)
+        cr = rng.permutation(M)
+        ref = c.astype(np.complex128).copy()
+        ref[cr] -= a[ar].astype(np.complex128) @ b[br].astype(np.complex128)
+        bound = np.zeros((M, N))
+        bound[cr] = np.abs(a[ar]).astype(np.float64) @ np.abs(b[br]).astype(np.float64)
+        bound += np.abs(c)
+        errs = {}
+        for v in [19] + list(range(22, 29)):
+            out = T.gemm_run(a, b, c, 1, v, ar, br, cr, 1)
+            errs[v] = float((np.abs(out - ref) / bound).max())
+        for v in range(22, 29):
+            assert errs[v] <= max(2.0 * errs[19], 1e-7) and errs[v] < 1e-5, (M, N, K, errs)
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_gemm_variants_vs_numpy(prec):
     """Every MFMA GEMM variant of a precision, through tags_gemm_run, against

@@ -673,6 +673,21 @@ __global__ void k_to_f32(const double* a, float* b, long n)
     if (i < n) b[i] = (float)a[i];
 }
 
+// The split-bf16 complex64 variants pack their operands into a work space:
+// give a stand-alone product one of its own.
+template <typename T>
+static int gemm_ws_attach(GemmArgs<T>& g, DevBuf<unsigned char>& ws)
+{
+    if (sizeof(T) != sizeof(float)) return 0;
+    const long bytes = gemm_x6_ws_bytes(g.M, g.N, g.K);
+    if (bytes <= 0) return 0;
+    int rc;
+    if ((rc = ws.alloc((size_t)bytes))) return rc;
+    g.ws = ws.p;
+    g.ws_item = bytes;
+    return 0;
+}
+
 // Times C -= A B (M x N x K, row-major, a random row map on A and C like the
 // LU trailing update) for one variant; returns the mean kernel time in ms.
 template <typename T>
@@ -713,6 +728,8 @@ static int gemm_bench_t(int M, int N, int K, int variant, int iters, double* ms)
     for (int i = 0; i < K; i++) hk[i] = K - 1 - i;
     TAGS_HIP(hipMemcpyAsync(krows.p, hk.data(), sizeof(int) * K, hipMemcpyHostToDevice, S.s));
     GemmArgs<T> g{M, N, K, A.p, K, rows.p, B.p, N, krows.p, C.p, N, rows.p, 1};
+    DevBuf<unsigned char> ws;
+    if ((rc = gemm_ws_attach(g, ws))) return rc;
     if ((rc = launch_gemm_variant<T>(g, variant, S.s))) return rc;  // warm-up
     hipEvent_t e0, e1;
     TAGS_HIP(hipEventCreate(&e0));
@@ -762,6 +779,8 @@ static int gemm_run_t(int M, int N, int K, int variant, const c64* A, const c64*
     if (crow) TAGS_HIP(hipMemcpyAsync(m3.p, crow, sizeof(int) * M, hipMemcpyHostToDevice, S.s));
     GemmArgs<T> g{M, N, K, dA.p, K, arow ? m1.p : nullptr, dB.p, N, brow ? m2.p : nullptr,
                   dC.p, N, crow ? m3.p : nullptr, mode ? 1 : 0};
+    DevBuf<unsigned char> ws;
+    if ((rc = gemm_ws_attach(g, ws))) return rc;
     if ((rc = launch_gemm_variant<T>(g, variant, S.s))) return rc;
     TAGS_HIP(hipMemcpyAsync(hc.data(), dC.p, sizeof(cx<T>) * nc, hipMemcpyDeviceToHost, S.s));
     TAGS_HIP(hipStreamSynchronize(S.s));

@@ -233,7 +233,20 @@ __device__ __forceinline__ void cgemm_block(const GemmArgs<T>& g)
                     re = acc0[i][j][v];
                     im = acc1[i][j][v];
                 }
-                if (g.mode == 0) {
+                if constexpr ((AL & 2) != 0) {
+                    // non-temporal C: streamed, so that it does not evict the
+                    // operand tiles other work-groups still read from L2
+                    T* f = (T*)(crow + col);
+                    if (g.mode == 0) {
+                        __builtin_nontemporal_store(re, f);
+                        __builtin_nontemporal_store(im, f + 1);
+                    } else {
+                        const T cr = __builtin_nontemporal_load(f);
+                        const T ci = __builtin_nontemporal_load(f + 1);
+                        __builtin_nontemporal_store(cr - re, f);
+                        __builtin_nontemporal_store(ci - im, f + 1);
+                    }
+                } else if (g.mode == 0) {
                     crow[col] = cx<T>{re, im};
                 } else {
                     const cx<T> c = crow[col];
@@ -440,6 +453,556 @@ static int launch_v32(const GemmArgs<float>& g, hipStream_t st, bool trailing)
 }
 
 // ---------------------------------------------------------------------------
+// complex64 on the bf16 MFMA cores (variants 22..28). Every fp32 operand value
+// x is split into three bf16 parts x = x1 + x2 + x3, each the round-to-nearest
+// bf16 of what the earlier parts leave: x1 holds the leading 8 significant
+// bits, x2 the next 8, x3 the last 8, so that the parts sum to x exactly (for
+// |x| well inside the normal range; the differences are exact in fp32). A real
+// product x y is formed from the six part products x1y1, x1y2, x2y1, x1y3,
+// x2y2, x3y1, accumulated in fp32; the three left out (x2y3, x3y2, x3y3) add up
+// to less than 2^-24 |x||y|, the size of one fp32 rounding of the term.
+// Complex products take four real ones (re re - im im, re im + im re): 24 bf16
+// MFMAs where the three-multiply fp32 form issues 3 fp32 ones, and bf16 runs at
+// 16x the fp32 MFMA rate on gfx950, so a complex product takes 24 / 16 = 1.5
+// fp32-MFMA issue slots instead of 3.
+//
+// The operands are first packed (k_pack_x6) into MFMA fragment order: for a
+// block of 32 rows of A (32 columns of B) and a slab of 16 k, part p (the
+// three parts of the real values, then of the imaginary ones) is 64 lanes x 8
+// bf16 = 1 KiB, lane l holding row (column) l % 32 at k = 8 (l / 32) .. + 7,
+// zero past the matrix. A wave loads each fragment with one 16-byte load per
+// lane straight into MFMA operand registers, the next slab's while the
+// current one's MFMAs run; blocks past the product's edge repeat the last
+// block and their results are dropped at the store.
+// v_mfma_f32_32x32x16_bf16: lane l supplies A[l % 32][8 (l / 32) + e] and
+// B[8 (l / 32) + e][l % 32], e = 0..7 (A and B packed alike, so the k order
+// inside a slab is the same on both sides, which is all the sum needs); result
+// value v of lane l is D[(v % 4) + 8 (v / 4) + 4 (l / 32)][l % 32], as for the
+// 32 x 32 f32 blocks above.
+// ---------------------------------------------------------------------------
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int X6_KMAX = 256;  // largest K (the LU's widest step)
+constexpr int X6_FRAG = 64;   // 16-byte words per fragment
+constexpr int X6_PARTS = 6;
+
+// 16-byte words of one item's pack of R rows (columns) and K
+static inline long x6_pack_words(int R, int K)
+{
+    return (long)((R + 31) / 32) * ((K + 15) / 16) * X6_PARTS * X6_FRAG;
+}
+
+long gemm_x6_ws_bytes(int M, int N, int K)
+{
+    if (M <= 0 || N <= 0 || K <= 0 || K > X6_KMAX) return 0;
+    return 16 * (x6_pack_words(M, K) + x6_pack_words(N, K));
+}
+
+// round to nearest even bf16 (finite x), as the upper 16 bits of an fp32
+__device__ inline unsigned bf16_rn(float x)
+{
+    unsigned u = __float_as_uint(x);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+}
+__device__ inline void x6_split(float x, unsigned& h1, unsigned& h2, unsigned& h3)
+{
+    h1 = bf16_rn(x);
+    const float r1 = x - __uint_as_float(h1 << 16);
+    h2 = bf16_rn(r1);
+    const float r2 = r1 - __uint_as_float(h2 << 16);
+    h3 = bf16_rn(r2);
+}
+
+// SIDE 0, rows of A: X(i, k) = X[(map ? map[i] : i) * ld + k], i < R;
+// SIDE 1, columns of B: X(k, i) = X[(map ? map[k] : k) * ld + i], i < R.
+// One thread per (block, slab, lane); item blockIdx.y. TR: the trailing
+// update's packs (their own name in kernel traces).
+template <int SIDE, bool TR>
+__global__ __launch_bounds__(256) void k_pack_x6(const c32* X, long ld, const int* map, int R,
+                                                  int K, u32x4* out, long sx, long smap,
+                                                  long sout)
+{
+    const int KS = (K + 15) / 16;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long)((R + 31) / 32) * KS * 64) return;
+    const long z = blockIdx.y;
+    X += z * sx;
+    if (map) map += z * smap;
+    out += z * sout;
+    const int lane = (int)(t & 63);
+    const long f = t >> 6;  // block * KS + slab
+    const int ks = (int)(f % KS), blk = (int)(f / KS);
+    const int i = blk * 32 + (lane & 31);
+    const int k0 = ks * 16 + 8 * (lane >> 5);
+    unsigned w[X6_PARTS][4];
+#pragma unroll
+    for (int e = 0; e < 8; e += 2) {
+        float v[2][2];  // [element][re, im]
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int k = k0 + e + q;
+            c32 x{0, 0};
+            if (i < R && k < K) {
+                const long off = SIDE == 0 ? (long)(map ? map[i] : i) * ld + k
+                                           : (long)(map ? map[k] : k) * ld + i;
+                x = X[off];
+            }
+            v[q][0] = x.x;
+            v[q][1] = x.y;
+        }
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            unsigned a1, a2, a3, b1, b2, b3;
+            x6_split(v[0][c], a1, a2, a3);
+            x6_split(v[1][c], b1, b2, b3);
+            w[3 * c + 0][e / 2] = a1 | (b1 << 16);
+            w[3 * c + 1][e / 2] = a2 | (b2 << 16);
+            w[3 * c + 2][e / 2] = a3 | (b3 << 16);
+        }
+    }
+    u32x4* o = out + f * (X6_PARTS * X6_FRAG) + lane;
+#pragma unroll
+    for (int p = 0; p < X6_PARTS; p++) {
+        u32x4 q;
+        q.x = w[p][0];
+        q.y = w[p][1];
+        q.z = w[p][2];
+        q.w = w[p][3];
+        o[p * X6_FRAG] = q;
+    }
+}
+
+struct X6Args {
+    int M, N, K;
+    const u32x4* pa;  // packed A rows
+    const u32x4* pb;  // packed B columns
+    long spk;         // item stride of the packs (16-byte words)
+    c32* C;
+    long ldc;
+    const int* crow;
+    long sc, scr;
+    int mode;  // 0: C = A B; 1: C -= A B
+};
+
+__device__ __forceinline__ bf16x8 as_bf16(u32x4 u) { return __builtin_bit_cast(bf16x8, u); }
+__device__ __forceinline__ f32x16 mfma_x6(u32x4 a, u32x4 b, f32x16 c)
+{
+    return __builtin_amdgcn_mfma_f32_32x32x16_bf16(as_bf16(a), as_bf16(b), c, 0, 0, 0);
+}
+
+// C = v (mode 0) or C -= v (mode 1) for one complex64 entry
+template <bool NTC>
+__device__ __forceinline__ void x6_store(c32* p, float re, float im, int mode)
+{
+    float* f = (float*)p;
+    if constexpr (NTC) {
+        if (mode == 0) {
+            __builtin_nontemporal_store(re, f);
+            __builtin_nontemporal_store(im, f + 1);
+        } else {
+            const float cr = __builtin_nontemporal_load(f), ci = __builtin_nontemporal_load(f + 1);
+            __builtin_nontemporal_store(cr - re, f);
+            __builtin_nontemporal_store(ci - im, f + 1);
+        }
+    } else {
+        if (mode == 0) {
+            *p = c32{re, im};
+        } else {
+            const c32 c = *p;
+            *p = c32{c.x - re, c.y - im};
+        }
+    }
+}
+
+// Work-group of WGM x WGN waves, each TM x TN blocks of 32 x 32; tiles in
+// windows of GMW tile rows; NTC: C read and written with non-temporal hints
+// (streamed through the caches, so that it does not evict the fragments).
+template <int WGM, int WGN, int TM, int TN, int GMW, bool NTC>
+__device__ __forceinline__ void x6_block(const X6Args& g)
+{
+    constexpr int BM = 32 * TM * WGM, BN = 32 * TN * WGN;
+    const long z = blockIdx.z;
+    const u32x4* pa = g.pa + z * g.spk;
+    const u32x4* pb = g.pb + z * g.spk;
+    c32* C = g.C + z * g.sc;
+    const int* crow = g.crow ? g.crow + z * g.scr : nullptr;
+    const int M = g.M, N = g.N, KS = (g.K + 15) / 16;
+    // the wave index is uniform: kept in scalar registers, so that the
+    // fragment addresses are a scalar base plus the lane's offset
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave / WGN, wn = wave % WGN;
+    const int tiles_n = (N + BN - 1) / BN, tiles_m = (M + BM - 1) / BM;
+    const int ntiles = tiles_n * tiles_m;
+    // XCD-aware order, as for the f32 tiles: a contiguous range of tiles per
+    // XCD, walked in windows of 8 tile rows
+    int bid;
+    {
+        const int per = (ntiles + 7) / 8;
+        bid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    }
+    if (bid >= ntiles) return;
+    constexpr int GM = GMW;
+    const int first_m = bid / (GM * tiles_n) * GM;
+    const int gm = min(tiles_m - first_m, GM);
+    const int rem = bid - first_m * tiles_n;
+    const int tm = first_m + rem % gm, tn = rem / gm;
+    const int m0 = tm * BM + wm * TM * 32, n0 = tn * BN + wn * TN * 32;
+    const int nbm = (M + 31) / 32, nbn = (N + 31) / 32;
+    const u32x4* fa[TM];
+    const u32x4* fb[TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+        fa[i] = pa + (long)min(m0 / 32 + i, nbm - 1) * KS * (X6_PARTS * X6_FRAG) + lane;
+#pragma unroll
+    for (int j = 0; j < TN; j++)
+        fb[j] = pb + (long)min(n0 / 32 + j, nbn - 1) * KS * (X6_PARTS * X6_FRAG) + lane;
+
+    f32x16 re[TM][TN], im[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                re[i][j][v] = 0.f;
+                im[i][j][v] = 0.f;
+            }
+        }
+    u32x4 a0[TM][X6_PARTS], b0[TN][X6_PARTS], a1[TM][X6_PARTS], b1[TN][X6_PARTS];
+    auto load = [&](u32x4(&a)[TM][X6_PARTS], u32x4(&b)[TN][X6_PARTS], int ks) {
+        const long o = (long)ks * (X6_PARTS * X6_FRAG);
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int p = 0; p < X6_PARTS; p++) a[i][p] = fa[i][o + p * X6_FRAG];
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int p = 0; p < X6_PARTS; p++) b[j][p] = fb[j][o + p * X6_FRAG];
+    };
+    auto mma = [&](const u32x4(&a)[TM][X6_PARTS], const u32x4(&b)[TN][X6_PARTS]) {
+        // -im(B), for the re re - im im half
+        u32x4 nbi[TN][3];
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) nbi[j][t] = b[j][3 + t] ^ 0x80008000u;
+        // part pairs (s, t) of the six kept products
+        constexpr int PS[6] = {2, 1, 0, 1, 0, 0}, PT[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int s = PS[q], t = PT[q];
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    re[i][j] = mfma_x6(a[i][s], b[j][t], re[i][j]);
+                    im[i][j] = mfma_x6(a[i][s], b[j][3 + t], im[i][j]);
+                }
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    re[i][j] = mfma_x6(a[i][3 + s], nbi[j][t], re[i][j]);
+                    im[i][j] = mfma_x6(a[i][3 + s], b[j][t], im[i][j]);
+                }
+        }
+    };
+    load(a0, b0, 0);
+    int ks = 0;
+    for (; ks + 1 < KS; ks += 2) {
+        load(a1, b1, ks + 1);
+        mma(a0, b0);
+        if (ks + 2 < KS) load(a0, b0, ks + 2);
+        mma(a1, b1);
+    }
+    if (ks < KS) mma(a0, b0);
+
+    const int kl = lane >> 5, ml = lane & 31;
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int row = m0 + i * 32 + (v & 3) + 8 * (v >> 2) + 4 * kl;
+            if (row >= M) continue;
+            c32* cr = C + (long)(crow ? crow[row] : row) * g.ldc;
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                const int col = n0 + j * 32 + ml;
+                if (col >= N) continue;
+                x6_store<NTC>(cr + col, re[i][j][v], im[i][j][v], g.mode);
+            }
+        }
+}
+
+// The same product with the work-group's fragments of each slab staged once
+// in LDS (double-buffered): every thread copies its share of the slab's
+// fragments (global -> registers -> LDS, 16 bytes at a time, a wave's words
+// contiguous) for the next slab while the MFMAs of the current one run, and
+// each wave reads its own fragments from LDS, one barrier per slab. The
+// work-group fetches a fragment once instead of once per wave that uses it.
+template <int WGM, int WGN, int TM, int TN, int GMW, bool NTC, int DEPTH>
+__device__ __forceinline__ void x6l_block(const X6Args& g)
+{
+    constexpr int NT = 64 * WGM * WGN;
+    constexpr int BM = 32 * TM * WGM, BN = 32 * TN * WGN;
+    constexpr int FA = BM / 32 * X6_PARTS, FB = BN / 32 * X6_PARTS;  // fragments per slab
+    constexpr int NF = FA + FB;
+    constexpr int NW = NF * X6_FRAG / NT;  // 16-byte words a thread stages
+    static_assert(NW * NT == NF * X6_FRAG, "slab words per thread");
+    static_assert(NT % 64 == 0 && (NT / 64) * NW == NF, "fragments per wave");
+    __shared__ u32x4 lds[2][NF * X6_FRAG];
+
+    const long z = blockIdx.z;
+    const u32x4* pa = g.pa + z * g.spk;
+    const u32x4* pb = g.pb + z * g.spk;
+    c32* C = g.C + z * g.sc;
+    const int* crow = g.crow ? g.crow + z * g.scr : nullptr;
+    const int M = g.M, N = g.N, KS = (g.K + 15) / 16;
+    // the wave index is uniform: kept in scalar registers, so that the
+    // fragment addresses are a scalar base plus the lane's offset
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave / WGN, wn = wave % WGN;
+    const int tiles_n = (N + BN - 1) / BN, tiles_m = (M + BM - 1) / BM;
+    const int ntiles = tiles_n * tiles_m;
+    int bid;
+    {
+        const int per = (ntiles + 7) / 8;
+        bid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
+    }
+    if (bid >= ntiles) return;
+    constexpr int GM = GMW;
+    const int first_m = bid / (GM * tiles_n) * GM;
+    const int gm = min(tiles_m - first_m, GM);
+    const int rem = bid - first_m * tiles_n;
+    const int tm = first_m + rem % gm, tn = rem / gm;
+    const int nbm = (M + 31) / 32, nbn = (N + 31) / 32;
+    // staging: word q of this thread is word `lane` of fragment wave + (NT/64) q
+    // of the slab (A fragments first: block, part; then B's)
+    const u32x4* src[NW];
+#pragma unroll
+    for (int q = 0; q < NW; q++) {
+        const int fi = wave + (NT / 64) * q;
+        if (fi < FA) {
+            const int b = min(tm * (BM / 32) + fi / X6_PARTS, nbm - 1);
+            src[q] = pa + (long)b * KS * (X6_PARTS * X6_FRAG) + (fi % X6_PARTS) * X6_FRAG + lane;
+        } else {
+            const int f2 = fi - FA;
+            const int b = min(tn * (BN / 32) + f2 / X6_PARTS, nbn - 1);
+            src[q] = pb + (long)b * KS * (X6_PARTS * X6_FRAG) + (f2 % X6_PARTS) * X6_FRAG + lane;
+        }
+    }
+    // two register sets: with DEPTH 2 a slab's words are fetched two slabs
+    // before they are stored to LDS, so that a fetch from beyond L2 is covered
+    u32x4 stage[2][NW];
+    auto gload = [&](int set, int ks) {
+        const long o = (long)ks * (X6_PARTS * X6_FRAG);
+#pragma unroll
+        for (int q = 0; q < NW; q++) stage[set][q] = src[q][o];
+    };
+    auto sstore = [&](int set, int buf) {
+#pragma unroll
+        for (int q = 0; q < NW; q++)
+            lds[buf][(wave + (NT / 64) * q) * X6_FRAG + lane] = stage[set][q];
+    };
+
+    f32x16 re[TM][TN], im[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                re[i][j][v] = 0.f;
+                im[i][j][v] = 0.f;
+            }
+        }
+    auto mma = [&](int buf) {
+        const u32x4* la = lds[buf] + (wm * TM * X6_PARTS) * X6_FRAG + lane;
+        const u32x4* lb = lds[buf] + (FA + wn * TN * X6_PARTS) * X6_FRAG + lane;
+        u32x4 a[TM][X6_PARTS], b[TN][X6_PARTS], nbi[TN][3];
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int p = 0; p < X6_PARTS; p++) a[i][p] = la[(i * X6_PARTS + p) * X6_FRAG];
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int p = 0; p < X6_PARTS; p++) b[j][p] = lb[(j * X6_PARTS + p) * X6_FRAG];
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int t = 0; t < 3; t++) nbi[j][t] = b[j][3 + t] ^ 0x80008000u;
+        constexpr int PS[6] = {2, 1, 0, 1, 0, 0}, PT[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+        for (int q = 0; q < 6; q++) {
+            const int s = PS[q], t = PT[q];
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    re[i][j] = mfma_x6(a[i][s], b[j][t], re[i][j]);
+                    im[i][j] = mfma_x6(a[i][s], b[j][3 + t], im[i][j]);
+                }
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    re[i][j] = mfma_x6(a[i][3 + s], nbi[j][t], re[i][j]);
+                    im[i][j] = mfma_x6(a[i][3 + s], b[j][t], im[i][j]);
+                }
+        }
+    };
+    // the same products one column block at a time, a scheduling barrier
+    // between blocks, so that only one block's B fragments are live (fewer
+    // registers, room for the second staging set)
+    auto mma_cols = [&](int buf) {
+        const u32x4* la = lds[buf] + (wm * TM * X6_PARTS) * X6_FRAG + lane;
+        const u32x4* lb = lds[buf] + (FA + wn * TN * X6_PARTS) * X6_FRAG + lane;
+        u32x4 a[TM][X6_PARTS];
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int p = 0; p < X6_PARTS; p++) a[i][p] = la[(i * X6_PARTS + p) * X6_FRAG];
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            u32x4 b[X6_PARTS], nbi[3];
+#pragma unroll
+            for (int p = 0; p < X6_PARTS; p++) b[p] = lb[(j * X6_PARTS + p) * X6_FRAG];
+#pragma unroll
+            for (int t = 0; t < 3; t++) nbi[t] = b[3 + t] ^ 0x80008000u;
+            constexpr int PS[6] = {2, 1, 0, 1, 0, 0}, PT[6] = {0, 1, 2, 0, 1, 0};
+#pragma unroll
+            for (int q = 0; q < 6; q++) {
+                const int s = PS[q], t = PT[q];
+#pragma unroll
+                for (int i = 0; i < TM; i++) {
+                    re[i][j] = mfma_x6(a[i][s], b[t], re[i][j]);
+                    im[i][j] = mfma_x6(a[i][s], b[3 + t], im[i][j]);
+                    re[i][j] = mfma_x6(a[i][3 + s], nbi[t], re[i][j]);
+                    im[i][j] = mfma_x6(a[i][3 + s], b[t], im[i][j]);
+                }
+            }
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    };
+    if constexpr (DEPTH == 1) {
+        gload(0, 0);
+        sstore(0, 0);
+        __syncthreads();
+        for (int ks = 0; ks < KS; ks++) {
+            const bool more = ks + 1 < KS;
+            if (more) gload(0, ks + 1);
+            mma(ks & 1);
+            if (more) sstore(0, (ks + 1) & 1);
+            __syncthreads();
+        }
+    } else {
+        // slab s is fetched into set s % 2 at slab s - 2 and stored to LDS
+        // buffer s % 2 at the end of slab s - 1
+        gload(0, 0);
+        sstore(0, 0);
+        if (KS > 1) gload(1, 1);
+        if (KS > 2) gload(0, 2);
+        __syncthreads();
+        for (int ks = 0; ks < KS; ks += 2) {
+            // even slab ks in LDS buffer 0; ks + 1 in set 1, ks + 2 in set 0
+            mma_cols(0);
+            if (ks + 1 < KS) sstore(1, 1);
+            if (ks + 3 < KS) gload(1, ks + 3);
+            __syncthreads();
+            if (ks + 1 >= KS) break;
+            mma_cols(1);
+            if (ks + 2 < KS) sstore(0, 0);
+            if (ks + 4 < KS) gload(0, ks + 4);
+            __syncthreads();
+        }
+    }
+
+    const int m0 = tm * BM + wm * TM * 32, n0 = tn * BN + wn * TN * 32;
+    const int kl = lane >> 5, ml = lane & 31;
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int row = m0 + i * 32 + (v & 3) + 8 * (v >> 2) + 4 * kl;
+            if (row >= M) continue;
+            c32* cr = C + (long)(crow ? crow[row] : row) * g.ldc;
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                const int col = n0 + j * 32 + ml;
+                if (col >= N) continue;
+                x6_store<NTC>(cr + col, re[i][j][v], im[i][j][v], g.mode);
+            }
+        }
+}
+
+// LDS: staged slabs; otherwise each wave loads its own fragments
+template <int WGM, int WGN, int TM, int TN, int LDS, int GMW, bool NTC>
+__global__ __launch_bounds__(64 * WGM * WGN, TM * TN >= 4 ? 1 : 2) void k_cgemm_x6(X6Args g)
+{
+    if constexpr (LDS > 0)
+        x6l_block<WGM, WGN, TM, TN, GMW, NTC, LDS>(g);
+    else
+        x6_block<WGM, WGN, TM, TN, GMW, NTC>(g);
+}
+// the same product under the trailing update's name (see k_trailing_update)
+template <int WGM, int WGN, int TM, int TN, int LDS, int GMW, bool NTC>
+__global__ __launch_bounds__(64 * WGM * WGN, TM * TN >= 4 ? 1 : 2) void k_trailing_update_x6(X6Args g)
+{
+    if constexpr (LDS > 0)
+        x6l_block<WGM, WGN, TM, TN, GMW, NTC, LDS>(g);
+    else
+        x6_block<WGM, WGN, TM, TN, GMW, NTC>(g);
+}
+
+// Packs both operands into the product's work space, then multiplies.
+template <int WGM, int WGN, int TM, int TN, int LDS, int GMW = 8, bool NTC = false>
+static int launch_x6(const GemmArgs<float>& g, hipStream_t st, bool trailing)
+{
+    const long need = gemm_x6_ws_bytes(g.M, g.N, g.K);
+    if (need <= 0 || !g.ws || g.ws_item < need || g.ws_item % 16) return TAGS_ERR_ARG;
+    const long wa = x6_pack_words(g.M, g.K), wb = x6_pack_words(g.N, g.K);
+    u32x4* pa = (u32x4*)g.ws;
+    u32x4* pb = pa + wa;
+    const long spk = g.ws_item / 16;
+    // threads: one per (block, slab, lane) = words / parts
+    const long ta = wa / X6_PARTS, tb = wb / X6_PARTS;
+    const dim3 ga((unsigned)((ta + 255) / 256), (unsigned)g.nb);
+    const dim3 gb((unsigned)((tb + 255) / 256), (unsigned)g.nb);
+    if (trailing) {
+        hipLaunchKernelGGL((k_pack_x6<0, true>), ga, dim3(256), 0, st, g.A, g.lda, g.arow, g.M,
+                           g.K, pa, g.sa, g.sar, spk);
+        hipLaunchKernelGGL((k_pack_x6<1, true>), gb, dim3(256), 0, st, g.B, g.ldb, g.brow, g.N,
+                           g.K, pb, g.sb, g.sbr, spk);
+    } else {
+        hipLaunchKernelGGL((k_pack_x6<0, false>), ga, dim3(256), 0, st, g.A, g.lda, g.arow, g.M,
+                           g.K, pa, g.sa, g.sar, spk);
+        hipLaunchKernelGGL((k_pack_x6<1, false>), gb, dim3(256), 0, st, g.B, g.ldb, g.brow, g.N,
+                           g.K, pb, g.sb, g.sbr, spk);
+    }
+    TAGS_HIP(hipGetLastError());
+    constexpr int BM = 32 * TM * WGM, BN = 32 * TN * WGN;
+    const X6Args x{g.M, g.N, g.K, pa, pb, spk, g.C, g.ldc, g.crow, g.sc, g.scr, g.mode};
+    const long tiles = (long)((g.M + BM - 1) / BM) * ((g.N + BN - 1) / BN);
+    const long grid = (tiles + 7) / 8 * 8;
+    const dim3 gd((unsigned)grid, 1, (unsigned)g.nb);
+    if (trailing)
+        hipLaunchKernelGGL((k_trailing_update_x6<WGM, WGN, TM, TN, LDS, GMW, NTC>), gd,
+                           dim3(64 * WGM * WGN), 0, st, x);
+    else
+        hipLaunchKernelGGL((k_cgemm_x6<WGM, WGN, TM, TN, LDS, GMW, NTC>), gd,
+                           dim3(64 * WGM * WGN), 0, st, x);
+    TAGS_HIP(hipGetLastError());
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
 // variants and selection
 // ---------------------------------------------------------------------------
 template <typename T, int BM, int BN, int BK, bool M3, int WGM = 2, int WGN = 2, int AL = 0>
@@ -467,7 +1030,7 @@ struct GemmShape {
 };
 template <>
 struct GemmTable<float> {
-    static constexpr int count = 22;
+    static constexpr int count = 30;
     static int launch(int v, const GemmArgs<float>& g, hipStream_t st, bool tr)
     {
         switch (v) {
@@ -492,6 +1055,17 @@ struct GemmTable<float> {
         case 19: return launch_v<float, 128, 64, 8, true, 4, 2, 1>(g, st, tr);
         case 20: return launch_v<float, 128, 64, 16, true, 4, 2, 1>(g, st, tr);
         case 21: return launch_v<float, 128, 128, 8, true, 4, 4, 1>(g, st, tr);
+        // split-bf16 (see k_pack_x6): slabs staged in LDS, fetched two slabs ahead
+        case 22: return launch_x6<4, 1, 1, 2, 2>(g, st, tr);
+        case 23: return launch_x6<4, 1, 1, 2, 2, 8, true>(g, st, tr);
+        case 24: return launch_x6<4, 1, 1, 2, 2, 4>(g, st, tr);
+        case 25: return launch_x6<2, 2, 1, 2, 2>(g, st, tr);
+        // one slab ahead; without LDS (each wave loads its own fragments)
+        case 26: return launch_x6<4, 1, 1, 2, 1>(g, st, tr);
+        case 27: return launch_x6<4, 1, 1, 2, 0>(g, st, tr);
+        case 28: return launch_x6<2, 2, 2, 2, 0>(g, st, tr);
+        // variant 19 with C streamed past the caches
+        case 29: return launch_v<float, 128, 64, 8, true, 4, 2, 3>(g, st, tr);
         default: return launch_v<float, 64, 64, 8, true, 2, 2>(g, st, tr);
         }
     }
@@ -502,7 +1076,9 @@ struct GemmTable<float> {
                                            {128, 128}, {256, 128}, {128, 64},  {128, 128},
                                            {128, 128}, {64, 128},  {128, 64},  {128, 128},
                                            {256, 64},  {64, 64},   {G32_BM, G32_BN},
-                                           {128, 64},  {128, 64},  {128, 128}};
+                                           {128, 64},  {128, 64},  {128, 128},
+                                           {128, 64},  {128, 64},  {128, 64},  {64, 128},
+                                           {128, 64},  {128, 64},  {128, 128}, {128, 64}};
         return t[v];
     }
     static int tile_m(int v) { return shape(v).bm; }
@@ -510,7 +1086,7 @@ struct GemmTable<float> {
 };
 template <>
 struct GemmTable<double> {
-    static constexpr int count = 15;
+    static constexpr int count = 16;
     static int launch(int v, const GemmArgs<double>& g, hipStream_t st, bool tr)
     {
         switch (v) {
@@ -528,6 +1104,7 @@ struct GemmTable<double> {
         case 11: return launch_v<double, 64, 64, 16, false, 2, 2>(g, st, tr);
         case 13: return launch_v<double, 128, 64, 8, false, 4, 2, 1>(g, st, tr);
         case 14: return launch_v<double, 128, 64, 16, false, 4, 2, 1>(g, st, tr);
+        case 15: return launch_v<double, 128, 64, 8, false, 4, 2, 3>(g, st, tr);
         default: return launch_v<double, 64, 64, 4, false, 2, 2>(g, st, tr);
         }
     }
@@ -536,7 +1113,7 @@ struct GemmTable<double> {
         static const GemmShape t[count] = {{64, 64},  {128, 64},  {64, 128}, {128, 64},
                                            {128, 128}, {128, 128}, {128, 64}, {128, 128},
                                            {128, 64}, {64, 128},  {256, 64},  {64, 64},
-                                           {64, 64},  {128, 64},  {128, 64}};
+                                           {64, 64},  {128, 64},  {128, 64},  {128, 64}};
         return t[v];
     }
     static int tile_m(int v) { return shape(v).bm; }
@@ -545,11 +1122,12 @@ struct GemmTable<double> {
 
 // Selected variant for large and small problems, [double, float]. The
 // defaults are the fastest on MI355X at the LU's shapes (K = 128 for
-// complex128, 256 for complex64): 128 x 64 tiles of 8 waves (32 x 32 per
-// wave; three-multiply for complex64; A tile row-major in LDS) for large
-// products, 64 x 64 tiles of 4 waves (K chunks of 8) for small ones;
-// tags_autotune may replace them.
-static std::atomic<int> g_big_variant[2] = {13, 19};
+// complex128, 256 for complex64): for large complex128 products 128 x 64
+// tiles of 8 waves (32 x 32 per wave, A tile row-major in LDS), for large
+// complex64 ones the split-bf16 product (128 x 64 tiles of 4 waves, slabs
+// fetched two ahead, tiles in windows of 4 rows); 64 x 64 tiles of 4 waves (K
+// chunks of 8) for small ones; tags_autotune may replace them.
+static std::atomic<int> g_big_variant[2] = {13, 24};
 static std::atomic<int> g_small_variant[2] = {0, 17};
 
 template <typename T>
@@ -575,7 +1153,13 @@ int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing)
     const int small = tu && tu->gemm_small[p] >= 0
                           ? tu->gemm_small[p]
                           : g_small_variant[p].load(std::memory_order_relaxed);
-    const int v = tiles_big >= 512 ? big : small;
+    int v = tiles_big >= 512 ? big : small;
+    // the split-bf16 variants need the caller's work space (and K <= 256);
+    // without it the complex64 product takes the three-multiply f32 tiles
+    if (p == 1 && v >= GEMM_X6_FIRST && v <= GEMM_X6_LAST &&
+        (!g.ws || g.ws_item < gemm_x6_ws_bytes(g.M, g.N, g.K) ||
+         gemm_x6_ws_bytes(g.M, g.N, g.K) <= 0))
+        v = 19;
     return GemmTable<T>::launch(v, g, st, trailing);
 }
 

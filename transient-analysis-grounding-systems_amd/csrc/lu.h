@@ -124,7 +124,17 @@ struct GemmArgs {
     // starts z times its stride (elements) after item 0's
     int nb = 1;
     long sa = 0, sar = 0, sb = 0, sbr = 0, sc = 0, scr = 0;
+    // work space of the split-bf16 variant (complex64 only): ws_item bytes per
+    // item, at least gemm_x6_ws_bytes(M, N, K)
+    void* ws = nullptr;
+    long ws_item = 0;
 };
+
+// Work space bytes per item of the split-bf16 complex64 product (its operands
+// packed in MFMA fragment order; 0 when K exceeds the LU's largest step).
+long gemm_x6_ws_bytes(int M, int N, int K);
+// complex64 variants of the split-bf16 product (they need the work space)
+constexpr int GEMM_X6_FIRST = 22, GEMM_X6_LAST = 28;
 
 // One work-group's pivot candidate for one column of the panel.
 template <typename T>
@@ -172,6 +182,8 @@ struct BatchStrides {
     long f = 0;       // flags (ints)
     long cand = 0;    // tournament nominee lists (ints)
     long cnt = 0;     // tournament arrival counters
+    void* ws = nullptr;  // split-bf16 product work space, ws_item bytes per item
+    long ws_item = 0;
 };
 template <typename T>
 int launch_tslu(cx<T>* A, long ld, int* perm, int c0, int w, int row_end, int cand_end,
@@ -402,6 +414,11 @@ struct LUWork {
     size_t cap_tctl = 0;
     long tctl_stride = 0;
     unsigned epoch = 0;
+    // complex64 only: work space of the split-bf16 products (operands packed
+    // in MFMA fragment order), x6_item bytes per item
+    void* x6ws = nullptr;
+    size_t cap_x6 = 0;
+    long x6_item = 0;
 
     int alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batch_ = 1);
     void release();
@@ -418,6 +435,8 @@ struct LUWork {
         b.f = 4;
         b.cand = cand_stride;
         b.cnt = cnt_stride;
+        b.ws = x6ws;
+        b.ws_item = x6_item;
         return b;
     }
     ~LUWork() { release(); }

@@ -1641,6 +1641,12 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
     if (rc) return rc;
     const size_t B = (size_t)batch_;
     size_t need_perm = (size_t)n_ * B, need_inv = 4096 * (size_t)nblk_ * B;
+    // split-bf16 packs of every product of the factorisation (M, N <= n_,
+    // K <= 256), rounded to 256 bytes per item
+    const long x6_item_ = sizeof(T) == sizeof(float)
+                              ? (gemm_x6_ws_bytes(n_, n_, 256) + 255) / 256 * 256
+                              : 0;
+    size_t need_x6 = (size_t)x6_item_ * B;
     size_t need_cand = (size_t)tr.cand_total * B;
     size_t need_cnt = (size_t)std::max(1, tr.cnt_total) * B;
     // flow-solve control words per item: ticket, spare, one per block group,
@@ -1656,7 +1662,7 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         TAGS_HIP(hipMemset(bar, 0, sizeof(unsigned) * 64));
     }
     if (need_perm > cap_perm || need_inv > cap_inv || B > cap_items || need_cand > cap_cand ||
-        need_cnt > cap_cnt || need_tctl > cap_tctl) {
+        need_cnt > cap_cnt || need_tctl > cap_tctl || need_x6 > cap_x6) {
         // grow to cover every earlier request as well: a work space that
         // serves systems of different orders in turn then reallocates (which
         // waits for the whole device) only while its largest request grows
@@ -1665,6 +1671,7 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         need_cand = std::max(need_cand, cap_cand);
         need_cnt = std::max(need_cnt, cap_cnt);
         need_tctl = std::max(need_tctl, cap_tctl);
+        need_x6 = std::max(need_x6, cap_x6);
         const size_t items = std::max(B, cap_items);
         (void)hipStreamSynchronize(s2);
         (void)hipFree(perm);
@@ -1678,10 +1685,12 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         (void)hipFree(tcand);
         (void)hipFree(tcnt);
         (void)hipFree(tctl);
+        (void)hipFree(x6ws);
         perm = perm_alt = flags = d_init = d_orders = tcand = nullptr;
         invL = invU = vec = nullptr;
         tcnt = tctl = nullptr;
-        cap_perm = cap_inv = cap_items = cap_cand = cap_cnt = cap_tctl = 0;
+        x6ws = nullptr;
+        cap_perm = cap_inv = cap_items = cap_cand = cap_cnt = cap_tctl = cap_x6 = 0;
         TAGS_HIP(hipMalloc(&perm, sizeof(int) * need_perm));
         TAGS_HIP(hipMalloc(&perm_alt, sizeof(int) * need_perm));
         TAGS_HIP(hipMalloc(&invL, sizeof(cx<T>) * need_inv));
@@ -1700,6 +1709,7 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         // completion words start below every epoch a launch uses
         TAGS_HIP(hipMalloc(&tctl, sizeof(unsigned) * need_tctl));
         TAGS_HIP(hipMemset(tctl, 0, sizeof(unsigned) * need_tctl));
+        if (need_x6) TAGS_HIP(hipMalloc(&x6ws, need_x6));
         epoch = 0;
         cap_perm = need_perm;
         cap_inv = need_inv;
@@ -1707,6 +1717,7 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         cap_cand = need_cand;
         cap_cnt = need_cnt;
         cap_tctl = need_tctl;
+        cap_x6 = need_x6;
         tctl_stride = tstride;
     }
     if (tstride != tctl_stride) {
@@ -1720,6 +1731,7 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
     nb = 1;
     sA = 0;
     nblk_cap = nblk_;
+    x6_item = x6_item_;
     cand_stride = tr.cand_total;
     cnt_stride = std::max(1, tr.cnt_total);
     h_init.clear();
@@ -1752,10 +1764,13 @@ void LUWork<T>::release()
     (void)hipFree(tcand);
     (void)hipFree(tcnt);
     (void)hipFree(tctl);
+    (void)hipFree(x6ws);
     tcand = nullptr;
     tcnt = nullptr;
     tctl = nullptr;
-    cap_tctl = 0;
+    x6ws = nullptr;
+    cap_tctl = cap_x6 = 0;
+    x6_item = 0;
     d_init = nullptr;
     d_orders = nullptr;
     h_init.clear();
@@ -1785,6 +1800,8 @@ static int gemm(Profiler* pr, hipStream_t st, int M, int N, int K, const cx<T>* 
     if (M <= 0 || N <= 0 || K <= 0) return 0;
     GemmArgs<T> g{M, N, K, A, lda, arow, B, ldb, brow, C, ldc, crow, mode};
     g.nb = bs.nb;
+    g.ws = bs.ws;
+    g.ws_item = bs.ws_item;
     g.sa = inv_op == 1 ? bs.inv : bs.a;
     g.sb = inv_op == 2 ? bs.inv : bs.a;
     g.sc = bs.a;
