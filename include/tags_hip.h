@@ -683,11 +683,16 @@ const char *tags_version(void);
 int tags_device_synchronize(void);
 
 /* GEMM tuning. The trailing update C -= A B of the LU has several MFMA tile
- * variants per precision (precision: tags_precision). tags_gemm_bench times
- * one variant on an M x N x K problem (mean ms per launch); tags_autotune
- * times every variant on a large (8192 x 8192 x 128) and a small
- * (1024 x 1024 x 128) problem and installs the fastest of each for this
- * process; tags_gemm_select installs variants explicitly. */
+ * variants per precision (precision: tags_precision). Complex64 variants
+ * 0-21 and 29 run on the fp32 MFMA cores; 22-28 run on the bf16 MFMA cores,
+ * every fp32 operand value split into three bf16 parts (six part products per
+ * real product, fp32 accumulation: the accuracy of the fp32 tiles) after a
+ * packing pass into a work space that the LU's work space provides (K <= 256;
+ * a product without one runs variant 19). tags_gemm_bench times one variant
+ * on an M x N x K problem (mean ms per launch); tags_autotune times every
+ * variant on a large (8192 x 8192 x K) and a small (1024 x 1024 x K) problem,
+ * K the LU's step (128 complex128, 256 complex64), and installs the fastest
+ * of each for this process; tags_gemm_select installs variants explicitly. */
 int tags_gemm_bench(int M, int N, int K, int precision, int variant, int iters, double *ms);
 int tags_gemm_variants(int precision);
 /* One product through one variant, for validation: C (M x N) = A B (mode 0) or

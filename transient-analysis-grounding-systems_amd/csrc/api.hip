@@ -853,10 +853,12 @@ int tags_gemm_select(int precision, int big_variant, int small_variant)
 
 int tags_autotune(int precision, int* big_out, int* small_out)
 {
-    // large: the first trailing updates of the 21,421 system; small: the tail
+    // large: the first trailing updates of the 21,421 system; small: the tail;
+    // K: the LU's step width (128 for complex128 factors, 256 for complex64)
     const int nv = gemm_variant_count(precision);
     int best[2] = {0, 0};
-    const int sizes[2][3] = {{8192, 8192, 128}, {1024, 1024, 128}};
+    const int K = precision == TAGS_PREC_MIXED ? 256 : 128;
+    const int sizes[2][3] = {{8192, 8192, K}, {1024, 1024, K}};
     for (int s = 0; s < 2; s++) {
         double bt = 1e30;
         for (int v = 0; v < nv; v++) {
