@@ -130,3 +130,33 @@ def test_c_driver_sweep_with_autotune(tmp_path):
     dp = np.max(np.abs(z64 - u64) / np.abs(u64))
     print("C driver: mixed vs FP64 %.2e, FP64 vs Python %.2e" % (d, dp), flush=True)
     assert d <= 1e-8 and dp <= 1e-12
+
+
+def test_split_bf16_products_answer_as_fp32_tiles():
+    """The large complex64 products on the bf16 MFMA cores (split-bf16
+    variant 24: trailing updates and the node columns' products of the nodal
+    plan) answer as the fp32 three-multiply tiles (variant 19) to rounding:
+    the 60 x 60 m grid (N = 21,421) in the mixed form at three frequencies
+    (one below the 20 kHz FGMRES switch), Z_h within 1e-9 of each other and
+    within 1e-8 of the FP64 form, no FP64 fallback."""
+    _, e, nodes = T.electrode_grid(31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)
+    f = np.array([100.0, 3e5, 5e6])
+    s = 2j * np.pi * f
+    kap = 2e-3 + s * 20.0 * T.EPS0
+    base = dict(gemm_small=17, step_width=256, update_chunks=-1, solve_rows=-1, inflight=0,
+                batch=-1, nodal_above=-1.0, gmres_steps=0)
+    sw = T.Sweep(e, nodes)
+    u64, _ = sw.solve(s, kap, 0, T.PREC_FP64)
+    out = {}
+    for big in (19, 24):
+        sw.set_tuning(T.PREC_MIXED, dict(base, gemm_big=big))
+        u, st = sw.solve(s, kap, 0, T.PREC_MIXED)
+        assert all(q["precision_used"] == 1 for q in st), (big, st)
+        out[big] = u
+        print("variant %d vs FP64: %.2e" % (big, float(np.max(np.abs(u - u64) / np.abs(u64)))),
+              flush=True)
+        assert np.max(np.abs(u - u64) / np.abs(u64)) <= 1e-8
+    sw.close()
+    d = float(np.max(np.abs(out[24] - out[19]) / np.abs(out[19])))
+    print("split-bf16 vs fp32 tiles: %.2e" % d, flush=True)
+    assert d <= 1e-9

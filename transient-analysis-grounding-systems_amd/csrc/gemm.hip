@@ -1155,9 +1155,13 @@ int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing)
                           : g_small_variant[p].load(std::memory_order_relaxed);
     int v = tiles_big >= 512 ? big : small;
     // the split-bf16 variants need the caller's work space (and K <= 256);
-    // without it the complex64 product takes the three-multiply f32 tiles
+    // without it the complex64 product takes the three-multiply f32 tiles.
+    // They also serve only products with K > 64, the step-width products the
+    // LU queues on its main stream (trailing update, columns of a step): the
+    // panel's products (K <= 64) run on the look-ahead stream beside them and
+    // must not share the work space.
     if (p == 1 && v >= GEMM_X6_FIRST && v <= GEMM_X6_LAST &&
-        (!g.ws || g.ws_item < gemm_x6_ws_bytes(g.M, g.N, g.K) ||
+        (!g.ws || g.K <= 64 || g.ws_item < gemm_x6_ws_bytes(g.M, g.N, g.K) ||
          gemm_x6_ws_bytes(g.M, g.N, g.K) <= 0))
         v = 19;
     return GemmTable<T>::launch(v, g, st, trailing);
