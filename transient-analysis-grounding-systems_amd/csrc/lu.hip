@@ -1041,8 +1041,17 @@ __global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A_, long ld
                         break;
                     }
                 }
+                // the later groups this group reads that are published already:
+                // one acquire fence covers them all (a fence waits for every
+                // load in flight, so a group behind the chain's front would
+                // otherwise drain its loads once per chunk)
+                int seen = need;
+                const int lim = (nu - 1) / RB;
+                while (seen < lim && __hip_atomic_load(&done[seen + 1], __ATOMIC_RELAXED,
+                                                       __HIP_MEMORY_SCOPE_AGENT) == epoch)
+                    seen++;
                 __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-                known = need;
+                known = seen;
             }
 #pragma unroll
             for (int c = 0; c < KU; c++) {
