@@ -453,7 +453,7 @@ static int launch_v32(const GemmArgs<float>& g, hipStream_t st, bool trailing)
 }
 
 // ---------------------------------------------------------------------------
-// complex64 on the bf16 MFMA cores (variants 22..28). Every fp32 operand value
+// complex64 on the bf16 MFMA cores (variants 22..30). Every fp32 operand value
 // x is split into three bf16 parts x = x1 + x2 + x3, each the round-to-nearest
 // bf16 of what the earlier parts leave: x1 holds the leading 8 significant
 // bits, x2 the next 8, x3 the last 8, so that the parts sum to x exactly (for
@@ -1030,7 +1030,7 @@ struct GemmShape {
 };
 template <>
 struct GemmTable<float> {
-    static constexpr int count = 30;
+    static constexpr int count = 32;
     static int launch(int v, const GemmArgs<float>& g, hipStream_t st, bool tr)
     {
         switch (v) {
@@ -1064,8 +1064,11 @@ struct GemmTable<float> {
         case 26: return launch_x6<4, 1, 1, 2, 1>(g, st, tr);
         case 27: return launch_x6<4, 1, 1, 2, 0>(g, st, tr);
         case 28: return launch_x6<2, 2, 2, 2, 0>(g, st, tr);
+        // 128 x 128 tiles of 8 waves, two slabs ahead (one work-group per CU)
+        case 29: return launch_x6<4, 2, 1, 2, 2>(g, st, tr);
+        case 30: return launch_x6<4, 2, 1, 2, 2, 8, true>(g, st, tr);
         // variant 19 with C streamed past the caches
-        case 29: return launch_v<float, 128, 64, 8, true, 4, 2, 3>(g, st, tr);
+        case 31: return launch_v<float, 128, 64, 8, true, 4, 2, 3>(g, st, tr);
         default: return launch_v<float, 64, 64, 8, true, 2, 2>(g, st, tr);
         }
     }
@@ -1078,7 +1081,8 @@ struct GemmTable<float> {
                                            {256, 64},  {64, 64},   {G32_BM, G32_BN},
                                            {128, 64},  {128, 64},  {128, 128},
                                            {128, 64},  {128, 64},  {128, 64},  {64, 128},
-                                           {128, 64},  {128, 64},  {128, 128}, {128, 64}};
+                                           {128, 64},  {128, 64},  {128, 128}, {128, 128},
+                                           {128, 128}, {128, 64}};
         return t[v];
     }
     static int tile_m(int v) { return shape(v).bm; }

@@ -134,7 +134,7 @@ struct GemmArgs {
 // packed in MFMA fragment order; 0 when K exceeds the LU's largest step).
 long gemm_x6_ws_bytes(int M, int N, int K);
 // complex64 variants of the split-bf16 product (they need the work space)
-constexpr int GEMM_X6_FIRST = 22, GEMM_X6_LAST = 28;
+constexpr int GEMM_X6_FIRST = 22, GEMM_X6_LAST = 30;
 
 // One work-group's pivot candidate for one column of the panel.
 template <typename T>
