@@ -2722,22 +2722,44 @@ int tune_cache_write(const char* path, const std::string& key, const tags_tuning
     return 0;
 }
 
-// Mean ms per launch of the fastest variant for C -= A B (M x N x K) in the
-// given precision; *best = its index (-1 when none ran).
-int tune_gemm(int precision, int M, int N, int K, int iters, int* best, double* best_ms)
+// The fastest variant for C -= A B (M x N x K) in the given precision; *best =
+// its index (-1 when none ran), *best_ms its mean ms per launch. With
+// mid > 0 a variant is scored by its time at M x N plus mid times its time at
+// (M / 2) x (N / 2): an LU's trailing updates shrink step by step, and a
+// variant that wins only at the first one can lose over the rest.
+int tune_gemm(int precision, int M, int N, int K, int iters, int* best, double* best_ms,
+              double mid = 0.0, int* second = nullptr)
 {
     *best = -1;
     *best_ms = 1e30;
+    double best_score = 1e30, second_score = 1e30;
+    if (second) *second = -1;
     const int nv = gemm_variant_count(precision == TAGS_PREC_MIXED ? 1 : 0);
+    auto bench = [&](int m, int n, int v, double* ms) {
+        return precision == TAGS_PREC_MIXED ? gemm_bench_t<float>(m, n, K, v, iters, ms)
+                                            : gemm_bench_t<double>(m, n, K, v, iters, ms);
+    };
     for (int v = 0; v < nv; v++) {
-        double ms = 0;
-        const int rc = precision == TAGS_PREC_MIXED ? gemm_bench_t<float>(M, N, K, v, iters, &ms)
-                                                    : gemm_bench_t<double>(M, N, K, v, iters, &ms);
+        double ms = 0, ms2 = 0;
+        int rc = bench(M, N, v, &ms);
         if (rc == TAGS_ERR_ARG) continue;  // a variant that does not take this shape
         if (rc) return rc;
-        if (ms < *best_ms) {
+        double score = ms;
+        if (mid > 0) {
+            rc = bench(std::max(64, M / 2), std::max(64, N / 2), v, &ms2);
+            if (rc == TAGS_ERR_ARG) continue;
+            if (rc) return rc;
+            score += mid * ms2;
+        }
+        if (score < best_score) {
+            if (second) *second = *best;
+            second_score = best_score;
+            best_score = score;
             *best_ms = ms;
             *best = v;
+        } else if (score < second_score) {
+            second_score = score;
+            if (second) *second = v;
         }
     }
     return 0;
@@ -2808,13 +2830,13 @@ int tags_sweep_autotune(tags_sweep* sw, int precision, size_t nprobe, const TAGS
     for (const LUPhase& ph : plan->phases)
         if (cols(ph) > cols(*wide)) wide = &ph;
     const int widths[2] = {128, 256};
-    int big[2] = {-1, -1}, small[2] = {-1, -1};
+    int big[2] = {-1, -1}, small[2] = {-1, -1}, big2[2] = {-1, -1};
     for (int wi = 0; wi < 2; wi++) {
         const int w = widths[wi];
         const int M = std::min(12288, std::max(64, wide->row_end - wide->c_begin - w));
         const int N = std::min(12288, std::max(64, cols(*wide) - w));
         double ms = 0;
-        if ((rc = tune_gemm(precision, M, N, w, 3, &big[wi], &ms))) return rc;
+        if ((rc = tune_gemm(precision, M, N, w, 3, &big[wi], &ms, 2.0, &big2[wi]))) return rc;
         const int Ms = std::min(1024, M), Ns = std::min(1024, N);
         if ((rc = tune_gemm(precision, Ms, Ns, w, 10, &small[wi], &ms))) return rc;
         if (big[wi] < 0 || small[wi] < 0) return TAGS_ERR_ARG;
@@ -2882,6 +2904,16 @@ int tags_sweep_autotune(tags_sweep* sw, int precision, size_t nprobe, const TAGS
         c.gemm_big = big[1 - w0];
         c.gemm_small = small[1 - w0];
         if ((rc = consider(c))) return fail(rc);
+    }
+    {
+        // the runner-up of the product timings, on the probe solves (the
+        // timings see two product sizes, the solves the whole factorisation)
+        const int wi = cur.step_width == widths[0] ? 0 : 1;
+        if (big2[wi] >= 0 && big2[wi] != cur.gemm_big) {
+            tags_tuning c = cur;
+            c.gemm_big = big2[wi];
+            if ((rc = consider(c))) return fail(rc);
+        }
     }
     for (int ch : {1, 2, 8}) {
         tags_tuning c = cur;
