@@ -48,7 +48,7 @@ PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16": 2500.0}  # dense MFMA TFLOP/s, MI35
 # complex64 GEMM variants that run on the bf16 MFMA cores (every fp32 operand
 # as three bf16 parts, six bf16 part products per real product), and the
 # library's default large-product variants
-X6_VARIANTS = range(22, 31)
+X6_VARIANTS = range(22, 33)
 X6_PARTS_PRODUCTS = 6
 DEFAULT_BIG = {"mixed": 24, "fp64": 13}
 GRID = (31, 31, 60.0, 60.0, 4, 4, 5e-3, -0.6)

@@ -684,7 +684,7 @@ int tags_device_synchronize(void);
 
 /* GEMM tuning. The trailing update C -= A B of the LU has several MFMA tile
  * variants per precision (precision: tags_precision). Complex64 variants
- * 0-21 and 31 run on the fp32 MFMA cores; 22-30 run on the bf16 MFMA cores,
+ * 0-21 and 33 run on the fp32 MFMA cores; 22-32 run on the bf16 MFMA cores,
  * every fp32 operand value split into three bf16 parts (six part products per
  * real product, fp32 accumulation: the accuracy of the fp32 tiles) after a
  * packing pass into a work space that the LU's work space provides (K <= 256;

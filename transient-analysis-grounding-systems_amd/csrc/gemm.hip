@@ -453,7 +453,7 @@ static int launch_v32(const GemmArgs<float>& g, hipStream_t st, bool trailing)
 }
 
 // ---------------------------------------------------------------------------
-// complex64 on the bf16 MFMA cores (variants 22..30). Every fp32 operand value
+// complex64 on the bf16 MFMA cores (variants 22..32). Every fp32 operand value
 // x is split into three bf16 parts x = x1 + x2 + x3, each the round-to-nearest
 // bf16 of what the earlier parts leave: x1 holds the leading 8 significant
 // bits, x2 the next 8, x3 the last 8, so that the parts sum to x exactly (for
@@ -746,7 +746,7 @@ __device__ __forceinline__ void x6_block(const X6Args& g)
 // each wave reads its own fragments from LDS, one barrier per slab. The
 // work-group fetches a fragment once instead of once per wave that uses it.
 template <int WGM, int WGN, int TM, int TN, int GMW, bool NTC, int DEPTH>
-__device__ __forceinline__ void x6l_block(const X6Args& g)
+__device__ __forceinline__ void x6l_tile(const X6Args& g, int bid)
 {
     constexpr int NT = 64 * WGM * WGN;
     constexpr int BM = 32 * TM * WGM, BN = 32 * TN * WGN;
@@ -769,13 +769,6 @@ __device__ __forceinline__ void x6l_block(const X6Args& g)
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave / WGN, wn = wave % WGN;
     const int tiles_n = (N + BN - 1) / BN, tiles_m = (M + BM - 1) / BM;
-    const int ntiles = tiles_n * tiles_m;
-    int bid;
-    {
-        const int per = (ntiles + 7) / 8;
-        bid = (int)(blockIdx.x & 7) * per + (int)(blockIdx.x >> 3);
-    }
-    if (bid >= ntiles) return;
     constexpr int GM = GMW;
     const int first_m = bid / (GM * tiles_n) * GM;
     const int gm = min(tiles_m - first_m, GM);
@@ -942,6 +935,18 @@ __device__ __forceinline__ void x6l_block(const X6Args& g)
         }
 }
 
+// Tiles of a launch: work-group b takes tile b (XCD-aware order: dispatch b
+// runs on XCD b % 8, which gets a contiguous range of tiles).
+template <int WGM, int WGN, int TM, int TN, int GMW, bool NTC, int DEPTH>
+__device__ __forceinline__ void x6l_block(const X6Args& g)
+{
+    constexpr int BM = 32 * TM * WGM, BN = 32 * TN * WGN;
+    const int ntiles = ((g.N + BN - 1) / BN) * ((g.M + BM - 1) / BM);
+    const int per = (ntiles + 7) / 8;
+    const int vb = blockIdx.x, bid = (vb & 7) * per + (vb >> 3);
+    if (bid < ntiles) x6l_tile<WGM, WGN, TM, TN, GMW, NTC, DEPTH>(g, bid);
+}
+
 // LDS: staged slabs; otherwise each wave loads its own fragments
 template <int WGM, int WGN, int TM, int TN, int LDS, int GMW, bool NTC>
 __global__ __launch_bounds__(64 * WGM * WGN, TM * TN >= 4 ? 1 : 2) void k_cgemm_x6(X6Args g)
@@ -1043,7 +1048,7 @@ struct GemmShape {
 };
 template <>
 struct GemmTable<float> {
-    static constexpr int count = 32;
+    static constexpr int count = 34;
     static int launch(int v, const GemmArgs<float>& g, hipStream_t st, bool tr)
     {
         switch (v) {
@@ -1080,8 +1085,12 @@ struct GemmTable<float> {
         // 128 x 128 tiles of 8 waves, two slabs ahead (one work-group per CU)
         case 29: return launch_x6<4, 2, 1, 2, 2>(g, st, tr);
         case 30: return launch_x6<4, 2, 1, 2, 2, 8, true>(g, st, tr);
+        // 128 x 128 tiles of 16 waves (32 x 32 each), one slab ahead
+        case 31: return launch_x6<4, 4, 1, 1, 1>(g, st, tr);
+        // 64 x 64 tiles of 4 waves (32 x 32 each), two slabs ahead
+        case 32: return launch_x6<2, 2, 1, 1, 2>(g, st, tr);
         // variant 19 with C streamed past the caches
-        case 31: return launch_v<float, 128, 64, 8, true, 4, 2, 3>(g, st, tr);
+        case 33: return launch_v<float, 128, 64, 8, true, 4, 2, 3>(g, st, tr);
         default: return launch_v<float, 64, 64, 8, true, 2, 2>(g, st, tr);
         }
     }
@@ -1095,7 +1104,7 @@ struct GemmTable<float> {
                                            {128, 64},  {128, 64},  {128, 128},
                                            {128, 64},  {128, 64},  {128, 64},  {64, 128},
                                            {128, 64},  {128, 64},  {128, 128}, {128, 128},
-                                           {128, 128}, {128, 64}};
+                                           {128, 128}, {128, 128}, {64, 64},   {128, 64}};
         return t[v];
     }
     static int tile_m(int v) { return shape(v).bm; }
@@ -1177,7 +1186,7 @@ int launch_gemm(const GemmArgs<T>& g, hipStream_t st, bool trailing)
     // LU queues on its main stream (trailing update, columns of a step): the
     // panel's products (K <= 64) run on the look-ahead stream beside them and
     // must not share the work space.
-    if (p == 1 && v >= GEMM_X6_FIRST && v <= GEMM_X6_LAST &&
+    if (p == 1 && gemm_variant_is_x6(v) &&
         (!g.ws || g.K <= 64 || g.ws_item < gemm_x6_ws_bytes(g.M, g.N, g.K) ||
          gemm_x6_ws_bytes(g.M, g.N, g.K) <= 0))
         v = tiles_big >= 512 ? 19 : 17;

@@ -134,7 +134,7 @@ struct GemmArgs {
 // packed in MFMA fragment order; 0 when K exceeds the LU's largest step).
 long gemm_x6_ws_bytes(int M, int N, int K);
 // complex64 variants of the split-bf16 product (they need the work space)
-constexpr int GEMM_X6_FIRST = 22, GEMM_X6_LAST = 30;
+constexpr bool gemm_variant_is_x6(int v) { return v >= 22 && v <= 32; }
 
 // One work-group's pivot candidate for one column of the panel.
 template <typename T>
