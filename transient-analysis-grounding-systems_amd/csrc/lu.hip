@@ -686,6 +686,8 @@ __global__ __launch_bounds__(256) void k_rows_gemv(const cx<T>* A, long ld, cons
     if (i >= hi) return;
     const cx<T>* row = A + (long)perm[i] * ld + c0;
     T sx = 0, sy = 0;
+    // unrolled so that several loads per lane are in flight
+#pragma unroll 4
     for (int k = threadIdx.x; k < w; k += 256) {
         const cx<T> a = row[k], v = y[c0 + k];
         sx = fma(a.x, v.x, sx);
@@ -1275,7 +1277,9 @@ __global__ __launch_bounds__(256) void k_hem_apply(const c64* W, long ld, HemOp 
         const c64* xz = x + off;
         DD sx_{0, 0}, sy_{0, 0};
         double an = 0;
+        // unrolled so that several loads per lane are in flight
         if (comp) {
+#pragma unroll 4
             for (long c = threadIdx.x; c < m; c += 256) {
                 const c64 a = Wr[c], v = xz[c];
                 dd_mac(sx_, a.x, v.x);
@@ -1285,6 +1289,7 @@ __global__ __launch_bounds__(256) void k_hem_apply(const c64* W, long ld, HemOp 
                 if (want_w) an += sqrt(a.x * a.x + a.y * a.y);
             }
         } else {
+#pragma unroll 4
             for (long c = threadIdx.x; c < m; c += 256) {
                 const c64 a = Wr[c], v = xz[c];
                 sx_.hi += a.x * v.x - a.y * v.y;
