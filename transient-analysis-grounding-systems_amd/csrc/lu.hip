@@ -1008,6 +1008,10 @@ __global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A_, long ld
         cx<T> part[16];
 #pragma unroll
         for (int i = 0; i < 16; i++) part[i] = cx<T>{0, 0};
+        // this wave's right-hand side entries (lane i < 16: row rb0 + i), read
+        // before any wait: only this group writes its rows, at its publish
+        cx<T> yrhs{0, 0};
+        if (lane < 16 && sb < nsb && rb0 + lane < sb_w) yrhs = y[sb_row0 + rb0 + lane];
         // 1. off-diagonal block columns: forward b_lo .. q0 - 1, backward q1 .. b_hi - 1
         const int u_first = UP ? b_hi - 1 : b_lo;
         const int nu = UP ? b_hi - q1 : q0 - b_lo;
@@ -1071,8 +1075,7 @@ __global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A_, long ld
         for (int i = 0; i < 16; i++) {
             const cx<T> s = wave_sum_cx<T>(part[i]);
             const int r = rb0 + i;
-            if (lane == 0 && sb < nsb)
-                acc[64 * sb + r] = r < sb_w ? csub(y[sb_row0 + r], s) : cx<T>{0, 0};
+            if (lane == i && sb < nsb) acc[64 * sb + r] = r < sb_w ? csub(yrhs, s) : cx<T>{0, 0};
         }
         // 2. the group's own blocks, in dependency order (RB <= 2: the later
         // block takes the first one's contribution)
