@@ -367,6 +367,16 @@ __host__ __device__ inline void trsv_block(const TrsvGeom& g, int q, int& row0, 
     row0 = j + 64 * t;
     wq = jb - 64 * t < 64 ? jb - 64 * t : 64;
 }
+// Two ranges of one flow launch: with `on`, each item's second range (items
+// 2 i and 2 i + 1 of the launch are item i's two ranges) has the same block
+// structure as its first, shifted by `a` columns (matrix), `p` rows (row map),
+// `v` entries (vector) and `inv` elements (block inverses); its completion
+// words follow the first range's `done` groups.
+struct FlowPair {
+    int on = 0;
+    long a = 0, p = 0, v = 0, inv = 0;
+    int done = 0;
+};
 // Triangular solves: flow (one launch per range and direction, work-groups
 // of 64 rpb rows in dependency order; rpb 1 or 2) or the per-step launches
 // (0)

@@ -927,7 +927,7 @@ __global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A_, long ld
                                                         int b_lo, int b_hi, int nitems,
                                                         unsigned* ctl, int* err_, unsigned epoch,
                                                         long sa, long sp, long sv, long sinv,
-                                                        long sctl, long sf)
+                                                        long sctl, long sf, FlowPair pr)
 {
     static_assert(RB == 1 || RB == 2, "groups of one or two blocks");
     // block columns per chunk of loads (a lane holds 16 KU values of A)
@@ -960,13 +960,16 @@ __global__ __launch_bounds__(256 * RB) void k_trsv_flow(const cx<T>* A_, long ld
         __syncthreads();
         if (s_ticket >= total) return;
         const int gi = s_ticket / nitems;
-        const long z_ = s_ticket - (long)gi * nitems;
-        const cx<T>* A = A_ + z_ * sa;
-        const int* perm = perm_ + z_ * sp;
-        cx<T>* y = y_ + z_ * sv;
-        const cx<T>* inv = inv_ + z_ * sinv;
+        const long zv = s_ticket - (long)gi * nitems;
+        // item and range (two ranges per item with pr.on)
+        const long z_ = pr.on ? zv >> 1 : zv;
+        const bool r2 = pr.on && (zv & 1);
+        const cx<T>* A = A_ + z_ * sa + (r2 ? pr.a : 0);
+        const int* perm = perm_ + z_ * sp + (r2 ? pr.p : 0);
+        cx<T>* y = y_ + z_ * sv + (r2 ? pr.v : 0);
+        const cx<T>* inv = inv_ + z_ * sinv + (r2 ? pr.inv : 0);
         int* err = err_ + z_ * sf;
-        unsigned* done = ctl + z_ * sctl + 2;
+        unsigned* done = ctl + z_ * sctl + 2 + (r2 ? pr.done : 0);
         // this group's blocks [q0, q1) and rows [r_lo, r_hi)
         int q0, q1;
         if (!UP) {
@@ -2398,32 +2401,24 @@ int lu_solve_rows() { return g_solve_rows.load(); }
 
 template <typename T, bool UP, int RB>
 static void flow_launch(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
-                        int b_lo, int b_hi, unsigned epoch)
+                        int b_lo, int b_hi, unsigned epoch, const FlowPair& pr = FlowPair())
 {
     const BatchStrides b = w.bs();
     const int ngrp = (b_hi - b_lo + RB - 1) / RB;
-    // 16 work-groups per item take the groups in turn (enough to stream ahead
-    // of the chain), at most one per group
-    const int nwg = std::min(ngrp * b.nb, 16 * b.nb);
+    const int nitems = b.nb * (pr.on ? 2 : 1);
+    // 16 work-groups per item and range take the groups in turn (enough to
+    // stream ahead of the chain), at most one per group
+    const int nwg = std::min(ngrp * nitems, 16 * nitems);
     hipLaunchKernelGGL((k_trsv_flow<T, UP, RB>), dim3(nwg), dim3(256 * RB), 0, st, A, ld,
                        (const int*)w.perm, y, (const cx<T>*)(UP ? w.invU : w.invL), w.geom, b_lo,
-                       b_hi, b.nb, w.tctl, w.flags + 1, epoch, b.a, b.p, b.v, b.inv,
-                       w.tctl_stride, b.f);
+                       b_hi, nitems, w.tctl, w.flags + 1, epoch, b.a, b.p, b.v, b.inv,
+                       w.tctl_stride, b.f, pr);
 }
 
-// Flow solves over the blocks of the rows [c_lo, c_hi), forward then
-// backward; false when the factorisation's blocks do not allow them.
-template <typename T>
-static bool flow_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
-                       int c_lo, int c_hi)
+// The blocks [b_lo, b_hi) of the rows [c_lo, c_hi); false when the range does
+// not start and end on block boundaries.
+static bool flow_range(const TrsvGeom& g, int c_lo, int c_hi, int* b_lo_, int* b_hi_)
 {
-    const Tuning* tu = tuning_current();
-    int rb = tu && tu->solve_rows >= 0 ? tu->solve_rows
-                                       : g_solve_rows.load(std::memory_order_relaxed);
-    // complex128 groups take one block (two would not fit the registers)
-    if (sizeof(T) == 8 && rb > 1) rb = 1;
-    if (!rb || !w.geom.ok || !w.tctl) return false;
-    const TrsvGeom& g = w.geom;
     const int nbk = g.first[g.np];
     int b_lo = nbk, b_hi = nbk;
     for (int q = 0; q < nbk; q++) {
@@ -2435,23 +2430,62 @@ static bool flow_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStrea
             break;
         }
     }
+    *b_lo_ = b_lo;
+    *b_hi_ = b_hi;
     if (b_lo >= b_hi) return true;
-    // the range must start and end on block boundaries
     int r0, wq;
     trsv_block(g, b_lo, r0, wq);
     if (r0 != c_lo) return false;
     trsv_block(g, b_hi - 1, r0, wq);
-    if (r0 + wq != c_hi) return false;
+    return r0 + wq == c_hi;
+}
+
+// Flow solves over the blocks of the rows [c_lo, c_hi), forward then
+// backward; false when the factorisation's blocks do not allow them. With a
+// second range [c_lo2, c_hi2) (c_lo2 >= 0) independent of the first and of the
+// same block structure, both run in the same launches (FlowPair); false when
+// the structures differ.
+template <typename T>
+static bool flow_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st,
+                       int c_lo, int c_hi, int c_lo2 = -1, int c_hi2 = -1)
+{
+    const Tuning* tu = tuning_current();
+    int rb = tu && tu->solve_rows >= 0 ? tu->solve_rows
+                                       : g_solve_rows.load(std::memory_order_relaxed);
+    // complex128 groups take one block (two would not fit the registers)
+    if (sizeof(T) == 8 && rb > 1) rb = 1;
+    if (!rb || !w.geom.ok || !w.tctl) return false;
+    const TrsvGeom& g = w.geom;
+    int b_lo, b_hi;
+    if (!flow_range(g, c_lo, c_hi, &b_lo, &b_hi)) return false;
+    FlowPair pr;
+    if (c_lo2 >= 0) {
+        int b_lo2, b_hi2;
+        if (!flow_range(g, c_lo2, c_hi2, &b_lo2, &b_hi2)) return false;
+        if (b_hi2 - b_lo2 != b_hi - b_lo || b_lo >= b_hi) return false;
+        for (int q = b_lo; q < b_hi; q++) {
+            int r0, wq, r2, w2;
+            trsv_block(g, q, r0, wq);
+            trsv_block(g, q + (b_lo2 - b_lo), r2, w2);
+            if (w2 != wq || r2 - r0 != c_lo2 - c_lo) return false;
+        }
+        const int R = sizeof(T) == 8 ? 1 : rb;
+        pr.on = 1;
+        pr.a = pr.p = pr.v = c_lo2 - c_lo;
+        pr.inv = 4096L * (b_lo2 - b_lo);
+        pr.done = (b_hi - b_lo + R - 1) / R;
+    }
+    if (b_lo >= b_hi) return true;
     for (int dir = 0; dir < 2; dir++) {
         if (++w.epoch == 0) ++w.epoch;
         const unsigned ep = w.epoch;
         constexpr int R2 = sizeof(T) == 8 ? 1 : 2;
         if (dir == 0) {
-            if (rb == 1) flow_launch<T, false, 1>(w, A, ld, y, st, b_lo, b_hi, ep);
-            else flow_launch<T, false, R2>(w, A, ld, y, st, b_lo, b_hi, ep);
+            if (rb == 1) flow_launch<T, false, 1>(w, A, ld, y, st, b_lo, b_hi, ep, pr);
+            else flow_launch<T, false, R2>(w, A, ld, y, st, b_lo, b_hi, ep, pr);
         } else {
-            if (rb == 1) flow_launch<T, true, 1>(w, A, ld, y, st, b_lo, b_hi, ep);
-            else flow_launch<T, true, R2>(w, A, ld, y, st, b_lo, b_hi, ep);
+            if (rb == 1) flow_launch<T, true, 1>(w, A, ld, y, st, b_lo, b_hi, ep, pr);
+            else flow_launch<T, true, R2>(w, A, ld, y, st, b_lo, b_hi, ep, pr);
         }
     }
     return true;
@@ -2503,8 +2537,12 @@ template <typename T>
 static void nodal_solve(LUWork<T>& w, const cx<T>* A, long ld, cx<T>* y, hipStream_t st)
 {
     const int n = w.n, m = w.nodal_m;
-    range_solve<T>(w, A, ld, y, st, 0, m);
-    range_solve<T>(w, A, ld, y, st, m, 2 * m);
+    // the Z_L and Z_T blocks are independent: one pair of flow launches when
+    // their blocks match
+    if (!flow_solve<T>(w, A, ld, y, st, 0, m, m, 2 * m)) {
+        range_solve<T>(w, A, ld, y, st, 0, m);
+        range_solve<T>(w, A, ld, y, st, m, 2 * m);
+    }
     const BatchStrides b = w.bs();
     hipLaunchKernelGGL(k_rows_gemv<T>, dim3(n - 2 * m, 1, b.nb), dim3(256), 0, st, A, ld, w.perm,
                        y, 2 * m, n, 0, 2 * m, b.a, b.p, b.v);
