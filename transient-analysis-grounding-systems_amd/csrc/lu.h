@@ -427,6 +427,7 @@ struct LUWork {
     // complex64 only: work space of the split-bf16 products (operands packed
     // in MFMA fragment order), x6_item bytes per item
     void* x6ws = nullptr;
+    void* x6ws2 = nullptr;  // the same for products on the look-ahead stream
     size_t cap_x6 = 0;
     long x6_item = 0;
 

@@ -1706,6 +1706,8 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         (void)hipFree(tcnt);
         (void)hipFree(tctl);
         (void)hipFree(x6ws);
+        (void)hipFree(x6ws2);
+        x6ws2 = nullptr;
         perm = perm_alt = flags = d_init = d_orders = tcand = nullptr;
         invL = invU = vec = nullptr;
         tcnt = tctl = nullptr;
@@ -1729,7 +1731,10 @@ int LUWork<T>::alloc(int n_, int max_panel_wgs, size_t panel_lds_limit, int batc
         // completion words start below every epoch a launch uses
         TAGS_HIP(hipMalloc(&tctl, sizeof(unsigned) * need_tctl));
         TAGS_HIP(hipMemset(tctl, 0, sizeof(unsigned) * need_tctl));
-        if (need_x6) TAGS_HIP(hipMalloc(&x6ws, need_x6));
+        if (need_x6) {
+            TAGS_HIP(hipMalloc(&x6ws, need_x6));
+            TAGS_HIP(hipMalloc(&x6ws2, need_x6));
+        }
         epoch = 0;
         cap_perm = need_perm;
         cap_inv = need_inv;
@@ -1785,6 +1790,8 @@ void LUWork<T>::release()
     (void)hipFree(tcnt);
     (void)hipFree(tctl);
     (void)hipFree(x6ws);
+    (void)hipFree(x6ws2);
+    x6ws2 = nullptr;
     tcand = nullptr;
     tcnt = nullptr;
     tctl = nullptr;
@@ -2102,8 +2109,11 @@ static int step_update(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, lon
 template <typename T>
 static int backward_cols(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long ld,
                          const int* pm, const std::vector<LUStep>& steps, int p, int c_lo,
-                         int x0, int nx)
+                         int x0, int nx, void* ws = nullptr)
 {
+    // the split-bf16 products' work space (another one for a second stream)
+    BatchStrides bw = w.bs();
+    if (ws) bw.ws = ws;
     constexpr int CW = sizeof(T) == 4 ? 32 : 16, KC = sizeof(T) == 4 ? 16 : 8;
     int rc;
     for (int k = (int)steps.size() - 1; k >= 0; k--) {
@@ -2134,7 +2144,7 @@ static int backward_cols(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, l
         }
         if (s.j > c_lo &&
             (rc = gemm<T>(pr, st, s.j - c_lo, nx, s.jb, A + s.j, ld, pm + c_lo, A + x0, ld,
-                          pm + s.j, A + x0, ld, pm + c_lo, 1, false, &w.flops, w.bs())))
+                          pm + s.j, A + x0, ld, pm + c_lo, 1, false, &w.flops, bw)))
             return rc;
     }
     return 0;
@@ -2150,10 +2160,26 @@ static int nodal_prepare(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, l
 {
     const int m = plan.nodal_m, nn = w.n - 2 * m, x0 = 2 * m;
     int rc;
-    for (int p = 0; p < 2; p++)
-        if ((rc = backward_cols<T>(w, pr, st, A, ld, pm, steps, p, plan.phases[p].c_begin, x0,
+    if (w.s2 && (w.x6ws2 || !w.x6ws)) {
+        // the two phases' rows are independent: the Z_T phase's on the
+        // look-ahead stream (idle here; its own split-bf16 work space) beside
+        // the Z_L phase's
+        TAGS_HIP(hipEventRecord(w.ev_left, st));
+        TAGS_HIP(hipStreamWaitEvent(w.s2, w.ev_left, 0));
+        if ((rc = backward_cols<T>(w, pr, w.s2, A, ld, pm, steps, 1, plan.phases[1].c_begin, x0,
+                                   nn, w.x6ws2)))
+            return rc;
+        if ((rc = backward_cols<T>(w, pr, st, A, ld, pm, steps, 0, plan.phases[0].c_begin, x0,
                                    nn)))
             return rc;
+        TAGS_HIP(hipEventRecord(w.ev_panel, w.s2));
+        TAGS_HIP(hipStreamWaitEvent(st, w.ev_panel, 0));
+    } else {
+        for (int p = 0; p < 2; p++)
+            if ((rc = backward_cols<T>(w, pr, st, A, ld, pm, steps, p, plan.phases[p].c_begin,
+                                       x0, nn)))
+                return rc;
+    }
     Timed tm(pr, FAM_OTHER, st);
     hipLaunchKernelGGL(k_nodal_schur<T>, dim3(nn, 1, w.nb), dim3(256), 0, st, A, ld, pm, m, nn,
                        w.sA, (long)w.n);
