@@ -228,50 +228,36 @@ __global__ __launch_bounds__(256) void k_panel(PanelArgs<T> p)
 }
 
 // ---------------------------------------------------------------------------
-// Diagonal block of a panel (w x w at column c0, logical rows c0 .. c0+w,
-// physical pm[c0 + r]): with FACTOR, its LU without row exchanges in place;
-// then (in every case) the dense 64 x 64 inverses of its unit-lower and upper
-// triangles (identity padding past w), which the panel's L21 = A21 inv(U11),
-// the rows' U12 = inv(L11) A12 and the triangular solves use. Work-groups of
-// 256 threads; thread (r, p) holds columns 16p .. 16p+15 of row r in
-// registers. Rows and columns past w are padded with the identity, which the
-// sweeps leave unchanged. Launched with two work-groups of 256 threads (one
-// per inverse) without FACTOR. With FACTOR, one work-group of 512 threads: two
-// groups of 256 both run the LU on their own register copies (group 0 alone
-// writes LDS and the block back), then group 0 sweeps inv(L) while group 1
-// sweeps inv(U), passing the same barriers.
-//   LU:      at step k row k is final; its entries right of k and the
-//            reciprocal of its pivot are broadcast through LDS and every row
-//            r > k forms l = a[r][k] / u_kk and does a[r] -= l u[k].
-//   inverses: inv(L) top-down (row k final at step k, X[r] -= L[r][k] X[k]
-//            for r > k), inv(U) bottom-up (row k divided by u_kk at its step,
-//            X[r] -= U[r][k] X[k] for r < k).
-// One barrier per step (row buffers alternate by parity). A zero pivot gives
-// zero multipliers in its column and is reported in *info (first such column,
-// 1-based, as LAPACK); a caller then falls back to a pivoted factorisation.
+// Diagonal block of a pivoted panel (w x w at column c0, logical rows
+// c0 .. c0+w, physical pm[c0 + r], already L11 \ U11): the dense 64 x 64
+// inverses of its unit-lower and upper triangles (identity padding past w),
+// which the panel's L21 = A21 inv(U11), the rows' U12 = inv(L11) A12 and the
+// triangular solves use. Two work-groups of 256 threads, one per inverse;
+// thread (r, p) holds columns 16p .. 16p+15 of row r in registers (rows and
+// columns past w are padded with the identity, which the sweeps leave
+// unchanged).
+//   inv(L): top-down (row k final at step k, X[r] -= L[r][k] X[k] for r > k);
+//   inv(U): bottom-up (row k divided by u_kk at its step, X[r] -= U[r][k] X[k]
+//           for r < k).
+// One barrier per step (row buffers alternate by parity).
 // ---------------------------------------------------------------------------
-template <typename T, bool FACTOR>
-__global__ __launch_bounds__(512) void k_diag_block(cx<T>* A, long ld, const int* pm, int c0,
-                                                    int w, cx<T>* invL, cx<T>* invU, int* info,
-                                                    long sa, long sp, long sinv, long sf)
+template <typename T>
+__global__ __launch_bounds__(256) void k_diag_block(const cx<T>* A, long ld, const int* pm, int c0,
+                                                    int w, cx<T>* invL, cx<T>* invU, long sa,
+                                                    long sp, long sinv)
 {
     const long z_ = blockIdx.z;
     A += z_ * sa;
     pm += z_ * sp;
     invL += z_ * sinv;
     invU += z_ * sinv;
-    info += z_ * sf;
-    __shared__ cx<T> s_u[2][64];
-    __shared__ cx<T> s_v[2][2][64];
-    __shared__ cx<T> s_rp[2];
+    __shared__ cx<T> s_v[2][64];
     __shared__ int s_pm[64];
-    __shared__ int s_zero;
     __builtin_amdgcn_s_setprio(1);  // on the panel's critical path
-    const int tid = threadIdx.x, t = tid & 255, grp = tid >> 8;
+    const int t = threadIdx.x;
     const int r = t >> 2, p = t & 3, lane = t & 63;
     const int quad = lane & ~3;
-    if (tid < 64) s_pm[tid] = tid < w ? pm[c0 + tid] : 0;
-    if (tid == 0) s_zero = -1;
+    if (t < 64) s_pm[t] = t < w ? pm[c0 + t] : 0;
     __syncthreads();
     cx<T> a[16];
     {
@@ -282,17 +268,111 @@ __global__ __launch_bounds__(512) void k_diag_block(cx<T>* A, long ld, const int
             a[q] = src ? (c < w ? src[q] : cx<T>{0, 0}) : cx<T>{c == r ? T(1) : T(0), T(0)};
         }
     }
-    // steps past the block's width would act on identity padding only: skipped
-    // (the width is uniform, so whole steps, barriers included, are skipped)
-    for (int kb = 0; FACTOR && kb < 4 && 16 * kb < w; kb++) {
+    const bool lower = blockIdx.x == 0;
+    cx<T> xv[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) xv[q] = cx<T>{16 * p + q == r ? T(1) : T(0), T(0)};
+    for (int kb = 0; kb < 4; kb++) {
+#pragma unroll
+        for (int kk = 0; kk < 16; kk++) {
+            // inv(L): row k = 16 kb + kk (top-down); inv(U): row 63 - k (bottom-up)
+            const int kbx = lower ? kb : 3 - kb;
+            const int k = lower ? kb * 16 + kk : 63 - (kb * 16 + kk), par = kk & 1;
+            if (k >= w) continue;  // identity padding: no change
+            // column k of this row (the multiplier L[r][k], or U[r][k] and for
+            // row k itself its pivot) from the quad's thread kbx
+            const cx<T> ak = lower ? a[kk] : a[15 - kk];
+            cx<T> c;
+            c.x = __shfl(ak.x, quad | kbx);
+            c.y = __shfl(ak.y, quad | kbx);
+            if (r == k) {
+                if (!lower) {
+                    const bool z = c.x == T(0) && c.y == T(0);
+                    const cx<T> rd = z ? cx<T>{0, 0} : crecip_fast(c);
+#pragma unroll
+                    for (int q = 0; q < 16; q++) xv[q] = cmul(xv[q], rd);
+                }
+#pragma unroll
+                for (int q = 0; q < 16; q++) s_v[par][16 * p + q] = xv[q];
+            }
+            __syncthreads();
+            if (lower ? r > k : r < k) {
+#pragma unroll
+                for (int q = 0; q < 16; q++) xv[q] = cfnma(xv[q], c, s_v[par][16 * p + q]);
+            }
+        }
+    }
+    cx<T>* out = lower ? invL : invU;
+#pragma unroll
+    for (int q = 0; q < 16; q++) out[r * 64 + 16 * p + q] = xv[q];
+}
+
+// ---------------------------------------------------------------------------
+// Diagonal block of a panel without row exchanges (w x w at column c0, logical
+// rows c0 .. c0+w, physical pm[c0 + r]): its LU in place and the same two
+// inverses as k_diag_block, all in one sweep of w steps with one barrier each.
+// Work-group of 512 threads, two groups of 256 with thread (r, p) holding
+// columns 16p .. 16p+15 of row r: group 0 the block and the rows of inv(L),
+// group 1 the rows of inv(U). At step k row k of U and row k of inv(L) are
+// final; the row's owner puts both, and 1/u_kk, in LDS (buffers alternate by
+// parity), then
+//   group 0, rows r > k:  l = a[r][k] / u_kk,  a[r] -= l u[k],  y[r] -= l y[k]
+//                         (L Y = I row by row; l is stored as L[r][k]);
+//   group 1, rows r <= k: x[r][k] = t[r][k] / u_kk,  t[r][q] -= x[r][k] u[k][q]
+//                         for q > k (X U = I column by column; rows r > k have
+//                         t[r][k] = 0 exactly and are skipped).
+// Rows and columns past w are the identity's, and steps past w are skipped
+// (w is uniform, so whole steps, barriers included). A zero pivot gives zero
+// multipliers in its column and is reported in *info (first such column,
+// 1-based, as LAPACK); a caller then falls back to a pivoted factorisation.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(512)
+__attribute__((amdgpu_waves_per_eu(sizeof(T) == 4 ? 4 : 2))) void k_diag_factor(cx<T>* A, long ld, const int* pm, int c0,
+                                                     int w, cx<T>* invL, cx<T>* invU, int* info,
+                                                     long sa, long sp, long sinv, long sf)
+{
+    const long z_ = blockIdx.z;
+    A += z_ * sa;
+    pm += z_ * sp;
+    invL += z_ * sinv;
+    invU += z_ * sinv;
+    info += z_ * sf;
+    __shared__ cx<T> s_u[2][64];  // row k of U right of k (zero at and left of k)
+    __shared__ cx<T> s_y[2][64];  // row k of inv(L)
+    __shared__ cx<T> s_rp[2];     // 1 / u_kk (zero for a zero pivot)
+    __shared__ int s_pm[64];
+    __shared__ int s_zero;
+    __builtin_amdgcn_s_setprio(1);  // on the panel's critical path
+    const int tid = threadIdx.x, t = tid & 255, grp = tid >> 8;
+    const int r = t >> 2, p = t & 3, lane = t & 63;
+    const int quad = lane & ~3;
+    if (tid < 64) s_pm[tid] = tid < w ? pm[c0 + tid] : 0;
+    if (tid == 0) s_zero = -1;
+    __syncthreads();
+    // group 0: a = row r of the block; both groups: y = row r of the identity
+    cx<T> a[16], y[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++) y[q] = cx<T>{16 * p + q == r ? T(1) : T(0), T(0)};
+    if (grp == 0) {
+        const cx<T>* src = r < w ? A + (long)s_pm[r] * ld + c0 + 16 * p : nullptr;
+#pragma unroll
+        for (int q = 0; q < 16; q++) {
+            const int c = 16 * p + q;
+            a[q] = src ? (c < w ? src[q] : cx<T>{0, 0}) : y[q];
+        }
+    }
+    for (int kb = 0; kb < 4 && 16 * kb < w; kb++) {
 #pragma unroll
         for (int kk = 0; kk < 16; kk++) {
             const int k = kb * 16 + kk, par = k & 1;
             if (k >= w) break;
-            if (r == k && grp == 0) {
+            if (grp == 0 && r == k) {
 #pragma unroll
-                for (int q = 0; q < 16; q++)
+                for (int q = 0; q < 16; q++) {
                     s_u[par][16 * p + q] = 16 * p + q > k ? a[q] : cx<T>{0, 0};
+                    s_y[par][16 * p + q] = y[q];
+                }
                 if (p == kb) {
                     const bool z = a[kk].x == T(0) && a[kk].y == T(0);
                     s_rp[par] = z ? cx<T>{0, 0} : crecip_fast(a[kk]);
@@ -300,68 +380,43 @@ __global__ __launch_bounds__(512) void k_diag_block(cx<T>* A, long ld, const int
                 }
             }
             __syncthreads();
-            const cx<T> lo = cmul(a[kk], s_rp[par]);
-            cx<T> l;
-            l.x = __shfl(lo.x, quad | kb);
-            l.y = __shfl(lo.y, quad | kb);
-            if (r > k) {
-                if (p == kb) a[kk] = l;
+            const cx<T> rp = s_rp[par];
+            if (grp == 0) {
+                const cx<T> lo = cmul(a[kk], rp);
+                cx<T> l;
+                l.x = __shfl(lo.x, quad | kb);
+                l.y = __shfl(lo.y, quad | kb);
+                if (r > k) {
+                    if (p == kb) a[kk] = l;
 #pragma unroll
-                for (int q = 0; q < 16; q++) a[q] = cfnma(a[q], l, s_u[par][16 * p + q]);
+                    for (int q = 0; q < 16; q++) {
+                        a[q] = cfnma(a[q], l, s_u[par][16 * p + q]);
+                        y[q] = cfnma(y[q], l, s_y[par][16 * p + q]);
+                    }
+                }
+            } else {
+                const cx<T> xo = cmul(y[kk], rp);
+                cx<T> x;
+                x.x = __shfl(xo.x, quad | kb);
+                x.y = __shfl(xo.y, quad | kb);
+                if (r <= k) {
+                    if (p == kb) y[kk] = x;
+#pragma unroll
+                    for (int q = 0; q < 16; q++) y[q] = cfnma(y[q], x, s_u[par][16 * p + q]);
+                }
             }
         }
     }
-    // a: L \ U of row r. Without FACTOR the block is only read: work-group 0
-    // sweeps inv(L) and work-group 1 inv(U). With FACTOR group 0 writes the
-    // factored block back, then sweeps inv(L) while group 1 sweeps inv(U).
-    if (FACTOR && r < w && grp == 0) {
+    if (grp == 0 && r < w) {
         cx<T>* dst = A + (long)s_pm[r] * ld + c0 + 16 * p;
 #pragma unroll
         for (int q = 0; q < 16; q++)
             if (16 * p + q < w) dst[q] = a[q];
     }
-    if (FACTOR && tid == 0 && s_zero >= 0 && *info == 0) *info = c0 + s_zero + 1;
-    {
-        const int pass = FACTOR ? grp : (int)blockIdx.x;
-        const bool lower = pass == 0;
-        cx<T> xv[16];
+    if (tid == 0 && s_zero >= 0 && *info == 0) *info = c0 + s_zero + 1;
+    cx<T>* out = grp == 0 ? invL : invU;
 #pragma unroll
-        for (int q = 0; q < 16; q++) xv[q] = cx<T>{16 * p + q == r ? T(1) : T(0), T(0)};
-        for (int kb = 0; kb < 4; kb++) {
-#pragma unroll
-            for (int kk = 0; kk < 16; kk++) {
-                // inv(L): row k = 16 kb + kk (top-down); inv(U): row 63 - k (bottom-up)
-                const int kbx = lower ? kb : 3 - kb;
-                const int k = lower ? kb * 16 + kk : 63 - (kb * 16 + kk), par = kk & 1;
-                if (k >= w) continue;  // identity padding: no change
-                // column k of this row (the multiplier L[r][k], or U[r][k] and for
-                // row k itself its pivot) from the quad's thread kbx
-                const cx<T> ak = lower ? a[kk] : a[15 - kk];
-                cx<T> c;
-                c.x = __shfl(ak.x, quad | kbx);
-                c.y = __shfl(ak.y, quad | kbx);
-                if (r == k) {
-                    if (!lower) {
-                        const bool z = c.x == T(0) && c.y == T(0);
-                        const cx<T> rd = z ? cx<T>{0, 0} : crecip_fast(c);
-#pragma unroll
-                        for (int q = 0; q < 16; q++) xv[q] = cmul(xv[q], rd);
-                    }
-#pragma unroll
-                    for (int q = 0; q < 16; q++) s_v[grp][par][16 * p + q] = xv[q];
-                }
-                __syncthreads();
-                if (lower ? r > k : r < k) {
-#pragma unroll
-                    for (int q = 0; q < 16; q++)
-                        xv[q] = cfnma(xv[q], c, s_v[grp][par][16 * p + q]);
-                }
-            }
-        }
-        cx<T>* out = lower ? invL : invU;
-#pragma unroll
-        for (int q = 0; q < 16; q++) out[r * 64 + 16 * p + q] = xv[q];
-    }
+    for (int q = 0; q < 16; q++) out[r * 64 + 16 * p + q] = y[q];
 }
 
 // U12 of a step for the columns [cb, cb + cols), in place: the blocked forward
@@ -1995,8 +2050,8 @@ static int l21_gemm(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long l
 }
 
 // One sub-panel of `cols` columns at c0 whose diagonal block is block `slot`:
-//   without row exchanges  k_diag_block<FACTOR> (LU of the diagonal block and
-//                          the inverses of its triangles), L21 product;
+//   without row exchanges  k_diag_factor (LU of the diagonal block and the
+//                          inverses of its triangles), L21 product;
 //   tournament             k_tslu (pivots, L11 \ U11), the inverses, L21 product;
 //   partial                cooperative k_panel (pivots, L11 \ U11 and L21), the
 //                          inverses.
@@ -2015,11 +2070,11 @@ static int sub_panel(LUWork<T>& w, Profiler* pr, hipStream_t st, cx<T>* A, long 
         Timed tm(pr, ph.pivot ? FAM_OTHER : FAM_PANEL, st);
         const BatchStrides b = w.bs();
         if (ph.pivot)
-            hipLaunchKernelGGL((k_diag_block<T, false>), dim3(2, 1, b.nb), dim3(256), 0, st, A,
-                               ld, pm, c0, cols, invl, invu, w.flags, b.a, b.p, b.inv, b.f);
+            hipLaunchKernelGGL((k_diag_block<T>), dim3(2, 1, b.nb), dim3(256), 0, st, A, ld, pm,
+                               c0, cols, invl, invu, b.a, b.p, b.inv);
         else
-            hipLaunchKernelGGL((k_diag_block<T, true>), dim3(1, 1, b.nb), dim3(512), 0, st, A,
-                               ld, pm, c0, cols, invl, invu, w.flags, b.a, b.p, b.inv, b.f);
+            hipLaunchKernelGGL((k_diag_factor<T>), dim3(1, 1, b.nb), dim3(512), 0, st, A, ld, pm,
+                               c0, cols, invl, invu, w.flags, b.a, b.p, b.inv, b.f);
         TAGS_HIP(hipGetLastError());
         tm.end();
     }
