@@ -529,7 +529,7 @@ def cases_config(T, threads, step=8, tuning=("table", TUNING_TABLE)):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=24)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -631,9 +631,9 @@ def main():
     else:
         cache = TUNING_TABLE
     tuning = (args.tuning, cache)
-    # probe samples of the autotuner: a spread of the sweep other than the
-    # timed samples
-    probe = np.arange(20, NFREQ, 40)
+    # probe samples of the autotuner: a spread of the sweep (16 samples, so
+    # that groups of up to four samples, four in flight, can be compared)
+    probe = np.arange(8, NFREQ, 16)
     s_probe = 2j * np.pi * freqs[probe]
     kap_probe = SIGMA + s_probe * EPSR * T.EPS0
     tuned = {}
@@ -659,8 +659,10 @@ def main():
             tuned[prec] = dict(base, gemm_big=args.gemm_mixed)
         tu = tuned[prec]
         inflight = args.inflight
+        group = 1
         if isinstance(tu, dict):
             inflight = tu["inflight"] if tu["inflight"] > 0 else args.inflight
+            group = max(1, tu["batch"])
             if one_at_a_time:
                 sw.set_tuning(prec, dict(tu, inflight=1))
             else:
@@ -669,7 +671,10 @@ def main():
             inflight = 1
             sw.set_inflight(1)
         if W:
-            sw.solve(s[:max(W, inflight)], kap[:max(W, inflight)], 0, prec)
+            # the warm-up sets up every slot: `inflight` groups of `group`
+            # samples
+            nw = max(W, inflight * group)
+            sw.solve(s[:nw], kap[:nw], 0, prec)
         sw.set_profiling(profile)
         L.tags_device_synchronize()
         t0 = time.perf_counter()

@@ -492,11 +492,15 @@ typedef struct {
  * update of the plan's widest phase, and a 1024 x 1024 tail product) at both
  * step widths; then, by coordinate descent from the current settings, step
  * width (with its fastest variants), update launches, solve rows, samples in
- * flight, samples per group (orders below 8192) and, for the mixed form of
+ * flight, samples per group (half and a quarter of the automatic size below
+ * order 8192; 2, 3 and 4 from 8192 up, as far as the groups in flight fit in
+ * 80 % of the device's memory) and, for the mixed form of
  * the hybrid plan, the nodal switch (0 or 20 kHz) and the Arnoldi steps per
  * FGMRES refinement step (6, 8, 10) are each kept when a
  * timed solve of the `nprobe` probe samples (s, kappa, mur, inj_node as in
- * tags_sweep_solve; one untimed solve first for the allocations) runs more
+ * tags_sweep_solve; one untimed solve first for the allocations; for a fair
+ * comparison of group sizes nprobe should be a few times the samples in
+ * flight times the largest group tried) runs more
  * than 3 % faster. With cache_path (nullable) an entry for the same device,
  * formulation, plan, precision, N and m is used instead when present (unless
  * force != 0), and the result is written there (one line per key). `out`

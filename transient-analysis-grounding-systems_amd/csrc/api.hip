@@ -2849,8 +2849,9 @@ int tags_sweep_autotune(tags_sweep* sw, int precision, size_t nprobe, const TAGS
     auto eval = [&](const tags_tuning& t, double* ms) -> int {
         int r = tags_sweep_set_tuning(sw, precision, &t);
         if (r) return r;
+        // the warm-up sets up every slot (k groups of up to b samples)
         const int k = t.inflight > 0 ? t.inflight : sw->inflight;
-        const size_t nw = std::min(nprobe, (size_t)k);
+        const size_t nw = std::min(nprobe, (size_t)k * (size_t)std::max(1, t.batch));
         r = tags_sweep_solve(sw, nw, s, kappa, mur, nullptr, nullptr, nullptr, inj_node,
                              precision, u.data(), nullptr, nullptr);
         if (r) return r;
@@ -2935,6 +2936,21 @@ int tags_sweep_autotune(tags_sweep* sw, int precision, size_t nprobe, const TAGS
         const int b0 = sweep_batch(sw, mixed);
         for (int b : {b0 / 2, b0 / 4}) {
             if (b < 1 || b == b0) continue;
+            tags_tuning c = cur;
+            c.batch = b;
+            if ((rc = consider(c))) return fail(rc);
+        }
+    } else {
+        // large systems take one sample per group by default; groups of two
+        // to four share each launch (and its place on the panel chain) among
+        // their samples, as far as k groups' W (and W32) fit in 80 % of the
+        // device's memory
+        size_t mem_free = 0, mem_total = 0;
+        if (hipMemGetInfo(&mem_free, &mem_total) != hipSuccess) mem_total = 0;
+        const double per = (double)sw->N * (double)sw->N * (mixed ? 24.0 : 16.0);
+        const int k = cur.inflight > 0 ? cur.inflight : sw->inflight;
+        for (int b = 2; b <= 4; b++) {
+            if ((double)b * k * per > 0.8 * (double)mem_total) break;
             tags_tuning c = cur;
             c.batch = b;
             if ((rc = consider(c))) return fail(rc);
