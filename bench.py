@@ -631,8 +631,8 @@ def main():
     else:
         cache = TUNING_TABLE
     tuning = (args.tuning, cache)
-    # probe samples of the autotuner: a spread of the sweep (16 samples, so
-    # that groups of up to four samples, four in flight, can be compared)
+    # probe samples of the autotuner: a spread of the sweep (16 samples, on
+    # which it also compares groups of up to four samples, four in flight)
     probe = np.arange(8, NFREQ, 16)
     s_probe = 2j * np.pi * freqs[probe]
     kap_probe = SIGMA + s_probe * EPSR * T.EPS0
